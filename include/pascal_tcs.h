@@ -1,0 +1,191 @@
+/*
+ * pascal_tcs.h -- C-ABI of the MI355X-native PaScaL_TCS MPM-STD time step.
+ *
+ * One context owns the whole (single-rank) domain on one GPU: persistent
+ * device copies of the module arrays of mpi_thermal / mpi_momentum /
+ * mpi_pressure, the 1-D mesh metrics, scratch for the ADI and Poisson
+ * solves, and one HIP stream.  Every entry point only enqueues work on that
+ * stream and returns; the host waits only where a scalar comes back
+ * (tcs_post_div, tcs_post_cfl, tcs_nusselt) or on an explicit download /
+ * tcs_sync.  Plain pointers and sizes only; no torch or Fortran types.
+ *
+ * Error convention: every entry returns 0 on success, a negative TCS_E_*
+ * code otherwise; tcs_last_error() returns a message for the last failure.
+ * Entries are not re-entrant for one context (the reference is single-
+ * threaded per rank; so is this).
+ *
+ * Array layouts are those of the reference's Fortran arrays:
+ *   field  (0:n1sub,0:n2sub,0:n3sub) FP64, i fastest, n?sub = n?m+1
+ *   wall   (0:n1sub,0:n3sub)          FP64, i fastest
+ *   metric (0:n?sub)                  FP64
+ *   TDMA   (1:n_sys,1:n_row)          FP64, system index fastest
+ */
+#ifndef PASCAL_TCS_H
+#define PASCAL_TCS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TCS_OK 0
+#define TCS_E_ARG (-1)     /* bad argument / unsupported shape */
+#define TCS_E_HIP (-2)     /* HIP runtime error */
+#define TCS_E_STATE (-3)   /* call out of order */
+#define TCS_E_NOMEM (-4)   /* device allocation failed */
+
+/* Physical constants and switches of module global
+   (src/module_global.f90:30-70, values set in :107-165). */
+typedef struct tcs_params {
+    int32_t n1m, n2m, n3m;        /* interior cells (namelist meshes) */
+    int32_t pbc1, pbc2, pbc3;     /* periodicity; only (1,0,1) is supported */
+    int32_t precision;            /* 0 = plain FP64 (the only form built) */
+    int32_t reserved;
+    double Ct, Cmu, Cmt, Cmp;     /* :139-141 */
+    double DeltaT;                /* namelist sim_parameter */
+    double a10, a11, a12, a13, a14, a15;  /* density polynomial   :117 */
+    double b10, b11, b12, b13, b14, b15;  /* heat capacity        :118 */
+    double c10, c11, c12, c13, c14, c15;  /* conductivity         :119 */
+    double d10, d11, d12, d13, d14, d15;  /* viscosity            :120 */
+    double KPP0, Rho0, Cp0, Mu0, a12pera11;  /* :122-137 */
+    double L1, L2, L3;            /* domain lengths :149-151 */
+    double MaxCFL, dtMax;         /* :161-165 and namelist */
+} tcs_params;
+
+typedef struct tcs_ctx tcs_ctx;
+
+/* Field ids: module variables of the reference solver modules.
+   T: src/module_solve_thermal.f90:29; U,V,W,dU,dV,dW,P,invRho,Mu:
+   src/module_solve_momentum.f90:31-44; dPhat,dP0,dP: module_solve_pressure.f90:32.
+   invRho and Mu are stored as (0:n2sub,0:n3sub) tables because the
+   reference broadcasts the i=n1sub value along every x-row
+   (src/module_solve_momentum.f90:156-171); upload/download expand them. */
+enum tcs_field {
+    TCS_T = 0, TCS_U, TCS_V, TCS_W, TCS_P,
+    TCS_DU, TCS_DV, TCS_DW, TCS_DP, TCS_DPHAT, TCS_DP0,
+    TCS_INVRHO, TCS_MU,
+    TCS_KPP, TCS_DKPP, TCS_INVRHOCP, TCS_DINVRHOCP,
+    TCS_NFIELD
+};
+/* Wall tables {T,U,V,W}BC{up,bt}_sub: module_solve_thermal.f90:32,
+   module_solve_momentum.f90:35-36. */
+enum tcs_wall {
+    TCS_TBCUP = 0, TCS_TBCBT, TCS_UBCUP, TCS_UBCBT, TCS_VBCUP, TCS_VBCBT,
+    TCS_WBCUP, TCS_WBCBT, TCS_NWALL
+};
+
+/* ---- lifecycle --------------------------------------------------------- */
+const char *tcs_last_error(void);
+int tcs_device_count(int *count);
+
+/* One axis of mpi_subdomain_mesh on one rank
+   (src/module_mpi_subdomain.f90:153-299): x, dx, dmx each hold nm+2 values
+   (0:n?sub).  uniform=1 uniform spacing, else tanh stretching by gamma. */
+int tcs_mesh_axis(int32_t nm, double L, int32_t uniform, double gamma,
+                  int32_t periodic, double *x, double *dx, double *dmx);
+
+/* Replaces mpi_subdomain_make/_mesh/_indices + the *_allocation routines
+   (src/module_mpi_subdomain.f90:98-374; module_solve_*.f90 allocation):
+   copies the metrics (0:n?sub) to the device and allocates every module
+   array once (all zero) plus solver scratch.  n1m and n3m must be powers of
+   two (Poisson FFTs), n2m >= 3. */
+int tcs_init(const tcs_params *prm,
+             const double *dx1, const double *dx2, const double *dx3,
+             const double *dmx1, const double *dmx2, const double *dmx3,
+             tcs_ctx **out);
+int tcs_finalize(tcs_ctx *ctx);
+int tcs_sync(tcs_ctx *ctx);
+/* the context's HIP stream (a hipStream_t) for callers that interoperate */
+void *tcs_stream(tcs_ctx *ctx);
+
+/* ---- module arrays ----------------------------------------------------- */
+int tcs_field_upload(tcs_ctx *ctx, int32_t id, const double *host);
+int tcs_field_download(tcs_ctx *ctx, int32_t id, double *host);
+/* device address of a field (full layout); INVRHO/MU return the 2-D table */
+int tcs_field_device_ptr(tcs_ctx *ctx, int32_t id, double **dptr);
+int tcs_wall_upload(tcs_ctx *ctx, int32_t id, const double *host);
+int tcs_wall_download(tcs_ctx *ctx, int32_t id, double *host);
+
+/* ---- module routines (one entry per reference subroutine) --------------- */
+/* mpi_thermal_boundary, src/module_solve_thermal.f90:88-111: TBC tables :=
+   Tup/Tbt and the wall row of T.  On a single y-rank the reference's
+   if/else-if writes only the bottom row (j=0). */
+int tcs_thermal_boundary(tcs_ctx *ctx, double Tup, double Tbt);
+/* mpi_momentum_boundary, src/module_solve_momentum.f90:105-140 (same rule) */
+int tcs_momentum_boundary(tcs_ctx *ctx, double Uup, double Ubt, double Vup,
+                          double Vbt, double Wup, double Wbt);
+/* mpi_thermal_coeffi, src/module_solve_thermal.f90:116-156: materialises
+   KPP, dKPP, invRhoCp, dinvRhoCp.  The thermal solver evaluates the same
+   polynomials from T inline, so the time step does not need this call. */
+int tcs_thermal_coeffi(tcs_ctx *ctx);
+/* mpi_thermal_solver(U,V,W), src/module_solve_thermal.f90:175-532 */
+int tcs_thermal_solver(tcs_ctx *ctx, double dt);
+/* mpi_momentum_coeffi(T), src/module_solve_momentum.f90:145-173 */
+int tcs_momentum_coeffi(tcs_ctx *ctx);
+/* mpi_momentum_solvedU/V/W(T), src/module_solve_momentum.f90:225-563,
+   :569-924, :930-1272 */
+int tcs_momentum_solvedU(tcs_ctx *ctx, double dt);
+int tcs_momentum_solvedV(tcs_ctx *ctx, double dt);
+int tcs_momentum_solvedW(tcs_ctx *ctx, double dt);
+/* mpi_momentum_blockLdV/blockLdU(T), :1278-1356, :1362-1460 */
+int tcs_momentum_blockLdV(tcs_ctx *ctx, double dt);
+int tcs_momentum_blockLdU(tcs_ctx *ctx, double dt);
+/* mpi_momentum_pseudoupdateUVW, :178-208 */
+int tcs_momentum_pseudoupdateUVW(tcs_ctx *ctx);
+/* mpi_pressure_wave_number, src/module_solve_pressure.f90:93-129 (tcs_init
+   already calls it; exposed for the drop-in module shell) */
+int tcs_pressure_wave_number(tcs_ctx *ctx);
+/* mpi_pressure_RHS(invRho,U,V,W,VBCup,VBCbt), :140-231; the RHS stays on
+   the device for tcs_pressure_poisson_fft2 */
+int tcs_pressure_rhs(tcs_ctx *ctx, double dt);
+/* mpi_pressure_Poisson_FFT2(dx2,dmx2), :489-744: FFT x (r2c), FFT z, TDMA
+   in y per wavenumber pair, inverse FFTs, scaling, mean removal, wall
+   extrapolation of dP */
+int tcs_pressure_poisson_fft2(tcs_ctx *ctx);
+/* mpi_pressure_Projection(invRho,U,V,W,P,Jrank,Jnprocs), :756-874 */
+int tcs_pressure_projection(tcs_ctx *ctx, double dt);
+/* mpi_subdomain_ghostcell_update(X), src/module_mpi_subdomain.f90:758-791:
+   x then y (walls: no-op) then z periodic plane copies on one rank */
+int tcs_ghostcell_update(tcs_ctx *ctx, int32_t id);
+/* mpi_Post_Div(U,V,W,maxDivU), src/module_post.f90:100-158 (signed max) */
+int tcs_post_div(tcs_ctx *ctx, double *maxdiv);
+/* mpi_Post_CFL(U,V,W,newCFL,newdt), src/module_post.f90:168-230 */
+int tcs_post_cfl(tcs_ctx *ctx, double *newcfl, double *newdt);
+/* The loop body of src/main.f90:131-203 (thermal, momentum, pressure,
+   every halo update in main's order), fully on the device. */
+int tcs_step(tcs_ctx *ctx, double dt);
+/* Wall Nusselt numbers (no counterpart in the reference): mean over (i,k)
+   of (TBCbt - T(i,1,k))/dmx2(1) and (T(i,n2m,k) - TBCup)/dmx2(n2sub),
+   times L2/(Tbt - Tup). */
+int tcs_nusselt(tcs_ctx *ctx, double Tup, double Tbt, double *nu_bottom, double *nu_top);
+
+/* ---- PaScaL_TDMA ----------------------------------------------------- */
+/* PaScaL_TDMA_many_solve(plan,A,B,C,D,n_sys,n_row),
+   PaScaL_TDMA/src/pascal_tdma.f90:487-578, and PaScaL_TDMA_many_solve_cycle,
+   :715-806, on one rank (plan = nprocs 1; reduced 2-row solve by tdma_many /
+   tdma_cycl_many, PaScaL_TDMA/src/tdmas.f90:113-212).  Device pointers to
+   (n_sys,n_row) arrays; in place: D returns the solution, A and C return
+   the reference's modified coefficients, B is read only.  Same operation
+   order as the reference, so results are bit-identical to it.  stream may
+   be NULL (the null stream).  n_row >= 2. */
+int ptdma_many_solve_dev(double *A, const double *B, double *C, double *D,
+                         int32_t n_sys, int32_t n_row, void *stream);
+int ptdma_many_solve_cycle_dev(double *A, const double *B, double *C, double *D,
+                               int32_t n_sys, int32_t n_row, void *stream);
+/* Same contract with host arrays (copied in and out; synchronous). */
+int ptdma_many_solve_host(double *A, double *B, double *C, double *D,
+                          int32_t n_sys, int32_t n_row, int32_t cyclic);
+/* Axis solve on a (n1,n2,n3) array (i fastest) in place, systems along
+   `axis` (0=x,1=y,2=z) -- the layout change the reference makes with host
+   transposes before each PaScaL call (e.g. src/module_solve_thermal.f90:
+   371-379) done by strided access instead.  Same arithmetic per system as
+   ptdma_many_solve[_cycle]_dev. */
+int ptdma_axis_solve_dev(double *A, const double *B, double *C, double *D,
+                         int32_t n1, int32_t n2, int32_t n3, int32_t axis,
+                         int32_t cyclic, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,464 @@
+// pressure.hip -- mpi_pressure_RHS / _Poisson_FFT2 / _Projection
+// (src/module_solve_pressure.f90:140-231, :489-744, :756-874) on one GPU.
+//
+// The reference moves the RHS through six MPI_Alltoallw pencil transposes
+// around FFTW calls.  Here the spectrum lives in one device buffer laid out
+// [j][kx][k] (k fastest), the layout of the reference's RHSIKhat_Kline after
+// its z transform, so:
+//   * the x r2c kernel transforms TX consecutive k-lines of one j per
+//     workgroup in LDS and writes each wavenumber row as TX contiguous
+//     complex values;
+//   * the z FFTs run along the contiguous axis in place;
+//   * the y tridiagonal solve takes one (kx,kz) system per thread with
+//     consecutive threads on consecutive kz, its coefficients generated from
+//     1-D tables (no coefficient arrays are stored), real and imaginary
+//     parts sharing one elimination (same matrix, so identical to solving
+//     them in two calls);
+//   * the inverse x c2r kernel mirrors the forward one and also produces
+//     the per-line partial sums for the mean.
+// FFTs are radix-2 Stockham with a host twiddle table, computed in place in
+// LDS with the butterfly outputs staged in registers across barriers.
+#include "tcs_internal.h"
+
+namespace tcs {
+
+constexpr int FFT_THREADS = 256;
+constexpr int FFT_MAXB = 8;  // butterflies per thread per stage
+
+// In-place radix-2 Stockham on nl lines (re/im at [l*n + t]) in LDS.
+// sign<0 forward (exp(-i...)), sign>0 backward.  Caller synchronises before.
+template <int SIGN>
+__device__ __forceinline__ void lds_fft(double *re, double *im, int n, int nl, const double2 *__restrict__ tw) {
+    const int half = n >> 1;
+    const int nb = half * nl;
+    const int tid = threadIdx.x;
+    for (int Ns = 1; Ns < n; Ns <<= 1) {
+        double yr0[FFT_MAXB], yi0[FFT_MAXB], yr1[FFT_MAXB], yi1[FFT_MAXB];
+        int oi[FFT_MAXB];
+#pragma unroll
+        for (int u = 0; u < FFT_MAXB; u++) {
+            const int e = tid + u * FFT_THREADS;
+            oi[u] = -1;
+            if (e < nb) {
+                const int l = e / half;
+                const int jb = e - l * half;
+                const int kk = jb & (Ns - 1);
+                const double2 w = tw[kk * (n / (2 * Ns))];
+                const double ar = re[l * n + jb], ai = im[l * n + jb];
+                const double br = re[l * n + jb + half], bi = im[l * n + jb + half];
+                double tr, ti;
+                if (SIGN < 0) {
+                    tr = br * w.x - bi * w.y;
+                    ti = br * w.y + bi * w.x;
+                } else {
+                    tr = br * w.x + bi * w.y;
+                    ti = bi * w.x - br * w.y;
+                }
+                yr0[u] = ar + tr; yi0[u] = ai + ti;
+                yr1[u] = ar - tr; yi1[u] = ai - ti;
+                oi[u] = l * n + (jb / Ns) * 2 * Ns + kk;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < FFT_MAXB; u++) {
+            if (oi[u] >= 0) {
+                re[oi[u]] = yr0[u]; im[oi[u]] = yi0[u];
+                re[oi[u] + Ns] = yr1[u]; im[oi[u] + Ns] = yi1[u];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// PRHS (:182-229) into the interior layout
+__global__ void __launch_bounds__(256)
+k_pressure_rhs(Grid g, Phys p, double dt, const double *__restrict__ U, const double *__restrict__ V,
+               const double *__restrict__ W, const double *__restrict__ dPhat, const double *__restrict__ iR,
+               const double *__restrict__ VBCup, const double *__restrict__ VBCbt, double *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.n1m || j > g.n2m) return;
+    const double *dx1 = g.dx1, *dx2 = g.dx2, *dx3 = g.dx3, *dmx1 = g.dmx1, *dmx2 = g.dmx2, *dmx3 = g.dmx3;
+    const int ip = i + 1, im = i - 1, jp = j + 1, jm = j - 1, kp = k + 1, km = k - 1;
+    const int jvm = g.jC[jm], jvp = g.jC[jp];
+#define IRt(jj, kk) iR[g.t2((jj), (kk))]
+#define dPh(a, b, c) dPhat[g.ix(a, b, c)]
+    const double DivUm = (U[g.ix(ip, j, k)] - U[g.ix(i, j, k)]) / dx1[i]
+                       + ((double)jvp * V[g.ix(i, jp, k)] - (double)jvm * V[g.ix(i, j, k)]) / dx2[j]
+                       + (W[g.ix(i, j, kp)] - W[g.ix(i, j, k)]) / dx3[k];
+    const double cbc = (1.0 - (double)jvm) * VBCbt[g.b2(i, k)] / dx2[j]
+                     - (1.0 - (double)jvp) * VBCup[g.b2(i, k)] / dx2[j];
+    const double ddpdx1 = (dPh(i, j, k) - dPh(im, j, k)) / dmx1[i];
+    const double ddpdx2 = (dPh(ip, j, k) - dPh(i, j, k)) / dmx1[ip];
+    const double ddpdy3 = (dPh(i, j, k) - dPh(i, jm, k)) / dmx2[j];
+    const double ddpdy4 = (dPh(i, jp, k) - dPh(i, j, k)) / dmx2[jp];
+    const double ddpdz5 = (dPh(i, j, k) - dPh(i, j, km)) / dmx3[k];
+    const double ddpdz6 = (dPh(i, j, kp) - dPh(i, j, k)) / dmx3[kp];
+    const double invRho1 = 0.5 / dmx1[i] * (dx1[im] * IRt(j, k) + dx1[i] * IRt(j, k));
+    const double invRho2 = 0.5 / dmx1[ip] * (dx1[ip] * IRt(j, k) + dx1[i] * IRt(j, k));
+    const double invRho3 = 0.5 / dmx2[j] * (dx2[jm] * IRt(j, k) + dx2[j] * IRt(jm, k));
+    const double invRho4 = 0.5 / dmx2[jp] * (dx2[jp] * IRt(j, k) + dx2[j] * IRt(jp, k));
+    const double invRho5 = 0.5 / dmx3[k] * (dx3[km] * IRt(j, k) + dx3[k] * IRt(j, km));
+    const double invRho6 = 0.5 / dmx3[kp] * (dx3[kp] * IRt(j, k) + dx3[k] * IRt(j, kp));
+    const double ExtraTerm = ((1.0 - invRho2) * ddpdx2 - (1.0 - invRho1) * ddpdx1) / dx1[i]
+                           + ((1.0 - invRho4) * ddpdy4 - (1.0 - invRho3) * ddpdy3) / dx2[j]
+                           + ((1.0 - invRho6) * ddpdz6 - (1.0 - invRho5) * ddpdz5) / dx3[k];
+    out[g.r3(i, j, k)] = (DivUm - cbc) / dt / p.Cmp + ExtraTerm;
+}
+
+// x r2c: TX k-lines (k0..k0+TX-1) of row j per workgroup.
+// spec[(k + n3m*(kx + h1p*(j-1)))] complex, k 0-based.
+__global__ void __launch_bounds__(FFT_THREADS)
+k_fft_x_fwd(Grid g, int h1p, int TX, const double *__restrict__ prhs, const double2 *__restrict__ tw,
+            double2 *__restrict__ spec) {
+    extern __shared__ double lds[];
+    const int n = g.n1m;
+    double *re = lds, *im = lds + (size_t)TX * n;
+    const int j = blockIdx.y + 1;
+    const int k0 = blockIdx.x * TX;  // 0-based
+    const int tid = threadIdx.x;
+    for (int e = tid; e < TX * n; e += FFT_THREADS) {
+        const int l = e / n, t = e - l * n;
+        re[e] = prhs[g.r3(t + 1, j, k0 + l + 1)];
+        im[e] = 0.0;
+    }
+    __syncthreads();
+    lds_fft<-1>(re, im, n, TX, tw);
+    const size_t rowbase = (size_t)g.n3m * h1p * (j - 1);
+    for (int e = tid; e < TX * h1p; e += FFT_THREADS) {
+        const int kx = e / TX, l = e - kx * TX;
+        spec[rowbase + (size_t)g.n3m * kx + k0 + l] = make_double2(re[l * n + kx], im[l * n + kx]);
+    }
+}
+
+// z c2c in place along the contiguous axis: one (kx, j) line per workgroup
+template <int SIGN>
+__global__ void __launch_bounds__(FFT_THREADS)
+k_fft_z(int n3m, int h1p, const double2 *__restrict__ tw, double2 *__restrict__ spec) {
+    extern __shared__ double lds[];
+    double *re = lds, *im = lds + n3m;
+    const size_t base = (size_t)n3m * ((size_t)blockIdx.x + (size_t)h1p * blockIdx.y);
+    const int tid = threadIdx.x;
+    for (int t = tid; t < n3m; t += FFT_THREADS) {
+        const double2 v = spec[base + t];
+        re[t] = v.x; im[t] = v.y;
+    }
+    __syncthreads();
+    lds_fft<SIGN>(re, im, n3m, 1, tw);
+    for (int t = tid; t < n3m; t += FFT_THREADS) spec[base + t] = make_double2(re[t], im[t]);
+}
+
+// y tridiagonal solves per (kx,kz), PaScaL single-rank order (:599-652).
+// system s = kz + n3m*kx, row j stride n3m*h1p.  amj/apj/acj: 1-D tables.
+__global__ void __launch_bounds__(256)
+k_poisson_tdma(int n3m, int h1p, int n2m, int h3p, const double *__restrict__ amj, const double *__restrict__ apj,
+               const double *__restrict__ acj, const double *__restrict__ dxk2, const double *__restrict__ dzk,
+               double2 *__restrict__ S, double *__restrict__ pA, double *__restrict__ pC) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nsys = n3m * h1p;
+    if (s >= nsys) return;
+    const int kx = s / n3m, kz = s - kx * n3m;
+    const size_t rs = (size_t)nsys;
+    double2 *d = S + s;
+    double *a = pA + s, *c = pC + s;
+    const double shift = dxk2[kx];
+    const double zk = dzk[kz];
+    const bool pin = (kx == 0 && kz == 0);
+    // row coefficients: A = amj, B = (acj - dxk2) - dzk, C = apj; (1,1,1) pinned
+    auto coef = [&](int jj, double &A, double &B, double &C) {
+        A = amj[jj]; C = apj[jj]; B = acj[jj] - shift - zk;
+        if (pin && jj == 0) { A = 0.0; B = 1.0; C = 0.0; }
+    };
+    double A0, B0, C0, A1, B1, C1;
+    coef(0, A0, B0, C0);
+    coef(1, A1, B1, C1);
+    double2 D0 = d[0], D1 = d[rs];
+    if (pin) D0.x = 0.0;
+    double a0 = A0 / B0;
+    double dr0 = D0.x / B0, di0 = D0.y / B0;
+    double c0 = C0 / B0;
+    const double a1 = A1 / B1;
+    const double dr1 = D1.x / B1, di1 = D1.y / B1;
+    const double c1 = C1 / B1;
+    a[rs] = a1; c[rs] = c1; d[rs] = make_double2(dr1, di1);
+    double ap = a1, cp = c1, drp = dr1, dip = di1;
+    for (int jj = 2; jj < n2m; jj++) {
+        double A, B, C;
+        coef(jj, A, B, C);
+        const double2 Dj = d[(size_t)jj * rs];
+        const double r = 1.0 / (B - A * cp);
+        const double dr = r * (Dj.x - A * drp);
+        const double di = r * (Dj.y - A * dip);
+        const double cn = r * C;
+        const double an = -r * A * ap;
+        a[(size_t)jj * rs] = an; c[(size_t)jj * rs] = cn; d[(size_t)jj * rs] = make_double2(dr, di);
+        ap = an; cp = cn; drp = dr; dip = di;
+    }
+    const double aN = ap, drN = drp, diN = dip;
+    double ax = a1, cx = c1, drx = dr1, dix = di1;
+    if (n2m >= 3) {
+        const size_t o2 = (size_t)(n2m - 2) * rs;
+        ax = a[o2]; cx = c[o2];
+        const double2 v = d[o2];
+        drx = v.x; dix = v.y;
+        for (int jj = n2m - 3; jj >= 1; jj--) {
+            const size_t o = (size_t)jj * rs;
+            const double aj = a[o], cj = c[o];
+            const double2 dj = d[o];
+            const double dr = dj.x - cj * drx;
+            const double di = dj.y - cj * dix;
+            const double an = aj - cj * ax;
+            const double cn = -cj * cx;
+            a[o] = an; c[o] = cn; d[o] = make_double2(dr, di);
+            ax = an; cx = cn; drx = dr; dix = di;
+        }
+    }
+    {
+        const double r = 1.0 / (1.0 - ax * c0);
+        dr0 = r * (dr0 - c0 * drx);
+        di0 = r * (di0 - c0 * dix);
+        a0 = r * a0;
+        c0 = -r * c0 * cx;
+    }
+    const double rr = 1.0 / (1.0 - aN * c0);
+    const double xnr = rr * (drN - aN * dr0);
+    const double xni = rr * (diN - aN * di0);
+    const double x1r = dr0 - c0 * xnr;
+    const double x1i = di0 - c0 * xni;
+    // imaginary parts zeroed at kx=0 for kz in {0, h3p-1} (:643-650)
+    const bool zim = (kx == 0 && (kz == 0 || kz == h3p - 1));
+    d[0] = make_double2(x1r, zim ? 0.0 : x1i);
+    d[(size_t)(n2m - 1) * rs] = make_double2(xnr, zim ? 0.0 : xni);
+    for (int jj = 1; jj < n2m - 1; jj++) {
+        const size_t o = (size_t)jj * rs;
+        const double aj = a[o], cj = c[o];
+        const double2 dj = d[o];
+        const double vr = dj.x - aj * x1r - cj * xnr;
+        const double vi = dj.y - aj * x1i - cj * xni;
+        d[o] = make_double2(vr, zim ? 0.0 : vi);
+    }
+}
+
+// x c2r of TX k-lines of row j, scaling by 1/n1m/n3m into dP's interior,
+// and the per-line pairwise sums for the mean.
+__global__ void __launch_bounds__(FFT_THREADS)
+k_fft_x_inv(Grid g, int h1p, int TX, const double2 *__restrict__ spec, const double2 *__restrict__ tw,
+            double *__restrict__ dP, double *__restrict__ linesum) {
+    extern __shared__ double lds[];
+    const int n = g.n1m;
+    double *re = lds, *im = lds + (size_t)TX * n;
+    const int j = blockIdx.y + 1;
+    const int k0 = blockIdx.x * TX;
+    const int tid = threadIdx.x;
+    const int nh = n / 2;
+    const size_t rowbase = (size_t)g.n3m * h1p * (j - 1);
+    for (int e = tid; e < TX * h1p; e += FFT_THREADS) {
+        const int kx = e / TX, l = e - kx * TX;
+        const double2 v = spec[rowbase + (size_t)g.n3m * kx + k0 + l];
+        re[l * n + kx] = v.x;
+        im[l * n + kx] = (kx == 0 || kx == nh) ? 0.0 : v.y;
+        if (kx > 0 && kx < nh) {
+            re[l * n + n - kx] = v.x;
+            im[l * n + n - kx] = -v.y;
+        }
+    }
+    __syncthreads();
+    lds_fft<+1>(re, im, n, TX, tw);
+    const double fn1 = (double)g.n1m, fn3 = (double)g.n3m;
+    for (int e = tid; e < TX * n; e += FFT_THREADS) {
+        const int l = e / n, t = e - l * n;
+        const double v = re[e] / fn1 / fn3;
+        dP[g.ix(t + 1, j, k0 + l + 1)] = v;
+        im[e] = v;  // reuse as the reduction buffer
+    }
+    __syncthreads();
+    for (int h = n / 2; h >= 1; h >>= 1) {
+        for (int e = tid; e < TX * h; e += FFT_THREADS) {
+            const int l = e / h, t = e - l * h;
+            im[l * n + t] = im[l * n + t] + im[l * n + t + h];
+        }
+        __syncthreads();
+    }
+    for (int l = tid; l < TX; l += FFT_THREADS) linesum[(size_t)(j - 1) + (size_t)g.n2m * (k0 + l)] = im[l * n];
+}
+
+// pairwise sum of `cnt` values at src + q*stride, zero padded to a power of two
+__device__ double block_tree_sum(const double *__restrict__ src, int cnt, size_t stride, double *buf) {
+    int P = 1;
+    while (P < cnt) P <<= 1;
+    for (int t = threadIdx.x; t < P; t += blockDim.x) buf[t] = (t < cnt) ? src[(size_t)t * stride] : 0.0;
+    __syncthreads();
+    for (int h = P / 2; h >= 1; h >>= 1) {
+        for (int t = threadIdx.x; t < h; t += blockDim.x) buf[t] = buf[t] + buf[t + h];
+        __syncthreads();
+    }
+    return buf[0];
+}
+
+__global__ void k_plane_sum(int n2m, const double *__restrict__ linesum, double *__restrict__ planesum) {
+    extern __shared__ double buf[];
+    const int k = blockIdx.x;
+    const double s = block_tree_sum(linesum + (size_t)n2m * k, n2m, 1, buf);
+    if (threadIdx.x == 0) planesum[k] = s;
+}
+
+__global__ void k_total_mean(int n1m, int n2m, int n3m, const double *__restrict__ planesum,
+                             double *__restrict__ scal) {
+    extern __shared__ double buf[];
+    const double s = block_tree_sum(planesum, n3m, 1, buf);
+    if (threadIdx.x == 0) scal[0] = s / (double)n1m / (double)n2m / (double)n3m;
+}
+
+__global__ void k_dp_subtract(Grid g, const double *__restrict__ scal, double *__restrict__ dP) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.n1m || j > g.n2m) return;
+    const size_t q = g.ix(i, j, k);
+    dP[q] = dP[q] - scal[0];
+}
+
+// quadratic wall extrapolation (:719-736) of dP (mode 0) or P (mode 1)
+__global__ void k_wall_extrap(Grid g, double *__restrict__ F) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y + 1;
+    if (i > g.n1m) return;
+    const double *dmx2 = g.dmx2;
+    const int n2sub = g.n2m + 1, n2msub = g.n2m;
+    const double s = dmx2[1] + dmx2[2];
+    const double a0 = (s * s) / ((s * s) - (dmx2[1] * dmx2[1]));
+    const double b0 = (dmx2[1] * dmx2[1]) / ((s * s) - (dmx2[1] * dmx2[1]));
+    F[g.ix(i, 0, k)] = a0 * F[g.ix(i, 1, k)] - b0 * F[g.ix(i, 2, k)];
+    const double t = dmx2[n2sub] + dmx2[n2msub];
+    const double a1 = (t * t) / ((t * t) - (dmx2[n2sub] * dmx2[n2sub]));
+    const double b1 = (dmx2[n2sub] * dmx2[n2sub]) / ((t * t) - (dmx2[n2sub] * dmx2[n2sub]));
+    F[g.ix(i, n2sub, k)] = a1 * F[g.ix(i, n2msub, k)] - b1 * F[g.ix(i, n2sub - 2, k)];
+}
+
+// projection (:793-851).  The reference overwrites dPhat(i,j,k) inside the
+// same loop nest (k outer, j = 2..n2m then j = 1, i inner), so the
+// neighbours it reads at (i-1,j,k) [i>=2], (i,j-1,k) [j>=3] and (i,j,k-1)
+// [k>=2] already hold the new value 2 dP - (dPhat + dP0)/2.  This kernel
+// reads only old values and recomputes those neighbours' new values; a
+// second kernel then stores the new dPhat and dP0.
+__device__ __forceinline__ double dphat_new(double dP, double dPh, double dP0) {
+    const double d0 = (dPh + dP0) / 2.0;
+    return 2.0 * dP - d0;
+}
+
+__global__ void __launch_bounds__(256)
+k_projection(Grid g, Phys p, double dt, const double *__restrict__ iR, const double *__restrict__ dP,
+             const double *__restrict__ dPhat, const double *__restrict__ dP0, double *__restrict__ U,
+             double *__restrict__ V, double *__restrict__ W, double *__restrict__ P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.n1m || j > g.n2m) return;
+    const double *dx1 = g.dx1, *dx2 = g.dx2, *dx3 = g.dx3, *dmx1 = g.dmx1, *dmx2 = g.dmx2, *dmx3 = g.dmx3;
+    const int im = i - 1, jm = j - 1, km = k - 1;
+    const double Cmp = p.Cmp;
+    const size_t q = g.ix(i, j, k);
+    const double dPc = dP[q], dPhc = dPhat[q];
+    const size_t qi = g.ix(im, j, k), qj = g.ix(i, jm, k), qk = g.ix(i, j, km);
+    const double hi = (i >= 2) ? dphat_new(dP[qi], dPhat[qi], dP0[qi]) : dPhat[qi];
+    const double hj = (j >= 3) ? dphat_new(dP[qj], dPhat[qj], dP0[qj]) : dPhat[qj];
+    const double hk = (k >= 2) ? dphat_new(dP[qk], dPhat[qk], dP0[qk]) : dPhat[qk];
+    double invRhoc = 0.5 / dmx1[i] * (dx1[i] * IRt(j, k) + dx1[im] * IRt(j, k));
+    U[q] = U[q] - dt * Cmp * ((dPc - dP[qi]) / dmx1[i] + (invRhoc - 1.0) * (dPhc - hi) / dmx1[i]);
+    invRhoc = 0.5 / dmx2[j] * (dx2[j] * IRt(jm, k) + dx2[jm] * IRt(j, k));
+    if (j >= 2)
+        V[q] = V[q] - dt * Cmp * ((dPc - dP[qj]) / dmx2[j] + (invRhoc - 1.0) * (dPhc - hj) / dmx2[j]);
+    else
+        V[q] = V[q] - dt * Cmp * ((dPc - dP[qj]) / dmx2[j] + (invRhoc - 1.0) * (dPhc - hj) / dmx2[j]) * 0.0;
+    invRhoc = 0.5 / dmx3[k] * (dx3[k] * IRt(j, km) + dx3[km] * IRt(j, k));
+    W[q] = W[q] - dt * Cmp * ((dPc - dP[qk]) / dmx3[k] + (invRhoc - 1.0) * (dPhc - hk) / dmx3[k]);
+    P[q] = P[q] + dPc;
+}
+
+__global__ void __launch_bounds__(256)
+k_dphat_update(Grid g, const double *__restrict__ dP, double *__restrict__ dPhat, double *__restrict__ dP0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.n1m || j > g.n2m) return;
+    const size_t q = g.ix(i, j, k);
+    const double d0 = (dPhat[q] + dP0[q]) / 2.0;
+    dP0[q] = d0;
+    dPhat[q] = 2.0 * dP[q] - d0;
+}
+
+// ---------------------------------------------------------------- host -----
+static int fft_tile(int n1m, int n3m) {
+    int tx = 1;
+    while (tx < 8 && tx * 2 <= n3m && (tx * 2) * (n1m / 2) <= FFT_MAXB * FFT_THREADS &&
+           (size_t)(tx * 2) * n1m * 16 <= 65536)
+        tx *= 2;
+    return tx;
+}
+
+int pressure_rhs(tcs_ctx *c, double dt) {
+    const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
+    hipLaunchKernelGGL(k_pressure_rhs, grd, blk, 0, c->stream, c->grid(), c->phys, dt, c->f[TCS_U], c->f[TCS_V],
+                       c->f[TCS_W], c->f[TCS_DPHAT], c->f[TCS_INVRHO], c->wall[TCS_VBCUP], c->wall[TCS_VBCBT],
+                       c->prhs);
+    TCS_LAUNCH_CHECK();
+    return TCS_OK;
+}
+
+int pressure_poisson(tcs_ctx *c) {
+    const Grid g = c->grid();
+    const int n1m = c->n1m, n2m = c->n2m, n3m = c->n3m, h1p = c->h1p;
+    const int TX = fft_tile(n1m, n3m);
+    double2 *spec = (double2 *)c->spec_x;
+    const size_t ldsx = (size_t)TX * n1m * 2 * sizeof(double);
+    hipLaunchKernelGGL(k_fft_x_fwd, dim3(n3m / TX, n2m), dim3(FFT_THREADS), ldsx, c->stream, g, h1p, TX, c->prhs,
+                       (const double2 *)c->tw1, spec);
+    TCS_LAUNCH_CHECK();
+    const size_t ldsz = (size_t)n3m * 2 * sizeof(double);
+    hipLaunchKernelGGL(k_fft_z<-1>, dim3(h1p, n2m), dim3(FFT_THREADS), ldsz, c->stream, n3m, h1p,
+                       (const double2 *)c->tw3, spec);
+    TCS_LAUNCH_CHECK();
+    const int nsys = n3m * h1p;
+    hipLaunchKernelGGL(k_poisson_tdma, dim3((nsys + 255) / 256), dim3(256), 0, c->stream, n3m, h1p, n2m, c->h3p,
+                       c->ptab, c->ptab + n2m, c->ptab + 2 * n2m, c->dxk2, c->dzk, spec, c->pA, c->pC);
+    TCS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_fft_z<+1>, dim3(h1p, n2m), dim3(FFT_THREADS), ldsz, c->stream, n3m, h1p,
+                       (const double2 *)c->tw3, spec);
+    TCS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_fft_x_inv, dim3(n3m / TX, n2m), dim3(FFT_THREADS), ldsx, c->stream, g, h1p, TX,
+                       (const double2 *)spec, (const double2 *)c->tw1, c->f[TCS_DP], c->linesum);
+    TCS_LAUNCH_CHECK();
+    int P2 = 1;
+    while (P2 < n2m) P2 <<= 1;
+    hipLaunchKernelGGL(k_plane_sum, dim3(n3m), dim3(256), P2 * sizeof(double), c->stream, n2m, c->linesum,
+                       c->planesum);
+    TCS_LAUNCH_CHECK();
+    int P3 = 1;
+    while (P3 < n3m) P3 <<= 1;
+    hipLaunchKernelGGL(k_total_mean, dim3(1), dim3(256), P3 * sizeof(double), c->stream, n1m, n2m, n3m,
+                       c->planesum, c->scal);
+    TCS_LAUNCH_CHECK();
+    const dim3 blk = interior_block(), grd = interior_grid(n1m, n2m, n3m);
+    hipLaunchKernelGGL(k_dp_subtract, grd, blk, 0, c->stream, g, c->scal, c->f[TCS_DP]);
+    TCS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_wall_extrap, dim3((n1m + 63) / 64, n3m), dim3(64), 0, c->stream, g, c->f[TCS_DP]);
+    TCS_LAUNCH_CHECK();
+    return TCS_OK;
+}
+
+int pressure_projection(tcs_ctx *c, double dt) {
+    const Grid g = c->grid();
+    const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
+    hipLaunchKernelGGL(k_projection, grd, blk, 0, c->stream, g, c->phys, dt, c->f[TCS_INVRHO], c->f[TCS_DP],
+                       c->f[TCS_DPHAT], c->f[TCS_DP0], c->f[TCS_U], c->f[TCS_V], c->f[TCS_W], c->f[TCS_P]);
+    TCS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_dphat_update, grd, blk, 0, c->stream, g, c->f[TCS_DP], c->f[TCS_DPHAT], c->f[TCS_DP0]);
+    TCS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_wall_extrap, dim3((c->n1m + 63) / 64, c->n3m), dim3(64), 0, c->stream, g, c->f[TCS_P]);
+    TCS_LAUNCH_CHECK();
+    return TCS_OK;
+}
+
+}  // namespace tcs

@@ -1,0 +1,635 @@
+// runtime.hip -- context lifecycle, module-array residency, halo, post
+// reductions and the time-step composition behind the C-ABI of pascal_tcs.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "tcs_internal.h"
+
+namespace tcs {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+// ------------------------------------------------------------ halo --------
+// Single-rank mpi_subdomain_ghostcell_update (src/module_mpi_subdomain.f90:
+// 758-791): the x pass copies planes i=n1m -> 0 and i=1 -> n1sub for every
+// (j,k), then the z pass copies whole (i,j) planes k=n3m -> 0, k=1 -> n3sub
+// (including the x ghosts just written); y is a wall (no exchange).  Both
+// passes together equal: every ghost cell with i or k on a ghost plane takes
+// the value at (mx(i), j, mz(k)), where mx/mz fold ghost indices onto the
+// periodic interior.  Sources are never ghosts in x or z, so one kernel does
+// it without ordering hazards.
+__global__ void k_halo(Grid g, double *__restrict__ f) {
+    const int N1 = g.N1, N2 = g.N2, N3 = g.N3;
+    const int n1m = g.n1m, n3m = g.n3m;
+    // threads over the x-ghost faces (2*N2*N3) then the z-ghost faces (2*N1*N2)
+    const size_t nx = (size_t)2 * N2 * N3, nz = (size_t)2 * N1 * N2;
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nx) {
+        const int side = (int)(q % 2);
+        const size_t r = q / 2;
+        const int j = (int)(r % N2), k = (int)(r / N2);
+        if (k == 0 || k == N3 - 1) return;  // written by the z faces
+        const int i = side ? n1m + 1 : 0;
+        const int is = side ? 1 : n1m;
+        f[g.ix(i, j, k)] = f[g.ix(is, j, k)];
+    } else if (q < nx + nz) {
+        const size_t r = q - nx;
+        const int side = (int)(r % 2);
+        const size_t t = r / 2;
+        const int i = (int)(t % N1), j = (int)(t / N1);
+        const int k = side ? n3m + 1 : 0;
+        const int ks = side ? 1 : n3m;
+        const int is = (i == 0) ? n1m : (i == n1m + 1 ? 1 : i);
+        f[g.ix(i, j, k)] = f[g.ix(is, j, ks)];
+    }
+}
+
+int halo(tcs_ctx *c, double *f) {
+    const size_t n = (size_t)2 * c->N2 * c->N3 + (size_t)2 * c->N1 * c->N2;
+    hipLaunchKernelGGL(k_halo, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->grid(), f);
+    TCS_LAUNCH_CHECK();
+    return TCS_OK;
+}
+
+// ------------------------------------------------------------ reductions --
+constexpr int RED_BLOCKS = 1024;
+
+// mode 0: signed divergence (src/module_post.f90:134-150)
+// mode 1: sum |u|/dx (src/module_post.f90:200-216)
+template <int MODE>
+__global__ void __launch_bounds__(256)
+k_post_max(Grid g, const double *__restrict__ U, const double *__restrict__ V, const double *__restrict__ W,
+           double *__restrict__ partial) {
+    __shared__ double s[256];
+    const size_t n = (size_t)g.n1m * g.n2m * g.n3m;
+    double m = -INFINITY;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(q % g.n1m) + 1;
+        const size_t r = q / g.n1m;
+        const int j = (int)(r % g.n2m) + 1, k = (int)(r / g.n2m) + 1;
+        double v;
+        if (MODE == 0)
+            v = (U[g.ix(i + 1, j, k)] - U[g.ix(i, j, k)]) / g.dx1[i]
+              + (V[g.ix(i, j + 1, k)] - V[g.ix(i, j, k)]) / g.dx2[j]
+              + (W[g.ix(i, j, k + 1)] - W[g.ix(i, j, k)]) / g.dx3[k];
+        else
+            v = (fabs(U[g.ix(i + 1, j, k)] + U[g.ix(i, j, k)]) / 2.0) / g.dx1[i]
+              + (fabs(V[g.ix(i, j + 1, k)] + V[g.ix(i, j, k)]) / 2.0) / g.dx2[j]
+              + (fabs(W[g.ix(i, j, k + 1)] + W[g.ix(i, j, k)]) / 2.0) / g.dx3[k];
+        m = fmax(m, v);
+    }
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int h = 128; h >= 1; h >>= 1) {
+        if ((int)threadIdx.x < h) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+__global__ void k_final_max(int n, const double *__restrict__ partial, double *__restrict__ out) {
+    __shared__ double s[256];
+    double m = -INFINITY;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) m = fmax(m, partial[q]);
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int h = 128; h >= 1; h >>= 1) {
+        if ((int)threadIdx.x < h) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = s[0];
+}
+
+template <int MODE>
+static int post_max(tcs_ctx *c, double *out) {
+    hipLaunchKernelGGL(k_post_max<MODE>, dim3(RED_BLOCKS), dim3(256), 0, c->stream, c->grid(), c->f[TCS_U],
+                       c->f[TCS_V], c->f[TCS_W], c->red);
+    TCS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_final_max, dim3(1), dim3(256), 0, c->stream, RED_BLOCKS, c->red, c->red + RED_BLOCKS);
+    TCS_LAUNCH_CHECK();
+    TCS_HIP(hipMemcpyAsync(c->h_pinned, c->red + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    TCS_HIP(hipStreamSynchronize(c->stream));
+    *out = c->h_pinned[0];
+    return TCS_OK;
+}
+
+int post_div(tcs_ctx *c, double *out) { return post_max<0>(c, out); }
+int post_cfl(tcs_ctx *c, double *out) { return post_max<1>(c, out); }
+
+// wall Nusselt numbers: pairwise sums over i per k, then over k
+__global__ void k_nusselt_lines(Grid g, double Tup, double Tbt, const double *__restrict__ T,
+                                double *__restrict__ lines) {
+    extern __shared__ double buf[];
+    const int k = blockIdx.x + 1;
+    const int side = blockIdx.y;
+    int P = 1;
+    while (P < g.n1m) P <<= 1;
+    for (int t = threadIdx.x; t < P; t += blockDim.x) {
+        double v = 0.0;
+        if (t < g.n1m) {
+            const int i = t + 1;
+            if (side == 0) v = (Tbt - T[g.ix(i, 1, k)]) / g.dmx2[1];
+            else v = (T[g.ix(i, g.n2m, k)] - Tup) / g.dmx2[g.n2m + 1];
+        }
+        buf[t] = v;
+    }
+    __syncthreads();
+    for (int h = P / 2; h >= 1; h >>= 1) {
+        for (int t = threadIdx.x; t < h; t += blockDim.x) buf[t] = buf[t] + buf[t + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) lines[(size_t)side * g.n3m + (k - 1)] = buf[0];
+}
+
+__global__ void k_nusselt_total(int n1m, int n3m, double L2, double dT, const double *__restrict__ lines,
+                                double *__restrict__ out) {
+    extern __shared__ double buf[];
+    int P = 1;
+    while (P < n3m) P <<= 1;
+    for (int side = 0; side < 2; side++) {
+        for (int t = threadIdx.x; t < P; t += blockDim.x) buf[t] = (t < n3m) ? lines[(size_t)side * n3m + t] : 0.0;
+        __syncthreads();
+        for (int h = P / 2; h >= 1; h >>= 1) {
+            for (int t = threadIdx.x; t < h; t += blockDim.x) buf[t] = buf[t] + buf[t + h];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[side] = buf[0] / (double)n1m / (double)n3m * L2 / dT;
+        __syncthreads();
+    }
+}
+
+int nusselt(tcs_ctx *c, double Tup, double Tbt, double *nb, double *nt) {
+    int P1 = 1, P3 = 1;
+    while (P1 < c->n1m) P1 <<= 1;
+    while (P3 < c->n3m) P3 <<= 1;
+    hipLaunchKernelGGL(k_nusselt_lines, dim3(c->n3m, 2), dim3(256), P1 * sizeof(double), c->stream, c->grid(), Tup,
+                       Tbt, c->f[TCS_T], c->red);
+    TCS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_nusselt_total, dim3(1), dim3(256), P3 * sizeof(double), c->stream, c->n1m, c->n3m,
+                       c->prm.L2, Tbt - Tup, c->red, c->red + 2 * c->n3m);
+    TCS_LAUNCH_CHECK();
+    TCS_HIP(hipMemcpyAsync(c->h_pinned, c->red + 2 * c->n3m, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    TCS_HIP(hipStreamSynchronize(c->stream));
+    *nb = c->h_pinned[0];
+    *nt = c->h_pinned[1];
+    return TCS_OK;
+}
+
+// ------------------------------------------------------ boundary rows ----
+__global__ void k_fill_wall(int n, double v, double *__restrict__ w) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) w[q] = v;
+}
+
+// f(:,jrow,:) = w(:,:)
+__global__ void k_set_row(Grid g, int jrow, const double *__restrict__ w, double *__restrict__ f) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (i >= g.N1) return;
+    f[g.ix(i, jrow, k)] = w[g.b2(i, k)];
+}
+
+// expand a (j,k) table to the full field layout (row broadcast along i)
+__global__ void k_expand_table(Grid g, const double *__restrict__ t, double *__restrict__ out) {
+    const size_t n = (size_t)g.N1 * g.N2 * g.N3;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const size_t r = q / g.N1;
+        out[q] = t[r];
+    }
+}
+
+// the table of a full array's i = n1sub column
+__global__ void k_gather_table(Grid g, const double *__restrict__ full, double *__restrict__ t) {
+    const size_t n = (size_t)g.N2 * g.N3;
+    for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (size_t)gridDim.x * blockDim.x)
+        t[r] = full[(size_t)(g.N1 - 1) + (size_t)g.N1 * r];
+}
+
+}  // namespace tcs
+
+using namespace tcs;
+
+// ---------------------------------------------------------------- C-ABI ---
+extern "C" const char *tcs_last_error(void) { return g_err.c_str(); }
+
+extern "C" int tcs_device_count(int *count) {
+    if (!count) return TCS_E_ARG;
+    TCS_HIP(hipGetDeviceCount(count));
+    return TCS_OK;
+}
+
+extern "C" int tcs_mesh_axis(int32_t nm, double L, int32_t uniform, double gamma, int32_t periodic, double *x,
+                             double *dx, double *dmx) {
+    if (nm < 1 || !x || !dx || !dmx) {
+        set_error("tcs_mesh_axis: bad arguments");
+        return TCS_E_ARG;
+    }
+    const int n = nm + 1;
+    for (int i = 0; i <= n; i++) { x[i] = 0.0; dx[i] = 0.0; dmx[i] = 0.0; }
+    for (int i = 0; i <= n; i++) {
+        if (uniform == 1)
+            x[i] = (double)(i - 1) * L / (double)nm + 0.0;
+        else
+            x[i] = L * 0.5 * (1.0 + std::tanh(0.5 * gamma * (2.0 * (double)(i - 1) / (double)nm - 1.0)) / std::tanh(gamma * 0.5)) + 0.0;
+    }
+    if (!periodic) x[0] = x[1];
+    for (int i = 1; i <= nm; i++) dx[i] = x[i + 1] - x[i];
+    if (periodic) { dx[0] = dx[nm]; dx[n] = dx[1]; }
+    else { dx[0] = 0.0; dx[n] = 0.0; }
+    for (int i = 1; i <= nm; i++) dmx[i] = 0.5 * (dx[i - 1] + dx[i]);
+    if (periodic) { dmx[0] = dmx[nm]; dmx[n] = dmx[1]; }
+    else dmx[n] = 0.5 * (dx[nm] + dx[n]);
+    return TCS_OK;
+}
+
+static bool pow2(int n) { return n >= 2 && (n & (n - 1)) == 0; }
+
+static int dalloc(double **p, size_t n) {
+    if (hipMalloc((void **)p, n * sizeof(double) > 0 ? n * sizeof(double) : 8) != hipSuccess) {
+        set_error("hipMalloc failed (" + std::to_string(n * sizeof(double)) + " bytes)");
+        *p = nullptr;
+        return TCS_E_NOMEM;
+    }
+    if (hipMemset(*p, 0, n * sizeof(double)) != hipSuccess) {
+        set_error("hipMemset failed");
+        return TCS_E_HIP;
+    }
+    return TCS_OK;
+}
+
+static void free_ctx(tcs_ctx *c) {
+    if (!c) return;
+    double **bufs[] = {&c->dx1, &c->dx2, &c->dx3, &c->dmx1, &c->dmx2, &c->dmx3, &c->am, &c->ac, &c->ap, &c->rhs,
+                       &c->stale, &c->prhs, &c->spec_x, &c->spec_y, &c->pA, &c->pC, &c->dxk2, &c->dzk, &c->ptab,
+                       &c->tw1, &c->tw3, &c->linesum, &c->planesum, &c->scal, &c->red};
+    for (double **b : bufs)
+        if (*b) (void)hipFree(*b);
+    for (int q = 0; q < TCS_NFIELD; q++)
+        if (c->f[q]) (void)hipFree(c->f[q]);
+    for (int q = 0; q < TCS_NWALL; q++)
+        if (c->wall[q]) (void)hipFree(c->wall[q]);
+    if (c->jC) (void)hipFree(c->jC);
+    if (c->jS) (void)hipFree(c->jS);
+    if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    free(c->h_dx1); free(c->h_dx2); free(c->h_dx3); free(c->h_dmx1); free(c->h_dmx2); free(c->h_dmx3);
+    delete c;
+}
+
+static int upload_tables(tcs_ctx *c) {
+    const int n1m = c->n1m, n2m = c->n2m, n3m = c->n3m;
+    const double PI = std::acos(-1.0);
+    // mpi_pressure_wave_number (src/module_solve_pressure.f90:93-129), one rank
+    std::vector<double> dzk(n3m), dxk2(c->h1p);
+    const double L1 = c->prm.L1, L3 = c->prm.L3;
+    const double ddx = L1 / (double)n1m, ddz = L3 / (double)n3m;
+    for (int k = 1; k <= n3m; k++) {
+        const int km = (k <= c->h3p) ? k - 1 : n3m - k + 1;
+        dzk[k - 1] = 2.0 * (1.0 - std::cos(2.0 * PI * (double)km * ddz / L3)) / (ddz * ddz);
+    }
+    for (int i = 1; i <= c->h1p; i++)
+        dxk2[i - 1] = 2.0 * (1.0 - std::cos(2.0 * PI * (double)(i - 1) * ddx / L1)) / (ddx * ddx);
+    TCS_HIP(hipMemcpy(c->dzk, dzk.data(), n3m * sizeof(double), hipMemcpyHostToDevice));
+    TCS_HIP(hipMemcpy(c->dxk2, dxk2.data(), c->h1p * sizeof(double), hipMemcpyHostToDevice));
+    // y coefficients of the Poisson systems (:599-611)
+    std::vector<double> tab(3 * (size_t)n2m);
+    for (int j = 1; j <= n2m; j++) {
+        double amj = 1.0 / c->h_dx2[j] / c->h_dmx2[j];
+        double apj = 1.0 / c->h_dx2[j] / c->h_dmx2[j + 1];
+        if (j == 1) amj = 0.0;
+        if (j == n2m) apj = 0.0;
+        tab[j - 1] = amj;
+        tab[n2m + j - 1] = apj;
+        tab[2 * n2m + j - 1] = -amj - apj;
+    }
+    TCS_HIP(hipMemcpy(c->ptab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    // twiddles W_n^k = exp(-2 pi i k / n), k < n/2
+    auto twid = [&](int n, double *dst) -> int {
+        std::vector<double> w((size_t)n);
+        for (int k = 0; k < n / 2; k++) {
+            const double a = 2.0 * PI * (double)k / (double)n;
+            w[2 * k] = std::cos(a);
+            w[2 * k + 1] = -std::sin(a);
+        }
+        TCS_HIP(hipMemcpy(dst, w.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+        return TCS_OK;
+    };
+    TCS_TRY(twid(n1m, c->tw1));
+    TCS_TRY(twid(n3m, c->tw3));
+    return TCS_OK;
+}
+
+extern "C" int tcs_init(const tcs_params *prm, const double *dx1, const double *dx2, const double *dx3,
+                        const double *dmx1, const double *dmx2, const double *dmx3, tcs_ctx **out) {
+    if (!prm || !dx1 || !dx2 || !dx3 || !dmx1 || !dmx2 || !dmx3 || !out) {
+        set_error("tcs_init: null argument");
+        return TCS_E_ARG;
+    }
+    *out = nullptr;
+    if (!(prm->pbc1 == 1 && prm->pbc2 == 0 && prm->pbc3 == 1)) {
+        set_error("tcs_init: only periodic x/z with walls in y (pbc = T,F,T) is supported");
+        return TCS_E_ARG;
+    }
+    if (!pow2(prm->n1m) || !pow2(prm->n3m) || prm->n1m < 4 || prm->n3m < 4 || prm->n1m > 4096 ||
+        prm->n3m > 4096 || prm->n2m < 3) {
+        set_error("tcs_init: n1m and n3m must be powers of two in [4, 4096] and n2m >= 3");
+        return TCS_E_ARG;
+    }
+    if (prm->precision != 0) {
+        set_error("tcs_init: only the plain FP64 form (precision = 0) is built");
+        return TCS_E_ARG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        set_error("tcs_init: no HIP device");
+        return TCS_E_HIP;
+    }
+    tcs_ctx *c = new tcs_ctx();
+    std::memset((void *)c, 0, sizeof(tcs_ctx));
+    c->prm = *prm;
+    Phys &p = c->phys;
+    p.Ct = prm->Ct; p.Cmu = prm->Cmu; p.Cmt = prm->Cmt; p.Cmp = prm->Cmp; p.DeltaT = prm->DeltaT;
+    p.a10 = prm->a10; p.a11 = prm->a11; p.a12 = prm->a12; p.a13 = prm->a13; p.a14 = prm->a14; p.a15 = prm->a15;
+    p.b10 = prm->b10; p.b11 = prm->b11; p.b12 = prm->b12; p.b13 = prm->b13; p.b14 = prm->b14; p.b15 = prm->b15;
+    p.c10 = prm->c10; p.c11 = prm->c11; p.c12 = prm->c12; p.c13 = prm->c13; p.c14 = prm->c14; p.c15 = prm->c15;
+    p.d10 = prm->d10; p.d11 = prm->d11; p.d12 = prm->d12; p.d13 = prm->d13; p.d14 = prm->d14; p.d15 = prm->d15;
+    p.KPP0 = prm->KPP0; p.Rho0 = prm->Rho0; p.Cp0 = prm->Cp0; p.Mu0 = prm->Mu0; p.a12pera11 = prm->a12pera11;
+
+    c->n1m = prm->n1m; c->n2m = prm->n2m; c->n3m = prm->n3m;
+    c->N1 = c->n1m + 2; c->N2 = c->n2m + 2; c->N3 = c->n3m + 2;
+    c->nfull = (size_t)c->N1 * c->N2 * c->N3;
+    c->nint = (size_t)c->n1m * c->n2m * c->n3m;
+    c->nwall = (size_t)c->N1 * c->N3;
+    c->ntab = (size_t)c->N2 * c->N3;
+    c->h1p = c->n1m / 2 + 1;
+    c->h3p = c->n3m / 2 + 1;
+    int rc = TCS_OK;
+    auto fail = [&](int r) { free_ctx(c); return r; };
+    if (hipGetDevice(&c->device) != hipSuccess) { set_error("hipGetDevice failed"); return fail(TCS_E_HIP); }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        set_error("hipStreamCreate failed");
+        return fail(TCS_E_HIP);
+    }
+    auto hcopy = [](const double *src, int n) {
+        double *d = (double *)malloc(sizeof(double) * n);
+        std::memcpy(d, src, sizeof(double) * n);
+        return d;
+    };
+    c->h_dx1 = hcopy(dx1, c->N1); c->h_dmx1 = hcopy(dmx1, c->N1);
+    c->h_dx2 = hcopy(dx2, c->N2); c->h_dmx2 = hcopy(dmx2, c->N2);
+    c->h_dx3 = hcopy(dx3, c->N3); c->h_dmx3 = hcopy(dmx3, c->N3);
+
+    struct { double **p; size_t n; } allocs[] = {
+        {&c->dx1, (size_t)c->N1}, {&c->dmx1, (size_t)c->N1}, {&c->dx2, (size_t)c->N2}, {&c->dmx2, (size_t)c->N2},
+        {&c->dx3, (size_t)c->N3}, {&c->dmx3, (size_t)c->N3},
+        {&c->am, c->nint}, {&c->ac, c->nint}, {&c->ap, c->nint}, {&c->rhs, c->nint},
+        {&c->stale, 16}, {&c->prhs, c->nint},
+        {&c->spec_x, (size_t)2 * c->h1p * c->n2m * c->n3m},
+        {&c->pA, (size_t)c->h1p * c->n2m * c->n3m}, {&c->pC, (size_t)c->h1p * c->n2m * c->n3m},
+        {&c->dxk2, (size_t)c->h1p}, {&c->dzk, (size_t)c->n3m}, {&c->ptab, (size_t)3 * c->n2m},
+        {&c->tw1, (size_t)c->n1m}, {&c->tw3, (size_t)c->n3m},
+        {&c->linesum, (size_t)c->n2m * c->n3m}, {&c->planesum, (size_t)c->n3m}, {&c->scal, 8},
+        {&c->red, (size_t)RED_BLOCKS + 16 + (size_t)4 * c->n3m},
+    };
+    for (auto &a : allocs)
+        if ((rc = dalloc(a.p, a.n)) != TCS_OK) return fail(rc);
+    for (int q = 0; q < TCS_NFIELD; q++) {
+        const size_t n = (q == TCS_INVRHO || q == TCS_MU) ? c->ntab : c->nfull;
+        if ((rc = dalloc(&c->f[q], n)) != TCS_OK) return fail(rc);
+    }
+    for (int q = 0; q < TCS_NWALL; q++)
+        if ((rc = dalloc(&c->wall[q], c->nwall)) != TCS_OK) return fail(rc);
+    if (hipMalloc((void **)&c->jC, c->N2 * sizeof(int)) != hipSuccess ||
+        hipMalloc((void **)&c->jS, c->N2 * sizeof(int)) != hipSuccess) {
+        set_error("hipMalloc failed");
+        return fail(TCS_E_NOMEM);
+    }
+    if (hipHostMalloc((void **)&c->h_pinned, 64, hipHostMallocDefault) != hipSuccess) {
+        set_error("hipHostMalloc failed");
+        return fail(TCS_E_NOMEM);
+    }
+    // masks (src/module_mpi_subdomain.f90:304-374), pbc2 = F, one rank
+    std::vector<int> jC(c->N2, 1), jS(c->N2, 1);
+    jC[0] = 0; jS[0] = 0; jS[1] = 0;
+    jC[c->n2m + 1] = 0; jS[c->n2m + 1] = 0;
+    if (hipMemcpy(c->jC, jC.data(), c->N2 * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->jS, jS.data(), c->N2 * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->dx1, dx1, c->N1 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->dmx1, dmx1, c->N1 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->dx2, dx2, c->N2 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->dmx2, dmx2, c->N2 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->dx3, dx3, c->N3 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->dmx3, dmx3, c->N3 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("tcs_init: metric upload failed");
+        return fail(TCS_E_HIP);
+    }
+    if ((rc = upload_tables(c)) != TCS_OK) return fail(rc);
+    if (hipDeviceSynchronize() != hipSuccess) {
+        set_error("tcs_init: device error");
+        return fail(TCS_E_HIP);
+    }
+    *out = c;
+    return TCS_OK;
+}
+
+extern "C" int tcs_finalize(tcs_ctx *c) {
+    if (!c) return TCS_E_ARG;
+    (void)hipStreamSynchronize(c->stream);
+    free_ctx(c);
+    return TCS_OK;
+}
+
+extern "C" int tcs_sync(tcs_ctx *c) {
+    if (!c) return TCS_E_ARG;
+    TCS_HIP(hipStreamSynchronize(c->stream));
+    return TCS_OK;
+}
+
+extern "C" void *tcs_stream(tcs_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+static int check_field(tcs_ctx *c, int id) {
+    if (!c || id < 0 || id >= TCS_NFIELD) {
+        set_error("bad context or field id");
+        return TCS_E_ARG;
+    }
+    return TCS_OK;
+}
+
+extern "C" int tcs_field_upload(tcs_ctx *c, int32_t id, const double *host) {
+    TCS_TRY(check_field(c, id));
+    if (!host) return TCS_E_ARG;
+    if (id == TCS_INVRHO || id == TCS_MU) {
+        // the reference's arrays are row-broadcast: keep the i = n1sub column
+        std::vector<double> t(c->ntab);
+        for (size_t r = 0; r < c->ntab; r++) t[r] = host[(size_t)(c->N1 - 1) + (size_t)c->N1 * r];
+        TCS_HIP(hipMemcpyAsync(c->f[id], t.data(), c->ntab * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        TCS_HIP(hipStreamSynchronize(c->stream));
+        return TCS_OK;
+    }
+    TCS_HIP(hipMemcpyAsync(c->f[id], host, c->nfull * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    TCS_HIP(hipStreamSynchronize(c->stream));
+    return TCS_OK;
+}
+
+extern "C" int tcs_field_download(tcs_ctx *c, int32_t id, double *host) {
+    TCS_TRY(check_field(c, id));
+    if (!host) return TCS_E_ARG;
+    if (id == TCS_INVRHO || id == TCS_MU) {
+        std::vector<double> t(c->ntab);
+        TCS_HIP(hipMemcpyAsync(t.data(), c->f[id], c->ntab * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        TCS_HIP(hipStreamSynchronize(c->stream));
+        for (size_t r = 0; r < c->ntab; r++)
+            for (int i = 0; i < c->N1; i++) host[(size_t)i + (size_t)c->N1 * r] = t[r];
+        return TCS_OK;
+    }
+    TCS_HIP(hipMemcpyAsync(host, c->f[id], c->nfull * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    TCS_HIP(hipStreamSynchronize(c->stream));
+    return TCS_OK;
+}
+
+extern "C" int tcs_field_device_ptr(tcs_ctx *c, int32_t id, double **dptr) {
+    TCS_TRY(check_field(c, id));
+    if (!dptr) return TCS_E_ARG;
+    *dptr = c->f[id];
+    return TCS_OK;
+}
+
+extern "C" int tcs_wall_upload(tcs_ctx *c, int32_t id, const double *host) {
+    if (!c || id < 0 || id >= TCS_NWALL || !host) return TCS_E_ARG;
+    TCS_HIP(hipMemcpyAsync(c->wall[id], host, c->nwall * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    TCS_HIP(hipStreamSynchronize(c->stream));
+    return TCS_OK;
+}
+
+extern "C" int tcs_wall_download(tcs_ctx *c, int32_t id, double *host) {
+    if (!c || id < 0 || id >= TCS_NWALL || !host) return TCS_E_ARG;
+    TCS_HIP(hipMemcpyAsync(host, c->wall[id], c->nwall * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    TCS_HIP(hipStreamSynchronize(c->stream));
+    return TCS_OK;
+}
+
+static int fill_wall(tcs_ctx *c, int id, double v) {
+    hipLaunchKernelGGL(k_fill_wall, dim3((unsigned)((c->nwall + 255) / 256)), dim3(256), 0, c->stream,
+                       (int)c->nwall, v, c->wall[id]);
+    TCS_LAUNCH_CHECK();
+    return TCS_OK;
+}
+
+static int set_row(tcs_ctx *c, int field, int jrow, int wall) {
+    hipLaunchKernelGGL(k_set_row, dim3((c->N1 + 63) / 64, c->N3), dim3(64), 0, c->stream, c->grid(), jrow,
+                       c->wall[wall], c->f[field]);
+    TCS_LAUNCH_CHECK();
+    return TCS_OK;
+}
+
+extern "C" int tcs_thermal_boundary(tcs_ctx *c, double Tup, double Tbt) {
+    if (!c) return TCS_E_ARG;
+    TCS_TRY(fill_wall(c, TCS_TBCUP, Tup));
+    TCS_TRY(fill_wall(c, TCS_TBCBT, Tbt));
+    // comm_1d_x2%myrank == 0 branch only (single y-rank)
+    TCS_TRY(set_row(c, TCS_T, 0, TCS_TBCBT));
+    return TCS_OK;
+}
+
+extern "C" int tcs_momentum_boundary(tcs_ctx *c, double Uup, double Ubt, double Vup, double Vbt, double Wup,
+                                     double Wbt) {
+    if (!c) return TCS_E_ARG;
+    TCS_TRY(fill_wall(c, TCS_UBCUP, Uup));
+    TCS_TRY(fill_wall(c, TCS_VBCUP, Vup));
+    TCS_TRY(fill_wall(c, TCS_WBCUP, Wup));
+    TCS_TRY(fill_wall(c, TCS_UBCBT, Ubt));
+    TCS_TRY(fill_wall(c, TCS_VBCBT, Vbt));
+    TCS_TRY(fill_wall(c, TCS_WBCBT, Wbt));
+    TCS_TRY(set_row(c, TCS_U, 0, TCS_UBCBT));
+    TCS_TRY(set_row(c, TCS_W, 0, TCS_WBCBT));
+    TCS_TRY(set_row(c, TCS_V, 0, TCS_VBCBT));
+    TCS_TRY(set_row(c, TCS_V, 1, TCS_VBCBT));
+    return TCS_OK;
+}
+
+#define CTX_GUARD(c) \
+    if (!(c)) { set_error("null context"); return TCS_E_ARG; }
+
+extern "C" int tcs_thermal_coeffi(tcs_ctx *c) { CTX_GUARD(c); return thermal_coeffi(c); }
+extern "C" int tcs_thermal_solver(tcs_ctx *c, double dt) { CTX_GUARD(c); return thermal_solver(c, dt); }
+extern "C" int tcs_momentum_coeffi(tcs_ctx *c) { CTX_GUARD(c); return momentum_coeffi(c); }
+extern "C" int tcs_momentum_solvedU(tcs_ctx *c, double dt) { CTX_GUARD(c); return momentum_solvedU(c, dt); }
+extern "C" int tcs_momentum_solvedV(tcs_ctx *c, double dt) { CTX_GUARD(c); return momentum_solvedV(c, dt); }
+extern "C" int tcs_momentum_solvedW(tcs_ctx *c, double dt) { CTX_GUARD(c); return momentum_solvedW(c, dt); }
+extern "C" int tcs_momentum_blockLdV(tcs_ctx *c, double dt) { CTX_GUARD(c); return momentum_blockLdV(c, dt); }
+extern "C" int tcs_momentum_blockLdU(tcs_ctx *c, double dt) { CTX_GUARD(c); return momentum_blockLdU(c, dt); }
+extern "C" int tcs_momentum_pseudoupdateUVW(tcs_ctx *c) { CTX_GUARD(c); return momentum_pseudoupdate(c); }
+extern "C" int tcs_pressure_wave_number(tcs_ctx *c) { CTX_GUARD(c); return upload_tables(c); }
+extern "C" int tcs_pressure_rhs(tcs_ctx *c, double dt) { CTX_GUARD(c); return pressure_rhs(c, dt); }
+extern "C" int tcs_pressure_poisson_fft2(tcs_ctx *c) { CTX_GUARD(c); return pressure_poisson(c); }
+extern "C" int tcs_pressure_projection(tcs_ctx *c, double dt) { CTX_GUARD(c); return pressure_projection(c, dt); }
+
+extern "C" int tcs_ghostcell_update(tcs_ctx *c, int32_t id) {
+    TCS_TRY(check_field(c, id));
+    if (id == TCS_INVRHO || id == TCS_MU) return TCS_OK;  // tables carry no x/z ghosts of their own
+    return halo(c, c->f[id]);
+}
+
+extern "C" int tcs_post_div(tcs_ctx *c, double *maxdiv) {
+    CTX_GUARD(c);
+    if (!maxdiv) return TCS_E_ARG;
+    return post_div(c, maxdiv);
+}
+
+extern "C" int tcs_post_cfl(tcs_ctx *c, double *newcfl, double *newdt) {
+    CTX_GUARD(c);
+    if (!newcfl || !newdt) return TCS_E_ARG;
+    double mx = 0.0;
+    TCS_TRY(post_cfl(c, &mx));
+    // src/module_post.f90:223-226
+    double nd = c->prm.MaxCFL / mx;
+    if (nd > c->prm.dtMax) nd = c->prm.dtMax;
+    *newdt = nd;
+    *newcfl = nd * mx;
+    return TCS_OK;
+}
+
+extern "C" int tcs_nusselt(tcs_ctx *c, double Tup, double Tbt, double *nb, double *nt) {
+    CTX_GUARD(c);
+    if (!nb || !nt) return TCS_E_ARG;
+    return nusselt(c, Tup, Tbt, nb, nt);
+}
+
+// src/main.f90:131-203 (thermal -> momentum -> pressure with main's halos)
+extern "C" int tcs_step(tcs_ctx *c, double dt) {
+    CTX_GUARD(c);
+    TCS_TRY(thermal_solver(c, dt));
+    TCS_TRY(halo(c, c->f[TCS_T]));
+    TCS_TRY(momentum_coeffi(c));
+    TCS_TRY(momentum_solvedU(c, dt));
+    TCS_TRY(halo(c, c->f[TCS_DU]));
+    TCS_TRY(momentum_solvedV(c, dt));
+    TCS_TRY(halo(c, c->f[TCS_DV]));
+    TCS_TRY(momentum_solvedW(c, dt));
+    TCS_TRY(halo(c, c->f[TCS_DW]));
+    TCS_TRY(momentum_blockLdV(c, dt));
+    TCS_TRY(halo(c, c->f[TCS_DV]));
+    TCS_TRY(momentum_blockLdU(c, dt));
+    TCS_TRY(momentum_pseudoupdate(c));
+    TCS_TRY(halo(c, c->f[TCS_U]));
+    TCS_TRY(halo(c, c->f[TCS_V]));
+    TCS_TRY(halo(c, c->f[TCS_W]));
+    TCS_TRY(pressure_rhs(c, dt));
+    TCS_TRY(pressure_poisson(c));
+    TCS_TRY(halo(c, c->f[TCS_DP]));
+    TCS_TRY(pressure_projection(c, dt));
+    TCS_TRY(halo(c, c->f[TCS_U]));
+    TCS_TRY(halo(c, c->f[TCS_V]));
+    TCS_TRY(halo(c, c->f[TCS_W]));
+    TCS_TRY(halo(c, c->f[TCS_P]));
+    TCS_TRY(halo(c, c->f[TCS_DPHAT]));
+    return TCS_OK;
+}

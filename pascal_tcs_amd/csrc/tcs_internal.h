@@ -1,0 +1,150 @@
+// tcs_internal.h -- context layout and device helpers shared by the HIP
+// translation units of libpascal_tcs.  Not part of the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/pascal_tcs.h"
+
+namespace tcs {
+
+void set_error(const std::string &msg);
+
+#define TCS_HIP(expr)                                                          \
+    do {                                                                       \
+        hipError_t _e = (expr);                                                \
+        if (_e != hipSuccess) {                                                \
+            ::tcs::set_error(std::string(#expr) + ": " + hipGetErrorString(_e)); \
+            return TCS_E_HIP;                                                  \
+        }                                                                      \
+    } while (0)
+
+#define TCS_LAUNCH_CHECK()                                                     \
+    do {                                                                       \
+        hipError_t _e = hipGetLastError();                                     \
+        if (_e != hipSuccess) {                                                \
+            ::tcs::set_error(std::string("kernel launch: ") + hipGetErrorString(_e)); \
+            return TCS_E_HIP;                                                  \
+        }                                                                      \
+    } while (0)
+
+#define TCS_TRY(expr)                                                          \
+    do {                                                                       \
+        int _r = (expr);                                                       \
+        if (_r != TCS_OK) return _r;                                           \
+    } while (0)
+
+// Grid description passed by value to every stencil kernel.
+// Full fields: (0:n1m+1, 0:n2m+1, 0:n3m+1), i fastest.
+// Interior work arrays: (1:n1m, 1:n2m, 1:n3m), i fastest.
+struct Grid {
+    int n1m, n2m, n3m;
+    int N1, N2, N3;  // full extents n?m+2
+    const double *dx1, *dx2, *dx3, *dmx1, *dmx2, *dmx3;
+    const int *jC, *jS;
+    __device__ __forceinline__ size_t ix(int i, int j, int k) const {
+        return (size_t)i + (size_t)N1 * ((size_t)j + (size_t)N2 * (size_t)k);
+    }
+    __device__ __forceinline__ size_t b2(int i, int k) const { return (size_t)i + (size_t)N1 * (size_t)k; }
+    __device__ __forceinline__ size_t t2(int j, int k) const { return (size_t)j + (size_t)N2 * (size_t)k; }
+    __device__ __forceinline__ size_t r3(int i, int j, int k) const {
+        return (size_t)(i - 1) + (size_t)n1m * ((size_t)(j - 1) + (size_t)n2m * (size_t)(k - 1));
+    }
+};
+
+// Polynomial property coefficients and nondimensional constants.
+struct Phys {
+    double Ct, Cmu, Cmt, Cmp, DeltaT;
+    double a10, a11, a12, a13, a14, a15;
+    double b10, b11, b12, b13, b14, b15;
+    double c10, c11, c12, c13, c14, c15;
+    double d10, d11, d12, d13, d14, d15;
+    double KPP0, Rho0, Cp0, Mu0, a12pera11;
+};
+
+}  // namespace tcs
+
+struct tcs_ctx {
+    tcs_params prm;
+    tcs::Phys phys;
+    int n1m, n2m, n3m;
+    int N1, N2, N3;
+    size_t nfull, nint, nwall, ntab;
+    int h1p, h3p;
+    int device;
+    hipStream_t stream;
+
+    // host copies of the metrics
+    double *h_dx1, *h_dx2, *h_dx3, *h_dmx1, *h_dmx2, *h_dmx3;
+    // device metrics / masks
+    double *dx1, *dx2, *dx3, *dmx1, *dmx2, *dmx3;
+    int *jC, *jS;
+
+    double *f[TCS_NFIELD];  // INVRHO / MU hold (0:n2sub,0:n3sub) tables
+    double *wall[TCS_NWALL];
+
+    // ADI scratch (interior size each)
+    double *am, *ac, *ap, *rhs;
+    // stale scalars of the momentum stages (mu1,mu2 of dV; mu1..mu4 of dW)
+    double *stale;
+
+    // Poisson
+    double *prhs;            // interior
+    double *spec_x;          // complex (h1p, n2m, n3m), kx fastest
+    double *spec_y;          // complex (h1p, n3m, n2m), kx fastest then kz
+    double *pA, *pC;         // TDMA scratch (h1p*n3m, n2m)
+    double *dxk2, *dzk;      // wave numbers
+    double *ptab;            // y coefficients am(j), ap(j), ac(j) of the Poisson systems
+    double *tw1, *tw3;       // twiddles (n/2 complex each)
+    double *linesum, *planesum, *scal;  // mean reduction
+    double *red;             // reduction scratch
+    double *h_pinned;        // pinned host scalars
+
+    tcs::Grid grid() const {
+        tcs::Grid g;
+        g.n1m = n1m; g.n2m = n2m; g.n3m = n3m;
+        g.N1 = N1; g.N2 = N2; g.N3 = N3;
+        g.dx1 = dx1; g.dx2 = dx2; g.dx3 = dx3;
+        g.dmx1 = dmx1; g.dmx2 = dmx2; g.dmx3 = dmx3;
+        g.jC = jC; g.jS = jS;
+        return g;
+    }
+};
+
+namespace tcs {
+
+// launch helpers -------------------------------------------------------------
+inline dim3 interior_block() { return dim3(64, 4, 1); }
+inline dim3 interior_grid(int nx, int ny, int nz) {
+    return dim3((unsigned)((nx + 63) / 64), (unsigned)((ny + 3) / 4), (unsigned)nz);
+}
+
+// TDMA launcher (exact PaScaL order) on strided systems:
+//   system (s0, s1) starts at base + s0*ss0 + s1*ss1, rows step rs.
+int ptdma_exact_launch(double *A, const double *B, double *C, double *D,
+                       int n0, long ss0, int n1, long ss1, int n_row, long rs,
+                       bool cyclic, hipStream_t st);
+
+// stage entry points implemented in the kernel translation units
+int thermal_solver(tcs_ctx *c, double dt);
+int thermal_coeffi(tcs_ctx *c);
+int momentum_coeffi(tcs_ctx *c);
+int momentum_solvedU(tcs_ctx *c, double dt);
+int momentum_solvedV(tcs_ctx *c, double dt);
+int momentum_solvedW(tcs_ctx *c, double dt);
+int momentum_blockLdV(tcs_ctx *c, double dt);
+int momentum_blockLdU(tcs_ctx *c, double dt);
+int momentum_pseudoupdate(tcs_ctx *c);
+int pressure_rhs(tcs_ctx *c, double dt);
+int pressure_poisson(tcs_ctx *c);
+int pressure_projection(tcs_ctx *c, double dt);
+int halo(tcs_ctx *c, double *f);
+int post_div(tcs_ctx *c, double *out);
+int post_cfl(tcs_ctx *c, double *maxuovx);
+int nusselt(tcs_ctx *c, double Tup, double Tbt, double *nb, double *nt);
+
+}  // namespace tcs

@@ -1,0 +1,194 @@
+// tdma.hip -- batched tridiagonal solves with PaScaL_TDMA's single-rank
+// operation order (PaScaL_TDMA/src/pascal_tdma.f90:487-578 and :715-806,
+// reduced 2-row solves of PaScaL_TDMA/src/tdmas.f90:113-212).
+//
+// One thread owns one system and walks its rows; consecutive threads own
+// systems that are adjacent in memory wherever the caller's layout allows
+// (the z and y sweeps of a field), so every row step is one coalesced
+// 512-byte wavefront access per array.  The three passes of the modified
+// Thomas algorithm (forward elimination with A fill-in, backward elimination
+// with C fill-in, update from the reduced solution) keep their operands in
+// place, exactly as the reference does with its A, C, D arrays.
+#include "tcs_internal.h"
+
+namespace tcs {
+
+template <bool CYC>
+__global__ void __launch_bounds__(256)
+k_ptdma_exact(double *__restrict__ A, const double *__restrict__ B, double *__restrict__ C,
+              double *__restrict__ D, int n0, long ss0, long ss1, int n_row, long rs) {
+    const int s0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s0 >= n0) return;
+    const long base = (long)s0 * ss0 + (long)blockIdx.y * ss1;
+    double *a = A + base;
+    const double *b = B + base;
+    double *c = C + base;
+    double *d = D + base;
+
+    // rows 1 and 2 normalised by their own diagonal (:503-511)
+    const double b0 = b[0], b1 = b[rs];
+    double a0 = a[0] / b0;
+    double d0 = d[0] / b0;
+    double c0 = c[0] / b0;
+    const double a1 = a[rs] / b1;
+    const double d1 = d[rs] / b1;
+    const double c1 = c[rs] / b1;
+    a[rs] = a1; d[rs] = d1; c[rs] = c1;
+
+    // forward elimination j = 3..n_row (:513-520)
+    double ap = a1, cp = c1, dp = d1;
+    for (int j = 2; j < n_row; j++) {
+        const long o = (long)j * rs;
+        const double aj = a[o], bj = b[o], cj = c[o], dj = d[o];
+        const double r = 1.0 / (bj - aj * cp);
+        const double dn = r * (dj - aj * dp);
+        const double cn = r * cj;
+        const double an = -r * aj * ap;
+        a[o] = an; c[o] = cn; d[o] = dn;
+        ap = an; cp = cn; dp = dn;
+    }
+    const double aN = ap, cN = cp, dN = dp;  // last row (forward values)
+
+    // backward elimination j = n_row-2..2 (:523-529)
+    double ax = a1, cx = c1, dx = d1;  // row 2 after the backward sweep
+    if (n_row >= 3) {
+        const long o2 = (long)(n_row - 2) * rs;
+        ax = a[o2]; cx = c[o2]; dx = d[o2];
+        for (int j = n_row - 3; j >= 1; j--) {
+            const long o = (long)j * rs;
+            const double aj = a[o], cj = c[o], dj = d[o];
+            const double dn = dj - cj * dx;
+            const double an = aj - cj * ax;
+            const double cn = -cj * cx;
+            a[o] = an; c[o] = cn; d[o] = dn;
+            ax = an; cx = cn; dx = dn;
+        }
+    }
+
+    // row 1 and the reduced 2-row system (:531-542)
+    {
+        const double r = 1.0 / (1.0 - ax * c0);
+        d0 = r * (d0 - c0 * dx);
+        a0 = r * a0;
+        c0 = -r * c0 * cx;
+    }
+    a[0] = a0; c[0] = c0;
+
+    double x1, xn;
+    if (!CYC) {
+        // tdma_many with n2=2, b=1 (tdmas.f90:113-146); x/1.0 is exact
+        const double r = 1.0 / (1.0 - aN * c0);
+        xn = r * (dN - aN * d0);
+        x1 = d0 - c0 * xn;
+    } else {
+        // tdma_cycl_many with n2=2 (tdmas.f90:159-212): e(2) is set to -a(2)
+        // and then overwritten by e(n2) = -c(n2)
+        const double e2 = -cN;
+        x1 = (d0 - a0 * dN - c0 * dN) / (1.0 + a0 * e2 + c0 * e2);
+        xn = dN + x1 * e2;
+    }
+    d[0] = x1;
+    d[(long)(n_row - 1) * rs] = xn;
+
+    // update of the inner rows (:572-576)
+    for (int j = 1; j < n_row - 1; j++) {
+        const long o = (long)j * rs;
+        d[o] = d[o] - a[o] * x1 - c[o] * xn;
+    }
+}
+
+int ptdma_exact_launch(double *A, const double *B, double *C, double *D,
+                       int n0, long ss0, int n1, long ss1, int n_row, long rs,
+                       bool cyclic, hipStream_t st) {
+    if (n0 <= 0 || n1 <= 0) return TCS_OK;
+    if (n_row < 2 || n1 > 65535) {
+        set_error("ptdma: n_row must be >= 2 and the outer system count <= 65535");
+        return TCS_E_ARG;
+    }
+    dim3 blk(256), grd((unsigned)((n0 + 255) / 256), (unsigned)n1);
+    if (cyclic)
+        hipLaunchKernelGGL(k_ptdma_exact<true>, grd, blk, 0, st, A, B, C, D, n0, ss0, ss1, n_row, rs);
+    else
+        hipLaunchKernelGGL(k_ptdma_exact<false>, grd, blk, 0, st, A, B, C, D, n0, ss0, ss1, n_row, rs);
+    TCS_LAUNCH_CHECK();
+    return TCS_OK;
+}
+
+}  // namespace tcs
+
+using namespace tcs;
+
+static int many_solve(double *A, const double *B, double *C, double *D, int32_t n_sys,
+                      int32_t n_row, bool cyc, void *stream) {
+    if (!A || !B || !C || !D || n_sys < 0 || n_row < 2) {
+        set_error("ptdma_many_solve: null pointer or n_row < 2");
+        return TCS_E_ARG;
+    }
+    // (n_sys, n_row): systems contiguous; split n_sys into (n0 = chunk) x n1
+    // so the outer count stays within the grid's y limit.
+    const int chunk = 65536;
+    int n0 = n_sys < chunk ? n_sys : chunk;
+    int n1 = (n_sys + chunk - 1) / chunk;
+    if (n_sys % n0 != 0) {
+        // ragged tail: two launches
+        int full = n_sys / n0;
+        TCS_TRY(ptdma_exact_launch(A, B, C, D, n0, 1, full, n0, n_row, n_sys, cyc, (hipStream_t)stream));
+        long off = (long)full * n0;
+        return ptdma_exact_launch(A + off, B + off, C + off, D + off, n_sys - (int)off, 1, 1, 0, n_row,
+                                  n_sys, cyc, (hipStream_t)stream);
+    }
+    return ptdma_exact_launch(A, B, C, D, n0, 1, n1, n0, n_row, n_sys, cyc, (hipStream_t)stream);
+}
+
+extern "C" int ptdma_many_solve_dev(double *A, const double *B, double *C, double *D, int32_t n_sys,
+                                    int32_t n_row, void *stream) {
+    return many_solve(A, B, C, D, n_sys, n_row, false, stream);
+}
+
+extern "C" int ptdma_many_solve_cycle_dev(double *A, const double *B, double *C, double *D,
+                                          int32_t n_sys, int32_t n_row, void *stream) {
+    return many_solve(A, B, C, D, n_sys, n_row, true, stream);
+}
+
+extern "C" int ptdma_many_solve_host(double *A, double *B, double *C, double *D, int32_t n_sys,
+                                     int32_t n_row, int32_t cyclic) {
+    if (!A || !B || !C || !D || n_sys <= 0 || n_row < 2) {
+        set_error("ptdma_many_solve_host: bad arguments");
+        return TCS_E_ARG;
+    }
+    size_t bytes = (size_t)n_sys * (size_t)n_row * sizeof(double);
+    double *d[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *h[4] = {A, B, C, D};
+    int rc = TCS_OK;
+    for (int q = 0; q < 4 && rc == TCS_OK; q++) {
+        if (hipMalloc(&d[q], bytes) != hipSuccess) { set_error("hipMalloc failed"); rc = TCS_E_NOMEM; }
+        else if (hipMemcpy(d[q], h[q], bytes, hipMemcpyHostToDevice) != hipSuccess) { set_error("H2D copy failed"); rc = TCS_E_HIP; }
+    }
+    if (rc == TCS_OK) rc = many_solve(d[0], d[1], d[2], d[3], n_sys, n_row, cyclic != 0, nullptr);
+    if (rc == TCS_OK && hipDeviceSynchronize() != hipSuccess) { set_error("ptdma kernel failed"); rc = TCS_E_HIP; }
+    if (rc == TCS_OK) {
+        for (int q : {0, 2, 3})
+            if (hipMemcpy(h[q], d[q], bytes, hipMemcpyDeviceToHost) != hipSuccess) { set_error("D2H copy failed"); rc = TCS_E_HIP; }
+    }
+    for (int q = 0; q < 4; q++)
+        if (d[q]) (void)hipFree(d[q]);
+    return rc;
+}
+
+extern "C" int ptdma_axis_solve_dev(double *A, const double *B, double *C, double *D, int32_t n1,
+                                    int32_t n2, int32_t n3, int32_t axis, int32_t cyclic, void *stream) {
+    if (!A || !B || !C || !D || n1 <= 0 || n2 <= 0 || n3 <= 0 || axis < 0 || axis > 2) {
+        set_error("ptdma_axis_solve_dev: bad arguments");
+        return TCS_E_ARG;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    long s1 = n1, s2 = (long)n1 * n2;
+    switch (axis) {
+    case 0:  // rows along i: systems (j,k)
+        return ptdma_exact_launch(A, B, C, D, n2, s1, n3, s2, n1, 1, cyclic != 0, st);
+    case 1:  // rows along j: systems (i,k)
+        return ptdma_exact_launch(A, B, C, D, n1, 1, n3, s2, n2, s1, cyclic != 0, st);
+    default:  // rows along k: systems (i,j)
+        return ptdma_exact_launch(A, B, C, D, n1, 1, n2, s1, n3, s2, cyclic != 0, st);
+    }
+}

@@ -1,0 +1,373 @@
+// thermal.hip -- mpi_thermal_coeffi / mpi_thermal_solver
+// (src/module_solve_thermal.f90:116-156, :175-532) as HIP kernels.
+//
+// The property polynomials KPP, dKPP, invRhoCp, dinvRhoCp are pure
+// functions of T, so the assembly kernels evaluate them from T in
+// registers instead of reading four stored 3-D arrays; the arithmetic is
+// the reference's, so the values are identical to the stored ones.
+// Each ADI stage is an assembly kernel (one thread per cell, i along the
+// wavefront) that writes am/ac/ap (and the RHS in stage 1) in the interior
+// (i,j,k) layout, followed by the exact-order PaScaL sweep along the
+// stage's axis with strided system access (no transposes).
+#include "tcs_internal.h"
+
+namespace tcs {
+
+__device__ __forceinline__ void th_kpp(const Phys &p, double T, double &kpp, double &dkpp) {
+    const double tmp1 = p.DeltaT * T;
+    const double tmp2 = tmp1 * tmp1;
+    const double tmp3 = tmp2 * tmp1;
+    const double tmp4 = tmp3 * tmp1;
+    const double tmp5 = tmp4 * tmp1;
+    kpp = p.c10 * (1.0 + p.c11 * tmp1 + p.c12 * tmp2 + p.c13 * tmp3 + p.c14 * tmp4 + p.c15 * tmp5) / p.KPP0;
+    dkpp = p.c10 * (p.c11 + 2.0 * p.c12 * tmp1 + 3.0 * p.c13 * tmp2 + 4.0 * p.c14 * tmp3 + 5.0 * p.c15 * tmp4) / p.KPP0;
+}
+
+__device__ __forceinline__ double th_kpp_only(const Phys &p, double T) {
+    double k, dk;
+    th_kpp(p, T, k, dk);
+    return k;
+}
+
+__device__ __forceinline__ double th_dkpp_only(const Phys &p, double T) {
+    double k, dk;
+    th_kpp(p, T, k, dk);
+    return dk;
+}
+
+__device__ __forceinline__ void th_rhocp(const Phys &p, double T, double &irc, double &dirc) {
+    const double tmp1 = p.DeltaT * T;
+    const double tmp2 = tmp1 * tmp1;
+    const double tmp3 = tmp2 * tmp1;
+    const double tmp4 = tmp3 * tmp1;
+    const double tmp5 = tmp4 * tmp1;
+    irc = p.Rho0 * p.Cp0
+        / (p.a10 + p.a10 * (p.a11 * tmp1 + p.a12 * tmp2 + p.a13 * tmp3 + p.a14 * tmp4 + p.a15 * tmp5))
+        / (p.b10 + p.b10 * (p.b11 * tmp1 + p.b12 * tmp2 + p.b13 * tmp3 + p.b14 * tmp4 + p.b15 * tmp5));
+    const double da = 1.0 + p.a11 * tmp1 + p.a12 * tmp2 + p.a13 * tmp3;
+    const double db = 1.0 + p.b11 * tmp1 + p.b12 * tmp2 + p.b13 * tmp3;
+    dirc = -(p.b11 * (1.0 + 3.0 * p.a12 * tmp2 + 4.0 * p.a13 * tmp3)
+             + p.a11 * (1.0 + 2.0 * p.b11 * tmp1 + 3.0 * p.b12 * tmp2 + 4.0 * p.b13 * tmp3)
+             + tmp1 * (p.b12 * (2.0 + 5.0 * p.a13 * tmp3)
+                       + p.a12 * (2.0 + 4.0 * p.b12 * tmp2 + 5.0 * p.b13 * tmp3)
+                       + 3.0 * tmp1 * (p.a13 + p.b13 + 2.0 * p.a13 * p.b13 * tmp3)))
+         / (da * da) / (db * db);
+}
+
+// mpi_thermal_coeffi: materialise the four property arrays over the full field
+__global__ void k_thermal_coeffi(Grid g, Phys p, const double *__restrict__ T, double *__restrict__ KPP,
+                                 double *__restrict__ dKPP, double *__restrict__ iRC, double *__restrict__ diRC) {
+    const size_t n = (size_t)g.N1 * g.N2 * g.N3;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        double k, dk, irc, dirc;
+        th_kpp(p, T[q], k, dk);
+        th_rhocp(p, T[q], irc, dirc);
+        KPP[q] = k; dKPP[q] = dk; iRC[q] = irc; diRC[q] = dirc;
+    }
+}
+
+// stage 1: z (:236-361)
+__global__ void __launch_bounds__(256)
+k_thermal_z(Grid g, Phys p, double dt, const double *__restrict__ T, const double *__restrict__ U,
+            const double *__restrict__ V, const double *__restrict__ W, const double *__restrict__ TBCup,
+            const double *__restrict__ TBCbt, double *__restrict__ am, double *__restrict__ ac,
+            double *__restrict__ ap, double *__restrict__ rhsout) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.n1m || j > g.n2m) return;
+    const int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1, km = k - 1, kp = k + 1;
+    const int jem = g.jC[jm], jep = g.jC[jp];
+    const double *dx1 = g.dx1, *dx2 = g.dx2, *dx3 = g.dx3, *dmx1 = g.dmx1, *dmx2 = g.dmx2, *dmx3 = g.dmx3;
+    const double Ct = p.Ct;
+
+    const double Tc = T[g.ix(i, j, k)];
+    const double Tim = T[g.ix(im, j, k)], Tip = T[g.ix(ip, j, k)];
+    const double Tjm = T[g.ix(i, jm, k)], Tjp = T[g.ix(i, jp, k)];
+    const double Tkm = T[g.ix(i, j, km)], Tkp = T[g.ix(i, j, kp)];
+
+    double irc, dirc;
+    th_rhocp(p, Tc, irc, dirc);
+    double Kc, dKc, Kim, dKim, Kip, dKip, Kjm, dKjm, Kjp, dKjp, Kkm, dKkm, Kkp, dKkp;
+    th_kpp(p, Tc, Kc, dKc);
+    th_kpp(p, Tim, Kim, dKim);
+    th_kpp(p, Tip, Kip, dKip);
+    th_kpp(p, Tjm, Kjm, dKjm);
+    th_kpp(p, Tjp, Kjp, dKjp);
+    th_kpp(p, Tkm, Kkm, dKkm);
+    th_kpp(p, Tkp, Kkp, dKkp);
+
+    const double invRhoCpCt_half = 0.5 * Ct * irc;
+    const double dinvRhoCpCt_half = 0.5 * Ct * dirc;
+
+    const double e1 = (dx1[i] * Tim + dx1[im] * Tc) * (0.5 / dmx1[i]);
+    const double e2 = (dx1[ip] * Tc + dx1[i] * Tip) * (0.5 / dmx1[ip]);
+    const double e3 = (dx2[j] * Tjm + dx2[jm] * Tc) * (0.5 / dmx2[j]);
+    const double e4 = (dx2[jp] * Tc + dx2[j] * Tjp) * (0.5 / dmx2[jp]);
+    const double e5 = (dx3[k] * Tkm + dx3[km] * Tc) * (0.5 / dmx3[k]);
+    const double e6 = (dx3[kp] * Tc + dx3[k] * Tkp) * (0.5 / dmx3[kp]);
+    const double dedx1 = (Tc - Tim) / dmx1[i];
+    const double dedx2 = (Tip - Tc) / dmx1[ip];
+    const double dedy3 = (Tc - Tjm) / dmx2[j];
+    const double dedy4 = (Tjp - Tc) / dmx2[jp];
+    const double dedz5 = (Tc - Tkm) / dmx3[k];
+    const double dedz6 = (Tkp - Tc) / dmx3[kp];
+
+    const double Uc = U[g.ix(i, j, k)], Uip = U[g.ix(ip, j, k)];
+    const double Vc = V[g.ix(i, j, k)], Vjp = V[g.ix(i, jp, k)];
+    const double Wc = W[g.ix(i, j, k)], Wkp = W[g.ix(i, j, kp)];
+    const double convect_e1 = (Uc * dedx1 + Uip * dedx2) * 0.5;
+    const double convect_e2 = (Vc * dedy3 + Vjp * dedy4) * 0.5;
+    const double convect_e3 = (Wc * dedz5 + Wkp * dedz6) * 0.5;
+    double rhs = -0.5 * (convect_e1 + convect_e2 + convect_e3);
+
+    const double kPP1 = 0.5 / dmx1[i] * (dx1[i] * Kim + dx1[im] * Kc);
+    const double kPP2 = 0.5 / dmx1[ip] * (dx1[ip] * Kc + dx1[i] * Kip);
+    const double kPP3 = 0.5 / dmx2[j] * (dx2[j] * Kjm + dx2[jm] * Kc);
+    const double kPP4 = 0.5 / dmx2[jp] * (dx2[jp] * Kc + dx2[j] * Kjp);
+    const double kPP5 = 0.5 / dmx3[k] * (dx3[k] * Kkm + dx3[km] * Kc);
+    const double kPP6 = 0.5 / dmx3[kp] * (dx3[kp] * Kc + dx3[k] * Kkp);
+    const double dkPP1 = 0.5 / dmx1[i] * (dx1[i] * dKim + dx1[im] * dKc);
+    const double dkPP2 = 0.5 / dmx1[ip] * (dx1[ip] * dKc + dx1[i] * dKip);
+    const double dkPP3 = 0.5 / dmx2[j] * (dx2[j] * dKjm + dx2[jm] * dKc);
+    const double dkPP4 = 0.5 / dmx2[jp] * (dx2[jp] * dKc + dx2[j] * dKjp);
+    const double dkPP5 = 0.5 / dmx3[k] * (dx3[k] * dKkm + dx3[km] * dKc);
+    const double dkPP6 = 0.5 / dmx3[kp] * (dx3[kp] * dKc + dx3[k] * dKkp);
+
+    const double viscous_e1 = (kPP2 * dedx2 - kPP1 * dedx1 - e2 * dkPP2 * dedx2 + e1 * dkPP1 * dedx1) / dx1[i];
+    const double viscous_e2 = (kPP4 * dedy4 - kPP3 * dedy3 - e4 * dkPP4 * dedy4 + e3 * dkPP3 * dedy3) / dx2[j];
+    const double viscous_e3 = (kPP6 * dedz6 - kPP5 * dedz5 - e6 * dkPP6 * dedz6 + e5 * dkPP5 * dedz5) / dx3[k];
+
+    rhs = rhs
+        + invRhoCpCt_half * (viscous_e1 + viscous_e2 + viscous_e3)
+        - dinvRhoCpCt_half * Tc * ((kPP2 * dedx2 - kPP1 * dedx1) / dx1[i]
+                                   + (kPP4 * dedy4 - kPP3 * dedy3) / dx2[j]
+                                   + (kPP6 * dedz6 - kPP5 * dedz5) / dx3[k]);
+
+    const double tup = TBCup[g.b2(i, k)], tbt = TBCbt[g.b2(i, k)];
+    double ebc_up = -0.25 * Vjp / dmx2[jp] * tup;
+    double ebc_down = 0.25 * Vc / dmx2[j] * tbt;
+    ebc_up = ebc_up
+           + invRhoCpCt_half / dx2[j] * kPP4 / dmx2[jp] * tup
+           + invRhoCpCt_half / dx2[j] * dkPP4 * dedy4 * tup;
+    ebc_down = ebc_down
+             + invRhoCpCt_half / dx2[j] * kPP3 / dmx2[j] * tbt
+             - invRhoCpCt_half / dx2[j] * dkPP3 * dedy3 * tbt;
+    rhs = rhs + (double)(1 - jem) * ebc_down + (double)(1 - jep) * ebc_up;
+
+    const double eACK = invRhoCpCt_half / dx3[k] * (kPP6 / dmx3[kp] + kPP5 / dmx3[k]
+                                                    - dx3[kp] * (0.5 / dmx3[kp]) * dkPP6 * dedz6
+                                                    + dx3[km] * (0.5 / dmx3[k]) * dkPP5 * dedz5)
+                      + (-0.25 * Wkp / dmx3[kp] + 0.25 * Wc / dmx3[k]);
+    const double eAPK = -invRhoCpCt_half / dx3[k] * (kPP6 / dmx3[kp] + dx3[k] * (0.5 / dmx3[kp]) * dkPP6 * dedz6) + 0.25 * Wkp / dmx3[kp];
+    const double eAMK = -invRhoCpCt_half / dx3[k] * (kPP5 / dmx3[k] - dx3[k] * (0.5 / dmx3[k]) * dkPP5 * dedz5) - 0.25 * Wc / dmx3[k];
+
+    const double eACI = invRhoCpCt_half / dx1[i] * (kPP2 / dmx1[ip] + kPP1 / dmx1[i]
+                                                    - dx1[ip] * (0.5 / dmx1[ip]) * dkPP2 * dedx2
+                                                    + dx1[im] * (0.5 / dmx1[i]) * dkPP1 * dedx1)
+                      + (0.25 * Uc / dmx1[i] - 0.25 * Uip / dmx1[ip]);
+    const double eAPI = -invRhoCpCt_half / dx1[i] * (kPP2 / dmx1[ip] + dx1[i] * (0.5 / dmx1[ip]) * dkPP2 * dedx2) + 0.25 * Uip / dmx1[ip];
+    const double eAMI = -invRhoCpCt_half / dx1[i] * (kPP1 / dmx1[i] - dx1[i] * (0.5 / dmx1[i]) * dkPP1 * dedx1) - 0.25 * Uc / dmx1[i];
+
+    const double eACJ = invRhoCpCt_half / dx2[j] * (kPP4 / dmx2[jp] + kPP3 / dmx2[j]
+                                                    - dx2[jp] * (0.5 / dmx2[jp]) * dkPP4 * dedy4 * (double)jep
+                                                    + dx2[jm] * (0.5 / dmx2[j]) * dkPP3 * dedy3 * (double)jem)
+                      + (0.25 * Vc / dmx2[j] - 0.25 * Vjp / dmx2[jp]);
+    double eAPJ = -invRhoCpCt_half / dx2[j] * (kPP4 / dmx2[jp] + dx2[j] * (0.5 / dmx2[jp]) * dkPP4 * dedy4) + 0.25 * Vjp / dmx2[jp];
+    double eAMJ = -invRhoCpCt_half / dx2[j] * (kPP3 / dmx2[j] - dx2[j] * (0.5 / dmx2[j]) * dkPP3 * dedy3) - 0.25 * Vc / dmx2[j];
+    eAPJ = eAPJ * (double)jep;
+    eAMJ = eAMJ * (double)jem;
+
+    rhs = rhs
+        - (eAPK * Tkp + eACK * Tc + eAMK * Tkm
+           + eAPJ * Tjp + eACJ * Tc + eAMJ * Tjm
+           + eAPI * Tip + eACI * Tc + eAMI * Tim
+           - dinvRhoCpCt_half * Tc * ((kPP2 * dedx2 - kPP1 * dedx1) / dx1[i]
+                                      + (kPP4 * dedy4 - kPP3 * dedy3) / dx2[j]
+                                      + (kPP6 * dedz6 - kPP5 * dedz5) / dx3[k]));
+
+    const size_t q = g.r3(i, j, k);
+    ac[q] = eACK * dt
+          - 0.5 * dinvRhoCpCt_half * ((kPP2 * dedx2 - kPP1 * dedx1) / dx1[i]
+                                      + (kPP4 * dedy4 - kPP3 * dedy3) / dx2[j]
+                                      + (kPP6 * dedz6 - kPP5 * dedz5) / dx3[k]) * dt
+          + 1.0;
+    ap[q] = eAPK * dt;
+    am[q] = eAMK * dt;
+    rhsout[q] = rhs * dt;
+}
+
+// stage 2: x (:388-429).  jm,jp,km,kp are the values the reference's loop
+// variables keep after stage 1: n2m-1, n2m+1, n3m-1, n3m+1.
+__global__ void __launch_bounds__(256)
+k_thermal_x(Grid g, Phys p, double dt, const double *__restrict__ T, const double *__restrict__ U,
+            double *__restrict__ am, double *__restrict__ ac, double *__restrict__ ap) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.n1m || j > g.n2m) return;
+    const int im = i - 1, ip = i + 1;
+    const int jm = g.n2m - 1, jp = g.n2m + 1, km = g.n3m - 1, kp = g.n3m + 1;
+    const double *dx1 = g.dx1, *dx2 = g.dx2, *dx3 = g.dx3, *dmx1 = g.dmx1, *dmx2 = g.dmx2, *dmx3 = g.dmx3;
+
+    const double Tc = T[g.ix(i, j, k)];
+    const double Tim = T[g.ix(im, j, k)], Tip = T[g.ix(ip, j, k)];
+    const double Tjm = T[g.ix(i, jm, k)], Tjp = T[g.ix(i, jp, k)];
+    const double Tkm = T[g.ix(i, j, km)], Tkp = T[g.ix(i, j, kp)];
+    double irc, dirc;
+    th_rhocp(p, Tc, irc, dirc);
+    double Kc, dKc, Kim, dKim, Kip, dKip;
+    th_kpp(p, Tc, Kc, dKc);
+    th_kpp(p, Tim, Kim, dKim);
+    th_kpp(p, Tip, Kip, dKip);
+    const double Kjm = th_kpp_only(p, Tjm), Kjp = th_kpp_only(p, Tjp);
+    const double Kkm = th_kpp_only(p, Tkm), Kkp = th_kpp_only(p, Tkp);
+
+    const double invRhoCpCt_half = 0.5 * p.Ct * irc;
+    const double kPP1 = 0.5 / dmx1[i] * (dx1[i] * Kim + dx1[im] * Kc);
+    const double kPP2 = 0.5 / dmx1[ip] * (dx1[ip] * Kc + dx1[i] * Kip);
+    const double kPP3 = 0.5 / dmx2[j] * (dx2[j] * Kjm + dx2[jm] * Kc);
+    const double kPP4 = 0.5 / dmx2[jp] * (dx2[jp] * Kc + dx2[j] * Kjp);
+    const double kPP5 = 0.5 / dmx3[k] * (dx3[k] * Kkm + dx3[km] * Kc);
+    const double kPP6 = 0.5 / dmx3[kp] * (dx3[kp] * Kc + dx3[k] * Kkp);
+    const double dkPP1 = 0.5 / dmx1[i] * (dx1[i] * dKim + dx1[im] * dKc);
+    const double dkPP2 = 0.5 / dmx1[ip] * (dx1[ip] * dKc + dx1[i] * dKip);
+    const double dedx1 = (Tc - Tim) / dmx1[i];
+    const double dedx2 = (Tip - Tc) / dmx1[ip];
+    const double dedy3 = (Tc - Tjm) / dmx2[j];
+    const double dedy4 = (Tjp - Tc) / dmx2[jp];
+    const double dedz5 = (Tc - Tkm) / dmx3[k];
+    const double dedz6 = (Tkp - Tc) / dmx3[kp];
+    const double Uc = U[g.ix(i, j, k)], Uip = U[g.ix(ip, j, k)];
+
+    const double eACI = invRhoCpCt_half / dx1[i] * (kPP2 / dmx1[ip] + kPP1 / dmx1[i]
+                                                    - dx1[ip] * (0.5 / dmx1[ip]) * dkPP2 * dedx2
+                                                    + dx1[im] * (0.5 / dmx1[i]) * dkPP1 * dedx1)
+                      + (0.25 * Uc / dmx1[i] - 0.25 * Uip / dmx1[ip])
+                      - 0.5 * invRhoCpCt_half * ((kPP2 * dedx2 - kPP1 * dedx1) / dx1[i]
+                                                 + (kPP4 * dedy4 - kPP3 * dedy3) / dx2[j]
+                                                 + (kPP6 * dedz6 - kPP5 * dedz5) / dx3[k]);
+    const double eAPI = -invRhoCpCt_half / dx1[i] * (kPP2 / dmx1[ip] + dx1[i] * (0.5 / dmx1[ip]) * dkPP2 * dedx2) + 0.25 * Uip / dmx1[ip];
+    const double eAMI = -invRhoCpCt_half / dx1[i] * (kPP1 / dmx1[i] - dx1[i] * (0.5 / dmx1[i]) * dkPP1 * dedx1) - 0.25 * Uc / dmx1[i];
+    const size_t q = g.r3(i, j, k);
+    ac[q] = eACI * dt + 1.0;
+    ap[q] = eAPI * dt;
+    am[q] = eAMI * dt;
+}
+
+// stage 3: y (:456-507); km,kp keep their stage-1 values n3m-1, n3m+1.
+__global__ void __launch_bounds__(256)
+k_thermal_y(Grid g, Phys p, double dt, const double *__restrict__ T, const double *__restrict__ V,
+            double *__restrict__ am, double *__restrict__ ac, double *__restrict__ ap) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.n1m || j > g.n2m) return;
+    const int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1;
+    const int km = g.n3m - 1, kp = g.n3m + 1;
+    const int jem = g.jC[jm], jep = g.jC[jp];
+    const double *dx1 = g.dx1, *dx2 = g.dx2, *dx3 = g.dx3, *dmx1 = g.dmx1, *dmx2 = g.dmx2, *dmx3 = g.dmx3;
+
+    const double Tc = T[g.ix(i, j, k)];
+    const double Tim = T[g.ix(im, j, k)], Tip = T[g.ix(ip, j, k)];
+    const double Tjm = T[g.ix(i, jm, k)], Tjp = T[g.ix(i, jp, k)];
+    const double Tkm = T[g.ix(i, j, km)], Tkp = T[g.ix(i, j, kp)];
+    double irc, dirc;
+    th_rhocp(p, Tc, irc, dirc);
+    double Kc, dKc, Kjm, dKjm, Kjp, dKjp;
+    th_kpp(p, Tc, Kc, dKc);
+    th_kpp(p, Tjm, Kjm, dKjm);
+    th_kpp(p, Tjp, Kjp, dKjp);
+    const double Kim = th_kpp_only(p, Tim), Kip = th_kpp_only(p, Tip);
+    const double Kkm = th_kpp_only(p, Tkm), Kkp = th_kpp_only(p, Tkp);
+
+    const double invRhoCpCt_half = 0.5 * p.Ct * irc;
+    const double kPP1 = 0.5 / dmx1[i] * (dx1[i] * Kim + dx1[im] * Kc);
+    const double kPP2 = 0.5 / dmx1[ip] * (dx1[ip] * Kc + dx1[i] * Kip);
+    const double kPP3 = 0.5 / dmx2[j] * (dx2[j] * Kjm + dx2[jm] * Kc);
+    const double kPP4 = 0.5 / dmx2[jp] * (dx2[jp] * Kc + dx2[j] * Kjp);
+    const double kPP5 = 0.5 / dmx3[k] * (dx3[k] * Kkm + dx3[km] * Kc);
+    const double kPP6 = 0.5 / dmx3[kp] * (dx3[kp] * Kc + dx3[k] * Kkp);
+    const double dkPP3 = 0.5 / dmx2[j] * (dx2[j] * dKjm + dx2[jm] * dKc);
+    const double dkPP4 = 0.5 / dmx2[jp] * (dx2[jp] * dKc + dx2[j] * dKjp);
+    const double dedx1 = (Tc - Tim) / dmx1[i];
+    const double dedx2 = (Tip - Tc) / dmx1[ip];
+    const double dedy3 = (Tc - Tjm) / dmx2[j];
+    const double dedy4 = (Tjp - Tc) / dmx2[jp];
+    const double dedz5 = (Tc - Tkm) / dmx3[k];
+    const double dedz6 = (Tkp - Tc) / dmx3[kp];
+    const double Vc = V[g.ix(i, j, k)], Vjp = V[g.ix(i, jp, k)];
+
+    const double eACJ = invRhoCpCt_half / dx2[j] * (kPP4 / dmx2[jp] + kPP3 / dmx2[j]
+                                                    - dx2[jp] * (0.5 / dmx2[jp]) * dkPP4 * dedy4 * (double)jep
+                                                    + dx2[jm] * (0.5 / dmx2[j]) * dkPP3 * dedy3 * (double)jem)
+                      + (0.25 * Vc / dmx2[j] - 0.25 * Vjp / dmx2[jp])
+                      - 0.5 * invRhoCpCt_half * ((kPP2 * dedx2 - kPP1 * dedx1) / dx1[i]
+                                                 + (kPP4 * dedy4 - kPP3 * dedy3) / dx2[j]
+                                                 + (kPP6 * dedz6 - kPP5 * dedz5) / dx3[k]);
+    double eAPJ = -invRhoCpCt_half / dx2[j] * (kPP4 / dmx2[jp] + dx2[j] * (0.5 / dmx2[jp]) * dkPP4 * dedy4) + 0.25 * Vjp / dmx2[jp];
+    double eAMJ = -invRhoCpCt_half / dx2[j] * (kPP3 / dmx2[j] - dx2[j] * (0.5 / dmx2[j]) * dkPP3 * dedy3) - 0.25 * Vc / dmx2[j];
+    eAPJ = eAPJ * (double)jep;
+    eAMJ = eAMJ * (double)jem;
+    const size_t q = g.r3(i, j, k);
+    ac[q] = eACJ * dt + 1.0;
+    ap[q] = eAPJ * dt;
+    am[q] = eAMJ * dt;
+}
+
+// T(i,j,k) += RHSJ (:518-524)
+__global__ void k_add_interior(Grid g, double *__restrict__ F, const double *__restrict__ inc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.n1m || j > g.n2m) return;
+    const size_t q = g.ix(i, j, k);
+    F[q] = F[q] + inc[g.r3(i, j, k)];
+}
+
+// sweeps over the interior (i,j,k) layout
+int sweep_z(tcs_ctx *c, bool cyc) {
+    const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m;
+    return ptdma_exact_launch(c->am, c->ac, c->ap, c->rhs, c->n1m, 1, c->n2m, n1, c->n3m, n12, cyc, c->stream);
+}
+int sweep_x(tcs_ctx *c, bool cyc) {
+    const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m;
+    return ptdma_exact_launch(c->am, c->ac, c->ap, c->rhs, c->n2m, n1, c->n3m, n12, c->n1m, 1, cyc, c->stream);
+}
+int sweep_y(tcs_ctx *c, int j0, int nrows) {
+    // rows j = j0..j0+nrows-1 (1-based)
+    const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m;
+    const long off = (long)(j0 - 1) * n1;
+    return ptdma_exact_launch(c->am + off, c->ac + off, c->ap + off, c->rhs + off, c->n1m, 1, c->n3m, n12,
+                              nrows, n1, false, c->stream);
+}
+
+int thermal_coeffi(tcs_ctx *c) {
+    hipLaunchKernelGGL(k_thermal_coeffi, dim3(2048), dim3(256), 0, c->stream, c->grid(), c->phys,
+                       c->f[TCS_T], c->f[TCS_KPP], c->f[TCS_DKPP], c->f[TCS_INVRHOCP], c->f[TCS_DINVRHOCP]);
+    TCS_LAUNCH_CHECK();
+    return TCS_OK;
+}
+
+int thermal_solver(tcs_ctx *c, double dt) {
+    const Grid g = c->grid();
+    const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
+    hipLaunchKernelGGL(k_thermal_z, grd, blk, 0, c->stream, g, c->phys, dt, c->f[TCS_T], c->f[TCS_U],
+                       c->f[TCS_V], c->f[TCS_W], c->wall[TCS_TBCUP], c->wall[TCS_TBCBT], c->am, c->ac, c->ap,
+                       c->rhs);
+    TCS_LAUNCH_CHECK();
+    TCS_TRY(sweep_z(c, true));
+    hipLaunchKernelGGL(k_thermal_x, grd, blk, 0, c->stream, g, c->phys, dt, c->f[TCS_T], c->f[TCS_U], c->am,
+                       c->ac, c->ap);
+    TCS_LAUNCH_CHECK();
+    TCS_TRY(sweep_x(c, true));
+    hipLaunchKernelGGL(k_thermal_y, grd, blk, 0, c->stream, g, c->phys, dt, c->f[TCS_T], c->f[TCS_V], c->am,
+                       c->ac, c->ap);
+    TCS_LAUNCH_CHECK();
+    TCS_TRY(sweep_y(c, 1, c->n2m));
+    hipLaunchKernelGGL(k_add_interior, grd, blk, 0, c->stream, g, c->f[TCS_T], c->rhs);
+    TCS_LAUNCH_CHECK();
+    return TCS_OK;
+}
+
+}  // namespace tcs

@@ -1,0 +1,137 @@
+"""GPU module routines vs the CPU oracle, bit for bit, from identical
+randomized module state (plain FP64 form: same operation order, no FMA)."""
+import numpy as np
+import pytest
+
+from tests.cases import FIELDS, load_gpu, load_oracle, make_oracle, random_state, small_config
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(16, 12, 8), (32, 9, 16), (8, 5, 4)]
+
+
+def _pair(shape, props="water", seed=7):
+    from pascal_tcs_amd.solver import TCSSolver
+    cfg = small_config(*shape, props=props)
+    o = make_oracle(cfg)
+    s = TCSSolver(cfg)
+    st, walls = random_state(cfg, seed=seed)
+    load_oracle(o, st, walls)
+    load_gpu(s, st, walls)
+    return cfg, o, s
+
+
+def _compare(o, s, names=FIELDS):
+    bad = []
+    for n in names:
+        g = s.get(n)
+        if not np.array_equal(g, o.f[n]):
+            d = np.abs(g - o.f[n])
+            idx = np.unravel_index(np.argmax(d), d.shape)
+            bad.append(f"{n}: max|diff|={d.max():.3e} at (k,j,i)={idx} n_diff={(d > 0).sum()}")
+    assert not bad, "\n".join(bad)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_halo(shape):
+    cfg, o, s = _pair(shape)
+    for n in ("T", "U", "dP"):
+        o.halo(n)
+        s.mpi_subdomain_ghostcell_update(n)
+    _compare(o, s)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_thermal_coeffi(shape):
+    cfg, o, s = _pair(shape)
+    o.thermal_coeffi()
+    s.mpi_thermal_coeffi()
+    for n in ("KPP", "dKPP", "invRhoCp", "dinvRhoCp"):
+        assert np.array_equal(s.get(n), o.f[n]), n
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("props", ["water", "glycerol"])
+def test_thermal_solver(shape, props):
+    cfg, o, s = _pair(shape, props)
+    o.thermal_coeffi()
+    o.thermal_solver(cfg.dtStart)
+    s.mpi_thermal_solver(cfg.dtStart)
+    _compare(o, s)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_momentum_coeffi(shape):
+    cfg, o, s = _pair(shape)
+    o.momentum_coeffi()
+    s.mpi_momentum_coeffi()
+    for n in ("Mu", "invRho"):
+        assert np.array_equal(s.get(n), o.f[n]), n
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("routine", ["solvedU", "solvedV", "solvedW", "blockLdV", "blockLdU"])
+def test_momentum_routines(shape, routine):
+    cfg, o, s = _pair(shape)
+    o.momentum_coeffi()
+    s.mpi_momentum_coeffi()
+    getattr(o, "momentum_" + routine)(cfg.dtStart)
+    getattr(s, "mpi_momentum_" + routine)(cfg.dtStart)
+    _compare(o, s)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_pseudoupdate(shape):
+    cfg, o, s = _pair(shape)
+    o.momentum_pseudoupdateUVW()
+    s.mpi_momentum_pseudoupdateUVW()
+    _compare(o, s)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_pressure_rhs_poisson(shape):
+    cfg, o, s = _pair(shape)
+    o.momentum_coeffi()
+    s.mpi_momentum_coeffi()
+    prhs = o.pressure_rhs(cfg.dtStart)
+    o.poisson(prhs)
+    s.mpi_pressure_RHS(cfg.dtStart)
+    s.mpi_pressure_Poisson_FFT2()
+    _compare(o, s)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_projection(shape):
+    cfg, o, s = _pair(shape)
+    o.momentum_coeffi()
+    s.mpi_momentum_coeffi()
+    o.pressure_Projection(cfg.dtStart)
+    s.mpi_pressure_Projection(cfg.dtStart)
+    _compare(o, s)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_post_div_cfl(shape):
+    cfg, o, s = _pair(shape)
+    assert s.mpi_Post_Div() == o.div()
+    cfl_o, dt_o = o.cfl(cfg.dtStart)
+    cfl_g, dt_g = s.mpi_Post_CFL()
+    assert (cfl_g, dt_g) == (cfl_o, dt_o)
+
+
+@pytest.mark.parametrize("shape", [(16, 12, 8), (32, 20, 16)])
+def test_full_step_seeded(shape):
+    """Three steps of the whole time step from a seeded conduction state."""
+    from pascal_tcs_amd.solver import TCSSolver
+    cfg = small_config(*shape)
+    s = TCSSolver(cfg)
+    s.setup_seeded(amp=0.05, seed=1234)
+    o = make_oracle(cfg)
+    for n in FIELDS:
+        o.f[n][...] = s.get(n)
+    for w in ("TBCup", "TBCbt", "UBCup", "UBCbt", "VBCup", "VBCbt", "WBCup", "WBCbt"):
+        o.bc[w][...] = s.get_wall(w)
+    for step in range(3):
+        o.step(cfg.dtStart)
+        s.step(cfg.dtStart)
+        _compare(o, s, ["T", "U", "V", "W", "P", "dPhat", "dP0"])
