@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Throughput of the plain FP64 MPM-STD time step on one MI355X.
+
+Workload: 512^3 cells (n1m=n2m=n3m=512), Lx=Lz=2*pi, Ly=2, periodic x/z,
+no-slip walls in y with a tanh-stretched y grid (gamma 2.0), Ra=1e8,
+Pr=0.71, water property polynomials, T = conduction profile + seeded noise
+of amplitude 0.01.  The time step is fixed (default 1e-3) while Post_CFL is
+still evaluated every step: from rest the reference's CFL rule returns
+dtMax = 100*min(dx) (0.39 at 512^3), and that second step drives the
+reference scheme past its own blow-up threshold (maxDivU >= 1e-3).
+One step is the loop body of the reference's main program: thermal,
+momentum (dU, dV, dW, block corrections, update), pressure (RHS, FFT/TDMA
+Poisson, projection), every ghost-cell update, Post_Div and Post_CFL.
+
+Prints one JSON line (metric Mcell-steps/s = cells * steps / seconds / 1e6)
+with the dominant kernel's HBM roofline (HIP events on the library stream)
+and the CPU oracle timed on this host as cpu_baseline.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mcell-steps/s at 512³: FP64 plain vs FP32-mixed at stated error; HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def workload_config(n, props="water"):
+    from pascal_tcs_amd.config import Properties, TCSConfig
+    p = {"water": Properties.water(), "constant": Properties.constant()}[props]
+    return TCSConfig(n1m=n, n2m=n, n3m=n, uniform2=0, gamma2=2.0, Ra=1e8, Pr=0.71, DeltaT=10.0,
+                     MaxCFL=0.5, dtStart=1e-3, props=p)
+
+
+def kernel_bytes(label, cfg):
+    """Algorithmic HBM bytes of one launch of a step kernel: each input array
+    read once and each output written once (1-D metrics and (j,k) tables are
+    free).  TDMA sweeps: read a,b,c,d and write x, 40 B per unknown."""
+    n1, n2, n3 = cfg.n1m, cfg.n2m, cfg.n3m
+    N = n1 * n2 * n3
+    fv = (n2 - 1) / n2  # dV systems skip the wall row
+    table = {
+        # 4 non-cyclic y sweeps (T, dU, dV[n2-1 rows], dW) per step
+        "ptdma": 40.0 * N * (3 + fv) / 4,
+        # 8 cyclic z/x sweeps (T, dU, dW full; dV without the wall row)
+        "ptdma_cyclic": 40.0 * N * (6 + 2 * fv) / 8,
+        "thermal_z": 8.0 * N * (4 + 4), "thermal_x": 8.0 * N * (2 + 3), "thermal_y": 8.0 * N * (2 + 3),
+        "dU_z": 8.0 * N * (6 + 4), "dV_z": 8.0 * N * (6 + 4) * fv, "dW_z": 8.0 * N * (7 + 4),
+        "dU_x": 8.0 * N * (1 + 3), "dU_y": 8.0 * N * (1 + 3), "dV_x": 8.0 * N * (1 + 3) * fv,
+        "dV_y": 8.0 * N * (1 + 3) * fv, "dW_x": 8.0 * N * (1 + 3), "dW_y": 8.0 * N * (1 + 3),
+        "blockLdV": 8.0 * N * (3 + 1) * fv, "blockLdU": 8.0 * N * (4 + 1),
+        "pseudoupdate": 8.0 * N * 9, "projection": 8.0 * N * (6 + 4), "dphat_update": 8.0 * N * 5,
+        "pressure_rhs": 8.0 * N * (4 + 1), "add_interior": 8.0 * N * 3, "store_interior": 8.0 * N * 2,
+    }
+    return table.get(label)
+
+
+def run_gpu(args):
+    import torch
+    from pascal_tcs_amd.solver import TCSSolver
+
+    cfg = workload_config(args.n)
+    s = TCSSolver(cfg)
+    s.setup_seeded(amp=0.01, seed=20240512)
+    state = {"dt": args.dt if args.dt > 0 else cfg.dtStart, "div": 0.0, "cfl": 0.0}
+
+    def one_step():
+        s.step(state["dt"])
+        state["div"] = s.mpi_Post_Div()
+        cfl, newdt = s.mpi_Post_CFL()
+        if args.dt > 0:
+            state["cfl"] = state["dt"] * cfl / newdt  # CFL number of the step just taken
+        else:
+            state["cfl"], state["dt"] = cfl, newdt
+
+    for _ in range(args.warmup):
+        one_step()
+    torch.cuda.synchronize()
+    s.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        one_step()
+    s.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    sec = t1 - t0
+
+    # per-stage / per-kernel HIP-event timers on the library's stream
+    s.profile(True)
+    for _ in range(args.profile_steps):
+        one_step()
+    s.sync()
+    rep = s.profile_report()
+    s.profile(False)
+    nu = s.nusselt()
+    s.close()
+    return cfg, sec, rep, state, nu
+
+
+def roofline(cfg, rep, psteps):
+    kernels = {k: v for k, v in rep.items() if kernel_bytes(k, cfg) is not None}
+    if not kernels:
+        return None, None
+    dom = max(kernels, key=lambda k: kernels[k][0])
+    ms, cnt = kernels[dom]
+    avg_s = ms / cnt / 1e3
+    ach = kernel_bytes(dom, cfg) / avg_s / 1e9
+    info = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": dom, "avg_launch_ms": round(ms / cnt, 4), "launches_per_step": cnt // max(psteps, 1),
+            "bytes_per_launch": kernel_bytes(dom, cfg)}
+    return dom, info
+
+
+def run_cpu_baseline(seconds):
+    """The CPU oracle (single thread) on a 512 x 512 x 8 slab of the same
+    workload (same box, grids and physics; z reduced to 8 planes)."""
+    from oracle import oracle as O
+    from pascal_tcs_amd.ic import conduction_noise_T, embed_interior
+    from tests.cases import make_oracle
+
+    cfg = workload_config(512)
+    cfg.n3m = 8
+    o = make_oracle(cfg)
+    o.f["T"][...] = embed_interior(conduction_noise_T(cfg, o.x2, 0.01, 20240512))
+    o.bc["TBCup"][...] = cfg.Tup
+    o.bc["TBCbt"][...] = cfg.Tbt
+    o.thermal_boundary()
+    o.momentum_boundary()
+    for n in ("T", "U", "V", "W", "P"):
+        o.halo(n)
+    dt = 1e-3
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        o.step(dt)
+        o.div()
+        o.cfl(dt)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or steps >= 50:
+            break
+    cells = cfg.n1m * cfg.n2m * cfg.n3m
+    _ = O  # the oracle module is the checker/baseline, never the measured product
+    return {"value": round(cells * steps / el / 1e6, 4), "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"CPU oracle (C restatement, 1 thread) on a 512x512x8 slab of the workload, "
+                      f"{steps} steps in {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--profile-steps", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--dt", type=float, default=1e-3, help="fixed time step (<= 0: the reference's CFL rule)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs the whole domain on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    cfg, sec, rep, state, nu = run_gpu(args)
+    cells = cfg.n1m * cfg.n2m * cfg.n3m
+    value = cells * args.steps / sec / 1e6
+    dom, roof = roofline(cfg, rep, args.profile_steps)
+    stages = {k: round(v[0] / args.profile_steps, 3) for k, v in rep.items() if " " in k or k == "step"}
+    kernels = {k: [round(v[0] / args.profile_steps, 3), v[1] // args.profile_steps]
+               for k, v in rep.items() if " " not in k and k != "step"}
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"plain FP64 MPM-STD step, {cfg.n1m}x{cfg.n2m}x{cfg.n3m}, tanh y (gamma 2.0), "
+                               "Ra=1e8, Pr=0.71, water properties, T noise 0.01, "
+                               + (f"fixed dt {args.dt:g} (CFL evaluated every step)" if args.dt > 0 else "CFL 0.5"),
+                   "cells": cells, "form": "plain FP64 (reference operation order)"},
+        "roofline": roof,
+        "stage_ms_per_step": stages,
+        "kernel_ms_per_step": kernels,
+        "final": {"dt": state["dt"], "maxDivU": state["div"], "CFL": state["cfl"], "Nu_bottom": nu[0],
+                  "Nu_top": nu[1]},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = run_cpu_baseline(args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -156,9 +156,21 @@ int tcs_post_cfl(tcs_ctx *ctx, double *newcfl, double *newdt);
    every halo update in main's order), fully on the device. */
 int tcs_step(tcs_ctx *ctx, double dt);
 /* Wall Nusselt numbers (no counterpart in the reference): mean over (i,k)
-   of (TBCbt - T(i,1,k))/dmx2(1) and (T(i,n2m,k) - TBCup)/dmx2(n2sub),
-   times L2/(Tbt - Tup). */
+   of (T(i,0,k) - T(i,1,k))/dmx2(1) and (T(i,n2m,k) - T(i,n2sub,k))/dmx2(n2sub)
+   (the wall rows the solver uses), times L2/(Tbt - Tup). */
 int tcs_nusselt(tcs_ctx *ctx, double Tup, double Tbt, double *nu_bottom, double *nu_top);
+
+/* ---- timers ----------------------------------------------------------- */
+/* HIP-event timing per stage and per kernel on the context stream, the
+   device counterpart of the reference's labelled timers
+   (PaScaL_TDMA/src/timer.f90:14-118; labels src/main.f90:55-72).
+   tcs_profile(ctx,1) clears and enables, (ctx,0) disables.  Entries are
+   reported in first-seen order: stage labels ("thermal solve", ...) and
+   kernel labels ("ptdma", "thermal_z", ...). */
+int tcs_profile(tcs_ctx *ctx, int32_t enable);
+int tcs_profile_count(tcs_ctx *ctx, int32_t *n);
+int tcs_profile_entry(tcs_ctx *ctx, int32_t idx, char *name, int32_t name_len,
+                      double *total_ms, int64_t *count);
 
 /* ---- PaScaL_TDMA ----------------------------------------------------- */
 /* PaScaL_TDMA_many_solve(plan,A,B,C,D,n_sys,n_row),

@@ -71,6 +71,9 @@ SIGNATURES = {
     "tcs_post_cfl": (ctypes.c_int, [_vp, _dp, _dp]),
     "tcs_step": (ctypes.c_int, [_vp, _f64]),
     "tcs_nusselt": (ctypes.c_int, [_vp, _f64, _f64, _dp, _dp]),
+    "tcs_profile": (ctypes.c_int, [_vp, _i32]),
+    "tcs_profile_count": (ctypes.c_int, [_vp, ctypes.POINTER(_i32)]),
+    "tcs_profile_entry": (ctypes.c_int, [_vp, _i32, ctypes.c_char_p, _i32, _dp, ctypes.POINTER(ctypes.c_int64)]),
     "ptdma_many_solve_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _i32, _vp]),
     "ptdma_many_solve_cycle_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _i32, _vp]),
     "ptdma_many_solve_host": (ctypes.c_int, [_dp, _dp, _dp, _dp, _i32, _i32, _i32]),
@@ -88,6 +91,15 @@ def load():
             raise TCSError(
                 f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
                 "(there is no CPU fallback)")
+        # torch ships its own HIP/HSA runtime, which its libraries reach under
+        # the name libamdhip64.so.  Loaded after ours (soname libamdhip64.so.7
+        # from /opt/rocm) it would be a second runtime in the process and find
+        # no GPU.  Importing torch first makes our library bind to torch's
+        # runtime (same soname), so both share one runtime and one device view.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)

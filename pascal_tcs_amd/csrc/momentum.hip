@@ -592,15 +592,13 @@ k_pseudoupdate(Grid g, double *__restrict__ U, double *__restrict__ V, double *_
 
 // ------------------------------------------------------------- host side ---
 int momentum_coeffi(tcs_ctx *c) {
-    hipLaunchKernelGGL(k_momentum_coeffi, dim3((c->N2 + 63) / 64, c->N3), dim3(64), 0, c->stream, c->grid(),
+    TCS_LAUNCH(c->stream, "momentum_coeffi", k_momentum_coeffi, dim3((c->N2 + 63) / 64, c->N3), dim3(64), 0, c->grid(),
                        c->phys, c->f[TCS_T], c->f[TCS_MU], c->f[TCS_INVRHO]);
-    TCS_LAUNCH_CHECK();
     return TCS_OK;
 }
 
 static int launch_stale(tcs_ctx *c) {
-    hipLaunchKernelGGL(k_momentum_stale, dim3(1), dim3(64), 0, c->stream, c->grid(), c->f[TCS_MU], c->stale);
-    TCS_LAUNCH_CHECK();
+    TCS_LAUNCH(c->stream, "momentum_stale", k_momentum_stale, dim3(1), dim3(64), 0, c->grid(), c->f[TCS_MU], c->stale);
     return TCS_OK;
 }
 
@@ -608,21 +606,17 @@ int momentum_solvedU(tcs_ctx *c, double dt) {
     const Grid g = c->grid();
     const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
     double **f = c->f;
-    hipLaunchKernelGGL(k_dU_z, grd, blk, 0, c->stream, g, c->phys, dt, f[TCS_U], f[TCS_V], f[TCS_W], f[TCS_P],
+    TCS_LAUNCH(c->stream, "dU_z", k_dU_z, grd, blk, 0, g, c->phys, dt, f[TCS_U], f[TCS_V], f[TCS_W], f[TCS_P],
                        f[TCS_T], f[TCS_MU], f[TCS_INVRHO], c->wall[TCS_UBCUP], c->wall[TCS_UBCBT],
                        c->wall[TCS_VBCUP], c->wall[TCS_VBCBT], c->am, c->ac, c->ap, c->rhs);
-    TCS_LAUNCH_CHECK();
     TCS_TRY(sweep_z(c, true));
-    hipLaunchKernelGGL(k_dU_x, grd, blk, 0, c->stream, g, c->phys, dt, f[TCS_U], f[TCS_MU], f[TCS_INVRHO], c->am,
+    TCS_LAUNCH(c->stream, "dU_x", k_dU_x, grd, blk, 0, g, c->phys, dt, f[TCS_U], f[TCS_MU], f[TCS_INVRHO], c->am,
                        c->ac, c->ap);
-    TCS_LAUNCH_CHECK();
     TCS_TRY(sweep_x(c, true));
-    hipLaunchKernelGGL(k_dU_y, grd, blk, 0, c->stream, g, c->phys, dt, f[TCS_V], f[TCS_MU], f[TCS_INVRHO], c->am,
+    TCS_LAUNCH(c->stream, "dU_y", k_dU_y, grd, blk, 0, g, c->phys, dt, f[TCS_V], f[TCS_MU], f[TCS_INVRHO], c->am,
                        c->ac, c->ap);
-    TCS_LAUNCH_CHECK();
     TCS_TRY(sweep_y(c, 1, c->n2m));
-    hipLaunchKernelGGL(k_store_interior, grd, blk, 0, c->stream, g, 1, f[TCS_DU], c->rhs);
-    TCS_LAUNCH_CHECK();
+    TCS_LAUNCH(c->stream, "store_interior", k_store_interior, grd, blk, 0, g, 1, f[TCS_DU], c->rhs);
     return TCS_OK;
 }
 
@@ -631,32 +625,27 @@ int momentum_solvedV(tcs_ctx *c, double dt) {
     const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
     double **f = c->f;
     TCS_TRY(launch_stale(c));
-    hipLaunchKernelGGL(k_dV_z, grd, blk, 0, c->stream, g, c->phys, dt, f[TCS_U], f[TCS_V], f[TCS_W], f[TCS_P],
+    TCS_LAUNCH(c->stream, "dV_z", k_dV_z, grd, blk, 0, g, c->phys, dt, f[TCS_U], f[TCS_V], f[TCS_W], f[TCS_P],
                        f[TCS_DU], f[TCS_MU], f[TCS_INVRHO], c->wall[TCS_VBCUP], c->wall[TCS_VBCBT], c->am, c->ac,
                        c->ap, c->rhs);
-    TCS_LAUNCH_CHECK();
     // rows j=1 are not part of the dV systems; sweep only j=2..n2m in z and x
     {
         const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m;
         TCS_TRY(ptdma_exact_launch(c->am + n1, c->ac + n1, c->ap + n1, c->rhs + n1, c->n1m, 1, c->n2m - 1, n1,
                                    c->n3m, n12, true, c->stream));
     }
-    hipLaunchKernelGGL(k_dV_x, grd, blk, 0, c->stream, g, c->phys, dt, f[TCS_U], f[TCS_INVRHO], c->stale, c->am,
+    TCS_LAUNCH(c->stream, "dV_x", k_dV_x, grd, blk, 0, g, c->phys, dt, f[TCS_U], f[TCS_INVRHO], c->stale, c->am,
                        c->ac, c->ap);
-    TCS_LAUNCH_CHECK();
     {
         const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m;
         TCS_TRY(ptdma_exact_launch(c->am + n1, c->ac + n1, c->ap + n1, c->rhs + n1, c->n2m - 1, n1, c->n3m, n12,
                                    c->n1m, 1, true, c->stream));
     }
-    hipLaunchKernelGGL(k_dV_y, grd, blk, 0, c->stream, g, c->phys, dt, f[TCS_V], f[TCS_MU], f[TCS_INVRHO], c->am,
+    TCS_LAUNCH(c->stream, "dV_y", k_dV_y, grd, blk, 0, g, c->phys, dt, f[TCS_V], f[TCS_MU], f[TCS_INVRHO], c->am,
                        c->ac, c->ap);
-    TCS_LAUNCH_CHECK();
     TCS_TRY(sweep_y(c, 2, c->n2m - 1));
-    hipLaunchKernelGGL(k_dV_zero_rows, dim3((c->N1 + 63) / 64, c->N3), dim3(64), 0, c->stream, g, f[TCS_DV]);
-    TCS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_store_interior, grd, blk, 0, c->stream, g, 2, f[TCS_DV], c->rhs);
-    TCS_LAUNCH_CHECK();
+    TCS_LAUNCH(c->stream, "dV_zero_rows", k_dV_zero_rows, dim3((c->N1 + 63) / 64, c->N3), dim3(64), 0, g, f[TCS_DV]);
+    TCS_LAUNCH(c->stream, "store_interior", k_store_interior, grd, blk, 0, g, 2, f[TCS_DV], c->rhs);
     return TCS_OK;
 }
 
@@ -665,45 +654,38 @@ int momentum_solvedW(tcs_ctx *c, double dt) {
     const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
     double **f = c->f;
     TCS_TRY(launch_stale(c));
-    hipLaunchKernelGGL(k_dW_z, grd, blk, 0, c->stream, g, c->phys, dt, f[TCS_U], f[TCS_V], f[TCS_W], f[TCS_P],
+    TCS_LAUNCH(c->stream, "dW_z", k_dW_z, grd, blk, 0, g, c->phys, dt, f[TCS_U], f[TCS_V], f[TCS_W], f[TCS_P],
                        f[TCS_DU], f[TCS_DV], f[TCS_MU], f[TCS_INVRHO], c->wall[TCS_VBCUP], c->wall[TCS_VBCBT],
                        c->wall[TCS_WBCUP], c->wall[TCS_WBCBT], c->am, c->ac, c->ap, c->rhs);
-    TCS_LAUNCH_CHECK();
     TCS_TRY(sweep_z(c, true));
-    hipLaunchKernelGGL(k_dW_x, grd, blk, 0, c->stream, g, c->phys, dt, f[TCS_U], f[TCS_INVRHO], c->stale, c->am,
+    TCS_LAUNCH(c->stream, "dW_x", k_dW_x, grd, blk, 0, g, c->phys, dt, f[TCS_U], f[TCS_INVRHO], c->stale, c->am,
                        c->ac, c->ap);
-    TCS_LAUNCH_CHECK();
     TCS_TRY(sweep_x(c, true));
-    hipLaunchKernelGGL(k_dW_y, grd, blk, 0, c->stream, g, c->phys, dt, f[TCS_V], f[TCS_INVRHO], c->stale, c->am,
+    TCS_LAUNCH(c->stream, "dW_y", k_dW_y, grd, blk, 0, g, c->phys, dt, f[TCS_V], f[TCS_INVRHO], c->stale, c->am,
                        c->ac, c->ap);
-    TCS_LAUNCH_CHECK();
     TCS_TRY(sweep_y(c, 1, c->n2m));
-    hipLaunchKernelGGL(k_store_interior, grd, blk, 0, c->stream, g, 1, f[TCS_DW], c->rhs);
-    TCS_LAUNCH_CHECK();
+    TCS_LAUNCH(c->stream, "store_interior", k_store_interior, grd, blk, 0, g, 1, f[TCS_DW], c->rhs);
     return TCS_OK;
 }
 
 int momentum_blockLdV(tcs_ctx *c, double dt) {
     const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
-    hipLaunchKernelGGL(k_blockLdV, grd, blk, 0, c->stream, c->grid(), c->phys, dt, c->f[TCS_V], c->f[TCS_DW],
+    TCS_LAUNCH(c->stream, "blockLdV", k_blockLdV, grd, blk, 0, c->grid(), c->phys, dt, c->f[TCS_V], c->f[TCS_DW],
                        c->f[TCS_MU], c->f[TCS_INVRHO], c->f[TCS_DV]);
-    TCS_LAUNCH_CHECK();
     return TCS_OK;
 }
 
 int momentum_blockLdU(tcs_ctx *c, double dt) {
     const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
-    hipLaunchKernelGGL(k_blockLdU, grd, blk, 0, c->stream, c->grid(), c->phys, dt, c->f[TCS_U], c->f[TCS_DV],
+    TCS_LAUNCH(c->stream, "blockLdU", k_blockLdU, grd, blk, 0, c->grid(), c->phys, dt, c->f[TCS_U], c->f[TCS_DV],
                        c->f[TCS_DW], c->f[TCS_MU], c->f[TCS_INVRHO], c->f[TCS_DU]);
-    TCS_LAUNCH_CHECK();
     return TCS_OK;
 }
 
 int momentum_pseudoupdate(tcs_ctx *c) {
     const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
-    hipLaunchKernelGGL(k_pseudoupdate, grd, blk, 0, c->stream, c->grid(), c->f[TCS_U], c->f[TCS_V], c->f[TCS_W],
+    TCS_LAUNCH(c->stream, "pseudoupdate", k_pseudoupdate, grd, blk, 0, c->grid(), c->f[TCS_U], c->f[TCS_V], c->f[TCS_W],
                        c->f[TCS_DU], c->f[TCS_DV], c->f[TCS_DW]);
-    TCS_LAUNCH_CHECK();
     return TCS_OK;
 }
 

@@ -400,10 +400,9 @@ static int fft_tile(int n1m, int n3m) {
 
 int pressure_rhs(tcs_ctx *c, double dt) {
     const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
-    hipLaunchKernelGGL(k_pressure_rhs, grd, blk, 0, c->stream, c->grid(), c->phys, dt, c->f[TCS_U], c->f[TCS_V],
+    TCS_LAUNCH(c->stream, "pressure_rhs", k_pressure_rhs, grd, blk, 0, c->grid(), c->phys, dt, c->f[TCS_U], c->f[TCS_V],
                        c->f[TCS_W], c->f[TCS_DPHAT], c->f[TCS_INVRHO], c->wall[TCS_VBCUP], c->wall[TCS_VBCBT],
                        c->prhs);
-    TCS_LAUNCH_CHECK();
     return TCS_OK;
 }
 
@@ -413,51 +412,39 @@ int pressure_poisson(tcs_ctx *c) {
     const int TX = fft_tile(n1m, n3m);
     double2 *spec = (double2 *)c->spec_x;
     const size_t ldsx = (size_t)TX * n1m * 2 * sizeof(double);
-    hipLaunchKernelGGL(k_fft_x_fwd, dim3(n3m / TX, n2m), dim3(FFT_THREADS), ldsx, c->stream, g, h1p, TX, c->prhs,
+    TCS_LAUNCH(c->stream, "fft_x_fwd", k_fft_x_fwd, dim3(n3m / TX, n2m), dim3(FFT_THREADS), ldsx, g, h1p, TX, c->prhs,
                        (const double2 *)c->tw1, spec);
-    TCS_LAUNCH_CHECK();
     const size_t ldsz = (size_t)n3m * 2 * sizeof(double);
-    hipLaunchKernelGGL(k_fft_z<-1>, dim3(h1p, n2m), dim3(FFT_THREADS), ldsz, c->stream, n3m, h1p,
+    TCS_LAUNCH(c->stream, "fft_z<-1>", k_fft_z<-1>, dim3(h1p, n2m), dim3(FFT_THREADS), ldsz, n3m, h1p,
                        (const double2 *)c->tw3, spec);
-    TCS_LAUNCH_CHECK();
     const int nsys = n3m * h1p;
-    hipLaunchKernelGGL(k_poisson_tdma, dim3((nsys + 255) / 256), dim3(256), 0, c->stream, n3m, h1p, n2m, c->h3p,
+    TCS_LAUNCH(c->stream, "poisson_tdma", k_poisson_tdma, dim3((nsys + 255) / 256), dim3(256), 0, n3m, h1p, n2m, c->h3p,
                        c->ptab, c->ptab + n2m, c->ptab + 2 * n2m, c->dxk2, c->dzk, spec, c->pA, c->pC);
-    TCS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_fft_z<+1>, dim3(h1p, n2m), dim3(FFT_THREADS), ldsz, c->stream, n3m, h1p,
+    TCS_LAUNCH(c->stream, "fft_z<+1>", k_fft_z<+1>, dim3(h1p, n2m), dim3(FFT_THREADS), ldsz, n3m, h1p,
                        (const double2 *)c->tw3, spec);
-    TCS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_fft_x_inv, dim3(n3m / TX, n2m), dim3(FFT_THREADS), ldsx, c->stream, g, h1p, TX,
+    TCS_LAUNCH(c->stream, "fft_x_inv", k_fft_x_inv, dim3(n3m / TX, n2m), dim3(FFT_THREADS), ldsx, g, h1p, TX,
                        (const double2 *)spec, (const double2 *)c->tw1, c->f[TCS_DP], c->linesum);
-    TCS_LAUNCH_CHECK();
     int P2 = 1;
     while (P2 < n2m) P2 <<= 1;
-    hipLaunchKernelGGL(k_plane_sum, dim3(n3m), dim3(256), P2 * sizeof(double), c->stream, n2m, c->linesum,
+    TCS_LAUNCH(c->stream, "plane_sum", k_plane_sum, dim3(n3m), dim3(256), P2 * sizeof(double), n2m, c->linesum,
                        c->planesum);
-    TCS_LAUNCH_CHECK();
     int P3 = 1;
     while (P3 < n3m) P3 <<= 1;
-    hipLaunchKernelGGL(k_total_mean, dim3(1), dim3(256), P3 * sizeof(double), c->stream, n1m, n2m, n3m,
+    TCS_LAUNCH(c->stream, "total_mean", k_total_mean, dim3(1), dim3(256), P3 * sizeof(double), n1m, n2m, n3m,
                        c->planesum, c->scal);
-    TCS_LAUNCH_CHECK();
     const dim3 blk = interior_block(), grd = interior_grid(n1m, n2m, n3m);
-    hipLaunchKernelGGL(k_dp_subtract, grd, blk, 0, c->stream, g, c->scal, c->f[TCS_DP]);
-    TCS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_wall_extrap, dim3((n1m + 63) / 64, n3m), dim3(64), 0, c->stream, g, c->f[TCS_DP]);
-    TCS_LAUNCH_CHECK();
+    TCS_LAUNCH(c->stream, "dp_subtract", k_dp_subtract, grd, blk, 0, g, c->scal, c->f[TCS_DP]);
+    TCS_LAUNCH(c->stream, "wall_extrap", k_wall_extrap, dim3((n1m + 63) / 64, n3m), dim3(64), 0, g, c->f[TCS_DP]);
     return TCS_OK;
 }
 
 int pressure_projection(tcs_ctx *c, double dt) {
     const Grid g = c->grid();
     const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
-    hipLaunchKernelGGL(k_projection, grd, blk, 0, c->stream, g, c->phys, dt, c->f[TCS_INVRHO], c->f[TCS_DP],
+    TCS_LAUNCH(c->stream, "projection", k_projection, grd, blk, 0, g, c->phys, dt, c->f[TCS_INVRHO], c->f[TCS_DP],
                        c->f[TCS_DPHAT], c->f[TCS_DP0], c->f[TCS_U], c->f[TCS_V], c->f[TCS_W], c->f[TCS_P]);
-    TCS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_dphat_update, grd, blk, 0, c->stream, g, c->f[TCS_DP], c->f[TCS_DPHAT], c->f[TCS_DP0]);
-    TCS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_wall_extrap, dim3((c->n1m + 63) / 64, c->n3m), dim3(64), 0, c->stream, g, c->f[TCS_P]);
-    TCS_LAUNCH_CHECK();
+    TCS_LAUNCH(c->stream, "dphat_update", k_dphat_update, grd, blk, 0, g, c->f[TCS_DP], c->f[TCS_DPHAT], c->f[TCS_DP0]);
+    TCS_LAUNCH(c->stream, "wall_extrap", k_wall_extrap, dim3((c->n1m + 63) / 64, c->n3m), dim3(64), 0, g, c->f[TCS_P]);
     return TCS_OK;
 }
 

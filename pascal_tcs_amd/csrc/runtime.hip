@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -16,6 +17,82 @@ namespace tcs {
 
 static thread_local std::string g_err;
 void set_error(const std::string &msg) { g_err = msg; }
+
+// ------------------------------------------------------------ profiler ----
+struct Profiler {
+    struct Rec { std::string name; hipEvent_t a, b; };
+    std::vector<Rec> pending;
+    std::vector<size_t> open;
+    std::vector<hipEvent_t> pool;
+    std::vector<std::string> names;              // first-seen order
+    std::map<std::string, std::pair<double, long>> acc;
+
+    hipEvent_t get() {
+        if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    void begin(const char *name, hipStream_t s) {
+        Rec r{name, get(), get()};
+        if (!r.a || !r.b) return;
+        (void)hipEventRecord(r.a, s);
+        open.push_back(pending.size());
+        pending.push_back(r);
+    }
+    void end(hipStream_t s) {
+        if (open.empty()) return;
+        (void)hipEventRecord(pending[open.back()].b, s);
+        open.pop_back();
+    }
+    void flush() {
+        for (auto &r : pending) {
+            float ms = 0.f;
+            if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+                if (!acc.count(r.name)) names.push_back(r.name);
+                auto &v = acc[r.name];
+                v.first += ms;
+                v.second += 1;
+            }
+            pool.push_back(r.a);
+            pool.push_back(r.b);
+        }
+        pending.clear();
+    }
+    ~Profiler() {
+        flush();
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+static thread_local Profiler *tl_prof = nullptr;
+
+void prof_begin(const char *label, hipStream_t s) {
+    if (tl_prof) tl_prof->begin(label, s);
+}
+void prof_end(hipStream_t s) {
+    if (tl_prof) tl_prof->end(s);
+}
+
+// Makes a context's profiler current for the calling thread and times the
+// enclosed stage under `label` (no-op unless profiling is enabled).
+struct ProfStage {
+    Profiler *prev;
+    hipStream_t s;
+    bool on;
+    ProfStage(tcs_ctx *c, const char *label) : prev(tl_prof), s(c ? c->stream : nullptr), on(c && c->prof) {
+        if (on) {
+            tl_prof = c->prof;
+            tl_prof->begin(label, s);
+        }
+    }
+    ~ProfStage() {
+        if (on) {
+            tl_prof->end(s);
+            tl_prof = prev;
+        }
+    }
+};
 
 // ------------------------------------------------------------ halo --------
 // Single-rank mpi_subdomain_ghostcell_update (src/module_mpi_subdomain.f90:
@@ -54,8 +131,7 @@ __global__ void k_halo(Grid g, double *__restrict__ f) {
 
 int halo(tcs_ctx *c, double *f) {
     const size_t n = (size_t)2 * c->N2 * c->N3 + (size_t)2 * c->N1 * c->N2;
-    hipLaunchKernelGGL(k_halo, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->grid(), f);
-    TCS_LAUNCH_CHECK();
+    TCS_LAUNCH(c->stream, "halo", k_halo, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->grid(), f);
     return TCS_OK;
 }
 
@@ -110,11 +186,9 @@ __global__ void k_final_max(int n, const double *__restrict__ partial, double *_
 
 template <int MODE>
 static int post_max(tcs_ctx *c, double *out) {
-    hipLaunchKernelGGL(k_post_max<MODE>, dim3(RED_BLOCKS), dim3(256), 0, c->stream, c->grid(), c->f[TCS_U],
+    TCS_LAUNCH(c->stream, MODE == 0 ? "post_div" : "post_cfl", k_post_max<MODE>, dim3(RED_BLOCKS), dim3(256), 0, c->grid(), c->f[TCS_U],
                        c->f[TCS_V], c->f[TCS_W], c->red);
-    TCS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_final_max, dim3(1), dim3(256), 0, c->stream, RED_BLOCKS, c->red, c->red + RED_BLOCKS);
-    TCS_LAUNCH_CHECK();
+    TCS_LAUNCH(c->stream, "final_max", k_final_max, dim3(1), dim3(256), 0, RED_BLOCKS, c->red, c->red + RED_BLOCKS);
     TCS_HIP(hipMemcpyAsync(c->h_pinned, c->red + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     TCS_HIP(hipStreamSynchronize(c->stream));
     *out = c->h_pinned[0];
@@ -124,9 +198,9 @@ static int post_max(tcs_ctx *c, double *out) {
 int post_div(tcs_ctx *c, double *out) { return post_max<0>(c, out); }
 int post_cfl(tcs_ctx *c, double *out) { return post_max<1>(c, out); }
 
-// wall Nusselt numbers: pairwise sums over i per k, then over k
-__global__ void k_nusselt_lines(Grid g, double Tup, double Tbt, const double *__restrict__ T,
-                                double *__restrict__ lines) {
+// wall Nusselt numbers from the wall rows the solver sees (T(:,0,:) and
+// T(:,n2sub,:)): pairwise sums over i per k, then over k
+__global__ void k_nusselt_lines(Grid g, const double *__restrict__ T, double *__restrict__ lines) {
     extern __shared__ double buf[];
     const int k = blockIdx.x + 1;
     const int side = blockIdx.y;
@@ -136,8 +210,8 @@ __global__ void k_nusselt_lines(Grid g, double Tup, double Tbt, const double *__
         double v = 0.0;
         if (t < g.n1m) {
             const int i = t + 1;
-            if (side == 0) v = (Tbt - T[g.ix(i, 1, k)]) / g.dmx2[1];
-            else v = (T[g.ix(i, g.n2m, k)] - Tup) / g.dmx2[g.n2m + 1];
+            if (side == 0) v = (T[g.ix(i, 0, k)] - T[g.ix(i, 1, k)]) / g.dmx2[1];
+            else v = (T[g.ix(i, g.n2m, k)] - T[g.ix(i, g.n2m + 1, k)]) / g.dmx2[g.n2m + 1];
         }
         buf[t] = v;
     }
@@ -170,12 +244,10 @@ int nusselt(tcs_ctx *c, double Tup, double Tbt, double *nb, double *nt) {
     int P1 = 1, P3 = 1;
     while (P1 < c->n1m) P1 <<= 1;
     while (P3 < c->n3m) P3 <<= 1;
-    hipLaunchKernelGGL(k_nusselt_lines, dim3(c->n3m, 2), dim3(256), P1 * sizeof(double), c->stream, c->grid(), Tup,
-                       Tbt, c->f[TCS_T], c->red);
-    TCS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_nusselt_total, dim3(1), dim3(256), P3 * sizeof(double), c->stream, c->n1m, c->n3m,
+    TCS_LAUNCH(c->stream, "nusselt_lines", k_nusselt_lines, dim3(c->n3m, 2), dim3(256), P1 * sizeof(double), c->grid(),
+               c->f[TCS_T], c->red);
+    TCS_LAUNCH(c->stream, "nusselt_total", k_nusselt_total, dim3(1), dim3(256), P3 * sizeof(double), c->n1m, c->n3m,
                        c->prm.L2, Tbt - Tup, c->red, c->red + 2 * c->n3m);
-    TCS_LAUNCH_CHECK();
     TCS_HIP(hipMemcpyAsync(c->h_pinned, c->red + 2 * c->n3m, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     TCS_HIP(hipStreamSynchronize(c->stream));
     *nb = c->h_pinned[0];
@@ -279,6 +351,7 @@ static void free_ctx(tcs_ctx *c) {
     if (c->jC) (void)hipFree(c->jC);
     if (c->jS) (void)hipFree(c->jS);
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+    if (c->prof) delete c->prof;
     if (c->stream) (void)hipStreamDestroy(c->stream);
     free(c->h_dx1); free(c->h_dx2); free(c->h_dx3); free(c->h_dmx1); free(c->h_dmx2); free(c->h_dmx3);
     delete c;
@@ -517,16 +590,14 @@ extern "C" int tcs_wall_download(tcs_ctx *c, int32_t id, double *host) {
 }
 
 static int fill_wall(tcs_ctx *c, int id, double v) {
-    hipLaunchKernelGGL(k_fill_wall, dim3((unsigned)((c->nwall + 255) / 256)), dim3(256), 0, c->stream,
+    TCS_LAUNCH(c->stream, "fill_wall", k_fill_wall, dim3((unsigned)((c->nwall + 255) / 256)), dim3(256), 0,
                        (int)c->nwall, v, c->wall[id]);
-    TCS_LAUNCH_CHECK();
     return TCS_OK;
 }
 
 static int set_row(tcs_ctx *c, int field, int jrow, int wall) {
-    hipLaunchKernelGGL(k_set_row, dim3((c->N1 + 63) / 64, c->N3), dim3(64), 0, c->stream, c->grid(), jrow,
+    TCS_LAUNCH(c->stream, "set_row", k_set_row, dim3((c->N1 + 63) / 64, c->N3), dim3(64), 0, c->grid(), jrow,
                        c->wall[wall], c->f[field]);
-    TCS_LAUNCH_CHECK();
     return TCS_OK;
 }
 
@@ -558,29 +629,34 @@ extern "C" int tcs_momentum_boundary(tcs_ctx *c, double Uup, double Ubt, double 
 #define CTX_GUARD(c) \
     if (!(c)) { set_error("null context"); return TCS_E_ARG; }
 
-extern "C" int tcs_thermal_coeffi(tcs_ctx *c) { CTX_GUARD(c); return thermal_coeffi(c); }
-extern "C" int tcs_thermal_solver(tcs_ctx *c, double dt) { CTX_GUARD(c); return thermal_solver(c, dt); }
-extern "C" int tcs_momentum_coeffi(tcs_ctx *c) { CTX_GUARD(c); return momentum_coeffi(c); }
-extern "C" int tcs_momentum_solvedU(tcs_ctx *c, double dt) { CTX_GUARD(c); return momentum_solvedU(c, dt); }
-extern "C" int tcs_momentum_solvedV(tcs_ctx *c, double dt) { CTX_GUARD(c); return momentum_solvedV(c, dt); }
-extern "C" int tcs_momentum_solvedW(tcs_ctx *c, double dt) { CTX_GUARD(c); return momentum_solvedW(c, dt); }
-extern "C" int tcs_momentum_blockLdV(tcs_ctx *c, double dt) { CTX_GUARD(c); return momentum_blockLdV(c, dt); }
-extern "C" int tcs_momentum_blockLdU(tcs_ctx *c, double dt) { CTX_GUARD(c); return momentum_blockLdU(c, dt); }
-extern "C" int tcs_momentum_pseudoupdateUVW(tcs_ctx *c) { CTX_GUARD(c); return momentum_pseudoupdate(c); }
+#define STAGE(label) ProfStage _stage(c, label)
+
+// stage labels follow the reference's timer labels (src/main.f90:55-72)
+extern "C" int tcs_thermal_coeffi(tcs_ctx *c) { CTX_GUARD(c); STAGE("thermal coeff"); return thermal_coeffi(c); }
+extern "C" int tcs_thermal_solver(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("thermal solve"); return thermal_solver(c, dt); }
+extern "C" int tcs_momentum_coeffi(tcs_ctx *c) { CTX_GUARD(c); STAGE("momentum coeff"); return momentum_coeffi(c); }
+extern "C" int tcs_momentum_solvedU(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("momentum solvedU"); return momentum_solvedU(c, dt); }
+extern "C" int tcs_momentum_solvedV(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("momentum solvedV"); return momentum_solvedV(c, dt); }
+extern "C" int tcs_momentum_solvedW(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("momentum solvedW"); return momentum_solvedW(c, dt); }
+extern "C" int tcs_momentum_blockLdV(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("momentum blockLdV"); return momentum_blockLdV(c, dt); }
+extern "C" int tcs_momentum_blockLdU(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("momentum blockLdU"); return momentum_blockLdU(c, dt); }
+extern "C" int tcs_momentum_pseudoupdateUVW(tcs_ctx *c) { CTX_GUARD(c); STAGE("momentum updatePseudoUVW"); return momentum_pseudoupdate(c); }
 extern "C" int tcs_pressure_wave_number(tcs_ctx *c) { CTX_GUARD(c); return upload_tables(c); }
-extern "C" int tcs_pressure_rhs(tcs_ctx *c, double dt) { CTX_GUARD(c); return pressure_rhs(c, dt); }
-extern "C" int tcs_pressure_poisson_fft2(tcs_ctx *c) { CTX_GUARD(c); return pressure_poisson(c); }
-extern "C" int tcs_pressure_projection(tcs_ctx *c, double dt) { CTX_GUARD(c); return pressure_projection(c, dt); }
+extern "C" int tcs_pressure_rhs(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("pressure RHS"); return pressure_rhs(c, dt); }
+extern "C" int tcs_pressure_poisson_fft2(tcs_ctx *c) { CTX_GUARD(c); STAGE("pressure PoissonSolve"); return pressure_poisson(c); }
+extern "C" int tcs_pressure_projection(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("pressure projection"); return pressure_projection(c, dt); }
 
 extern "C" int tcs_ghostcell_update(tcs_ctx *c, int32_t id) {
     TCS_TRY(check_field(c, id));
     if (id == TCS_INVRHO || id == TCS_MU) return TCS_OK;  // tables carry no x/z ghosts of their own
+    STAGE("update ghostcell");
     return halo(c, c->f[id]);
 }
 
 extern "C" int tcs_post_div(tcs_ctx *c, double *maxdiv) {
     CTX_GUARD(c);
     if (!maxdiv) return TCS_E_ARG;
+    STAGE("post div");
     return post_div(c, maxdiv);
 }
 
@@ -588,7 +664,10 @@ extern "C" int tcs_post_cfl(tcs_ctx *c, double *newcfl, double *newdt) {
     CTX_GUARD(c);
     if (!newcfl || !newdt) return TCS_E_ARG;
     double mx = 0.0;
-    TCS_TRY(post_cfl(c, &mx));
+    {
+        STAGE("post CFL");
+        TCS_TRY(post_cfl(c, &mx));
+    }
     // src/module_post.f90:223-226
     double nd = c->prm.MaxCFL / mx;
     if (nd > c->prm.dtMax) nd = c->prm.dtMax;
@@ -604,32 +683,73 @@ extern "C" int tcs_nusselt(tcs_ctx *c, double Tup, double Tbt, double *nb, doubl
 }
 
 // src/main.f90:131-203 (thermal -> momentum -> pressure with main's halos)
+#define RUN(label, expr)              \
+    do {                              \
+        ProfStage _s(c, label);       \
+        TCS_TRY(expr);                \
+    } while (0)
+
 extern "C" int tcs_step(tcs_ctx *c, double dt) {
     CTX_GUARD(c);
-    TCS_TRY(thermal_solver(c, dt));
-    TCS_TRY(halo(c, c->f[TCS_T]));
-    TCS_TRY(momentum_coeffi(c));
-    TCS_TRY(momentum_solvedU(c, dt));
-    TCS_TRY(halo(c, c->f[TCS_DU]));
-    TCS_TRY(momentum_solvedV(c, dt));
-    TCS_TRY(halo(c, c->f[TCS_DV]));
-    TCS_TRY(momentum_solvedW(c, dt));
-    TCS_TRY(halo(c, c->f[TCS_DW]));
-    TCS_TRY(momentum_blockLdV(c, dt));
-    TCS_TRY(halo(c, c->f[TCS_DV]));
-    TCS_TRY(momentum_blockLdU(c, dt));
-    TCS_TRY(momentum_pseudoupdate(c));
-    TCS_TRY(halo(c, c->f[TCS_U]));
-    TCS_TRY(halo(c, c->f[TCS_V]));
-    TCS_TRY(halo(c, c->f[TCS_W]));
-    TCS_TRY(pressure_rhs(c, dt));
-    TCS_TRY(pressure_poisson(c));
-    TCS_TRY(halo(c, c->f[TCS_DP]));
-    TCS_TRY(pressure_projection(c, dt));
-    TCS_TRY(halo(c, c->f[TCS_U]));
-    TCS_TRY(halo(c, c->f[TCS_V]));
-    TCS_TRY(halo(c, c->f[TCS_W]));
-    TCS_TRY(halo(c, c->f[TCS_P]));
-    TCS_TRY(halo(c, c->f[TCS_DPHAT]));
+    STAGE("step");
+    const char *H = "update ghostcell";
+    RUN("thermal solve", thermal_solver(c, dt));
+    RUN(H, halo(c, c->f[TCS_T]));
+    RUN("momentum coeff", momentum_coeffi(c));
+    RUN("momentum solvedU", momentum_solvedU(c, dt));
+    RUN(H, halo(c, c->f[TCS_DU]));
+    RUN("momentum solvedV", momentum_solvedV(c, dt));
+    RUN(H, halo(c, c->f[TCS_DV]));
+    RUN("momentum solvedW", momentum_solvedW(c, dt));
+    RUN(H, halo(c, c->f[TCS_DW]));
+    RUN("momentum blockLdV", momentum_blockLdV(c, dt));
+    RUN(H, halo(c, c->f[TCS_DV]));
+    RUN("momentum blockLdU", momentum_blockLdU(c, dt));
+    RUN("momentum updatePseudoUVW", momentum_pseudoupdate(c));
+    RUN(H, halo(c, c->f[TCS_U]));
+    RUN(H, halo(c, c->f[TCS_V]));
+    RUN(H, halo(c, c->f[TCS_W]));
+    RUN("pressure RHS", pressure_rhs(c, dt));
+    RUN("pressure PoissonSolve", pressure_poisson(c));
+    RUN(H, halo(c, c->f[TCS_DP]));
+    RUN("pressure projection", pressure_projection(c, dt));
+    RUN(H, halo(c, c->f[TCS_U]));
+    RUN(H, halo(c, c->f[TCS_V]));
+    RUN(H, halo(c, c->f[TCS_W]));
+    RUN(H, halo(c, c->f[TCS_P]));
+    RUN(H, halo(c, c->f[TCS_DPHAT]));
+    return TCS_OK;
+}
+
+// ---- profiling -------------------------------------------------------------
+extern "C" int tcs_profile(tcs_ctx *c, int32_t enable) {
+    CTX_GUARD(c);
+    if (c->prof) {
+        delete c->prof;
+        c->prof = nullptr;
+    }
+    if (enable) c->prof = new Profiler();
+    return TCS_OK;
+}
+
+extern "C" int tcs_profile_count(tcs_ctx *c, int32_t *n) {
+    CTX_GUARD(c);
+    if (!n) return TCS_E_ARG;
+    if (!c->prof) { *n = 0; return TCS_OK; }
+    c->prof->flush();
+    *n = (int32_t)c->prof->names.size();
+    return TCS_OK;
+}
+
+extern "C" int tcs_profile_entry(tcs_ctx *c, int32_t idx, char *name, int32_t name_len, double *total_ms,
+                                 int64_t *count) {
+    CTX_GUARD(c);
+    if (!c->prof || idx < 0 || idx >= (int32_t)c->prof->names.size() || !name || name_len < 1 || !total_ms || !count)
+        return TCS_E_ARG;
+    const std::string &nm = c->prof->names[idx];
+    std::snprintf(name, (size_t)name_len, "%s", nm.c_str());
+    const auto &v = c->prof->acc[nm];
+    *total_ms = v.first;
+    *count = v.second;
     return TCS_OK;
 }

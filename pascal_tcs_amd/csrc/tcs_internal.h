@@ -14,6 +14,14 @@ namespace tcs {
 
 void set_error(const std::string &msg);
 
+// Per-label HIP-event timing of kernel launches and stage entry points
+// (the device counterpart of the reference's labelled timers,
+// PaScaL_TDMA/src/timer.f90).  Active only while a context has it enabled;
+// every launch in this library goes through TCS_LAUNCH.
+struct Profiler;
+void prof_begin(const char *label, hipStream_t s);
+void prof_end(hipStream_t s);
+
 #define TCS_HIP(expr)                                                          \
     do {                                                                       \
         hipError_t _e = (expr);                                                \
@@ -30,6 +38,14 @@ void set_error(const std::string &msg);
             ::tcs::set_error(std::string("kernel launch: ") + hipGetErrorString(_e)); \
             return TCS_E_HIP;                                                  \
         }                                                                      \
+    } while (0)
+
+#define TCS_LAUNCH(st, label, kern, grd, blk, lds, ...)                        \
+    do {                                                                       \
+        ::tcs::prof_begin(label, st);                                          \
+        hipLaunchKernelGGL(kern, grd, blk, lds, st, __VA_ARGS__);              \
+        ::tcs::prof_end(st);                                                   \
+        TCS_LAUNCH_CHECK();                                                    \
     } while (0)
 
 #define TCS_TRY(expr)                                                          \
@@ -103,6 +119,7 @@ struct tcs_ctx {
     double *linesum, *planesum, *scal;  // mean reduction
     double *red;             // reduction scratch
     double *h_pinned;        // pinned host scalars
+    tcs::Profiler *prof;     // null unless profiling is enabled
 
     tcs::Grid grid() const {
         tcs::Grid g;

@@ -97,6 +97,175 @@ k_ptdma_exact(double *__restrict__ A, const double *__restrict__ B, double *__re
     }
 }
 
+// Systems whose rows are contiguous in memory (the x sweep of an (i,j,k)
+// array): one wavefront owns 64 systems and streams their rows through LDS
+// in chunks of ROWS_R, so every global access is a coalesced run of
+// ROWS_R doubles per line while each lane still walks its own system row by
+// row in the reference's order.  LDS rows are padded to ROWS_R+1 doubles.
+constexpr int ROWS_R = 16;
+constexpr int ROWS_P = ROWS_R + 1;
+
+// tile slot a holds array a of {A, B, C, D}; MASK selects the arrays moved
+template <int MASK>
+__device__ __forceinline__ void rows_load(double (*tile)[64 * ROWS_P], double *A, double *B, double *C, double *D,
+                                          long lbase0, long ss0, int nlines, int r0, int nr) {
+    const int t = threadIdx.x;
+    double *const src[4] = {A, B, C, D};
+    for (int q = 0; q < ROWS_R; q++) {
+        const int idx = q * 64 + t;
+        const int l = idx / ROWS_R, rr = idx - l * ROWS_R;
+        if (l < nlines && rr < nr) {
+            const long g = lbase0 + (long)l * ss0 + r0 + rr;
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+                if (MASK & (1 << a)) tile[a][l * ROWS_P + rr] = src[a][g];
+        }
+    }
+}
+
+// stores tile rows [lo, hi) of the chunk starting at global row r0
+template <int MASK>
+__device__ __forceinline__ void rows_store(double (*tile)[64 * ROWS_P], double *A, double *B, double *C, double *D,
+                                           long lbase0, long ss0, int nlines, int r0, int lo, int hi) {
+    const int t = threadIdx.x;
+    double *const dst[4] = {A, B, C, D};
+    for (int q = 0; q < ROWS_R; q++) {
+        const int idx = q * 64 + t;
+        const int l = idx / ROWS_R, rr = idx - l * ROWS_R;
+        if (l < nlines && rr >= lo && rr < hi) {
+            const long g = lbase0 + (long)l * ss0 + r0 + rr;
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+                if (MASK & (1 << a)) dst[a][g] = tile[a][l * ROWS_P + rr];
+        }
+    }
+}
+
+constexpr int M_ABCD = 15, M_ACD = 13, M_D = 8;
+
+template <bool CYC>
+__global__ void __launch_bounds__(64)
+k_ptdma_rows(double *__restrict__ A, double *__restrict__ B, double *__restrict__ C, double *__restrict__ D,
+             int n0, long ss0, long ss1, int n_row) {
+    __shared__ double tile[4][64 * ROWS_P];
+    const int t = threadIdx.x;
+    const int s0a = blockIdx.x * 64;
+    const int nlines = min(64, n0 - s0a);
+    const long lbase0 = (long)s0a * ss0 + (long)blockIdx.y * ss1;
+    const bool active = t < nlines;
+    double *TA = tile[0] + t * ROWS_P;
+    double *TB = tile[1] + t * ROWS_P;
+    double *TC = tile[2] + t * ROWS_P;
+    double *TD = tile[3] + t * ROWS_P;
+    const int nch = (n_row + ROWS_R - 1) / ROWS_R;
+
+    // forward elimination (rows 1,2 normalised; j = 3..n_row)
+    double a0 = 0, c0 = 0, d0 = 0, a1 = 0, c1 = 0, d1 = 0;
+    double ap = 0, cp = 0, dp = 0;
+    for (int ch = 0; ch < nch; ch++) {
+        const int r0 = ch * ROWS_R, nr = min(ROWS_R, n_row - r0);
+        rows_load<M_ABCD>(tile, A, B, C, D, lbase0, ss0, nlines, r0, nr);
+        __syncthreads();
+        if (active) {
+            for (int rr = 0; rr < nr; rr++) {
+                const int j = r0 + rr;
+                const double aj = TA[rr], bj = TB[rr], cj = TC[rr], dj = TD[rr];
+                if (j == 0) {
+                    a0 = aj / bj; d0 = dj / bj; c0 = cj / bj;
+                } else if (j == 1) {
+                    a1 = aj / bj; d1 = dj / bj; c1 = cj / bj;
+                    TA[rr] = a1; TC[rr] = c1; TD[rr] = d1;
+                    ap = a1; cp = c1; dp = d1;
+                } else {
+                    const double r = 1.0 / (bj - aj * cp);
+                    const double dn = r * (dj - aj * dp);
+                    const double cn = r * cj;
+                    const double an = -r * aj * ap;
+                    TA[rr] = an; TC[rr] = cn; TD[rr] = dn;
+                    ap = an; cp = cn; dp = dn;
+                }
+            }
+        }
+        __syncthreads();
+        // row 0 is written back after the row-1 fix-up; keep its input bits here
+        rows_store<M_ACD>(tile, A, B, C, D, lbase0, ss0, nlines, r0, ch == 0 ? 1 : 0, nr);
+        __syncthreads();
+    }
+    (void)a0;
+    const double aN = ap, cN = cp, dN = dp;
+
+    // backward elimination j = n_row-2..2 (1-based)
+    double ax = a1, cx = c1, dx = d1;
+    if (n_row >= 3) {
+        const int jtop = n_row - 2;  // 0-based row whose forward values seed the sweep
+        for (int ch = jtop / ROWS_R; ch >= 0; ch--) {
+            const int r0 = ch * ROWS_R, nr = min(ROWS_R, n_row - r0);
+            rows_load<M_ACD>(tile, A, B, C, D, lbase0, ss0, nlines, r0, nr);
+            __syncthreads();
+            if (active) {
+                const int hi = min(r0 + nr - 1, jtop);
+                for (int j = hi; j >= max(r0, 1); j--) {
+                    const int rr = j - r0;
+                    if (j == jtop) {
+                        ax = TA[rr]; cx = TC[rr]; dx = TD[rr];
+                        continue;
+                    }
+                    const double aj = TA[rr], cj = TC[rr], dj = TD[rr];
+                    const double dn = dj - cj * dx;
+                    const double an = aj - cj * ax;
+                    const double cn = -cj * cx;
+                    TA[rr] = an; TC[rr] = cn; TD[rr] = dn;
+                    ax = an; cx = cn; dx = dn;
+                }
+            }
+            __syncthreads();
+            rows_store<M_ACD>(tile, A, B, C, D, lbase0, ss0, nlines, r0, ch == 0 ? 1 : 0, nr);
+            __syncthreads();
+        }
+    }
+
+    // row 1 fix-up and the reduced 2-row system
+    double x1 = 0, xn = 0;
+    {
+        const double r = 1.0 / (1.0 - ax * c0);
+        d0 = r * (d0 - c0 * dx);
+        a0 = r * a0;
+        c0 = -r * c0 * cx;
+    }
+    if (!CYC) {
+        const double r = 1.0 / (1.0 - aN * c0);
+        xn = r * (dN - aN * d0);
+        x1 = d0 - c0 * xn;
+    } else {
+        const double e2 = -cN;
+        x1 = (d0 - a0 * dN - c0 * dN) / (1.0 + a0 * e2 + c0 * e2);
+        xn = dN + x1 * e2;
+    }
+
+    // update pass: rows 2..n_row-1 (1-based), rows 1 and n_row from the reduced solve
+    for (int ch = 0; ch < nch; ch++) {
+        const int r0 = ch * ROWS_R, nr = min(ROWS_R, n_row - r0);
+        rows_load<M_ACD>(tile, A, B, C, D, lbase0, ss0, nlines, r0, nr);
+        __syncthreads();
+        if (active) {
+            for (int rr = 0; rr < nr; rr++) {
+                const int j = r0 + rr;
+                if (j == 0) {
+                    TA[rr] = a0; TC[rr] = c0; TD[rr] = x1;
+                } else if (j == n_row - 1) {
+                    TD[rr] = xn;
+                } else {
+                    TD[rr] = TD[rr] - TA[rr] * x1 - TC[rr] * xn;
+                }
+            }
+        }
+        __syncthreads();
+        if (ch == 0) rows_store<M_ACD>(tile, A, B, C, D, lbase0, ss0, nlines, r0, 0, 1);  // row 1: A, C, D
+        rows_store<M_D>(tile, A, B, C, D, lbase0, ss0, nlines, r0, ch == 0 ? 1 : 0, nr);
+        __syncthreads();
+    }
+}
+
 int ptdma_exact_launch(double *A, const double *B, double *C, double *D,
                        int n0, long ss0, int n1, long ss1, int n_row, long rs,
                        bool cyclic, hipStream_t st) {
@@ -105,12 +274,19 @@ int ptdma_exact_launch(double *A, const double *B, double *C, double *D,
         set_error("ptdma: n_row must be >= 2 and the outer system count <= 65535");
         return TCS_E_ARG;
     }
+    if (rs == 1 && n_row >= 2) {
+        dim3 rblk(64), rgrd((unsigned)((n0 + 63) / 64), (unsigned)n1);
+        if (cyclic)
+            TCS_LAUNCH(st, "ptdma_rows_cyclic", k_ptdma_rows<true>, rgrd, rblk, 0, A, (double *)B, C, D, n0, ss0, ss1, n_row);
+        else
+            TCS_LAUNCH(st, "ptdma_rows", k_ptdma_rows<false>, rgrd, rblk, 0, A, (double *)B, C, D, n0, ss0, ss1, n_row);
+        return TCS_OK;
+    }
     dim3 blk(256), grd((unsigned)((n0 + 255) / 256), (unsigned)n1);
     if (cyclic)
-        hipLaunchKernelGGL(k_ptdma_exact<true>, grd, blk, 0, st, A, B, C, D, n0, ss0, ss1, n_row, rs);
+        TCS_LAUNCH(st, "ptdma_cyclic", k_ptdma_exact<true>, grd, blk, 0, A, B, C, D, n0, ss0, ss1, n_row, rs);
     else
-        hipLaunchKernelGGL(k_ptdma_exact<false>, grd, blk, 0, st, A, B, C, D, n0, ss0, ss1, n_row, rs);
-    TCS_LAUNCH_CHECK();
+        TCS_LAUNCH(st, "ptdma", k_ptdma_exact<false>, grd, blk, 0, A, B, C, D, n0, ss0, ss1, n_row, rs);
     return TCS_OK;
 }
 

@@ -343,30 +343,25 @@ int sweep_y(tcs_ctx *c, int j0, int nrows) {
 }
 
 int thermal_coeffi(tcs_ctx *c) {
-    hipLaunchKernelGGL(k_thermal_coeffi, dim3(2048), dim3(256), 0, c->stream, c->grid(), c->phys,
+    TCS_LAUNCH(c->stream, "thermal_coeffi", k_thermal_coeffi, dim3(2048), dim3(256), 0, c->grid(), c->phys,
                        c->f[TCS_T], c->f[TCS_KPP], c->f[TCS_DKPP], c->f[TCS_INVRHOCP], c->f[TCS_DINVRHOCP]);
-    TCS_LAUNCH_CHECK();
     return TCS_OK;
 }
 
 int thermal_solver(tcs_ctx *c, double dt) {
     const Grid g = c->grid();
     const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
-    hipLaunchKernelGGL(k_thermal_z, grd, blk, 0, c->stream, g, c->phys, dt, c->f[TCS_T], c->f[TCS_U],
+    TCS_LAUNCH(c->stream, "thermal_z", k_thermal_z, grd, blk, 0, g, c->phys, dt, c->f[TCS_T], c->f[TCS_U],
                        c->f[TCS_V], c->f[TCS_W], c->wall[TCS_TBCUP], c->wall[TCS_TBCBT], c->am, c->ac, c->ap,
                        c->rhs);
-    TCS_LAUNCH_CHECK();
     TCS_TRY(sweep_z(c, true));
-    hipLaunchKernelGGL(k_thermal_x, grd, blk, 0, c->stream, g, c->phys, dt, c->f[TCS_T], c->f[TCS_U], c->am,
+    TCS_LAUNCH(c->stream, "thermal_x", k_thermal_x, grd, blk, 0, g, c->phys, dt, c->f[TCS_T], c->f[TCS_U], c->am,
                        c->ac, c->ap);
-    TCS_LAUNCH_CHECK();
     TCS_TRY(sweep_x(c, true));
-    hipLaunchKernelGGL(k_thermal_y, grd, blk, 0, c->stream, g, c->phys, dt, c->f[TCS_T], c->f[TCS_V], c->am,
+    TCS_LAUNCH(c->stream, "thermal_y", k_thermal_y, grd, blk, 0, g, c->phys, dt, c->f[TCS_T], c->f[TCS_V], c->am,
                        c->ac, c->ap);
-    TCS_LAUNCH_CHECK();
     TCS_TRY(sweep_y(c, 1, c->n2m));
-    hipLaunchKernelGGL(k_add_interior, grd, blk, 0, c->stream, g, c->f[TCS_T], c->rhs);
-    TCS_LAUNCH_CHECK();
+    TCS_LAUNCH(c->stream, "add_interior", k_add_interior, grd, blk, 0, g, c->f[TCS_T], c->rhs);
     return TCS_OK;
 }
 

@@ -171,6 +171,23 @@ class TCSSolver:
         """One MPM-STD step: the loop body of src/main.f90:131-203."""
         _lib.check(self.lib.tcs_step(self.h, self._dt(dt)), "tcs_step")
 
+    def profile(self, enable=True):
+        """Enable (and clear) or disable the per-stage / per-kernel timers."""
+        _lib.check(self.lib.tcs_profile(self.h, int(bool(enable))), "tcs_profile")
+
+    def profile_report(self):
+        """{label: (total_ms, count)} in first-seen order."""
+        n = ctypes.c_int32()
+        _lib.check(self.lib.tcs_profile_count(self.h, ctypes.byref(n)), "tcs_profile_count")
+        out = {}
+        buf = ctypes.create_string_buffer(128)
+        for q in range(n.value):
+            ms, cnt = ctypes.c_double(), ctypes.c_int64()
+            _lib.check(self.lib.tcs_profile_entry(self.h, q, buf, 128, ctypes.byref(ms), ctypes.byref(cnt)),
+                       "tcs_profile_entry")
+            out[buf.value.decode()] = (ms.value, cnt.value)
+        return out
+
     def _dt(self, dt):
         return float(self.dt if dt is None else dt)
 

@@ -135,3 +135,20 @@ def test_full_step_seeded(shape):
         o.step(cfg.dtStart)
         s.step(cfg.dtStart)
         _compare(o, s, ["T", "U", "V", "W", "P", "dPhat", "dP0"])
+
+
+def test_nusselt_conduction_profile_is_one():
+    """Closed form: the linear conduction profile between the wall rows gives
+    Nu = 1 at both walls (stretched grid)."""
+    from pascal_tcs_amd.solver import TCSSolver
+    cfg = small_config(16, 24, 8, props="constant")
+    s = TCSSolver(cfg)
+    x2 = s.x2
+    yc = 0.5 * (x2[1:-1] + x2[2:])
+    T = np.zeros(s.shape)
+    T[:, 1:-1, :] = (cfg.Tbt + (cfg.Tup - cfg.Tbt) * yc / cfg.L2)[None, :, None]
+    T[:, 0, :] = cfg.Tbt
+    T[:, -1, :] = cfg.Tup
+    s.set("T", T)
+    nb, nt = s.nusselt()
+    assert abs(nb - 1.0) < 1e-12 and abs(nt - 1.0) < 1e-12
