@@ -3,11 +3,20 @@
 
 Workload: 512^3 cells (n1m=n2m=n3m=512), Lx=Lz=2*pi, Ly=2, periodic x/z,
 no-slip walls in y with a tanh-stretched y grid (gamma 2.0), Ra=1e8,
-Pr=0.71, water property polynomials, T = conduction profile + seeded noise
-of amplitude 0.01.  The time step is fixed (default 1e-3) while Post_CFL is
-still evaluated every step: from rest the reference's CFL rule returns
-dtMax = 100*min(dx) (0.39 at 512^3), and that second step drives the
-reference scheme past its own blow-up threshold (maxDivU >= 1e-3).
+Pr=0.71, water property polynomials with DeltaT=0.2, T = conduction profile
++ seeded noise of amplitude 0.01.  The time step is fixed (default 1e-3)
+while Post_CFL is still evaluated every step.  Two properties of the
+reference scheme fix these choices (the CPU oracle shows them):
+  * from rest its CFL rule returns dtMax = 100*min(dx) (0.39 at 512^3), and
+    a step of that size drives maxDivU to ~1e3;
+  * on one y-rank its thermal boundary routine sets only the bottom ghost
+    row of T (if / else if), so the top row is set here to Tup, as the top
+    rank of a multi-rank run sets it;
+  * its variable-density pressure correction (ExtraTerm with the in-place
+    dPhat update of the projection and a wall ghost of dPhat that is never
+    set) amplifies the divergence at the wall-adjacent cells by a factor
+    that grows with the density contrast and the wall-normal resolution: at
+    n2m=512 it diverges for DeltaT >= 0.5 and decays for DeltaT = 0.2.
 One step is the loop body of the reference's main program: thermal,
 momentum (dU, dV, dW, block corrections, update), pressure (RHS, FFT/TDMA
 Poisson, projection), every ghost-cell update, Post_Div and Post_CFL.
@@ -29,10 +38,10 @@ METRIC = "Mcell-steps/s at 512³: FP64 plain vs FP32-mixed at stated error; HBM 
 HBM_PEAK_GBS = 8000.0
 
 
-def workload_config(n, props="water"):
+def workload_config(n, props="water", deltat=0.2):
     from pascal_tcs_amd.config import Properties, TCSConfig
     p = {"water": Properties.water(), "constant": Properties.constant()}[props]
-    return TCSConfig(n1m=n, n2m=n, n3m=n, uniform2=0, gamma2=2.0, Ra=1e8, Pr=0.71, DeltaT=10.0,
+    return TCSConfig(n1m=n, n2m=n, n3m=n, uniform2=0, gamma2=2.0, Ra=1e8, Pr=0.71, DeltaT=deltat,
                      MaxCFL=0.5, dtStart=1e-3, props=p)
 
 
@@ -48,6 +57,10 @@ def kernel_bytes(label, cfg):
         "ptdma": 40.0 * N * (3 + fv) / 4,
         # 8 cyclic z/x sweeps (T, dU, dW full; dV without the wall row)
         "ptdma_cyclic": 40.0 * N * (6 + 2 * fv) / 8,
+        # split-line kernels: y sweeps; z sweeps (strided, cyclic); x sweeps (contiguous, cyclic)
+        "ptdma_part": 40.0 * N * (3 + fv) / 4,
+        "ptdma_part_cyclic": 40.0 * N * (3 + fv) / 4,
+        "ptdma_part_rows_cyclic": 40.0 * N * (3 + fv) / 4,
         "thermal_z": 8.0 * N * (4 + 4), "thermal_x": 8.0 * N * (2 + 3), "thermal_y": 8.0 * N * (2 + 3),
         "dU_z": 8.0 * N * (6 + 4), "dV_z": 8.0 * N * (6 + 4) * fv, "dW_z": 8.0 * N * (7 + 4),
         "dU_x": 8.0 * N * (1 + 3), "dU_y": 8.0 * N * (1 + 3), "dV_x": 8.0 * N * (1 + 3) * fv,
@@ -63,14 +76,19 @@ def run_gpu(args):
     import torch
     from pascal_tcs_amd.solver import TCSSolver
 
-    cfg = workload_config(args.n)
-    s = TCSSolver(cfg)
+    cfg = workload_config(args.n, deltat=args.deltat)
+    s = TCSSolver(cfg, tdma_split=args.tdma_split)
     s.setup_seeded(amp=0.01, seed=20240512)
-    state = {"dt": args.dt if args.dt > 0 else cfg.dtStart, "div": 0.0, "cfl": 0.0}
+    T = s.get("T")
+    T[:, -1, :] = cfg.Tup  # top wall row, as the top y-rank of a multi-rank run sets it
+    s.set("T", T)
+    del T
+    state = {"dt": args.dt if args.dt > 0 else cfg.dtStart, "div": 0.0, "cfl": 0.0, "divs": []}
 
     def one_step():
         s.step(state["dt"])
         state["div"] = s.mpi_Post_Div()
+        state["divs"].append(state["div"])
         cfl, newdt = s.mpi_Post_CFL()
         if args.dt > 0:
             state["cfl"] = state["dt"] * cfl / newdt  # CFL number of the step just taken
@@ -116,14 +134,14 @@ def roofline(cfg, rep, psteps):
     return dom, info
 
 
-def run_cpu_baseline(seconds):
+def run_cpu_baseline(seconds, deltat):
     """The CPU oracle (single thread) on a 512 x 512 x 8 slab of the same
     workload (same box, grids and physics; z reduced to 8 planes)."""
     from oracle import oracle as O
     from pascal_tcs_amd.ic import conduction_noise_T, embed_interior
     from tests.cases import make_oracle
 
-    cfg = workload_config(512)
+    cfg = workload_config(512, deltat=deltat)
     cfg.n3m = 8
     o = make_oracle(cfg)
     o.f["T"][...] = embed_interior(conduction_noise_T(cfg, o.x2, 0.01, 20240512))
@@ -131,6 +149,7 @@ def run_cpu_baseline(seconds):
     o.bc["TBCbt"][...] = cfg.Tbt
     o.thermal_boundary()
     o.momentum_boundary()
+    o.f["T"][:, -1, :] = cfg.Tup
     for n in ("T", "U", "V", "W", "P"):
         o.halo(n)
     dt = 1e-3
@@ -160,6 +179,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dt", type=float, default=1e-3, help="fixed time step (<= 0: the reference's CFL rule)")
+    ap.add_argument("--deltat", type=float, default=0.2, help="DeltaT of the property polynomials")
+    ap.add_argument("--tdma-split", type=int, default=1, choices=(0, 1),
+                    help="1: ADI lines split into ceil(n/16) parts (PaScaL_TDMA's multi-rank order, "
+                         "within 1e-15 of the single-rank order); 0: single-rank order")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs the whole domain on one GPU; --gpus {args.gpus} is not supported "
@@ -187,17 +210,20 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"plain FP64 MPM-STD step, {cfg.n1m}x{cfg.n2m}x{cfg.n3m}, tanh y (gamma 2.0), "
-                               "Ra=1e8, Pr=0.71, water properties, T noise 0.01, "
+                               f"Ra=1e8, Pr=0.71, water properties DeltaT={cfg.DeltaT:g}, T noise 0.01, "
                                + (f"fixed dt {args.dt:g} (CFL evaluated every step)" if args.dt > 0 else "CFL 0.5"),
-                   "cells": cells, "form": "plain FP64 (reference operation order)"},
+                   "cells": cells,
+                   "form": "plain FP64, reference operation order" + (
+                       "; ADI lines solved split into ceil(n/16) parts as PaScaL_TDMA splits them over ranks"
+                       if args.tdma_split else "; single-rank line solves")},
         "roofline": roof,
         "stage_ms_per_step": stages,
         "kernel_ms_per_step": kernels,
         "final": {"dt": state["dt"], "maxDivU": state["div"], "CFL": state["cfl"], "Nu_bottom": nu[0],
-                  "Nu_top": nu[1]},
+                  "Nu_top": nu[1], "maxDivU_per_step": [float(f"{d:.3e}") for d in state["divs"]]},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = run_cpu_baseline(args.cpu_seconds)
+        out["cpu_baseline"] = run_cpu_baseline(args.cpu_seconds, args.deltat)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

@@ -41,7 +41,10 @@ typedef struct tcs_params {
     int32_t n1m, n2m, n3m;        /* interior cells (namelist meshes) */
     int32_t pbc1, pbc2, pbc3;     /* periodicity; only (1,0,1) is supported */
     int32_t precision;            /* 0 = plain FP64 (the only form built) */
-    int32_t reserved;
+    int32_t tdma_split;           /* 0: every ADI line solved as on one rank;
+                                     1: lines of up to 512 rows split into
+                                     ceil(n/16) parts, solved as PaScaL_TDMA
+                                     solves lines split over that many ranks */
     double Ct, Cmu, Cmt, Cmp;     /* :139-141 */
     double DeltaT;                /* namelist sim_parameter */
     double a10, a11, a12, a13, a14, a15;  /* density polynomial   :117 */
@@ -156,8 +159,10 @@ int tcs_post_cfl(tcs_ctx *ctx, double *newcfl, double *newdt);
    every halo update in main's order), fully on the device. */
 int tcs_step(tcs_ctx *ctx, double dt);
 /* Wall Nusselt numbers (no counterpart in the reference): mean over (i,k)
-   of (T(i,0,k) - T(i,1,k))/dmx2(1) and (T(i,n2m,k) - T(i,n2sub,k))/dmx2(n2sub)
-   (the wall rows the solver uses), times L2/(Tbt - Tup). */
+   of (TBCbt(i,k) - T(i,1,k))/dmx2(1) and (T(i,n2m,k) - TBCup(i,k))/dmx2(n2sub),
+   times L2/(Tbt - Tup).  The wall arrays are the wall temperatures the
+   thermal solver applies; on one y-rank the top ghost row of T is not set
+   (src/module_solve_thermal.f90:102-110) and is not used here. */
 int tcs_nusselt(tcs_ctx *ctx, double Tup, double Tbt, double *nu_bottom, double *nu_top);
 
 /* ---- timers ----------------------------------------------------------- */
@@ -196,6 +201,21 @@ int ptdma_many_solve_host(double *A, double *B, double *C, double *D,
 int ptdma_axis_solve_dev(double *A, const double *B, double *C, double *D,
                          int32_t n1, int32_t n2, int32_t n3, int32_t axis,
                          int32_t cyclic, void *stream);
+/* PaScaL_TDMA_many_solve[_cycle] with every system's rows split over
+   nprocs = ceil(n_row/16) ranks (para_range, PaScaL_TDMA/src/para_range.f90:
+   26-40): per-rank modified Thomas (pascal_tdma.f90:501-536), the reduced
+   system of 2*nprocs rows in rank order (pascal_tdma.f90:243-266) solved by
+   tdma_many / tdma_cycl_many, per-rank update (:567-576).  The operation
+   order of a run of the reference on nprocs ranks along the solve
+   direction; each thread of the kernel plays one rank.  2 <= n_row <= 512.
+   D returns the solution; A and C are read only here (the reference leaves
+   its per-rank modified coefficients in them). */
+int ptdma_many_solve_part_dev(const double *A, const double *B, const double *C, double *D,
+                              int32_t n_sys, int32_t n_row, int32_t cyclic, void *stream);
+/* Axis solve of ptdma_axis_solve_dev with the split lines above. */
+int ptdma_axis_solve_part_dev(const double *A, const double *B, const double *C, double *D,
+                              int32_t n1, int32_t n2, int32_t n3, int32_t axis,
+                              int32_t cyclic, void *stream);
 
 #ifdef __cplusplus
 }

@@ -71,6 +71,10 @@ def lib():
             getattr(L, name).argtypes = [_dp] * 4 + [ctypes.c_int] * 2
         for name in ["orc_tdma_single", "orc_tdma_cycl_single"]:
             getattr(L, name).argtypes = [_dp] * 4 + [ctypes.c_int]
+        L.orc_ptdma_many_solve_ranks.argtypes = [_dp] * 4 + [ctypes.c_int] * 4
+        L.orc_tdma_parts.argtypes = [ctypes.c_int]
+        L.orc_tdma_parts.restype = ctypes.c_int
+        L.orc_set_tdma_split.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_fft.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int]
         L.orc_twiddles.argtypes = [ctypes.c_int, _dp, _dp]
         L.orc_wave_numbers.argtypes = [ctypes.c_void_p, _dp, _dp]
@@ -117,6 +121,20 @@ def ptdma_many_solve(A, B, C, D, cyclic=False):
     f(_p(A), _p(B), _p(C), _p(D), n_sys, n_row)
 
 
+def ptdma_many_solve_ranks(A, B, C, D, nprocs, cyclic=False):
+    """The same solve with every system's rows split over `nprocs` ranks
+    (para_range); (n_row, n_sys) C-order arrays, in place."""
+    n_row, n_sys = A.shape
+    if not (1 <= nprocs and n_row // nprocs >= 2):
+        raise ValueError("every rank needs at least two rows")
+    lib().orc_ptdma_many_solve_ranks(_p(A), _p(B), _p(C), _p(D), n_sys, n_row, int(nprocs), int(bool(cyclic)))
+
+
+def tdma_parts(n_row):
+    """Parts per line of the split line solves (ceil(n_row/16) up to 512 rows)."""
+    return lib().orc_tdma_parts(int(n_row))
+
+
 def tdma_many(a, b, c, d, cyclic=False):
     n2, n1 = a.shape
     f = lib().orc_tdma_cycl_many if cyclic else lib().orc_tdma_many
@@ -159,6 +177,10 @@ class Oracle:
         for i, name in enumerate(BCS):
             ptr = L.orc_bc(self.h, i)
             self.bc[name] = np.ctypeslib.as_array(ptr, shape=((n3m + 2) * (n1m + 2),)).reshape(n3m + 2, n1m + 2)
+
+    def set_tdma_split(self, split):
+        """1: the step's ADI lines are solved split into tdma_parts(n) ranks."""
+        lib().orc_set_tdma_split(self.h, int(split))
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -39,6 +39,7 @@ struct orc {
     double *f[OF_NFIELD];
     double *bc[OB_N];
     double *dzk, *dxk2;
+    int tdma_split;          /* 0: single-rank line solves; 1: lines split into parts */
 };
 
 #define IX(i, j, k) ((size_t)(i) + (size_t)o->N1 * ((size_t)(j) + (size_t)o->N2 * (size_t)(k)))
@@ -222,70 +223,121 @@ void orc_tdma_cycl_many(double *a, double *b, double *c, double *d, int n1, int 
 }
 
 /* ------------------------------------------------------------------------ */
-/* PaScaL_TDMA_many_solve / _cycle on one rank:                              */
-/* PaScaL_TDMA/src/pascal_tdma.f90:487-578 and :715-806                      */
+/* PaScaL_TDMA_many_solve / _cycle: PaScaL_TDMA/src/pascal_tdma.f90:487-578   */
+/* and :715-806.  The rows of every system are split over `nprocs` ranks by   */
+/* para_range (PaScaL_TDMA/src/para_range.f90:26-40); each rank runs the      */
+/* modified Thomas algorithm on its own rows, the reduced 2-rows-per-rank     */
+/* systems are gathered in rank order (the Alltoallw of plan_many_create,     */
+/* pascal_tdma.f90:243-266) and solved by tdma_many / tdma_cycl_many, and     */
+/* each rank updates its rows.  nprocs = 1 is the single-rank solve.          */
 /* ------------------------------------------------------------------------ */
-static void ptdma_many_common(double *A, double *B, double *C, double *D,
-                              int n_sys, int n_row, int cyclic)
-{
 #define P2(x, i, j) x[(size_t)(i) + (size_t)n_sys * (size_t)(j)]
+/* one rank's rows j0 .. j0+m-1: elimination of both off-diagonals */
+static void ptdma_rank_reduce(double *A, double *B, double *C, double *D, int n_sys, int j0, int m)
+{
     for (int i = 0; i < n_sys; i++) {
-        P2(A, i, 0) = P2(A, i, 0) / P2(B, i, 0);
-        P2(D, i, 0) = P2(D, i, 0) / P2(B, i, 0);
-        P2(C, i, 0) = P2(C, i, 0) / P2(B, i, 0);
-        P2(A, i, 1) = P2(A, i, 1) / P2(B, i, 1);
-        P2(D, i, 1) = P2(D, i, 1) / P2(B, i, 1);
-        P2(C, i, 1) = P2(C, i, 1) / P2(B, i, 1);
+        P2(A, i, j0) = P2(A, i, j0) / P2(B, i, j0);
+        P2(D, i, j0) = P2(D, i, j0) / P2(B, i, j0);
+        P2(C, i, j0) = P2(C, i, j0) / P2(B, i, j0);
+        P2(A, i, j0 + 1) = P2(A, i, j0 + 1) / P2(B, i, j0 + 1);
+        P2(D, i, j0 + 1) = P2(D, i, j0 + 1) / P2(B, i, j0 + 1);
+        P2(C, i, j0 + 1) = P2(C, i, j0 + 1) / P2(B, i, j0 + 1);
     }
-    for (int j = 2; j < n_row; j++)
+    for (int j = j0 + 2; j < j0 + m; j++)
         for (int i = 0; i < n_sys; i++) {
             double r = 1.0 / (P2(B, i, j) - P2(A, i, j) * P2(C, i, j - 1));
             P2(D, i, j) = r * (P2(D, i, j) - P2(A, i, j) * P2(D, i, j - 1));
             P2(C, i, j) = r * P2(C, i, j);
             P2(A, i, j) = -r * P2(A, i, j) * P2(A, i, j - 1);
         }
-    for (int j = n_row - 3; j >= 1; j--)
+    for (int j = j0 + m - 3; j >= j0 + 1; j--)
         for (int i = 0; i < n_sys; i++) {
             P2(D, i, j) = P2(D, i, j) - P2(C, i, j) * P2(D, i, j + 1);
             P2(A, i, j) = P2(A, i, j) - P2(C, i, j) * P2(A, i, j + 1);
             P2(C, i, j) = -P2(C, i, j) * P2(C, i, j + 1);
         }
-    /* reduced 2-row systems (plan%A_rd .. D_rd), single rank: no exchange */
-    double *ra = (double *)malloc(sizeof(double) * (size_t)n_sys * 2);
-    double *rb = (double *)malloc(sizeof(double) * (size_t)n_sys * 2);
-    double *rc = (double *)malloc(sizeof(double) * (size_t)n_sys * 2);
-    double *rd = (double *)malloc(sizeof(double) * (size_t)n_sys * 2);
     for (int i = 0; i < n_sys; i++) {
-        double r = 1.0 / (1.0 - P2(A, i, 1) * P2(C, i, 0));
-        P2(D, i, 0) = r * (P2(D, i, 0) - P2(C, i, 0) * P2(D, i, 1));
-        P2(A, i, 0) = r * P2(A, i, 0);
-        P2(C, i, 0) = -r * P2(C, i, 0) * P2(C, i, 1);
-        ra[i] = P2(A, i, 0); ra[i + n_sys] = P2(A, i, n_row - 1);
-        rb[i] = 1.0;         rb[i + n_sys] = 1.0;
-        rc[i] = P2(C, i, 0); rc[i + n_sys] = P2(C, i, n_row - 1);
-        rd[i] = P2(D, i, 0); rd[i + n_sys] = P2(D, i, n_row - 1);
+        double r = 1.0 / (1.0 - P2(A, i, j0 + 1) * P2(C, i, j0));
+        P2(D, i, j0) = r * (P2(D, i, j0) - P2(C, i, j0) * P2(D, i, j0 + 1));
+        P2(A, i, j0) = r * P2(A, i, j0);
+        P2(C, i, j0) = -r * P2(C, i, j0) * P2(C, i, j0 + 1);
     }
-    if (cyclic) orc_tdma_cycl_many(ra, rb, rc, rd, n_sys, 2);
-    else orc_tdma_many(ra, rb, rc, rd, n_sys, 2);
-    for (int i = 0; i < n_sys; i++) {
-        P2(D, i, 0) = rd[i];
-        P2(D, i, n_row - 1) = rd[i + n_sys];
+}
+
+static void ptdma_many_ranks(double *A, double *B, double *C, double *D,
+                             int n_sys, int n_row, int nprocs, int cyclic)
+{
+    const int nr = 2 * nprocs;
+    int *j0 = (int *)malloc(sizeof(int) * (size_t)nprocs);
+    int *mm = (int *)malloc(sizeof(int) * (size_t)nprocs);
+    const int w1 = n_row / nprocs, w2 = n_row % nprocs;
+    for (int p = 0; p < nprocs; p++) {
+        j0[p] = p * w1 + (p < w2 ? p : w2);
+        mm[p] = w1 + (p < w2 ? 1 : 0);
     }
-    for (int j = 1; j < n_row - 1; j++)
-        for (int i = 0; i < n_sys; i++)
-            P2(D, i, j) = P2(D, i, j) - P2(A, i, j) * P2(D, i, 0) - P2(C, i, j) * P2(D, i, n_row - 1);
-    free(ra); free(rb); free(rc); free(rd);
+    double *ra = (double *)malloc(sizeof(double) * (size_t)n_sys * nr);
+    double *rb = (double *)malloc(sizeof(double) * (size_t)n_sys * nr);
+    double *rc = (double *)malloc(sizeof(double) * (size_t)n_sys * nr);
+    double *rd = (double *)malloc(sizeof(double) * (size_t)n_sys * nr);
+    for (int p = 0; p < nprocs; p++) {
+        const int a = j0[p], b = j0[p] + mm[p] - 1;
+        ptdma_rank_reduce(A, B, C, D, n_sys, a, mm[p]);
+        for (int i = 0; i < n_sys; i++) {
+            /* plan%A_rd(i,1:2) .. D_rd, placed at reduced rows 2p, 2p+1 */
+            size_t q0 = (size_t)i + (size_t)n_sys * (size_t)(2 * p), q1 = q0 + (size_t)n_sys;
+            ra[q0] = P2(A, i, a); ra[q1] = P2(A, i, b);
+            rb[q0] = 1.0;         rb[q1] = 1.0;
+            rc[q0] = P2(C, i, a); rc[q1] = P2(C, i, b);
+            rd[q0] = P2(D, i, a); rd[q1] = P2(D, i, b);
+        }
+    }
+    if (cyclic) orc_tdma_cycl_many(ra, rb, rc, rd, n_sys, nr);
+    else orc_tdma_many(ra, rb, rc, rd, n_sys, nr);
+    for (int p = 0; p < nprocs; p++) {
+        const int a = j0[p], b = j0[p] + mm[p] - 1;
+        for (int i = 0; i < n_sys; i++) {
+            size_t q0 = (size_t)i + (size_t)n_sys * (size_t)(2 * p), q1 = q0 + (size_t)n_sys;
+            P2(D, i, a) = rd[q0];
+            P2(D, i, b) = rd[q1];
+        }
+        for (int j = a + 1; j < b; j++)
+            for (int i = 0; i < n_sys; i++)
+                P2(D, i, j) = P2(D, i, j) - P2(A, i, j) * P2(D, i, a) - P2(C, i, j) * P2(D, i, b);
+    }
+    free(ra); free(rb); free(rc); free(rd); free(j0); free(mm);
+}
 #undef P2
+
+void orc_ptdma_many_solve_ranks(double *A, double *B, double *C, double *D, int n_sys, int n_row,
+                                int nprocs, int cyclic)
+{
+    ptdma_many_ranks(A, B, C, D, n_sys, n_row, nprocs, cyclic);
 }
 
 void orc_ptdma_many_solve(double *A, double *B, double *C, double *D, int n_sys, int n_row)
 {
-    ptdma_many_common(A, B, C, D, n_sys, n_row, 0);
+    ptdma_many_ranks(A, B, C, D, n_sys, n_row, 1, 0);
 }
 
 void orc_ptdma_many_solve_cycle(double *A, double *B, double *C, double *D, int n_sys, int n_row)
 {
-    ptdma_many_common(A, B, C, D, n_sys, n_row, 1);
+    ptdma_many_ranks(A, B, C, D, n_sys, n_row, 1, 1);
+}
+
+/* Parts per line of the split solve: lines of 2..512 rows are split into
+   ceil(n_row/16) parts (at most 16 rows each, at least 8 once there are two
+   parts), longer lines are solved on one rank. */
+int orc_tdma_parts(int n_row)
+{
+    return (n_row >= 2 && n_row <= 512) ? (n_row + 15) / 16 : 1;
+}
+
+void orc_set_tdma_split(orc *o, int split) { o->tdma_split = split; }
+
+/* the step's line solves: single rank, or split as above */
+static void lines_solve(const orc *o, double *A, double *B, double *C, double *D, int n_sys, int n_row, int cyclic)
+{
+    ptdma_many_ranks(A, B, C, D, n_sys, n_row, o->tdma_split ? orc_tdma_parts(n_row) : 1, cyclic);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -499,7 +551,7 @@ void orc_thermal_solver(orc *o, double dt)
             }
         }
     }
-    orc_ptdma_many_solve_cycle(am, ac, ap, RHS, n1msub * n2msub, n3msub);
+    lines_solve(o, am, ac, ap, RHS, n1msub * n2msub, n3msub, 1);
 
     /* transpose (i,j,k) -> (j,k,i) (:371-379) */
     double *RHSI = (double *)calloc(n, sizeof(double));
@@ -542,7 +594,7 @@ void orc_thermal_solver(orc *o, double dt)
                 ap[q] = eAPI * dt;
                 am[q] = eAMI * dt;
             }
-    orc_ptdma_many_solve_cycle(am, ac, ap, RHSI, n2msub * n3msub, n1msub);
+    lines_solve(o, am, ac, ap, RHSI, n2msub * n3msub, n1msub, 1);
 
     /* transpose (j,k,i) -> (i,k,j) (:439-447) */
     for (j = 1; j <= n2msub; j++)
@@ -589,7 +641,7 @@ void orc_thermal_solver(orc *o, double dt)
                 am[q] = eAMJ * dt;
             }
         }
-    orc_ptdma_many_solve(am, ac, ap, RHS, n3msub * n1msub, n2msub);
+    lines_solve(o, am, ac, ap, RHS, n3msub * n1msub, n2msub, 0);
 
     for (k = 1; k <= n3msub; k++)
         for (j = 1; j <= n2msub; j++)
@@ -765,7 +817,7 @@ void orc_momentum_solvedU(orc *o, double dt)
             }
         }
     }
-    orc_ptdma_many_solve_cycle(am, ac, ap, RHS, n1msub * n2msub, n3msub);
+    lines_solve(o, am, ac, ap, RHS, n1msub * n2msub, n3msub, 1);
 
     double *RHSI = (double *)calloc(n, sizeof(double));
     for (k = 1; k <= n3msub; k++)
@@ -793,7 +845,7 @@ void orc_momentum_solvedU(orc *o, double dt)
                 ap[q] = mAPI * dt;
                 am[q] = mAMI * dt;
             }
-    orc_ptdma_many_solve_cycle(am, ac, ap, RHSI, n2msub * n3msub, n1msub);
+    lines_solve(o, am, ac, ap, RHSI, n2msub * n3msub, n1msub, 1);
 
     for (int jj = 1; jj <= n2msub; jj++)
         for (i = o->i_indexS; i <= n1msub; i++)
@@ -827,7 +879,7 @@ void orc_momentum_solvedU(orc *o, double dt)
                 am[q] = mAMJ * dt;
             }
         }
-    orc_ptdma_many_solve(am, ac, ap, RHS, n3msub * n1msub, n2msub);
+    lines_solve(o, am, ac, ap, RHS, n3msub * n1msub, n2msub, 0);
 
     for (k = 1; k <= n3msub; k++)
         for (j = 1; j <= n2msub; j++)
@@ -959,7 +1011,7 @@ void orc_momentum_solvedV(orc *o, double dt)
             }
         }
     }
-    orc_ptdma_many_solve_cycle(am, ac, ap, RHS, n1msub * n2msub, n3msub);
+    lines_solve(o, am, ac, ap, RHS, n1msub * n2msub, n3msub, 1);
 
     double *RHSI = (double *)calloc(n, sizeof(double));
     for (k = 1; k <= n3msub; k++)
@@ -988,7 +1040,7 @@ void orc_momentum_solvedV(orc *o, double dt)
                 am[q] = mAMI * dt;
             }
         }
-    orc_ptdma_many_solve_cycle(am, ac, ap, RHSI, n2msub * n3msub, n1msub);
+    lines_solve(o, am, ac, ap, RHSI, n2msub * n3msub, n1msub, 1);
 
     int ny = o->n2sub - j_indexS; /* rows j=j_indexS..n2msub */
     size_t nyv = (size_t)n1msub * n3msub * ny;
@@ -1026,7 +1078,7 @@ void orc_momentum_solvedV(orc *o, double dt)
                 amj[q] = mAMJ * dt;
             }
         }
-    orc_ptdma_many_solve(amj, acj, apj, RHSJ, n3msub * n1msub, ny);
+    lines_solve(o, amj, acj, apj, RHSJ, n3msub * n1msub, ny, 0);
 
     for (k = 0; k <= o->n3sub; k++)
         for (i = 0; i <= o->n1sub; i++) {
@@ -1170,7 +1222,7 @@ void orc_momentum_solvedW(orc *o, double dt)
             }
         }
     }
-    orc_ptdma_many_solve_cycle(am, ac, ap, RHS, n1msub * n2msub, n3msub);
+    lines_solve(o, am, ac, ap, RHS, n1msub * n2msub, n3msub, 1);
 
     double *RHSI = (double *)calloc(n, sizeof(double));
     for (k = o->k_indexS; k <= n3msub; k++)
@@ -1200,7 +1252,7 @@ void orc_momentum_solvedW(orc *o, double dt)
             }
         }
     }
-    orc_ptdma_many_solve_cycle(am, ac, ap, RHSI, n2msub * n3msub, n1msub);
+    lines_solve(o, am, ac, ap, RHSI, n2msub * n3msub, n1msub, 1);
 
     for (k = 1; k <= n3msub; k++)
         for (j = 1; j <= n2msub; j++)
@@ -1228,7 +1280,7 @@ void orc_momentum_solvedW(orc *o, double dt)
                 am[q] = mAMJ * dt;
             }
         }
-    orc_ptdma_many_solve(am, ac, ap, RHS, n3msub * n1msub, n2msub);
+    lines_solve(o, am, ac, ap, RHS, n3msub * n1msub, n2msub, 0);
 
     for (k = o->k_indexS; k <= n3msub; k++)
         for (j = 1; j <= n2msub; j++)

@@ -88,6 +88,14 @@ void orc_step(orc *o, double dt);
 /* PaScaL_TDMA (single rank): arrays (n_sys,n_row), system index fastest */
 void orc_ptdma_many_solve(double *A, double *B, double *C, double *D, int n_sys, int n_row);
 void orc_ptdma_many_solve_cycle(double *A, double *B, double *C, double *D, int n_sys, int n_row);
+/* the same solve with the rows of each system split over nprocs ranks
+   (para_range), reduced systems of 2*nprocs rows; nprocs=1 is the above */
+/* parts per line of the split line solves, and the switch that selects
+   them for the step's sweeps (default 0: single-rank solves) */
+int orc_tdma_parts(int n_row);
+void orc_set_tdma_split(orc *o, int split);
+void orc_ptdma_many_solve_ranks(double *A, double *B, double *C, double *D, int n_sys, int n_row,
+                                int nprocs, int cyclic);
 void orc_tdma_many(double *a, double *b, double *c, double *d, int n1, int n2);
 void orc_tdma_cycl_many(double *a, double *b, double *c, double *d, int n1, int n2);
 void orc_tdma_single(double *a, double *b, double *c, double *d, int n1);

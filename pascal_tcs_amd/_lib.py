@@ -24,7 +24,7 @@ class tcs_params(ctypes.Structure):
     _fields_ = (
         [("n1m", _i32), ("n2m", _i32), ("n3m", _i32),
          ("pbc1", _i32), ("pbc2", _i32), ("pbc3", _i32),
-         ("precision", _i32), ("reserved", _i32)]
+         ("precision", _i32), ("tdma_split", _i32)]
         + [(n, _f64) for n in ("Ct", "Cmu", "Cmt", "Cmp", "DeltaT")]
         + [(f"{c}1{d}", _f64) for c in "abcd" for d in range(6)]
         + [(n, _f64) for n in ("KPP0", "Rho0", "Cp0", "Mu0", "a12pera11", "L1", "L2", "L3", "MaxCFL", "dtMax")]
@@ -78,6 +78,8 @@ SIGNATURES = {
     "ptdma_many_solve_cycle_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _i32, _vp]),
     "ptdma_many_solve_host": (ctypes.c_int, [_dp, _dp, _dp, _dp, _i32, _i32, _i32]),
     "ptdma_axis_solve_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp]),
+    "ptdma_many_solve_part_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _vp]),
+    "ptdma_axis_solve_part_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp]),
 }
 
 _LIB = None

@@ -11,7 +11,12 @@
 // stage 1's last point and km = n3m-1 from the x-stage's last k (:1216-1246).
 #include "tcs_internal.h"
 
+#ifndef TCS_FORM
+#define TCS_FORM exact
+#endif
+
 namespace tcs {
+namespace TCS_FORM {
 
 int sweep_z(tcs_ctx *c, bool cyc);
 int sweep_x(tcs_ctx *c, bool cyc);
@@ -631,15 +636,15 @@ int momentum_solvedV(tcs_ctx *c, double dt) {
     // rows j=1 are not part of the dV systems; sweep only j=2..n2m in z and x
     {
         const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m;
-        TCS_TRY(ptdma_exact_launch(c->am + n1, c->ac + n1, c->ap + n1, c->rhs + n1, c->n1m, 1, c->n2m - 1, n1,
-                                   c->n3m, n12, true, c->stream));
+        TCS_TRY(ptdma_sweep(c, c->am + n1, c->ac + n1, c->ap + n1, c->rhs + n1, c->n1m, 1, c->n2m - 1, n1,
+                                   c->n3m, n12, true));
     }
     TCS_LAUNCH(c->stream, "dV_x", k_dV_x, grd, blk, 0, g, c->phys, dt, f[TCS_U], f[TCS_INVRHO], c->stale, c->am,
                        c->ac, c->ap);
     {
         const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m;
-        TCS_TRY(ptdma_exact_launch(c->am + n1, c->ac + n1, c->ap + n1, c->rhs + n1, c->n2m - 1, n1, c->n3m, n12,
-                                   c->n1m, 1, true, c->stream));
+        TCS_TRY(ptdma_sweep(c, c->am + n1, c->ac + n1, c->ap + n1, c->rhs + n1, c->n2m - 1, n1, c->n3m, n12,
+                                   c->n1m, 1, true));
     }
     TCS_LAUNCH(c->stream, "dV_y", k_dV_y, grd, blk, 0, g, c->phys, dt, f[TCS_V], f[TCS_MU], f[TCS_INVRHO], c->am,
                        c->ac, c->ap);
@@ -689,4 +694,5 @@ int momentum_pseudoupdate(tcs_ctx *c) {
     return TCS_OK;
 }
 
+}  // namespace TCS_FORM
 }  // namespace tcs

@@ -20,7 +20,12 @@
 // LDS with the butterfly outputs staged in registers across barriers.
 #include "tcs_internal.h"
 
+#ifndef TCS_FORM
+#define TCS_FORM exact
+#endif
+
 namespace tcs {
+namespace TCS_FORM {
 
 constexpr int FFT_THREADS = 256;
 constexpr int FFT_MAXB = 8;  // butterflies per thread per stage
@@ -448,4 +453,5 @@ int pressure_projection(tcs_ctx *c, double dt) {
     return TCS_OK;
 }
 
+}  // namespace TCS_FORM
 }  // namespace tcs

@@ -200,7 +200,8 @@ int post_cfl(tcs_ctx *c, double *out) { return post_max<1>(c, out); }
 
 // wall Nusselt numbers from the wall rows the solver sees (T(:,0,:) and
 // T(:,n2sub,:)): pairwise sums over i per k, then over k
-__global__ void k_nusselt_lines(Grid g, const double *__restrict__ T, double *__restrict__ lines) {
+__global__ void k_nusselt_lines(Grid g, const double *__restrict__ T, const double *__restrict__ wbt,
+                                const double *__restrict__ wup, double *__restrict__ lines) {
     extern __shared__ double buf[];
     const int k = blockIdx.x + 1;
     const int side = blockIdx.y;
@@ -210,8 +211,8 @@ __global__ void k_nusselt_lines(Grid g, const double *__restrict__ T, double *__
         double v = 0.0;
         if (t < g.n1m) {
             const int i = t + 1;
-            if (side == 0) v = (T[g.ix(i, 0, k)] - T[g.ix(i, 1, k)]) / g.dmx2[1];
-            else v = (T[g.ix(i, g.n2m, k)] - T[g.ix(i, g.n2m + 1, k)]) / g.dmx2[g.n2m + 1];
+            if (side == 0) v = (wbt[g.b2(i, k)] - T[g.ix(i, 1, k)]) / g.dmx2[1];
+            else v = (T[g.ix(i, g.n2m, k)] - wup[g.b2(i, k)]) / g.dmx2[g.n2m + 1];
         }
         buf[t] = v;
     }
@@ -245,7 +246,7 @@ int nusselt(tcs_ctx *c, double Tup, double Tbt, double *nb, double *nt) {
     while (P1 < c->n1m) P1 <<= 1;
     while (P3 < c->n3m) P3 <<= 1;
     TCS_LAUNCH(c->stream, "nusselt_lines", k_nusselt_lines, dim3(c->n3m, 2), dim3(256), P1 * sizeof(double), c->grid(),
-               c->f[TCS_T], c->red);
+               c->f[TCS_T], c->wall[TCS_TBCBT], c->wall[TCS_TBCUP], c->red);
     TCS_LAUNCH(c->stream, "nusselt_total", k_nusselt_total, dim3(1), dim3(256), P3 * sizeof(double), c->n1m, c->n3m,
                        c->prm.L2, Tbt - Tup, c->red, c->red + 2 * c->n3m);
     TCS_HIP(hipMemcpyAsync(c->h_pinned, c->red + 2 * c->n3m, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -416,8 +417,13 @@ extern "C" int tcs_init(const tcs_params *prm, const double *dx1, const double *
         set_error("tcs_init: n1m and n3m must be powers of two in [4, 4096] and n2m >= 3");
         return TCS_E_ARG;
     }
-    if (prm->precision != 0) {
-        set_error("tcs_init: only the plain FP64 form (precision = 0) is built");
+    if (prm->precision != 0 && prm->precision != 1) {
+        set_error("tcs_init: precision must be 0 (plain FP64, reference operation order) or 1 (plain FP64, "
+                  "contracted/reciprocal arithmetic)");
+        return TCS_E_ARG;
+    }
+    if (prm->tdma_split != 0 && prm->tdma_split != 1) {
+        set_error("tcs_init: tdma_split must be 0 (single-rank line solves) or 1 (split lines)");
         return TCS_E_ARG;
     }
     int ndev = 0;
@@ -437,6 +443,7 @@ extern "C" int tcs_init(const tcs_params *prm, const double *dx1, const double *
     p.KPP0 = prm->KPP0; p.Rho0 = prm->Rho0; p.Cp0 = prm->Cp0; p.Mu0 = prm->Mu0; p.a12pera11 = prm->a12pera11;
 
     c->n1m = prm->n1m; c->n2m = prm->n2m; c->n3m = prm->n3m;
+    c->tdma_split = prm->tdma_split;
     c->N1 = c->n1m + 2; c->N2 = c->n2m + 2; c->N3 = c->n3m + 2;
     c->nfull = (size_t)c->N1 * c->N2 * c->N3;
     c->nint = (size_t)c->n1m * c->n2m * c->n3m;
@@ -631,20 +638,23 @@ extern "C" int tcs_momentum_boundary(tcs_ctx *c, double Uup, double Ubt, double 
 
 #define STAGE(label) ProfStage _stage(c, label)
 
+// the stage routine of the context's arithmetic form (precision 0 / 1)
+#define FORM(c, fn, ...) ((c)->prm.precision == 1 ? ::tcs::fast::fn((c), ##__VA_ARGS__) : ::tcs::exact::fn((c), ##__VA_ARGS__))
+
 // stage labels follow the reference's timer labels (src/main.f90:55-72)
-extern "C" int tcs_thermal_coeffi(tcs_ctx *c) { CTX_GUARD(c); STAGE("thermal coeff"); return thermal_coeffi(c); }
-extern "C" int tcs_thermal_solver(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("thermal solve"); return thermal_solver(c, dt); }
-extern "C" int tcs_momentum_coeffi(tcs_ctx *c) { CTX_GUARD(c); STAGE("momentum coeff"); return momentum_coeffi(c); }
-extern "C" int tcs_momentum_solvedU(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("momentum solvedU"); return momentum_solvedU(c, dt); }
-extern "C" int tcs_momentum_solvedV(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("momentum solvedV"); return momentum_solvedV(c, dt); }
-extern "C" int tcs_momentum_solvedW(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("momentum solvedW"); return momentum_solvedW(c, dt); }
-extern "C" int tcs_momentum_blockLdV(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("momentum blockLdV"); return momentum_blockLdV(c, dt); }
-extern "C" int tcs_momentum_blockLdU(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("momentum blockLdU"); return momentum_blockLdU(c, dt); }
-extern "C" int tcs_momentum_pseudoupdateUVW(tcs_ctx *c) { CTX_GUARD(c); STAGE("momentum updatePseudoUVW"); return momentum_pseudoupdate(c); }
+extern "C" int tcs_thermal_coeffi(tcs_ctx *c) { CTX_GUARD(c); STAGE("thermal coeff"); return FORM(c, thermal_coeffi); }
+extern "C" int tcs_thermal_solver(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("thermal solve"); return FORM(c, thermal_solver, dt); }
+extern "C" int tcs_momentum_coeffi(tcs_ctx *c) { CTX_GUARD(c); STAGE("momentum coeff"); return FORM(c, momentum_coeffi); }
+extern "C" int tcs_momentum_solvedU(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("momentum solvedU"); return FORM(c, momentum_solvedU, dt); }
+extern "C" int tcs_momentum_solvedV(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("momentum solvedV"); return FORM(c, momentum_solvedV, dt); }
+extern "C" int tcs_momentum_solvedW(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("momentum solvedW"); return FORM(c, momentum_solvedW, dt); }
+extern "C" int tcs_momentum_blockLdV(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("momentum blockLdV"); return FORM(c, momentum_blockLdV, dt); }
+extern "C" int tcs_momentum_blockLdU(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("momentum blockLdU"); return FORM(c, momentum_blockLdU, dt); }
+extern "C" int tcs_momentum_pseudoupdateUVW(tcs_ctx *c) { CTX_GUARD(c); STAGE("momentum updatePseudoUVW"); return FORM(c, momentum_pseudoupdate); }
 extern "C" int tcs_pressure_wave_number(tcs_ctx *c) { CTX_GUARD(c); return upload_tables(c); }
-extern "C" int tcs_pressure_rhs(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("pressure RHS"); return pressure_rhs(c, dt); }
-extern "C" int tcs_pressure_poisson_fft2(tcs_ctx *c) { CTX_GUARD(c); STAGE("pressure PoissonSolve"); return pressure_poisson(c); }
-extern "C" int tcs_pressure_projection(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("pressure projection"); return pressure_projection(c, dt); }
+extern "C" int tcs_pressure_rhs(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("pressure RHS"); return FORM(c, pressure_rhs, dt); }
+extern "C" int tcs_pressure_poisson_fft2(tcs_ctx *c) { CTX_GUARD(c); STAGE("pressure PoissonSolve"); return FORM(c, pressure_poisson); }
+extern "C" int tcs_pressure_projection(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("pressure projection"); return FORM(c, pressure_projection, dt); }
 
 extern "C" int tcs_ghostcell_update(tcs_ctx *c, int32_t id) {
     TCS_TRY(check_field(c, id));
@@ -693,26 +703,26 @@ extern "C" int tcs_step(tcs_ctx *c, double dt) {
     CTX_GUARD(c);
     STAGE("step");
     const char *H = "update ghostcell";
-    RUN("thermal solve", thermal_solver(c, dt));
+    RUN("thermal solve", FORM(c, thermal_solver, dt));
     RUN(H, halo(c, c->f[TCS_T]));
-    RUN("momentum coeff", momentum_coeffi(c));
-    RUN("momentum solvedU", momentum_solvedU(c, dt));
+    RUN("momentum coeff", FORM(c, momentum_coeffi));
+    RUN("momentum solvedU", FORM(c, momentum_solvedU, dt));
     RUN(H, halo(c, c->f[TCS_DU]));
-    RUN("momentum solvedV", momentum_solvedV(c, dt));
+    RUN("momentum solvedV", FORM(c, momentum_solvedV, dt));
     RUN(H, halo(c, c->f[TCS_DV]));
-    RUN("momentum solvedW", momentum_solvedW(c, dt));
+    RUN("momentum solvedW", FORM(c, momentum_solvedW, dt));
     RUN(H, halo(c, c->f[TCS_DW]));
-    RUN("momentum blockLdV", momentum_blockLdV(c, dt));
+    RUN("momentum blockLdV", FORM(c, momentum_blockLdV, dt));
     RUN(H, halo(c, c->f[TCS_DV]));
-    RUN("momentum blockLdU", momentum_blockLdU(c, dt));
-    RUN("momentum updatePseudoUVW", momentum_pseudoupdate(c));
+    RUN("momentum blockLdU", FORM(c, momentum_blockLdU, dt));
+    RUN("momentum updatePseudoUVW", FORM(c, momentum_pseudoupdate));
     RUN(H, halo(c, c->f[TCS_U]));
     RUN(H, halo(c, c->f[TCS_V]));
     RUN(H, halo(c, c->f[TCS_W]));
-    RUN("pressure RHS", pressure_rhs(c, dt));
-    RUN("pressure PoissonSolve", pressure_poisson(c));
+    RUN("pressure RHS", FORM(c, pressure_rhs, dt));
+    RUN("pressure PoissonSolve", FORM(c, pressure_poisson));
     RUN(H, halo(c, c->f[TCS_DP]));
-    RUN("pressure projection", pressure_projection(c, dt));
+    RUN("pressure projection", FORM(c, pressure_projection, dt));
     RUN(H, halo(c, c->f[TCS_U]));
     RUN(H, halo(c, c->f[TCS_V]));
     RUN(H, halo(c, c->f[TCS_W]));

@@ -92,6 +92,7 @@ struct tcs_ctx {
     size_t nfull, nint, nwall, ntab;
     int h1p, h3p;
     int device;
+    int tdma_split;          // tcs_params.tdma_split
     hipStream_t stream;
 
     // host copies of the metrics
@@ -145,20 +146,42 @@ inline dim3 interior_grid(int nx, int ny, int nz) {
 int ptdma_exact_launch(double *A, const double *B, double *C, double *D,
                        int n0, long ss0, int n1, long ss1, int n_row, long rs,
                        bool cyclic, hipStream_t st);
+// The same systems split into parts (PaScaL_TDMA's multi-rank algorithm,
+// one thread per rank); n_row in [2, 512]; strided lines need ss0 = 1.
+bool ptdma_part_plan(int n_row, int &P, int &w1, int &w2);
+int ptdma_part_launch(double *A, const double *B, double *C, double *D, int n0, long ss0, int n1, long ss1,
+                      int n_row, long rs, bool cyclic, hipStream_t st);
+// ADI sweep of a context: split solve when the context asks for it and the
+// line length allows, else the single-rank order.
+inline int ptdma_sweep(const tcs_ctx *c, double *A, const double *B, double *C, double *D, int n0, long ss0,
+                       int n1, long ss1, int n_row, long rs, bool cyclic) {
+    int P, w1, w2;
+    if (c->tdma_split && ptdma_part_plan(n_row, P, w1, w2) && (rs == 1 || ss0 == 1))
+        return ptdma_part_launch(A, B, C, D, n0, ss0, n1, ss1, n_row, rs, cyclic, c->stream);
+    return ptdma_exact_launch(A, B, C, D, n0, ss0, n1, ss1, n_row, rs, cyclic, c->stream);
+}
 
-// stage entry points implemented in the kernel translation units
-int thermal_solver(tcs_ctx *c, double dt);
-int thermal_coeffi(tcs_ctx *c);
-int momentum_coeffi(tcs_ctx *c);
-int momentum_solvedU(tcs_ctx *c, double dt);
-int momentum_solvedV(tcs_ctx *c, double dt);
-int momentum_solvedW(tcs_ctx *c, double dt);
-int momentum_blockLdV(tcs_ctx *c, double dt);
-int momentum_blockLdU(tcs_ctx *c, double dt);
-int momentum_pseudoupdate(tcs_ctx *c);
-int pressure_rhs(tcs_ctx *c, double dt);
-int pressure_poisson(tcs_ctx *c);
-int pressure_projection(tcs_ctx *c, double dt);
+// Stage entry points of the kernel translation units (thermal, momentum,
+// pressure).  Each of them is compiled twice: into namespace `exact` with FP
+// contraction off (the reference's operation order, precision 0) and into
+// namespace `fast` with contraction and reciprocal arithmetic allowed
+// (precision 1).  TCS_FORM names the namespace a translation unit builds.
+#define TCS_STAGE_DECLS                                   \
+    int thermal_solver(tcs_ctx *c, double dt);            \
+    int thermal_coeffi(tcs_ctx *c);                       \
+    int momentum_coeffi(tcs_ctx *c);                      \
+    int momentum_solvedU(tcs_ctx *c, double dt);          \
+    int momentum_solvedV(tcs_ctx *c, double dt);          \
+    int momentum_solvedW(tcs_ctx *c, double dt);          \
+    int momentum_blockLdV(tcs_ctx *c, double dt);         \
+    int momentum_blockLdU(tcs_ctx *c, double dt);         \
+    int momentum_pseudoupdate(tcs_ctx *c);                \
+    int pressure_rhs(tcs_ctx *c, double dt);              \
+    int pressure_poisson(tcs_ctx *c);                     \
+    int pressure_projection(tcs_ctx *c, double dt);
+namespace exact { TCS_STAGE_DECLS }
+namespace fast { TCS_STAGE_DECLS }
+#undef TCS_STAGE_DECLS
 int halo(tcs_ctx *c, double *f);
 int post_div(tcs_ctx *c, double *out);
 int post_cfl(tcs_ctx *c, double *maxuovx);

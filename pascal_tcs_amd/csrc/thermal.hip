@@ -11,7 +11,12 @@
 // stage's axis with strided system access (no transposes).
 #include "tcs_internal.h"
 
+#ifndef TCS_FORM
+#define TCS_FORM exact
+#endif
+
 namespace tcs {
+namespace TCS_FORM {
 
 __device__ __forceinline__ void th_kpp(const Phys &p, double T, double &kpp, double &dkpp) {
     const double tmp1 = p.DeltaT * T;
@@ -328,18 +333,18 @@ __global__ void k_add_interior(Grid g, double *__restrict__ F, const double *__r
 // sweeps over the interior (i,j,k) layout
 int sweep_z(tcs_ctx *c, bool cyc) {
     const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m;
-    return ptdma_exact_launch(c->am, c->ac, c->ap, c->rhs, c->n1m, 1, c->n2m, n1, c->n3m, n12, cyc, c->stream);
+    return ptdma_sweep(c, c->am, c->ac, c->ap, c->rhs, c->n1m, 1, c->n2m, n1, c->n3m, n12, cyc);
 }
 int sweep_x(tcs_ctx *c, bool cyc) {
     const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m;
-    return ptdma_exact_launch(c->am, c->ac, c->ap, c->rhs, c->n2m, n1, c->n3m, n12, c->n1m, 1, cyc, c->stream);
+    return ptdma_sweep(c, c->am, c->ac, c->ap, c->rhs, c->n2m, n1, c->n3m, n12, c->n1m, 1, cyc);
 }
 int sweep_y(tcs_ctx *c, int j0, int nrows) {
     // rows j = j0..j0+nrows-1 (1-based)
     const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m;
     const long off = (long)(j0 - 1) * n1;
-    return ptdma_exact_launch(c->am + off, c->ac + off, c->ap + off, c->rhs + off, c->n1m, 1, c->n3m, n12,
-                              nrows, n1, false, c->stream);
+    return ptdma_sweep(c, c->am + off, c->ac + off, c->ap + off, c->rhs + off, c->n1m, 1, c->n3m, n12,
+                              nrows, n1, false);
 }
 
 int thermal_coeffi(tcs_ctx *c) {
@@ -365,4 +370,5 @@ int thermal_solver(tcs_ctx *c, double dt) {
     return TCS_OK;
 }
 
+}  // namespace TCS_FORM
 }  // namespace tcs

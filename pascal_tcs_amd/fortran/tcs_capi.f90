@@ -11,7 +11,7 @@ module tcs_capi
     implicit none
 
     type, bind(C) :: tcs_params_t
-        integer(c_int32_t) :: n1m, n2m, n3m, pbc1, pbc2, pbc3, precision, reserved
+        integer(c_int32_t) :: n1m, n2m, n3m, pbc1, pbc2, pbc3, precision, tdma_split
         real(c_double) :: Ct, Cmu, Cmt, Cmp, DeltaT
         real(c_double) :: a10, a11, a12, a13, a14, a15
         real(c_double) :: b10, b11, b12, b13, b14, b15

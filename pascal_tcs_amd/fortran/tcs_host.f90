@@ -13,7 +13,7 @@ contains
         if (c_associated(tcs_ctx)) return
         p%n1m = n1m; p%n2m = n2m; p%n3m = n3m
         p%pbc1 = merge(1, 0, pbc1); p%pbc2 = merge(1, 0, pbc2); p%pbc3 = merge(1, 0, pbc3)
-        p%precision = 0; p%reserved = 0
+        p%precision = 0; p%tdma_split = 0
         p%Ct = Ct; p%Cmu = Cmu; p%Cmt = Cmt; p%Cmp = Cmp; p%DeltaT = DeltaT
         p%a10 = a10; p%a11 = a11; p%a12 = a12; p%a13 = a13; p%a14 = a14; p%a15 = a15
         p%b10 = b10; p%b11 = b11; p%b12 = b12; p%b13 = b13; p%b14 = b14; p%b15 = b15

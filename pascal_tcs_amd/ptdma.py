@@ -73,3 +73,35 @@ def axis_solve(A, B, C, D, axis, cyclic=False):
     stream = torch.cuda.current_stream().cuda_stream
     _lib.check(lib.ptdma_axis_solve_dev(A.data_ptr(), B.data_ptr(), C.data_ptr(), D.data_ptr(),
                                         n1, n2, n3, int(axis), int(cyclic), stream), "ptdma_axis_solve")
+
+
+def _dev_check(*ts):
+    import torch
+    for t in ts:
+        if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+            raise ValueError("expected contiguous float64 CUDA tensors")
+
+
+def many_solve_split(A, B, C, D, cyclic=False):
+    """PaScaL_TDMA_many_solve[_cycle] on (n_row, n_sys) CUDA tensors with every
+    system's rows split over ceil(n_row/16) ranks (n_row <= 512): the
+    reference's multi-rank operation order, one GPU thread per rank.  D
+    returns the solution; A, B, C are not modified."""
+    import torch
+    _dev_check(A, B, C, D)
+    lib = _lib.load()
+    n_row, n_sys = A.shape
+    stream = torch.cuda.current_stream().cuda_stream
+    _lib.check(lib.ptdma_many_solve_part_dev(A.data_ptr(), B.data_ptr(), C.data_ptr(), D.data_ptr(),
+                                             n_sys, n_row, int(cyclic), stream), "ptdma_many_solve_part_dev")
+
+
+def axis_solve_split(A, B, C, D, axis, cyclic=False):
+    """axis_solve with split lines (see many_solve_split)."""
+    import torch
+    _dev_check(A, B, C, D)
+    lib = _lib.load()
+    n3, n2, n1 = A.shape
+    stream = torch.cuda.current_stream().cuda_stream
+    _lib.check(lib.ptdma_axis_solve_part_dev(A.data_ptr(), B.data_ptr(), C.data_ptr(), D.data_ptr(),
+                                             n1, n2, n3, int(axis), int(cyclic), stream), "ptdma_axis_solve_part")

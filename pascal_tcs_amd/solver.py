@@ -34,7 +34,14 @@ def mesh_axis(nm, L, uniform, gamma, periodic):
 
 
 class TCSSolver:
-    def __init__(self, cfg: TCSConfig):
+    """tdma_split=1 solves every ADI line split into ceil(n/16) parts, as
+    PaScaL_TDMA solves lines split over that many ranks (HBM-minimal
+    kernels); 0 keeps the single-rank order of a 1x1x1 reference run.
+    precision=0 keeps the reference's operation order in the stencils;
+    1 lets them contract to FMAs and use reciprocals (plain FP64 within
+    1e-12 per step of the reference order)."""
+
+    def __init__(self, cfg: TCSConfig, tdma_split=0, precision=0):
         self.cfg = cfg
         lib = _lib.load()
         self.lib = lib
@@ -44,7 +51,8 @@ class TCSSolver:
         p = _lib.tcs_params()
         p.n1m, p.n2m, p.n3m = cfg.n1m, cfg.n2m, cfg.n3m
         p.pbc1, p.pbc2, p.pbc3 = int(cfg.pbc1), int(cfg.pbc2), int(cfg.pbc3)
-        p.precision = 0
+        p.precision = int(precision)
+        p.tdma_split = int(tdma_split)
         for k, v in cfg.constants().items():
             setattr(p, k, float(v))
         self.params = p

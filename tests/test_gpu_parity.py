@@ -10,11 +10,11 @@ pytestmark = pytest.mark.gpu
 SHAPES = [(16, 12, 8), (32, 9, 16), (8, 5, 4)]
 
 
-def _pair(shape, props="water", seed=7):
+def _pair(shape, props="water", seed=7, split=0):
     from pascal_tcs_amd.solver import TCSSolver
     cfg = small_config(*shape, props=props)
-    o = make_oracle(cfg)
-    s = TCSSolver(cfg)
+    o = make_oracle(cfg, tdma_split=split)
+    s = TCSSolver(cfg, tdma_split=split)
     st, walls = random_state(cfg, seed=seed)
     load_oracle(o, st, walls)
     load_gpu(s, st, walls)
@@ -120,13 +120,13 @@ def test_post_div_cfl(shape):
 
 
 @pytest.mark.parametrize("shape", [(16, 12, 8), (32, 20, 16)])
-def test_full_step_seeded(shape):
+def test_full_step_seeded(shape, split=0):
     """Three steps of the whole time step from a seeded conduction state."""
     from pascal_tcs_amd.solver import TCSSolver
     cfg = small_config(*shape)
-    s = TCSSolver(cfg)
+    s = TCSSolver(cfg, tdma_split=split)
     s.setup_seeded(amp=0.05, seed=1234)
-    o = make_oracle(cfg)
+    o = make_oracle(cfg, tdma_split=split)
     for n in FIELDS:
         o.f[n][...] = s.get(n)
     for w in ("TBCup", "TBCbt", "UBCup", "UBCbt", "VBCup", "VBCbt", "WBCup", "WBCbt"):
@@ -135,6 +135,34 @@ def test_full_step_seeded(shape):
         o.step(cfg.dtStart)
         s.step(cfg.dtStart)
         _compare(o, s, ["T", "U", "V", "W", "P", "dPhat", "dP0"])
+
+
+SPLIT_SHAPES = [(64, 40, 32), (32, 100, 16), (16, 12, 8)]
+
+
+@pytest.mark.parametrize("shape", SPLIT_SHAPES)
+def test_full_step_seeded_split_lines(shape):
+    """The step with every ADI line split into ceil(n/16) parts (x lines of
+    64 -> 4 parts, y lines of 100/99 -> 7 parts, ...), bit for bit against
+    the oracle run with the same split."""
+    test_full_step_seeded(shape, split=1)
+
+
+@pytest.mark.parametrize("shape", SPLIT_SHAPES[:2])
+@pytest.mark.parametrize("routine", ["thermal", "solvedU", "solvedV", "solvedW"])
+def test_routines_split_lines(shape, routine):
+    cfg, o, s = _pair(shape, split=1)
+    dt = cfg.dtStart
+    if routine == "thermal":
+        o.thermal_coeffi()
+        o.thermal_solver(dt)
+        s.mpi_thermal_solver(dt)
+    else:
+        o.momentum_coeffi()
+        s.mpi_momentum_coeffi()
+        getattr(o, "momentum_" + routine)(dt)
+        getattr(s, "mpi_momentum_" + routine)(dt)
+    _compare(o, s)
 
 
 def test_nusselt_conduction_profile_is_one():
@@ -147,8 +175,11 @@ def test_nusselt_conduction_profile_is_one():
     yc = 0.5 * (x2[1:-1] + x2[2:])
     T = np.zeros(s.shape)
     T[:, 1:-1, :] = (cfg.Tbt + (cfg.Tup - cfg.Tbt) * yc / cfg.L2)[None, :, None]
-    T[:, 0, :] = cfg.Tbt
-    T[:, -1, :] = cfg.Tup
     s.set("T", T)
+    s.set_wall("TBCbt", np.full(s.wall_shape, cfg.Tbt))
+    s.set_wall("TBCup", np.full(s.wall_shape, cfg.Tup))
     nb, nt = s.nusselt()
     assert abs(nb - 1.0) < 1e-12 and abs(nt - 1.0) < 1e-12
+    # the thermal boundary routine fills the wall arrays the same way
+    s.mpi_thermal_boundary()
+    assert s.nusselt() == (nb, nt)

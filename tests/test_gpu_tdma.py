@@ -118,3 +118,93 @@ def test_axis_solve_512_cube_sampled(axis):
         sys_ = [t[tuple(sl)].cpu().numpy().reshape(n, 1).copy() for t in (a, b, c, d)]
         O.ptdma_many_solve(*sys_, cyclic=False)
         assert np.array_equal(D[tuple(sl)].cpu().numpy(), sys_[3][:, 0])
+
+
+# ---- split lines: PaScaL_TDMA's multi-rank order, one thread per rank -------
+
+@pytest.mark.parametrize("n_sys,n_row", [(1, 2), (5, 3), (37, 16), (300, 17), (4096, 64), (70001, 33),
+                                         (257, 511), (64, 512), (17, 100)])
+@pytest.mark.parametrize("cyclic", [False, True])
+def test_split_many_solve_device(n_sys, n_row, cyclic):
+    import torch
+    from pascal_tcs_amd import ptdma
+    rng = np.random.default_rng(n_sys * 13 + n_row)
+    a, b, c, d = dominant(rng, (n_row, n_sys))
+    A, B, C, D = _torch((a, b, c, d))
+    ptdma.many_solve_split(A, B, C, D, cyclic=cyclic)
+    torch.cuda.synchronize()
+    ra, rb, rc, rd = (x.copy() for x in (a, b, c, d))
+    O.ptdma_many_solve_ranks(ra, rb, rc, rd, nprocs=O.tdma_parts(n_row), cyclic=cyclic)
+    assert np.array_equal(D.cpu().numpy(), rd)
+    for t, x in zip((A, B, C), (a, b, c)):
+        assert np.array_equal(t.cpu().numpy(), x)
+
+
+@pytest.mark.parametrize("shape", [(8, 5, 3), (64, 16, 20), (37, 70, 9), (512, 8, 8), (16, 3, 130), (100, 33, 17)])
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("cyclic", [False, True])
+def test_split_axis_solve(shape, axis, cyclic):
+    import torch
+    from pascal_tcs_amd import ptdma
+    n1, n2, n3 = shape
+    rng = np.random.default_rng(n1 + 10 * n2 + 100 * n3 + axis + 7)
+    a, b, c, d = dominant(rng, (n3, n2, n1))
+    A, B, C, D = _torch((a, b, c, d))
+    ptdma.axis_solve_split(A, B, C, D, axis, cyclic)
+    torch.cuda.synchronize()
+    perm = {0: (2, 0, 1), 1: (1, 0, 2), 2: (0, 1, 2)}[axis]
+    inv = np.argsort(perm)
+    arrs = [np.ascontiguousarray(x.transpose(perm)) for x in (a, b, c, d)]
+    n_row = arrs[0].shape[0]
+    flat = [x.reshape(n_row, -1).copy() for x in arrs]
+    O.ptdma_many_solve_ranks(*flat, nprocs=O.tdma_parts(n_row), cyclic=cyclic)
+    exp = flat[3].reshape(arrs[0].shape).transpose(inv)
+    g = D.cpu().numpy()
+    assert np.array_equal(g, exp), f"D differs: {np.abs(g - exp).max()}"
+
+
+def test_split_rejects_long_lines():
+    import torch
+    from pascal_tcs_amd import ptdma, _lib
+    a = torch.zeros((513, 4), dtype=torch.float64, device="cuda")
+    with pytest.raises(_lib.TCSError):
+        ptdma.many_solve_split(a, a.clone() + 1.0, a.clone(), a.clone())
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("cyclic", [False, True])
+def test_split_axis_solve_512_cube(axis, cyclic):
+    """512^3, the microbenchmark shape: residual of every system below 1e-12
+    (cyclic closure included) and bit-exactness on sampled systems."""
+    import torch
+    from pascal_tcs_amd import ptdma
+    n = 512
+    g = torch.Generator(device="cuda").manual_seed(21 + axis + 3 * cyclic)
+    a = -torch.rand((n, n, n), dtype=torch.float64, device="cuda", generator=g)
+    c = -torch.rand((n, n, n), dtype=torch.float64, device="cuda", generator=g)
+    b = a.abs() + c.abs() + 1.0 + torch.rand((n, n, n), dtype=torch.float64, device="cuda", generator=g)
+    d = 2.0 * torch.rand((n, n, n), dtype=torch.float64, device="cuda", generator=g) - 1.0
+    D = d.clone()
+    ptdma.axis_solve_split(a, b, c, D, axis, cyclic=cyclic)
+    torch.cuda.synchronize()
+    dim = 2 - axis
+    xm = torch.roll(D, 1, dims=dim)
+    xp = torch.roll(D, -1, dims=dim)
+    if cyclic:
+        r = b * D + a * xm + c * xp - d
+    else:
+        idx = torch.arange(n, device="cuda")
+        shape = [1, 1, 1]
+        shape[dim] = n
+        first = (idx == 0).view(shape)
+        last = (idx == n - 1).view(shape)
+        r = b * D + torch.where(first, torch.zeros_like(D), a * xm) + torch.where(last, torch.zeros_like(D), c * xp) - d
+    assert r.abs().max().item() < 1e-12
+    del xm, xp, r
+    rng = np.random.default_rng(axis + 10 * cyclic)
+    for p, q in rng.integers(0, n, size=(32, 2)):
+        sl = [int(p), int(q)]
+        sl.insert(dim, slice(None))
+        sys_ = [t[tuple(sl)].cpu().numpy().reshape(n, 1).copy() for t in (a, b, c, d)]
+        O.ptdma_many_solve_ranks(*sys_, nprocs=O.tdma_parts(n), cyclic=cyclic)
+        assert np.array_equal(D[tuple(sl)].cpu().numpy(), sys_[3][:, 0])

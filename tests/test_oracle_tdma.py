@@ -139,3 +139,82 @@ def test_dominant_underflow_is_finite():
         A, B, C, D = (x.copy() for x in (a, b, c, d))
         O.ptdma_many_solve(A, B, C, D, cyclic=cyc)
         assert np.all(np.isfinite(D)) and np.all(np.isfinite(A))
+
+
+def _dense(a, b, c, s, cyclic):
+    n = a.shape[0]
+    M = np.diag(b[:, s]) + np.diag(c[:-1, s], 1) + np.diag(a[1:, s], -1)
+    if cyclic:
+        M[0, -1] = a[0, s]
+        M[-1, 0] = c[-1, s]
+    return M
+
+
+@pytest.mark.parametrize("cyclic", [False, True])
+@pytest.mark.parametrize("n_row", [2, 5, 64, 511])
+def test_ranks_one_is_the_single_rank_solve(cyclic, n_row):
+    rng = np.random.default_rng(n_row + 7 * cyclic)
+    a, b, c, d = dominant(rng, n_row, 19)
+    one = [x.copy() for x in (a, b, c, d)]
+    ranks = [x.copy() for x in (a, b, c, d)]
+    O.ptdma_many_solve(*one, cyclic=cyclic)
+    O.ptdma_many_solve_ranks(*ranks, nprocs=1, cyclic=cyclic)
+    for u, v in zip(one, ranks):
+        assert np.array_equal(u, v)
+
+
+@pytest.mark.parametrize("cyclic", [False, True])
+@pytest.mark.parametrize("n_row,nprocs", [(6, 2), (13, 4), (18, 6), (64, 4), (100, 7), (511, 32), (512, 32),
+                                          (512, 64), (50, 16)])
+def test_ranks_solve_is_exact_solution(cyclic, n_row, nprocs):
+    """The partitioned solve (pascal_tdma.f90 with nprocs ranks along the
+    rows) solves the system exactly for every split with at least three rows
+    per rank, including ragged para_range splits; with nprocs >= 2 the
+    cyclic reduced system has >= 4 rows, so the cyclic solve is exact at any
+    size.  (A rank with two rows is not: its row-1 fix-up,
+    pascal_tdma.f90:531-536, then couples row 1 to the next rank's first row
+    while the reduced system takes it as the rank's own last row.)"""
+    rng = np.random.default_rng(n_row * 31 + nprocs)
+    ns = 9
+    a, b, c, d = dominant(rng, n_row, ns)
+    if not cyclic:
+        a[0] = 0.0
+        c[-1] = 0.0
+    A, B, C, D = (x.copy() for x in (a, b, c, d))
+    O.ptdma_many_solve_ranks(A, B, C, D, nprocs=nprocs, cyclic=cyclic)
+    assert np.array_equal(B, b)
+    for s in range(ns):
+        x = np.linalg.solve(_dense(a, b, c, s, cyclic), d[:, s])
+        assert np.allclose(D[:, s], x, rtol=1e-13, atol=1e-14)
+
+
+def test_ranks_with_two_rows_are_not_exact():
+    rng = np.random.default_rng(8)
+    a, b, c, d = dominant(rng, 8, 3)
+    a[0] = 0.0
+    c[-1] = 0.0
+    A, B, C, D = (x.copy() for x in (a, b, c, d))
+    O.ptdma_many_solve_ranks(A, B, C, D, nprocs=4)
+    err = max(np.abs(_dense(a, b, c, s, False) @ D[:, s] - d[:, s]).max() for s in range(3))
+    assert err > 1e-6
+
+
+def test_ranks_reduced_rows_are_rank_ordered():
+    """A wrong reduced-row order (e.g. row-major over ranks instead of two
+    rows per rank) would couple the wrong rows: a system whose ranks are
+    decoupled (zero coupling at the rank edges) must give each rank's own
+    block solution."""
+    rng = np.random.default_rng(3)
+    n_row, nprocs, ns = 12, 3, 4
+    a, b, c, d = dominant(rng, n_row, ns)
+    for r0 in (0, 4, 8):
+        a[r0] = 0.0
+        c[r0 + 3] = 0.0
+    A, B, C, D = (x.copy() for x in (a, b, c, d))
+    O.ptdma_many_solve_ranks(A, B, C, D, nprocs=nprocs)
+    for s in range(ns):
+        for r0 in (0, 4, 8):
+            blk = slice(r0, r0 + 4)
+            aa, bb, cc = a[blk], b[blk], c[blk]
+            x = np.linalg.solve(_dense(aa, bb, cc, s, False), d[blk, s])
+            assert np.allclose(D[blk, s], x, rtol=1e-13, atol=1e-14)
