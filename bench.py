@@ -77,7 +77,7 @@ def run_gpu(args):
     from pascal_tcs_amd.solver import TCSSolver
 
     cfg = workload_config(args.n, deltat=args.deltat)
-    s = TCSSolver(cfg, tdma_split=args.tdma_split)
+    s = TCSSolver(cfg, tdma_split=args.tdma_split, precision=args.precision)
     s.setup_seeded(amp=0.01, seed=20240512)
     T = s.get("T")
     T[:, -1, :] = cfg.Tup  # top wall row, as the top y-rank of a multi-rank run sets it
@@ -180,6 +180,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dt", type=float, default=1e-3, help="fixed time step (<= 0: the reference's CFL rule)")
     ap.add_argument("--deltat", type=float, default=0.2, help="DeltaT of the property polynomials")
+    ap.add_argument("--precision", type=int, default=0, choices=(0, 1),
+                    help="0: stencils in the reference's operation order; 1: FMA contraction and reciprocals "
+                         "(within 1e-12 per step of the reference order)")
     ap.add_argument("--tdma-split", type=int, default=1, choices=(0, 1),
                     help="1: ADI lines split into ceil(n/16) parts (PaScaL_TDMA's multi-rank order, "
                          "within 1e-15 of the single-rank order); 0: single-rank order")
@@ -213,7 +216,8 @@ def main():
                                f"Ra=1e8, Pr=0.71, water properties DeltaT={cfg.DeltaT:g}, T noise 0.01, "
                                + (f"fixed dt {args.dt:g} (CFL evaluated every step)" if args.dt > 0 else "CFL 0.5"),
                    "cells": cells,
-                   "form": "plain FP64, reference operation order" + (
+                   "form": ("plain FP64, reference operation order" if args.precision == 0 else
+                            "plain FP64, contracted/reciprocal stencil arithmetic") + (
                        "; ADI lines solved split into ceil(n/16) parts as PaScaL_TDMA splits them over ranks"
                        if args.tdma_split else "; single-rank line solves")},
         "roofline": roof,
