@@ -31,6 +31,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -61,6 +63,9 @@ def kernel_bytes(label, cfg):
         "ptdma_part": 40.0 * N * (3 + fv) / 4,
         "ptdma_part_cyclic": 40.0 * N * (3 + fv) / 4,
         "ptdma_part_rows_cyclic": 40.0 * N * (3 + fv) / 4,
+        "ptdma_part_f32": 20.0 * N * (3 + fv) / 4,
+        "ptdma_part_cyclic_f32": 20.0 * N * (3 + fv) / 4,
+        "ptdma_part_rows_cyclic_f32": 20.0 * N * (3 + fv) / 4,
         "thermal_z": 8.0 * N * (4 + 4), "thermal_x": 8.0 * N * (2 + 3), "thermal_y": 8.0 * N * (2 + 3),
         "dU_z": 8.0 * N * (6 + 4), "dV_z": 8.0 * N * (6 + 4) * fv, "dW_z": 8.0 * N * (7 + 4),
         "dU_x": 8.0 * N * (1 + 3), "dU_y": 8.0 * N * (1 + 3), "dV_x": 8.0 * N * (1 + 3) * fv,
@@ -72,12 +77,12 @@ def kernel_bytes(label, cfg):
     return table.get(label)
 
 
-def run_gpu(args):
+def run_gpu(args, precision):
     import torch
     from pascal_tcs_amd.solver import TCSSolver
 
     cfg = workload_config(args.n, deltat=args.deltat)
-    s = TCSSolver(cfg, tdma_split=args.tdma_split, precision=args.precision)
+    s = TCSSolver(cfg, tdma_split=args.tdma_split, precision=precision)
     s.setup_seeded(amp=0.01, seed=20240512)
     T = s.get("T")
     T[:, -1, :] = cfg.Tup  # top wall row, as the top y-rank of a multi-rank run sets it
@@ -115,8 +120,9 @@ def run_gpu(args):
     rep = s.profile_report()
     s.profile(False)
     nu = s.nusselt()
+    fields = {n: s.get(n) for n in ("T", "U", "V", "W", "P")} if args.mixed else None
     s.close()
-    return cfg, sec, rep, state, nu
+    return cfg, sec, rep, state, nu, fields
 
 
 def roofline(cfg, rep, psteps):
@@ -180,19 +186,25 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dt", type=float, default=1e-3, help="fixed time step (<= 0: the reference's CFL rule)")
     ap.add_argument("--deltat", type=float, default=0.2, help="DeltaT of the property polynomials")
-    ap.add_argument("--precision", type=int, default=0, choices=(0, 1),
+    ap.add_argument("--precision", type=int, default=1, choices=(0, 1, 2),
                     help="0: stencils in the reference's operation order; 1: FMA contraction and reciprocals "
-                         "(within 1e-12 per step of the reference order)")
+                         "(within 1e-12 per step of the reference order); 2: mixed form, FP32 ADI scratch and "
+                         "line solves for the increments, FP64 state and pressure")
     ap.add_argument("--tdma-split", type=int, default=1, choices=(0, 1),
-                    help="1: ADI lines split into ceil(n/16) parts (PaScaL_TDMA's multi-rank order, "
-                         "within 1e-15 of the single-rank order); 0: single-rank order")
+                    help="1: ADI lines split into parts as PaScaL_TDMA splits them over ranks, reduced "
+                         "systems by parallel cyclic reduction (within 1e-14 of the single-rank order); "
+                         "0: single-rank order")
+    ap.add_argument("--no-mixed", dest="mixed", action="store_false",
+                    help="skip the mixed form's run (by default it follows the plain run on the same workload)")
     args = ap.parse_args()
+    if args.precision == 2:
+        args.mixed = False
     if args.gpus != 1:
         print(f"bench.py: this build runs the whole domain on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
 
-    cfg, sec, rep, state, nu = run_gpu(args)
+    cfg, sec, rep, state, nu, fields = run_gpu(args, args.precision)
     cells = cfg.n1m * cfg.n2m * cfg.n3m
     value = cells * args.steps / sec / 1e6
     dom, roof = roofline(cfg, rep, args.profile_steps)
@@ -210,15 +222,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision < 2 else "f64 state / f32 ADI",
         "data": "synthetic",
         "config": {"workload": f"plain FP64 MPM-STD step, {cfg.n1m}x{cfg.n2m}x{cfg.n3m}, tanh y (gamma 2.0), "
                                f"Ra=1e8, Pr=0.71, water properties DeltaT={cfg.DeltaT:g}, T noise 0.01, "
                                + (f"fixed dt {args.dt:g} (CFL evaluated every step)" if args.dt > 0 else "CFL 0.5"),
                    "cells": cells,
-                   "form": ("plain FP64, reference operation order" if args.precision == 0 else
-                            "plain FP64, contracted/reciprocal stencil arithmetic") + (
-                       "; ADI lines solved split into ceil(n/16) parts as PaScaL_TDMA splits them over ranks"
+                   "form": {0: "plain FP64, reference operation order",
+                            1: "plain FP64, contracted/reciprocal stencil arithmetic",
+                            2: "mixed: FP32 ADI scratch and line solves, FP64 state/pressure/reductions"}[args.precision] + (
+                       "; ADI lines split into 32 parts per line as PaScaL_TDMA splits them over ranks, "
+                       "reduced systems by parallel cyclic reduction"
                        if args.tdma_split else "; single-rank line solves")},
         "roofline": roof,
         "stage_ms_per_step": stages,
@@ -226,6 +240,23 @@ def main():
         "final": {"dt": state["dt"], "maxDivU": state["div"], "CFL": state["cfl"], "Nu_bottom": nu[0],
                   "Nu_top": nu[1], "maxDivU_per_step": [float(f"{d:.3e}") for d in state["divs"]]},
     }
+    if args.mixed:
+        # the mixed form on the same workload and step count, measured the same
+        # way; its fields after warmup + steps + profile steps are compared with
+        # the plain run's (relative to each field's magnitude)
+        cfg2, sec2, rep2, state2, nu2, fields2 = run_gpu(args, 2)
+        v2 = cells * args.steps / sec2 / 1e6
+        err = {n: float(f"{np.abs(fields2[n] - fields[n]).max() / max(np.abs(fields[n]).max(), 1e-300):.3e}")
+               for n in fields}
+        out["mixed"] = {"value": round(v2, 2), "unit": "Mcell-steps/s", "ms_per_step": round(sec2 / args.steps * 1e3, 3),
+                        "speedup_vs_plain": round(v2 / value, 3),
+                        "form": "FP32 ADI scratch and line solves for the increments; FP64 state, pressure, "
+                                "divergence/CFL/Nusselt reductions",
+                        "rel_error_vs_plain_after_steps": err, "steps_compared": args.warmup + args.steps + args.profile_steps,
+                        "Nu_bottom": nu2[0], "Nu_top": nu2[1],
+                        "kernel_ms_per_step": {k: round(v[0] / args.profile_steps, 3) for k, v in rep2.items()
+                                               if " " not in k and k != "step"}}
+        del fields, fields2
     if not args.no_cpu:
         out["cpu_baseline"] = run_cpu_baseline(args.cpu_seconds, args.deltat)
     else:

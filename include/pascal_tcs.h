@@ -43,8 +43,8 @@ typedef struct tcs_params {
     int32_t precision;            /* 0 = plain FP64 (the only form built) */
     int32_t tdma_split;           /* 0: every ADI line solved as on one rank;
                                      1: lines of up to 512 rows split into
-                                     ceil(n/16) parts, solved as PaScaL_TDMA
-                                     solves lines split over that many ranks */
+                                     parts as PaScaL_TDMA splits them over
+                                     ranks (see ptdma_many_solve_part_dev) */
     double Ct, Cmu, Cmt, Cmp;     /* :139-141 */
     double DeltaT;                /* namelist sim_parameter */
     double a10, a11, a12, a13, a14, a15;  /* density polynomial   :117 */
@@ -202,12 +202,14 @@ int ptdma_axis_solve_dev(double *A, const double *B, double *C, double *D,
                          int32_t n1, int32_t n2, int32_t n3, int32_t axis,
                          int32_t cyclic, void *stream);
 /* PaScaL_TDMA_many_solve[_cycle] with every system's rows split over
-   nprocs = ceil(n_row/16) ranks (para_range, PaScaL_TDMA/src/para_range.f90:
-   26-40): per-rank modified Thomas (pascal_tdma.f90:501-536), the reduced
-   system of 2*nprocs rows in rank order (pascal_tdma.f90:243-266) solved by
-   tdma_many / tdma_cycl_many, per-rank update (:567-576).  The operation
-   order of a run of the reference on nprocs ranks along the solve
-   direction; each thread of the kernel plays one rank.  2 <= n_row <= 512.
+   P ranks, P the power of two >= n_row/16 (rows per rank by para_range,
+   PaScaL_TDMA/src/para_range.f90:26-40): per-rank modified Thomas
+   (pascal_tdma.f90:501-536), the reduced system of 2P rows in rank order
+   (pascal_tdma.f90:243-266), per-rank update (:567-576).  Each rank is one
+   GPU thread.  The reduced system is solved by tdma_many / tdma_cycl_many
+   when P = 1 (n_row <= 16: identical to the single-rank solve) and by
+   parallel cyclic reduction across the P threads when P >= 2 (within
+   rounding of the reference's serial reduced solve).  2 <= n_row <= 512.
    D returns the solution; A and C are read only here (the reference leaves
    its per-rank modified coefficients in them). */
 int ptdma_many_solve_part_dev(const double *A, const double *B, const double *C, double *D,

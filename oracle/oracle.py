@@ -72,6 +72,7 @@ def lib():
         for name in ["orc_tdma_single", "orc_tdma_cycl_single"]:
             getattr(L, name).argtypes = [_dp] * 4 + [ctypes.c_int]
         L.orc_ptdma_many_solve_ranks.argtypes = [_dp] * 4 + [ctypes.c_int] * 4
+        L.orc_ptdma_many_solve_split.argtypes = [_dp] * 4 + [ctypes.c_int] * 3
         L.orc_tdma_parts.argtypes = [ctypes.c_int]
         L.orc_tdma_parts.restype = ctypes.c_int
         L.orc_set_tdma_split.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -131,8 +132,17 @@ def ptdma_many_solve_ranks(A, B, C, D, nprocs, cyclic=False):
 
 
 def tdma_parts(n_row):
-    """Parts per line of the split line solves (ceil(n_row/16) up to 512 rows)."""
+    """Parts per line of the split line solves (the power of two >= n_row/16,
+    up to 512 rows)."""
     return lib().orc_tdma_parts(int(n_row))
+
+
+def ptdma_many_solve_split(A, B, C, D, cyclic=False):
+    """The split solve of the GPU kernels on (n_row, n_sys) arrays, in place:
+    tdma_parts(n_row) ranks, reduced system by parallel cyclic reduction
+    when there are two or more parts (D returns the solution)."""
+    n_row, n_sys = A.shape
+    lib().orc_ptdma_many_solve_split(_p(A), _p(B), _p(C), _p(D), n_sys, n_row, int(bool(cyclic)))
 
 
 def tdma_many(a, b, c, d, cyclic=False):

@@ -264,8 +264,65 @@ static void ptdma_rank_reduce(double *A, double *B, double *C, double *D, int n_
     }
 }
 
+/* Reduced system of one line (N = 2*nprocs rows, unit diagonal at entry)
+   by parallel cyclic reduction, the split solver's replacement of the serial
+   tdma_many / tdma_cycl_many when there are two or more parts.  Step with
+   stride s: every row i eliminates its couplings to rows i-s and i+s using
+   those rows' current values (all rows updated from the same old values):
+     k1 = a_i/b_{i-s}, k2 = c_i/b_{i+s},
+     a_i' = -a_{i-s} k1, c_i' = -c_{i+s} k2,
+     b_i' = b_i - c_{i-s} k1 - a_{i+s} k2, d_i' = d_i - d_{i-s} k1 - d_{i+s} k2.
+   Non-cyclic: rows outside 0..N-1 are identity rows (a=c=d=0, b=1); strides
+   1..N/2, then x_i = d_i/b_i.  Cyclic (N a power of two): indices mod N,
+   strides 1..N/4, after which row i couples only to j = i+N/2:
+     x_i = (d_i b_j - e_i d_j) / (b_i b_j - e_i e_j), e = a + c.
+   The arrays hold the n_sys systems interleaved (system index fastest). */
+static void reduced_pcr(double *ra, double *rb, double *rc, double *rd, int n_sys, int N, int cyclic)
+{
+    double *na = (double *)malloc(sizeof(double) * (size_t)N), *nb = (double *)malloc(sizeof(double) * (size_t)N);
+    double *nc = (double *)malloc(sizeof(double) * (size_t)N), *nd = (double *)malloc(sizeof(double) * (size_t)N);
+    double *a = (double *)malloc(sizeof(double) * (size_t)N), *b = (double *)malloc(sizeof(double) * (size_t)N);
+    double *c = (double *)malloc(sizeof(double) * (size_t)N), *d = (double *)malloc(sizeof(double) * (size_t)N);
+    for (int q = 0; q < n_sys; q++) {
+        for (int i = 0; i < N; i++) {
+            size_t o = (size_t)q + (size_t)n_sys * (size_t)i;
+            a[i] = ra[o]; b[i] = rb[o]; c[i] = rc[o]; d[i] = rd[o];
+        }
+        const int lim = cyclic ? N / 2 : N;
+        for (int s = 1; s < lim; s <<= 1) {
+            for (int i = 0; i < N; i++) {
+                int im = i - s, ip = i + s;
+                double am = 0.0, bm = 1.0, cm = 0.0, dm = 0.0, ap = 0.0, bp = 1.0, cp = 0.0, dp = 0.0;
+                if (cyclic) { im = (im + N) % N; ip = ip % N; }
+                if (cyclic || im >= 0) { am = a[im]; bm = b[im]; cm = c[im]; dm = d[im]; }
+                if (cyclic || ip < N) { ap = a[ip]; bp = b[ip]; cp = c[ip]; dp = d[ip]; }
+                const double k1 = a[i] / bm, k2 = c[i] / bp;
+                na[i] = -am * k1;
+                nc[i] = -cp * k2;
+                nb[i] = b[i] - cm * k1 - ap * k2;
+                nd[i] = d[i] - dm * k1 - dp * k2;
+            }
+            memcpy(a, na, sizeof(double) * (size_t)N); memcpy(b, nb, sizeof(double) * (size_t)N);
+            memcpy(c, nc, sizeof(double) * (size_t)N); memcpy(d, nd, sizeof(double) * (size_t)N);
+        }
+        for (int i = 0; i < N; i++) {
+            double x;
+            if (cyclic) {
+                const int j = (i + N / 2) % N;
+                const double ei = a[i] + c[i], ej = a[j] + c[j];
+                x = (d[i] * b[j] - ei * d[j]) / (b[i] * b[j] - ei * ej);
+            } else {
+                x = d[i] / b[i];
+            }
+            nd[i] = x;
+        }
+        for (int i = 0; i < N; i++) rd[(size_t)q + (size_t)n_sys * (size_t)i] = nd[i];
+    }
+    free(na); free(nb); free(nc); free(nd); free(a); free(b); free(c); free(d);
+}
+
 static void ptdma_many_ranks(double *A, double *B, double *C, double *D,
-                             int n_sys, int n_row, int nprocs, int cyclic)
+                             int n_sys, int n_row, int nprocs, int cyclic, int pcr)
 {
     const int nr = 2 * nprocs;
     int *j0 = (int *)malloc(sizeof(int) * (size_t)nprocs);
@@ -291,7 +348,8 @@ static void ptdma_many_ranks(double *A, double *B, double *C, double *D,
             rd[q0] = P2(D, i, a); rd[q1] = P2(D, i, b);
         }
     }
-    if (cyclic) orc_tdma_cycl_many(ra, rb, rc, rd, n_sys, nr);
+    if (pcr) reduced_pcr(ra, rb, rc, rd, n_sys, nr, cyclic);
+    else if (cyclic) orc_tdma_cycl_many(ra, rb, rc, rd, n_sys, nr);
     else orc_tdma_many(ra, rb, rc, rd, n_sys, nr);
     for (int p = 0; p < nprocs; p++) {
         const int a = j0[p], b = j0[p] + mm[p] - 1;
@@ -311,25 +369,34 @@ static void ptdma_many_ranks(double *A, double *B, double *C, double *D,
 void orc_ptdma_many_solve_ranks(double *A, double *B, double *C, double *D, int n_sys, int n_row,
                                 int nprocs, int cyclic)
 {
-    ptdma_many_ranks(A, B, C, D, n_sys, n_row, nprocs, cyclic);
+    ptdma_many_ranks(A, B, C, D, n_sys, n_row, nprocs, cyclic, 0);
+}
+
+void orc_ptdma_many_solve_split(double *A, double *B, double *C, double *D, int n_sys, int n_row, int cyclic)
+{
+    const int P = orc_tdma_parts(n_row);
+    ptdma_many_ranks(A, B, C, D, n_sys, n_row, P, cyclic, P > 1);
 }
 
 void orc_ptdma_many_solve(double *A, double *B, double *C, double *D, int n_sys, int n_row)
 {
-    ptdma_many_ranks(A, B, C, D, n_sys, n_row, 1, 0);
+    ptdma_many_ranks(A, B, C, D, n_sys, n_row, 1, 0, 0);
 }
 
 void orc_ptdma_many_solve_cycle(double *A, double *B, double *C, double *D, int n_sys, int n_row)
 {
-    ptdma_many_ranks(A, B, C, D, n_sys, n_row, 1, 1);
+    ptdma_many_ranks(A, B, C, D, n_sys, n_row, 1, 1, 0);
 }
 
 /* Parts per line of the split solve: lines of 2..512 rows are split into
-   ceil(n_row/16) parts (at most 16 rows each, at least 8 once there are two
-   parts), longer lines are solved on one rank. */
+   the power of two P >= ceil(n_row/16) parts (at most 16 rows each, at
+   least 8 once there are two parts); longer lines are solved on one rank. */
 int orc_tdma_parts(int n_row)
 {
-    return (n_row >= 2 && n_row <= 512) ? (n_row + 15) / 16 : 1;
+    if (n_row < 2 || n_row > 512) return 1;
+    int P = 1;
+    while (P * 16 < n_row) P <<= 1;
+    return P;
 }
 
 void orc_set_tdma_split(orc *o, int split) { o->tdma_split = split; }
@@ -337,7 +404,8 @@ void orc_set_tdma_split(orc *o, int split) { o->tdma_split = split; }
 /* the step's line solves: single rank, or split as above */
 static void lines_solve(const orc *o, double *A, double *B, double *C, double *D, int n_sys, int n_row, int cyclic)
 {
-    ptdma_many_ranks(A, B, C, D, n_sys, n_row, o->tdma_split ? orc_tdma_parts(n_row) : 1, cyclic);
+    if (o->tdma_split) orc_ptdma_many_solve_split(A, B, C, D, n_sys, n_row, cyclic);
+    else ptdma_many_ranks(A, B, C, D, n_sys, n_row, 1, cyclic, 0);
 }
 
 /* ------------------------------------------------------------------------ */

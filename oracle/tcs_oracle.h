@@ -96,6 +96,9 @@ int orc_tdma_parts(int n_row);
 void orc_set_tdma_split(orc *o, int split);
 void orc_ptdma_many_solve_ranks(double *A, double *B, double *C, double *D, int n_sys, int n_row,
                                 int nprocs, int cyclic);
+/* the split solve of the GPU kernels: orc_tdma_parts(n_row) ranks, reduced
+   system by parallel cyclic reduction when there are two or more parts */
+void orc_ptdma_many_solve_split(double *A, double *B, double *C, double *D, int n_sys, int n_row, int cyclic);
 void orc_tdma_many(double *a, double *b, double *c, double *d, int n1, int n2);
 void orc_tdma_cycl_many(double *a, double *b, double *c, double *d, int n1, int n2);
 void orc_tdma_single(double *a, double *b, double *c, double *d, int n1);

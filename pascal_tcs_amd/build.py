@@ -4,8 +4,9 @@ Every translation unit under csrc/ is compiled for gfx950 with FP contraction
 off (the plain FP64 form reproduces the reference's operation order, so no
 FMA fusion may change a rounding) and packed-FP32 instructions disabled, then
 linked into one shared library next to this file.  The stage units
-(thermal, momentum, pressure) are compiled a second time as the `fast`
-arithmetic form (contraction and reciprocals allowed; precision 1).  The Fortran drop-in
+(thermal, momentum, pressure) are compiled twice more: as the `fast`
+arithmetic form (contraction and reciprocals allowed; precision 1) and as
+the `mixed` form (fast arithmetic, FP32 ADI scratch; precision 2).  The Fortran drop-in
 shells under fortran/ are compiled too when flang is present.
 """
 import os
@@ -17,8 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpascal_tcs.so")
 SOURCES = ["runtime.hip", "tdma.hip", "tdma_part.hip", "thermal.hip", "momentum.hip", "pressure.hip"]
-# stage translation units built a second time as the `fast` arithmetic form
-FAST_SOURCES = ["thermal.hip", "momentum.hip", "pressure.hip"]
+# stage translation units built again as the `fast` and `mixed` forms
+FORM_SOURCES = ["thermal.hip", "momentum.hip", "pressure.hip"]
 ARCH = os.environ.get("PASCAL_TCS_ARCH", "gfx950")
 
 
@@ -29,11 +30,14 @@ def hipcc():
     raise RuntimeError("hipcc not found: the HIP library cannot be built")
 
 
-def flags(fast=False):
+def flags(form="exact"):
     # exact form: no contraction, IEEE division; fast form: FMA contraction,
-    # reciprocals and reassociation (inf/NaN semantics kept)
-    arith = (["-ffp-contract=fast", "-ffast-math", "-fno-finite-math-only", "-DTCS_FORM=fast"] if fast
-             else ["-ffp-contract=off", "-fno-fast-math"])
+    # reciprocals and reassociation (inf/NaN semantics kept); mixed form:
+    # the fast arithmetic with FP32 ADI scratch
+    fastm = ["-ffp-contract=fast", "-ffast-math", "-fno-finite-math-only"]
+    arith = {"exact": ["-ffp-contract=off", "-fno-fast-math"],
+             "fast": fastm + ["-DTCS_FORM=fast"],
+             "mixed": fastm + ["-DTCS_FORM=mixed", "-DTCS_SCRATCH=float"]}[form]
     return [
         "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
         *arith,
@@ -55,11 +59,12 @@ def build(verbose=False, jobs=8):
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     procs, objs = [], []
-    units = [(s, False) for s in SOURCES] + [(s, True) for s in FAST_SOURCES]
-    for src, fast in units:
-        obj = os.path.join(objdir, src.replace(".hip", "_fast.o" if fast else ".o"))
+    units = ([(s, "exact") for s in SOURCES] + [(s, "fast") for s in FORM_SOURCES]
+             + [(s, "mixed") for s in FORM_SOURCES])
+    for src, form in units:
+        obj = os.path.join(objdir, src.replace(".hip", ".o" if form == "exact" else f"_{form}.o"))
         objs.append(obj)
-        cmd = [cc] + flags(fast) + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [cc] + flags(form) + ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd))
         procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))

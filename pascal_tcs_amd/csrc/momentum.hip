@@ -18,6 +18,14 @@
 namespace tcs {
 namespace TCS_FORM {
 
+#ifndef TCS_SCRATCH
+#define TCS_SCRATCH double
+#endif
+// ADI scratch (am, ac, ap and the right-hand side / increment): FP64, or
+// FP32 in the mixed form, whose line solves then run in FP32 too.
+using scr_t = TCS_SCRATCH;
+#define SCR(p) (reinterpret_cast<scr_t *>(p))
+
 int sweep_z(tcs_ctx *c, bool cyc);
 int sweep_x(tcs_ctx *c, bool cyc);
 int sweep_y(tcs_ctx *c, int j0, int nrows);
@@ -85,7 +93,7 @@ k_dU_z(Grid g, Phys p, double dt, const double *__restrict__ U, const double *__
        const double *__restrict__ W, const double *__restrict__ P, const double *__restrict__ T,
        const double *__restrict__ Mu, const double *__restrict__ iR, const double *__restrict__ UBCup,
        const double *__restrict__ UBCbt, const double *__restrict__ VBCup, const double *__restrict__ VBCbt,
-       double *__restrict__ am, double *__restrict__ ac, double *__restrict__ ap, double *__restrict__ rhsout) {
+       scr_t *__restrict__ am, scr_t *__restrict__ ac, scr_t *__restrict__ ap, scr_t *__restrict__ rhsout) {
     KERNEL_IJK
     const int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1, km = k - 1, kp = k + 1;
     const int jum = g.jC[jm], jup = g.jC[jp];
@@ -175,7 +183,7 @@ k_dU_z(Grid g, Phys p, double dt, const double *__restrict__ U, const double *__
 
 __global__ void __launch_bounds__(256)
 k_dU_x(Grid g, Phys p, double dt, const double *__restrict__ U, const double *__restrict__ Mu,
-       const double *__restrict__ iR, double *__restrict__ am, double *__restrict__ ac, double *__restrict__ ap) {
+       const double *__restrict__ iR, scr_t *__restrict__ am, scr_t *__restrict__ ac, scr_t *__restrict__ ap) {
     KERNEL_IJK
     const int im = i - 1, ip = i + 1;
     const double u1 = 0.5 * (Uf(im, j, k) + Uf(i, j, k));
@@ -196,7 +204,7 @@ k_dU_x(Grid g, Phys p, double dt, const double *__restrict__ U, const double *__
 
 __global__ void __launch_bounds__(256)
 k_dU_y(Grid g, Phys p, double dt, const double *__restrict__ V, const double *__restrict__ Mu,
-       const double *__restrict__ iR, double *__restrict__ am, double *__restrict__ ac, double *__restrict__ ap) {
+       const double *__restrict__ iR, scr_t *__restrict__ am, scr_t *__restrict__ ac, scr_t *__restrict__ ap) {
     KERNEL_IJK
     const int im = i - 1, jm = j - 1, jp = j + 1;
     const int jum = g.jC[jm], jup = g.jC[jp];
@@ -221,7 +229,7 @@ k_dU_y(Grid g, Phys p, double dt, const double *__restrict__ V, const double *__
 }
 
 // interior copy: F(i,j,k) = RHS for j in [j0, n2m]
-__global__ void k_store_interior(Grid g, int j0, double *__restrict__ F, const double *__restrict__ src) {
+__global__ void k_store_interior(Grid g, int j0, double *__restrict__ F, const scr_t *__restrict__ src) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
@@ -244,8 +252,8 @@ __global__ void __launch_bounds__(256)
 k_dV_z(Grid g, Phys p, double dt, const double *__restrict__ U, const double *__restrict__ V,
        const double *__restrict__ W, const double *__restrict__ P, const double *__restrict__ dU,
        const double *__restrict__ Mu, const double *__restrict__ iR, const double *__restrict__ VBCup,
-       const double *__restrict__ VBCbt, double *__restrict__ am, double *__restrict__ ac, double *__restrict__ ap,
-       double *__restrict__ rhsout) {
+       const double *__restrict__ VBCbt, scr_t *__restrict__ am, scr_t *__restrict__ ac, scr_t *__restrict__ ap,
+       scr_t *__restrict__ rhsout) {
     KERNEL_IJK
     if (j < 2) return;  // j starts at j_indexS = 2
     const int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1, km = k - 1, kp = k + 1;
@@ -333,7 +341,7 @@ k_dV_z(Grid g, Phys p, double dt, const double *__restrict__ U, const double *__
 
 __global__ void __launch_bounds__(256)
 k_dV_x(Grid g, Phys p, double dt, const double *__restrict__ U, const double *__restrict__ iR,
-       const double *__restrict__ stale, double *__restrict__ am, double *__restrict__ ac, double *__restrict__ ap) {
+       const double *__restrict__ stale, scr_t *__restrict__ am, scr_t *__restrict__ ac, scr_t *__restrict__ ap) {
     KERNEL_IJK
     if (j < 2) return;
     const int ip = i + 1, jm = j - 1;
@@ -353,7 +361,7 @@ k_dV_x(Grid g, Phys p, double dt, const double *__restrict__ U, const double *__
 
 __global__ void __launch_bounds__(256)
 k_dV_y(Grid g, Phys p, double dt, const double *__restrict__ V, const double *__restrict__ Mu,
-       const double *__restrict__ iR, double *__restrict__ am, double *__restrict__ ac, double *__restrict__ ap) {
+       const double *__restrict__ iR, scr_t *__restrict__ am, scr_t *__restrict__ ac, scr_t *__restrict__ ap) {
     KERNEL_IJK
     if (j < 2) return;
     const int jm = j - 1, jp = j + 1;
@@ -382,8 +390,8 @@ k_dW_z(Grid g, Phys p, double dt, const double *__restrict__ U, const double *__
        const double *__restrict__ W, const double *__restrict__ P, const double *__restrict__ dU,
        const double *__restrict__ dV, const double *__restrict__ Mu, const double *__restrict__ iR,
        const double *__restrict__ VBCup, const double *__restrict__ VBCbt, const double *__restrict__ WBCup,
-       const double *__restrict__ WBCbt, double *__restrict__ am, double *__restrict__ ac, double *__restrict__ ap,
-       double *__restrict__ rhsout) {
+       const double *__restrict__ WBCbt, scr_t *__restrict__ am, scr_t *__restrict__ ac, scr_t *__restrict__ ap,
+       scr_t *__restrict__ rhsout) {
     KERNEL_IJK
     const int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1, km = k - 1, kp = k + 1;
     const int jwp = g.jC[jp], jwm = g.jC[jm];
@@ -480,7 +488,7 @@ k_dW_z(Grid g, Phys p, double dt, const double *__restrict__ U, const double *__
 
 __global__ void __launch_bounds__(256)
 k_dW_x(Grid g, Phys p, double dt, const double *__restrict__ U, const double *__restrict__ iR,
-       const double *__restrict__ stale, double *__restrict__ am, double *__restrict__ ac, double *__restrict__ ap) {
+       const double *__restrict__ stale, scr_t *__restrict__ am, scr_t *__restrict__ ac, scr_t *__restrict__ ap) {
     KERNEL_IJK
     const int ip = i + 1, km = k - 1;
     const double mu1 = stale[2], mu2 = stale[3];
@@ -499,7 +507,7 @@ k_dW_x(Grid g, Phys p, double dt, const double *__restrict__ U, const double *__
 
 __global__ void __launch_bounds__(256)
 k_dW_y(Grid g, Phys p, double dt, const double *__restrict__ V, const double *__restrict__ iR,
-       const double *__restrict__ stale, double *__restrict__ am, double *__restrict__ ac, double *__restrict__ ap) {
+       const double *__restrict__ stale, scr_t *__restrict__ am, scr_t *__restrict__ ac, scr_t *__restrict__ ap) {
     KERNEL_IJK
     const int jp = j + 1, jm = j - 1;
     const int km = g.n3m - 1;  // stale from the x-stage's last k
@@ -613,15 +621,15 @@ int momentum_solvedU(tcs_ctx *c, double dt) {
     double **f = c->f;
     TCS_LAUNCH(c->stream, "dU_z", k_dU_z, grd, blk, 0, g, c->phys, dt, f[TCS_U], f[TCS_V], f[TCS_W], f[TCS_P],
                        f[TCS_T], f[TCS_MU], f[TCS_INVRHO], c->wall[TCS_UBCUP], c->wall[TCS_UBCBT],
-                       c->wall[TCS_VBCUP], c->wall[TCS_VBCBT], c->am, c->ac, c->ap, c->rhs);
+                       c->wall[TCS_VBCUP], c->wall[TCS_VBCBT], SCR(c->am), SCR(c->ac), SCR(c->ap), SCR(c->rhs));
     TCS_TRY(sweep_z(c, true));
-    TCS_LAUNCH(c->stream, "dU_x", k_dU_x, grd, blk, 0, g, c->phys, dt, f[TCS_U], f[TCS_MU], f[TCS_INVRHO], c->am,
-                       c->ac, c->ap);
+    TCS_LAUNCH(c->stream, "dU_x", k_dU_x, grd, blk, 0, g, c->phys, dt, f[TCS_U], f[TCS_MU], f[TCS_INVRHO], SCR(c->am),
+                       SCR(c->ac), SCR(c->ap));
     TCS_TRY(sweep_x(c, true));
-    TCS_LAUNCH(c->stream, "dU_y", k_dU_y, grd, blk, 0, g, c->phys, dt, f[TCS_V], f[TCS_MU], f[TCS_INVRHO], c->am,
-                       c->ac, c->ap);
+    TCS_LAUNCH(c->stream, "dU_y", k_dU_y, grd, blk, 0, g, c->phys, dt, f[TCS_V], f[TCS_MU], f[TCS_INVRHO], SCR(c->am),
+                       SCR(c->ac), SCR(c->ap));
     TCS_TRY(sweep_y(c, 1, c->n2m));
-    TCS_LAUNCH(c->stream, "store_interior", k_store_interior, grd, blk, 0, g, 1, f[TCS_DU], c->rhs);
+    TCS_LAUNCH(c->stream, "store_interior", k_store_interior, grd, blk, 0, g, 1, f[TCS_DU], SCR(c->rhs));
     return TCS_OK;
 }
 
@@ -631,26 +639,26 @@ int momentum_solvedV(tcs_ctx *c, double dt) {
     double **f = c->f;
     TCS_TRY(launch_stale(c));
     TCS_LAUNCH(c->stream, "dV_z", k_dV_z, grd, blk, 0, g, c->phys, dt, f[TCS_U], f[TCS_V], f[TCS_W], f[TCS_P],
-                       f[TCS_DU], f[TCS_MU], f[TCS_INVRHO], c->wall[TCS_VBCUP], c->wall[TCS_VBCBT], c->am, c->ac,
-                       c->ap, c->rhs);
+                       f[TCS_DU], f[TCS_MU], f[TCS_INVRHO], c->wall[TCS_VBCUP], c->wall[TCS_VBCBT], SCR(c->am), SCR(c->ac),
+                       SCR(c->ap), SCR(c->rhs));
     // rows j=1 are not part of the dV systems; sweep only j=2..n2m in z and x
     {
         const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m;
-        TCS_TRY(ptdma_sweep(c, c->am + n1, c->ac + n1, c->ap + n1, c->rhs + n1, c->n1m, 1, c->n2m - 1, n1,
+        TCS_TRY(ptdma_sweep(c, SCR(c->am) + n1, SCR(c->ac) + n1, SCR(c->ap) + n1, SCR(c->rhs) + n1, c->n1m, 1, c->n2m - 1, n1,
                                    c->n3m, n12, true));
     }
-    TCS_LAUNCH(c->stream, "dV_x", k_dV_x, grd, blk, 0, g, c->phys, dt, f[TCS_U], f[TCS_INVRHO], c->stale, c->am,
-                       c->ac, c->ap);
+    TCS_LAUNCH(c->stream, "dV_x", k_dV_x, grd, blk, 0, g, c->phys, dt, f[TCS_U], f[TCS_INVRHO], c->stale, SCR(c->am),
+                       SCR(c->ac), SCR(c->ap));
     {
         const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m;
-        TCS_TRY(ptdma_sweep(c, c->am + n1, c->ac + n1, c->ap + n1, c->rhs + n1, c->n2m - 1, n1, c->n3m, n12,
+        TCS_TRY(ptdma_sweep(c, SCR(c->am) + n1, SCR(c->ac) + n1, SCR(c->ap) + n1, SCR(c->rhs) + n1, c->n2m - 1, n1, c->n3m, n12,
                                    c->n1m, 1, true));
     }
-    TCS_LAUNCH(c->stream, "dV_y", k_dV_y, grd, blk, 0, g, c->phys, dt, f[TCS_V], f[TCS_MU], f[TCS_INVRHO], c->am,
-                       c->ac, c->ap);
+    TCS_LAUNCH(c->stream, "dV_y", k_dV_y, grd, blk, 0, g, c->phys, dt, f[TCS_V], f[TCS_MU], f[TCS_INVRHO], SCR(c->am),
+                       SCR(c->ac), SCR(c->ap));
     TCS_TRY(sweep_y(c, 2, c->n2m - 1));
     TCS_LAUNCH(c->stream, "dV_zero_rows", k_dV_zero_rows, dim3((c->N1 + 63) / 64, c->N3), dim3(64), 0, g, f[TCS_DV]);
-    TCS_LAUNCH(c->stream, "store_interior", k_store_interior, grd, blk, 0, g, 2, f[TCS_DV], c->rhs);
+    TCS_LAUNCH(c->stream, "store_interior", k_store_interior, grd, blk, 0, g, 2, f[TCS_DV], SCR(c->rhs));
     return TCS_OK;
 }
 
@@ -661,15 +669,15 @@ int momentum_solvedW(tcs_ctx *c, double dt) {
     TCS_TRY(launch_stale(c));
     TCS_LAUNCH(c->stream, "dW_z", k_dW_z, grd, blk, 0, g, c->phys, dt, f[TCS_U], f[TCS_V], f[TCS_W], f[TCS_P],
                        f[TCS_DU], f[TCS_DV], f[TCS_MU], f[TCS_INVRHO], c->wall[TCS_VBCUP], c->wall[TCS_VBCBT],
-                       c->wall[TCS_WBCUP], c->wall[TCS_WBCBT], c->am, c->ac, c->ap, c->rhs);
+                       c->wall[TCS_WBCUP], c->wall[TCS_WBCBT], SCR(c->am), SCR(c->ac), SCR(c->ap), SCR(c->rhs));
     TCS_TRY(sweep_z(c, true));
-    TCS_LAUNCH(c->stream, "dW_x", k_dW_x, grd, blk, 0, g, c->phys, dt, f[TCS_U], f[TCS_INVRHO], c->stale, c->am,
-                       c->ac, c->ap);
+    TCS_LAUNCH(c->stream, "dW_x", k_dW_x, grd, blk, 0, g, c->phys, dt, f[TCS_U], f[TCS_INVRHO], c->stale, SCR(c->am),
+                       SCR(c->ac), SCR(c->ap));
     TCS_TRY(sweep_x(c, true));
-    TCS_LAUNCH(c->stream, "dW_y", k_dW_y, grd, blk, 0, g, c->phys, dt, f[TCS_V], f[TCS_INVRHO], c->stale, c->am,
-                       c->ac, c->ap);
+    TCS_LAUNCH(c->stream, "dW_y", k_dW_y, grd, blk, 0, g, c->phys, dt, f[TCS_V], f[TCS_INVRHO], c->stale, SCR(c->am),
+                       SCR(c->ac), SCR(c->ap));
     TCS_TRY(sweep_y(c, 1, c->n2m));
-    TCS_LAUNCH(c->stream, "store_interior", k_store_interior, grd, blk, 0, g, 1, f[TCS_DW], c->rhs);
+    TCS_LAUNCH(c->stream, "store_interior", k_store_interior, grd, blk, 0, g, 1, f[TCS_DW], SCR(c->rhs));
     return TCS_OK;
 }
 

@@ -30,50 +30,96 @@ namespace TCS_FORM {
 constexpr int FFT_THREADS = 256;
 constexpr int FFT_MAXB = 8;  // butterflies per thread per stage
 
-// In-place radix-2 Stockham on nl lines (re/im at [l*n + t]) in LDS.
-// sign<0 forward (exp(-i...)), sign>0 backward.  Caller synchronises before.
+// In-place radix-2 Stockham on nl lines (re/im at [l*n + t]) in LDS, n a
+// power of two, nl*n <= 2*FFT_MAXB*FFT_THREADS.  sign<0 forward
+// (exp(-i...)), sign>0 backward.  Caller synchronises before.
+//
+// Stage s (Ns = 2^s) combines x[jb] and x[jb + n/2] with the twiddle
+// tw[(jb mod Ns) * n/(2Ns)] into positions 2Ns*floor(jb/Ns) + jb mod Ns
+// (+Ns).  Three consecutive stages s, s+1, s+2 only mix the 8 values
+// x[t*n/8 + g], t = 0..7, of a group g, so each thread runs them in
+// registers and touches LDS once per three stages: the same butterflies with
+// the same operands as the stage-by-stage loop, so the same results bit for
+// bit.  The group's outputs land at 8(g - kk) + kk + r*Ns, kk = g mod Ns,
+// with r the three result bits (first stage's bit lowest).
+template <int SIGN, int R>
+__device__ __forceinline__ void fft_pass(double *re, double *im, int n, int nl, int s,
+                                         const double2 *__restrict__ tw) {
+    constexpr int M = 1 << R;
+    constexpr int GMAX = 2 * FFT_MAXB / M;  // groups per thread
+    const int G = n >> R;
+    const int total = G * nl;
+    const int Ns = 1 << s;
+    const int tid = threadIdx.x;
+    double xr[GMAX][M], xi[GMAX][M];
+    int obase[GMAX];
+#pragma unroll
+    for (int u = 0; u < GMAX; u++) {
+        const int e = tid + u * FFT_THREADS;
+        obase[u] = -1;
+        if (e < total) {
+            const int l = e / G;
+            const int g = e - l * G;
+            const int kk = g & (Ns - 1);
+#pragma unroll
+            for (int t = 0; t < M; t++) {
+                xr[u][t] = re[l * n + t * G + g];
+                xi[u][t] = im[l * n + t * G + g];
+            }
+#pragma unroll
+            for (int i = 0; i < R; i++) {
+                double yr[M], yi[M];
+#pragma unroll
+                for (int a = 0; a < M / 2; a++) {
+                    // result bits r0..r_{i-1} of the pair: the low i bits of a, reversed
+                    int rs = 0;
+#pragma unroll
+                    for (int m = 0; m < i; m++) rs |= ((a >> (i - 1 - m)) & 1) << m;
+                    const double2 w = tw[(kk + rs * Ns) * (n >> (s + i + 1))];
+                    const double ar = xr[u][a], ai = xi[u][a];
+                    const double br = xr[u][a + M / 2], bi = xi[u][a + M / 2];
+                    double tr, ti;
+                    if (SIGN < 0) {
+                        tr = br * w.x - bi * w.y;
+                        ti = br * w.y + bi * w.x;
+                    } else {
+                        tr = br * w.x + bi * w.y;
+                        ti = bi * w.x - br * w.y;
+                    }
+                    yr[2 * a] = ar + tr; yi[2 * a] = ai + ti;
+                    yr[2 * a + 1] = ar - tr; yi[2 * a + 1] = ai - ti;
+                }
+#pragma unroll
+                for (int t = 0; t < M; t++) { xr[u][t] = yr[t]; xi[u][t] = yi[t]; }
+            }
+            obase[u] = l * n + (g - kk) * M + kk;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < GMAX; u++) {
+        if (obase[u] >= 0) {
+#pragma unroll
+            for (int t = 0; t < M; t++) {
+                int r = 0;  // register index t holds result bits (r0..r_{R-1}) with r0 highest
+#pragma unroll
+                for (int m = 0; m < R; m++) r |= ((t >> (R - 1 - m)) & 1) << m;
+                re[obase[u] + r * Ns] = xr[u][t];
+                im[obase[u] + r * Ns] = xi[u][t];
+            }
+        }
+    }
+    __syncthreads();
+}
+
 template <int SIGN>
 __device__ __forceinline__ void lds_fft(double *re, double *im, int n, int nl, const double2 *__restrict__ tw) {
-    const int half = n >> 1;
-    const int nb = half * nl;
-    const int tid = threadIdx.x;
-    for (int Ns = 1; Ns < n; Ns <<= 1) {
-        double yr0[FFT_MAXB], yi0[FFT_MAXB], yr1[FFT_MAXB], yi1[FFT_MAXB];
-        int oi[FFT_MAXB];
-#pragma unroll
-        for (int u = 0; u < FFT_MAXB; u++) {
-            const int e = tid + u * FFT_THREADS;
-            oi[u] = -1;
-            if (e < nb) {
-                const int l = e / half;
-                const int jb = e - l * half;
-                const int kk = jb & (Ns - 1);
-                const double2 w = tw[kk * (n / (2 * Ns))];
-                const double ar = re[l * n + jb], ai = im[l * n + jb];
-                const double br = re[l * n + jb + half], bi = im[l * n + jb + half];
-                double tr, ti;
-                if (SIGN < 0) {
-                    tr = br * w.x - bi * w.y;
-                    ti = br * w.y + bi * w.x;
-                } else {
-                    tr = br * w.x + bi * w.y;
-                    ti = bi * w.x - br * w.y;
-                }
-                yr0[u] = ar + tr; yi0[u] = ai + ti;
-                yr1[u] = ar - tr; yi1[u] = ai - ti;
-                oi[u] = l * n + (jb / Ns) * 2 * Ns + kk;
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < FFT_MAXB; u++) {
-            if (oi[u] >= 0) {
-                re[oi[u]] = yr0[u]; im[oi[u]] = yi0[u];
-                re[oi[u] + Ns] = yr1[u]; im[oi[u] + Ns] = yi1[u];
-            }
-        }
-        __syncthreads();
-    }
+    int L = 0;
+    while ((1 << L) < n) L++;
+    int s = 0;
+    for (; s + 3 <= L; s += 3) fft_pass<SIGN, 3>(re, im, n, nl, s, tw);
+    if (L - s == 2) fft_pass<SIGN, 2>(re, im, n, nl, s, tw);
+    else if (L - s == 1) fft_pass<SIGN, 1>(re, im, n, nl, s, tw);
 }
 
 // PRHS (:182-229) into the interior layout

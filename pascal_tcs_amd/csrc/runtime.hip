@@ -417,9 +417,13 @@ extern "C" int tcs_init(const tcs_params *prm, const double *dx1, const double *
         set_error("tcs_init: n1m and n3m must be powers of two in [4, 4096] and n2m >= 3");
         return TCS_E_ARG;
     }
-    if (prm->precision != 0 && prm->precision != 1) {
-        set_error("tcs_init: precision must be 0 (plain FP64, reference operation order) or 1 (plain FP64, "
-                  "contracted/reciprocal arithmetic)");
+    if (prm->precision < 0 || prm->precision > 2) {
+        set_error("tcs_init: precision must be 0 (plain FP64, reference operation order), 1 (plain FP64, "
+                  "contracted/reciprocal arithmetic) or 2 (mixed: FP32 ADI scratch and line solves)");
+        return TCS_E_ARG;
+    }
+    if (prm->precision == 2 && (prm->n1m > 512 || prm->n2m > 512 || prm->n3m > 512)) {
+        set_error("tcs_init: the mixed form solves its lines split in parts and needs n1m, n2m, n3m <= 512");
         return TCS_E_ARG;
     }
     if (prm->tdma_split != 0 && prm->tdma_split != 1) {
@@ -638,8 +642,11 @@ extern "C" int tcs_momentum_boundary(tcs_ctx *c, double Uup, double Ubt, double 
 
 #define STAGE(label) ProfStage _stage(c, label)
 
-// the stage routine of the context's arithmetic form (precision 0 / 1)
-#define FORM(c, fn, ...) ((c)->prm.precision == 1 ? ::tcs::fast::fn((c), ##__VA_ARGS__) : ::tcs::exact::fn((c), ##__VA_ARGS__))
+// the stage routine of the context's arithmetic form (precision 0 / 1 / 2)
+#define FORM(c, fn, ...)                                                     \
+    ((c)->prm.precision == 2   ? ::tcs::mixed::fn((c), ##__VA_ARGS__)        \
+     : (c)->prm.precision == 1 ? ::tcs::fast::fn((c), ##__VA_ARGS__)         \
+                               : ::tcs::exact::fn((c), ##__VA_ARGS__))
 
 // stage labels follow the reference's timer labels (src/main.f90:55-72)
 extern "C" int tcs_thermal_coeffi(tcs_ctx *c) { CTX_GUARD(c); STAGE("thermal coeff"); return FORM(c, thermal_coeffi); }

@@ -151,6 +151,9 @@ int ptdma_exact_launch(double *A, const double *B, double *C, double *D,
 bool ptdma_part_plan(int n_row, int &P, int &w1, int &w2);
 int ptdma_part_launch(double *A, const double *B, double *C, double *D, int n0, long ss0, int n1, long ss1,
                       int n_row, long rs, bool cyclic, hipStream_t st);
+// FP32 lines (the mixed form's ADI solves), same algorithm and layouts.
+int ptdma_part_launch(float *A, const float *B, float *C, float *D, int n0, long ss0, int n1, long ss1,
+                      int n_row, long rs, bool cyclic, hipStream_t st);
 // ADI sweep of a context: split solve when the context asks for it and the
 // line length allows, else the single-rank order.
 inline int ptdma_sweep(const tcs_ctx *c, double *A, const double *B, double *C, double *D, int n0, long ss0,
@@ -160,12 +163,19 @@ inline int ptdma_sweep(const tcs_ctx *c, double *A, const double *B, double *C, 
         return ptdma_part_launch(A, B, C, D, n0, ss0, n1, ss1, n_row, rs, cyclic, c->stream);
     return ptdma_exact_launch(A, B, C, D, n0, ss0, n1, ss1, n_row, rs, cyclic, c->stream);
 }
+// FP32 sweep of the mixed form: always the split solve (lines <= 512 rows).
+inline int ptdma_sweep(const tcs_ctx *c, float *A, const float *B, float *C, float *D, int n0, long ss0, int n1,
+                       long ss1, int n_row, long rs, bool cyclic) {
+    return ptdma_part_launch(A, B, C, D, n0, ss0, n1, ss1, n_row, rs, cyclic, c->stream);
+}
 
 // Stage entry points of the kernel translation units (thermal, momentum,
-// pressure).  Each of them is compiled twice: into namespace `exact` with FP
-// contraction off (the reference's operation order, precision 0) and into
-// namespace `fast` with contraction and reciprocal arithmetic allowed
-// (precision 1).  TCS_FORM names the namespace a translation unit builds.
+// pressure).  Each of them is compiled three times: into namespace `exact`
+// with FP contraction off (the reference's operation order, precision 0),
+// into namespace `fast` with contraction and reciprocal arithmetic allowed
+// (precision 1), and into namespace `mixed`, the fast arithmetic with FP32
+// ADI scratch and FP32 line solves (precision 2).  TCS_FORM names the
+// namespace a translation unit builds, TCS_SCRATCH its scratch type.
 #define TCS_STAGE_DECLS                                   \
     int thermal_solver(tcs_ctx *c, double dt);            \
     int thermal_coeffi(tcs_ctx *c);                       \
@@ -181,6 +191,7 @@ inline int ptdma_sweep(const tcs_ctx *c, double *A, const double *B, double *C, 
     int pressure_projection(tcs_ctx *c, double dt);
 namespace exact { TCS_STAGE_DECLS }
 namespace fast { TCS_STAGE_DECLS }
+namespace mixed { TCS_STAGE_DECLS }
 #undef TCS_STAGE_DECLS
 int halo(tcs_ctx *c, double *f);
 int post_div(tcs_ctx *c, double *out);

@@ -11,22 +11,25 @@
 // solution (pascal_tdma.f90:567-576).
 //
 // Here a "rank" is one thread: a line of n_row <= 512 rows is split into
-// P = ceil(n_row/16) parts, each thread keeps its part (at most 16 rows of
-// A, C, D) in registers through the whole solve, the reduced systems of a
-// workgroup sit in LDS and one thread per system solves its reduced system
-// in the reference's order.  HBM traffic is the algorithmic minimum: A, B,
-// C, D read once and the solution written once (40 bytes per unknown); A and
-// C are not written back.
+// the power of two P >= n_row/16 parts, each thread keeps its part (at most
+// 16 rows of A, C, D) in registers through the whole solve, and the reduced
+// systems of a workgroup sit in LDS.  With one part the reduced 2-row system
+// is solved in the reference's order; with P >= 2 the line's P threads solve
+// it together by parallel cyclic reduction (log2(2P) steps instead of a
+// serial sweep over 2P rows, whose latency otherwise dominates the kernel).
+// HBM traffic is the algorithmic minimum: A, B, C, D read once and the
+// solution written once (40 bytes per unknown in FP64); A and C are not
+// written back.
 //
 //   k_ptdma_part       lines strided in memory (y and z sweeps): 16
 //                      neighbouring lines per workgroup, so every row step
 //                      of a wavefront is four 128-byte segments;
-//   k_ptdma_part_rows  contiguous lines (x sweeps): a workgroup stages 4
-//                      lines of A, B, C, D through LDS with coalesced loads,
-//                      each thread then reads its part from LDS.
+//   k_ptdma_part_rows  contiguous lines (x sweeps): a workgroup stages 16
+//                      lines of A, B, C, D one array at a time through LDS
+//                      with coalesced loads, each thread then reads its part.
 //
-// Every operation follows the oracle's ptdma_many_ranks in order, with FP
-// contraction off, so results are bit-identical to it.
+// Every operation follows the oracle's orc_ptdma_many_solve_split in order,
+// with FP contraction off, so results are bit-identical to it.
 #include "tcs_internal.h"
 
 namespace tcs {
@@ -35,12 +38,11 @@ constexpr int PT_MMAX = 16;               // rows per part
 constexpr int PT_PMAX = 32;               // parts per line (n_row <= 512)
 constexpr int PT_RS = 2 * PT_PMAX + 1;    // reduced-row stride in LDS (odd: no bank conflicts)
 constexpr int PT_SX = 16;                 // lines per workgroup, strided kernel
-constexpr int PT_SXR = 4;                 // lines per workgroup, rows kernel
-constexpr int PT_UMAX = PT_MMAX / PT_SXR; // staged values per thread, line and array
 
 bool ptdma_part_plan(int n_row, int &P, int &w1, int &w2) {
     if (n_row < 2 || n_row > PT_MMAX * PT_PMAX) return false;
-    P = (n_row + PT_MMAX - 1) / PT_MMAX;
+    P = 1;
+    while (P * PT_MMAX < n_row) P <<= 1;  // power of two: the cyclic reduction pairs rows i, i + P
     w1 = n_row / P;
     w2 = n_row % P;
     return true;
@@ -55,8 +57,9 @@ __device__ __forceinline__ void part_range(int p, int w1, int w2, int &j0, int &
 
 // One part (rows 0..m-1, m >= 2) in registers: pascal_tdma.f90:501-536.
 // b is consumed; a, c, d come back as the reduced-form coefficients.
-__device__ __forceinline__ void part_reduce(double (&a)[PT_MMAX], double (&b)[PT_MMAX], double (&c)[PT_MMAX],
-                                            double (&d)[PT_MMAX], int m) {
+template <typename T>
+__device__ __forceinline__ void part_reduce(T (&a)[PT_MMAX], T (&b)[PT_MMAX], T (&c)[PT_MMAX],
+                                            T (&d)[PT_MMAX], int m) {
     a[0] = a[0] / b[0];
     d[0] = d[0] / b[0];
     c[0] = c[0] / b[0];
@@ -66,7 +69,7 @@ __device__ __forceinline__ void part_reduce(double (&a)[PT_MMAX], double (&b)[PT
 #pragma unroll
     for (int r = 2; r < PT_MMAX; r++) {
         if (r < m) {
-            const double rr = 1.0 / (b[r] - a[r] * c[r - 1]);
+            const T rr = T(1) / (b[r] - a[r] * c[r - 1]);
             d[r] = rr * (d[r] - a[r] * d[r - 1]);
             c[r] = rr * c[r];
             a[r] = -rr * a[r] * a[r - 1];
@@ -80,15 +83,16 @@ __device__ __forceinline__ void part_reduce(double (&a)[PT_MMAX], double (&b)[PT
             c[r] = -c[r] * c[r + 1];
         }
     }
-    const double rr = 1.0 / (1.0 - a[1] * c[0]);
+    const T rr = T(1) / (T(1) - a[1] * c[0]);
     d[0] = rr * (d[0] - c[0] * d[1]);
     a[0] = rr * a[0];
     c[0] = -rr * c[0] * c[1];
 }
 
 // values of row m-1 (a runtime index) without dynamic register indexing
-__device__ __forceinline__ void part_last(const double (&a)[PT_MMAX], const double (&c)[PT_MMAX],
-                                          const double (&d)[PT_MMAX], int m, double &la, double &lc, double &ld) {
+template <typename T>
+__device__ __forceinline__ void part_last(const T (&a)[PT_MMAX], const T (&c)[PT_MMAX],
+                                          const T (&d)[PT_MMAX], int m, T &la, T &lc, T &ld) {
     la = a[0]; lc = c[0]; ld = d[0];
 #pragma unroll
     for (int r = 1; r < PT_MMAX; r++)
@@ -97,8 +101,9 @@ __device__ __forceinline__ void part_last(const double (&a)[PT_MMAX], const doub
 
 // x(r) of the part from its reduced solution xf (row 0) and xl (row m-1):
 // pascal_tdma.f90:567-576
-__device__ __forceinline__ double part_x(const double (&a)[PT_MMAX], const double (&c)[PT_MMAX],
-                                         const double (&d)[PT_MMAX], int r, int m, double xf, double xl) {
+template <typename T>
+__device__ __forceinline__ T part_x(const T (&a)[PT_MMAX], const T (&c)[PT_MMAX],
+                                         const T (&d)[PT_MMAX], int r, int m, T xf, T xl) {
     if (r == 0) return xf;
     if (r == m - 1) return xl;
     return d[r] - a[r] * xf - c[r] * xl;
@@ -108,21 +113,21 @@ __device__ __forceinline__ double part_x(const double (&a)[PT_MMAX], const doubl
 // in place in ra/rc/rd by tdma_many (tdmas.f90:113-146) or tdma_cycl_many
 // (tdmas.f90:159-212; re is its e scratch).  Division by the unit diagonal
 // is exact and left out.
-template <bool CYC>
-__device__ void reduced_solve(double *ra, double *rc, double *rd, double *re, int n2) {
+template <typename T, bool CYC>
+__device__ void reduced_solve(T *ra, T *rc, T *rd, T *re, int n2) {
     if (!CYC) {
         for (int j = 1; j < n2; j++) {
-            const double r = 1.0 / (1.0 - ra[j] * rc[j - 1]);
+            const T r = T(1) / (T(1) - ra[j] * rc[j - 1]);
             rd[j] = r * (rd[j] - ra[j] * rd[j - 1]);
             rc[j] = r * rc[j];
         }
         for (int j = n2 - 2; j >= 0; j--) rd[j] = rd[j] - rc[j] * rd[j + 1];
     } else {
-        for (int j = 0; j < n2; j++) re[j] = 0.0;
+        for (int j = 0; j < n2; j++) re[j] = T(0);
         re[1] = -ra[1];
         re[n2 - 1] = -rc[n2 - 1];
         for (int j = 2; j < n2; j++) {
-            const double rr = 1.0 / (1.0 - ra[j] * rc[j - 1]);
+            const T rr = T(1) / (T(1) - ra[j] * rc[j - 1]);
             rd[j] = rr * (rd[j] - ra[j] * rd[j - 1]);
             re[j] = rr * (re[j] - ra[j] * re[j - 1]);
             rc[j] = rr * rc[j];
@@ -131,195 +136,255 @@ __device__ void reduced_solve(double *ra, double *rc, double *rd, double *re, in
             rd[j] = rd[j] - rc[j] * rd[j + 1];
             re[j] = re[j] - rc[j] * re[j + 1];
         }
-        rd[0] = (rd[0] - ra[0] * rd[n2 - 1] - rc[0] * rd[1]) / (1.0 + ra[0] * re[n2 - 1] + rc[0] * re[1]);
+        rd[0] = (rd[0] - ra[0] * rd[n2 - 1] - rc[0] * rd[1]) / (T(1) + ra[0] * re[n2 - 1] + rc[0] * re[1]);
         for (int j = 1; j < n2; j++) rd[j] = rd[j] + rd[0] * re[j];
     }
 }
 
 // Reduced arrays of a workgroup: [q][line][row], q = a, c, d, e.
-template <int SX>
+template <typename T, int SX>
 struct RedLDS {
-    double v[4 * SX * PT_RS];
-    __device__ double *q(int which, int line) { return v + ((size_t)which * SX + line) * PT_RS; }
+    T v[4 * SX * PT_RS];
+    __device__ T *q(int which, int line) { return v + ((size_t)which * SX + line) * PT_RS; }
 };
 
-// After part_reduce: publish the part's two reduced rows, solve, read back.
-template <bool CYC, int SX>
-__device__ __forceinline__ void part_exchange(RedLDS<SX> &red, int line, int p, int P, const double (&a)[PT_MMAX],
-                                              const double (&c)[PT_MMAX], const double (&d)[PT_MMAX], int m,
-                                              double &xf, double &xl) {
-    double la, lc, ld;
+// After part_reduce: publish the part's two reduced rows (2p, 2p+1 of the
+// line's reduced system), solve the reduced system, return the part's
+// solution at its first (xf) and last (xl) row.
+//
+// One part (P = 1): the reference's serial tdma_many / tdma_cycl_many on the
+// 2-row system.  P >= 2 (a power of two): parallel cyclic reduction by the
+// line's P threads, two rows each.  Step with stride s: row i eliminates its
+// couplings to rows i-s and i+s with their current values,
+//   k1 = a_i/b_{i-s}, k2 = c_i/b_{i+s}, a_i' = -a_{i-s} k1, c_i' = -c_{i+s} k2,
+//   b_i' = b_i - c_{i-s} k1 - a_{i+s} k2, d_i' = d_i - d_{i-s} k1 - d_{i+s} k2;
+// outside a non-cyclic system the rows are identity rows.  Non-cyclic:
+// strides 1..N/2 (N = 2P) and x_i = d_i/b_i.  Cyclic: strides 1..N/4, then
+// row i couples only to j = i + N/2 and the pair is solved directly,
+// x_i = (d_i b_j - e_i d_j)/(b_i b_j - e_i e_j), e = a + c.
+template <typename T, bool CYC, int SX>
+__device__ __forceinline__ void part_exchange(RedLDS<T, SX> &red, int line, int p, int P, const T (&a)[PT_MMAX],
+                                              const T (&c)[PT_MMAX], const T (&d)[PT_MMAX], int m,
+                                              T &xf, T &xl) {
+    T la, lc, ld;
     part_last(a, c, d, m, la, lc, ld);
-    double *ra = red.q(0, line), *rc = red.q(1, line), *rd = red.q(2, line);
+    T *ra = red.q(0, line), *rc = red.q(1, line), *rd = red.q(2, line), *rb = red.q(3, line);
     ra[2 * p] = a[0]; rc[2 * p] = c[0]; rd[2 * p] = d[0];
     ra[2 * p + 1] = la; rc[2 * p + 1] = lc; rd[2 * p + 1] = ld;
+    if (P == 1) {
+        __syncthreads();
+        if (p == 0) reduced_solve<T, CYC>(ra, rc, rd, rb, 2);
+        __syncthreads();
+        xf = rd[0];
+        xl = rd[1];
+        return;
+    }
+    rb[2 * p] = T(1); rb[2 * p + 1] = T(1);
     __syncthreads();
-    if (p == 0) reduced_solve<CYC>(ra, rc, rd, red.q(3, line), 2 * P);
-    __syncthreads();
-    xf = rd[2 * p];
-    xl = rd[2 * p + 1];
+    const int N = 2 * P, lim = CYC ? N / 2 : N;
+    T va[2] = {a[0], la}, vb[2] = {T(1), T(1)}, vc[2] = {c[0], lc}, vd[2] = {d[0], ld};
+    for (int s = 1; s < lim; s <<= 1) {
+        T na[2], nb[2], nc[2], nd[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            int im = 2 * p + h - s, ip = 2 * p + h + s;
+            T am = T(0), bm = T(1), cm = T(0), dm = T(0), ap = T(0), bp = T(1), cp = T(0), dp = T(0);
+            if (CYC) { im &= N - 1; ip &= N - 1; }
+            if (CYC || im >= 0) { am = ra[im]; bm = rb[im]; cm = rc[im]; dm = rd[im]; }
+            if (CYC || ip < N) { ap = ra[ip]; bp = rb[ip]; cp = rc[ip]; dp = rd[ip]; }
+            const T k1 = va[h] / bm, k2 = vc[h] / bp;
+            na[h] = -am * k1;
+            nc[h] = -cp * k2;
+            nb[h] = vb[h] - cm * k1 - ap * k2;
+            nd[h] = vd[h] - dm * k1 - dp * k2;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            ra[2 * p + h] = na[h]; rb[2 * p + h] = nb[h]; rc[2 * p + h] = nc[h]; rd[2 * p + h] = nd[h];
+            va[h] = na[h]; vb[h] = nb[h]; vc[h] = nc[h]; vd[h] = nd[h];
+        }
+        __syncthreads();
+    }
+    T x[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        if (CYC) {
+            const int j = (2 * p + h + P) & (N - 1);
+            const T ei = va[h] + vc[h], ej = ra[j] + rc[j];
+            x[h] = (vd[h] * rb[j] - ei * rd[j]) / (vb[h] * rb[j] - ei * ej);
+        } else {
+            x[h] = vd[h] / vb[h];
+        }
+    }
+    xf = x[0];
+    xl = x[1];
+}
+
+// Workgroup -> group of SX lines.  With SX = 8 two workgroups share every
+// 128-byte row segment of 16 lines; they are placed 8 apart in dispatch
+// order so that both run on the same XCD (workgroups go to the 8 XCDs
+// round-robin) and the second finds the segment in that XCD's L2.
+template <int SX>
+__device__ __forceinline__ int line_group(int bx, int nbx) {
+    if (SX != 8 || (nbx & 15) != 0) return bx;
+    const int q = bx >> 4, r = bx & 15;
+    return ((q << 3) + (r & 7)) * 2 + (r >> 3);
 }
 
 // Lines strided in memory: line (s0, s1) starts at s0 + s1*ss1 and its rows
-// are rs apart.  blockDim.x = PT_SX * P, thread t = line + PT_SX * part.
-template <bool CYC>
-__global__ void __launch_bounds__(PT_SX * PT_PMAX)
-k_ptdma_part(const double *__restrict__ A, const double *__restrict__ B, const double *__restrict__ C,
-             double *__restrict__ D, int n0, long ss1, long rs, int P, int w1, int w2) {
-    __shared__ RedLDS<PT_SX> red;
+// are rs apart.  blockDim.x = SX * P, thread t = line + SX * part.
+template <typename T, bool CYC, int SX>
+__global__ void __launch_bounds__(SX * PT_PMAX)
+k_ptdma_part(const T *__restrict__ A, const T *__restrict__ B, const T *__restrict__ C,
+             T *__restrict__ D, int n0, long ss1, long rs, int P, int w1, int w2) {
+    __shared__ RedLDS<T, SX> red;
     const int t = threadIdx.x;
-    const int line = t % PT_SX, p = t / PT_SX;
-    const int s0 = blockIdx.x * PT_SX + line;
+    const int line = t % SX, p = t / SX;
+    const int s0 = line_group<SX>(blockIdx.x, gridDim.x) * SX + line;
     const bool act = s0 < n0;
     const long base = (long)s0 + (long)blockIdx.y * ss1;
     int j0, m;
     part_range(p, w1, w2, j0, m);
 
-    double a[PT_MMAX], c[PT_MMAX], d[PT_MMAX];
+    T a[PT_MMAX], c[PT_MMAX], d[PT_MMAX];
     {
-        double b[PT_MMAX];
+        T b[PT_MMAX];
 #pragma unroll
         for (int r = 0; r < PT_MMAX; r++) {
             if (act && r < m) {
                 const long o = base + (long)(j0 + r) * rs;
                 a[r] = A[o]; b[r] = B[o]; c[r] = C[o]; d[r] = D[o];
             } else {
-                a[r] = 0.0; b[r] = 1.0; c[r] = 0.0; d[r] = 0.0;
+                a[r] = T(0); b[r] = T(1); c[r] = T(0); d[r] = T(0);
             }
         }
         part_reduce(a, b, c, d, m);
     }
-    double xf, xl;
-    part_exchange<CYC, PT_SX>(red, line, p, P, a, c, d, m, xf, xl);
+    T xf, xl;
+    part_exchange<T, CYC, SX>(red, line, p, P, a, c, d, m, xf, xl);
     if (!act) return;
 #pragma unroll
     for (int r = 0; r < PT_MMAX; r++)
         if (r < m) D[base + (long)(j0 + r) * rs] = part_x(a, c, d, r, m, xf, xl);
 }
 
-// padded LDS position of row j of a staged line (one pad double per 16 rows)
+// padded LDS position of row j of a staged line (one pad value per 16 rows)
 __device__ __forceinline__ int stage_pos(int j) { return j + (j >> 4); }
 
 // Contiguous lines (rows 1 apart): line (s0, s1) starts at s0*ss0 + s1*ss1.
-// blockDim.x = PT_SXR * P, thread t = part + P * line.  Dynamic LDS: the 4
-// staged arrays, 4 * PT_SXR * ls doubles with ls = stage_pos(n_row-1) + 1.
-template <bool CYC>
-__global__ void __launch_bounds__(PT_SXR * PT_PMAX)
-k_ptdma_part_rows(const double *__restrict__ A, const double *__restrict__ B, const double *__restrict__ C,
-                  double *__restrict__ D, int n0, long ss0, long ss1, int n_row, int ls, int P, int w1, int w2) {
-    extern __shared__ double stage[];
-    __shared__ RedLDS<PT_SXR> red;
-    const int t = threadIdx.x, nt = blockDim.x;
-    const int p = t % P, line = t / P;
-    const int s0 = blockIdx.x * PT_SXR + line;
+// The same thread layout as k_ptdma_part (thread t = line + PT_SX * part,
+// blockDim.x = PT_SX * P >= n_row).  A, B, C, D pass one after the other
+// through an LDS stage of PT_SX lines (dynamic LDS, PT_SX * ls values with
+// ls = stage_pos(n_row-1) + 1): every thread loads row t of each of the 16
+// lines (coalesced along the lines), the next array's loads are issued
+// before the stage is read back part-wise.  The solution goes back out
+// through the stage the same way.
+template <typename T, bool CYC>
+__global__ void __launch_bounds__(PT_SX * PT_PMAX)
+k_ptdma_part_rows(const T *__restrict__ A, const T *__restrict__ B, const T *__restrict__ C, T *__restrict__ D,
+                  int n0, long ss0, long ss1, int n_row, int ls, int P, int w1, int w2) {
+    extern __shared__ __align__(16) unsigned char stage_raw[];
+    T *stage = reinterpret_cast<T *>(stage_raw);
+    __shared__ RedLDS<T, PT_SX> red;
+    const int t = threadIdx.x;
+    const int line = t % PT_SX, p = t / PT_SX;
+    const int s0 = blockIdx.x * PT_SX + line;
     const bool act = s0 < n0;
     int j0, m;
     part_range(p, w1, w2, j0, m);
+    const long gbase = (long)blockIdx.x * PT_SX * ss0 + (long)blockIdx.y * ss1;
+    const int nl = min(PT_SX, n0 - (int)blockIdx.x * PT_SX);  // lines present in this workgroup
+    const bool row_ok = t < n_row;
 
-    // Stage A, B, C, D of the workgroup's lines, coalesced along each line.
-    // A line has n_row <= 16*P rows and the workgroup 4*P threads, so each
-    // thread moves at most PT_UMAX values per line and array; all of its
-    // loads are issued before the first LDS store.
-    const double *src[4] = {A, B, C, D};
-    {
-        double v[4][PT_SXR][PT_UMAX];
+    T a[PT_MMAX], b[PT_MMAX], c[PT_MMAX], d[PT_MMAX];
+    T v[PT_SX];
 #pragma unroll
-        for (int q = 0; q < 4; q++)
+    for (int l = 0; l < PT_SX; l++) v[l] = (l < nl && row_ok) ? A[gbase + l * ss0 + t] : T(0);
 #pragma unroll
-            for (int l = 0; l < PT_SXR; l++)
+    for (int q = 0; q < 4; q++) {
+        if (q > 0) __syncthreads();  // the stage's previous array has been read
+        if (row_ok) {
 #pragma unroll
-                for (int u = 0; u < PT_UMAX; u++) {
-                    const int s = blockIdx.x * PT_SXR + l, j = t + nt * u;
-                    v[q][l][u] = (s < n0 && j < n_row) ? src[q][(long)s * ss0 + (long)blockIdx.y * ss1 + j] : 0.0;
-                }
-#pragma unroll
-        for (int q = 0; q < 4; q++)
-#pragma unroll
-            for (int l = 0; l < PT_SXR; l++)
-#pragma unroll
-                for (int u = 0; u < PT_UMAX; u++) {
-                    const int j = t + nt * u;
-                    if (j < n_row) stage[((size_t)q * PT_SXR + l) * ls + stage_pos(j)] = v[q][l][u];
-                }
-    }
-    __syncthreads();
-
-    double a[PT_MMAX], c[PT_MMAX], d[PT_MMAX];
-    {
-        double b[PT_MMAX];
-        const double *sa = stage + ((size_t)0 * PT_SXR + line) * ls;
-        const double *sb = stage + ((size_t)1 * PT_SXR + line) * ls;
-        const double *sc = stage + ((size_t)2 * PT_SXR + line) * ls;
-        const double *sd = stage + ((size_t)3 * PT_SXR + line) * ls;
-#pragma unroll
-        for (int r = 0; r < PT_MMAX; r++) {
-            if (act && r < m) {
-                const int o = stage_pos(j0 + r);
-                a[r] = sa[o]; b[r] = sb[o]; c[r] = sc[o]; d[r] = sd[o];
-            } else {
-                a[r] = 0.0; b[r] = 1.0; c[r] = 0.0; d[r] = 0.0;
-            }
+            for (int l = 0; l < PT_SX; l++) stage[l * ls + stage_pos(t)] = v[l];
         }
-        part_reduce(a, b, c, d, m);
+        if (q < 3) {
+            const T *g = q == 0 ? B : (q == 1 ? C : D);
+#pragma unroll
+            for (int l = 0; l < PT_SX; l++) v[l] = (l < nl && row_ok) ? g[gbase + l * ss0 + t] : T(0);
+        }
+        __syncthreads();
+        T *dst = q == 0 ? a : (q == 1 ? b : (q == 2 ? c : d));
+        const T pad = q == 1 ? T(1) : T(0);
+#pragma unroll
+        for (int r = 0; r < PT_MMAX; r++) dst[r] = (act && r < m) ? stage[line * ls + stage_pos(j0 + r)] : pad;
     }
-    double xf, xl;
-    part_exchange<CYC, PT_SXR>(red, line, p, P, a, c, d, m, xf, xl);
-
-    // solution back through LDS (the D slot) for coalesced stores
-    double *sd = stage + ((size_t)3 * PT_SXR + line) * ls;
+    part_reduce(a, b, c, d, m);
+    T xf, xl;
+    part_exchange<T, CYC, PT_SX>(red, line, p, P, a, c, d, m, xf, xl);
+    // every thread has read the stage before part_exchange's first barrier
     if (act) {
 #pragma unroll
         for (int r = 0; r < PT_MMAX; r++)
-            if (r < m) sd[stage_pos(j0 + r)] = part_x(a, c, d, r, m, xf, xl);
+            if (r < m) stage[line * ls + stage_pos(j0 + r)] = part_x(a, c, d, r, m, xf, xl);
     }
     __syncthreads();
+    if (row_ok) {
 #pragma unroll
-    for (int l = 0; l < PT_SXR; l++)
-#pragma unroll
-        for (int u = 0; u < PT_UMAX; u++) {
-            const int s = blockIdx.x * PT_SXR + l, j = t + nt * u;
-            if (s < n0 && j < n_row)
-                D[(long)s * ss0 + (long)blockIdx.y * ss1 + j] = stage[((size_t)3 * PT_SXR + l) * ls + stage_pos(j)];
-        }
+        for (int l = 0; l < PT_SX; l++)
+            if (l < nl) D[gbase + l * ss0 + t] = stage[l * ls + stage_pos(t)];
+    }
 }
 
 }  // namespace
 
-int ptdma_part_lds_bytes(int n_row) {
+template <typename T>
+static int part_lds_bytes(int n_row) {
     const int ls = n_row - 1 + ((n_row - 1) >> 4) + 1;
-    return 4 * PT_SXR * ls * (int)sizeof(double);
+    return PT_SX * ls * (int)sizeof(T);
 }
 
+template <typename T> struct PartLabels;
+template <> struct PartLabels<double> {
+    static constexpr const char *rows_cyc = "ptdma_part_rows_cyclic", *rows = "ptdma_part_rows",
+                                *cyc = "ptdma_part_cyclic", *plain = "ptdma_part";
+};
+template <> struct PartLabels<float> {
+    static constexpr const char *rows_cyc = "ptdma_part_rows_cyclic_f32", *rows = "ptdma_part_rows_f32",
+                                *cyc = "ptdma_part_cyclic_f32", *plain = "ptdma_part_f32";
+};
+
 // Partitioned solve of n0 x n1 lines (see the kernels for the layouts).
-// Returns TCS_E_ARG when the line length is outside the partitioned range;
-// callers then use the single-rank kernels.
-int ptdma_part_launch(double *A, const double *B, double *C, double *D, int n0, long ss0, int n1, long ss1,
-                      int n_row, long rs, bool cyclic, hipStream_t st) {
+// Returns TCS_E_ARG when the line length is outside the partitioned range.
+template <typename T>
+static int part_launch(T *A, const T *B, T *C, T *D, int n0, long ss0, int n1, long ss1, int n_row, long rs,
+                       bool cyclic, hipStream_t st) {
     int P, w1, w2;
     if (n0 <= 0 || n1 <= 0) return TCS_OK;
     if (!ptdma_part_plan(n_row, P, w1, w2) || n1 > 65535) {
         set_error("ptdma_part: n_row must be in [2, 512] and the outer line count <= 65535");
         return TCS_E_ARG;
     }
+    using L = PartLabels<T>;
     if (rs == 1) {
         const int ls = n_row - 1 + ((n_row - 1) >> 4) + 1;
-        const int lds = 4 * PT_SXR * ls * (int)sizeof(double);
+        const int lds = part_lds_bytes<T>(n_row);
         static bool lds_raised = false;  // allow more than 64 KiB of dynamic LDS (gfx950: 160 KiB per CU)
         if (!lds_raised) {
-            const int most = ptdma_part_lds_bytes(PT_MMAX * PT_PMAX);
-            TCS_HIP(hipFuncSetAttribute((const void *)k_ptdma_part_rows<false>,
+            const int most = part_lds_bytes<T>(PT_MMAX * PT_PMAX);
+            TCS_HIP(hipFuncSetAttribute((const void *)k_ptdma_part_rows<T, false>,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, most));
-            TCS_HIP(hipFuncSetAttribute((const void *)k_ptdma_part_rows<true>,
+            TCS_HIP(hipFuncSetAttribute((const void *)k_ptdma_part_rows<T, true>,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, most));
             lds_raised = true;
         }
-        dim3 blk(PT_SXR * P), grd((unsigned)((n0 + PT_SXR - 1) / PT_SXR), (unsigned)n1);
+        dim3 blk(PT_SX * P), grd((unsigned)((n0 + PT_SX - 1) / PT_SX), (unsigned)n1);
         if (cyclic)
-            TCS_LAUNCH(st, "ptdma_part_rows_cyclic", k_ptdma_part_rows<true>, grd, blk, lds, A, B, C, D, n0, ss0,
-                       ss1, n_row, ls, P, w1, w2);
+            TCS_LAUNCH(st, L::rows_cyc, (k_ptdma_part_rows<T, true>), grd, blk, lds, A, B, C, D, n0, ss0, ss1, n_row,
+                       ls, P, w1, w2);
         else
-            TCS_LAUNCH(st, "ptdma_part_rows", k_ptdma_part_rows<false>, grd, blk, lds, A, B, C, D, n0, ss0, ss1,
-                       n_row, ls, P, w1, w2);
+            TCS_LAUNCH(st, L::rows, (k_ptdma_part_rows<T, false>), grd, blk, lds, A, B, C, D, n0, ss0, ss1, n_row, ls,
+                       P, w1, w2);
         return TCS_OK;
     }
     if (ss0 != 1) {
@@ -328,10 +393,20 @@ int ptdma_part_launch(double *A, const double *B, double *C, double *D, int n0, 
     }
     dim3 blk(PT_SX * P), grd((unsigned)((n0 + PT_SX - 1) / PT_SX), (unsigned)n1);
     if (cyclic)
-        TCS_LAUNCH(st, "ptdma_part_cyclic", k_ptdma_part<true>, grd, blk, 0, A, B, C, D, n0, ss1, rs, P, w1, w2);
+        TCS_LAUNCH(st, L::cyc, (k_ptdma_part<T, true, PT_SX>), grd, blk, 0, A, B, C, D, n0, ss1, rs, P, w1, w2);
     else
-        TCS_LAUNCH(st, "ptdma_part", k_ptdma_part<false>, grd, blk, 0, A, B, C, D, n0, ss1, rs, P, w1, w2);
+        TCS_LAUNCH(st, L::plain, (k_ptdma_part<T, false, PT_SX>), grd, blk, 0, A, B, C, D, n0, ss1, rs, P, w1, w2);
     return TCS_OK;
+}
+
+int ptdma_part_launch(double *A, const double *B, double *C, double *D, int n0, long ss0, int n1, long ss1,
+                      int n_row, long rs, bool cyclic, hipStream_t st) {
+    return part_launch<double>(A, B, C, D, n0, ss0, n1, ss1, n_row, rs, cyclic, st);
+}
+
+int ptdma_part_launch(float *A, const float *B, float *C, float *D, int n0, long ss0, int n1, long ss1,
+                      int n_row, long rs, bool cyclic, hipStream_t st) {
+    return part_launch<float>(A, B, C, D, n0, ss0, n1, ss1, n_row, rs, cyclic, st);
 }
 
 }  // namespace tcs

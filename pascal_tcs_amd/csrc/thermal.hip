@@ -18,6 +18,14 @@
 namespace tcs {
 namespace TCS_FORM {
 
+#ifndef TCS_SCRATCH
+#define TCS_SCRATCH double
+#endif
+// ADI scratch (am, ac, ap and the right-hand side / increment): FP64, or
+// FP32 in the mixed form, whose line solves then run in FP32 too.
+using scr_t = TCS_SCRATCH;
+#define SCR(p) (reinterpret_cast<scr_t *>(p))
+
 __device__ __forceinline__ void th_kpp(const Phys &p, double T, double &kpp, double &dkpp) {
     const double tmp1 = p.DeltaT * T;
     const double tmp2 = tmp1 * tmp1;
@@ -75,8 +83,8 @@ __global__ void k_thermal_coeffi(Grid g, Phys p, const double *__restrict__ T, d
 __global__ void __launch_bounds__(256)
 k_thermal_z(Grid g, Phys p, double dt, const double *__restrict__ T, const double *__restrict__ U,
             const double *__restrict__ V, const double *__restrict__ W, const double *__restrict__ TBCup,
-            const double *__restrict__ TBCbt, double *__restrict__ am, double *__restrict__ ac,
-            double *__restrict__ ap, double *__restrict__ rhsout) {
+            const double *__restrict__ TBCbt, scr_t *__restrict__ am, scr_t *__restrict__ ac,
+            scr_t *__restrict__ ap, scr_t *__restrict__ rhsout) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
@@ -206,7 +214,7 @@ k_thermal_z(Grid g, Phys p, double dt, const double *__restrict__ T, const doubl
 // variables keep after stage 1: n2m-1, n2m+1, n3m-1, n3m+1.
 __global__ void __launch_bounds__(256)
 k_thermal_x(Grid g, Phys p, double dt, const double *__restrict__ T, const double *__restrict__ U,
-            double *__restrict__ am, double *__restrict__ ac, double *__restrict__ ap) {
+            scr_t *__restrict__ am, scr_t *__restrict__ ac, scr_t *__restrict__ ap) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
@@ -263,7 +271,7 @@ k_thermal_x(Grid g, Phys p, double dt, const double *__restrict__ T, const doubl
 // stage 3: y (:456-507); km,kp keep their stage-1 values n3m-1, n3m+1.
 __global__ void __launch_bounds__(256)
 k_thermal_y(Grid g, Phys p, double dt, const double *__restrict__ T, const double *__restrict__ V,
-            double *__restrict__ am, double *__restrict__ ac, double *__restrict__ ap) {
+            scr_t *__restrict__ am, scr_t *__restrict__ ac, scr_t *__restrict__ ap) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
@@ -321,7 +329,7 @@ k_thermal_y(Grid g, Phys p, double dt, const double *__restrict__ T, const doubl
 }
 
 // T(i,j,k) += RHSJ (:518-524)
-__global__ void k_add_interior(Grid g, double *__restrict__ F, const double *__restrict__ inc) {
+__global__ void k_add_interior(Grid g, double *__restrict__ F, const scr_t *__restrict__ inc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
@@ -333,17 +341,17 @@ __global__ void k_add_interior(Grid g, double *__restrict__ F, const double *__r
 // sweeps over the interior (i,j,k) layout
 int sweep_z(tcs_ctx *c, bool cyc) {
     const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m;
-    return ptdma_sweep(c, c->am, c->ac, c->ap, c->rhs, c->n1m, 1, c->n2m, n1, c->n3m, n12, cyc);
+    return ptdma_sweep(c, SCR(c->am), SCR(c->ac), SCR(c->ap), SCR(c->rhs), c->n1m, 1, c->n2m, n1, c->n3m, n12, cyc);
 }
 int sweep_x(tcs_ctx *c, bool cyc) {
     const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m;
-    return ptdma_sweep(c, c->am, c->ac, c->ap, c->rhs, c->n2m, n1, c->n3m, n12, c->n1m, 1, cyc);
+    return ptdma_sweep(c, SCR(c->am), SCR(c->ac), SCR(c->ap), SCR(c->rhs), c->n2m, n1, c->n3m, n12, c->n1m, 1, cyc);
 }
 int sweep_y(tcs_ctx *c, int j0, int nrows) {
     // rows j = j0..j0+nrows-1 (1-based)
     const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m;
     const long off = (long)(j0 - 1) * n1;
-    return ptdma_sweep(c, c->am + off, c->ac + off, c->ap + off, c->rhs + off, c->n1m, 1, c->n3m, n12,
+    return ptdma_sweep(c, SCR(c->am) + off, SCR(c->ac) + off, SCR(c->ap) + off, SCR(c->rhs) + off, c->n1m, 1, c->n3m, n12,
                               nrows, n1, false);
 }
 
@@ -357,16 +365,16 @@ int thermal_solver(tcs_ctx *c, double dt) {
     const Grid g = c->grid();
     const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
     TCS_LAUNCH(c->stream, "thermal_z", k_thermal_z, grd, blk, 0, g, c->phys, dt, c->f[TCS_T], c->f[TCS_U],
-                       c->f[TCS_V], c->f[TCS_W], c->wall[TCS_TBCUP], c->wall[TCS_TBCBT], c->am, c->ac, c->ap,
-                       c->rhs);
+                       c->f[TCS_V], c->f[TCS_W], c->wall[TCS_TBCUP], c->wall[TCS_TBCBT], SCR(c->am), SCR(c->ac), SCR(c->ap),
+                       SCR(c->rhs));
     TCS_TRY(sweep_z(c, true));
-    TCS_LAUNCH(c->stream, "thermal_x", k_thermal_x, grd, blk, 0, g, c->phys, dt, c->f[TCS_T], c->f[TCS_U], c->am,
-                       c->ac, c->ap);
+    TCS_LAUNCH(c->stream, "thermal_x", k_thermal_x, grd, blk, 0, g, c->phys, dt, c->f[TCS_T], c->f[TCS_U], SCR(c->am),
+                       SCR(c->ac), SCR(c->ap));
     TCS_TRY(sweep_x(c, true));
-    TCS_LAUNCH(c->stream, "thermal_y", k_thermal_y, grd, blk, 0, g, c->phys, dt, c->f[TCS_T], c->f[TCS_V], c->am,
-                       c->ac, c->ap);
+    TCS_LAUNCH(c->stream, "thermal_y", k_thermal_y, grd, blk, 0, g, c->phys, dt, c->f[TCS_T], c->f[TCS_V], SCR(c->am),
+                       SCR(c->ac), SCR(c->ap));
     TCS_TRY(sweep_y(c, 1, c->n2m));
-    TCS_LAUNCH(c->stream, "add_interior", k_add_interior, grd, blk, 0, g, c->f[TCS_T], c->rhs);
+    TCS_LAUNCH(c->stream, "add_interior", k_add_interior, grd, blk, 0, g, c->f[TCS_T], SCR(c->rhs));
     return TCS_OK;
 }
 

@@ -84,9 +84,10 @@ def _dev_check(*ts):
 
 def many_solve_split(A, B, C, D, cyclic=False):
     """PaScaL_TDMA_many_solve[_cycle] on (n_row, n_sys) CUDA tensors with every
-    system's rows split over ceil(n_row/16) ranks (n_row <= 512): the
-    reference's multi-rank operation order, one GPU thread per rank.  D
-    returns the solution; A, B, C are not modified."""
+    system's rows split over P ranks (P the power of two >= n_row/16,
+    n_row <= 512), one GPU thread per rank, reduced system by parallel
+    cyclic reduction when P >= 2.  D returns the solution; A, B, C are not
+    modified."""
     import torch
     _dev_check(A, B, C, D)
     lib = _lib.load()

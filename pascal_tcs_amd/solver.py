@@ -34,9 +34,10 @@ def mesh_axis(nm, L, uniform, gamma, periodic):
 
 
 class TCSSolver:
-    """tdma_split=1 solves every ADI line split into ceil(n/16) parts, as
-    PaScaL_TDMA solves lines split over that many ranks (HBM-minimal
-    kernels); 0 keeps the single-rank order of a 1x1x1 reference run.
+    """tdma_split=1 solves every ADI line split into parts as PaScaL_TDMA
+    splits lines over ranks (HBM-minimal kernels, reduced systems by
+    parallel cyclic reduction); 0 keeps the single-rank order of a 1x1x1
+    reference run.
     precision=0 keeps the reference's operation order in the stencils;
     1 lets them contract to FMAs and use reciprocals (plain FP64 within
     1e-12 per step of the reference order)."""

@@ -142,9 +142,9 @@ SPLIT_SHAPES = [(64, 40, 32), (32, 100, 16), (16, 12, 8)]
 
 @pytest.mark.parametrize("shape", SPLIT_SHAPES)
 def test_full_step_seeded_split_lines(shape):
-    """The step with every ADI line split into ceil(n/16) parts (x lines of
-    64 -> 4 parts, y lines of 100/99 -> 7 parts, ...), bit for bit against
-    the oracle run with the same split."""
+    """The step with every ADI line split into parts (x lines of 64 -> 4
+    parts, y lines of 100/99 -> 8 parts, ...), reduced systems by parallel
+    cyclic reduction, bit for bit against the oracle run with the same split."""
     test_full_step_seeded(shape, split=1)
 
 

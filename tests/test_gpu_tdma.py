@@ -120,7 +120,8 @@ def test_axis_solve_512_cube_sampled(axis):
         assert np.array_equal(D[tuple(sl)].cpu().numpy(), sys_[3][:, 0])
 
 
-# ---- split lines: PaScaL_TDMA's multi-rank order, one thread per rank -------
+# ---- split lines: PaScaL_TDMA's multi-rank algorithm, one thread per rank,
+# reduced system by parallel cyclic reduction (the oracle's split solve) ----
 
 @pytest.mark.parametrize("n_sys,n_row", [(1, 2), (5, 3), (37, 16), (300, 17), (4096, 64), (70001, 33),
                                          (257, 511), (64, 512), (17, 100)])
@@ -134,7 +135,7 @@ def test_split_many_solve_device(n_sys, n_row, cyclic):
     ptdma.many_solve_split(A, B, C, D, cyclic=cyclic)
     torch.cuda.synchronize()
     ra, rb, rc, rd = (x.copy() for x in (a, b, c, d))
-    O.ptdma_many_solve_ranks(ra, rb, rc, rd, nprocs=O.tdma_parts(n_row), cyclic=cyclic)
+    O.ptdma_many_solve_split(ra, rb, rc, rd, cyclic=cyclic)
     assert np.array_equal(D.cpu().numpy(), rd)
     for t, x in zip((A, B, C), (a, b, c)):
         assert np.array_equal(t.cpu().numpy(), x)
@@ -157,7 +158,7 @@ def test_split_axis_solve(shape, axis, cyclic):
     arrs = [np.ascontiguousarray(x.transpose(perm)) for x in (a, b, c, d)]
     n_row = arrs[0].shape[0]
     flat = [x.reshape(n_row, -1).copy() for x in arrs]
-    O.ptdma_many_solve_ranks(*flat, nprocs=O.tdma_parts(n_row), cyclic=cyclic)
+    O.ptdma_many_solve_split(*flat, cyclic=cyclic)
     exp = flat[3].reshape(arrs[0].shape).transpose(inv)
     g = D.cpu().numpy()
     assert np.array_equal(g, exp), f"D differs: {np.abs(g - exp).max()}"
@@ -206,5 +207,5 @@ def test_split_axis_solve_512_cube(axis, cyclic):
         sl = [int(p), int(q)]
         sl.insert(dim, slice(None))
         sys_ = [t[tuple(sl)].cpu().numpy().reshape(n, 1).copy() for t in (a, b, c, d)]
-        O.ptdma_many_solve_ranks(*sys_, nprocs=O.tdma_parts(n), cyclic=cyclic)
+        O.ptdma_many_solve_split(*sys_, cyclic=cyclic)
         assert np.array_equal(D[tuple(sl)].cpu().numpy(), sys_[3][:, 0])

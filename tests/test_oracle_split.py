@@ -1,9 +1,10 @@
 """The step with split ADI line solves against the single-rank step (CPU).
 
-Splitting every line into ceil(n/16) parts changes only the rounding of the
-tridiagonal solves (PaScaL_TDMA's multi-rank order instead of its one-rank
-order); the plain form's bar against the reference's FP64 run is 1e-12
-relative per step, checked here field by field over three steps."""
+Splitting every line into parts (PaScaL_TDMA's multi-rank algorithm, the
+reduced system by parallel cyclic reduction) changes only the rounding of
+the tridiagonal solves; the plain form's bar against the reference's FP64
+run is 1e-12 relative per step, checked here field by field over three
+steps."""
 import numpy as np
 import pytest
 

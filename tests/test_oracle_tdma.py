@@ -218,3 +218,43 @@ def test_ranks_reduced_rows_are_rank_ordered():
             aa, bb, cc = a[blk], b[blk], c[blk]
             x = np.linalg.solve(_dense(aa, bb, cc, s, False), d[blk, s])
             assert np.allclose(D[blk, s], x, rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("cyclic", [False, True])
+@pytest.mark.parametrize("n_row", [17, 33, 48, 100, 130, 257, 511, 512])
+def test_split_solve_is_exact_solution(cyclic, n_row):
+    """The GPU split solve (power-of-two part count, reduced system by
+    parallel cyclic reduction) is the exact solution for every line length
+    that splits, cyclic or not."""
+    assert O.tdma_parts(n_row) >= 2
+    rng = np.random.default_rng(n_row * 3 + cyclic)
+    ns = 7
+    a, b, c, d = dominant(rng, n_row, ns)
+    if not cyclic:
+        a[0] = 0.0
+        c[-1] = 0.0
+    A, B, C, D = (x.copy() for x in (a, b, c, d))
+    O.ptdma_many_solve_split(A, B, C, D, cyclic=cyclic)
+    for s in range(ns):
+        x = np.linalg.solve(_dense(a, b, c, s, cyclic), d[:, s])
+        assert np.allclose(D[:, s], x, rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("cyclic", [False, True])
+@pytest.mark.parametrize("n_row", [2, 3, 9, 16])
+def test_split_solve_one_part_is_single_rank(cyclic, n_row):
+    """Up to 16 rows there is one part: the split solve is then the
+    single-rank solve, quirks included (cyclic fill-in drop for short lines)."""
+    assert O.tdma_parts(n_row) == 1
+    rng = np.random.default_rng(n_row)
+    a, b, c, d = dominant(rng, n_row, 5)
+    one = [x.copy() for x in (a, b, c, d)]
+    spl = [x.copy() for x in (a, b, c, d)]
+    O.ptdma_many_solve(*one, cyclic=cyclic)
+    O.ptdma_many_solve_split(*spl, cyclic=cyclic)
+    assert np.array_equal(one[3], spl[3])
+
+
+def test_split_parts_policy():
+    assert [O.tdma_parts(n) for n in (2, 16, 17, 32, 33, 48, 64, 65, 100, 256, 257, 511, 512, 513)] == \
+        [1, 1, 2, 2, 4, 4, 4, 8, 8, 16, 32, 32, 32, 1]
