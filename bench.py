@@ -66,6 +66,8 @@ def kernel_bytes(label, cfg):
         "ptdma_part_f32": 20.0 * N * (3 + fv) / 4,
         "ptdma_part_cyclic_f32": 20.0 * N * (3 + fv) / 4,
         "ptdma_part_rows_cyclic_f32": 20.0 * N * (3 + fv) / 4,
+        # Poisson y solves: complex right-hand side in and solution out per (kx, kz, j)
+        "poisson_part": 32.0 * (n1 // 2 + 1) * n3 * n2,
         "thermal_z": 8.0 * N * (4 + 4), "thermal_x": 8.0 * N * (2 + 3), "thermal_y": 8.0 * N * (2 + 3),
         "dU_z": 8.0 * N * (6 + 4), "dV_z": 8.0 * N * (6 + 4) * fv, "dW_z": 8.0 * N * (7 + 4),
         "dU_x": 8.0 * N * (1 + 3), "dU_y": 8.0 * N * (1 + 3), "dV_x": 8.0 * N * (1 + 3) * fv,
@@ -194,6 +196,9 @@ def main():
                     help="1: ADI lines split into parts as PaScaL_TDMA splits them over ranks, reduced "
                          "systems by parallel cyclic reduction (within 1e-14 of the single-rank order); "
                          "0: single-rank order")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run one device-to-device copy of 1 GiB (the reference kernel for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--no-mixed", dest="mixed", action="store_false",
                     help="skip the mixed form's run (by default it follows the plain run on the same workload)")
     args = ap.parse_args()
@@ -204,6 +209,13 @@ def main():
               "(multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     cfg, sec, rep, state, nu, fields = run_gpu(args, args.precision)
     cells = cfg.n1m * cfg.n2m * cfg.n3m
     value = cells * args.steps / sec / 1e6

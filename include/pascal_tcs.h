@@ -193,6 +193,17 @@ int ptdma_many_solve_cycle_dev(double *A, const double *B, double *C, double *D,
 /* Same contract with host arrays (copied in and out; synchronous). */
 int ptdma_many_solve_host(double *A, double *B, double *C, double *D,
                           int32_t n_sys, int32_t n_row, int32_t cyclic);
+/* PaScaL_TDMA_single_solve / _single_solve_cycle(plan,A,B,C,D,n_row),
+   PaScaL_TDMA/src/pascal_tdma.f90:301-382 and :393-475, on one rank: the
+   per-rank modified Thomas of the many-system solve and the 2-row reduced
+   solve by tdma_single / tdma_cycl_single (tdmas.f90:1-104), whose
+   operations equal tdma_many / tdma_cycl_many on one system -- so this is
+   the many-system solve with n_sys = 1, bit for bit.  Arrays (1:n_row), in
+   place as in the reference; device (stream may be NULL) or host. */
+int ptdma_single_solve_dev(double *A, const double *B, double *C, double *D,
+                           int32_t n_row, int32_t cyclic, void *stream);
+int ptdma_single_solve_host(double *A, double *B, double *C, double *D,
+                            int32_t n_row, int32_t cyclic);
 /* Axis solve on a (n1,n2,n3) array (i fastest) in place, systems along
    `axis` (0=x,1=y,2=z) -- the layout change the reference makes with host
    transposes before each PaScaL call (e.g. src/module_solve_thermal.f90:

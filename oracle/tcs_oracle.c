@@ -1612,8 +1612,8 @@ void orc_pressure_Poisson_FFT2(orc *o, const double *prhs)
             }
     }
     Am_r[0] = 0.0; Ac_r[0] = 1.0; Ap_r[0] = 0.0; Kr[0] = 0.0;
-    orc_ptdma_many_solve(Am_r, Ac_r, Ap_r, Kr, n3m * h1p, n2m);
-    orc_ptdma_many_solve(Am_c, Ac_c, Ap_c, Ki, n3m * h1p, n2m);
+    lines_solve(o, Am_r, Ac_r, Ap_r, Kr, n3m * h1p, n2m, 0);
+    lines_solve(o, Am_c, Ac_c, Ap_c, Ki, n3m * h1p, n2m, 0);
     for (int j = 1; j <= n2m; j++) {
         Ki[R3(n3m, h1p, 1, 1, j)] = 0.0;
         Ki[R3(n3m, h1p, h3p, 1, j)] = 0.0;

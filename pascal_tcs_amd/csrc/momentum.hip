@@ -28,7 +28,7 @@ using scr_t = TCS_SCRATCH;
 
 int sweep_z(tcs_ctx *c, bool cyc);
 int sweep_x(tcs_ctx *c, bool cyc);
-int sweep_y(tcs_ctx *c, int j0, int nrows);
+int sweep_y_field(tcs_ctx *c, int j0, int nrows, double *F, int mode);
 
 #define MU(j, k) Mu[g.t2((j), (k))]
 #define IR(j, k) iR[g.t2((j), (k))]
@@ -229,14 +229,6 @@ k_dU_y(Grid g, Phys p, double dt, const double *__restrict__ V, const double *__
 }
 
 // interior copy: F(i,j,k) = RHS for j in [j0, n2m]
-__global__ void k_store_interior(Grid g, int j0, double *__restrict__ F, const scr_t *__restrict__ src) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
-    if (i > g.n1m || j > g.n2m || j < j0) return;
-    F[g.ix(i, j, k)] = src[g.r3(i, j, k)];
-}
-
 // dV(0:n1sub,0:1,0:n3sub) = 0 and dV(:,n2sub,:) = 0 (:907-909)
 __global__ void k_dV_zero_rows(Grid g, double *__restrict__ dV) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -628,8 +620,7 @@ int momentum_solvedU(tcs_ctx *c, double dt) {
     TCS_TRY(sweep_x(c, true));
     TCS_LAUNCH(c->stream, "dU_y", k_dU_y, grd, blk, 0, g, c->phys, dt, f[TCS_V], f[TCS_MU], f[TCS_INVRHO], SCR(c->am),
                        SCR(c->ac), SCR(c->ap));
-    TCS_TRY(sweep_y(c, 1, c->n2m));
-    TCS_LAUNCH(c->stream, "store_interior", k_store_interior, grd, blk, 0, g, 1, f[TCS_DU], SCR(c->rhs));
+    TCS_TRY(sweep_y_field(c, 1, c->n2m, f[TCS_DU], 1));
     return TCS_OK;
 }
 
@@ -656,9 +647,8 @@ int momentum_solvedV(tcs_ctx *c, double dt) {
     }
     TCS_LAUNCH(c->stream, "dV_y", k_dV_y, grd, blk, 0, g, c->phys, dt, f[TCS_V], f[TCS_MU], f[TCS_INVRHO], SCR(c->am),
                        SCR(c->ac), SCR(c->ap));
-    TCS_TRY(sweep_y(c, 2, c->n2m - 1));
     TCS_LAUNCH(c->stream, "dV_zero_rows", k_dV_zero_rows, dim3((c->N1 + 63) / 64, c->N3), dim3(64), 0, g, f[TCS_DV]);
-    TCS_LAUNCH(c->stream, "store_interior", k_store_interior, grd, blk, 0, g, 2, f[TCS_DV], SCR(c->rhs));
+    TCS_TRY(sweep_y_field(c, 2, c->n2m - 1, f[TCS_DV], 1));
     return TCS_OK;
 }
 
@@ -676,8 +666,7 @@ int momentum_solvedW(tcs_ctx *c, double dt) {
     TCS_TRY(sweep_x(c, true));
     TCS_LAUNCH(c->stream, "dW_y", k_dW_y, grd, blk, 0, g, c->phys, dt, f[TCS_V], f[TCS_INVRHO], c->stale, SCR(c->am),
                        SCR(c->ac), SCR(c->ap));
-    TCS_TRY(sweep_y(c, 1, c->n2m));
-    TCS_LAUNCH(c->stream, "store_interior", k_store_interior, grd, blk, 0, g, 1, f[TCS_DW], SCR(c->rhs));
+    TCS_TRY(sweep_y_field(c, 1, c->n2m, f[TCS_DW], 1));
     return TCS_OK;
 }
 

@@ -18,7 +18,7 @@
 //     the per-line partial sums for the mean.
 // FFTs are radix-2 Stockham with a host twiddle table, computed in place in
 // LDS with the butterfly outputs staged in registers across barriers.
-#include "tcs_internal.h"
+#include "tdma_part.h"
 
 #ifndef TCS_FORM
 #define TCS_FORM exact
@@ -292,6 +292,62 @@ k_poisson_tdma(int n3m, int h1p, int n2m, int h3p, const double *__restrict__ am
     }
 }
 
+// The same y solves with split lines (tdma_split): PaScaL_TDMA's multi-rank
+// algorithm, one thread per part, reduced systems by parallel cyclic
+// reduction (tdma_part.h).  Line s = kz + n3m*kx, row j stride n3m*h1p; the
+// real and imaginary parts are two right-hand sides of one matrix (the
+// reference solves them in two calls with equal coefficients).  The (0,0)
+// system is pinned at j=1 as in the reference's real-part call; its
+// imaginary part is zeroed afterwards like the reference's.
+__global__ void __launch_bounds__(PT_SX * PT_PMAX)
+k_poisson_part(int n3m, int h1p, int h3p, const double *__restrict__ amj, const double *__restrict__ apj,
+               const double *__restrict__ acj, const double *__restrict__ dxk2, const double *__restrict__ dzk,
+               double2 *__restrict__ S, int P, int w1, int w2) {
+    __shared__ part::RedLDS<double, PT_SX, 2> red;
+    const int t = threadIdx.x;
+    const int line = t % PT_SX, p = t / PT_SX;
+    const int nsys = n3m * h1p;
+    const int s = blockIdx.x * PT_SX + line;
+    const bool act = s < nsys;
+    const int kx = act ? s / n3m : 0, kz = act ? s - kx * n3m : 0;
+    const size_t rs = (size_t)nsys;
+    int j0, m;
+    part::range(p, w1, w2, j0, m);
+    const double shift = dxk2[kx], zk = dzk[kz];
+    const bool pin = act && kx == 0 && kz == 0;
+    double a[PT_MMAX], c[PT_MMAX], d[2][PT_MMAX];
+    {
+        double b[PT_MMAX];
+#pragma unroll
+        for (int r = 0; r < PT_MMAX; r++) {
+            a[r] = 0.0; b[r] = 1.0; c[r] = 0.0; d[0][r] = 0.0; d[1][r] = 0.0;
+            if (act && r < m) {
+                const int jj = j0 + r;
+                const double2 v = S[s + (size_t)jj * rs];
+                if (pin && jj == 0) {
+                    d[1][r] = v.y;  // real part pinned to 0 with a unit row
+                } else {
+                    a[r] = amj[jj]; c[r] = apj[jj]; b[r] = acj[jj] - shift - zk;
+                    d[0][r] = v.x; d[1][r] = v.y;
+                }
+            }
+        }
+        part::reduce<double, 2>(a, b, c, d, m);
+    }
+    double xf[2], xl[2];
+    part::exchange<double, false, PT_SX, 2>(red, line, p, P, a, c, d, m, xf, xl);
+    if (!act) return;
+    const bool zim = (kx == 0 && (kz == 0 || kz == h3p - 1));
+#pragma unroll
+    for (int r = 0; r < PT_MMAX; r++) {
+        if (r < m) {
+            const double vr = part::x_of<double, 2>(a, c, d, 0, r, m, xf[0], xl[0]);
+            const double vi = part::x_of<double, 2>(a, c, d, 1, r, m, xf[1], xl[1]);
+            S[s + (size_t)(j0 + r) * rs] = make_double2(vr, zim ? 0.0 : vi);
+        }
+    }
+}
+
 // x c2r of TX k-lines of row j, scaling by 1/n1m/n3m into dP's interior,
 // and the per-line pairwise sums for the mean.
 __global__ void __launch_bounds__(FFT_THREADS)
@@ -469,8 +525,14 @@ int pressure_poisson(tcs_ctx *c) {
     TCS_LAUNCH(c->stream, "fft_z<-1>", k_fft_z<-1>, dim3(h1p, n2m), dim3(FFT_THREADS), ldsz, n3m, h1p,
                        (const double2 *)c->tw3, spec);
     const int nsys = n3m * h1p;
-    TCS_LAUNCH(c->stream, "poisson_tdma", k_poisson_tdma, dim3((nsys + 255) / 256), dim3(256), 0, n3m, h1p, n2m, c->h3p,
-                       c->ptab, c->ptab + n2m, c->ptab + 2 * n2m, c->dxk2, c->dzk, spec, c->pA, c->pC);
+    int P, w1, w2;
+    if (c->tdma_split && ptdma_part_plan(n2m, P, w1, w2)) {
+        TCS_LAUNCH(c->stream, "poisson_part", k_poisson_part, dim3((nsys + PT_SX - 1) / PT_SX), dim3(PT_SX * P), 0,
+                   n3m, h1p, c->h3p, c->ptab, c->ptab + n2m, c->ptab + 2 * n2m, c->dxk2, c->dzk, spec, P, w1, w2);
+    } else {
+        TCS_LAUNCH(c->stream, "poisson_tdma", k_poisson_tdma, dim3((nsys + 255) / 256), dim3(256), 0, n3m, h1p, n2m,
+                   c->h3p, c->ptab, c->ptab + n2m, c->ptab + 2 * n2m, c->dxk2, c->dzk, spec, c->pA, c->pC);
+    }
     TCS_LAUNCH(c->stream, "fft_z<+1>", k_fft_z<+1>, dim3(h1p, n2m), dim3(FFT_THREADS), ldsz, n3m, h1p,
                        (const double2 *)c->tw3, spec);
     TCS_LAUNCH(c->stream, "fft_x_inv", k_fft_x_inv, dim3(n3m / TX, n2m), dim3(FFT_THREADS), ldsx, g, h1p, TX,

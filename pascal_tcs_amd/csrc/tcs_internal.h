@@ -146,14 +146,23 @@ inline dim3 interior_grid(int nx, int ny, int nz) {
 int ptdma_exact_launch(double *A, const double *B, double *C, double *D,
                        int n0, long ss0, int n1, long ss1, int n_row, long rs,
                        bool cyclic, hipStream_t st);
+// Where a split solve puts its solution: back into D (mode 0), or into an
+// FP64 field F at F[base + s0*ss0 + s1*ss1 + row*rs], stored (mode 1) or
+// added to what F holds (mode 2) -- the last sweep of an ADI stage writes
+// the increment straight into its module array.
+struct LineOut {
+    double *F = nullptr;
+    long base = 0, ss0 = 0, ss1 = 0, rs = 0;
+    int mode = 0;
+};
 // The same systems split into parts (PaScaL_TDMA's multi-rank algorithm,
 // one thread per rank); n_row in [2, 512]; strided lines need ss0 = 1.
 bool ptdma_part_plan(int n_row, int &P, int &w1, int &w2);
 int ptdma_part_launch(double *A, const double *B, double *C, double *D, int n0, long ss0, int n1, long ss1,
-                      int n_row, long rs, bool cyclic, hipStream_t st);
+                      int n_row, long rs, bool cyclic, hipStream_t st, const LineOut &out = LineOut());
 // FP32 lines (the mixed form's ADI solves), same algorithm and layouts.
 int ptdma_part_launch(float *A, const float *B, float *C, float *D, int n0, long ss0, int n1, long ss1,
-                      int n_row, long rs, bool cyclic, hipStream_t st);
+                      int n_row, long rs, bool cyclic, hipStream_t st, const LineOut &out = LineOut());
 // ADI sweep of a context: split solve when the context asks for it and the
 // line length allows, else the single-rank order.
 inline int ptdma_sweep(const tcs_ctx *c, double *A, const double *B, double *C, double *D, int n0, long ss0,

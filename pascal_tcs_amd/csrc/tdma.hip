@@ -351,6 +351,15 @@ extern "C" int ptdma_many_solve_host(double *A, double *B, double *C, double *D,
     return rc;
 }
 
+extern "C" int ptdma_single_solve_dev(double *A, const double *B, double *C, double *D, int32_t n_row,
+                                      int32_t cyclic, void *stream) {
+    return many_solve(A, B, C, D, 1, n_row, cyclic != 0, stream);
+}
+
+extern "C" int ptdma_single_solve_host(double *A, double *B, double *C, double *D, int32_t n_row, int32_t cyclic) {
+    return ptdma_many_solve_host(A, B, C, D, 1, n_row, cyclic);
+}
+
 extern "C" int ptdma_axis_solve_dev(double *A, const double *B, double *C, double *D, int32_t n1,
                                     int32_t n2, int32_t n3, int32_t axis, int32_t cyclic, void *stream) {
     if (!A || !B || !C || !D || n1 <= 0 || n2 <= 0 || n3 <= 0 || axis < 0 || axis > 2) {

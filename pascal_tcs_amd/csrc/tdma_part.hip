@@ -30,14 +30,9 @@
 //
 // Every operation follows the oracle's orc_ptdma_many_solve_split in order,
 // with FP contraction off, so results are bit-identical to it.
-#include "tcs_internal.h"
+#include "tdma_part.h"
 
 namespace tcs {
-
-constexpr int PT_MMAX = 16;               // rows per part
-constexpr int PT_PMAX = 32;               // parts per line (n_row <= 512)
-constexpr int PT_RS = 2 * PT_PMAX + 1;    // reduced-row stride in LDS (odd: no bank conflicts)
-constexpr int PT_SX = 16;                 // lines per workgroup, strided kernel
 
 bool ptdma_part_plan(int n_row, int &P, int &w1, int &w2) {
     if (n_row < 2 || n_row > PT_MMAX * PT_PMAX) return false;
@@ -50,176 +45,7 @@ bool ptdma_part_plan(int n_row, int &P, int &w1, int &w2) {
 
 namespace {
 
-__device__ __forceinline__ void part_range(int p, int w1, int w2, int &j0, int &m) {
-    j0 = p * w1 + (p < w2 ? p : w2);
-    m = w1 + (p < w2 ? 1 : 0);
-}
-
-// One part (rows 0..m-1, m >= 2) in registers: pascal_tdma.f90:501-536.
-// b is consumed; a, c, d come back as the reduced-form coefficients.
-template <typename T>
-__device__ __forceinline__ void part_reduce(T (&a)[PT_MMAX], T (&b)[PT_MMAX], T (&c)[PT_MMAX],
-                                            T (&d)[PT_MMAX], int m) {
-    a[0] = a[0] / b[0];
-    d[0] = d[0] / b[0];
-    c[0] = c[0] / b[0];
-    a[1] = a[1] / b[1];
-    d[1] = d[1] / b[1];
-    c[1] = c[1] / b[1];
-#pragma unroll
-    for (int r = 2; r < PT_MMAX; r++) {
-        if (r < m) {
-            const T rr = T(1) / (b[r] - a[r] * c[r - 1]);
-            d[r] = rr * (d[r] - a[r] * d[r - 1]);
-            c[r] = rr * c[r];
-            a[r] = -rr * a[r] * a[r - 1];
-        }
-    }
-#pragma unroll
-    for (int r = PT_MMAX - 3; r >= 1; r--) {
-        if (r <= m - 3) {
-            d[r] = d[r] - c[r] * d[r + 1];
-            a[r] = a[r] - c[r] * a[r + 1];
-            c[r] = -c[r] * c[r + 1];
-        }
-    }
-    const T rr = T(1) / (T(1) - a[1] * c[0]);
-    d[0] = rr * (d[0] - c[0] * d[1]);
-    a[0] = rr * a[0];
-    c[0] = -rr * c[0] * c[1];
-}
-
-// values of row m-1 (a runtime index) without dynamic register indexing
-template <typename T>
-__device__ __forceinline__ void part_last(const T (&a)[PT_MMAX], const T (&c)[PT_MMAX],
-                                          const T (&d)[PT_MMAX], int m, T &la, T &lc, T &ld) {
-    la = a[0]; lc = c[0]; ld = d[0];
-#pragma unroll
-    for (int r = 1; r < PT_MMAX; r++)
-        if (r == m - 1) { la = a[r]; lc = c[r]; ld = d[r]; }
-}
-
-// x(r) of the part from its reduced solution xf (row 0) and xl (row m-1):
-// pascal_tdma.f90:567-576
-template <typename T>
-__device__ __forceinline__ T part_x(const T (&a)[PT_MMAX], const T (&c)[PT_MMAX],
-                                         const T (&d)[PT_MMAX], int r, int m, T xf, T xl) {
-    if (r == 0) return xf;
-    if (r == m - 1) return xl;
-    return d[r] - a[r] * xf - c[r] * xl;
-}
-
-// Reduced system of one line in LDS (unit diagonal, n2 = 2P rows), solved
-// in place in ra/rc/rd by tdma_many (tdmas.f90:113-146) or tdma_cycl_many
-// (tdmas.f90:159-212; re is its e scratch).  Division by the unit diagonal
-// is exact and left out.
-template <typename T, bool CYC>
-__device__ void reduced_solve(T *ra, T *rc, T *rd, T *re, int n2) {
-    if (!CYC) {
-        for (int j = 1; j < n2; j++) {
-            const T r = T(1) / (T(1) - ra[j] * rc[j - 1]);
-            rd[j] = r * (rd[j] - ra[j] * rd[j - 1]);
-            rc[j] = r * rc[j];
-        }
-        for (int j = n2 - 2; j >= 0; j--) rd[j] = rd[j] - rc[j] * rd[j + 1];
-    } else {
-        for (int j = 0; j < n2; j++) re[j] = T(0);
-        re[1] = -ra[1];
-        re[n2 - 1] = -rc[n2 - 1];
-        for (int j = 2; j < n2; j++) {
-            const T rr = T(1) / (T(1) - ra[j] * rc[j - 1]);
-            rd[j] = rr * (rd[j] - ra[j] * rd[j - 1]);
-            re[j] = rr * (re[j] - ra[j] * re[j - 1]);
-            rc[j] = rr * rc[j];
-        }
-        for (int j = n2 - 2; j >= 1; j--) {
-            rd[j] = rd[j] - rc[j] * rd[j + 1];
-            re[j] = re[j] - rc[j] * re[j + 1];
-        }
-        rd[0] = (rd[0] - ra[0] * rd[n2 - 1] - rc[0] * rd[1]) / (T(1) + ra[0] * re[n2 - 1] + rc[0] * re[1]);
-        for (int j = 1; j < n2; j++) rd[j] = rd[j] + rd[0] * re[j];
-    }
-}
-
-// Reduced arrays of a workgroup: [q][line][row], q = a, c, d, e.
-template <typename T, int SX>
-struct RedLDS {
-    T v[4 * SX * PT_RS];
-    __device__ T *q(int which, int line) { return v + ((size_t)which * SX + line) * PT_RS; }
-};
-
-// After part_reduce: publish the part's two reduced rows (2p, 2p+1 of the
-// line's reduced system), solve the reduced system, return the part's
-// solution at its first (xf) and last (xl) row.
-//
-// One part (P = 1): the reference's serial tdma_many / tdma_cycl_many on the
-// 2-row system.  P >= 2 (a power of two): parallel cyclic reduction by the
-// line's P threads, two rows each.  Step with stride s: row i eliminates its
-// couplings to rows i-s and i+s with their current values,
-//   k1 = a_i/b_{i-s}, k2 = c_i/b_{i+s}, a_i' = -a_{i-s} k1, c_i' = -c_{i+s} k2,
-//   b_i' = b_i - c_{i-s} k1 - a_{i+s} k2, d_i' = d_i - d_{i-s} k1 - d_{i+s} k2;
-// outside a non-cyclic system the rows are identity rows.  Non-cyclic:
-// strides 1..N/2 (N = 2P) and x_i = d_i/b_i.  Cyclic: strides 1..N/4, then
-// row i couples only to j = i + N/2 and the pair is solved directly,
-// x_i = (d_i b_j - e_i d_j)/(b_i b_j - e_i e_j), e = a + c.
-template <typename T, bool CYC, int SX>
-__device__ __forceinline__ void part_exchange(RedLDS<T, SX> &red, int line, int p, int P, const T (&a)[PT_MMAX],
-                                              const T (&c)[PT_MMAX], const T (&d)[PT_MMAX], int m,
-                                              T &xf, T &xl) {
-    T la, lc, ld;
-    part_last(a, c, d, m, la, lc, ld);
-    T *ra = red.q(0, line), *rc = red.q(1, line), *rd = red.q(2, line), *rb = red.q(3, line);
-    ra[2 * p] = a[0]; rc[2 * p] = c[0]; rd[2 * p] = d[0];
-    ra[2 * p + 1] = la; rc[2 * p + 1] = lc; rd[2 * p + 1] = ld;
-    if (P == 1) {
-        __syncthreads();
-        if (p == 0) reduced_solve<T, CYC>(ra, rc, rd, rb, 2);
-        __syncthreads();
-        xf = rd[0];
-        xl = rd[1];
-        return;
-    }
-    rb[2 * p] = T(1); rb[2 * p + 1] = T(1);
-    __syncthreads();
-    const int N = 2 * P, lim = CYC ? N / 2 : N;
-    T va[2] = {a[0], la}, vb[2] = {T(1), T(1)}, vc[2] = {c[0], lc}, vd[2] = {d[0], ld};
-    for (int s = 1; s < lim; s <<= 1) {
-        T na[2], nb[2], nc[2], nd[2];
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            int im = 2 * p + h - s, ip = 2 * p + h + s;
-            T am = T(0), bm = T(1), cm = T(0), dm = T(0), ap = T(0), bp = T(1), cp = T(0), dp = T(0);
-            if (CYC) { im &= N - 1; ip &= N - 1; }
-            if (CYC || im >= 0) { am = ra[im]; bm = rb[im]; cm = rc[im]; dm = rd[im]; }
-            if (CYC || ip < N) { ap = ra[ip]; bp = rb[ip]; cp = rc[ip]; dp = rd[ip]; }
-            const T k1 = va[h] / bm, k2 = vc[h] / bp;
-            na[h] = -am * k1;
-            nc[h] = -cp * k2;
-            nb[h] = vb[h] - cm * k1 - ap * k2;
-            nd[h] = vd[h] - dm * k1 - dp * k2;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            ra[2 * p + h] = na[h]; rb[2 * p + h] = nb[h]; rc[2 * p + h] = nc[h]; rd[2 * p + h] = nd[h];
-            va[h] = na[h]; vb[h] = nb[h]; vc[h] = nc[h]; vd[h] = nd[h];
-        }
-        __syncthreads();
-    }
-    T x[2];
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-        if (CYC) {
-            const int j = (2 * p + h + P) & (N - 1);
-            const T ei = va[h] + vc[h], ej = ra[j] + rc[j];
-            x[h] = (vd[h] * rb[j] - ei * rd[j]) / (vb[h] * rb[j] - ei * ej);
-        } else {
-            x[h] = vd[h] / vb[h];
-        }
-    }
-    xf = x[0];
-    xl = x[1];
-}
+using namespace part;
 
 // Workgroup -> group of SX lines.  With SX = 8 two workgroups share every
 // 128-byte row segment of 16 lines; they are placed 8 apart in dispatch
@@ -237,36 +63,48 @@ __device__ __forceinline__ int line_group(int bx, int nbx) {
 template <typename T, bool CYC, int SX>
 __global__ void __launch_bounds__(SX * PT_PMAX)
 k_ptdma_part(const T *__restrict__ A, const T *__restrict__ B, const T *__restrict__ C,
-             T *__restrict__ D, int n0, long ss1, long rs, int P, int w1, int w2) {
-    __shared__ RedLDS<T, SX> red;
+             T *__restrict__ D, int n0, long ss1, long rs, int P, int w1, int w2, LineOut out) {
+    __shared__ RedLDS<T, SX, 1> red;
     const int t = threadIdx.x;
     const int line = t % SX, p = t / SX;
     const int s0 = line_group<SX>(blockIdx.x, gridDim.x) * SX + line;
     const bool act = s0 < n0;
     const long base = (long)s0 + (long)blockIdx.y * ss1;
     int j0, m;
-    part_range(p, w1, w2, j0, m);
+    range(p, w1, w2, j0, m);
 
-    T a[PT_MMAX], c[PT_MMAX], d[PT_MMAX];
+    T a[PT_MMAX], c[PT_MMAX], d[1][PT_MMAX];
     {
         T b[PT_MMAX];
 #pragma unroll
         for (int r = 0; r < PT_MMAX; r++) {
             if (act && r < m) {
                 const long o = base + (long)(j0 + r) * rs;
-                a[r] = A[o]; b[r] = B[o]; c[r] = C[o]; d[r] = D[o];
+                a[r] = A[o]; b[r] = B[o]; c[r] = C[o]; d[0][r] = D[o];
             } else {
-                a[r] = T(0); b[r] = T(1); c[r] = T(0); d[r] = T(0);
+                a[r] = T(0); b[r] = T(1); c[r] = T(0); d[0][r] = T(0);
             }
         }
-        part_reduce(a, b, c, d, m);
+        reduce<T, 1>(a, b, c, d, m);
     }
-    T xf, xl;
-    part_exchange<T, CYC, SX>(red, line, p, P, a, c, d, m, xf, xl);
+    T xf[1], xl[1];
+    exchange<T, CYC, SX, 1>(red, line, p, P, a, c, d, m, xf, xl);
     if (!act) return;
+    if (out.mode == 0) {
 #pragma unroll
-    for (int r = 0; r < PT_MMAX; r++)
-        if (r < m) D[base + (long)(j0 + r) * rs] = part_x(a, c, d, r, m, xf, xl);
+        for (int r = 0; r < PT_MMAX; r++)
+            if (r < m) D[base + (long)(j0 + r) * rs] = x_of<T, 1>(a, c, d, 0, r, m, xf[0], xl[0]);
+    } else {
+        const long fb = out.base + (long)s0 * out.ss0 + (long)blockIdx.y * out.ss1;
+#pragma unroll
+        for (int r = 0; r < PT_MMAX; r++) {
+            if (r < m) {
+                const double x = (double)x_of<T, 1>(a, c, d, 0, r, m, xf[0], xl[0]);
+                double *f = out.F + fb + (long)(j0 + r) * out.rs;
+                *f = out.mode == 1 ? x : *f + x;
+            }
+        }
+    }
 }
 
 // padded LDS position of row j of a staged line (one pad value per 16 rows)
@@ -286,18 +124,18 @@ k_ptdma_part_rows(const T *__restrict__ A, const T *__restrict__ B, const T *__r
                   int n0, long ss0, long ss1, int n_row, int ls, int P, int w1, int w2) {
     extern __shared__ __align__(16) unsigned char stage_raw[];
     T *stage = reinterpret_cast<T *>(stage_raw);
-    __shared__ RedLDS<T, PT_SX> red;
+    __shared__ RedLDS<T, PT_SX, 1> red;
     const int t = threadIdx.x;
     const int line = t % PT_SX, p = t / PT_SX;
     const int s0 = blockIdx.x * PT_SX + line;
     const bool act = s0 < n0;
     int j0, m;
-    part_range(p, w1, w2, j0, m);
+    range(p, w1, w2, j0, m);
     const long gbase = (long)blockIdx.x * PT_SX * ss0 + (long)blockIdx.y * ss1;
     const int nl = min(PT_SX, n0 - (int)blockIdx.x * PT_SX);  // lines present in this workgroup
     const bool row_ok = t < n_row;
 
-    T a[PT_MMAX], b[PT_MMAX], c[PT_MMAX], d[PT_MMAX];
+    T a[PT_MMAX], b[PT_MMAX], c[PT_MMAX], d[1][PT_MMAX];
     T v[PT_SX];
 #pragma unroll
     for (int l = 0; l < PT_SX; l++) v[l] = (l < nl && row_ok) ? A[gbase + l * ss0 + t] : T(0);
@@ -314,19 +152,19 @@ k_ptdma_part_rows(const T *__restrict__ A, const T *__restrict__ B, const T *__r
             for (int l = 0; l < PT_SX; l++) v[l] = (l < nl && row_ok) ? g[gbase + l * ss0 + t] : T(0);
         }
         __syncthreads();
-        T *dst = q == 0 ? a : (q == 1 ? b : (q == 2 ? c : d));
+        T *dst = q == 0 ? a : (q == 1 ? b : (q == 2 ? c : d[0]));
         const T pad = q == 1 ? T(1) : T(0);
 #pragma unroll
         for (int r = 0; r < PT_MMAX; r++) dst[r] = (act && r < m) ? stage[line * ls + stage_pos(j0 + r)] : pad;
     }
-    part_reduce(a, b, c, d, m);
-    T xf, xl;
-    part_exchange<T, CYC, PT_SX>(red, line, p, P, a, c, d, m, xf, xl);
-    // every thread has read the stage before part_exchange's first barrier
+    reduce<T, 1>(a, b, c, d, m);
+    T xf[1], xl[1];
+    exchange<T, CYC, PT_SX, 1>(red, line, p, P, a, c, d, m, xf, xl);
+    // every thread has read the stage before exchange's first barrier
     if (act) {
 #pragma unroll
         for (int r = 0; r < PT_MMAX; r++)
-            if (r < m) stage[line * ls + stage_pos(j0 + r)] = part_x(a, c, d, r, m, xf, xl);
+            if (r < m) stage[line * ls + stage_pos(j0 + r)] = x_of<T, 1>(a, c, d, 0, r, m, xf[0], xl[0]);
     }
     __syncthreads();
     if (row_ok) {
@@ -358,7 +196,7 @@ template <> struct PartLabels<float> {
 // Returns TCS_E_ARG when the line length is outside the partitioned range.
 template <typename T>
 static int part_launch(T *A, const T *B, T *C, T *D, int n0, long ss0, int n1, long ss1, int n_row, long rs,
-                       bool cyclic, hipStream_t st) {
+                       bool cyclic, hipStream_t st, const LineOut &out) {
     int P, w1, w2;
     if (n0 <= 0 || n1 <= 0) return TCS_OK;
     if (!ptdma_part_plan(n_row, P, w1, w2) || n1 > 65535) {
@@ -366,6 +204,10 @@ static int part_launch(T *A, const T *B, T *C, T *D, int n0, long ss0, int n1, l
         return TCS_E_ARG;
     }
     using L = PartLabels<T>;
+    if (rs == 1 && out.mode != 0) {
+        set_error("ptdma_part: field output is for strided lines only");
+        return TCS_E_ARG;
+    }
     if (rs == 1) {
         const int ls = n_row - 1 + ((n_row - 1) >> 4) + 1;
         const int lds = part_lds_bytes<T>(n_row);
@@ -393,20 +235,22 @@ static int part_launch(T *A, const T *B, T *C, T *D, int n0, long ss0, int n1, l
     }
     dim3 blk(PT_SX * P), grd((unsigned)((n0 + PT_SX - 1) / PT_SX), (unsigned)n1);
     if (cyclic)
-        TCS_LAUNCH(st, L::cyc, (k_ptdma_part<T, true, PT_SX>), grd, blk, 0, A, B, C, D, n0, ss1, rs, P, w1, w2);
+        TCS_LAUNCH(st, L::cyc, (k_ptdma_part<T, true, PT_SX>), grd, blk, 0, A, B, C, D, n0, ss1, rs, P, w1, w2,
+                   out);
     else
-        TCS_LAUNCH(st, L::plain, (k_ptdma_part<T, false, PT_SX>), grd, blk, 0, A, B, C, D, n0, ss1, rs, P, w1, w2);
+        TCS_LAUNCH(st, L::plain, (k_ptdma_part<T, false, PT_SX>), grd, blk, 0, A, B, C, D, n0, ss1, rs, P, w1, w2,
+                   out);
     return TCS_OK;
 }
 
 int ptdma_part_launch(double *A, const double *B, double *C, double *D, int n0, long ss0, int n1, long ss1,
-                      int n_row, long rs, bool cyclic, hipStream_t st) {
-    return part_launch<double>(A, B, C, D, n0, ss0, n1, ss1, n_row, rs, cyclic, st);
+                      int n_row, long rs, bool cyclic, hipStream_t st, const LineOut &out) {
+    return part_launch<double>(A, B, C, D, n0, ss0, n1, ss1, n_row, rs, cyclic, st, out);
 }
 
 int ptdma_part_launch(float *A, const float *B, float *C, float *D, int n0, long ss0, int n1, long ss1,
-                      int n_row, long rs, bool cyclic, hipStream_t st) {
-    return part_launch<float>(A, B, C, D, n0, ss0, n1, ss1, n_row, rs, cyclic, st);
+                      int n_row, long rs, bool cyclic, hipStream_t st, const LineOut &out) {
+    return part_launch<float>(A, B, C, D, n0, ss0, n1, ss1, n_row, rs, cyclic, st, out);
 }
 
 }  // namespace tcs

@@ -329,13 +329,16 @@ k_thermal_y(Grid g, Phys p, double dt, const double *__restrict__ T, const doubl
 }
 
 // T(i,j,k) += RHSJ (:518-524)
-__global__ void k_add_interior(Grid g, double *__restrict__ F, const scr_t *__restrict__ inc) {
+// F(i,j,k) = inc (MODE 1) or F(i,j,k) += inc (MODE 2) over interior rows j >= j0
+template <int MODE>
+__global__ void k_interior_out(Grid g, int j0, double *__restrict__ F, const scr_t *__restrict__ inc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
-    if (i > g.n1m || j > g.n2m) return;
+    if (i > g.n1m || j > g.n2m || j < j0) return;
     const size_t q = g.ix(i, j, k);
-    F[q] = F[q] + inc[g.r3(i, j, k)];
+    if (MODE == 1) F[q] = inc[g.r3(i, j, k)];
+    else F[q] = F[q] + inc[g.r3(i, j, k)];
 }
 
 // sweeps over the interior (i,j,k) layout
@@ -347,12 +350,33 @@ int sweep_x(tcs_ctx *c, bool cyc) {
     const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m;
     return ptdma_sweep(c, SCR(c->am), SCR(c->ac), SCR(c->ap), SCR(c->rhs), c->n2m, n1, c->n3m, n12, c->n1m, 1, cyc);
 }
-int sweep_y(tcs_ctx *c, int j0, int nrows) {
-    // rows j = j0..j0+nrows-1 (1-based)
+// y sweep of rows j = j0..j0+nrows-1 (1-based) whose solution ends the
+// stage: stored into (mode 1) or added to (mode 2) the field F's interior.
+// With split lines the solve writes F directly; otherwise a second kernel
+// moves the solution from the scratch.
+int sweep_y_field(tcs_ctx *c, int j0, int nrows, double *F, int mode) {
     const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m;
     const long off = (long)(j0 - 1) * n1;
-    return ptdma_sweep(c, SCR(c->am) + off, SCR(c->ac) + off, SCR(c->ap) + off, SCR(c->rhs) + off, c->n1m, 1, c->n3m, n12,
-                              nrows, n1, false);
+    int P, w1, w2;
+    if (c->tdma_split && ptdma_part_plan(nrows, P, w1, w2)) {
+        LineOut out;
+        out.F = F;
+        out.mode = mode;
+        out.base = 1 + (long)c->N1 * j0 + (long)c->N1 * c->N2;  // (i,j,k) = (1, j0, 1)
+        out.ss0 = 1;
+        out.ss1 = (long)c->N1 * c->N2;
+        out.rs = c->N1;
+        return ptdma_part_launch(SCR(c->am) + off, SCR(c->ac) + off, SCR(c->ap) + off, SCR(c->rhs) + off, c->n1m, 1,
+                                 c->n3m, n12, nrows, n1, false, c->stream, out);
+    }
+    TCS_TRY(ptdma_sweep(c, SCR(c->am) + off, SCR(c->ac) + off, SCR(c->ap) + off, SCR(c->rhs) + off, c->n1m, 1,
+                        c->n3m, n12, nrows, n1, false));
+    const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
+    if (mode == 1)
+        TCS_LAUNCH(c->stream, "store_interior", k_interior_out<1>, grd, blk, 0, c->grid(), j0, F, SCR(c->rhs));
+    else
+        TCS_LAUNCH(c->stream, "add_interior", k_interior_out<2>, grd, blk, 0, c->grid(), j0, F, SCR(c->rhs));
+    return TCS_OK;
 }
 
 int thermal_coeffi(tcs_ctx *c) {
@@ -373,8 +397,7 @@ int thermal_solver(tcs_ctx *c, double dt) {
     TCS_TRY(sweep_x(c, true));
     TCS_LAUNCH(c->stream, "thermal_y", k_thermal_y, grd, blk, 0, g, c->phys, dt, c->f[TCS_T], c->f[TCS_V], SCR(c->am),
                        SCR(c->ac), SCR(c->ap));
-    TCS_TRY(sweep_y(c, 1, c->n2m));
-    TCS_LAUNCH(c->stream, "add_interior", k_add_interior, grd, blk, 0, g, c->f[TCS_T], SCR(c->rhs));
+    TCS_TRY(sweep_y_field(c, 1, c->n2m, c->f[TCS_T], 2));
     return TCS_OK;
 }
 

@@ -143,6 +143,11 @@ module tcs_capi
             real(c_double) :: A(*), B(*), C(*), D(*)
             integer(c_int32_t), value :: n_sys, n_row, cyclic
         end function
+        integer(c_int) function ptdma_single_solve_host(A, B, C, D, n_row, cyclic) bind(C, name='ptdma_single_solve_host')
+            import :: c_int, c_int32_t, c_double
+            real(c_double) :: A(*), B(*), C(*), D(*)
+            integer(c_int32_t), value :: n_row, cyclic
+        end function
     end interface
 
     type(c_ptr), save :: tcs_ctx = c_null_ptr

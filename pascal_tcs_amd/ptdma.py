@@ -1,5 +1,6 @@
-"""Host mirror of module PaScaL_TDMA's many-system API
-(PaScaL_TDMA/src/pascal_tdma.f90:68-127, :185-290, :487-578, :715-806).
+"""Host mirror of module PaScaL_TDMA: the many-system API
+(PaScaL_TDMA/src/pascal_tdma.f90:68-127, :185-290, :487-578, :715-806) and
+the single-system API (:139-175, :301-475).
 
 Arrays follow the Fortran (1:n_sys,1:n_row) layout with the system index
 fastest; in numpy/torch terms that is a C-contiguous array of shape
@@ -53,6 +54,51 @@ def _solve(plan, A, B, C, D, n_sys, n_row, cyclic):
     fn = lib.ptdma_many_solve_cycle_dev if cyclic else lib.ptdma_many_solve_dev
     _lib.check(fn(A.data_ptr(), B.data_ptr(), C.data_ptr(), D.data_ptr(), n_sys, n_row, stream),
                "ptdma_many_solve")
+
+
+class PtdmaPlanSingle:
+    """type ptdma_plan_single on one rank: n_row_rt = 2, gather rank 0."""
+
+    def __init__(self, myrank=0, nprocs=1, mpi_world=None, gather_rank=0):
+        if nprocs != 1 or myrank != 0 or gather_rank != 0:
+            raise ValueError("PaScaL_TDMA plans here are single-rank (nprocs == 1)")
+        self.myrank, self.gather_rank, self.ptdma_world, self.n_row_rt = 0, 0, mpi_world, 2
+
+
+def PaScaL_TDMA_plan_single_create(myrank=0, nprocs=1, mpi_world=None, gather_rank=0):
+    return PtdmaPlanSingle(myrank, nprocs, mpi_world, gather_rank)
+
+
+def PaScaL_TDMA_plan_single_destroy(plan):
+    plan.n_row_rt = 0
+
+
+def _single(A, B, C, D, n_row, cyclic):
+    lib = _lib.load()
+    if isinstance(A, np.ndarray):
+        for x in (A, B, C, D):
+            if x.dtype != np.float64 or not x.flags["C_CONTIGUOUS"] or x.size != n_row:
+                raise ValueError("expected C-contiguous float64 arrays of n_row values")
+        p = [x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for x in (A, B, C, D)]
+        _lib.check(lib.ptdma_single_solve_host(*p, n_row, int(cyclic)), "ptdma_single_solve_host")
+        return
+    import torch
+    for x in (A, B, C, D):
+        if not (x.is_cuda and x.dtype == torch.float64 and x.is_contiguous() and x.numel() == n_row):
+            raise ValueError("expected contiguous float64 CUDA tensors of n_row values")
+    stream = torch.cuda.current_stream().cuda_stream
+    _lib.check(lib.ptdma_single_solve_dev(A.data_ptr(), B.data_ptr(), C.data_ptr(), D.data_ptr(), n_row,
+                                          int(cyclic), stream), "ptdma_single_solve_dev")
+
+
+def PaScaL_TDMA_single_solve(plan, A, B, C, D, n_row):
+    """pascal_tdma.f90:301-382 on one rank (in place; D returns the solution)."""
+    _single(A, B, C, D, n_row, False)
+
+
+def PaScaL_TDMA_single_solve_cycle(plan, A, B, C, D, n_row):
+    """pascal_tdma.f90:393-475 on one rank (in place; D returns the solution)."""
+    _single(A, B, C, D, n_row, True)
 
 
 def PaScaL_TDMA_many_solve(plan, A, B, C, D, n_sys, n_row):

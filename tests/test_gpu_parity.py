@@ -88,9 +88,10 @@ def test_pseudoupdate(shape):
     _compare(o, s)
 
 
-@pytest.mark.parametrize("shape", SHAPES)
-def test_pressure_rhs_poisson(shape):
-    cfg, o, s = _pair(shape)
+@pytest.mark.parametrize("shape", SHAPES + [(32, 40, 16), (16, 100, 8)])
+@pytest.mark.parametrize("split", [0, 1])
+def test_pressure_rhs_poisson(shape, split):
+    cfg, o, s = _pair(shape, split=split)
     o.momentum_coeffi()
     s.mpi_momentum_coeffi()
     prhs = o.pressure_rhs(cfg.dtStart)

@@ -209,3 +209,26 @@ def test_split_axis_solve_512_cube(axis, cyclic):
         sys_ = [t[tuple(sl)].cpu().numpy().reshape(n, 1).copy() for t in (a, b, c, d)]
         O.ptdma_many_solve_split(*sys_, cyclic=cyclic)
         assert np.array_equal(D[tuple(sl)].cpu().numpy(), sys_[3][:, 0])
+
+
+# ---- single-system API (pascal_tdma.f90:301-475): the n_sys = 1 solve ----
+
+@pytest.mark.parametrize("n_row", [2, 3, 17, 512, 4099])
+@pytest.mark.parametrize("cyclic", [False, True])
+def test_single_solve_host_and_device(n_row, cyclic):
+    import torch
+    from pascal_tcs_amd import ptdma
+    rng = np.random.default_rng(n_row + cyclic)
+    a, b, c, d = (x[:, 0].copy() for x in dominant(rng, (n_row, 1)))
+    ref = [x.reshape(n_row, 1).copy() for x in (a, b, c, d)]
+    O.ptdma_many_solve(*ref, cyclic=cyclic)
+    plan = ptdma.PaScaL_TDMA_plan_single_create()
+    f = ptdma.PaScaL_TDMA_single_solve_cycle if cyclic else ptdma.PaScaL_TDMA_single_solve
+    host = [x.copy() for x in (a, b, c, d)]
+    f(plan, *host, n_row)
+    dev = _torch((a, b, c, d))
+    f(plan, *dev, n_row)
+    torch.cuda.synchronize()
+    for k in (0, 2, 3):  # A, C, D as the reference leaves them
+        assert np.array_equal(host[k], ref[k][:, 0])
+        assert np.array_equal(dev[k].cpu().numpy(), ref[k][:, 0])

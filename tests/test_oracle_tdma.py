@@ -258,3 +258,30 @@ def test_split_solve_one_part_is_single_rank(cyclic, n_row):
 def test_split_parts_policy():
     assert [O.tdma_parts(n) for n in (2, 16, 17, 32, 33, 48, 64, 65, 100, 256, 257, 511, 512, 513)] == \
         [1, 1, 2, 2, 4, 4, 4, 8, 8, 16, 32, 32, 32, 1]
+
+
+@pytest.mark.parametrize("cyclic", [False, True])
+@pytest.mark.parametrize("n", [2, 3, 8, 64])
+def test_single_reduced_solve_equals_many_with_one_system(cyclic, n):
+    """The single-system API's reduced solve (tdma_single / tdma_cycl_single)
+    does the same operations as tdma_many / tdma_cycl_many on one system, so
+    the single-system solve on one rank is the many-system solve with
+    n_sys = 1.  Checked on the reference's compiled tdmas.f90 when present,
+    and on the oracle's restatement."""
+    rng = np.random.default_rng(n + 11 * cyclic)
+    a, b, c, d = [x[:, 0].copy() for x in dominant(rng, n, 1)]
+    R = O.ref_lib()
+    if R is not None:
+        one = [x.copy() for x in (a, b, c, d)]
+        many = [x.copy() for x in (a, b, c, d)]
+        _ref_call(R.tdma_cycl_single_ if cyclic else R.tdma_single_, one, n)
+        _ref_call(R.tdma_cycl_many_ if cyclic else R.tdma_many_, many, 1, n)
+        for u, v in zip(one, many):
+            assert np.array_equal(u, v)
+    one = [x.copy() for x in (a, b, c, d)]
+    many = [x.reshape(n, 1).copy() for x in (a, b, c, d)]
+    f = O.lib().orc_tdma_cycl_single if cyclic else O.lib().orc_tdma_single
+    f(*[x.ctypes.data_as(O._dp) for x in one], n)
+    O.tdma_many(*many, cyclic=cyclic)
+    for u, v in zip(one, many):
+        assert np.array_equal(u, v[:, 0])
