@@ -127,6 +127,19 @@ def run_gpu(args, precision):
     return cfg, sec, rep, state, nu, fields
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_step512_fp64.json")
+
+
+def measured_traffic(label, cfg):
+    """HBM bytes per launch of a kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (512^3 workload only), or None."""
+    if (cfg.n1m, cfg.n2m, cfg.n3m) != (512, 512, 512) or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as fh:
+        t = json.load(fh)["bytes_per_launch"].get(label)
+    return None if t is None else t["read_bytes"] + t["write_bytes"]
+
+
 def roofline(cfg, rep, psteps):
     kernels = {k: v for k, v in rep.items() if kernel_bytes(k, cfg) is not None}
     if not kernels:
@@ -136,7 +149,7 @@ def roofline(cfg, rep, psteps):
     avg_s = ms / cnt / 1e3
     ach = kernel_bytes(dom, cfg) / avg_s / 1e9
     info = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, cfg),
             "kernel": dom, "avg_launch_ms": round(ms / cnt, 4), "launches_per_step": cnt // max(psteps, 1),
             "bytes_per_launch": kernel_bytes(dom, cfg)}
     return dom, info

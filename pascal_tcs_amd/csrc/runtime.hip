@@ -145,22 +145,23 @@ __global__ void __launch_bounds__(256)
 k_post_max(Grid g, const double *__restrict__ U, const double *__restrict__ V, const double *__restrict__ W,
            double *__restrict__ partial) {
     __shared__ double s[256];
-    const size_t n = (size_t)g.n1m * g.n2m * g.n3m;
+    const int nrows = g.n2m * g.n3m;
     double m = -INFINITY;
-    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
-        const int i = (int)(q % g.n1m) + 1;
-        const size_t r = q / g.n1m;
-        const int j = (int)(r % g.n2m) + 1, k = (int)(r / g.n2m) + 1;
-        double v;
-        if (MODE == 0)
-            v = (U[g.ix(i + 1, j, k)] - U[g.ix(i, j, k)]) / g.dx1[i]
-              + (V[g.ix(i, j + 1, k)] - V[g.ix(i, j, k)]) / g.dx2[j]
-              + (W[g.ix(i, j, k + 1)] - W[g.ix(i, j, k)]) / g.dx3[k];
-        else
-            v = (fabs(U[g.ix(i + 1, j, k)] + U[g.ix(i, j, k)]) / 2.0) / g.dx1[i]
-              + (fabs(V[g.ix(i, j + 1, k)] + V[g.ix(i, j, k)]) / 2.0) / g.dx2[j]
-              + (fabs(W[g.ix(i, j, k + 1)] + W[g.ix(i, j, k)]) / 2.0) / g.dx3[k];
-        m = fmax(m, v);
+    // x-rows (j,k) spread over the workgroups, i over the threads (coalesced)
+    for (int r = blockIdx.x; r < nrows; r += gridDim.x) {
+        const int j = r % g.n2m + 1, k = r / g.n2m + 1;
+        for (int i = threadIdx.x + 1; i <= g.n1m; i += blockDim.x) {
+            double v;
+            if (MODE == 0)
+                v = (U[g.ix(i + 1, j, k)] - U[g.ix(i, j, k)]) / g.dx1[i]
+                  + (V[g.ix(i, j + 1, k)] - V[g.ix(i, j, k)]) / g.dx2[j]
+                  + (W[g.ix(i, j, k + 1)] - W[g.ix(i, j, k)]) / g.dx3[k];
+            else
+                v = (fabs(U[g.ix(i + 1, j, k)] + U[g.ix(i, j, k)]) / 2.0) / g.dx1[i]
+                  + (fabs(V[g.ix(i, j + 1, k)] + V[g.ix(i, j, k)]) / 2.0) / g.dx2[j]
+                  + (fabs(W[g.ix(i, j, k + 1)] + W[g.ix(i, j, k)]) / 2.0) / g.dx3[k];
+            m = fmax(m, v);
+        }
     }
     s[threadIdx.x] = m;
     __syncthreads();
