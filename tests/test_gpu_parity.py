@@ -120,9 +120,10 @@ def test_post_div_cfl(shape):
     assert (cfl_g, dt_g) == (cfl_o, dt_o)
 
 
-@pytest.mark.parametrize("shape", [(16, 12, 8), (32, 20, 16)])
+@pytest.mark.parametrize("shape", [(16, 12, 8), (32, 20, 16), (128, 256, 8)])
 def test_full_step_seeded(shape, split=0):
-    """Three steps of the whole time step from a seeded conduction state."""
+    """Three steps of the whole time step from a seeded conduction state
+    (the last shape with lines of 128 and 256 rows)."""
     from pascal_tcs_amd.solver import TCSSolver
     cfg = small_config(*shape)
     s = TCSSolver(cfg, tdma_split=split)
