@@ -69,12 +69,19 @@ def kernel_bytes(label, cfg):
         # Poisson y solves: complex right-hand side in and solution out per (kx, kz, j)
         "poisson_part": 32.0 * (n1 // 2 + 1) * n3 * n2,
         "thermal_z": 8.0 * N * (4 + 4), "thermal_x": 8.0 * N * (2 + 3), "thermal_y": 8.0 * N * (2 + 3),
-        "dU_z": 8.0 * N * (6 + 4), "dV_z": 8.0 * N * (6 + 4) * fv, "dW_z": 8.0 * N * (7 + 4),
+        "dU_z": 8.0 * N * (5 + 4), "dV_z": 8.0 * N * (5 + 4) * fv, "dW_z": 8.0 * N * (6 + 4),
         "dU_x": 8.0 * N * (1 + 3), "dU_y": 8.0 * N * (1 + 3), "dV_x": 8.0 * N * (1 + 3) * fv,
         "dV_y": 8.0 * N * (1 + 3) * fv, "dW_x": 8.0 * N * (1 + 3), "dW_y": 8.0 * N * (1 + 3),
         "blockLdV": 8.0 * N * (3 + 1) * fv, "blockLdU": 8.0 * N * (4 + 1),
         "pseudoupdate": 8.0 * N * 9, "projection": 8.0 * N * (6 + 4), "dphat_update": 8.0 * N * 5,
         "pressure_rhs": 8.0 * N * (4 + 1), "add_interior": 8.0 * N * 3, "store_interior": 8.0 * N * 2,
+        # split line solves that compute their rows: the stencil's fields in,
+        # the solution out (z: into the scratch; y: the scratch's right-hand
+        # side in, the increment field out)
+        "thermal_z_solve": 8.0 * N * (4 + 1), "dU_z_solve": 8.0 * N * (5 + 1),
+        "dV_z_solve": 8.0 * N * (5 + 1) * fv, "dW_z_solve": 8.0 * N * (6 + 1),
+        "thermal_y_solve": 8.0 * N * (3 + 1), "dU_y_solve": 8.0 * N * (2 + 1),
+        "dV_y_solve": 8.0 * N * (2 + 1) * fv, "dW_y_solve": 8.0 * N * (2 + 1),
     }
     return table.get(label)
 

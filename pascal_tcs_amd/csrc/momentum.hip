@@ -9,7 +9,7 @@
 // Stage-2/3 quirks reproduced: dV x-stage uses mu1,mu2 of stage 1's last
 // point (:821-826); dW x-stage mu1,mu2 (:1177-1182); dW y-stage mu3,mu4 of
 // stage 1's last point and km = n3m-1 from the x-stage's last k (:1216-1246).
-#include "tcs_internal.h"
+#include "stage_rows.h"
 
 #ifndef TCS_FORM
 #define TCS_FORM exact
@@ -26,8 +26,6 @@ namespace TCS_FORM {
 using scr_t = TCS_SCRATCH;
 #define SCR(p) (reinterpret_cast<scr_t *>(p))
 
-int sweep_z(tcs_ctx *c, bool cyc);
-int sweep_x(tcs_ctx *c, bool cyc);
 int sweep_y_field(tcs_ctx *c, int j0, int nrows, double *F, int mode);
 
 #define MU(j, k) Mu[g.t2((j), (k))]
@@ -87,14 +85,24 @@ __global__ void k_momentum_stale(Grid g, const double *__restrict__ Mu, double *
     const double *dmx1 = g.dmx1, *dmx2 = g.dmx2, *dmx3 = g.dmx3;              \
     (void)dx1; (void)dx2; (void)dx3; (void)dmx1; (void)dmx2; (void)dmx3;
 
+// the metric arrays under their reference names, in a row routine
+#define ROW_METRICS                                                           \
+    const double *dx1 = g.dx1, *dx2 = g.dx2, *dx3 = g.dx3;                    \
+    const double *dmx1 = g.dmx1, *dmx2 = g.dmx2, *dmx3 = g.dmx3;              \
+    (void)dx1; (void)dx2; (void)dx3; (void)dmx1; (void)dmx2; (void)dmx3;
+
+// Each stage's row routine gives row (i, j, k) of the stage's tridiagonal
+// systems (am, ac, ap: lower, diagonal, upper) and, in the first stage, the
+// right-hand side; the functor after it carries its arguments.
+
 // ---------------------------------------------------------------- dU -------
-__global__ void __launch_bounds__(256)
-k_dU_z(Grid g, Phys p, double dt, const double *__restrict__ U, const double *__restrict__ V,
-       const double *__restrict__ W, const double *__restrict__ P, const double *__restrict__ T,
-       const double *__restrict__ Mu, const double *__restrict__ iR, const double *__restrict__ UBCup,
-       const double *__restrict__ UBCbt, const double *__restrict__ VBCup, const double *__restrict__ VBCbt,
-       scr_t *__restrict__ am, scr_t *__restrict__ ac, scr_t *__restrict__ ap, scr_t *__restrict__ rhsout) {
-    KERNEL_IJK
+__device__ __forceinline__ void dU_z_row(const Grid &g, const Phys &p, double dt, const double *__restrict__ U,
+    const double *__restrict__ V, const double *__restrict__ W, const double *__restrict__ P,
+    const double *__restrict__ T, const double *__restrict__ Mu, const double *__restrict__ iR,
+    const double *__restrict__ UBCup, const double *__restrict__ UBCbt, const double *__restrict__ VBCup,
+    const double *__restrict__ VBCbt,
+    int i, int j, int k, double &am, double &ac, double &ap, double &rhsout) {
+    ROW_METRICS
     const int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1, km = k - 1, kp = k + 1;
     const int jum = g.jC[jm], jup = g.jC[jp];
     const double Cmu = p.Cmu;
@@ -174,17 +182,26 @@ k_dU_z(Grid g, Phys p, double dt, const double *__restrict__ U, const double *__
                  - invRhocCmu_half / dx2[j] * (mu4 * dvdx4 * (double)jup - mu3 * dvdx3 * (double)jum));
     rhs = rhs - (0.25 * (w6 * dudz6 + w5 * dudz5) - invRhocCmu_half / dx3[k] * (mu6 * dwdx6 - mu5 * dwdx5));
 
-    const size_t q = g.r3(i, j, k);
-    ac[q] = mACK * dt + 1.0;
-    ap[q] = mAPK * dt;
-    am[q] = mAMK * dt;
-    rhsout[q] = rhs * dt;
+    ac = mACK * dt + 1.0;
+    ap = mAPK * dt;
+    am = mAMK * dt;
+    rhsout = rhs * dt;
 }
+struct DUZ {
+    Grid g;
+    Phys p;
+    double dt;
+    const double *U, *V, *W, *P, *T, *Mu, *iR, *UBCup, *UBCbt, *VBCup, *VBCbt;
+    __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
+                                               double &d) const {
+        dU_z_row(g, p, dt, U, V, W, P, T, Mu, iR, UBCup, UBCbt, VBCup, VBCbt, i, j, k, a, b, c, d);
+    }
+};
 
-__global__ void __launch_bounds__(256)
-k_dU_x(Grid g, Phys p, double dt, const double *__restrict__ U, const double *__restrict__ Mu,
-       const double *__restrict__ iR, scr_t *__restrict__ am, scr_t *__restrict__ ac, scr_t *__restrict__ ap) {
-    KERNEL_IJK
+__device__ __forceinline__ void dU_x_row(const Grid &g, const Phys &p, double dt, const double *__restrict__ U,
+    const double *__restrict__ Mu, const double *__restrict__ iR,
+    int i, int j, int k, double &am, double &ac, double &ap) {
+    ROW_METRICS
     const int im = i - 1, ip = i + 1;
     const double u1 = 0.5 * (Uf(im, j, k) + Uf(i, j, k));
     const double u2 = 0.5 * (Uf(ip, j, k) + Uf(i, j, k));
@@ -196,16 +213,25 @@ k_dU_x(Grid g, Phys p, double dt, const double *__restrict__ U, const double *__
                       + 0.25 * (dudx2 * 0.5 + dudx1 * 0.5 - u2 / dx1[i] + u1 / dx1[im]);
     const double mAPI = -invRhocCmu_half / dmx1[i] * MU(j, k) / dx1[i] * 2.0 + 0.25 * (u2 / dx1[i] + dudx2 * 0.5);
     const double mAMI = -invRhocCmu_half / dmx1[i] * MU(j, k) / dx1[im] * 2.0 + 0.25 * (-u1 / dx1[im] + dudx1 * 0.5);
-    const size_t q = g.r3(i, j, k);
-    ac[q] = mACI * dt + 1.0;
-    ap[q] = mAPI * dt;
-    am[q] = mAMI * dt;
+    ac = mACI * dt + 1.0;
+    ap = mAPI * dt;
+    am = mAMI * dt;
 }
+struct DUX {
+    Grid g;
+    Phys p;
+    double dt;
+    const double *U, *Mu, *iR;
+    __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
+                                               double &d) const {
+        dU_x_row(g, p, dt, U, Mu, iR, i, j, k, a, b, c);
+    }
+};
 
-__global__ void __launch_bounds__(256)
-k_dU_y(Grid g, Phys p, double dt, const double *__restrict__ V, const double *__restrict__ Mu,
-       const double *__restrict__ iR, scr_t *__restrict__ am, scr_t *__restrict__ ac, scr_t *__restrict__ ap) {
-    KERNEL_IJK
+__device__ __forceinline__ void dU_y_row(const Grid &g, const Phys &p, double dt, const double *__restrict__ V,
+    const double *__restrict__ Mu, const double *__restrict__ iR,
+    int i, int j, int k, double &am, double &ac, double &ap) {
+    ROW_METRICS
     const int im = i - 1, jm = j - 1, jp = j + 1;
     const int jum = g.jC[jm], jup = g.jC[jp];
     const double v3 = 0.5 * (dx1[im] * Vf(i, j, k) + dx1[i] * Vf(im, j, k)) / dmx1[i];
@@ -222,11 +248,20 @@ k_dU_y(Grid g, Phys p, double dt, const double *__restrict__ V, const double *__
     double mAMJ = -invRhocCmu_half / dx2[j] * mu3 / dmx2[j] + 0.25 * (-v3 / dmx2[j]);
     mAPJ = mAPJ * (double)jup;
     mAMJ = mAMJ * (double)jum;
-    const size_t q = g.r3(i, j, k);
-    ac[q] = mACJ * dt + 1.0;
-    ap[q] = mAPJ * dt;
-    am[q] = mAMJ * dt;
+    ac = mACJ * dt + 1.0;
+    ap = mAPJ * dt;
+    am = mAMJ * dt;
 }
+struct DUY {
+    Grid g;
+    Phys p;
+    double dt;
+    const double *V, *Mu, *iR;
+    __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
+                                               double &d) const {
+        dU_y_row(g, p, dt, V, Mu, iR, i, j, k, a, b, c);
+    }
+};
 
 // interior copy: F(i,j,k) = RHS for j in [j0, n2m]
 // dV(0:n1sub,0:1,0:n3sub) = 0 and dV(:,n2sub,:) = 0 (:907-909)
@@ -240,14 +275,12 @@ __global__ void k_dV_zero_rows(Grid g, double *__restrict__ dV) {
 }
 
 // ---------------------------------------------------------------- dV -------
-__global__ void __launch_bounds__(256)
-k_dV_z(Grid g, Phys p, double dt, const double *__restrict__ U, const double *__restrict__ V,
-       const double *__restrict__ W, const double *__restrict__ P, const double *__restrict__ dU,
-       const double *__restrict__ Mu, const double *__restrict__ iR, const double *__restrict__ VBCup,
-       const double *__restrict__ VBCbt, scr_t *__restrict__ am, scr_t *__restrict__ ac, scr_t *__restrict__ ap,
-       scr_t *__restrict__ rhsout) {
-    KERNEL_IJK
-    if (j < 2) return;  // j starts at j_indexS = 2
+__device__ __forceinline__ void dV_z_row(const Grid &g, const Phys &p, double dt, const double *__restrict__ U,
+    const double *__restrict__ V, const double *__restrict__ W, const double *__restrict__ P,
+    const double *__restrict__ dU, const double *__restrict__ Mu, const double *__restrict__ iR,
+    const double *__restrict__ VBCup, const double *__restrict__ VBCbt,
+    int i, int j, int k, double &am, double &ac, double &ap, double &rhsout) {
+    ROW_METRICS
     const int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1, km = k - 1, kp = k + 1;
     const int jvp = g.jS[jm], jvm = g.jS[jp];
     const double Cmu = p.Cmu;
@@ -324,18 +357,26 @@ k_dV_z(Grid g, Phys p, double dt, const double *__restrict__ U, const double *__
     rhs = rhs - (0.25 * (w6 * dvdz6 + w5 * dvdz5) - invRhocCmu_half / dx3[k] * (mu6 * dwdy6 - mu5 * dwdy5));
     rhs = rhs - (0.25 * (ddu2 * dvdx2 + ddu1 * dvdx1) - invRhocCmu_half / dx1[i] * (mu2 * ddudy2 - mu1 * ddudy1));
 
-    const size_t q = g.r3(i, j, k);
-    ac[q] = mACK * dt + 1.0;
-    ap[q] = mAPK * dt;
-    am[q] = mAMK * dt;
-    rhsout[q] = rhs * dt;
+    ac = mACK * dt + 1.0;
+    ap = mAPK * dt;
+    am = mAMK * dt;
+    rhsout = rhs * dt;
 }
+struct DVZ {
+    Grid g;
+    Phys p;
+    double dt;
+    const double *U, *V, *W, *P, *dU, *Mu, *iR, *VBCup, *VBCbt;
+    __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
+                                               double &d) const {
+        dV_z_row(g, p, dt, U, V, W, P, dU, Mu, iR, VBCup, VBCbt, i, j, k, a, b, c, d);
+    }
+};
 
-__global__ void __launch_bounds__(256)
-k_dV_x(Grid g, Phys p, double dt, const double *__restrict__ U, const double *__restrict__ iR,
-       const double *__restrict__ stale, scr_t *__restrict__ am, scr_t *__restrict__ ac, scr_t *__restrict__ ap) {
-    KERNEL_IJK
-    if (j < 2) return;
+__device__ __forceinline__ void dV_x_row(const Grid &g, const Phys &p, double dt, const double *__restrict__ U,
+    const double *__restrict__ iR, const double *__restrict__ stale,
+    int i, int j, int k, double &am, double &ac, double &ap) {
+    ROW_METRICS
     const int ip = i + 1, jm = j - 1;
     const double mu1 = stale[0], mu2 = stale[1];
     const double u1 = (dx2[jm] * Uf(i, j, k) + dx2[j] * Uf(i, jm, k)) / dmx2[j] * 0.5;
@@ -345,17 +386,25 @@ k_dV_x(Grid g, Phys p, double dt, const double *__restrict__ U, const double *__
     const double mACI = invRhocCmu_half / dx1[i] * (mu2 / dmx1[ip] + mu1 / dmx1[i]) + 0.25 * (-u2 / dmx1[ip] + u1 / dmx1[i]);
     const double mAPI = -invRhocCmu_half / dx1[i] * mu2 / dmx1[ip] + 0.25 * (u2 / dmx1[ip]);
     const double mAMI = -invRhocCmu_half / dx1[i] * mu1 / dmx1[i] + 0.25 * (-u1 / dmx1[i]);
-    const size_t q = g.r3(i, j, k);
-    ac[q] = mACI * dt + 1.0;
-    ap[q] = mAPI * dt;
-    am[q] = mAMI * dt;
+    ac = mACI * dt + 1.0;
+    ap = mAPI * dt;
+    am = mAMI * dt;
 }
+struct DVX {
+    Grid g;
+    Phys p;
+    double dt;
+    const double *U, *iR, *stale;
+    __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
+                                               double &d) const {
+        dV_x_row(g, p, dt, U, iR, stale, i, j, k, a, b, c);
+    }
+};
 
-__global__ void __launch_bounds__(256)
-k_dV_y(Grid g, Phys p, double dt, const double *__restrict__ V, const double *__restrict__ Mu,
-       const double *__restrict__ iR, scr_t *__restrict__ am, scr_t *__restrict__ ac, scr_t *__restrict__ ap) {
-    KERNEL_IJK
-    if (j < 2) return;
+__device__ __forceinline__ void dV_y_row(const Grid &g, const Phys &p, double dt, const double *__restrict__ V,
+    const double *__restrict__ Mu, const double *__restrict__ iR,
+    int i, int j, int k, double &am, double &ac, double &ap) {
+    ROW_METRICS
     const int jm = j - 1, jp = j + 1;
     const int jvp = g.jS[jm], jvm = g.jS[jp];
     const double v3 = 0.5 * (Vf(i, jm, k) + Vf(i, j, k));
@@ -370,21 +419,29 @@ k_dV_y(Grid g, Phys p, double dt, const double *__restrict__ V, const double *__
     mAPJ = mAPJ * (double)jvp;
     double mAMJ = -invRhocCmu_half / dmx2[j] * MU(jm, k) / dx2[jm] * 2.0 + 0.25 * (-v3 / dx2[jm] + dvdy3 * 0.5);
     mAMJ = mAMJ * (double)jvm;
-    const size_t q = g.r3(i, j, k);
-    ac[q] = mACJ * dt + 1.0;
-    ap[q] = mAPJ * dt;
-    am[q] = mAMJ * dt;
+    ac = mACJ * dt + 1.0;
+    ap = mAPJ * dt;
+    am = mAMJ * dt;
 }
+struct DVY {
+    Grid g;
+    Phys p;
+    double dt;
+    const double *V, *Mu, *iR;
+    __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
+                                               double &d) const {
+        dV_y_row(g, p, dt, V, Mu, iR, i, j, k, a, b, c);
+    }
+};
 
 // ---------------------------------------------------------------- dW -------
-__global__ void __launch_bounds__(256)
-k_dW_z(Grid g, Phys p, double dt, const double *__restrict__ U, const double *__restrict__ V,
-       const double *__restrict__ W, const double *__restrict__ P, const double *__restrict__ dU,
-       const double *__restrict__ dV, const double *__restrict__ Mu, const double *__restrict__ iR,
-       const double *__restrict__ VBCup, const double *__restrict__ VBCbt, const double *__restrict__ WBCup,
-       const double *__restrict__ WBCbt, scr_t *__restrict__ am, scr_t *__restrict__ ac, scr_t *__restrict__ ap,
-       scr_t *__restrict__ rhsout) {
-    KERNEL_IJK
+__device__ __forceinline__ void dW_z_row(const Grid &g, const Phys &p, double dt, const double *__restrict__ U,
+    const double *__restrict__ V, const double *__restrict__ W, const double *__restrict__ P,
+    const double *__restrict__ dU, const double *__restrict__ dV, const double *__restrict__ Mu,
+    const double *__restrict__ iR, const double *__restrict__ VBCup, const double *__restrict__ VBCbt,
+    const double *__restrict__ WBCup, const double *__restrict__ WBCbt,
+    int i, int j, int k, double &am, double &ac, double &ap, double &rhsout) {
+    ROW_METRICS
     const int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1, km = k - 1, kp = k + 1;
     const int jwp = g.jC[jp], jwm = g.jC[jm];
     const double Cmu = p.Cmu;
@@ -471,17 +528,26 @@ k_dW_z(Grid g, Phys p, double dt, const double *__restrict__ U, const double *__
     rhs = rhs - (0.25 * (ddv3 * (double)jwm * dwdy3 + ddv4 * (double)jwp * dwdy4)
                  - invRhocCmu_half / dx2[j] * (mu4 * ddvdz4 * (double)jwp - mu3 * ddvdz3 * (double)jwm));
 
-    const size_t q = g.r3(i, j, k);
-    ac[q] = mACK * dt + 1.0;
-    ap[q] = mAPK * dt;
-    am[q] = mAMK * dt;
-    rhsout[q] = rhs * dt;
+    ac = mACK * dt + 1.0;
+    ap = mAPK * dt;
+    am = mAMK * dt;
+    rhsout = rhs * dt;
 }
+struct DWZ {
+    Grid g;
+    Phys p;
+    double dt;
+    const double *U, *V, *W, *P, *dU, *dV, *Mu, *iR, *VBCup, *VBCbt, *WBCup, *WBCbt;
+    __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
+                                               double &d) const {
+        dW_z_row(g, p, dt, U, V, W, P, dU, dV, Mu, iR, VBCup, VBCbt, WBCup, WBCbt, i, j, k, a, b, c, d);
+    }
+};
 
-__global__ void __launch_bounds__(256)
-k_dW_x(Grid g, Phys p, double dt, const double *__restrict__ U, const double *__restrict__ iR,
-       const double *__restrict__ stale, scr_t *__restrict__ am, scr_t *__restrict__ ac, scr_t *__restrict__ ap) {
-    KERNEL_IJK
+__device__ __forceinline__ void dW_x_row(const Grid &g, const Phys &p, double dt, const double *__restrict__ U,
+    const double *__restrict__ iR, const double *__restrict__ stale,
+    int i, int j, int k, double &am, double &ac, double &ap) {
+    ROW_METRICS
     const int ip = i + 1, km = k - 1;
     const double mu1 = stale[2], mu2 = stale[3];
     const double u1 = (dx3[km] * Uf(i, j, k) + dx3[k] * Uf(i, j, km)) / dmx3[k] * 0.5;
@@ -491,16 +557,25 @@ k_dW_x(Grid g, Phys p, double dt, const double *__restrict__ U, const double *__
     const double mACI = invRhocCmu_half / dx1[i] * (mu2 / dmx1[ip] + mu1 / dmx1[i]) + 0.25 * (-u2 / dmx1[ip] + u1 / dmx1[i]);
     const double mAPI = -invRhocCmu_half / dx1[i] * mu2 / dmx1[ip] + 0.25 * (u2 / dmx1[ip]);
     const double mAMI = -invRhocCmu_half / dx1[i] * mu1 / dmx1[i] + 0.25 * (-u1 / dmx1[i]);
-    const size_t q = g.r3(i, j, k);
-    ac[q] = mACI * dt + 1.0;
-    ap[q] = mAPI * dt;
-    am[q] = mAMI * dt;
+    ac = mACI * dt + 1.0;
+    ap = mAPI * dt;
+    am = mAMI * dt;
 }
+struct DWX {
+    Grid g;
+    Phys p;
+    double dt;
+    const double *U, *iR, *stale;
+    __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
+                                               double &d) const {
+        dW_x_row(g, p, dt, U, iR, stale, i, j, k, a, b, c);
+    }
+};
 
-__global__ void __launch_bounds__(256)
-k_dW_y(Grid g, Phys p, double dt, const double *__restrict__ V, const double *__restrict__ iR,
-       const double *__restrict__ stale, scr_t *__restrict__ am, scr_t *__restrict__ ac, scr_t *__restrict__ ap) {
-    KERNEL_IJK
+__device__ __forceinline__ void dW_y_row(const Grid &g, const Phys &p, double dt, const double *__restrict__ V,
+    const double *__restrict__ iR, const double *__restrict__ stale,
+    int i, int j, int k, double &am, double &ac, double &ap) {
+    ROW_METRICS
     const int jp = j + 1, jm = j - 1;
     const int km = g.n3m - 1;  // stale from the x-stage's last k
     const int jwp = g.jC[jp], jwm = g.jC[jm];
@@ -514,11 +589,20 @@ k_dW_y(Grid g, Phys p, double dt, const double *__restrict__ V, const double *__
     mAPJ = mAPJ * (double)jwp;
     double mAMJ = -invRhocCmu_half / dx2[j] * mu3 / dmx2[j] + 0.25 * (-v3 / dmx2[j]);
     mAMJ = mAMJ * (double)jwm;
-    const size_t q = g.r3(i, j, k);
-    ac[q] = mACJ * dt + 1.0;
-    ap[q] = mAPJ * dt;
-    am[q] = mAMJ * dt;
+    ac = mACJ * dt + 1.0;
+    ap = mAPJ * dt;
+    am = mAMJ * dt;
 }
+struct DWY {
+    Grid g;
+    Phys p;
+    double dt;
+    const double *V, *iR, *stale;
+    __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
+                                               double &d) const {
+        dW_y_row(g, p, dt, V, iR, stale, i, j, k, a, b, c);
+    }
+};
 
 // ------------------------------------------------------- block corrections --
 __global__ void __launch_bounds__(256)
@@ -607,67 +691,54 @@ static int launch_stale(tcs_ctx *c) {
     return TCS_OK;
 }
 
+// y stage of a momentum increment: rows j = jlo..n2m stored into the field F
+template <class Row>
+static int y_stage(tcs_ctx *c, const char *label_rows, const char *label_solve, const Row &row, int jlo, double *F) {
+    scr_t *am = SCR(c->am), *ac = SCR(c->ac), *ap = SCR(c->ap), *rhs = SCR(c->rhs);
+    const int nrows = c->n2m - jlo + 1;
+    if (rows_in_solve<scr_t>(c, nrows)) return y_stage_in_solve(c, label_solve, row, jlo, nrows, rhs, F, 1);
+    TCS_TRY(rows3d_launch<false>(c->stream, label_rows, row, jlo, am, ac, ap, rhs));
+    return sweep_y_field(c, jlo, nrows, F, 1);
+}
+
 int momentum_solvedU(tcs_ctx *c, double dt) {
     const Grid g = c->grid();
-    const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
     double **f = c->f;
-    TCS_LAUNCH(c->stream, "dU_z", k_dU_z, grd, blk, 0, g, c->phys, dt, f[TCS_U], f[TCS_V], f[TCS_W], f[TCS_P],
-                       f[TCS_T], f[TCS_MU], f[TCS_INVRHO], c->wall[TCS_UBCUP], c->wall[TCS_UBCBT],
-                       c->wall[TCS_VBCUP], c->wall[TCS_VBCBT], SCR(c->am), SCR(c->ac), SCR(c->ap), SCR(c->rhs));
-    TCS_TRY(sweep_z(c, true));
-    TCS_LAUNCH(c->stream, "dU_x", k_dU_x, grd, blk, 0, g, c->phys, dt, f[TCS_U], f[TCS_MU], f[TCS_INVRHO], SCR(c->am),
-                       SCR(c->ac), SCR(c->ap));
-    TCS_TRY(sweep_x(c, true));
-    TCS_LAUNCH(c->stream, "dU_y", k_dU_y, grd, blk, 0, g, c->phys, dt, f[TCS_V], f[TCS_MU], f[TCS_INVRHO], SCR(c->am),
-                       SCR(c->ac), SCR(c->ap));
-    TCS_TRY(sweep_y_field(c, 1, c->n2m, f[TCS_DU], 1));
-    return TCS_OK;
+    scr_t *am = SCR(c->am), *ac = SCR(c->ac), *ap = SCR(c->ap), *rhs = SCR(c->rhs);
+    const DUZ z{g, c->phys, dt, f[TCS_U], f[TCS_V], f[TCS_W], f[TCS_P], f[TCS_T], f[TCS_MU], f[TCS_INVRHO],
+                c->wall[TCS_UBCUP], c->wall[TCS_UBCBT], c->wall[TCS_VBCUP], c->wall[TCS_VBCBT]};
+    TCS_TRY(z_stage(c, "dU_z", "dU_z_solve", z, 1, am, ac, ap, rhs));
+    TCS_TRY(x_stage(c, "dU_x", "dU_x_solve", DUX{g, c->phys, dt, f[TCS_U], f[TCS_MU], f[TCS_INVRHO]}, 1, am, ac, ap,
+                    rhs));
+    return y_stage(c, "dU_y", "dU_y_solve", DUY{g, c->phys, dt, f[TCS_V], f[TCS_MU], f[TCS_INVRHO]}, 1, f[TCS_DU]);
 }
 
 int momentum_solvedV(tcs_ctx *c, double dt) {
     const Grid g = c->grid();
-    const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
     double **f = c->f;
+    scr_t *am = SCR(c->am), *ac = SCR(c->ac), *ap = SCR(c->ap), *rhs = SCR(c->rhs);
     TCS_TRY(launch_stale(c));
-    TCS_LAUNCH(c->stream, "dV_z", k_dV_z, grd, blk, 0, g, c->phys, dt, f[TCS_U], f[TCS_V], f[TCS_W], f[TCS_P],
-                       f[TCS_DU], f[TCS_MU], f[TCS_INVRHO], c->wall[TCS_VBCUP], c->wall[TCS_VBCBT], SCR(c->am), SCR(c->ac),
-                       SCR(c->ap), SCR(c->rhs));
-    // rows j=1 are not part of the dV systems; sweep only j=2..n2m in z and x
-    {
-        const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m;
-        TCS_TRY(ptdma_sweep(c, SCR(c->am) + n1, SCR(c->ac) + n1, SCR(c->ap) + n1, SCR(c->rhs) + n1, c->n1m, 1, c->n2m - 1, n1,
-                                   c->n3m, n12, true));
-    }
-    TCS_LAUNCH(c->stream, "dV_x", k_dV_x, grd, blk, 0, g, c->phys, dt, f[TCS_U], f[TCS_INVRHO], c->stale, SCR(c->am),
-                       SCR(c->ac), SCR(c->ap));
-    {
-        const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m;
-        TCS_TRY(ptdma_sweep(c, SCR(c->am) + n1, SCR(c->ac) + n1, SCR(c->ap) + n1, SCR(c->rhs) + n1, c->n2m - 1, n1, c->n3m, n12,
-                                   c->n1m, 1, true));
-    }
-    TCS_LAUNCH(c->stream, "dV_y", k_dV_y, grd, blk, 0, g, c->phys, dt, f[TCS_V], f[TCS_MU], f[TCS_INVRHO], SCR(c->am),
-                       SCR(c->ac), SCR(c->ap));
+    // rows j = 1 are not part of the dV systems: every stage covers j = 2..n2m
+    const DVZ z{g, c->phys, dt, f[TCS_U], f[TCS_V], f[TCS_W], f[TCS_P], f[TCS_DU], f[TCS_MU], f[TCS_INVRHO],
+                c->wall[TCS_VBCUP], c->wall[TCS_VBCBT]};
+    TCS_TRY(z_stage(c, "dV_z", "dV_z_solve", z, 2, am, ac, ap, rhs));
+    TCS_TRY(x_stage(c, "dV_x", "dV_x_solve", DVX{g, c->phys, dt, f[TCS_U], f[TCS_INVRHO], c->stale}, 2, am, ac, ap,
+                    rhs));
     TCS_LAUNCH(c->stream, "dV_zero_rows", k_dV_zero_rows, dim3((c->N1 + 63) / 64, c->N3), dim3(64), 0, g, f[TCS_DV]);
-    TCS_TRY(sweep_y_field(c, 2, c->n2m - 1, f[TCS_DV], 1));
-    return TCS_OK;
+    return y_stage(c, "dV_y", "dV_y_solve", DVY{g, c->phys, dt, f[TCS_V], f[TCS_MU], f[TCS_INVRHO]}, 2, f[TCS_DV]);
 }
 
 int momentum_solvedW(tcs_ctx *c, double dt) {
     const Grid g = c->grid();
-    const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
     double **f = c->f;
+    scr_t *am = SCR(c->am), *ac = SCR(c->ac), *ap = SCR(c->ap), *rhs = SCR(c->rhs);
     TCS_TRY(launch_stale(c));
-    TCS_LAUNCH(c->stream, "dW_z", k_dW_z, grd, blk, 0, g, c->phys, dt, f[TCS_U], f[TCS_V], f[TCS_W], f[TCS_P],
-                       f[TCS_DU], f[TCS_DV], f[TCS_MU], f[TCS_INVRHO], c->wall[TCS_VBCUP], c->wall[TCS_VBCBT],
-                       c->wall[TCS_WBCUP], c->wall[TCS_WBCBT], SCR(c->am), SCR(c->ac), SCR(c->ap), SCR(c->rhs));
-    TCS_TRY(sweep_z(c, true));
-    TCS_LAUNCH(c->stream, "dW_x", k_dW_x, grd, blk, 0, g, c->phys, dt, f[TCS_U], f[TCS_INVRHO], c->stale, SCR(c->am),
-                       SCR(c->ac), SCR(c->ap));
-    TCS_TRY(sweep_x(c, true));
-    TCS_LAUNCH(c->stream, "dW_y", k_dW_y, grd, blk, 0, g, c->phys, dt, f[TCS_V], f[TCS_INVRHO], c->stale, SCR(c->am),
-                       SCR(c->ac), SCR(c->ap));
-    TCS_TRY(sweep_y_field(c, 1, c->n2m, f[TCS_DW], 1));
-    return TCS_OK;
+    const DWZ z{g, c->phys, dt, f[TCS_U], f[TCS_V], f[TCS_W], f[TCS_P], f[TCS_DU], f[TCS_DV], f[TCS_MU],
+                f[TCS_INVRHO], c->wall[TCS_VBCUP], c->wall[TCS_VBCBT], c->wall[TCS_WBCUP], c->wall[TCS_WBCBT]};
+    TCS_TRY(z_stage(c, "dW_z", "dW_z_solve", z, 1, am, ac, ap, rhs));
+    TCS_TRY(x_stage(c, "dW_x", "dW_x_solve", DWX{g, c->phys, dt, f[TCS_U], f[TCS_INVRHO], c->stale}, 1, am, ac, ap,
+                    rhs));
+    return y_stage(c, "dW_y", "dW_y_solve", DWY{g, c->phys, dt, f[TCS_V], f[TCS_INVRHO], c->stale}, 1, f[TCS_DW]);
 }
 
 int momentum_blockLdV(tcs_ctx *c, double dt) {

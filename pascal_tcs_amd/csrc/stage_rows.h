@@ -1,0 +1,155 @@
+// stage_rows.h -- how the ADI stages' tridiagonal rows reach the line solves.
+//
+// Every stage of the thermal and momentum solvers has a row functor:
+// row(i, j, k, a, b, c, d) computes row (i, j, k) of the stage's systems
+// (a, b, c: lower, diagonal and upper coefficient; d: the right-hand side,
+// first stage only) from the module fields.  The rows go to the solve in one
+// of two ways:
+//
+//   k_rows3d           one thread per interior cell writes the rows into
+//                      the ADI scratch (am, ac, ap, rhs); a sweep then reads
+//                      them back (single-rank line solves);
+//   k_ptdma_part_src,  the split line solve computes each row where it needs
+//   k_ptdma_part_rows_src  it (z and y lines: the thread of a part makes its
+//                      own rows; x lines: the workgroup makes them block by
+//                      block and passes them through LDS): no coefficient
+//                      array is written to or read from memory.
+//
+// The rows are the same double values either way and the elimination is the
+// split solve's, so both ways give bit-identical solutions.
+#pragma once
+
+#include "tdma_part.h"
+
+namespace tcs {
+
+// One thread per interior cell (i fastest, rows j >= jlo): row (i, j, k) of
+// the stage's systems into am, ac, ap and, with RHS, its right-hand side.
+template <class Row, bool RHS, typename S>
+__global__ void __launch_bounds__(256)
+k_rows3d(Row row, int jlo, S *__restrict__ am, S *__restrict__ ac, S *__restrict__ ap, S *__restrict__ rhs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > row.g.n1m || j > row.g.n2m || j < jlo) return;
+    double a, b, c, d = 0.0;
+    row(i, j, k, a, b, c, d);
+    const size_t q = row.g.r3(i, j, k);
+    ac[q] = b;
+    ap[q] = c;
+    am[q] = a;
+    if (RHS) rhs[q] = d;
+}
+
+template <bool RHS, class Row, typename S>
+int rows3d_launch(hipStream_t st, const char *label, const Row &row, int jlo, S *am, S *ac, S *ap, S *rhs) {
+    const dim3 blk = interior_block(), grd = interior_grid(row.g.n1m, row.g.n2m, row.g.n3m);
+    TCS_LAUNCH(st, label, (k_rows3d<Row, RHS, S>), grd, blk, 0, row, jlo, am, ac, ap, rhs);
+    return TCS_OK;
+}
+
+// z lines as a split-solve source: line (s0, s1) = (i-1, j-jlo), row k-1;
+// the right-hand side comes with the row.
+template <class Row, typename S>
+struct ZLineRows {
+    Row row;
+    int jlo;
+    __device__ __forceinline__ void operator()(int s0, int s1, int r, S &a, S &b, S &c, S &d) const {
+        double ta, tb, tc, td = 0.0;
+        row(s0 + 1, s1 + jlo, r + 1, ta, tb, tc, td);
+        a = (S)ta; b = (S)tb; c = (S)tc; d = (S)td;
+    }
+};
+
+// y lines as a split-solve source: line (s0, s1) = (i-1, k-1), row j-jlo;
+// the right-hand side is the previous stage's solution in the scratch.
+template <class Row, typename S>
+struct YLineRows {
+    Row row;
+    const S *rhs;
+    int jlo;
+    __device__ __forceinline__ void operator()(int s0, int s1, int r, S &a, S &b, S &c, S &d) const {
+        const int i = s0 + 1, j = r + jlo, k = s1 + 1;
+        double ta, tb, tc, td = 0.0;
+        row(i, j, k, ta, tb, tc, td);
+        a = (S)ta; b = (S)tb; c = (S)tc;
+        d = rhs[row.g.r3(i, j, k)];
+    }
+};
+
+// x lines as a split-solve source: line (s0, s1) = (j-jlo, k-1), row i-1;
+// the right-hand side is the previous stage's solution in the scratch.
+template <class Row, typename S>
+struct XLineRows {
+    Row row;
+    const S *rhs;
+    int jlo;
+    __device__ __forceinline__ void operator()(int s0, int s1, int r, S &a, S &b, S &c, S &d) const {
+        const int i = r + 1, j = s0 + jlo, k = s1 + 1;
+        double ta, tb, tc, td = 0.0;
+        row(i, j, k, ta, tb, tc, td);
+        a = (S)ta; b = (S)tb; c = (S)tc;
+        d = rhs[row.g.r3(i, j, k)];
+    }
+};
+
+// Whether a stage's lines of n_row rows are solved with their rows computed
+// in the solve: split solves are asked for (the mixed form always splits)
+// and the lines are short enough to split.
+template <typename S>
+inline bool rows_in_solve(const tcs_ctx *c, int n_row) {
+    int P, w1, w2;
+    return (c->tdma_split || sizeof(S) == sizeof(float)) && ptdma_part_plan(n_row, P, w1, w2);
+}
+
+// z stage over the z lines of rows j >= jlo, solution into the scratch rhs:
+// rows computed in the split solve (label_solve), or written by k_rows3d
+// (label_rows) and swept.
+template <class Row, typename S>
+int z_stage(tcs_ctx *c, const char *label_rows, const char *label_solve, const Row &row, int jlo, S *am, S *ac,
+            S *ap, S *rhs) {
+    const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m, off = (long)(jlo - 1) * n1;
+    if (rows_in_solve<S>(c, c->n3m))
+        return ptdma_part_src_launch<true>(c->stream, label_solve, ZLineRows<Row, S>{row, jlo}, rhs + off, c->n1m,
+                                           c->n2m - jlo + 1, n1, c->n3m, n12);
+    TCS_TRY(rows3d_launch<true>(c->stream, label_rows, row, jlo, am, ac, ap, rhs));
+    return ptdma_sweep(c, am + off, ac + off, ap + off, rhs + off, c->n1m, 1, c->n2m - jlo + 1, n1, c->n3m, n12,
+                       true);
+}
+
+// x stage over the x lines of rows j >= jlo (right-hand side and solution in
+// the scratch rhs), in either of the two ways.
+template <class Row, typename S>
+int x_stage(tcs_ctx *c, const char *label_rows, const char *label_solve, const Row &row, int jlo, S *am, S *ac,
+            S *ap, S *rhs) {
+    const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m, off = (long)(jlo - 1) * n1;
+    const int nl = c->n2m - jlo + 1;
+    if (rows_in_solve<S>(c, c->n1m))
+        return ptdma_part_rows_src_launch<true>(c->stream, label_solve, XLineRows<Row, S>{row, rhs, jlo}, rhs + off,
+                                                nl, n1, c->n3m, n12, c->n1m);
+    TCS_TRY(rows3d_launch<false>(c->stream, label_rows, row, jlo, am, ac, ap, rhs));
+    return ptdma_sweep(c, am + off, ac + off, ap + off, rhs + off, nl, n1, c->n3m, n12, c->n1m, 1, true);
+}
+
+// y stage with its rows computed in the split solve (rows_in_solve holds):
+// the y lines of rows j = jlo..jlo+nrows-1, right-hand side from the scratch
+// rhs.  The solution is stored into (mode 1) or added to (mode 2) the
+// interior of the field F, or, with F null, written back into rhs.
+template <class Row, typename S>
+int y_stage_in_solve(tcs_ctx *c, const char *label, const Row &row, int jlo, int nrows, S *rhs, double *F,
+                     int mode) {
+    const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m, off = (long)(jlo - 1) * n1;
+    LineOut out;
+    if (F) {
+        out.F = F;
+        out.mode = mode;
+        out.base = 1 + (long)c->N1 * jlo + (long)c->N1 * c->N2;  // (i, j, k) = (1, jlo, 1)
+        out.ss0 = 1;
+        out.ss1 = (long)c->N1 * c->N2;
+        out.rs = c->N1;
+    }
+    return ptdma_part_src_launch<false>(c->stream, label, YLineRows<Row, S>{row, rhs, jlo}, rhs + off, c->n1m,
+                                        c->n3m, n12, nrows, n1, out);
+}
+
+}  // namespace tcs

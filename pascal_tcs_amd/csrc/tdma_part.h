@@ -10,6 +10,8 @@
 // operations a separate solve would give it.
 #pragma once
 
+#include <algorithm>
+
 #include "tcs_internal.h"
 
 namespace tcs {
@@ -26,30 +28,32 @@ __device__ __forceinline__ void range(int p, int w1, int w2, int &j0, int &m) {
     m = w1 + (p < w2 ? 1 : 0);
 }
 
-// One part (rows 0..m-1, m >= 2) in registers: the per-rank modified Thomas
-// algorithm of pascal_tdma.f90:501-536.  b is consumed; a, c, d come back as
-// the reduced-form coefficients.
+// One step of the per-rank modified Thomas algorithm of
+// pascal_tdma.f90:501-536 on a part held in registers: row r (coefficients
+// ar, br, cr, dr) enters the forward elimination, rows < r already hold
+// their eliminated form.  Rows 0 and 1 are only normalised.
 template <typename T, int NR>
-__device__ __forceinline__ void reduce(T (&a)[PT_MMAX], T (&b)[PT_MMAX], T (&c)[PT_MMAX],
-                                       T (&d)[NR][PT_MMAX], int m) {
-    a[0] = a[0] / b[0];
+__device__ __forceinline__ void reduce_row(int r, T ar, T br, T cr, const T (&dr)[NR], T (&a)[PT_MMAX],
+                                           T (&c)[PT_MMAX], T (&d)[NR][PT_MMAX]) {
+    if (r < 2) {
+        a[r] = ar / br;
 #pragma unroll
-    for (int k = 0; k < NR; k++) d[k][0] = d[k][0] / b[0];
-    c[0] = c[0] / b[0];
-    a[1] = a[1] / b[1];
+        for (int k = 0; k < NR; k++) d[k][r] = dr[k] / br;
+        c[r] = cr / br;
+    } else {
+        const T rr = T(1) / (br - ar * c[r - 1]);
 #pragma unroll
-    for (int k = 0; k < NR; k++) d[k][1] = d[k][1] / b[1];
-    c[1] = c[1] / b[1];
-#pragma unroll
-    for (int r = 2; r < PT_MMAX; r++) {
-        if (r < m) {
-            const T rr = T(1) / (b[r] - a[r] * c[r - 1]);
-#pragma unroll
-            for (int k = 0; k < NR; k++) d[k][r] = rr * (d[k][r] - a[r] * d[k][r - 1]);
-            c[r] = rr * c[r];
-            a[r] = -rr * a[r] * a[r - 1];
-        }
+        for (int k = 0; k < NR; k++) d[k][r] = rr * (dr[k] - ar * d[k][r - 1]);
+        c[r] = rr * cr;
+        a[r] = -rr * ar * a[r - 1];
     }
+}
+
+// The rest of the per-rank algorithm once rows 0..m-1 have been through
+// reduce_row: backward elimination and the fix-up of row 0.  a, c, d come
+// back as the reduced-form coefficients.
+template <typename T, int NR>
+__device__ __forceinline__ void reduce_back(T (&a)[PT_MMAX], T (&c)[PT_MMAX], T (&d)[NR][PT_MMAX], int m) {
 #pragma unroll
     for (int r = PT_MMAX - 3; r >= 1; r--) {
         if (r <= m - 3) {
@@ -64,6 +68,23 @@ __device__ __forceinline__ void reduce(T (&a)[PT_MMAX], T (&b)[PT_MMAX], T (&c)[
     for (int k = 0; k < NR; k++) d[k][0] = rr * (d[k][0] - c[0] * d[k][1]);
     a[0] = rr * a[0];
     c[0] = -rr * c[0] * c[1];
+}
+
+// One part (rows 0..m-1, m >= 2) in registers, all rows loaded: b is
+// consumed; a, c, d come back as the reduced-form coefficients.
+template <typename T, int NR>
+__device__ __forceinline__ void reduce(T (&a)[PT_MMAX], T (&b)[PT_MMAX], T (&c)[PT_MMAX],
+                                       T (&d)[NR][PT_MMAX], int m) {
+#pragma unroll
+    for (int r = 0; r < PT_MMAX; r++) {
+        if (r < m) {
+            T dr[NR];
+#pragma unroll
+            for (int k = 0; k < NR; k++) dr[k] = d[k][r];
+            reduce_row<T, NR>(r, a[r], b[r], c[r], dr, a, c, d);
+        }
+    }
+    reduce_back<T, NR>(a, c, d, m);
 }
 
 // a, c, d at row m-1 (a runtime index) in one unrolled pass: selecting all
@@ -243,5 +264,195 @@ __device__ __forceinline__ void exchange(RedLDS<T, SX, NR> &red, int line, int p
     }
 }
 
+// The solution of a part of line (s0, s1): into D at dbase + row*rs
+// (out.mode 0), or stored into / added to the field of out (modes 1, 2).
+template <typename T>
+__device__ __forceinline__ void store(const T (&a)[PT_MMAX], const T (&c)[PT_MMAX], const T (&d)[1][PT_MMAX],
+                                      int j0, int m, T xf, T xl, T *D, long dbase, long rs, const LineOut &out,
+                                      int s0, int s1) {
+    if (out.mode == 0) {
+#pragma unroll
+        for (int r = 0; r < PT_MMAX; r++)
+            if (r < m) D[dbase + (long)(j0 + r) * rs] = x_of<T, 1>(a, c, d, 0, r, m, xf, xl);
+    } else {
+        const long fb = out.base + (long)s0 * out.ss0 + (long)s1 * out.ss1;
+#pragma unroll
+        for (int r = 0; r < PT_MMAX; r++) {
+            if (r < m) {
+                const double x = (double)x_of<T, 1>(a, c, d, 0, r, m, xf, xl);
+                double *f = out.F + fb + (long)(j0 + r) * out.rs;
+                *f = out.mode == 1 ? x : *f + x;
+            }
+        }
+    }
+}
+
 }  // namespace part
+
+// Lines strided in memory whose rows are computed, not read: src(s0, s1,
+// row, a, b, c, d) gives row `row` of line (s0, s1) (s0 < n0 along the
+// adjacent lines, s1 = blockIdx.y).  Each row enters the forward elimination
+// as soon as it is made, so no coefficient array goes through memory; the
+// thread layout, the reduced solve and the output (D at s0 + s1*ss1 +
+// row*rs, or a field) are those of the split solve of stored lines.
+template <typename T, bool CYC, class Src>
+__global__ void __launch_bounds__(PT_SX * PT_PMAX)
+k_ptdma_part_src(Src src, T *D, int n0, long ss1, long rs, int P, int w1, int w2, LineOut out) {
+    using namespace part;
+    __shared__ RedLDS<T, PT_SX, 1> red;
+    const int t = threadIdx.x;
+    const int line = t % PT_SX, p = t / PT_SX;
+    const int s0 = blockIdx.x * PT_SX + line, s1 = blockIdx.y;
+    const bool act = s0 < n0;
+    int j0, m;
+    range(p, w1, w2, j0, m);
+
+    T a[PT_MMAX], c[PT_MMAX], d[1][PT_MMAX];
+#pragma unroll
+    for (int r = 0; r < PT_MMAX; r++) {
+        if (r < m) {
+            T ar = T(0), br = T(1), cr = T(0), dr[1] = {T(0)};
+            if (act) src(s0, s1, j0 + r, ar, br, cr, dr[0]);
+            reduce_row<T, 1>(r, ar, br, cr, dr, a, c, d);
+        } else {
+            a[r] = T(0); c[r] = T(0); d[0][r] = T(0);
+        }
+    }
+    reduce_back<T, 1>(a, c, d, m);
+    T xf[1], xl[1];
+    exchange<T, CYC, PT_SX, 1>(red, line, p, P, a, c, d, m, xf, xl);
+    if (!act) return;
+    store<T>(a, c, d, j0, m, xf[0], xl[0], D, (long)s0 + (long)s1 * ss1, rs, out, s0, s1);
+}
+
+// padded LDS position of row j of a staged line (one pad value per 16 rows)
+__device__ __forceinline__ int stage_pos(int j) { return j + (j >> 4); }
+inline int stage_len(int rows) { return rows - 1 + ((rows - 1) >> 4) + 1; }
+
+// Row blocks of the contiguous-line solve with computed rows: the parts of
+// a line go to PT_RBLK blocks (fewer when P is smaller), block g holding
+// parts [g*P/nb, (g+1)*P/nb) and so rows [row0(g), row0(g+1)).
+constexpr int PT_RBLK = 4;
+__host__ __device__ inline int rblocks(int P) { return P < PT_RBLK ? P : PT_RBLK; }
+__host__ __device__ inline int rblock_row0(int g, int P, int w1, int w2) {
+    const int p = g * (P / rblocks(P));
+    return p * w1 + (p < w2 ? p : w2);
+}
+
+// Contiguous lines (rows 1 apart, x sweeps) whose rows are computed:
+// src(s0, s1, row, a, b, c, d) as in k_ptdma_part_src, line (s0, s1) at
+// s0*ss0 + s1*ss1 of D.  The rows of the workgroup's PT_SX lines are made
+// one row block at a time by all its threads, consecutive rows on
+// consecutive threads (coalesced stencil reads), and pass through LDS, the
+// four arrays of the block together; the threads whose parts lie in the
+// block take their rows from there.  Then the solve of k_ptdma_part_rows,
+// whose LDS stage also takes the solution out.
+template <typename T, bool CYC, class Src>
+__global__ void __launch_bounds__(PT_SX * PT_PMAX)
+k_ptdma_part_rows_src(Src src, T *D, int n0, long ss0, long ss1, int n_row, int ls, int lsb, int P, int w1,
+                      int w2) {
+    using namespace part;
+    extern __shared__ __align__(16) unsigned char stage_raw[];
+    T *stage = reinterpret_cast<T *>(stage_raw);
+    __shared__ RedLDS<T, PT_SX, 1> red;
+    const int t = threadIdx.x, nt = blockDim.x;
+    const int line = t % PT_SX, p = t / PT_SX;
+    const int s0 = blockIdx.x * PT_SX + line, s1 = blockIdx.y;
+    const bool act = s0 < n0;
+    const int nl = min(PT_SX, n0 - (int)blockIdx.x * PT_SX);  // lines present in this workgroup
+    int j0, m;
+    range(p, w1, w2, j0, m);
+    const int nb = rblocks(P), mygrp = p / (P / nb);
+
+    T a[PT_MMAX], b[PT_MMAX], c[PT_MMAX], d[1][PT_MMAX];
+    for (int g = 0; g < nb; g++) {
+        const int r0 = rblock_row0(g, P, w1, w2), nr = rblock_row0(g + 1, P, w1, w2) - r0;
+        if (g > 0) __syncthreads();  // the previous block has been read
+        // stage[(q*PT_SX + l)*lsb + stage_pos(r - r0)], q = a, b, c, d
+        for (int e = t; e < nr * PT_SX; e += nt) {
+            const int l = e / nr, r = e - l * nr;
+            T ar = T(0), br = T(1), cr = T(0), dr = T(0);
+            if (l < nl) src((int)blockIdx.x * PT_SX + l, s1, r0 + r, ar, br, cr, dr);
+            const int o = l * lsb + stage_pos(r);
+            stage[o] = ar;
+            stage[o + PT_SX * lsb] = br;
+            stage[o + 2 * PT_SX * lsb] = cr;
+            stage[o + 3 * PT_SX * lsb] = dr;
+        }
+        __syncthreads();
+        if (mygrp == g) {
+#pragma unroll
+            for (int r = 0; r < PT_MMAX; r++) {
+                const bool in = r < m;
+                const T *q = stage + line * lsb + stage_pos(j0 - r0 + r);
+                a[r] = in ? q[0] : T(0);
+                b[r] = in ? q[PT_SX * lsb] : T(1);
+                c[r] = in ? q[2 * PT_SX * lsb] : T(0);
+                d[0][r] = in ? q[3 * PT_SX * lsb] : T(0);
+            }
+        }
+    }
+    reduce<T, 1>(a, b, c, d, m);
+    T xf[1], xl[1];
+    exchange<T, CYC, PT_SX, 1>(red, line, p, P, a, c, d, m, xf, xl);
+    // every thread has read the stage before exchange's first barrier
+    if (act) {
+#pragma unroll
+        for (int r = 0; r < PT_MMAX; r++)
+            if (r < m) stage[line * ls + stage_pos(j0 + r)] = x_of<T, 1>(a, c, d, 0, r, m, xf[0], xl[0]);
+    }
+    __syncthreads();
+    const long gbase = (long)blockIdx.x * PT_SX * ss0 + (long)s1 * ss1;
+    if (t < n_row) {
+#pragma unroll
+        for (int l = 0; l < PT_SX; l++)
+            if (l < nl) D[gbase + l * ss0 + t] = stage[l * ls + stage_pos(t)];
+    }
+}
+
+// Launch of k_ptdma_part_src over n0 x n1 lines of n_row rows (n_row in
+// [2, 512]); returns TCS_E_ARG outside the split range.
+template <bool CYC, typename T, class Src>
+int ptdma_part_src_launch(hipStream_t st, const char *label, const Src &src, T *D, int n0, int n1, long ss1,
+                          int n_row, long rs, const LineOut &out = LineOut()) {
+    int P, w1, w2;
+    if (n0 <= 0 || n1 <= 0) return TCS_OK;
+    if (!ptdma_part_plan(n_row, P, w1, w2) || n1 > 65535) {
+        set_error("ptdma_part: n_row must be in [2, 512] and the outer line count <= 65535");
+        return TCS_E_ARG;
+    }
+    const dim3 blk(PT_SX * P), grd((unsigned)((n0 + PT_SX - 1) / PT_SX), (unsigned)n1);
+    TCS_LAUNCH(st, label, (k_ptdma_part_src<T, CYC, Src>), grd, blk, 0, src, D, n0, ss1, rs, P, w1, w2, out);
+    return TCS_OK;
+}
+
+// Launch of k_ptdma_part_rows_src over n0 x n1 contiguous lines of n_row
+// rows (n_row in [2, 512]).
+template <bool CYC, typename T, class Src>
+int ptdma_part_rows_src_launch(hipStream_t st, const char *label, const Src &src, T *D, int n0, long ss0, int n1,
+                               long ss1, int n_row) {
+    int P, w1, w2;
+    if (n0 <= 0 || n1 <= 0) return TCS_OK;
+    if (!ptdma_part_plan(n_row, P, w1, w2) || n1 > 65535) {
+        set_error("ptdma_part: n_row must be in [2, 512] and the outer line count <= 65535");
+        return TCS_E_ARG;
+    }
+    int lsb = 0;
+    for (int g = 0; g < rblocks(P); g++)
+        lsb = std::max(lsb, stage_len(rblock_row0(g + 1, P, w1, w2) - rblock_row0(g, P, w1, w2)));
+    const int ls = stage_len(n_row);
+    const int lds = (int)sizeof(T) * std::max(PT_SX * ls, 4 * PT_SX * lsb);
+    static bool lds_raised = false;  // more than 64 KiB of dynamic LDS (gfx950: 160 KiB per CU)
+    if (!lds_raised) {
+        const int most = std::max(PT_SX * stage_len(PT_MMAX * PT_PMAX), 4 * PT_SX * stage_len(PT_MMAX * PT_PMAX / PT_RBLK));
+        TCS_HIP(hipFuncSetAttribute((const void *)k_ptdma_part_rows_src<T, CYC, Src>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(T) * most));
+        lds_raised = true;
+    }
+    const dim3 blk(PT_SX * P), grd((unsigned)((n0 + PT_SX - 1) / PT_SX), (unsigned)n1);
+    TCS_LAUNCH(st, label, (k_ptdma_part_rows_src<T, CYC, Src>), grd, blk, lds, src, D, n0, ss0, ss1, n_row, ls, lsb,
+               P, w1, w2);
+    return TCS_OK;
+}
+
 }  // namespace tcs

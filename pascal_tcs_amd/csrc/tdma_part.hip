@@ -90,25 +90,8 @@ k_ptdma_part(const T *__restrict__ A, const T *__restrict__ B, const T *__restri
     T xf[1], xl[1];
     exchange<T, CYC, SX, 1>(red, line, p, P, a, c, d, m, xf, xl);
     if (!act) return;
-    if (out.mode == 0) {
-#pragma unroll
-        for (int r = 0; r < PT_MMAX; r++)
-            if (r < m) D[base + (long)(j0 + r) * rs] = x_of<T, 1>(a, c, d, 0, r, m, xf[0], xl[0]);
-    } else {
-        const long fb = out.base + (long)s0 * out.ss0 + (long)blockIdx.y * out.ss1;
-#pragma unroll
-        for (int r = 0; r < PT_MMAX; r++) {
-            if (r < m) {
-                const double x = (double)x_of<T, 1>(a, c, d, 0, r, m, xf[0], xl[0]);
-                double *f = out.F + fb + (long)(j0 + r) * out.rs;
-                *f = out.mode == 1 ? x : *f + x;
-            }
-        }
-    }
+    store<T>(a, c, d, j0, m, xf[0], xl[0], D, base, rs, out, s0, blockIdx.y);
 }
-
-// padded LDS position of row j of a staged line (one pad value per 16 rows)
-__device__ __forceinline__ int stage_pos(int j) { return j + (j >> 4); }
 
 // Contiguous lines (rows 1 apart): line (s0, s1) starts at s0*ss0 + s1*ss1.
 // The same thread layout as k_ptdma_part (thread t = line + PT_SX * part,
@@ -178,8 +161,7 @@ k_ptdma_part_rows(const T *__restrict__ A, const T *__restrict__ B, const T *__r
 
 template <typename T>
 static int part_lds_bytes(int n_row) {
-    const int ls = n_row - 1 + ((n_row - 1) >> 4) + 1;
-    return PT_SX * ls * (int)sizeof(T);
+    return PT_SX * stage_len(n_row) * (int)sizeof(T);
 }
 
 template <typename T> struct PartLabels;
@@ -209,7 +191,7 @@ static int part_launch(T *A, const T *B, T *C, T *D, int n0, long ss0, int n1, l
         return TCS_E_ARG;
     }
     if (rs == 1) {
-        const int ls = n_row - 1 + ((n_row - 1) >> 4) + 1;
+        const int ls = stage_len(n_row);
         const int lds = part_lds_bytes<T>(n_row);
         static bool lds_raised = false;  // allow more than 64 KiB of dynamic LDS (gfx950: 160 KiB per CU)
         if (!lds_raised) {

@@ -9,7 +9,7 @@
 // wavefront) that writes am/ac/ap (and the RHS in stage 1) in the interior
 // (i,j,k) layout, followed by the exact-order PaScaL sweep along the
 // stage's axis with strided system access (no transposes).
-#include "tcs_internal.h"
+#include "stage_rows.h"
 
 #ifndef TCS_FORM
 #define TCS_FORM exact
@@ -80,15 +80,10 @@ __global__ void k_thermal_coeffi(Grid g, Phys p, const double *__restrict__ T, d
 }
 
 // stage 1: z (:236-361)
-__global__ void __launch_bounds__(256)
-k_thermal_z(Grid g, Phys p, double dt, const double *__restrict__ T, const double *__restrict__ U,
-            const double *__restrict__ V, const double *__restrict__ W, const double *__restrict__ TBCup,
-            const double *__restrict__ TBCbt, scr_t *__restrict__ am, scr_t *__restrict__ ac,
-            scr_t *__restrict__ ap, scr_t *__restrict__ rhsout) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
-    if (i > g.n1m || j > g.n2m) return;
+__device__ __forceinline__ void thermal_z_row(const Grid &g, const Phys &p, double dt,
+    const double *__restrict__ T, const double *__restrict__ U, const double *__restrict__ V,
+    const double *__restrict__ W, const double *__restrict__ TBCup, const double *__restrict__ TBCbt,
+    int i, int j, int k, double &am, double &ac, double &ap, double &rhsout) {
     const int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1, km = k - 1, kp = k + 1;
     const int jem = g.jC[jm], jep = g.jC[jp];
     const double *dx1 = g.dx1, *dx2 = g.dx2, *dx3 = g.dx3, *dmx1 = g.dmx1, *dmx2 = g.dmx2, *dmx3 = g.dmx3;
@@ -199,26 +194,31 @@ k_thermal_z(Grid g, Phys p, double dt, const double *__restrict__ T, const doubl
                                       + (kPP4 * dedy4 - kPP3 * dedy3) / dx2[j]
                                       + (kPP6 * dedz6 - kPP5 * dedz5) / dx3[k]));
 
-    const size_t q = g.r3(i, j, k);
-    ac[q] = eACK * dt
+    ac = eACK * dt
           - 0.5 * dinvRhoCpCt_half * ((kPP2 * dedx2 - kPP1 * dedx1) / dx1[i]
                                       + (kPP4 * dedy4 - kPP3 * dedy3) / dx2[j]
                                       + (kPP6 * dedz6 - kPP5 * dedz5) / dx3[k]) * dt
           + 1.0;
-    ap[q] = eAPK * dt;
-    am[q] = eAMK * dt;
-    rhsout[q] = rhs * dt;
+    ap = eAPK * dt;
+    am = eAMK * dt;
+    rhsout = rhs * dt;
 }
+struct ThermalZ {
+    Grid g;
+    Phys p;
+    double dt;
+    const double *T, *U, *V, *W, *TBCup, *TBCbt;
+    __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
+                                               double &d) const {
+        thermal_z_row(g, p, dt, T, U, V, W, TBCup, TBCbt, i, j, k, a, b, c, d);
+    }
+};
 
 // stage 2: x (:388-429).  jm,jp,km,kp are the values the reference's loop
 // variables keep after stage 1: n2m-1, n2m+1, n3m-1, n3m+1.
-__global__ void __launch_bounds__(256)
-k_thermal_x(Grid g, Phys p, double dt, const double *__restrict__ T, const double *__restrict__ U,
-            scr_t *__restrict__ am, scr_t *__restrict__ ac, scr_t *__restrict__ ap) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
-    if (i > g.n1m || j > g.n2m) return;
+__device__ __forceinline__ void thermal_x_row(const Grid &g, const Phys &p, double dt,
+    const double *__restrict__ T, const double *__restrict__ U,
+    int i, int j, int k, double &am, double &ac, double &ap) {
     const int im = i - 1, ip = i + 1;
     const int jm = g.n2m - 1, jp = g.n2m + 1, km = g.n3m - 1, kp = g.n3m + 1;
     const double *dx1 = g.dx1, *dx2 = g.dx2, *dx3 = g.dx3, *dmx1 = g.dmx1, *dmx2 = g.dmx2, *dmx3 = g.dmx3;
@@ -262,20 +262,25 @@ k_thermal_x(Grid g, Phys p, double dt, const double *__restrict__ T, const doubl
                                                  + (kPP6 * dedz6 - kPP5 * dedz5) / dx3[k]);
     const double eAPI = -invRhoCpCt_half / dx1[i] * (kPP2 / dmx1[ip] + dx1[i] * (0.5 / dmx1[ip]) * dkPP2 * dedx2) + 0.25 * Uip / dmx1[ip];
     const double eAMI = -invRhoCpCt_half / dx1[i] * (kPP1 / dmx1[i] - dx1[i] * (0.5 / dmx1[i]) * dkPP1 * dedx1) - 0.25 * Uc / dmx1[i];
-    const size_t q = g.r3(i, j, k);
-    ac[q] = eACI * dt + 1.0;
-    ap[q] = eAPI * dt;
-    am[q] = eAMI * dt;
+    ac = eACI * dt + 1.0;
+    ap = eAPI * dt;
+    am = eAMI * dt;
 }
+struct ThermalX {
+    Grid g;
+    Phys p;
+    double dt;
+    const double *T, *U;
+    __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
+                                               double &d) const {
+        thermal_x_row(g, p, dt, T, U, i, j, k, a, b, c);
+    }
+};
 
 // stage 3: y (:456-507); km,kp keep their stage-1 values n3m-1, n3m+1.
-__global__ void __launch_bounds__(256)
-k_thermal_y(Grid g, Phys p, double dt, const double *__restrict__ T, const double *__restrict__ V,
-            scr_t *__restrict__ am, scr_t *__restrict__ ac, scr_t *__restrict__ ap) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
-    if (i > g.n1m || j > g.n2m) return;
+__device__ __forceinline__ void thermal_y_row(const Grid &g, const Phys &p, double dt,
+    const double *__restrict__ T, const double *__restrict__ V,
+    int i, int j, int k, double &am, double &ac, double &ap) {
     const int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1;
     const int km = g.n3m - 1, kp = g.n3m + 1;
     const int jem = g.jC[jm], jep = g.jC[jp];
@@ -322,11 +327,20 @@ k_thermal_y(Grid g, Phys p, double dt, const double *__restrict__ T, const doubl
     double eAMJ = -invRhoCpCt_half / dx2[j] * (kPP3 / dmx2[j] - dx2[j] * (0.5 / dmx2[j]) * dkPP3 * dedy3) - 0.25 * Vc / dmx2[j];
     eAPJ = eAPJ * (double)jep;
     eAMJ = eAMJ * (double)jem;
-    const size_t q = g.r3(i, j, k);
-    ac[q] = eACJ * dt + 1.0;
-    ap[q] = eAPJ * dt;
-    am[q] = eAMJ * dt;
+    ac = eACJ * dt + 1.0;
+    ap = eAPJ * dt;
+    am = eAMJ * dt;
 }
+struct ThermalY {
+    Grid g;
+    Phys p;
+    double dt;
+    const double *T, *V;
+    __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
+                                               double &d) const {
+        thermal_y_row(g, p, dt, T, V, i, j, k, a, b, c);
+    }
+};
 
 // T(i,j,k) += RHSJ (:518-524)
 // F(i,j,k) = inc (MODE 1) or F(i,j,k) += inc (MODE 2) over interior rows j >= j0
@@ -341,15 +355,6 @@ __global__ void k_interior_out(Grid g, int j0, double *__restrict__ F, const scr
     else F[q] = F[q] + inc[g.r3(i, j, k)];
 }
 
-// sweeps over the interior (i,j,k) layout
-int sweep_z(tcs_ctx *c, bool cyc) {
-    const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m;
-    return ptdma_sweep(c, SCR(c->am), SCR(c->ac), SCR(c->ap), SCR(c->rhs), c->n1m, 1, c->n2m, n1, c->n3m, n12, cyc);
-}
-int sweep_x(tcs_ctx *c, bool cyc) {
-    const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m;
-    return ptdma_sweep(c, SCR(c->am), SCR(c->ac), SCR(c->ap), SCR(c->rhs), c->n2m, n1, c->n3m, n12, c->n1m, 1, cyc);
-}
 // y sweep of rows j = j0..j0+nrows-1 (1-based) whose solution ends the
 // stage: stored into (mode 1) or added to (mode 2) the field F's interior.
 // With split lines the solve writes F directly; otherwise a second kernel
@@ -387,18 +392,22 @@ int thermal_coeffi(tcs_ctx *c) {
 
 int thermal_solver(tcs_ctx *c, double dt) {
     const Grid g = c->grid();
-    const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
-    TCS_LAUNCH(c->stream, "thermal_z", k_thermal_z, grd, blk, 0, g, c->phys, dt, c->f[TCS_T], c->f[TCS_U],
-                       c->f[TCS_V], c->f[TCS_W], c->wall[TCS_TBCUP], c->wall[TCS_TBCBT], SCR(c->am), SCR(c->ac), SCR(c->ap),
-                       SCR(c->rhs));
-    TCS_TRY(sweep_z(c, true));
-    TCS_LAUNCH(c->stream, "thermal_x", k_thermal_x, grd, blk, 0, g, c->phys, dt, c->f[TCS_T], c->f[TCS_U], SCR(c->am),
-                       SCR(c->ac), SCR(c->ap));
-    TCS_TRY(sweep_x(c, true));
-    TCS_LAUNCH(c->stream, "thermal_y", k_thermal_y, grd, blk, 0, g, c->phys, dt, c->f[TCS_T], c->f[TCS_V], SCR(c->am),
-                       SCR(c->ac), SCR(c->ap));
-    TCS_TRY(sweep_y_field(c, 1, c->n2m, c->f[TCS_T], 2));
-    return TCS_OK;
+    double *T = c->f[TCS_T];
+    scr_t *am = SCR(c->am), *ac = SCR(c->ac), *ap = SCR(c->ap), *rhs = SCR(c->rhs);
+    const ThermalZ z{g, c->phys, dt, T, c->f[TCS_U], c->f[TCS_V], c->f[TCS_W], c->wall[TCS_TBCUP], c->wall[TCS_TBCBT]};
+    TCS_TRY(z_stage(c, "thermal_z", "thermal_z_solve", z, 1, am, ac, ap, rhs));
+    TCS_TRY(x_stage(c, "thermal_x", "thermal_x_solve", ThermalX{g, c->phys, dt, T, c->f[TCS_U]}, 1, am, ac, ap, rhs));
+    const ThermalY y{g, c->phys, dt, T, c->f[TCS_V]};
+    if (rows_in_solve<scr_t>(c, c->n2m)) {
+        // the y rows read T around every cell, so the increment goes through
+        // the scratch and is added once every line is solved
+        TCS_TRY(y_stage_in_solve(c, "thermal_y_solve", y, 1, c->n2m, rhs, nullptr, 0));
+        TCS_LAUNCH(c->stream, "add_interior", k_interior_out<2>, interior_grid(c->n1m, c->n2m, c->n3m),
+                   interior_block(), 0, g, 1, T, rhs);
+        return TCS_OK;
+    }
+    TCS_TRY(rows3d_launch<false>(c->stream, "thermal_y", y, 1, am, ac, ap, rhs));
+    return sweep_y_field(c, 1, c->n2m, T, 2);
 }
 
 }  // namespace TCS_FORM
