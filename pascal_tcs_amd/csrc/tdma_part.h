@@ -289,9 +289,30 @@ __device__ __forceinline__ void store(const T (&a)[PT_MMAX], const T (&c)[PT_MMA
 
 }  // namespace part
 
+// Workgroup order for the solves that compute their rows.  Workgroups go
+// to the 8 XCDs round-robin in dispatch order, and the rows of a line group
+// read the fields of its neighbour groups in both grid directions.  So each
+// run of 32 workgroups an XCD receives in a row covers a compact tile of 4 x
+// 8 line groups, and consecutive tiles go to consecutive XCDs: neighbouring
+// groups mostly run at the same time on the same XCD and share its L2.
+// Grids that do not divide into whole tiles keep the plain order.
+constexpr int XT_X = 4, XT_Y = 8;
+__device__ __forceinline__ void xcd_tiles(int &bx, int &by) {
+    const int nbx = gridDim.x, nby = gridDim.y;
+    bx = blockIdx.x;
+    by = blockIdx.y;
+    if (nbx % XT_X != 0 || nby % XT_Y != 0 || (nbx * nby) % (8 * XT_X * XT_Y) != 0) return;
+    const int w = blockIdx.x + nbx * blockIdx.y;
+    const int xcd = w & 7, q = w >> 3, u = q % (XT_X * XT_Y);
+    const int tile = (q / (XT_X * XT_Y)) * 8 + xcd, ntx = nbx / XT_X;
+    bx = (tile % ntx) * XT_X + u % XT_X;
+    by = (tile / ntx) * XT_Y + u / XT_X;
+}
+
 // Lines strided in memory whose rows are computed, not read: src(s0, s1,
 // row, a, b, c, d) gives row `row` of line (s0, s1) (s0 < n0 along the
-// adjacent lines, s1 = blockIdx.y).  Each row enters the forward elimination
+// adjacent lines, s1 < gridDim.y; workgroups in xcd_tiles order).  Each row
+// enters the forward elimination
 // as soon as it is made, so no coefficient array goes through memory; the
 // thread layout, the reduced solve and the output (D at s0 + s1*ss1 +
 // row*rs, or a field) are those of the split solve of stored lines.
@@ -302,7 +323,9 @@ k_ptdma_part_src(Src src, T *D, int n0, long ss1, long rs, int P, int w1, int w2
     __shared__ RedLDS<T, PT_SX, 1> red;
     const int t = threadIdx.x;
     const int line = t % PT_SX, p = t / PT_SX;
-    const int s0 = blockIdx.x * PT_SX + line, s1 = blockIdx.y;
+    int bx, by;
+    xcd_tiles(bx, by);
+    const int s0 = bx * PT_SX + line, s1 = by;
     const bool act = s0 < n0;
     int j0, m;
     range(p, w1, w2, j0, m);
@@ -333,6 +356,7 @@ inline int stage_len(int rows) { return rows - 1 + ((rows - 1) >> 4) + 1; }
 // a line go to PT_RBLK blocks (fewer when P is smaller), block g holding
 // parts [g*P/nb, (g+1)*P/nb) and so rows [row0(g), row0(g+1)).
 constexpr int PT_RBLK = 4;
+constexpr int PT_EU = 2;  // rows a thread makes at once
 __host__ __device__ inline int rblocks(int P) { return P < PT_RBLK ? P : PT_RBLK; }
 __host__ __device__ inline int rblock_row0(int g, int P, int w1, int w2) {
     const int p = g * (P / rblocks(P));
@@ -357,9 +381,11 @@ k_ptdma_part_rows_src(Src src, T *D, int n0, long ss0, long ss1, int n_row, int 
     __shared__ RedLDS<T, PT_SX, 1> red;
     const int t = threadIdx.x, nt = blockDim.x;
     const int line = t % PT_SX, p = t / PT_SX;
-    const int s0 = blockIdx.x * PT_SX + line, s1 = blockIdx.y;
+    int bx, s1;
+    xcd_tiles(bx, s1);
+    const int s0 = bx * PT_SX + line;
     const bool act = s0 < n0;
-    const int nl = min(PT_SX, n0 - (int)blockIdx.x * PT_SX);  // lines present in this workgroup
+    const int nl = min(PT_SX, n0 - bx * PT_SX);  // lines present in this workgroup
     int j0, m;
     range(p, w1, w2, j0, m);
     const int nb = rblocks(P), mygrp = p / (P / nb);
@@ -368,16 +394,31 @@ k_ptdma_part_rows_src(Src src, T *D, int n0, long ss0, long ss1, int n_row, int 
     for (int g = 0; g < nb; g++) {
         const int r0 = rblock_row0(g, P, w1, w2), nr = rblock_row0(g + 1, P, w1, w2) - r0;
         if (g > 0) __syncthreads();  // the previous block has been read
-        // stage[(q*PT_SX + l)*lsb + stage_pos(r - r0)], q = a, b, c, d
-        for (int e = t; e < nr * PT_SX; e += nt) {
-            const int l = e / nr, r = e - l * nr;
-            T ar = T(0), br = T(1), cr = T(0), dr = T(0);
-            if (l < nl) src((int)blockIdx.x * PT_SX + l, s1, r0 + r, ar, br, cr, dr);
-            const int o = l * lsb + stage_pos(r);
-            stage[o] = ar;
-            stage[o + PT_SX * lsb] = br;
-            stage[o + 2 * PT_SX * lsb] = cr;
-            stage[o + 3 * PT_SX * lsb] = dr;
+        // stage[(q*PT_SX + l)*lsb + stage_pos(r - r0)], q = a, b, c, d; a
+        // thread makes PT_EU rows before it stores any, so their reads overlap
+        for (int e0 = t; e0 < nr * PT_SX; e0 += PT_EU * nt) {
+            T ar[PT_EU], br[PT_EU], cr[PT_EU], dr[PT_EU];
+            int o[PT_EU];
+#pragma unroll
+            for (int u = 0; u < PT_EU; u++) {
+                const int e = e0 + u * nt;
+                ar[u] = T(0); br[u] = T(1); cr[u] = T(0); dr[u] = T(0);
+                o[u] = -1;
+                if (e < nr * PT_SX) {
+                    const int l = e / nr, r = e - l * nr;
+                    if (l < nl) src(bx * PT_SX + l, s1, r0 + r, ar[u], br[u], cr[u], dr[u]);
+                    o[u] = l * lsb + stage_pos(r);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < PT_EU; u++) {
+                if (o[u] >= 0) {
+                    stage[o[u]] = ar[u];
+                    stage[o[u] + PT_SX * lsb] = br[u];
+                    stage[o[u] + 2 * PT_SX * lsb] = cr[u];
+                    stage[o[u] + 3 * PT_SX * lsb] = dr[u];
+                }
+            }
         }
         __syncthreads();
         if (mygrp == g) {
@@ -402,7 +443,7 @@ k_ptdma_part_rows_src(Src src, T *D, int n0, long ss0, long ss1, int n_row, int 
             if (r < m) stage[line * ls + stage_pos(j0 + r)] = x_of<T, 1>(a, c, d, 0, r, m, xf[0], xl[0]);
     }
     __syncthreads();
-    const long gbase = (long)blockIdx.x * PT_SX * ss0 + (long)s1 * ss1;
+    const long gbase = (long)bx * PT_SX * ss0 + (long)s1 * ss1;
     if (t < n_row) {
 #pragma unroll
         for (int l = 0; l < PT_SX; l++)

@@ -396,7 +396,11 @@ int thermal_solver(tcs_ctx *c, double dt) {
     scr_t *am = SCR(c->am), *ac = SCR(c->ac), *ap = SCR(c->ap), *rhs = SCR(c->rhs);
     const ThermalZ z{g, c->phys, dt, T, c->f[TCS_U], c->f[TCS_V], c->f[TCS_W], c->wall[TCS_TBCUP], c->wall[TCS_TBCBT]};
     TCS_TRY(z_stage(c, "thermal_z", "thermal_z_solve", z, 1, am, ac, ap, rhs));
-    TCS_TRY(x_stage(c, "thermal_x", "thermal_x_solve", ThermalX{g, c->phys, dt, T, c->f[TCS_U]}, 1, am, ac, ap, rhs));
+    // The x rows evaluate seven property polynomials each: made by the
+    // workgroup inside the contiguous-line solve they leave it waiting on
+    // them, so here they go through the scratch as a kernel of their own.
+    TCS_TRY(rows3d_launch<false>(c->stream, "thermal_x", ThermalX{g, c->phys, dt, T, c->f[TCS_U]}, 1, am, ac, ap, rhs));
+    TCS_TRY(ptdma_sweep(c, am, ac, ap, rhs, c->n2m, c->n1m, c->n3m, (long)c->n1m * c->n2m, c->n1m, 1, true));
     const ThermalY y{g, c->phys, dt, T, c->f[TCS_V]};
     if (rows_in_solve<scr_t>(c, c->n2m)) {
         // the y rows read T around every cell, so the increment goes through
