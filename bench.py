@@ -197,6 +197,90 @@ def run_cpu_baseline(seconds, deltat):
                       f"{steps} steps in {el:.1f} s"}
 
 
+def run_tdma_microbench(n, reps=5):
+    """Batched tridiagonal solves on n^3 FP64 arrays along x, y and z in turn
+    (n^2 systems of n rows per axis), non-cyclic and cyclic, with the split
+    kernels and the single-rank kernels.  Seeded systems: a, c ~ U(-1, 0),
+    b = |a| + |c| + 1 + U(0, 1), d ~ U(-1, 1).  Each timed solve starts from
+    fresh copies (the single-rank solve overwrites A and C); HIP events on
+    the stream the solve runs on bracket the solve alone.  40 algorithmic
+    bytes per unknown (read a, b, c, d; write x)."""
+    import torch
+    from pascal_tcs_amd import ptdma
+
+    g = torch.Generator(device="cuda")
+    g.manual_seed(20240512)
+    shape = (n, n, n)
+    a0 = -torch.rand(shape, generator=g, device="cuda", dtype=torch.float64)
+    c0 = -torch.rand(shape, generator=g, device="cuda", dtype=torch.float64)
+    b0 = a0.abs() + c0.abs() + 1.0 + torch.rand(shape, generator=g, device="cuda", dtype=torch.float64)
+    d0 = torch.rand(shape, generator=g, device="cuda", dtype=torch.float64) * 2.0 - 1.0
+    A, C, D = torch.empty_like(a0), torch.empty_like(c0), torch.empty_like(d0)
+    nbytes = 40.0 * n ** 3
+    cases = []
+    for axis, name in ((0, "x"), (1, "y"), (2, "z")):
+        for cyclic in (False, True):
+            for kind, solve in (("split", ptdma.axis_solve_split), ("single-rank", ptdma.axis_solve)):
+                times = []
+                for _ in range(reps + 1):
+                    A.copy_(a0); C.copy_(c0); D.copy_(d0)
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record()
+                    solve(A, b0, C, D, axis, cyclic)
+                    e1.record()
+                    torch.cuda.synchronize()
+                    times.append(e0.elapsed_time(e1))
+                ms = float(np.median(times[1:]))
+                gbs = nbytes / (ms / 1e3) / 1e9
+                cases.append({"axis": name, "cyclic": cyclic, "kernel": kind, "ms": round(ms, 4),
+                              "GB_per_s": round(gbs, 1), "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)})
+                print(f"tdma {name} cyclic={cyclic} {kind}: {ms:.3f} ms {gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    return {"metric": f"batched tridiagonal solve at {n}^3 (40 B per unknown)", "unit": "GB/s",
+            "systems_per_axis": n * n, "rows": n, "dtype": "f64", "data": "synthetic", "cases": cases}
+
+
+def run_mixed_drift(n, steps, deltat, dt, report=(10, 100, 250, 500, 750, 1000)):
+    """The mixed form and the plain FP64 form stepped side by side from the
+    same seeded state: per-field max and L2 errors of the mixed run relative
+    to the plain run, and the wall Nusselt numbers, at the listed steps."""
+    from pascal_tcs_amd.solver import TCSSolver
+    cfg = workload_config(n, deltat=deltat)
+    plain = TCSSolver(cfg, tdma_split=1, precision=1)
+    plain.setup_seeded(amp=0.01, seed=20240512)
+    T = plain.get("T")
+    T[:, -1, :] = cfg.Tup  # top wall row, as in the throughput run
+    plain.set("T", T)
+    del T
+    mixed = TCSSolver(cfg, tdma_split=1, precision=2)
+    for name in ("T", "U", "V", "W", "P", "dPhat", "dP0"):
+        mixed.set(name, plain.get(name))
+    for w in ("TBCup", "TBCbt", "UBCup", "UBCbt", "VBCup", "VBCbt", "WBCup", "WBCbt"):
+        mixed.set_wall(w, plain.get_wall(w))
+    rows = []
+    for step in range(1, steps + 1):
+        plain.step(dt)
+        mixed.step(dt)
+        if step in report or step == steps:
+            row = {"step": step}
+            for name in ("T", "U", "V", "W", "P"):
+                a, b = plain.get(name), mixed.get(name)
+                d = b - a
+                row[name] = {"max": float(f"{np.abs(d).max() / max(np.abs(a).max(), 1e-300):.3e}"),
+                             "l2": float(f"{np.linalg.norm(d) / max(np.linalg.norm(a), 1e-300):.3e}")}
+                del a, b, d
+            nbp, ntp = plain.nusselt()
+            nbm, ntm = mixed.nusselt()
+            row["Nu_plain"] = [nbp, ntp]
+            row["Nu_rel_drift"] = [float(f"{abs(nbm - nbp) / abs(nbp):.3e}"), float(f"{abs(ntm - ntp) / abs(ntp):.3e}")]
+            row["maxDivU_plain"] = plain.mpi_Post_Div()
+            rows.append(row)
+            print(f"drift step {step}: " + json.dumps(row), file=sys.stderr, flush=True)
+    plain.close()
+    mixed.close()
+    return {"metric": f"mixed form vs plain FP64 at {n}^3 after {steps} steps", "dt": dt, "deltat": deltat,
+            "errors": rows}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,6 +305,11 @@ def main():
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--no-mixed", dest="mixed", action="store_false",
                     help="skip the mixed form's run (by default it follows the plain run on the same workload)")
+    ap.add_argument("--tdma", action="store_true",
+                    help="instead of the step: the isolated batched tridiagonal solves at n^3 along x, y, z")
+    ap.add_argument("--mixed-drift", type=int, default=0, metavar="STEPS",
+                    help="instead of the step: run the mixed and plain forms side by side for STEPS steps "
+                         "and report the mixed form's per-field errors and Nusselt drift")
     args = ap.parse_args()
     if args.precision == 2:
         args.mixed = False
@@ -229,6 +318,12 @@ def main():
               "(multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
 
+    if args.tdma:
+        print(json.dumps(run_tdma_microbench(args.n)))
+        return
+    if args.mixed_drift > 0:
+        print(json.dumps(run_mixed_drift(args.n, args.mixed_drift, args.deltat, args.dt)))
+        return
     if args.copy_calibration:
         import torch
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
