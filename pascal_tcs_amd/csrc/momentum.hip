@@ -31,41 +31,50 @@ int sweep_y_field(tcs_ctx *c, int j0, int nrows, double *F, int mode);
 #define MU(j, k) Mu[g.t2((j), (k))]
 #define IR(j, k) iR[g.t2((j), (k))]
 
-// mpi_momentum_coeffi (:145-173) on the (j,k) table from T(n1sub,j,k)
+// mpi_momentum_coeffi (:145-173): Mu and invRho of one temperature
+template <class S>
+__device__ __forceinline__ void mu_ir_of_T(const Phys &p, S T, S &mu, S &ir) {
+    const S tmp1 = p.DeltaT * T;
+    const S tmp2 = tmp1 * tmp1;
+    const S tmp3 = tmp2 * tmp1;
+    const S tmp4 = tmp3 * tmp1;
+    const S tmp5 = tmp4 * tmp1;
+    mu = (p.a10 + p.a10 * (p.a11 * tmp1 + p.a12 * tmp2 + p.a13 * tmp3 + p.a14 * tmp4 + p.a15 * tmp5))
+       * (p.d10 + p.d10 * (p.d11 * tmp1 + p.d12 * tmp2 + p.d13 * tmp3 + p.d14 * tmp4 + p.d15 * tmp5)) / p.Mu0;
+    ir = p.Rho0 / (p.a10 + p.a10 * (p.a11 * tmp1 + p.a12 * tmp2 + p.a13 * tmp3 + p.a14 * tmp4 + p.a15 * tmp5));
+}
+
+// on the (j,k) table from T(n1sub,j,k)
 __global__ void k_momentum_coeffi(Grid g, Phys p, const double *__restrict__ T, double *__restrict__ Mu,
                                   double *__restrict__ iR) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
     if (j >= g.N2) return;
-    const double tmp1 = p.DeltaT * T[g.ix(g.n1m + 1, j, k)];
-    const double tmp2 = tmp1 * tmp1;
-    const double tmp3 = tmp2 * tmp1;
-    const double tmp4 = tmp3 * tmp1;
-    const double tmp5 = tmp4 * tmp1;
-    Mu[g.t2(j, k)] = (p.a10 + p.a10 * (p.a11 * tmp1 + p.a12 * tmp2 + p.a13 * tmp3 + p.a14 * tmp4 + p.a15 * tmp5))
-                   * (p.d10 + p.d10 * (p.d11 * tmp1 + p.d12 * tmp2 + p.d13 * tmp3 + p.d14 * tmp4 + p.d15 * tmp5)) / p.Mu0;
-    iR[g.t2(j, k)] = p.Rho0 / (p.a10 + p.a10 * (p.a11 * tmp1 + p.a12 * tmp2 + p.a13 * tmp3 + p.a14 * tmp4 + p.a15 * tmp5));
+    double mu, ir;
+    mu_ir_of_T(p, T[g.ix(g.n1m + 1, j, k)], mu, ir);
+    Mu[g.t2(j, k)] = mu;
+    iR[g.t2(j, k)] = ir;
 }
 
 // the stage-1 last-point viscosities that later stages reuse
-__global__ void k_momentum_stale(Grid g, const double *__restrict__ Mu, double *__restrict__ out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+template <class S, class FP>
+__device__ __forceinline__ void stale_values(const Grid &g, FP Mu, S (&out)[6]) {
     const double *dx1 = g.dx1, *dx2 = g.dx2, *dx3 = g.dx3, *dmx1 = g.dmx1, *dmx2 = g.dmx2, *dmx3 = g.dmx3;
     const int i = g.n1m, j = g.n2m, k = g.n3m;
     const int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1, km = k - 1;
     // solvedV stage 1 (:661-665)
     {
-        const double mua = 0.5 * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
-        const double muc = 0.5 * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
-        const double mub = 0.5 * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
+        const S mua = 0.5 * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
+        const S muc = 0.5 * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
+        const S mub = 0.5 * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
         out[0] = 0.5 * (dx1[im] * muc + dx1[i] * mua) / dmx1[i];
         out[1] = 0.5 * (dx1[ip] * muc + dx1[i] * mub) / dmx1[ip];
     }
     // solvedW stage 1 (:1022-1032)
     {
-        double mua = 0.5 * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
-        double muc = 0.5 * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
-        double mub = 0.5 * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
+        S mua = 0.5 * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
+        S muc = 0.5 * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
+        S mub = 0.5 * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
         out[2] = 0.5 * (dx1[im] * muc + dx1[i] * mua) / dmx1[i];
         out[3] = 0.5 * (dx1[ip] * muc + dx1[i] * mub) / dmx1[ip];
         mua = 0.5 * (dx3[km] * MU(jm, k) + dx3[k] * MU(jm, km)) / dmx3[k];
@@ -74,6 +83,13 @@ __global__ void k_momentum_stale(Grid g, const double *__restrict__ Mu, double *
         out[4] = 0.5 * (dx2[jm] * muc + dx2[j] * mua) / dmx2[j];
         out[5] = 0.5 * (dx2[jp] * muc + dx2[j] * mub) / dmx2[jp];
     }
+}
+
+__global__ void k_momentum_stale(Grid g, const double *__restrict__ Mu, double *__restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double v[6];
+    stale_values<double>(g, Mu, v);
+    for (int q = 0; q < 6; q++) out[q] = v[q];
 }
 
 #define KERNEL_IJK                                                            \
@@ -96,65 +112,66 @@ __global__ void k_momentum_stale(Grid g, const double *__restrict__ Mu, double *
 // right-hand side; the functor after it carries its arguments.
 
 // ---------------------------------------------------------------- dU -------
-__device__ __forceinline__ void dU_z_row(const Grid &g, const Phys &p, double dt, const double *__restrict__ U,
-    const double *__restrict__ V, const double *__restrict__ W, const double *__restrict__ P,
-    const double *__restrict__ T, const double *__restrict__ Mu, const double *__restrict__ iR,
-    const double *__restrict__ UBCup, const double *__restrict__ UBCbt, const double *__restrict__ VBCup,
-    const double *__restrict__ VBCbt,
-    int i, int j, int k, double &am, double &ac, double &ap, double &rhsout) {
+template <class S, class FP>
+__device__ __forceinline__ void dU_z_row(const Grid &g, const Phys &p, double dt, FP U,
+    FP V, FP W, FP P,
+    FP T, FP Mu, FP iR,
+    FP UBCup, FP UBCbt, FP VBCup,
+    FP VBCbt,
+    int i, int j, int k, S &am, S &ac, S &ap, S &rhsout) {
     ROW_METRICS
     const int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1, km = k - 1, kp = k + 1;
     const int jum = g.jC[jm], jup = g.jC[jp];
-    const double Cmu = p.Cmu;
+    const S Cmu = p.Cmu;
 #define Uf(a, b, c) U[g.ix(a, b, c)]
 #define Vf(a, b, c) V[g.ix(a, b, c)]
 #define Wf(a, b, c) W[g.ix(a, b, c)]
-    const double u1 = 0.5 * (Uf(im, j, k) + Uf(i, j, k));
-    const double u2 = 0.5 * (Uf(ip, j, k) + Uf(i, j, k));
-    const double v3 = 0.5 * (dx1[im] * Vf(i, j, k) + dx1[i] * Vf(im, j, k)) / dmx1[i];
-    const double v4 = 0.5 * (dx1[im] * Vf(i, jp, k) + dx1[i] * Vf(im, jp, k)) / dmx1[i];
-    const double w5 = 0.5 * (dx1[im] * Wf(i, j, k) + dx1[i] * Wf(im, j, k)) / dmx1[i];
-    const double w6 = 0.5 * (dx1[im] * Wf(i, j, kp) + dx1[i] * Wf(im, j, kp)) / dmx1[i];
+    const S u1 = 0.5 * (Uf(im, j, k) + Uf(i, j, k));
+    const S u2 = 0.5 * (Uf(ip, j, k) + Uf(i, j, k));
+    const S v3 = 0.5 * (dx1[im] * Vf(i, j, k) + dx1[i] * Vf(im, j, k)) / dmx1[i];
+    const S v4 = 0.5 * (dx1[im] * Vf(i, jp, k) + dx1[i] * Vf(im, jp, k)) / dmx1[i];
+    const S w5 = 0.5 * (dx1[im] * Wf(i, j, k) + dx1[i] * Wf(im, j, k)) / dmx1[i];
+    const S w6 = 0.5 * (dx1[im] * Wf(i, j, kp) + dx1[i] * Wf(im, j, kp)) / dmx1[i];
 
-    const double dudx1 = (Uf(i, j, k) - Uf(im, j, k)) / dx1[im];
-    const double dudx2 = (Uf(ip, j, k) - Uf(i, j, k)) / dx1[i];
-    const double dudy3 = (Uf(i, j, k) - Uf(i, jm, k)) / dmx2[j];
-    const double dudy4 = (Uf(i, jp, k) - Uf(i, j, k)) / dmx2[jp];
-    const double dudz5 = (Uf(i, j, k) - Uf(i, j, km)) / dmx3[k];
-    const double dudz6 = (Uf(i, j, kp) - Uf(i, j, k)) / dmx3[kp];
-    const double dvdx3 = (Vf(i, j, k) - Vf(im, j, k)) / dmx1[i];
-    const double dvdx4 = (Vf(i, jp, k) - Vf(im, jp, k)) / dmx1[i];
-    const double dwdx5 = (Wf(i, j, k) - Wf(im, j, k)) / dmx1[i];
-    const double dwdx6 = (Wf(i, j, kp) - Wf(im, j, kp)) / dmx1[i];
+    const S dudx1 = (Uf(i, j, k) - Uf(im, j, k)) / dx1[im];
+    const S dudx2 = (Uf(ip, j, k) - Uf(i, j, k)) / dx1[i];
+    const S dudy3 = (Uf(i, j, k) - Uf(i, jm, k)) / dmx2[j];
+    const S dudy4 = (Uf(i, jp, k) - Uf(i, j, k)) / dmx2[jp];
+    const S dudz5 = (Uf(i, j, k) - Uf(i, j, km)) / dmx3[k];
+    const S dudz6 = (Uf(i, j, kp) - Uf(i, j, k)) / dmx3[kp];
+    const S dvdx3 = (Vf(i, j, k) - Vf(im, j, k)) / dmx1[i];
+    const S dvdx4 = (Vf(i, jp, k) - Vf(im, jp, k)) / dmx1[i];
+    const S dwdx5 = (Wf(i, j, k) - Wf(im, j, k)) / dmx1[i];
+    const S dwdx6 = (Wf(i, j, kp) - Wf(im, j, kp)) / dmx1[i];
 
-    double mua = 0.5 * (dx1[im] * MU(jm, k) + dx1[i] * MU(jm, k)) / dmx1[i];
-    double muc = 0.5 * (dx1[im] * MU(j, k) + dx1[i] * MU(j, k)) / dmx1[i];
-    double mub = 0.5 * (dx1[im] * MU(jp, k) + dx1[i] * MU(jp, k)) / dmx1[i];
-    const double mu3 = 0.5 * (dx2[jm] * muc + dx2[j] * mua) / dmx2[j];
-    const double mu4 = 0.5 * (dx2[jp] * muc + dx2[j] * mub) / dmx2[jp];
+    S mua = 0.5 * (dx1[im] * MU(jm, k) + dx1[i] * MU(jm, k)) / dmx1[i];
+    S muc = 0.5 * (dx1[im] * MU(j, k) + dx1[i] * MU(j, k)) / dmx1[i];
+    S mub = 0.5 * (dx1[im] * MU(jp, k) + dx1[i] * MU(jp, k)) / dmx1[i];
+    const S mu3 = 0.5 * (dx2[jm] * muc + dx2[j] * mua) / dmx2[j];
+    const S mu4 = 0.5 * (dx2[jp] * muc + dx2[j] * mub) / dmx2[jp];
     mua = 0.5 * (dx1[im] * MU(j, km) + dx1[i] * MU(j, km)) / dmx1[i];
     muc = 0.5 * (dx1[im] * MU(j, k) + dx1[i] * MU(j, k)) / dmx1[i];
     mub = 0.5 * (dx1[im] * MU(j, kp) + dx1[i] * MU(j, kp)) / dmx1[i];
-    const double mu5 = 0.5 * (dx3[km] * muc + dx3[k] * mua) / dmx3[k];
-    const double mu6 = 0.5 * (dx3[kp] * muc + dx3[k] * mub) / dmx3[kp];
+    const S mu5 = 0.5 * (dx3[km] * muc + dx3[k] * mua) / dmx3[k];
+    const S mu6 = 0.5 * (dx3[kp] * muc + dx3[k] * mub) / dmx3[kp];
 
-    const double invRhoc = 0.5 * (dx1[im] * IR(j, k) + dx1[i] * IR(j, k)) / dmx1[i];
-    const double invRhocCmu_half = 0.5 * Cmu * invRhoc;
+    const S invRhoc = 0.5 * (dx1[im] * IR(j, k) + dx1[i] * IR(j, k)) / dmx1[i];
+    const S invRhocCmu_half = 0.5 * Cmu * invRhoc;
 
-    const double viscous_u1 = 1.0 * (MU(j, k) * dudx2 - MU(j, k) * dudx1) / dmx1[i];
-    const double viscous_u2 = 1.0 * (mu4 * dudy4 - mu3 * dudy3) / dx2[j];
-    const double viscous_u3 = 1.0 * (mu6 * dudz6 - mu5 * dudz5) / dx3[k];
-    const double viscous_u12 = 1.0 * (mu4 * dvdx4 - mu3 * dvdx3) / dx2[j];
-    const double viscous_u13 = 1.0 * (mu6 * dwdx6 - mu5 * dwdx5) / dx3[k];
+    const S viscous_u1 = 1.0 * (MU(j, k) * dudx2 - MU(j, k) * dudx1) / dmx1[i];
+    const S viscous_u2 = 1.0 * (mu4 * dudy4 - mu3 * dudy3) / dx2[j];
+    const S viscous_u3 = 1.0 * (mu6 * dudz6 - mu5 * dudz5) / dx3[k];
+    const S viscous_u12 = 1.0 * (mu4 * dvdx4 - mu3 * dvdx3) / dx2[j];
+    const S viscous_u13 = 1.0 * (mu6 * dwdx6 - mu5 * dwdx5) / dx3[k];
 
-    double rhs = invRhocCmu_half * (2.0 * viscous_u1 + viscous_u2 + viscous_u3 + viscous_u12 + viscous_u13);
+    S rhs = invRhocCmu_half * (2.0 * viscous_u1 + viscous_u2 + viscous_u3 + viscous_u12 + viscous_u13);
     rhs = rhs - p.Cmp * invRhoc * (P[g.ix(i, j, k)] - P[g.ix(im, j, k)]) / dmx1[i];
-    const double Tc = 0.5 * (dx1[im] * T[g.ix(i, j, k)] + dx1[i] * T[g.ix(im, j, k)]) / dmx1[i];
+    const S Tc = 0.5 * (dx1[im] * T[g.ix(i, j, k)] + dx1[i] * T[g.ix(im, j, k)]) / dmx1[i];
     rhs = rhs + p.Cmt * (Tc + p.a12pera11 * (Tc * Tc) * p.DeltaT) * invRhoc;
 
-    double ubc_down = 0.25 * v3 / dmx2[j] * UBCbt[g.b2(i, k)]
+    S ubc_down = 0.25 * v3 / dmx2[j] * UBCbt[g.b2(i, k)]
                     - 0.25 * dudy3 * (dx1[i] * VBCbt[g.b2(im, k)] + dx1[im] * VBCbt[g.b2(i, k)]) / dmx1[i] * 0.5;
-    double ubc_up = -0.25 * v4 / dmx2[jp] * UBCup[g.b2(i, k)]
+    S ubc_up = -0.25 * v4 / dmx2[jp] * UBCup[g.b2(i, k)]
                   - 0.25 * dudy4 * (dx1[i] * VBCup[g.b2(im, k)] + dx1[im] * VBCup[g.b2(i, k)]) / dmx1[i] * 0.5;
     ubc_down = ubc_down + invRhocCmu_half / dx2[j] * mu3 / dmx2[j] * UBCbt[g.b2(i, k)];
     ubc_down = ubc_down - invRhocCmu_half / dx2[j] * mu3 / dmx1[i] * (VBCbt[g.b2(i, k)] - VBCbt[g.b2(im, k)]);
@@ -162,18 +179,18 @@ __device__ __forceinline__ void dU_z_row(const Grid &g, const Phys &p, double dt
     ubc_up = ubc_up + invRhocCmu_half / dx2[j] * mu4 / dmx1[i] * (VBCup[g.b2(i, k)] - VBCup[g.b2(im, k)]);
     rhs = rhs + (1.0 - (double)jum) * ubc_down + (1.0 - (double)jup) * ubc_up;
 
-    const double mACI = invRhocCmu_half / dmx1[i] * (MU(j, k) / dx1[i] + MU(j, k) / dx1[im]) * 2.0
+    const S mACI = invRhocCmu_half / dmx1[i] * (MU(j, k) / dx1[i] + MU(j, k) / dx1[im]) * 2.0
                       + 0.25 * (dudx2 * 0.5 + dudx1 * 0.5 - u2 / dx1[i] + u1 / dx1[im]);
-    const double mAPI = -invRhocCmu_half / dmx1[i] * MU(j, k) / dx1[i] * 2.0 + 0.25 * (u2 / dx1[i] + dudx2 * 0.5);
-    const double mAMI = -invRhocCmu_half / dmx1[i] * MU(j, k) / dx1[im] * 2.0 + 0.25 * (-u1 / dx1[im] + dudx1 * 0.5);
-    const double mACJ = invRhocCmu_half / dx2[j] * (mu4 / dmx2[jp] + mu3 / dmx2[j]) + 0.25 * (-v4 / dmx2[jp] + v3 / dmx2[j]);
-    double mAPJ = -invRhocCmu_half / dx2[j] * mu4 / dmx2[jp] + 0.25 * (v4 / dmx2[jp]);
-    double mAMJ = -invRhocCmu_half / dx2[j] * mu3 / dmx2[j] + 0.25 * (-v3 / dmx2[j]);
+    const S mAPI = -invRhocCmu_half / dmx1[i] * MU(j, k) / dx1[i] * 2.0 + 0.25 * (u2 / dx1[i] + dudx2 * 0.5);
+    const S mAMI = -invRhocCmu_half / dmx1[i] * MU(j, k) / dx1[im] * 2.0 + 0.25 * (-u1 / dx1[im] + dudx1 * 0.5);
+    const S mACJ = invRhocCmu_half / dx2[j] * (mu4 / dmx2[jp] + mu3 / dmx2[j]) + 0.25 * (-v4 / dmx2[jp] + v3 / dmx2[j]);
+    S mAPJ = -invRhocCmu_half / dx2[j] * mu4 / dmx2[jp] + 0.25 * (v4 / dmx2[jp]);
+    S mAMJ = -invRhocCmu_half / dx2[j] * mu3 / dmx2[j] + 0.25 * (-v3 / dmx2[j]);
     mAPJ = mAPJ * (double)jup;
     mAMJ = mAMJ * (double)jum;
-    const double mACK = invRhocCmu_half / dx3[k] * (mu6 / dmx3[kp] + mu5 / dmx3[k]) + 0.25 * (-w6 / dmx3[kp] + w5 / dmx3[k]);
-    const double mAPK = -invRhocCmu_half / dx3[k] * mu6 / dmx3[kp] + 0.25 * (w6 / dmx3[kp]);
-    const double mAMK = -invRhocCmu_half / dx3[k] * mu5 / dmx3[k] + 0.25 * (-w5 / dmx3[k]);
+    const S mACK = invRhocCmu_half / dx3[k] * (mu6 / dmx3[kp] + mu5 / dmx3[k]) + 0.25 * (-w6 / dmx3[kp] + w5 / dmx3[k]);
+    const S mAPK = -invRhocCmu_half / dx3[k] * mu6 / dmx3[kp] + 0.25 * (w6 / dmx3[kp]);
+    const S mAMK = -invRhocCmu_half / dx3[k] * mu5 / dmx3[k] + 0.25 * (-w5 / dmx3[k]);
 
     rhs = rhs - (mAPI * Uf(ip, j, k) + mACI * Uf(i, j, k) + mAMI * Uf(im, j, k)
                  + mAPJ * Uf(i, jp, k) + mACJ * Uf(i, j, k) + mAMJ * Uf(i, jm, k)
@@ -192,27 +209,33 @@ struct DUZ {
     Phys p;
     double dt;
     const double *U, *V, *W, *P, *T, *Mu, *iR, *UBCup, *UBCbt, *VBCup, *VBCbt;
+    static constexpr int NF = 11;  // field inputs, in the order below
+    template <class S, class Wr>
+    __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
+        dU_z_row<S>(g, p, dt, w(0, U), w(1, V), w(2, W), w(3, P), w(4, T), w(5, Mu), w(6, iR), w(7, UBCup), w(8, UBCbt), w(9, VBCup), w(10, VBCbt), i, j, k, a, b, c, d);
+    }
     __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
                                                double &d) const {
-        dU_z_row(g, p, dt, U, V, W, P, T, Mu, iR, UBCup, UBCbt, VBCup, VBCbt, i, j, k, a, b, c, d);
+        eval<double>(RawW(), i, j, k, a, b, c, d);
     }
 };
 
-__device__ __forceinline__ void dU_x_row(const Grid &g, const Phys &p, double dt, const double *__restrict__ U,
-    const double *__restrict__ Mu, const double *__restrict__ iR,
-    int i, int j, int k, double &am, double &ac, double &ap) {
+template <class S, class FP>
+__device__ __forceinline__ void dU_x_row(const Grid &g, const Phys &p, double dt, FP U,
+    FP Mu, FP iR,
+    int i, int j, int k, S &am, S &ac, S &ap) {
     ROW_METRICS
     const int im = i - 1, ip = i + 1;
-    const double u1 = 0.5 * (Uf(im, j, k) + Uf(i, j, k));
-    const double u2 = 0.5 * (Uf(ip, j, k) + Uf(i, j, k));
-    const double dudx1 = (Uf(i, j, k) - Uf(im, j, k)) / dx1[im];
-    const double dudx2 = (Uf(ip, j, k) - Uf(i, j, k)) / dx1[i];
-    const double invRhoc = 0.5 * (dx1[im] * IR(j, k) + dx1[i] * IR(j, k)) / dmx1[i];
-    const double invRhocCmu_half = 0.5 * p.Cmu * invRhoc;
-    const double mACI = invRhocCmu_half / dmx1[i] * (MU(j, k) / dx1[i] + MU(j, k) / dx1[im]) * 2.0
+    const S u1 = 0.5 * (Uf(im, j, k) + Uf(i, j, k));
+    const S u2 = 0.5 * (Uf(ip, j, k) + Uf(i, j, k));
+    const S dudx1 = (Uf(i, j, k) - Uf(im, j, k)) / dx1[im];
+    const S dudx2 = (Uf(ip, j, k) - Uf(i, j, k)) / dx1[i];
+    const S invRhoc = 0.5 * (dx1[im] * IR(j, k) + dx1[i] * IR(j, k)) / dmx1[i];
+    const S invRhocCmu_half = 0.5 * p.Cmu * invRhoc;
+    const S mACI = invRhocCmu_half / dmx1[i] * (MU(j, k) / dx1[i] + MU(j, k) / dx1[im]) * 2.0
                       + 0.25 * (dudx2 * 0.5 + dudx1 * 0.5 - u2 / dx1[i] + u1 / dx1[im]);
-    const double mAPI = -invRhocCmu_half / dmx1[i] * MU(j, k) / dx1[i] * 2.0 + 0.25 * (u2 / dx1[i] + dudx2 * 0.5);
-    const double mAMI = -invRhocCmu_half / dmx1[i] * MU(j, k) / dx1[im] * 2.0 + 0.25 * (-u1 / dx1[im] + dudx1 * 0.5);
+    const S mAPI = -invRhocCmu_half / dmx1[i] * MU(j, k) / dx1[i] * 2.0 + 0.25 * (u2 / dx1[i] + dudx2 * 0.5);
+    const S mAMI = -invRhocCmu_half / dmx1[i] * MU(j, k) / dx1[im] * 2.0 + 0.25 * (-u1 / dx1[im] + dudx1 * 0.5);
     ac = mACI * dt + 1.0;
     ap = mAPI * dt;
     am = mAMI * dt;
@@ -222,30 +245,36 @@ struct DUX {
     Phys p;
     double dt;
     const double *U, *Mu, *iR;
+    static constexpr int NF = 3;  // field inputs, in the order below
+    template <class S, class Wr>
+    __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
+        dU_x_row<S>(g, p, dt, w(0, U), w(1, Mu), w(2, iR), i, j, k, a, b, c);
+    }
     __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
                                                double &d) const {
-        dU_x_row(g, p, dt, U, Mu, iR, i, j, k, a, b, c);
+        eval<double>(RawW(), i, j, k, a, b, c, d);
     }
 };
 
-__device__ __forceinline__ void dU_y_row(const Grid &g, const Phys &p, double dt, const double *__restrict__ V,
-    const double *__restrict__ Mu, const double *__restrict__ iR,
-    int i, int j, int k, double &am, double &ac, double &ap) {
+template <class S, class FP>
+__device__ __forceinline__ void dU_y_row(const Grid &g, const Phys &p, double dt, FP V,
+    FP Mu, FP iR,
+    int i, int j, int k, S &am, S &ac, S &ap) {
     ROW_METRICS
     const int im = i - 1, jm = j - 1, jp = j + 1;
     const int jum = g.jC[jm], jup = g.jC[jp];
-    const double v3 = 0.5 * (dx1[im] * Vf(i, j, k) + dx1[i] * Vf(im, j, k)) / dmx1[i];
-    const double v4 = 0.5 * (dx1[im] * Vf(i, jp, k) + dx1[i] * Vf(im, jp, k)) / dmx1[i];
-    const double mua = 0.5 * (dx1[im] * MU(jm, k) + dx1[i] * MU(jm, k)) / dmx1[i];
-    const double muc = 0.5 * (dx1[im] * MU(j, k) + dx1[i] * MU(j, k)) / dmx1[i];
-    const double mub = 0.5 * (dx1[im] * MU(jp, k) + dx1[i] * MU(jp, k)) / dmx1[i];
-    const double mu3 = 0.5 * (dx2[jm] * muc + dx2[j] * mua) / dmx2[j];
-    const double mu4 = 0.5 * (dx2[jp] * muc + dx2[j] * mub) / dmx2[jp];
-    const double invRhoc = 0.5 * (dx1[im] * IR(j, k) + dx1[i] * IR(j, k)) / dmx1[i];
-    const double invRhocCmu_half = 0.5 * p.Cmu * invRhoc;
-    const double mACJ = invRhocCmu_half / dx2[j] * (mu4 / dmx2[jp] + mu3 / dmx2[j]) + 0.25 * (-v4 / dmx2[jp] + v3 / dmx2[j]);
-    double mAPJ = -invRhocCmu_half / dx2[j] * mu4 / dmx2[jp] + 0.25 * (v4 / dmx2[jp]);
-    double mAMJ = -invRhocCmu_half / dx2[j] * mu3 / dmx2[j] + 0.25 * (-v3 / dmx2[j]);
+    const S v3 = 0.5 * (dx1[im] * Vf(i, j, k) + dx1[i] * Vf(im, j, k)) / dmx1[i];
+    const S v4 = 0.5 * (dx1[im] * Vf(i, jp, k) + dx1[i] * Vf(im, jp, k)) / dmx1[i];
+    const S mua = 0.5 * (dx1[im] * MU(jm, k) + dx1[i] * MU(jm, k)) / dmx1[i];
+    const S muc = 0.5 * (dx1[im] * MU(j, k) + dx1[i] * MU(j, k)) / dmx1[i];
+    const S mub = 0.5 * (dx1[im] * MU(jp, k) + dx1[i] * MU(jp, k)) / dmx1[i];
+    const S mu3 = 0.5 * (dx2[jm] * muc + dx2[j] * mua) / dmx2[j];
+    const S mu4 = 0.5 * (dx2[jp] * muc + dx2[j] * mub) / dmx2[jp];
+    const S invRhoc = 0.5 * (dx1[im] * IR(j, k) + dx1[i] * IR(j, k)) / dmx1[i];
+    const S invRhocCmu_half = 0.5 * p.Cmu * invRhoc;
+    const S mACJ = invRhocCmu_half / dx2[j] * (mu4 / dmx2[jp] + mu3 / dmx2[j]) + 0.25 * (-v4 / dmx2[jp] + v3 / dmx2[j]);
+    S mAPJ = -invRhocCmu_half / dx2[j] * mu4 / dmx2[jp] + 0.25 * (v4 / dmx2[jp]);
+    S mAMJ = -invRhocCmu_half / dx2[j] * mu3 / dmx2[j] + 0.25 * (-v3 / dmx2[j]);
     mAPJ = mAPJ * (double)jup;
     mAMJ = mAMJ * (double)jum;
     ac = mACJ * dt + 1.0;
@@ -257,9 +286,14 @@ struct DUY {
     Phys p;
     double dt;
     const double *V, *Mu, *iR;
+    static constexpr int NF = 3;  // field inputs, in the order below
+    template <class S, class Wr>
+    __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
+        dU_y_row<S>(g, p, dt, w(0, V), w(1, Mu), w(2, iR), i, j, k, a, b, c);
+    }
     __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
                                                double &d) const {
-        dU_y_row(g, p, dt, V, Mu, iR, i, j, k, a, b, c);
+        eval<double>(RawW(), i, j, k, a, b, c, d);
     }
 };
 
@@ -275,80 +309,81 @@ __global__ void k_dV_zero_rows(Grid g, double *__restrict__ dV) {
 }
 
 // ---------------------------------------------------------------- dV -------
-__device__ __forceinline__ void dV_z_row(const Grid &g, const Phys &p, double dt, const double *__restrict__ U,
-    const double *__restrict__ V, const double *__restrict__ W, const double *__restrict__ P,
-    const double *__restrict__ dU, const double *__restrict__ Mu, const double *__restrict__ iR,
-    const double *__restrict__ VBCup, const double *__restrict__ VBCbt,
-    int i, int j, int k, double &am, double &ac, double &ap, double &rhsout) {
+template <class S, class FP>
+__device__ __forceinline__ void dV_z_row(const Grid &g, const Phys &p, double dt, FP U,
+    FP V, FP W, FP P,
+    FP dU, FP Mu, FP iR,
+    FP VBCup, FP VBCbt,
+    int i, int j, int k, S &am, S &ac, S &ap, S &rhsout) {
     ROW_METRICS
     const int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1, km = k - 1, kp = k + 1;
     const int jvp = g.jS[jm], jvm = g.jS[jp];
-    const double Cmu = p.Cmu;
+    const S Cmu = p.Cmu;
 #define dUf(a, b, c) dU[g.ix(a, b, c)]
-    const double v3 = 0.5 * (Vf(i, jm, k) + Vf(i, j, k));
-    const double v4 = 0.5 * (Vf(i, j, k) + Vf(i, jp, k));
-    const double u1 = (dx2[jm] * Uf(i, j, k) + dx2[j] * Uf(i, jm, k)) / dmx2[j] * 0.5;
-    const double u2 = (dx2[jm] * Uf(ip, j, k) + dx2[j] * Uf(ip, jm, k)) / dmx2[j] * 0.5;
-    const double w5 = (dx2[jm] * Wf(i, j, k) + dx2[j] * Wf(i, jm, k)) / dmx2[j] * 0.5;
-    const double w6 = (dx2[jm] * Wf(i, j, kp) + dx2[j] * Wf(i, jm, kp)) / dmx2[j] * 0.5;
-    const double ddu1 = (dx2[jm] * dUf(i, j, k) + dx2[j] * dUf(i, jm, k)) / dmx2[j] * 0.5;
-    const double ddu2 = (dx2[jm] * dUf(ip, j, k) + dx2[j] * dUf(ip, jm, k)) / dmx2[j] * 0.5;
+    const S v3 = 0.5 * (Vf(i, jm, k) + Vf(i, j, k));
+    const S v4 = 0.5 * (Vf(i, j, k) + Vf(i, jp, k));
+    const S u1 = (dx2[jm] * Uf(i, j, k) + dx2[j] * Uf(i, jm, k)) / dmx2[j] * 0.5;
+    const S u2 = (dx2[jm] * Uf(ip, j, k) + dx2[j] * Uf(ip, jm, k)) / dmx2[j] * 0.5;
+    const S w5 = (dx2[jm] * Wf(i, j, k) + dx2[j] * Wf(i, jm, k)) / dmx2[j] * 0.5;
+    const S w6 = (dx2[jm] * Wf(i, j, kp) + dx2[j] * Wf(i, jm, kp)) / dmx2[j] * 0.5;
+    const S ddu1 = (dx2[jm] * dUf(i, j, k) + dx2[j] * dUf(i, jm, k)) / dmx2[j] * 0.5;
+    const S ddu2 = (dx2[jm] * dUf(ip, j, k) + dx2[j] * dUf(ip, jm, k)) / dmx2[j] * 0.5;
 
-    const double dvdx1 = (Vf(i, j, k) - Vf(im, j, k)) / dmx1[i];
-    const double dvdx2 = (Vf(ip, j, k) - Vf(i, j, k)) / dmx1[ip];
-    const double dvdy3 = (Vf(i, j, k) - Vf(i, jm, k)) / dx2[jm];
-    const double dvdy4 = (Vf(i, jp, k) - Vf(i, j, k)) / dx2[j];
-    const double dvdz5 = (Vf(i, j, k) - Vf(i, j, km)) / dmx3[k];
-    const double dvdz6 = (Vf(i, j, kp) - Vf(i, j, k)) / dmx3[kp];
-    const double dudy1 = (Uf(i, j, k) - Uf(i, jm, k)) / dmx2[j];
-    const double dudy2 = (Uf(ip, j, k) - Uf(ip, jm, k)) / dmx2[j];
-    const double dwdy5 = (Wf(i, j, k) - Wf(i, jm, k)) / dmx2[j];
-    const double dwdy6 = (Wf(i, j, kp) - Wf(i, jm, kp)) / dmx2[j];
-    const double ddudy1 = (dUf(i, j, k) - dUf(i, jm, k)) / dmx2[j];
-    const double ddudy2 = (dUf(ip, j, k) - dUf(ip, jm, k)) / dmx2[j];
+    const S dvdx1 = (Vf(i, j, k) - Vf(im, j, k)) / dmx1[i];
+    const S dvdx2 = (Vf(ip, j, k) - Vf(i, j, k)) / dmx1[ip];
+    const S dvdy3 = (Vf(i, j, k) - Vf(i, jm, k)) / dx2[jm];
+    const S dvdy4 = (Vf(i, jp, k) - Vf(i, j, k)) / dx2[j];
+    const S dvdz5 = (Vf(i, j, k) - Vf(i, j, km)) / dmx3[k];
+    const S dvdz6 = (Vf(i, j, kp) - Vf(i, j, k)) / dmx3[kp];
+    const S dudy1 = (Uf(i, j, k) - Uf(i, jm, k)) / dmx2[j];
+    const S dudy2 = (Uf(ip, j, k) - Uf(ip, jm, k)) / dmx2[j];
+    const S dwdy5 = (Wf(i, j, k) - Wf(i, jm, k)) / dmx2[j];
+    const S dwdy6 = (Wf(i, j, kp) - Wf(i, jm, kp)) / dmx2[j];
+    const S ddudy1 = (dUf(i, j, k) - dUf(i, jm, k)) / dmx2[j];
+    const S ddudy2 = (dUf(ip, j, k) - dUf(ip, jm, k)) / dmx2[j];
 
-    double mua = 0.5 * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
-    double muc = 0.5 * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
-    double mub = 0.5 * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
-    const double mu1 = 0.5 * (dx1[im] * muc + dx1[i] * mua) / dmx1[i];
-    const double mu2 = 0.5 * (dx1[ip] * muc + dx1[i] * mub) / dmx1[ip];
+    S mua = 0.5 * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
+    S muc = 0.5 * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
+    S mub = 0.5 * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
+    const S mu1 = 0.5 * (dx1[im] * muc + dx1[i] * mua) / dmx1[i];
+    const S mu2 = 0.5 * (dx1[ip] * muc + dx1[i] * mub) / dmx1[ip];
     mua = 0.5 * (dx2[jm] * MU(j, km) + dx2[j] * MU(jm, km)) / dmx2[j];
     muc = 0.5 * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
     mub = 0.5 * (dx2[jm] * MU(j, kp) + dx2[j] * MU(jm, kp)) / dmx2[j];
-    const double mu5 = 0.5 * (dx3[km] * muc + dx3[k] * mua) / dmx3[k];
-    const double mu6 = 0.5 * (dx3[kp] * muc + dx3[k] * mub) / dmx3[kp];
+    const S mu5 = 0.5 * (dx3[km] * muc + dx3[k] * mua) / dmx3[k];
+    const S mu6 = 0.5 * (dx3[kp] * muc + dx3[k] * mub) / dmx3[kp];
 
-    const double invRhoc = 0.5 * (dx2[jm] * IR(j, k) + dx2[j] * IR(jm, k)) / dmx2[j];
-    const double invRhocCmu_half = 0.5 * Cmu * invRhoc;
+    const S invRhoc = 0.5 * (dx2[jm] * IR(j, k) + dx2[j] * IR(jm, k)) / dmx2[j];
+    const S invRhocCmu_half = 0.5 * Cmu * invRhoc;
 
-    const double viscous_v1 = 1.0 * (mu2 * dvdx2 - mu1 * dvdx1) / dx1[i];
-    const double viscous_v2 = 1.0 * (MU(j, k) * dvdy4 - MU(jm, k) * dvdy3) / dmx2[j];
-    const double viscous_v3 = 1.0 * (mu6 * dvdz6 - mu5 * dvdz5) / dx3[k];
-    const double viscous_v21 = 1.0 * (mu2 * dudy2 - mu1 * dudy1) / dx1[i];
-    const double viscous_v23 = 1.0 * (mu6 * dwdy6 - mu5 * dwdy5) / dx3[k];
+    const S viscous_v1 = 1.0 * (mu2 * dvdx2 - mu1 * dvdx1) / dx1[i];
+    const S viscous_v2 = 1.0 * (MU(j, k) * dvdy4 - MU(jm, k) * dvdy3) / dmx2[j];
+    const S viscous_v3 = 1.0 * (mu6 * dvdz6 - mu5 * dvdz5) / dx3[k];
+    const S viscous_v21 = 1.0 * (mu2 * dudy2 - mu1 * dudy1) / dx1[i];
+    const S viscous_v23 = 1.0 * (mu6 * dwdy6 - mu5 * dwdy5) / dx3[k];
 
-    double rhs = invRhocCmu_half * (viscous_v1 + 2.0 * viscous_v2 + viscous_v3 + viscous_v21 + viscous_v23);
+    S rhs = invRhocCmu_half * (viscous_v1 + 2.0 * viscous_v2 + viscous_v3 + viscous_v21 + viscous_v23);
     rhs = rhs - p.Cmp * invRhoc * (P[g.ix(i, j, k)] - P[g.ix(i, jm, k)]) / dmx2[j];
 
-    const double vbt = VBCbt[g.b2(i, k)], vup = VBCup[g.b2(i, k)];
-    double vbc_down = (0.25 / dx2[jm] * v3 * vbt - 0.25 * 0.5 * dvdy3 * vbt);
-    double vbc_up = (-0.25 / dx2[j] * v4 * vup - 0.25 * 0.5 * dvdy4 * vup);
+    const S vbt = VBCbt[g.b2(i, k)], vup = VBCup[g.b2(i, k)];
+    S vbc_down = (0.25 / dx2[jm] * v3 * vbt - 0.25 * 0.5 * dvdy3 * vbt);
+    S vbc_up = (-0.25 / dx2[j] * v4 * vup - 0.25 * 0.5 * dvdy4 * vup);
     vbc_down = vbc_down + invRhocCmu_half / dmx2[j] * MU(jm, k) / dx2[jm] * vbt * 2.0;
     vbc_up = vbc_up + invRhocCmu_half / dmx2[j] * MU(j, k) / dx2[j] * vup * 2.0;
     rhs = rhs + (1.0 - (double)jvm) * vbc_down + (1.0 - (double)jvp) * vbc_up;
 
-    const double mACI = invRhocCmu_half / dx1[i] * (mu2 / dmx1[ip] + mu1 / dmx1[i]) + 0.25 * (-u2 / dmx1[ip] + u1 / dmx1[i]);
-    const double mAPI = -invRhocCmu_half / dx1[i] * mu2 / dmx1[ip] + 0.25 * (u2 / dmx1[ip]);
-    const double mAMI = -invRhocCmu_half / dx1[i] * mu1 / dmx1[i] + 0.25 * (-u1 / dmx1[i]);
-    const double mACJ = invRhocCmu_half / dmx2[j] * (MU(j, k) / dx2[j] + MU(jm, k) / dx2[jm]) * 2.0
+    const S mACI = invRhocCmu_half / dx1[i] * (mu2 / dmx1[ip] + mu1 / dmx1[i]) + 0.25 * (-u2 / dmx1[ip] + u1 / dmx1[i]);
+    const S mAPI = -invRhocCmu_half / dx1[i] * mu2 / dmx1[ip] + 0.25 * (u2 / dmx1[ip]);
+    const S mAMI = -invRhocCmu_half / dx1[i] * mu1 / dmx1[i] + 0.25 * (-u1 / dmx1[i]);
+    const S mACJ = invRhocCmu_half / dmx2[j] * (MU(j, k) / dx2[j] + MU(jm, k) / dx2[jm]) * 2.0
                       + 0.25 * (dvdy4 * 0.5 + dvdy3 * 0.5 - v4 / dx2[j] + v3 / dx2[jm]);
-    double mAPJ = -invRhocCmu_half / dmx2[j] * MU(j, k) / dx2[j] * 2.0 + 0.25 * (v4 / dx2[j] + dvdy4 * 0.5);
+    S mAPJ = -invRhocCmu_half / dmx2[j] * MU(j, k) / dx2[j] * 2.0 + 0.25 * (v4 / dx2[j] + dvdy4 * 0.5);
     mAPJ = mAPJ * (double)jvp;
-    double mAMJ = -invRhocCmu_half / dmx2[j] * MU(jm, k) / dx2[jm] * 2.0 + 0.25 * (-v3 / dx2[jm] + dvdy3 * 0.5);
+    S mAMJ = -invRhocCmu_half / dmx2[j] * MU(jm, k) / dx2[jm] * 2.0 + 0.25 * (-v3 / dx2[jm] + dvdy3 * 0.5);
     mAMJ = mAMJ * (double)jvm;
-    const double mACK = invRhocCmu_half / dx3[k] * (mu6 / dmx3[kp] + mu5 / dmx3[k]) + 0.25 * (-w6 / dmx3[kp] + w5 / dmx3[k]);
-    const double mAPK = -invRhocCmu_half / dx3[k] * mu6 / dmx3[kp] + 0.25 * (w6 / dmx3[kp]);
-    const double mAMK = -invRhocCmu_half / dx3[k] * mu5 / dmx3[k] + 0.25 * (-w5 / dmx3[k]);
+    const S mACK = invRhocCmu_half / dx3[k] * (mu6 / dmx3[kp] + mu5 / dmx3[k]) + 0.25 * (-w6 / dmx3[kp] + w5 / dmx3[k]);
+    const S mAPK = -invRhocCmu_half / dx3[k] * mu6 / dmx3[kp] + 0.25 * (w6 / dmx3[kp]);
+    const S mAMK = -invRhocCmu_half / dx3[k] * mu5 / dmx3[k] + 0.25 * (-w5 / dmx3[k]);
 
     rhs = rhs - (mAPI * Vf(ip, j, k) + mACI * Vf(i, j, k) + mAMI * Vf(im, j, k)
                  + mAPJ * Vf(i, jp, k) + mACJ * Vf(i, j, k) + mAMJ * Vf(i, jm, k)
@@ -367,25 +402,31 @@ struct DVZ {
     Phys p;
     double dt;
     const double *U, *V, *W, *P, *dU, *Mu, *iR, *VBCup, *VBCbt;
+    static constexpr int NF = 9;  // field inputs, in the order below
+    template <class S, class Wr>
+    __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
+        dV_z_row<S>(g, p, dt, w(0, U), w(1, V), w(2, W), w(3, P), w(4, dU), w(5, Mu), w(6, iR), w(7, VBCup), w(8, VBCbt), i, j, k, a, b, c, d);
+    }
     __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
                                                double &d) const {
-        dV_z_row(g, p, dt, U, V, W, P, dU, Mu, iR, VBCup, VBCbt, i, j, k, a, b, c, d);
+        eval<double>(RawW(), i, j, k, a, b, c, d);
     }
 };
 
-__device__ __forceinline__ void dV_x_row(const Grid &g, const Phys &p, double dt, const double *__restrict__ U,
-    const double *__restrict__ iR, const double *__restrict__ stale,
-    int i, int j, int k, double &am, double &ac, double &ap) {
+template <class S, class FP>
+__device__ __forceinline__ void dV_x_row(const Grid &g, const Phys &p, double dt, FP U,
+    FP iR, FP stale,
+    int i, int j, int k, S &am, S &ac, S &ap) {
     ROW_METRICS
     const int ip = i + 1, jm = j - 1;
-    const double mu1 = stale[0], mu2 = stale[1];
-    const double u1 = (dx2[jm] * Uf(i, j, k) + dx2[j] * Uf(i, jm, k)) / dmx2[j] * 0.5;
-    const double u2 = (dx2[jm] * Uf(ip, j, k) + dx2[j] * Uf(ip, jm, k)) / dmx2[j] * 0.5;
-    const double invRhoc = 0.5 * (dx2[jm] * IR(j, k) + dx2[j] * IR(jm, k)) / dmx2[j];
-    const double invRhocCmu_half = 0.5 * p.Cmu * invRhoc;
-    const double mACI = invRhocCmu_half / dx1[i] * (mu2 / dmx1[ip] + mu1 / dmx1[i]) + 0.25 * (-u2 / dmx1[ip] + u1 / dmx1[i]);
-    const double mAPI = -invRhocCmu_half / dx1[i] * mu2 / dmx1[ip] + 0.25 * (u2 / dmx1[ip]);
-    const double mAMI = -invRhocCmu_half / dx1[i] * mu1 / dmx1[i] + 0.25 * (-u1 / dmx1[i]);
+    const S mu1 = stale[0], mu2 = stale[1];
+    const S u1 = (dx2[jm] * Uf(i, j, k) + dx2[j] * Uf(i, jm, k)) / dmx2[j] * 0.5;
+    const S u2 = (dx2[jm] * Uf(ip, j, k) + dx2[j] * Uf(ip, jm, k)) / dmx2[j] * 0.5;
+    const S invRhoc = 0.5 * (dx2[jm] * IR(j, k) + dx2[j] * IR(jm, k)) / dmx2[j];
+    const S invRhocCmu_half = 0.5 * p.Cmu * invRhoc;
+    const S mACI = invRhocCmu_half / dx1[i] * (mu2 / dmx1[ip] + mu1 / dmx1[i]) + 0.25 * (-u2 / dmx1[ip] + u1 / dmx1[i]);
+    const S mAPI = -invRhocCmu_half / dx1[i] * mu2 / dmx1[ip] + 0.25 * (u2 / dmx1[ip]);
+    const S mAMI = -invRhocCmu_half / dx1[i] * mu1 / dmx1[i] + 0.25 * (-u1 / dmx1[i]);
     ac = mACI * dt + 1.0;
     ap = mAPI * dt;
     am = mAMI * dt;
@@ -395,29 +436,35 @@ struct DVX {
     Phys p;
     double dt;
     const double *U, *iR, *stale;
+    static constexpr int NF = 3;  // field inputs, in the order below
+    template <class S, class Wr>
+    __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
+        dV_x_row<S>(g, p, dt, w(0, U), w(1, iR), w(2, stale), i, j, k, a, b, c);
+    }
     __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
                                                double &d) const {
-        dV_x_row(g, p, dt, U, iR, stale, i, j, k, a, b, c);
+        eval<double>(RawW(), i, j, k, a, b, c, d);
     }
 };
 
-__device__ __forceinline__ void dV_y_row(const Grid &g, const Phys &p, double dt, const double *__restrict__ V,
-    const double *__restrict__ Mu, const double *__restrict__ iR,
-    int i, int j, int k, double &am, double &ac, double &ap) {
+template <class S, class FP>
+__device__ __forceinline__ void dV_y_row(const Grid &g, const Phys &p, double dt, FP V,
+    FP Mu, FP iR,
+    int i, int j, int k, S &am, S &ac, S &ap) {
     ROW_METRICS
     const int jm = j - 1, jp = j + 1;
     const int jvp = g.jS[jm], jvm = g.jS[jp];
-    const double v3 = 0.5 * (Vf(i, jm, k) + Vf(i, j, k));
-    const double v4 = 0.5 * (Vf(i, j, k) + Vf(i, jp, k));
-    const double dvdy3 = (Vf(i, j, k) - Vf(i, jm, k)) / dx2[jm];
-    const double dvdy4 = (Vf(i, jp, k) - Vf(i, j, k)) / dx2[j];
-    const double invRhoc = 0.5 * (dx2[jm] * IR(j, k) + dx2[j] * IR(jm, k)) / dmx2[j];
-    const double invRhocCmu_half = 0.5 * p.Cmu * invRhoc;
-    const double mACJ = invRhocCmu_half / dmx2[j] * (MU(j, k) / dx2[j] + MU(jm, k) / dx2[jm]) * 2.0
+    const S v3 = 0.5 * (Vf(i, jm, k) + Vf(i, j, k));
+    const S v4 = 0.5 * (Vf(i, j, k) + Vf(i, jp, k));
+    const S dvdy3 = (Vf(i, j, k) - Vf(i, jm, k)) / dx2[jm];
+    const S dvdy4 = (Vf(i, jp, k) - Vf(i, j, k)) / dx2[j];
+    const S invRhoc = 0.5 * (dx2[jm] * IR(j, k) + dx2[j] * IR(jm, k)) / dmx2[j];
+    const S invRhocCmu_half = 0.5 * p.Cmu * invRhoc;
+    const S mACJ = invRhocCmu_half / dmx2[j] * (MU(j, k) / dx2[j] + MU(jm, k) / dx2[jm]) * 2.0
                       + 0.25 * (dvdy4 * 0.5 + dvdy3 * 0.5 - v4 / dx2[j] + v3 / dx2[jm]);
-    double mAPJ = -invRhocCmu_half / dmx2[j] * MU(j, k) / dx2[j] * 2.0 + 0.25 * (v4 / dx2[j] + dvdy4 * 0.5);
+    S mAPJ = -invRhocCmu_half / dmx2[j] * MU(j, k) / dx2[j] * 2.0 + 0.25 * (v4 / dx2[j] + dvdy4 * 0.5);
     mAPJ = mAPJ * (double)jvp;
-    double mAMJ = -invRhocCmu_half / dmx2[j] * MU(jm, k) / dx2[jm] * 2.0 + 0.25 * (-v3 / dx2[jm] + dvdy3 * 0.5);
+    S mAMJ = -invRhocCmu_half / dmx2[j] * MU(jm, k) / dx2[jm] * 2.0 + 0.25 * (-v3 / dx2[jm] + dvdy3 * 0.5);
     mAMJ = mAMJ * (double)jvm;
     ac = mACJ * dt + 1.0;
     ap = mAPJ * dt;
@@ -428,76 +475,82 @@ struct DVY {
     Phys p;
     double dt;
     const double *V, *Mu, *iR;
+    static constexpr int NF = 3;  // field inputs, in the order below
+    template <class S, class Wr>
+    __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
+        dV_y_row<S>(g, p, dt, w(0, V), w(1, Mu), w(2, iR), i, j, k, a, b, c);
+    }
     __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
                                                double &d) const {
-        dV_y_row(g, p, dt, V, Mu, iR, i, j, k, a, b, c);
+        eval<double>(RawW(), i, j, k, a, b, c, d);
     }
 };
 
 // ---------------------------------------------------------------- dW -------
-__device__ __forceinline__ void dW_z_row(const Grid &g, const Phys &p, double dt, const double *__restrict__ U,
-    const double *__restrict__ V, const double *__restrict__ W, const double *__restrict__ P,
-    const double *__restrict__ dU, const double *__restrict__ dV, const double *__restrict__ Mu,
-    const double *__restrict__ iR, const double *__restrict__ VBCup, const double *__restrict__ VBCbt,
-    const double *__restrict__ WBCup, const double *__restrict__ WBCbt,
-    int i, int j, int k, double &am, double &ac, double &ap, double &rhsout) {
+template <class S, class FP>
+__device__ __forceinline__ void dW_z_row(const Grid &g, const Phys &p, double dt, FP U,
+    FP V, FP W, FP P,
+    FP dU, FP dV, FP Mu,
+    FP iR, FP VBCup, FP VBCbt,
+    FP WBCup, FP WBCbt,
+    int i, int j, int k, S &am, S &ac, S &ap, S &rhsout) {
     ROW_METRICS
     const int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1, km = k - 1, kp = k + 1;
     const int jwp = g.jC[jp], jwm = g.jC[jm];
-    const double Cmu = p.Cmu;
+    const S Cmu = p.Cmu;
 #define dVf(a, b, c) dV[g.ix(a, b, c)]
-    const double w5 = 0.5 * (Wf(i, j, k) + Wf(i, j, km));
-    const double w6 = 0.5 * (Wf(i, j, k) + Wf(i, j, kp));
-    const double u1 = (dx3[km] * Uf(i, j, k) + dx3[k] * Uf(i, j, km)) / dmx3[k] * 0.5;
-    const double u2 = (dx3[km] * Uf(ip, j, k) + dx3[k] * Uf(ip, j, km)) / dmx3[k] * 0.5;
-    const double v3 = (dx3[km] * Vf(i, j, k) + dx3[k] * Vf(i, j, km)) / dmx3[k] * 0.5;
-    const double v4 = (dx3[km] * Vf(i, jp, k) + dx3[k] * Vf(i, jp, km)) / dmx3[k] * 0.5;
+    const S w5 = 0.5 * (Wf(i, j, k) + Wf(i, j, km));
+    const S w6 = 0.5 * (Wf(i, j, k) + Wf(i, j, kp));
+    const S u1 = (dx3[km] * Uf(i, j, k) + dx3[k] * Uf(i, j, km)) / dmx3[k] * 0.5;
+    const S u2 = (dx3[km] * Uf(ip, j, k) + dx3[k] * Uf(ip, j, km)) / dmx3[k] * 0.5;
+    const S v3 = (dx3[km] * Vf(i, j, k) + dx3[k] * Vf(i, j, km)) / dmx3[k] * 0.5;
+    const S v4 = (dx3[km] * Vf(i, jp, k) + dx3[k] * Vf(i, jp, km)) / dmx3[k] * 0.5;
 
-    const double dwdx1 = (Wf(i, j, k) - Wf(im, j, k)) / dmx1[i];
-    const double dwdx2 = (Wf(ip, j, k) - Wf(i, j, k)) / dmx1[ip];
-    const double dwdy3 = (Wf(i, j, k) - Wf(i, jm, k)) / dmx2[j];
-    const double dwdy4 = (Wf(i, jp, k) - Wf(i, j, k)) / dmx2[jp];
-    const double dwdz5 = (Wf(i, j, k) - Wf(i, j, km)) / dx3[km];
-    const double dwdz6 = (Wf(i, j, kp) - Wf(i, j, k)) / dx3[k];
-    const double dudz1 = (Uf(i, j, k) - Uf(i, j, km)) / dmx3[k];
-    const double dudz2 = (Uf(ip, j, k) - Uf(ip, j, km)) / dmx3[k];
-    const double dvdz3 = (Vf(i, j, k) - Vf(i, j, km)) / dmx3[k];
-    const double dvdz4 = (Vf(i, jp, k) - Vf(i, jp, km)) / dmx3[k];
-    const double ddudz1 = (dUf(i, j, k) - dUf(i, j, km)) / dmx3[k];
-    const double ddudz2 = (dUf(ip, j, k) - dUf(ip, j, km)) / dmx3[k];
-    const double ddvdz3 = (dVf(i, j, k) - dVf(i, j, km)) / dmx3[k];
-    const double ddvdz4 = (dVf(i, jp, k) - dVf(i, jp, km)) / dmx3[k];
-    const double ddu1 = (dx3[km] * dUf(i, j, k) + dx3[k] * dUf(i, j, km)) / dmx3[k] * 0.5;
-    const double ddu2 = (dx3[km] * dUf(ip, j, k) + dx3[k] * dUf(ip, j, km)) / dmx3[k] * 0.5;
-    const double ddv3 = (dx3[km] * dVf(i, j, k) + dx3[k] * dVf(i, j, km)) / dmx3[k] * 0.5 * (double)jwm;
-    const double ddv4 = (dx3[km] * dVf(i, jp, k) + dx3[k] * dVf(i, jp, km)) / dmx3[k] * 0.5 * (double)jwp;
+    const S dwdx1 = (Wf(i, j, k) - Wf(im, j, k)) / dmx1[i];
+    const S dwdx2 = (Wf(ip, j, k) - Wf(i, j, k)) / dmx1[ip];
+    const S dwdy3 = (Wf(i, j, k) - Wf(i, jm, k)) / dmx2[j];
+    const S dwdy4 = (Wf(i, jp, k) - Wf(i, j, k)) / dmx2[jp];
+    const S dwdz5 = (Wf(i, j, k) - Wf(i, j, km)) / dx3[km];
+    const S dwdz6 = (Wf(i, j, kp) - Wf(i, j, k)) / dx3[k];
+    const S dudz1 = (Uf(i, j, k) - Uf(i, j, km)) / dmx3[k];
+    const S dudz2 = (Uf(ip, j, k) - Uf(ip, j, km)) / dmx3[k];
+    const S dvdz3 = (Vf(i, j, k) - Vf(i, j, km)) / dmx3[k];
+    const S dvdz4 = (Vf(i, jp, k) - Vf(i, jp, km)) / dmx3[k];
+    const S ddudz1 = (dUf(i, j, k) - dUf(i, j, km)) / dmx3[k];
+    const S ddudz2 = (dUf(ip, j, k) - dUf(ip, j, km)) / dmx3[k];
+    const S ddvdz3 = (dVf(i, j, k) - dVf(i, j, km)) / dmx3[k];
+    const S ddvdz4 = (dVf(i, jp, k) - dVf(i, jp, km)) / dmx3[k];
+    const S ddu1 = (dx3[km] * dUf(i, j, k) + dx3[k] * dUf(i, j, km)) / dmx3[k] * 0.5;
+    const S ddu2 = (dx3[km] * dUf(ip, j, k) + dx3[k] * dUf(ip, j, km)) / dmx3[k] * 0.5;
+    const S ddv3 = (dx3[km] * dVf(i, j, k) + dx3[k] * dVf(i, j, km)) / dmx3[k] * 0.5 * (double)jwm;
+    const S ddv4 = (dx3[km] * dVf(i, jp, k) + dx3[k] * dVf(i, jp, km)) / dmx3[k] * 0.5 * (double)jwp;
 
-    double mua = 0.5 * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
-    double muc = 0.5 * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
-    double mub = 0.5 * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
-    const double mu1 = 0.5 * (dx1[im] * muc + dx1[i] * mua) / dmx1[i];
-    const double mu2 = 0.5 * (dx1[ip] * muc + dx1[i] * mub) / dmx1[ip];
+    S mua = 0.5 * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
+    S muc = 0.5 * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
+    S mub = 0.5 * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
+    const S mu1 = 0.5 * (dx1[im] * muc + dx1[i] * mua) / dmx1[i];
+    const S mu2 = 0.5 * (dx1[ip] * muc + dx1[i] * mub) / dmx1[ip];
     mua = 0.5 * (dx3[km] * MU(jm, k) + dx3[k] * MU(jm, km)) / dmx3[k];
     muc = 0.5 * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
     mub = 0.5 * (dx3[km] * MU(jp, k) + dx3[k] * MU(jp, km)) / dmx3[k];
-    const double mu3 = 0.5 * (dx2[jm] * muc + dx2[j] * mua) / dmx2[j];
-    const double mu4 = 0.5 * (dx2[jp] * muc + dx2[j] * mub) / dmx2[jp];
+    const S mu3 = 0.5 * (dx2[jm] * muc + dx2[j] * mua) / dmx2[j];
+    const S mu4 = 0.5 * (dx2[jp] * muc + dx2[j] * mub) / dmx2[jp];
 
-    const double invRhoc = 0.5 * (dx3[km] * IR(j, k) + dx3[k] * IR(j, km)) / dmx3[k];
-    const double invRhocCmu_half = 0.5 * Cmu * invRhoc;
+    const S invRhoc = 0.5 * (dx3[km] * IR(j, k) + dx3[k] * IR(j, km)) / dmx3[k];
+    const S invRhocCmu_half = 0.5 * Cmu * invRhoc;
 
-    const double viscous_w1 = 1.0 * (mu2 * dwdx2 - mu1 * dwdx1) / dx1[i];
-    const double viscous_w2 = 1.0 * (mu4 * dwdy4 - mu3 * dwdy3) / dx2[j];
-    const double viscous_w3 = 1.0 * (MU(j, k) * dwdz6 - MU(j, km) * dwdz5) / dmx3[k];
-    const double viscous_w31 = 1.0 * (mu2 * dudz2 - mu1 * dudz1) / dx1[i];
-    const double viscous_w32 = 1.0 * (mu4 * dvdz4 - mu3 * dvdz3) / dx2[j];
+    const S viscous_w1 = 1.0 * (mu2 * dwdx2 - mu1 * dwdx1) / dx1[i];
+    const S viscous_w2 = 1.0 * (mu4 * dwdy4 - mu3 * dwdy3) / dx2[j];
+    const S viscous_w3 = 1.0 * (MU(j, k) * dwdz6 - MU(j, km) * dwdz5) / dmx3[k];
+    const S viscous_w31 = 1.0 * (mu2 * dudz2 - mu1 * dudz1) / dx1[i];
+    const S viscous_w32 = 1.0 * (mu4 * dvdz4 - mu3 * dvdz3) / dx2[j];
 
-    double rhs = invRhocCmu_half * (viscous_w1 + viscous_w2 + 2.0 * viscous_w3 + viscous_w31 + viscous_w32);
+    S rhs = invRhocCmu_half * (viscous_w1 + viscous_w2 + 2.0 * viscous_w3 + viscous_w31 + viscous_w32);
     rhs = rhs - p.Cmp * invRhoc * (P[g.ix(i, j, k)] - P[g.ix(i, j, km)]) / dmx3[k];
 
-    double zbc_down = 0.25 * v3 / dmx2[j] * WBCbt[g.b2(i, k)]
+    S zbc_down = 0.25 * v3 / dmx2[j] * WBCbt[g.b2(i, k)]
                     - 0.25 * dwdy3 * (dx3[km] * VBCbt[g.b2(i, k)] + dx3[k] * VBCbt[g.b2(i, km)]) / dmx3[k] * 0.5;
-    double zbc_up = -0.25 * v4 / dmx2[jp] * WBCup[g.b2(i, k)]
+    S zbc_up = -0.25 * v4 / dmx2[jp] * WBCup[g.b2(i, k)]
                   - 0.25 * dwdy4 * (dx3[km] * VBCup[g.b2(i, k)] + dx3[k] * VBCup[g.b2(i, km)]) / dmx3[k] * 0.5;
     zbc_down = zbc_down + invRhocCmu_half / dx2[j] * mu3 / dmx2[jp] * WBCbt[g.b2(i, k)];
     zbc_up = zbc_up + invRhocCmu_half / dx2[j] * mu4 / dmx2[j] * WBCup[g.b2(i, k)];
@@ -505,18 +558,18 @@ __device__ __forceinline__ void dW_z_row(const Grid &g, const Phys &p, double dt
     zbc_up = zbc_up + invRhocCmu_half / dx2[j] * mu4 / dmx3[k] * (VBCup[g.b2(i, k)] - VBCup[g.b2(i, km)]);
     rhs = rhs + (1.0 - (double)jwm) * zbc_down + (1.0 - (double)jwp) * zbc_up;
 
-    const double mACI = invRhocCmu_half / dx1[i] * (mu2 / dmx1[ip] + mu1 / dmx1[i]) + 0.25 * (-u2 / dmx1[ip] + u1 / dmx1[i]);
-    const double mAPI = -invRhocCmu_half / dx1[i] * mu2 / dmx1[ip] + 0.25 * (u2 / dmx1[ip]);
-    const double mAMI = -invRhocCmu_half / dx1[i] * mu1 / dmx1[i] + 0.25 * (-u1 / dmx1[i]);
-    const double mACJ = invRhocCmu_half / dx2[j] * (mu4 / dmx2[jp] + mu3 / dmx2[j]) + 0.25 * (-v4 / dmx2[jp] + v3 / dmx2[j]);
-    double mAPJ = -invRhocCmu_half / dx2[j] * mu4 / dmx2[jp] + 0.25 * (v4 / dmx2[jp]);
+    const S mACI = invRhocCmu_half / dx1[i] * (mu2 / dmx1[ip] + mu1 / dmx1[i]) + 0.25 * (-u2 / dmx1[ip] + u1 / dmx1[i]);
+    const S mAPI = -invRhocCmu_half / dx1[i] * mu2 / dmx1[ip] + 0.25 * (u2 / dmx1[ip]);
+    const S mAMI = -invRhocCmu_half / dx1[i] * mu1 / dmx1[i] + 0.25 * (-u1 / dmx1[i]);
+    const S mACJ = invRhocCmu_half / dx2[j] * (mu4 / dmx2[jp] + mu3 / dmx2[j]) + 0.25 * (-v4 / dmx2[jp] + v3 / dmx2[j]);
+    S mAPJ = -invRhocCmu_half / dx2[j] * mu4 / dmx2[jp] + 0.25 * (v4 / dmx2[jp]);
     mAPJ = mAPJ * (double)jwp;
-    double mAMJ = -invRhocCmu_half / dx2[j] * mu3 / dmx2[j] + 0.25 * (-v3 / dmx2[j]);
+    S mAMJ = -invRhocCmu_half / dx2[j] * mu3 / dmx2[j] + 0.25 * (-v3 / dmx2[j]);
     mAMJ = mAMJ * (double)jwm;
-    const double mACK = invRhocCmu_half / dmx3[k] * (MU(j, k) / dx3[k] + MU(j, km) / dx3[km]) * 2.0
+    const S mACK = invRhocCmu_half / dmx3[k] * (MU(j, k) / dx3[k] + MU(j, km) / dx3[km]) * 2.0
                       + 0.25 * (-w6 / dx3[k] + w5 / dx3[km] + 0.5 * dwdz6 + 0.5 * dwdz5);
-    const double mAPK = -invRhocCmu_half / dmx3[k] * MU(j, k) / dx3[k] * 2.0 + 0.25 * (w6 / dx3[k] + 0.5 * dwdz6);
-    const double mAMK = -invRhocCmu_half / dmx3[k] * MU(j, km) / dx3[km] * 2.0 + 0.25 * (-w5 / dx3[km] + 0.5 * dwdz5);
+    const S mAPK = -invRhocCmu_half / dmx3[k] * MU(j, k) / dx3[k] * 2.0 + 0.25 * (w6 / dx3[k] + 0.5 * dwdz6);
+    const S mAMK = -invRhocCmu_half / dmx3[k] * MU(j, km) / dx3[km] * 2.0 + 0.25 * (-w5 / dx3[km] + 0.5 * dwdz5);
 
     rhs = rhs - (mAPI * Wf(ip, j, k) + mACI * Wf(i, j, k) + mAMI * Wf(im, j, k)
                  + mAPJ * Wf(i, jp, k) + mACJ * Wf(i, j, k) + mAMJ * Wf(i, jm, k)
@@ -538,25 +591,31 @@ struct DWZ {
     Phys p;
     double dt;
     const double *U, *V, *W, *P, *dU, *dV, *Mu, *iR, *VBCup, *VBCbt, *WBCup, *WBCbt;
+    static constexpr int NF = 12;  // field inputs, in the order below
+    template <class S, class Wr>
+    __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
+        dW_z_row<S>(g, p, dt, w(0, U), w(1, V), w(2, W), w(3, P), w(4, dU), w(5, dV), w(6, Mu), w(7, iR), w(8, VBCup), w(9, VBCbt), w(10, WBCup), w(11, WBCbt), i, j, k, a, b, c, d);
+    }
     __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
                                                double &d) const {
-        dW_z_row(g, p, dt, U, V, W, P, dU, dV, Mu, iR, VBCup, VBCbt, WBCup, WBCbt, i, j, k, a, b, c, d);
+        eval<double>(RawW(), i, j, k, a, b, c, d);
     }
 };
 
-__device__ __forceinline__ void dW_x_row(const Grid &g, const Phys &p, double dt, const double *__restrict__ U,
-    const double *__restrict__ iR, const double *__restrict__ stale,
-    int i, int j, int k, double &am, double &ac, double &ap) {
+template <class S, class FP>
+__device__ __forceinline__ void dW_x_row(const Grid &g, const Phys &p, double dt, FP U,
+    FP iR, FP stale,
+    int i, int j, int k, S &am, S &ac, S &ap) {
     ROW_METRICS
     const int ip = i + 1, km = k - 1;
-    const double mu1 = stale[2], mu2 = stale[3];
-    const double u1 = (dx3[km] * Uf(i, j, k) + dx3[k] * Uf(i, j, km)) / dmx3[k] * 0.5;
-    const double u2 = (dx3[km] * Uf(ip, j, k) + dx3[k] * Uf(ip, j, km)) / dmx3[k] * 0.5;
-    const double invRhoc = 0.5 * (dx3[km] * IR(j, k) + dx3[k] * IR(j, km)) / dmx3[k];
-    const double invRhocCmu_half = 0.5 * p.Cmu * invRhoc;
-    const double mACI = invRhocCmu_half / dx1[i] * (mu2 / dmx1[ip] + mu1 / dmx1[i]) + 0.25 * (-u2 / dmx1[ip] + u1 / dmx1[i]);
-    const double mAPI = -invRhocCmu_half / dx1[i] * mu2 / dmx1[ip] + 0.25 * (u2 / dmx1[ip]);
-    const double mAMI = -invRhocCmu_half / dx1[i] * mu1 / dmx1[i] + 0.25 * (-u1 / dmx1[i]);
+    const S mu1 = stale[2], mu2 = stale[3];
+    const S u1 = (dx3[km] * Uf(i, j, k) + dx3[k] * Uf(i, j, km)) / dmx3[k] * 0.5;
+    const S u2 = (dx3[km] * Uf(ip, j, k) + dx3[k] * Uf(ip, j, km)) / dmx3[k] * 0.5;
+    const S invRhoc = 0.5 * (dx3[km] * IR(j, k) + dx3[k] * IR(j, km)) / dmx3[k];
+    const S invRhocCmu_half = 0.5 * p.Cmu * invRhoc;
+    const S mACI = invRhocCmu_half / dx1[i] * (mu2 / dmx1[ip] + mu1 / dmx1[i]) + 0.25 * (-u2 / dmx1[ip] + u1 / dmx1[i]);
+    const S mAPI = -invRhocCmu_half / dx1[i] * mu2 / dmx1[ip] + 0.25 * (u2 / dmx1[ip]);
+    const S mAMI = -invRhocCmu_half / dx1[i] * mu1 / dmx1[i] + 0.25 * (-u1 / dmx1[i]);
     ac = mACI * dt + 1.0;
     ap = mAPI * dt;
     am = mAMI * dt;
@@ -566,28 +625,34 @@ struct DWX {
     Phys p;
     double dt;
     const double *U, *iR, *stale;
+    static constexpr int NF = 3;  // field inputs, in the order below
+    template <class S, class Wr>
+    __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
+        dW_x_row<S>(g, p, dt, w(0, U), w(1, iR), w(2, stale), i, j, k, a, b, c);
+    }
     __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
                                                double &d) const {
-        dW_x_row(g, p, dt, U, iR, stale, i, j, k, a, b, c);
+        eval<double>(RawW(), i, j, k, a, b, c, d);
     }
 };
 
-__device__ __forceinline__ void dW_y_row(const Grid &g, const Phys &p, double dt, const double *__restrict__ V,
-    const double *__restrict__ iR, const double *__restrict__ stale,
-    int i, int j, int k, double &am, double &ac, double &ap) {
+template <class S, class FP>
+__device__ __forceinline__ void dW_y_row(const Grid &g, const Phys &p, double dt, FP V,
+    FP iR, FP stale,
+    int i, int j, int k, S &am, S &ac, S &ap) {
     ROW_METRICS
     const int jp = j + 1, jm = j - 1;
     const int km = g.n3m - 1;  // stale from the x-stage's last k
     const int jwp = g.jC[jp], jwm = g.jC[jm];
-    const double mu3 = stale[4], mu4 = stale[5];
-    const double v3 = (dx3[km] * Vf(i, j, k) + dx3[k] * Vf(i, j, km)) / dmx3[k] * 0.5;
-    const double v4 = (dx3[km] * Vf(i, jp, k) + dx3[k] * Vf(i, jp, km)) / dmx3[k] * 0.5;
-    const double invRhoc = 0.5 * (dx3[km] * IR(j, k) + dx3[k] * IR(j, km)) / dmx3[k];
-    const double invRhocCmu_half = 0.5 * p.Cmu * invRhoc;
-    const double mACJ = invRhocCmu_half / dx2[j] * (mu4 / dmx2[jp] + mu3 / dmx2[j]) + 0.25 * (-v4 / dmx2[jp] + v3 / dmx2[j]);
-    double mAPJ = -invRhocCmu_half / dx2[j] * mu4 / dmx2[jp] + 0.25 * (v4 / dmx2[jp]);
+    const S mu3 = stale[4], mu4 = stale[5];
+    const S v3 = (dx3[km] * Vf(i, j, k) + dx3[k] * Vf(i, j, km)) / dmx3[k] * 0.5;
+    const S v4 = (dx3[km] * Vf(i, jp, k) + dx3[k] * Vf(i, jp, km)) / dmx3[k] * 0.5;
+    const S invRhoc = 0.5 * (dx3[km] * IR(j, k) + dx3[k] * IR(j, km)) / dmx3[k];
+    const S invRhocCmu_half = 0.5 * p.Cmu * invRhoc;
+    const S mACJ = invRhocCmu_half / dx2[j] * (mu4 / dmx2[jp] + mu3 / dmx2[j]) + 0.25 * (-v4 / dmx2[jp] + v3 / dmx2[j]);
+    S mAPJ = -invRhocCmu_half / dx2[j] * mu4 / dmx2[jp] + 0.25 * (v4 / dmx2[jp]);
     mAPJ = mAPJ * (double)jwp;
-    double mAMJ = -invRhocCmu_half / dx2[j] * mu3 / dmx2[j] + 0.25 * (-v3 / dmx2[j]);
+    S mAMJ = -invRhocCmu_half / dx2[j] * mu3 / dmx2[j] + 0.25 * (-v3 / dmx2[j]);
     mAMJ = mAMJ * (double)jwm;
     ac = mACJ * dt + 1.0;
     ap = mAPJ * dt;
@@ -598,71 +663,118 @@ struct DWY {
     Phys p;
     double dt;
     const double *V, *iR, *stale;
+    static constexpr int NF = 3;  // field inputs, in the order below
+    template <class S, class Wr>
+    __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
+        dW_y_row<S>(g, p, dt, w(0, V), w(1, iR), w(2, stale), i, j, k, a, b, c);
+    }
     __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
                                                double &d) const {
-        dW_y_row(g, p, dt, V, iR, stale, i, j, k, a, b, c);
+        eval<double>(RawW(), i, j, k, a, b, c, d);
     }
 };
 
 // ------------------------------------------------------- block corrections --
-__global__ void __launch_bounds__(256)
-k_blockLdV(Grid g, Phys p, double dt, const double *__restrict__ V, const double *__restrict__ dW,
-           const double *__restrict__ Mu, const double *__restrict__ iR, double *__restrict__ dV) {
-    KERNEL_IJK
-    if (j < 2) return;
+// mpi_momentum_blockLdV (:1278-1356): the new dV of cell (i,j,k), j >= 2
+template <class S, class FP>
+__device__ __forceinline__ void blockLdV_cell(const Grid &g, const Phys &p, double dt, FP V, FP dW, FP Mu,
+                                             FP iR, FP dV, int i, int j, int k, S &out) {
+    ROW_METRICS
     const int jm = j - 1, km = k - 1, kp = k + 1;
 #define dWf(a, b, c) dW[g.ix(a, b, c)]
-    const double dwm5 = (dx2[jm] * dWf(i, j, k) + dx2[j] * dWf(i, jm, k)) / dmx2[j] * 0.5;
-    const double dwm6 = (dx2[jm] * dWf(i, j, kp) + dx2[j] * dWf(i, jm, kp)) / dmx2[j] * 0.5;
-    const double ddwdy5 = (dWf(i, j, k) - dWf(i, jm, k)) / dmx2[j];
-    const double ddwdy6 = (dWf(i, j, kp) - dWf(i, jm, kp)) / dmx2[j];
-    const double dvdz5 = (Vf(i, j, k) - Vf(i, j, km)) / dmx3[k];
-    const double dvdz6 = (Vf(i, j, kp) - Vf(i, j, k)) / dmx3[kp];
-    const double invRhoc = 0.5 * (dx2[jm] * IR(j, k) + dx2[j] * IR(jm, k)) / dmx2[j];
-    const double mua = 0.5 * (dx2[jm] * MU(j, km) + dx2[j] * MU(jm, km)) / dmx2[j];
-    const double muc = 0.5 * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
-    const double mub = 0.5 * (dx2[jm] * MU(j, kp) + dx2[j] * MU(jm, kp)) / dmx2[j];
-    const double mu5 = 0.5 * (dx3[km] * muc + dx3[k] * mua) / dmx3[k];
-    const double mu6 = 0.5 * (dx3[kp] * muc + dx3[k] * mub) / dmx3[kp];
-    const size_t q = g.ix(i, j, k);
-    dV[q] = dV[q] - dt * (0.25 * (dwm5 * dvdz5 + dwm6 * dvdz6)
+    const S dwm5 = (dx2[jm] * dWf(i, j, k) + dx2[j] * dWf(i, jm, k)) / dmx2[j] * 0.5;
+    const S dwm6 = (dx2[jm] * dWf(i, j, kp) + dx2[j] * dWf(i, jm, kp)) / dmx2[j] * 0.5;
+    const S ddwdy5 = (dWf(i, j, k) - dWf(i, jm, k)) / dmx2[j];
+    const S ddwdy6 = (dWf(i, j, kp) - dWf(i, jm, kp)) / dmx2[j];
+    const S dvdz5 = (Vf(i, j, k) - Vf(i, j, km)) / dmx3[k];
+    const S dvdz6 = (Vf(i, j, kp) - Vf(i, j, k)) / dmx3[kp];
+    const S invRhoc = 0.5 * (dx2[jm] * IR(j, k) + dx2[j] * IR(jm, k)) / dmx2[j];
+    const S mua = 0.5 * (dx2[jm] * MU(j, km) + dx2[j] * MU(jm, km)) / dmx2[j];
+    const S muc = 0.5 * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
+    const S mub = 0.5 * (dx2[jm] * MU(j, kp) + dx2[j] * MU(jm, kp)) / dmx2[j];
+    const S mu5 = 0.5 * (dx3[km] * muc + dx3[k] * mua) / dmx3[k];
+    const S mu6 = 0.5 * (dx3[kp] * muc + dx3[k] * mub) / dmx3[kp];
+    out = dV[g.ix(i, j, k)] - dt * (0.25 * (dwm5 * dvdz5 + dwm6 * dvdz6)
                           - 0.5 * p.Cmu * invRhoc / dx3[k] * (mu6 * ddwdy6 - mu5 * ddwdy5));
 }
+struct BlockLdV {
+    Grid g;
+    Phys p;
+    double dt;
+    const double *V, *dW, *Mu, *iR, *dV;
+    static constexpr int NF = 5;
+    template <class S, class Wr>
+    __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
+        blockLdV_cell<S>(g, p, dt, w(0, V), w(1, dW), w(2, Mu), w(3, iR), w(4, dV), i, j, k, a);
+    }
+};
 
-__global__ void __launch_bounds__(256)
-k_blockLdU(Grid g, Phys p, double dt, const double *__restrict__ U, const double *__restrict__ dV,
-           const double *__restrict__ dW, const double *__restrict__ Mu, const double *__restrict__ iR,
-           double *__restrict__ dU) {
-    KERNEL_IJK
+__global__ void __launch_bounds__(256) k_blockLdV(BlockLdV cell, double *dV) {
+    const Grid &g = cell.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.n1m || j > g.n2m || j < 2) return;
+    double o, x;
+    cell.eval<double>(RawW(), i, j, k, o, x, x, x);
+    dV[g.ix(i, j, k)] = o;
+}
+
+// mpi_momentum_blockLdU (:1362-1460): the new dU of cell (i,j,k)
+template <class S, class FP>
+__device__ __forceinline__ void blockLdU_cell(const Grid &g, const Phys &p, double dt, FP U, FP dV, FP dW,
+                                             FP Mu, FP iR, FP dU, int i, int j, int k, S &out) {
+    ROW_METRICS
     const int im = i - 1, jm = j - 1, jp = j + 1, km = k - 1, kp = k + 1;
     const int jum = g.jC[jm], jup = g.jC[jp];
-    const double invRhoc = 0.5 * (dx1[im] * IR(j, k) + dx1[i] * IR(j, k)) / dmx1[i];
-    const double dvm3 = (dx1[i] * dVf(im, j, k) + dx1[im] * dVf(i, j, k)) / dmx1[i] * 0.5 * (double)jum;
-    const double dvm4 = (dx1[i] * dVf(im, jp, k) + dx1[im] * dVf(i, jp, k)) / dmx1[i] * 0.5 * (double)jup;
-    const double dudy3 = (Uf(i, j, k) - Uf(i, jm, k)) / dmx2[j];
-    const double dudy4 = (Uf(i, jp, k) - Uf(i, j, k)) / dmx2[jp];
-    const double dvmdx3 = (dVf(i, j, k) - dVf(im, j, k)) / dmx1[i] * (double)jum;
-    const double dvmdx4 = (dVf(i, jp, k) - dVf(im, jp, k)) / dmx1[i] * (double)jup;
-    double mua = 0.5 * (dx1[im] * MU(jm, k) + dx1[i] * MU(jm, k)) / dmx1[i];
-    double muc = 0.5 * (dx1[im] * MU(j, k) + dx1[i] * MU(j, k)) / dmx1[i];
-    double mub = 0.5 * (dx1[im] * MU(jp, k) + dx1[i] * MU(jp, k)) / dmx1[i];
-    const double mu3 = 0.5 * (dx2[jm] * muc + dx2[j] * mua) / dmx2[j];
-    const double mu4 = 0.5 * (dx2[jp] * muc + dx2[j] * mub) / dmx2[jp];
-    const double dwm5 = (dx1[i] * dWf(im, j, k) + dx1[im] * dWf(i, j, k)) / dmx1[i] * 0.5;
-    const double dwm6 = (dx1[i] * dWf(im, j, kp) + dx1[im] * dWf(i, j, kp)) / dmx1[i] * 0.5;
-    const double dudz5 = (Uf(i, j, k) - Uf(i, j, km)) / dmx3[k];
-    const double dudz6 = (Uf(i, j, kp) - Uf(i, j, k)) / dmx3[kp];
-    const double dwmdx5 = (dWf(i, j, k) - dWf(im, j, k)) / dmx1[i];
-    const double dwmdx6 = (dWf(i, j, kp) - dWf(im, j, kp)) / dmx1[i];
+    const S invRhoc = 0.5 * (dx1[im] * IR(j, k) + dx1[i] * IR(j, k)) / dmx1[i];
+    const S dvm3 = (dx1[i] * dVf(im, j, k) + dx1[im] * dVf(i, j, k)) / dmx1[i] * 0.5 * (double)jum;
+    const S dvm4 = (dx1[i] * dVf(im, jp, k) + dx1[im] * dVf(i, jp, k)) / dmx1[i] * 0.5 * (double)jup;
+    const S dudy3 = (Uf(i, j, k) - Uf(i, jm, k)) / dmx2[j];
+    const S dudy4 = (Uf(i, jp, k) - Uf(i, j, k)) / dmx2[jp];
+    const S dvmdx3 = (dVf(i, j, k) - dVf(im, j, k)) / dmx1[i] * (double)jum;
+    const S dvmdx4 = (dVf(i, jp, k) - dVf(im, jp, k)) / dmx1[i] * (double)jup;
+    S mua = 0.5 * (dx1[im] * MU(jm, k) + dx1[i] * MU(jm, k)) / dmx1[i];
+    S muc = 0.5 * (dx1[im] * MU(j, k) + dx1[i] * MU(j, k)) / dmx1[i];
+    S mub = 0.5 * (dx1[im] * MU(jp, k) + dx1[i] * MU(jp, k)) / dmx1[i];
+    const S mu3 = 0.5 * (dx2[jm] * muc + dx2[j] * mua) / dmx2[j];
+    const S mu4 = 0.5 * (dx2[jp] * muc + dx2[j] * mub) / dmx2[jp];
+    const S dwm5 = (dx1[i] * dWf(im, j, k) + dx1[im] * dWf(i, j, k)) / dmx1[i] * 0.5;
+    const S dwm6 = (dx1[i] * dWf(im, j, kp) + dx1[im] * dWf(i, j, kp)) / dmx1[i] * 0.5;
+    const S dudz5 = (Uf(i, j, k) - Uf(i, j, km)) / dmx3[k];
+    const S dudz6 = (Uf(i, j, kp) - Uf(i, j, k)) / dmx3[kp];
+    const S dwmdx5 = (dWf(i, j, k) - dWf(im, j, k)) / dmx1[i];
+    const S dwmdx6 = (dWf(i, j, kp) - dWf(im, j, kp)) / dmx1[i];
     mua = 0.5 * (dx1[im] * MU(j, km) + dx1[i] * MU(j, km)) / dmx1[i];
     muc = 0.5 * (dx1[im] * MU(j, k) + dx1[i] * MU(j, k)) / dmx1[i];
     mub = 0.5 * (dx1[im] * MU(j, kp) + dx1[i] * MU(j, kp)) / dmx1[i];
-    const double mu5 = 0.5 * (dx3[km] * muc + dx3[k] * mua) / dmx3[k];
-    const double mu6 = 0.5 * (dx3[kp] * muc + dx3[k] * mub) / dmx3[kp];
-    const size_t q = g.ix(i, j, k);
-    dU[q] = dU[q]
+    const S mu5 = 0.5 * (dx3[km] * muc + dx3[k] * mua) / dmx3[k];
+    const S mu6 = 0.5 * (dx3[kp] * muc + dx3[k] * mub) / dmx3[kp];
+    out = dU[g.ix(i, j, k)]
           - dt * (0.25 * (dvm4 * dudy4 + dvm3 * dudy3) - 0.5 * p.Cmu * invRhoc / dx2[j] * (mu4 * dvmdx4 - mu3 * dvmdx3))
           - dt * (0.25 * (dwm6 * dudz6 + dwm5 * dudz5) - 0.5 * p.Cmu * invRhoc / dx3[k] * (mu6 * dwmdx6 - mu5 * dwmdx5));
+}
+struct BlockLdU {
+    Grid g;
+    Phys p;
+    double dt;
+    const double *U, *dV, *dW, *Mu, *iR, *dU;
+    static constexpr int NF = 6;
+    template <class S, class Wr>
+    __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
+        blockLdU_cell<S>(g, p, dt, w(0, U), w(1, dV), w(2, dW), w(3, Mu), w(4, iR), w(5, dU), i, j, k, a);
+    }
+};
+
+__global__ void __launch_bounds__(256) k_blockLdU(BlockLdU cell, double *dU) {
+    const Grid &g = cell.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.n1m || j > g.n2m) return;
+    double o, x;
+    cell.eval<double>(RawW(), i, j, k, o, x, x, x);
+    dU[g.ix(i, j, k)] = o;
 }
 
 __global__ void __launch_bounds__(256)
@@ -743,15 +855,16 @@ int momentum_solvedW(tcs_ctx *c, double dt) {
 
 int momentum_blockLdV(tcs_ctx *c, double dt) {
     const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
-    TCS_LAUNCH(c->stream, "blockLdV", k_blockLdV, grd, blk, 0, c->grid(), c->phys, dt, c->f[TCS_V], c->f[TCS_DW],
-                       c->f[TCS_MU], c->f[TCS_INVRHO], c->f[TCS_DV]);
+    const BlockLdV cell{c->grid(), c->phys, dt, c->f[TCS_V], c->f[TCS_DW], c->f[TCS_MU], c->f[TCS_INVRHO], c->f[TCS_DV]};
+    TCS_LAUNCH(c->stream, "blockLdV", k_blockLdV, grd, blk, 0, cell, c->f[TCS_DV]);
     return TCS_OK;
 }
 
 int momentum_blockLdU(tcs_ctx *c, double dt) {
     const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
-    TCS_LAUNCH(c->stream, "blockLdU", k_blockLdU, grd, blk, 0, c->grid(), c->phys, dt, c->f[TCS_U], c->f[TCS_DV],
-                       c->f[TCS_DW], c->f[TCS_MU], c->f[TCS_INVRHO], c->f[TCS_DU]);
+    const BlockLdU cell{c->grid(), c->phys, dt,          c->f[TCS_U],      c->f[TCS_DV], c->f[TCS_DW],
+                        c->f[TCS_MU], c->f[TCS_INVRHO], c->f[TCS_DU]};
+    TCS_LAUNCH(c->stream, "blockLdU", k_blockLdU, grd, blk, 0, cell, c->f[TCS_DU]);
     return TCS_OK;
 }
 

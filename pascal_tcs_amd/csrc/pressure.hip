@@ -18,6 +18,7 @@
 //     the per-line partial sums for the mean.
 // FFTs are radix-2 Stockham with a host twiddle table, computed in place in
 // LDS with the butterfly outputs staged in registers across barriers.
+#include "ad.h"
 #include "tdma_part.h"
 
 #ifndef TCS_FORM
@@ -123,40 +124,59 @@ __device__ __forceinline__ void lds_fft(double *re, double *im, int n, int nl, c
 }
 
 // PRHS (:182-229) into the interior layout
-__global__ void __launch_bounds__(256)
-k_pressure_rhs(Grid g, Phys p, double dt, const double *__restrict__ U, const double *__restrict__ V,
-               const double *__restrict__ W, const double *__restrict__ dPhat, const double *__restrict__ iR,
-               const double *__restrict__ VBCup, const double *__restrict__ VBCbt, double *__restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
-    if (i > g.n1m || j > g.n2m) return;
+template <class S, class FP>
+__device__ __forceinline__ void pressure_rhs_cell(const Grid &g, const Phys &p, double dt, FP U, FP V, FP W,
+                                                 FP dPhat, FP iR, FP VBCup, FP VBCbt, int i, int j, int k,
+                                                 S &out) {
     const double *dx1 = g.dx1, *dx2 = g.dx2, *dx3 = g.dx3, *dmx1 = g.dmx1, *dmx2 = g.dmx2, *dmx3 = g.dmx3;
     const int ip = i + 1, im = i - 1, jp = j + 1, jm = j - 1, kp = k + 1, km = k - 1;
     const int jvm = g.jC[jm], jvp = g.jC[jp];
 #define IRt(jj, kk) iR[g.t2((jj), (kk))]
 #define dPh(a, b, c) dPhat[g.ix(a, b, c)]
-    const double DivUm = (U[g.ix(ip, j, k)] - U[g.ix(i, j, k)]) / dx1[i]
+    const S DivUm = (U[g.ix(ip, j, k)] - U[g.ix(i, j, k)]) / dx1[i]
                        + ((double)jvp * V[g.ix(i, jp, k)] - (double)jvm * V[g.ix(i, j, k)]) / dx2[j]
                        + (W[g.ix(i, j, kp)] - W[g.ix(i, j, k)]) / dx3[k];
-    const double cbc = (1.0 - (double)jvm) * VBCbt[g.b2(i, k)] / dx2[j]
+    const S cbc = (1.0 - (double)jvm) * VBCbt[g.b2(i, k)] / dx2[j]
                      - (1.0 - (double)jvp) * VBCup[g.b2(i, k)] / dx2[j];
-    const double ddpdx1 = (dPh(i, j, k) - dPh(im, j, k)) / dmx1[i];
-    const double ddpdx2 = (dPh(ip, j, k) - dPh(i, j, k)) / dmx1[ip];
-    const double ddpdy3 = (dPh(i, j, k) - dPh(i, jm, k)) / dmx2[j];
-    const double ddpdy4 = (dPh(i, jp, k) - dPh(i, j, k)) / dmx2[jp];
-    const double ddpdz5 = (dPh(i, j, k) - dPh(i, j, km)) / dmx3[k];
-    const double ddpdz6 = (dPh(i, j, kp) - dPh(i, j, k)) / dmx3[kp];
-    const double invRho1 = 0.5 / dmx1[i] * (dx1[im] * IRt(j, k) + dx1[i] * IRt(j, k));
-    const double invRho2 = 0.5 / dmx1[ip] * (dx1[ip] * IRt(j, k) + dx1[i] * IRt(j, k));
-    const double invRho3 = 0.5 / dmx2[j] * (dx2[jm] * IRt(j, k) + dx2[j] * IRt(jm, k));
-    const double invRho4 = 0.5 / dmx2[jp] * (dx2[jp] * IRt(j, k) + dx2[j] * IRt(jp, k));
-    const double invRho5 = 0.5 / dmx3[k] * (dx3[km] * IRt(j, k) + dx3[k] * IRt(j, km));
-    const double invRho6 = 0.5 / dmx3[kp] * (dx3[kp] * IRt(j, k) + dx3[k] * IRt(j, kp));
-    const double ExtraTerm = ((1.0 - invRho2) * ddpdx2 - (1.0 - invRho1) * ddpdx1) / dx1[i]
+    const S ddpdx1 = (dPh(i, j, k) - dPh(im, j, k)) / dmx1[i];
+    const S ddpdx2 = (dPh(ip, j, k) - dPh(i, j, k)) / dmx1[ip];
+    const S ddpdy3 = (dPh(i, j, k) - dPh(i, jm, k)) / dmx2[j];
+    const S ddpdy4 = (dPh(i, jp, k) - dPh(i, j, k)) / dmx2[jp];
+    const S ddpdz5 = (dPh(i, j, k) - dPh(i, j, km)) / dmx3[k];
+    const S ddpdz6 = (dPh(i, j, kp) - dPh(i, j, k)) / dmx3[kp];
+    const S invRho1 = 0.5 / dmx1[i] * (dx1[im] * IRt(j, k) + dx1[i] * IRt(j, k));
+    const S invRho2 = 0.5 / dmx1[ip] * (dx1[ip] * IRt(j, k) + dx1[i] * IRt(j, k));
+    const S invRho3 = 0.5 / dmx2[j] * (dx2[jm] * IRt(j, k) + dx2[j] * IRt(jm, k));
+    const S invRho4 = 0.5 / dmx2[jp] * (dx2[jp] * IRt(j, k) + dx2[j] * IRt(jp, k));
+    const S invRho5 = 0.5 / dmx3[k] * (dx3[km] * IRt(j, k) + dx3[k] * IRt(j, km));
+    const S invRho6 = 0.5 / dmx3[kp] * (dx3[kp] * IRt(j, k) + dx3[k] * IRt(j, kp));
+    const S ExtraTerm = ((1.0 - invRho2) * ddpdx2 - (1.0 - invRho1) * ddpdx1) / dx1[i]
                            + ((1.0 - invRho4) * ddpdy4 - (1.0 - invRho3) * ddpdy3) / dx2[j]
                            + ((1.0 - invRho6) * ddpdz6 - (1.0 - invRho5) * ddpdz5) / dx3[k];
-    out[g.r3(i, j, k)] = (DivUm - cbc) / dt / p.Cmp + ExtraTerm;
+    out = (DivUm - cbc) / dt / p.Cmp + ExtraTerm;
+}
+struct PressureRHS {
+    Grid g;
+    Phys p;
+    double dt;
+    const double *U, *V, *W, *dPhat, *iR, *VBCup, *VBCbt;
+    static constexpr int NF = 7;
+    template <class S, class Wr>
+    __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
+        pressure_rhs_cell<S>(g, p, dt, w(0, U), w(1, V), w(2, W), w(3, dPhat), w(4, iR), w(5, VBCup), w(6, VBCbt), i,
+                             j, k, a);
+    }
+};
+
+__global__ void __launch_bounds__(256) k_pressure_rhs(PressureRHS cell, double *__restrict__ out) {
+    const Grid &g = cell.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.n1m || j > g.n2m) return;
+    double o, x;
+    cell.eval<double>(RawW(), i, j, k, o, x, x, x);
+    out[g.r3(i, j, k)] = o;
 }
 
 // x r2c: TX k-lines (k0..k0+TX-1) of row j per workgroup.
@@ -450,50 +470,94 @@ __global__ void k_wall_extrap(Grid g, double *__restrict__ F) {
 // [k>=2] already hold the new value 2 dP - (dPhat + dP0)/2.  This kernel
 // reads only old values and recomputes those neighbours' new values; a
 // second kernel then stores the new dPhat and dP0.
-__device__ __forceinline__ double dphat_new(double dP, double dPh, double dP0) {
-    const double d0 = (dPh + dP0) / 2.0;
+template <class S>
+__device__ __forceinline__ S dphat_new(S dP, S dPh, S dP0) {
+    const S d0 = (dPh + dP0) / 2.0;
     return 2.0 * dP - d0;
 }
 
-__global__ void __launch_bounds__(256)
-k_projection(Grid g, Phys p, double dt, const double *__restrict__ iR, const double *__restrict__ dP,
-             const double *__restrict__ dPhat, const double *__restrict__ dP0, double *__restrict__ U,
-             double *__restrict__ V, double *__restrict__ W, double *__restrict__ P) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
-    if (i > g.n1m || j > g.n2m) return;
+template <class S, class FP>
+__device__ __forceinline__ void projection_cell(const Grid &g, const Phys &p, double dt, FP iR, FP dP,
+                                               FP dPhat, FP dP0, FP U, FP V, FP W, FP P, int i, int j, int k,
+                                               S &uo, S &vo, S &wo, S &po) {
     const double *dx1 = g.dx1, *dx2 = g.dx2, *dx3 = g.dx3, *dmx1 = g.dmx1, *dmx2 = g.dmx2, *dmx3 = g.dmx3;
     const int im = i - 1, jm = j - 1, km = k - 1;
-    const double Cmp = p.Cmp;
+    const S Cmp = p.Cmp;
     const size_t q = g.ix(i, j, k);
-    const double dPc = dP[q], dPhc = dPhat[q];
+    const S dPc = dP[q], dPhc = dPhat[q];
     const size_t qi = g.ix(im, j, k), qj = g.ix(i, jm, k), qk = g.ix(i, j, km);
-    const double hi = (i >= 2) ? dphat_new(dP[qi], dPhat[qi], dP0[qi]) : dPhat[qi];
-    const double hj = (j >= 3) ? dphat_new(dP[qj], dPhat[qj], dP0[qj]) : dPhat[qj];
-    const double hk = (k >= 2) ? dphat_new(dP[qk], dPhat[qk], dP0[qk]) : dPhat[qk];
-    double invRhoc = 0.5 / dmx1[i] * (dx1[i] * IRt(j, k) + dx1[im] * IRt(j, k));
-    U[q] = U[q] - dt * Cmp * ((dPc - dP[qi]) / dmx1[i] + (invRhoc - 1.0) * (dPhc - hi) / dmx1[i]);
+    const S hi = (i >= 2) ? dphat_new(dP[qi], dPhat[qi], dP0[qi]) : dPhat[qi];
+    const S hj = (j >= 3) ? dphat_new(dP[qj], dPhat[qj], dP0[qj]) : dPhat[qj];
+    const S hk = (k >= 2) ? dphat_new(dP[qk], dPhat[qk], dP0[qk]) : dPhat[qk];
+    S invRhoc = 0.5 / dmx1[i] * (dx1[i] * IRt(j, k) + dx1[im] * IRt(j, k));
+    uo = U[q] - dt * Cmp * ((dPc - dP[qi]) / dmx1[i] + (invRhoc - 1.0) * (dPhc - hi) / dmx1[i]);
     invRhoc = 0.5 / dmx2[j] * (dx2[j] * IRt(jm, k) + dx2[jm] * IRt(j, k));
     if (j >= 2)
-        V[q] = V[q] - dt * Cmp * ((dPc - dP[qj]) / dmx2[j] + (invRhoc - 1.0) * (dPhc - hj) / dmx2[j]);
+        vo = V[q] - dt * Cmp * ((dPc - dP[qj]) / dmx2[j] + (invRhoc - 1.0) * (dPhc - hj) / dmx2[j]);
     else
-        V[q] = V[q] - dt * Cmp * ((dPc - dP[qj]) / dmx2[j] + (invRhoc - 1.0) * (dPhc - hj) / dmx2[j]) * 0.0;
+        vo = V[q] - dt * Cmp * ((dPc - dP[qj]) / dmx2[j] + (invRhoc - 1.0) * (dPhc - hj) / dmx2[j]) * 0.0;
     invRhoc = 0.5 / dmx3[k] * (dx3[k] * IRt(j, km) + dx3[km] * IRt(j, k));
-    W[q] = W[q] - dt * Cmp * ((dPc - dP[qk]) / dmx3[k] + (invRhoc - 1.0) * (dPhc - hk) / dmx3[k]);
-    P[q] = P[q] + dPc;
+    wo = W[q] - dt * Cmp * ((dPc - dP[qk]) / dmx3[k] + (invRhoc - 1.0) * (dPhc - hk) / dmx3[k]);
+    po = P[q] + dPc;
 }
+struct Projection {
+    Grid g;
+    Phys p;
+    double dt;
+    const double *iR, *dP, *dPhat, *dP0, *U, *V, *W, *P;
+    static constexpr int NF = 8;
+    template <class S, class Wr>
+    __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
+        projection_cell<S>(g, p, dt, w(0, iR), w(1, dP), w(2, dPhat), w(3, dP0), w(4, U), w(5, V), w(6, W), w(7, P), i,
+                           j, k, a, b, c, d);
+    }
+};
 
-__global__ void __launch_bounds__(256)
-k_dphat_update(Grid g, const double *__restrict__ dP, double *__restrict__ dPhat, double *__restrict__ dP0) {
+__global__ void __launch_bounds__(256) k_projection(Projection cell, double *U, double *V, double *W, double *P) {
+    const Grid &g = cell.g;
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
     if (i > g.n1m || j > g.n2m) return;
+    double u, v, w, q;
+    cell.eval<double>(RawW(), i, j, k, u, v, w, q);
+    const size_t o = g.ix(i, j, k);
+    U[o] = u;
+    V[o] = v;
+    W[o] = w;
+    P[o] = q;
+}
+
+// the dPhat and dP0 history after the projection (:846-850)
+template <class S, class FP>
+__device__ __forceinline__ void dphat_cell(const Grid &g, FP dP, FP dPhat, FP dP0, int i, int j, int k, S &hn,
+                                          S &d0n) {
     const size_t q = g.ix(i, j, k);
-    const double d0 = (dPhat[q] + dP0[q]) / 2.0;
+    const S d0 = (dPhat[q] + dP0[q]) / 2.0;
+    d0n = d0;
+    hn = 2.0 * dP[q] - d0;
+}
+struct DPhatUpdate {
+    Grid g;
+    const double *dP, *dPhat, *dP0;
+    static constexpr int NF = 3;
+    template <class S, class Wr>
+    __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
+        dphat_cell<S>(g, w(0, dP), w(1, dPhat), w(2, dP0), i, j, k, a, b);
+    }
+};
+
+__global__ void __launch_bounds__(256) k_dphat_update(DPhatUpdate cell, double *dPhat, double *dP0) {
+    const Grid &g = cell.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.n1m || j > g.n2m) return;
+    double h, d0, x;
+    cell.eval<double>(RawW(), i, j, k, h, d0, x, x);
+    const size_t q = g.ix(i, j, k);
     dP0[q] = d0;
-    dPhat[q] = 2.0 * dP[q] - d0;
+    dPhat[q] = h;
 }
 
 // ---------------------------------------------------------------- host -----
@@ -507,9 +571,9 @@ static int fft_tile(int n1m, int n3m) {
 
 int pressure_rhs(tcs_ctx *c, double dt) {
     const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
-    TCS_LAUNCH(c->stream, "pressure_rhs", k_pressure_rhs, grd, blk, 0, c->grid(), c->phys, dt, c->f[TCS_U], c->f[TCS_V],
-                       c->f[TCS_W], c->f[TCS_DPHAT], c->f[TCS_INVRHO], c->wall[TCS_VBCUP], c->wall[TCS_VBCBT],
-                       c->prhs);
+    const PressureRHS cell{c->grid(),         c->phys,          dt, c->f[TCS_U], c->f[TCS_V], c->f[TCS_W], c->f[TCS_DPHAT],
+                           c->f[TCS_INVRHO], c->wall[TCS_VBCUP], c->wall[TCS_VBCBT]};
+    TCS_LAUNCH(c->stream, "pressure_rhs", k_pressure_rhs, grd, blk, 0, cell, c->prhs);
     return TCS_OK;
 }
 
@@ -554,9 +618,12 @@ int pressure_poisson(tcs_ctx *c) {
 int pressure_projection(tcs_ctx *c, double dt) {
     const Grid g = c->grid();
     const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
-    TCS_LAUNCH(c->stream, "projection", k_projection, grd, blk, 0, g, c->phys, dt, c->f[TCS_INVRHO], c->f[TCS_DP],
-                       c->f[TCS_DPHAT], c->f[TCS_DP0], c->f[TCS_U], c->f[TCS_V], c->f[TCS_W], c->f[TCS_P]);
-    TCS_LAUNCH(c->stream, "dphat_update", k_dphat_update, grd, blk, 0, g, c->f[TCS_DP], c->f[TCS_DPHAT], c->f[TCS_DP0]);
+    const Projection cell{g, c->phys, dt, c->f[TCS_INVRHO], c->f[TCS_DP], c->f[TCS_DPHAT], c->f[TCS_DP0],
+                          c->f[TCS_U], c->f[TCS_V], c->f[TCS_W], c->f[TCS_P]};
+    TCS_LAUNCH(c->stream, "projection", k_projection, grd, blk, 0, cell, c->f[TCS_U], c->f[TCS_V], c->f[TCS_W],
+               c->f[TCS_P]);
+    TCS_LAUNCH(c->stream, "dphat_update", k_dphat_update, grd, blk, 0, DPhatUpdate{g, c->f[TCS_DP], c->f[TCS_DPHAT],
+               c->f[TCS_DP0]}, c->f[TCS_DPHAT], c->f[TCS_DP0]);
     TCS_LAUNCH(c->stream, "wall_extrap", k_wall_extrap, dim3((c->n1m + 63) / 64, c->n3m), dim3(64), 0, g, c->f[TCS_P]);
     return TCS_OK;
 }

@@ -19,6 +19,7 @@
 // split solve's, so both ways give bit-identical solutions.
 #pragma once
 
+#include "ad.h"
 #include "tdma_part.h"
 
 namespace tcs {

@@ -26,39 +26,43 @@ namespace TCS_FORM {
 using scr_t = TCS_SCRATCH;
 #define SCR(p) (reinterpret_cast<scr_t *>(p))
 
-__device__ __forceinline__ void th_kpp(const Phys &p, double T, double &kpp, double &dkpp) {
-    const double tmp1 = p.DeltaT * T;
-    const double tmp2 = tmp1 * tmp1;
-    const double tmp3 = tmp2 * tmp1;
-    const double tmp4 = tmp3 * tmp1;
-    const double tmp5 = tmp4 * tmp1;
+template <class S>
+__device__ __forceinline__ void th_kpp(const Phys &p, S T, S &kpp, S &dkpp) {
+    const S tmp1 = p.DeltaT * T;
+    const S tmp2 = tmp1 * tmp1;
+    const S tmp3 = tmp2 * tmp1;
+    const S tmp4 = tmp3 * tmp1;
+    const S tmp5 = tmp4 * tmp1;
     kpp = p.c10 * (1.0 + p.c11 * tmp1 + p.c12 * tmp2 + p.c13 * tmp3 + p.c14 * tmp4 + p.c15 * tmp5) / p.KPP0;
     dkpp = p.c10 * (p.c11 + 2.0 * p.c12 * tmp1 + 3.0 * p.c13 * tmp2 + 4.0 * p.c14 * tmp3 + 5.0 * p.c15 * tmp4) / p.KPP0;
 }
 
-__device__ __forceinline__ double th_kpp_only(const Phys &p, double T) {
-    double k, dk;
+template <class S>
+__device__ __forceinline__ S th_kpp_only(const Phys &p, S T) {
+    S k, dk;
     th_kpp(p, T, k, dk);
     return k;
 }
 
-__device__ __forceinline__ double th_dkpp_only(const Phys &p, double T) {
-    double k, dk;
+template <class S>
+__device__ __forceinline__ S th_dkpp_only(const Phys &p, S T) {
+    S k, dk;
     th_kpp(p, T, k, dk);
     return dk;
 }
 
-__device__ __forceinline__ void th_rhocp(const Phys &p, double T, double &irc, double &dirc) {
-    const double tmp1 = p.DeltaT * T;
-    const double tmp2 = tmp1 * tmp1;
-    const double tmp3 = tmp2 * tmp1;
-    const double tmp4 = tmp3 * tmp1;
-    const double tmp5 = tmp4 * tmp1;
+template <class S>
+__device__ __forceinline__ void th_rhocp(const Phys &p, S T, S &irc, S &dirc) {
+    const S tmp1 = p.DeltaT * T;
+    const S tmp2 = tmp1 * tmp1;
+    const S tmp3 = tmp2 * tmp1;
+    const S tmp4 = tmp3 * tmp1;
+    const S tmp5 = tmp4 * tmp1;
     irc = p.Rho0 * p.Cp0
         / (p.a10 + p.a10 * (p.a11 * tmp1 + p.a12 * tmp2 + p.a13 * tmp3 + p.a14 * tmp4 + p.a15 * tmp5))
         / (p.b10 + p.b10 * (p.b11 * tmp1 + p.b12 * tmp2 + p.b13 * tmp3 + p.b14 * tmp4 + p.b15 * tmp5));
-    const double da = 1.0 + p.a11 * tmp1 + p.a12 * tmp2 + p.a13 * tmp3;
-    const double db = 1.0 + p.b11 * tmp1 + p.b12 * tmp2 + p.b13 * tmp3;
+    const S da = 1.0 + p.a11 * tmp1 + p.a12 * tmp2 + p.a13 * tmp3;
+    const S db = 1.0 + p.b11 * tmp1 + p.b12 * tmp2 + p.b13 * tmp3;
     dirc = -(p.b11 * (1.0 + 3.0 * p.a12 * tmp2 + 4.0 * p.a13 * tmp3)
              + p.a11 * (1.0 + 2.0 * p.b11 * tmp1 + 3.0 * p.b12 * tmp2 + 4.0 * p.b13 * tmp3)
              + tmp1 * (p.b12 * (2.0 + 5.0 * p.a13 * tmp3)
@@ -80,23 +84,24 @@ __global__ void k_thermal_coeffi(Grid g, Phys p, const double *__restrict__ T, d
 }
 
 // stage 1: z (:236-361)
+template <class S, class FP>
 __device__ __forceinline__ void thermal_z_row(const Grid &g, const Phys &p, double dt,
-    const double *__restrict__ T, const double *__restrict__ U, const double *__restrict__ V,
-    const double *__restrict__ W, const double *__restrict__ TBCup, const double *__restrict__ TBCbt,
-    int i, int j, int k, double &am, double &ac, double &ap, double &rhsout) {
+    FP T, FP U, FP V,
+    FP W, FP TBCup, FP TBCbt,
+    int i, int j, int k, S &am, S &ac, S &ap, S &rhsout) {
     const int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1, km = k - 1, kp = k + 1;
     const int jem = g.jC[jm], jep = g.jC[jp];
     const double *dx1 = g.dx1, *dx2 = g.dx2, *dx3 = g.dx3, *dmx1 = g.dmx1, *dmx2 = g.dmx2, *dmx3 = g.dmx3;
-    const double Ct = p.Ct;
+    const S Ct = p.Ct;
 
-    const double Tc = T[g.ix(i, j, k)];
-    const double Tim = T[g.ix(im, j, k)], Tip = T[g.ix(ip, j, k)];
-    const double Tjm = T[g.ix(i, jm, k)], Tjp = T[g.ix(i, jp, k)];
-    const double Tkm = T[g.ix(i, j, km)], Tkp = T[g.ix(i, j, kp)];
+    const S Tc = T[g.ix(i, j, k)];
+    const S Tim = T[g.ix(im, j, k)], Tip = T[g.ix(ip, j, k)];
+    const S Tjm = T[g.ix(i, jm, k)], Tjp = T[g.ix(i, jp, k)];
+    const S Tkm = T[g.ix(i, j, km)], Tkp = T[g.ix(i, j, kp)];
 
-    double irc, dirc;
+    S irc, dirc;
     th_rhocp(p, Tc, irc, dirc);
-    double Kc, dKc, Kim, dKim, Kip, dKip, Kjm, dKjm, Kjp, dKjp, Kkm, dKkm, Kkp, dKkp;
+    S Kc, dKc, Kim, dKim, Kip, dKip, Kjm, dKjm, Kjp, dKjp, Kkm, dKkm, Kkp, dKkp;
     th_kpp(p, Tc, Kc, dKc);
     th_kpp(p, Tim, Kim, dKim);
     th_kpp(p, Tip, Kip, dKip);
@@ -105,46 +110,46 @@ __device__ __forceinline__ void thermal_z_row(const Grid &g, const Phys &p, doub
     th_kpp(p, Tkm, Kkm, dKkm);
     th_kpp(p, Tkp, Kkp, dKkp);
 
-    const double invRhoCpCt_half = 0.5 * Ct * irc;
-    const double dinvRhoCpCt_half = 0.5 * Ct * dirc;
+    const S invRhoCpCt_half = 0.5 * Ct * irc;
+    const S dinvRhoCpCt_half = 0.5 * Ct * dirc;
 
-    const double e1 = (dx1[i] * Tim + dx1[im] * Tc) * (0.5 / dmx1[i]);
-    const double e2 = (dx1[ip] * Tc + dx1[i] * Tip) * (0.5 / dmx1[ip]);
-    const double e3 = (dx2[j] * Tjm + dx2[jm] * Tc) * (0.5 / dmx2[j]);
-    const double e4 = (dx2[jp] * Tc + dx2[j] * Tjp) * (0.5 / dmx2[jp]);
-    const double e5 = (dx3[k] * Tkm + dx3[km] * Tc) * (0.5 / dmx3[k]);
-    const double e6 = (dx3[kp] * Tc + dx3[k] * Tkp) * (0.5 / dmx3[kp]);
-    const double dedx1 = (Tc - Tim) / dmx1[i];
-    const double dedx2 = (Tip - Tc) / dmx1[ip];
-    const double dedy3 = (Tc - Tjm) / dmx2[j];
-    const double dedy4 = (Tjp - Tc) / dmx2[jp];
-    const double dedz5 = (Tc - Tkm) / dmx3[k];
-    const double dedz6 = (Tkp - Tc) / dmx3[kp];
+    const S e1 = (dx1[i] * Tim + dx1[im] * Tc) * (0.5 / dmx1[i]);
+    const S e2 = (dx1[ip] * Tc + dx1[i] * Tip) * (0.5 / dmx1[ip]);
+    const S e3 = (dx2[j] * Tjm + dx2[jm] * Tc) * (0.5 / dmx2[j]);
+    const S e4 = (dx2[jp] * Tc + dx2[j] * Tjp) * (0.5 / dmx2[jp]);
+    const S e5 = (dx3[k] * Tkm + dx3[km] * Tc) * (0.5 / dmx3[k]);
+    const S e6 = (dx3[kp] * Tc + dx3[k] * Tkp) * (0.5 / dmx3[kp]);
+    const S dedx1 = (Tc - Tim) / dmx1[i];
+    const S dedx2 = (Tip - Tc) / dmx1[ip];
+    const S dedy3 = (Tc - Tjm) / dmx2[j];
+    const S dedy4 = (Tjp - Tc) / dmx2[jp];
+    const S dedz5 = (Tc - Tkm) / dmx3[k];
+    const S dedz6 = (Tkp - Tc) / dmx3[kp];
 
-    const double Uc = U[g.ix(i, j, k)], Uip = U[g.ix(ip, j, k)];
-    const double Vc = V[g.ix(i, j, k)], Vjp = V[g.ix(i, jp, k)];
-    const double Wc = W[g.ix(i, j, k)], Wkp = W[g.ix(i, j, kp)];
-    const double convect_e1 = (Uc * dedx1 + Uip * dedx2) * 0.5;
-    const double convect_e2 = (Vc * dedy3 + Vjp * dedy4) * 0.5;
-    const double convect_e3 = (Wc * dedz5 + Wkp * dedz6) * 0.5;
-    double rhs = -0.5 * (convect_e1 + convect_e2 + convect_e3);
+    const S Uc = U[g.ix(i, j, k)], Uip = U[g.ix(ip, j, k)];
+    const S Vc = V[g.ix(i, j, k)], Vjp = V[g.ix(i, jp, k)];
+    const S Wc = W[g.ix(i, j, k)], Wkp = W[g.ix(i, j, kp)];
+    const S convect_e1 = (Uc * dedx1 + Uip * dedx2) * 0.5;
+    const S convect_e2 = (Vc * dedy3 + Vjp * dedy4) * 0.5;
+    const S convect_e3 = (Wc * dedz5 + Wkp * dedz6) * 0.5;
+    S rhs = -0.5 * (convect_e1 + convect_e2 + convect_e3);
 
-    const double kPP1 = 0.5 / dmx1[i] * (dx1[i] * Kim + dx1[im] * Kc);
-    const double kPP2 = 0.5 / dmx1[ip] * (dx1[ip] * Kc + dx1[i] * Kip);
-    const double kPP3 = 0.5 / dmx2[j] * (dx2[j] * Kjm + dx2[jm] * Kc);
-    const double kPP4 = 0.5 / dmx2[jp] * (dx2[jp] * Kc + dx2[j] * Kjp);
-    const double kPP5 = 0.5 / dmx3[k] * (dx3[k] * Kkm + dx3[km] * Kc);
-    const double kPP6 = 0.5 / dmx3[kp] * (dx3[kp] * Kc + dx3[k] * Kkp);
-    const double dkPP1 = 0.5 / dmx1[i] * (dx1[i] * dKim + dx1[im] * dKc);
-    const double dkPP2 = 0.5 / dmx1[ip] * (dx1[ip] * dKc + dx1[i] * dKip);
-    const double dkPP3 = 0.5 / dmx2[j] * (dx2[j] * dKjm + dx2[jm] * dKc);
-    const double dkPP4 = 0.5 / dmx2[jp] * (dx2[jp] * dKc + dx2[j] * dKjp);
-    const double dkPP5 = 0.5 / dmx3[k] * (dx3[k] * dKkm + dx3[km] * dKc);
-    const double dkPP6 = 0.5 / dmx3[kp] * (dx3[kp] * dKc + dx3[k] * dKkp);
+    const S kPP1 = 0.5 / dmx1[i] * (dx1[i] * Kim + dx1[im] * Kc);
+    const S kPP2 = 0.5 / dmx1[ip] * (dx1[ip] * Kc + dx1[i] * Kip);
+    const S kPP3 = 0.5 / dmx2[j] * (dx2[j] * Kjm + dx2[jm] * Kc);
+    const S kPP4 = 0.5 / dmx2[jp] * (dx2[jp] * Kc + dx2[j] * Kjp);
+    const S kPP5 = 0.5 / dmx3[k] * (dx3[k] * Kkm + dx3[km] * Kc);
+    const S kPP6 = 0.5 / dmx3[kp] * (dx3[kp] * Kc + dx3[k] * Kkp);
+    const S dkPP1 = 0.5 / dmx1[i] * (dx1[i] * dKim + dx1[im] * dKc);
+    const S dkPP2 = 0.5 / dmx1[ip] * (dx1[ip] * dKc + dx1[i] * dKip);
+    const S dkPP3 = 0.5 / dmx2[j] * (dx2[j] * dKjm + dx2[jm] * dKc);
+    const S dkPP4 = 0.5 / dmx2[jp] * (dx2[jp] * dKc + dx2[j] * dKjp);
+    const S dkPP5 = 0.5 / dmx3[k] * (dx3[k] * dKkm + dx3[km] * dKc);
+    const S dkPP6 = 0.5 / dmx3[kp] * (dx3[kp] * dKc + dx3[k] * dKkp);
 
-    const double viscous_e1 = (kPP2 * dedx2 - kPP1 * dedx1 - e2 * dkPP2 * dedx2 + e1 * dkPP1 * dedx1) / dx1[i];
-    const double viscous_e2 = (kPP4 * dedy4 - kPP3 * dedy3 - e4 * dkPP4 * dedy4 + e3 * dkPP3 * dedy3) / dx2[j];
-    const double viscous_e3 = (kPP6 * dedz6 - kPP5 * dedz5 - e6 * dkPP6 * dedz6 + e5 * dkPP5 * dedz5) / dx3[k];
+    const S viscous_e1 = (kPP2 * dedx2 - kPP1 * dedx1 - e2 * dkPP2 * dedx2 + e1 * dkPP1 * dedx1) / dx1[i];
+    const S viscous_e2 = (kPP4 * dedy4 - kPP3 * dedy3 - e4 * dkPP4 * dedy4 + e3 * dkPP3 * dedy3) / dx2[j];
+    const S viscous_e3 = (kPP6 * dedz6 - kPP5 * dedz5 - e6 * dkPP6 * dedz6 + e5 * dkPP5 * dedz5) / dx3[k];
 
     rhs = rhs
         + invRhoCpCt_half * (viscous_e1 + viscous_e2 + viscous_e3)
@@ -152,9 +157,9 @@ __device__ __forceinline__ void thermal_z_row(const Grid &g, const Phys &p, doub
                                    + (kPP4 * dedy4 - kPP3 * dedy3) / dx2[j]
                                    + (kPP6 * dedz6 - kPP5 * dedz5) / dx3[k]);
 
-    const double tup = TBCup[g.b2(i, k)], tbt = TBCbt[g.b2(i, k)];
-    double ebc_up = -0.25 * Vjp / dmx2[jp] * tup;
-    double ebc_down = 0.25 * Vc / dmx2[j] * tbt;
+    const S tup = TBCup[g.b2(i, k)], tbt = TBCbt[g.b2(i, k)];
+    S ebc_up = -0.25 * Vjp / dmx2[jp] * tup;
+    S ebc_down = 0.25 * Vc / dmx2[j] * tbt;
     ebc_up = ebc_up
            + invRhoCpCt_half / dx2[j] * kPP4 / dmx2[jp] * tup
            + invRhoCpCt_half / dx2[j] * dkPP4 * dedy4 * tup;
@@ -163,26 +168,26 @@ __device__ __forceinline__ void thermal_z_row(const Grid &g, const Phys &p, doub
              - invRhoCpCt_half / dx2[j] * dkPP3 * dedy3 * tbt;
     rhs = rhs + (double)(1 - jem) * ebc_down + (double)(1 - jep) * ebc_up;
 
-    const double eACK = invRhoCpCt_half / dx3[k] * (kPP6 / dmx3[kp] + kPP5 / dmx3[k]
+    const S eACK = invRhoCpCt_half / dx3[k] * (kPP6 / dmx3[kp] + kPP5 / dmx3[k]
                                                     - dx3[kp] * (0.5 / dmx3[kp]) * dkPP6 * dedz6
                                                     + dx3[km] * (0.5 / dmx3[k]) * dkPP5 * dedz5)
                       + (-0.25 * Wkp / dmx3[kp] + 0.25 * Wc / dmx3[k]);
-    const double eAPK = -invRhoCpCt_half / dx3[k] * (kPP6 / dmx3[kp] + dx3[k] * (0.5 / dmx3[kp]) * dkPP6 * dedz6) + 0.25 * Wkp / dmx3[kp];
-    const double eAMK = -invRhoCpCt_half / dx3[k] * (kPP5 / dmx3[k] - dx3[k] * (0.5 / dmx3[k]) * dkPP5 * dedz5) - 0.25 * Wc / dmx3[k];
+    const S eAPK = -invRhoCpCt_half / dx3[k] * (kPP6 / dmx3[kp] + dx3[k] * (0.5 / dmx3[kp]) * dkPP6 * dedz6) + 0.25 * Wkp / dmx3[kp];
+    const S eAMK = -invRhoCpCt_half / dx3[k] * (kPP5 / dmx3[k] - dx3[k] * (0.5 / dmx3[k]) * dkPP5 * dedz5) - 0.25 * Wc / dmx3[k];
 
-    const double eACI = invRhoCpCt_half / dx1[i] * (kPP2 / dmx1[ip] + kPP1 / dmx1[i]
+    const S eACI = invRhoCpCt_half / dx1[i] * (kPP2 / dmx1[ip] + kPP1 / dmx1[i]
                                                     - dx1[ip] * (0.5 / dmx1[ip]) * dkPP2 * dedx2
                                                     + dx1[im] * (0.5 / dmx1[i]) * dkPP1 * dedx1)
                       + (0.25 * Uc / dmx1[i] - 0.25 * Uip / dmx1[ip]);
-    const double eAPI = -invRhoCpCt_half / dx1[i] * (kPP2 / dmx1[ip] + dx1[i] * (0.5 / dmx1[ip]) * dkPP2 * dedx2) + 0.25 * Uip / dmx1[ip];
-    const double eAMI = -invRhoCpCt_half / dx1[i] * (kPP1 / dmx1[i] - dx1[i] * (0.5 / dmx1[i]) * dkPP1 * dedx1) - 0.25 * Uc / dmx1[i];
+    const S eAPI = -invRhoCpCt_half / dx1[i] * (kPP2 / dmx1[ip] + dx1[i] * (0.5 / dmx1[ip]) * dkPP2 * dedx2) + 0.25 * Uip / dmx1[ip];
+    const S eAMI = -invRhoCpCt_half / dx1[i] * (kPP1 / dmx1[i] - dx1[i] * (0.5 / dmx1[i]) * dkPP1 * dedx1) - 0.25 * Uc / dmx1[i];
 
-    const double eACJ = invRhoCpCt_half / dx2[j] * (kPP4 / dmx2[jp] + kPP3 / dmx2[j]
+    const S eACJ = invRhoCpCt_half / dx2[j] * (kPP4 / dmx2[jp] + kPP3 / dmx2[j]
                                                     - dx2[jp] * (0.5 / dmx2[jp]) * dkPP4 * dedy4 * (double)jep
                                                     + dx2[jm] * (0.5 / dmx2[j]) * dkPP3 * dedy3 * (double)jem)
                       + (0.25 * Vc / dmx2[j] - 0.25 * Vjp / dmx2[jp]);
-    double eAPJ = -invRhoCpCt_half / dx2[j] * (kPP4 / dmx2[jp] + dx2[j] * (0.5 / dmx2[jp]) * dkPP4 * dedy4) + 0.25 * Vjp / dmx2[jp];
-    double eAMJ = -invRhoCpCt_half / dx2[j] * (kPP3 / dmx2[j] - dx2[j] * (0.5 / dmx2[j]) * dkPP3 * dedy3) - 0.25 * Vc / dmx2[j];
+    S eAPJ = -invRhoCpCt_half / dx2[j] * (kPP4 / dmx2[jp] + dx2[j] * (0.5 / dmx2[jp]) * dkPP4 * dedy4) + 0.25 * Vjp / dmx2[jp];
+    S eAMJ = -invRhoCpCt_half / dx2[j] * (kPP3 / dmx2[j] - dx2[j] * (0.5 / dmx2[j]) * dkPP3 * dedy3) - 0.25 * Vc / dmx2[j];
     eAPJ = eAPJ * (double)jep;
     eAMJ = eAMJ * (double)jem;
 
@@ -208,60 +213,66 @@ struct ThermalZ {
     Phys p;
     double dt;
     const double *T, *U, *V, *W, *TBCup, *TBCbt;
+    static constexpr int NF = 6;  // field inputs, in the order below
+    template <class S, class Wr>
+    __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
+        thermal_z_row<S>(g, p, dt, w(0, T), w(1, U), w(2, V), w(3, W), w(4, TBCup), w(5, TBCbt), i, j, k, a, b, c, d);
+    }
     __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
                                                double &d) const {
-        thermal_z_row(g, p, dt, T, U, V, W, TBCup, TBCbt, i, j, k, a, b, c, d);
+        eval<double>(RawW(), i, j, k, a, b, c, d);
     }
 };
 
 // stage 2: x (:388-429).  jm,jp,km,kp are the values the reference's loop
 // variables keep after stage 1: n2m-1, n2m+1, n3m-1, n3m+1.
+template <class S, class FP>
 __device__ __forceinline__ void thermal_x_row(const Grid &g, const Phys &p, double dt,
-    const double *__restrict__ T, const double *__restrict__ U,
-    int i, int j, int k, double &am, double &ac, double &ap) {
+    FP T, FP U,
+    int i, int j, int k, S &am, S &ac, S &ap) {
     const int im = i - 1, ip = i + 1;
     const int jm = g.n2m - 1, jp = g.n2m + 1, km = g.n3m - 1, kp = g.n3m + 1;
     const double *dx1 = g.dx1, *dx2 = g.dx2, *dx3 = g.dx3, *dmx1 = g.dmx1, *dmx2 = g.dmx2, *dmx3 = g.dmx3;
 
-    const double Tc = T[g.ix(i, j, k)];
-    const double Tim = T[g.ix(im, j, k)], Tip = T[g.ix(ip, j, k)];
-    const double Tjm = T[g.ix(i, jm, k)], Tjp = T[g.ix(i, jp, k)];
-    const double Tkm = T[g.ix(i, j, km)], Tkp = T[g.ix(i, j, kp)];
-    double irc, dirc;
+    const S Tc = T[g.ix(i, j, k)];
+    const S Tim = T[g.ix(im, j, k)], Tip = T[g.ix(ip, j, k)];
+    const S Tjm = T[g.ix(i, jm, k)], Tjp = T[g.ix(i, jp, k)];
+    const S Tkm = T[g.ix(i, j, km)], Tkp = T[g.ix(i, j, kp)];
+    S irc, dirc;
     th_rhocp(p, Tc, irc, dirc);
-    double Kc, dKc, Kim, dKim, Kip, dKip;
+    S Kc, dKc, Kim, dKim, Kip, dKip;
     th_kpp(p, Tc, Kc, dKc);
     th_kpp(p, Tim, Kim, dKim);
     th_kpp(p, Tip, Kip, dKip);
-    const double Kjm = th_kpp_only(p, Tjm), Kjp = th_kpp_only(p, Tjp);
-    const double Kkm = th_kpp_only(p, Tkm), Kkp = th_kpp_only(p, Tkp);
+    const S Kjm = th_kpp_only(p, Tjm), Kjp = th_kpp_only(p, Tjp);
+    const S Kkm = th_kpp_only(p, Tkm), Kkp = th_kpp_only(p, Tkp);
 
-    const double invRhoCpCt_half = 0.5 * p.Ct * irc;
-    const double kPP1 = 0.5 / dmx1[i] * (dx1[i] * Kim + dx1[im] * Kc);
-    const double kPP2 = 0.5 / dmx1[ip] * (dx1[ip] * Kc + dx1[i] * Kip);
-    const double kPP3 = 0.5 / dmx2[j] * (dx2[j] * Kjm + dx2[jm] * Kc);
-    const double kPP4 = 0.5 / dmx2[jp] * (dx2[jp] * Kc + dx2[j] * Kjp);
-    const double kPP5 = 0.5 / dmx3[k] * (dx3[k] * Kkm + dx3[km] * Kc);
-    const double kPP6 = 0.5 / dmx3[kp] * (dx3[kp] * Kc + dx3[k] * Kkp);
-    const double dkPP1 = 0.5 / dmx1[i] * (dx1[i] * dKim + dx1[im] * dKc);
-    const double dkPP2 = 0.5 / dmx1[ip] * (dx1[ip] * dKc + dx1[i] * dKip);
-    const double dedx1 = (Tc - Tim) / dmx1[i];
-    const double dedx2 = (Tip - Tc) / dmx1[ip];
-    const double dedy3 = (Tc - Tjm) / dmx2[j];
-    const double dedy4 = (Tjp - Tc) / dmx2[jp];
-    const double dedz5 = (Tc - Tkm) / dmx3[k];
-    const double dedz6 = (Tkp - Tc) / dmx3[kp];
-    const double Uc = U[g.ix(i, j, k)], Uip = U[g.ix(ip, j, k)];
+    const S invRhoCpCt_half = 0.5 * p.Ct * irc;
+    const S kPP1 = 0.5 / dmx1[i] * (dx1[i] * Kim + dx1[im] * Kc);
+    const S kPP2 = 0.5 / dmx1[ip] * (dx1[ip] * Kc + dx1[i] * Kip);
+    const S kPP3 = 0.5 / dmx2[j] * (dx2[j] * Kjm + dx2[jm] * Kc);
+    const S kPP4 = 0.5 / dmx2[jp] * (dx2[jp] * Kc + dx2[j] * Kjp);
+    const S kPP5 = 0.5 / dmx3[k] * (dx3[k] * Kkm + dx3[km] * Kc);
+    const S kPP6 = 0.5 / dmx3[kp] * (dx3[kp] * Kc + dx3[k] * Kkp);
+    const S dkPP1 = 0.5 / dmx1[i] * (dx1[i] * dKim + dx1[im] * dKc);
+    const S dkPP2 = 0.5 / dmx1[ip] * (dx1[ip] * dKc + dx1[i] * dKip);
+    const S dedx1 = (Tc - Tim) / dmx1[i];
+    const S dedx2 = (Tip - Tc) / dmx1[ip];
+    const S dedy3 = (Tc - Tjm) / dmx2[j];
+    const S dedy4 = (Tjp - Tc) / dmx2[jp];
+    const S dedz5 = (Tc - Tkm) / dmx3[k];
+    const S dedz6 = (Tkp - Tc) / dmx3[kp];
+    const S Uc = U[g.ix(i, j, k)], Uip = U[g.ix(ip, j, k)];
 
-    const double eACI = invRhoCpCt_half / dx1[i] * (kPP2 / dmx1[ip] + kPP1 / dmx1[i]
+    const S eACI = invRhoCpCt_half / dx1[i] * (kPP2 / dmx1[ip] + kPP1 / dmx1[i]
                                                     - dx1[ip] * (0.5 / dmx1[ip]) * dkPP2 * dedx2
                                                     + dx1[im] * (0.5 / dmx1[i]) * dkPP1 * dedx1)
                       + (0.25 * Uc / dmx1[i] - 0.25 * Uip / dmx1[ip])
                       - 0.5 * invRhoCpCt_half * ((kPP2 * dedx2 - kPP1 * dedx1) / dx1[i]
                                                  + (kPP4 * dedy4 - kPP3 * dedy3) / dx2[j]
                                                  + (kPP6 * dedz6 - kPP5 * dedz5) / dx3[k]);
-    const double eAPI = -invRhoCpCt_half / dx1[i] * (kPP2 / dmx1[ip] + dx1[i] * (0.5 / dmx1[ip]) * dkPP2 * dedx2) + 0.25 * Uip / dmx1[ip];
-    const double eAMI = -invRhoCpCt_half / dx1[i] * (kPP1 / dmx1[i] - dx1[i] * (0.5 / dmx1[i]) * dkPP1 * dedx1) - 0.25 * Uc / dmx1[i];
+    const S eAPI = -invRhoCpCt_half / dx1[i] * (kPP2 / dmx1[ip] + dx1[i] * (0.5 / dmx1[ip]) * dkPP2 * dedx2) + 0.25 * Uip / dmx1[ip];
+    const S eAMI = -invRhoCpCt_half / dx1[i] * (kPP1 / dmx1[i] - dx1[i] * (0.5 / dmx1[i]) * dkPP1 * dedx1) - 0.25 * Uc / dmx1[i];
     ac = eACI * dt + 1.0;
     ap = eAPI * dt;
     am = eAMI * dt;
@@ -271,60 +282,66 @@ struct ThermalX {
     Phys p;
     double dt;
     const double *T, *U;
+    static constexpr int NF = 2;  // field inputs, in the order below
+    template <class S, class Wr>
+    __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
+        thermal_x_row<S>(g, p, dt, w(0, T), w(1, U), i, j, k, a, b, c);
+    }
     __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
                                                double &d) const {
-        thermal_x_row(g, p, dt, T, U, i, j, k, a, b, c);
+        eval<double>(RawW(), i, j, k, a, b, c, d);
     }
 };
 
 // stage 3: y (:456-507); km,kp keep their stage-1 values n3m-1, n3m+1.
+template <class S, class FP>
 __device__ __forceinline__ void thermal_y_row(const Grid &g, const Phys &p, double dt,
-    const double *__restrict__ T, const double *__restrict__ V,
-    int i, int j, int k, double &am, double &ac, double &ap) {
+    FP T, FP V,
+    int i, int j, int k, S &am, S &ac, S &ap) {
     const int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1;
     const int km = g.n3m - 1, kp = g.n3m + 1;
     const int jem = g.jC[jm], jep = g.jC[jp];
     const double *dx1 = g.dx1, *dx2 = g.dx2, *dx3 = g.dx3, *dmx1 = g.dmx1, *dmx2 = g.dmx2, *dmx3 = g.dmx3;
 
-    const double Tc = T[g.ix(i, j, k)];
-    const double Tim = T[g.ix(im, j, k)], Tip = T[g.ix(ip, j, k)];
-    const double Tjm = T[g.ix(i, jm, k)], Tjp = T[g.ix(i, jp, k)];
-    const double Tkm = T[g.ix(i, j, km)], Tkp = T[g.ix(i, j, kp)];
-    double irc, dirc;
+    const S Tc = T[g.ix(i, j, k)];
+    const S Tim = T[g.ix(im, j, k)], Tip = T[g.ix(ip, j, k)];
+    const S Tjm = T[g.ix(i, jm, k)], Tjp = T[g.ix(i, jp, k)];
+    const S Tkm = T[g.ix(i, j, km)], Tkp = T[g.ix(i, j, kp)];
+    S irc, dirc;
     th_rhocp(p, Tc, irc, dirc);
-    double Kc, dKc, Kjm, dKjm, Kjp, dKjp;
+    S Kc, dKc, Kjm, dKjm, Kjp, dKjp;
     th_kpp(p, Tc, Kc, dKc);
     th_kpp(p, Tjm, Kjm, dKjm);
     th_kpp(p, Tjp, Kjp, dKjp);
-    const double Kim = th_kpp_only(p, Tim), Kip = th_kpp_only(p, Tip);
-    const double Kkm = th_kpp_only(p, Tkm), Kkp = th_kpp_only(p, Tkp);
+    const S Kim = th_kpp_only(p, Tim), Kip = th_kpp_only(p, Tip);
+    const S Kkm = th_kpp_only(p, Tkm), Kkp = th_kpp_only(p, Tkp);
 
-    const double invRhoCpCt_half = 0.5 * p.Ct * irc;
-    const double kPP1 = 0.5 / dmx1[i] * (dx1[i] * Kim + dx1[im] * Kc);
-    const double kPP2 = 0.5 / dmx1[ip] * (dx1[ip] * Kc + dx1[i] * Kip);
-    const double kPP3 = 0.5 / dmx2[j] * (dx2[j] * Kjm + dx2[jm] * Kc);
-    const double kPP4 = 0.5 / dmx2[jp] * (dx2[jp] * Kc + dx2[j] * Kjp);
-    const double kPP5 = 0.5 / dmx3[k] * (dx3[k] * Kkm + dx3[km] * Kc);
-    const double kPP6 = 0.5 / dmx3[kp] * (dx3[kp] * Kc + dx3[k] * Kkp);
-    const double dkPP3 = 0.5 / dmx2[j] * (dx2[j] * dKjm + dx2[jm] * dKc);
-    const double dkPP4 = 0.5 / dmx2[jp] * (dx2[jp] * dKc + dx2[j] * dKjp);
-    const double dedx1 = (Tc - Tim) / dmx1[i];
-    const double dedx2 = (Tip - Tc) / dmx1[ip];
-    const double dedy3 = (Tc - Tjm) / dmx2[j];
-    const double dedy4 = (Tjp - Tc) / dmx2[jp];
-    const double dedz5 = (Tc - Tkm) / dmx3[k];
-    const double dedz6 = (Tkp - Tc) / dmx3[kp];
-    const double Vc = V[g.ix(i, j, k)], Vjp = V[g.ix(i, jp, k)];
+    const S invRhoCpCt_half = 0.5 * p.Ct * irc;
+    const S kPP1 = 0.5 / dmx1[i] * (dx1[i] * Kim + dx1[im] * Kc);
+    const S kPP2 = 0.5 / dmx1[ip] * (dx1[ip] * Kc + dx1[i] * Kip);
+    const S kPP3 = 0.5 / dmx2[j] * (dx2[j] * Kjm + dx2[jm] * Kc);
+    const S kPP4 = 0.5 / dmx2[jp] * (dx2[jp] * Kc + dx2[j] * Kjp);
+    const S kPP5 = 0.5 / dmx3[k] * (dx3[k] * Kkm + dx3[km] * Kc);
+    const S kPP6 = 0.5 / dmx3[kp] * (dx3[kp] * Kc + dx3[k] * Kkp);
+    const S dkPP3 = 0.5 / dmx2[j] * (dx2[j] * dKjm + dx2[jm] * dKc);
+    const S dkPP4 = 0.5 / dmx2[jp] * (dx2[jp] * dKc + dx2[j] * dKjp);
+    const S dedx1 = (Tc - Tim) / dmx1[i];
+    const S dedx2 = (Tip - Tc) / dmx1[ip];
+    const S dedy3 = (Tc - Tjm) / dmx2[j];
+    const S dedy4 = (Tjp - Tc) / dmx2[jp];
+    const S dedz5 = (Tc - Tkm) / dmx3[k];
+    const S dedz6 = (Tkp - Tc) / dmx3[kp];
+    const S Vc = V[g.ix(i, j, k)], Vjp = V[g.ix(i, jp, k)];
 
-    const double eACJ = invRhoCpCt_half / dx2[j] * (kPP4 / dmx2[jp] + kPP3 / dmx2[j]
+    const S eACJ = invRhoCpCt_half / dx2[j] * (kPP4 / dmx2[jp] + kPP3 / dmx2[j]
                                                     - dx2[jp] * (0.5 / dmx2[jp]) * dkPP4 * dedy4 * (double)jep
                                                     + dx2[jm] * (0.5 / dmx2[j]) * dkPP3 * dedy3 * (double)jem)
                       + (0.25 * Vc / dmx2[j] - 0.25 * Vjp / dmx2[jp])
                       - 0.5 * invRhoCpCt_half * ((kPP2 * dedx2 - kPP1 * dedx1) / dx1[i]
                                                  + (kPP4 * dedy4 - kPP3 * dedy3) / dx2[j]
                                                  + (kPP6 * dedz6 - kPP5 * dedz5) / dx3[k]);
-    double eAPJ = -invRhoCpCt_half / dx2[j] * (kPP4 / dmx2[jp] + dx2[j] * (0.5 / dmx2[jp]) * dkPP4 * dedy4) + 0.25 * Vjp / dmx2[jp];
-    double eAMJ = -invRhoCpCt_half / dx2[j] * (kPP3 / dmx2[j] - dx2[j] * (0.5 / dmx2[j]) * dkPP3 * dedy3) - 0.25 * Vc / dmx2[j];
+    S eAPJ = -invRhoCpCt_half / dx2[j] * (kPP4 / dmx2[jp] + dx2[j] * (0.5 / dmx2[jp]) * dkPP4 * dedy4) + 0.25 * Vjp / dmx2[jp];
+    S eAMJ = -invRhoCpCt_half / dx2[j] * (kPP3 / dmx2[j] - dx2[j] * (0.5 / dmx2[j]) * dkPP3 * dedy3) - 0.25 * Vc / dmx2[j];
     eAPJ = eAPJ * (double)jep;
     eAMJ = eAMJ * (double)jem;
     ac = eACJ * dt + 1.0;
@@ -336,9 +353,14 @@ struct ThermalY {
     Phys p;
     double dt;
     const double *T, *V;
+    static constexpr int NF = 2;  // field inputs, in the order below
+    template <class S, class Wr>
+    __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
+        thermal_y_row<S>(g, p, dt, w(0, T), w(1, V), i, j, k, a, b, c);
+    }
     __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
                                                double &d) const {
-        thermal_y_row(g, p, dt, T, V, i, j, k, a, b, c);
+        eval<double>(RawW(), i, j, k, a, b, c, d);
     }
 };
 
