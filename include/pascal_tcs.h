@@ -165,6 +165,26 @@ int tcs_step(tcs_ctx *ctx, double dt);
    (src/module_solve_thermal.f90:102-110) and is not used here. */
 int tcs_nusselt(tcs_ctx *ctx, double Tup, double Tbt, double *nu_bottom, double *nu_top);
 
+/* ---- adjoint of the step (no counterpart in the reference) -------------- */
+/* The discrete adjoint of tcs_step in the plain FP64 form (precision 0),
+   for gradients of scalar functions of a rollout.  The context keeps
+   adjoint fields for the step's state (T, U, V, W, P, dPhat, dP0; full
+   (0:n1sub,0:n2sub,0:n3sub) layout) and for the wall arrays.
+   tcs_adjoint_zero clears them.  tcs_adjoint_step(ctx, dt) takes the
+   context's fields as the state before a step and the adjoint fields as
+   the gradient with respect to the state after it; it runs the step (the
+   fields end as after tcs_step) and leaves the gradient with respect to the
+   state before it, adding the wall arrays' share to their adjoints.
+   tcs_adjoint_copy moves an adjoint array from (to_adjoint = 1) or to
+   (to_adjoint = 0) a device buffer of its size: kind 0 with a state field
+   id, kind 1 with a wall id.  tcs_adjoint_add_nusselt adds weight times
+   the gradient of tcs_nusselt's bottom value with respect to T and TBCbt.
+   Returns TCS_E_ARG for precision != 0. */
+int tcs_adjoint_zero(tcs_ctx *ctx);
+int tcs_adjoint_step(tcs_ctx *ctx, double dt);
+int tcs_adjoint_copy(tcs_ctx *ctx, int32_t kind, int32_t id, int32_t to_adjoint, double *dev);
+int tcs_adjoint_add_nusselt(tcs_ctx *ctx, double weight, double Tup, double Tbt);
+
 /* ---- timers ----------------------------------------------------------- */
 /* HIP-event timing per stage and per kernel on the context stream, the
    device counterpart of the reference's labelled timers

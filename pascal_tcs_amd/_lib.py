@@ -74,6 +74,10 @@ SIGNATURES = {
     "tcs_profile": (ctypes.c_int, [_vp, _i32]),
     "tcs_profile_count": (ctypes.c_int, [_vp, ctypes.POINTER(_i32)]),
     "tcs_profile_entry": (ctypes.c_int, [_vp, _i32, ctypes.c_char_p, _i32, _dp, ctypes.POINTER(ctypes.c_int64)]),
+    "tcs_adjoint_zero": (ctypes.c_int, [_vp]),
+    "tcs_adjoint_step": (ctypes.c_int, [_vp, _f64]),
+    "tcs_adjoint_copy": (ctypes.c_int, [_vp, _i32, _i32, _i32, _vp]),
+    "tcs_adjoint_add_nusselt": (ctypes.c_int, [_vp, _f64, _f64, _f64]),
     "ptdma_many_solve_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _i32, _vp]),
     "ptdma_many_solve_cycle_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _i32, _vp]),
     "ptdma_many_solve_host": (ctypes.c_int, [_dp, _dp, _dp, _dp, _i32, _i32, _i32]),

@@ -17,7 +17,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpascal_tcs.so")
-SOURCES = ["runtime.hip", "tdma.hip", "tdma_part.hip", "thermal.hip", "momentum.hip", "pressure.hip"]
+SOURCES = ["runtime.hip", "tdma.hip", "tdma_part.hip", "thermal.hip", "momentum.hip", "pressure.hip", "adjoint.hip"]
 # stage translation units built again as the `fast` and `mixed` forms
 FORM_SOURCES = ["thermal.hip", "momentum.hip", "pressure.hip"]
 ARCH = os.environ.get("PASCAL_TCS_ARCH", "gfx950")

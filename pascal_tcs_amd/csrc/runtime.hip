@@ -341,6 +341,7 @@ static int dalloc(double **p, size_t n) {
 
 static void free_ctx(tcs_ctx *c) {
     if (!c) return;
+    adjoint_free(c);
     double **bufs[] = {&c->dx1, &c->dx2, &c->dx3, &c->dmx1, &c->dmx2, &c->dmx3, &c->am, &c->ac, &c->ap, &c->rhs,
                        &c->stale, &c->prhs, &c->spec_x, &c->spec_y, &c->pA, &c->pC, &c->dxk2, &c->dzk, &c->ptab,
                        &c->tw1, &c->tw3, &c->linesum, &c->planesum, &c->scal, &c->red};

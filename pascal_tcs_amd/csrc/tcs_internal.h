@@ -19,6 +19,7 @@ void set_error(const std::string &msg);
 // PaScaL_TDMA/src/timer.f90).  Active only while a context has it enabled;
 // every launch in this library goes through TCS_LAUNCH.
 struct Profiler;
+struct AdjWork;  // adjoint fields and the step's kept values (adjoint.hip)
 void prof_begin(const char *label, hipStream_t s);
 void prof_end(hipStream_t s);
 
@@ -121,6 +122,7 @@ struct tcs_ctx {
     double *red;             // reduction scratch
     double *h_pinned;        // pinned host scalars
     tcs::Profiler *prof;     // null unless profiling is enabled
+    tcs::AdjWork *adj;       // null until an adjoint entry is called
 
     tcs::Grid grid() const {
         tcs::Grid g;
@@ -206,5 +208,6 @@ int halo(tcs_ctx *c, double *f);
 int post_div(tcs_ctx *c, double *out);
 int post_cfl(tcs_ctx *c, double *maxuovx);
 int nusselt(tcs_ctx *c, double Tup, double Tbt, double *nb, double *nt);
+void adjoint_free(tcs_ctx *c);
 
 }  // namespace tcs
