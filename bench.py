@@ -281,6 +281,62 @@ def run_mixed_drift(n, steps, deltat, dt, report=(10, 100, 250, 500, 750, 1000))
             "errors": rows}
 
 
+def run_adjoint_case(n, nsteps, every, ndir, deltat):
+    """Gradient of the time-averaged bottom-wall Nusselt number over an
+    nsteps rollout with respect to an n x n bottom-wall temperature
+    perturbation (seeded Gaussian, sigma 0.01): 2 pi x 2 x 2 pi box, uniform
+    grid, Ra 1e7, water properties, conduction profile plus noise 0.05,
+    dt 1e-3, plain FP64 step (precision 0, split lines), checkpoints every
+    `every` steps.  Checked against central differences on ndir random
+    directions, and against the gradient of the single-rank-order step."""
+    import torch
+    from pascal_tcs_amd.adjoint import STATE as ST, NusseltRollout, nusselt_loss_function
+    from pascal_tcs_amd.config import Properties, TCSConfig
+    from pascal_tcs_amd.solver import TCSSolver
+    cfg = TCSConfig(n1m=n, n2m=n, n3m=n, uniform2=1, Ra=1e7, Pr=0.71, DeltaT=deltat, props=Properties.water(),
+                    dtStart=1e-3)
+    out = {"metric": f"adjoint gradient of the mean bottom-wall Nusselt number, {n}^3, {nsteps} steps",
+           "config": {"workload": f"{n}^3 uniform box, Ra 1e7, water properties DeltaT {deltat:g}, noise 0.05, "
+                                  f"dt 1e-3, {nsteps} steps, checkpoints every {every}",
+                      "parameter": f"{n}x{n} bottom-wall temperature perturbation, Gaussian sigma 0.01"}}
+    grads = {}
+    for split in (1, 0):
+        s = TCSSolver(cfg, tdma_split=split, precision=0)
+        s.setup_seeded(amp=0.05, seed=20240512)
+        state0 = {k: s.get(k) for k in ST}
+        rng = np.random.default_rng(4)
+        theta0 = 0.01 * rng.standard_normal((n, n))
+        roll = NusseltRollout(s, state0, nsteps=nsteps, dt=cfg.dtStart, every=every,
+                              log=lambda m: print(m, file=sys.stderr, flush=True))
+        F = nusselt_loss_function()
+        th = torch.tensor(theta0, dtype=torch.float64, device="cuda", requires_grad=True)
+        t0 = time.perf_counter()
+        L = F.apply(th, roll)
+        t1 = time.perf_counter()
+        L.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        grads[split] = th.grad.detach().cpu().numpy()
+        if split == 1:
+            out["loss"] = float(L)
+            out["forward_s"] = round(t1 - t0, 3)
+            out["backward_s"] = round(t2 - t1, 3)
+            out["grad_norm"] = float(np.linalg.norm(grads[1]))
+            checks = []
+            eps = 1e-5
+            for r in range(ndir):
+                d = rng.standard_normal(theta0.shape)
+                fd = (roll.loss(theta0 + eps * d)[0] - roll.loss(theta0 - eps * d)[0]) / (2 * eps)
+                ad = float(np.vdot(grads[1], d))
+                checks.append({"fd": fd, "adjoint": ad, "rel": abs(fd - ad) / abs(fd)})
+                print(f"adjoint direction {r}: fd {fd:.10e} adjoint {ad:.10e}", file=sys.stderr, flush=True)
+            out["fd_checks"] = checks
+            out["fd_max_rel"] = max(c["rel"] for c in checks)
+        s.close()
+    out["vs_single_rank_order_gradient_rel"] = float(np.linalg.norm(grads[1] - grads[0]) / np.linalg.norm(grads[0]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -310,6 +366,9 @@ def main():
     ap.add_argument("--mixed-drift", type=int, default=0, metavar="STEPS",
                     help="instead of the step: run the mixed and plain forms side by side for STEPS steps "
                          "and report the mixed form's per-field errors and Nusselt drift")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="instead of the step: the adjoint gradient case (n^3, default 128, 50 steps) checked "
+                         "against central differences")
     args = ap.parse_args()
     if args.precision == 2:
         args.mixed = False
@@ -320,6 +379,10 @@ def main():
 
     if args.tdma:
         print(json.dumps(run_tdma_microbench(args.n)))
+        return
+    if args.adjoint:
+        n = args.n if args.n != 512 else 128
+        print(json.dumps(run_adjoint_case(n, 50, 10, 16, args.deltat)))
         return
     if args.mixed_drift > 0:
         print(json.dumps(run_mixed_drift(args.n, args.mixed_drift, args.deltat, args.dt)))

@@ -110,8 +110,9 @@ class NusseltRollout:
     with checkpoints every `every` steps (the state is kept at those steps
     only; the steps between are run again for the backward pass)."""
 
-    def __init__(self, solver, state0, nsteps, dt, every=10):
+    def __init__(self, solver, state0, nsteps, dt, every=10, log=None):
         self.s = solver
+        self.log = log
         self.state0 = {k: np.array(v) for k, v in state0.items()}
         self.n = nsteps
         self.dt = dt
@@ -159,6 +160,8 @@ class NusseltRollout:
                 a.add_nusselt(w)
                 self._load(states[t - 1 - seg_start], theta)
                 a.step(self.dt)
+            if self.log:
+                self.log(f"adjoint: steps {seg_start + 1}..{seg_end} done")
         # theta enters TBCbt and the bottom ghost row of T (all periodic images)
         gw = a.get_wall_bar("TBCbt") + a.get_bar("T")[:, 0, :]
         return _fold_wall(gw, cfg.n1m, cfg.n3m)

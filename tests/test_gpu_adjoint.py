@@ -119,3 +119,34 @@ def test_step_adjoint_field_matrix():
                 bad.append((y, x, fd, ad))
     s.close()
     assert not bad, bad
+
+
+def test_nusselt_rollout_gradient():
+    """The gradient of the time-averaged bottom-wall Nusselt number over a
+    short rollout with respect to the bottom-wall temperature, through the
+    torch.autograd.Function with checkpoints, against central differences of
+    the rollout on random directions (1e-5 relative)."""
+    import torch
+    from pascal_tcs_amd.adjoint import STATE as ST, NusseltRollout, nusselt_loss_function
+    from pascal_tcs_amd.solver import TCSSolver
+    cfg = small_config(16, 12, 8, props="water", stretched=False, Ra=1e7, DeltaT=0.2)
+    s = TCSSolver(cfg, tdma_split=1, precision=0)
+    s.setup_seeded(amp=0.05, seed=3)
+    state0 = {n: s.get(n) for n in ST}
+    rng = np.random.default_rng(17)
+    theta0 = 0.01 * rng.standard_normal((cfg.n3m, cfg.n1m))
+    roll = NusseltRollout(s, state0, nsteps=8, dt=cfg.dtStart, every=3)
+    F = nusselt_loss_function()
+    th = torch.tensor(theta0, dtype=torch.float64, device="cuda", requires_grad=True)
+    L = F.apply(th, roll)
+    L.backward()
+    g = th.grad.detach().cpu().numpy()
+    assert float(L.detach()) == roll.loss(theta0)[0]
+    eps = 1e-5
+    for r in range(4):
+        d = rng.standard_normal(theta0.shape)
+        fd = (roll.loss(theta0 + eps * d)[0] - roll.loss(theta0 - eps * d)[0]) / (2 * eps)
+        ad = float(np.vdot(g, d))
+        print(f"direction {r}: fd {fd:.10e} adjoint {ad:.10e}", flush=True)
+        assert abs(fd - ad) <= 1e-5 * abs(fd), (fd, ad)
+    s.close()
