@@ -75,6 +75,7 @@ def kernel_bytes(label, cfg):
         "blockLdV": 8.0 * N * (3 + 1) * fv, "blockLdU": 8.0 * N * (4 + 1),
         "pseudoupdate": 8.0 * N * 9, "projection": 8.0 * N * (6 + 4), "dphat_update": 8.0 * N * 5,
         "pressure_rhs": 8.0 * N * (4 + 1), "add_interior": 8.0 * N * 3, "store_interior": 8.0 * N * 2,
+        "post_div_cfl": 8.0 * N * 3,
         # split line solves that compute their rows: the stencil's fields in,
         # the solution out (z: into the scratch; y: the scratch's right-hand
         # side in, the increment field out)

@@ -36,8 +36,8 @@ def flags(form="exact"):
     # the fast arithmetic with FP32 ADI scratch
     fastm = ["-ffp-contract=fast", "-ffast-math", "-fno-finite-math-only"]
     arith = {"exact": ["-ffp-contract=off", "-fno-fast-math"],
-             "fast": fastm + ["-DTCS_FORM=fast"],
-             "mixed": fastm + ["-DTCS_FORM=mixed", "-DTCS_SCRATCH=float"]}[form]
+             "fast": fastm + ["-DTCS_FORM=fast", "-DTCS_RECIP"],
+             "mixed": fastm + ["-DTCS_FORM=mixed", "-DTCS_SCRATCH=float", "-DTCS_RECIP"]}[form]
     return [
         "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
         *arith,

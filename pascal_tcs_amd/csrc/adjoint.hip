@@ -744,6 +744,7 @@ extern "C" int tcs_adjoint_step(tcs_ctx *c, double dt) {
         return TCS_E_ARG;
     }
     TCS_TRY(adjoint_alloc(c));
+    c->cfl_valid = 0;
     TCS_TRY(adj::forward_keep(c, c->adj, dt));
     TCS_TRY(adj::backward(c, c->adj, dt));
     int flag = 0;

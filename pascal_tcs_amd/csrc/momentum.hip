@@ -18,6 +18,8 @@
 namespace tcs {
 namespace TCS_FORM {
 
+#include "metric.h"
+
 #ifndef TCS_SCRATCH
 #define TCS_SCRATCH double
 #endif
@@ -102,9 +104,9 @@ __global__ void k_momentum_stale(Grid g, const double *__restrict__ Mu, double *
     (void)dx1; (void)dx2; (void)dx3; (void)dmx1; (void)dmx2; (void)dmx3;
 
 // the metric arrays under their reference names, in a row routine
-#define ROW_METRICS                                                           \
-    const double *dx1 = g.dx1, *dx2 = g.dx2, *dx3 = g.dx3;                    \
-    const double *dmx1 = g.dmx1, *dmx2 = g.dmx2, *dmx3 = g.dmx3;              \
+#define ROW_METRICS                                                                                   \
+    const auto dx1 = metric(g.dx1, g.rdx1), dx2 = metric(g.dx2, g.rdx2), dx3 = metric(g.dx3, g.rdx3);     \
+    const auto dmx1 = metric(g.dmx1, g.rdmx1), dmx2 = metric(g.dmx2, g.rdmx2), dmx3 = metric(g.dmx3, g.rdmx3); \
     (void)dx1; (void)dx2; (void)dx3; (void)dmx1; (void)dmx2; (void)dmx3;
 
 // Each stage's row routine gives row (i, j, k) of the stage's tridiagonal

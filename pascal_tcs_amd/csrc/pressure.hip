@@ -28,6 +28,8 @@
 namespace tcs {
 namespace TCS_FORM {
 
+#include "metric.h"
+
 constexpr int FFT_THREADS = 256;
 constexpr int FFT_MAXB = 8;  // butterflies per thread per stage
 
@@ -128,7 +130,8 @@ template <class S, class FP>
 __device__ __forceinline__ void pressure_rhs_cell(const Grid &g, const Phys &p, double dt, FP U, FP V, FP W,
                                                  FP dPhat, FP iR, FP VBCup, FP VBCbt, int i, int j, int k,
                                                  S &out) {
-    const double *dx1 = g.dx1, *dx2 = g.dx2, *dx3 = g.dx3, *dmx1 = g.dmx1, *dmx2 = g.dmx2, *dmx3 = g.dmx3;
+    const auto dx1 = metric(g.dx1, g.rdx1), dx2 = metric(g.dx2, g.rdx2), dx3 = metric(g.dx3, g.rdx3);
+    const auto dmx1 = metric(g.dmx1, g.rdmx1), dmx2 = metric(g.dmx2, g.rdmx2), dmx3 = metric(g.dmx3, g.rdmx3);
     const int ip = i + 1, im = i - 1, jp = j + 1, jm = j - 1, kp = k + 1, km = k - 1;
     const int jvm = g.jC[jm], jvp = g.jC[jp];
 #define IRt(jj, kk) iR[g.t2((jj), (kk))]
@@ -480,7 +483,8 @@ template <class S, class FP>
 __device__ __forceinline__ void projection_cell(const Grid &g, const Phys &p, double dt, FP iR, FP dP,
                                                FP dPhat, FP dP0, FP U, FP V, FP W, FP P, int i, int j, int k,
                                                S &uo, S &vo, S &wo, S &po) {
-    const double *dx1 = g.dx1, *dx2 = g.dx2, *dx3 = g.dx3, *dmx1 = g.dmx1, *dmx2 = g.dmx2, *dmx3 = g.dmx3;
+    const auto dx1 = metric(g.dx1, g.rdx1), dx2 = metric(g.dx2, g.rdx2), dx3 = metric(g.dx3, g.rdx3);
+    const auto dmx1 = metric(g.dmx1, g.rdmx1), dmx2 = metric(g.dmx2, g.rdmx2), dmx3 = metric(g.dmx3, g.rdmx3);
     const int im = i - 1, jm = j - 1, km = k - 1;
     const S Cmp = p.Cmp;
     const size_t q = g.ix(i, j, k);

@@ -9,6 +9,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "tcs_internal.h"
@@ -172,6 +173,41 @@ k_post_max(Grid g, const double *__restrict__ U, const double *__restrict__ V, c
     if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
 }
 
+// both maxima in one pass over U, V, W: partial[b] (divergence) and
+// partial[RED_BLOCKS + b] (sum |u|/dx)
+__global__ void __launch_bounds__(256)
+k_post_both(Grid g, const double *__restrict__ U, const double *__restrict__ V, const double *__restrict__ W,
+            double *__restrict__ partial) {
+    __shared__ double s0[256], s1[256];
+    const int nrows = g.n2m * g.n3m;
+    double m0 = -INFINITY, m1 = -INFINITY;
+    for (int r = blockIdx.x; r < nrows; r += gridDim.x) {
+        const int j = r % g.n2m + 1, k = r / g.n2m + 1;
+        for (int i = threadIdx.x + 1; i <= g.n1m; i += blockDim.x) {
+            const double u0 = U[g.ix(i, j, k)], u1 = U[g.ix(i + 1, j, k)];
+            const double v0 = V[g.ix(i, j, k)], v1 = V[g.ix(i, j + 1, k)];
+            const double w0 = W[g.ix(i, j, k)], w1 = W[g.ix(i, j, k + 1)];
+            m0 = fmax(m0, (u1 - u0) / g.dx1[i] + (v1 - v0) / g.dx2[j] + (w1 - w0) / g.dx3[k]);
+            m1 = fmax(m1, (fabs(u1 + u0) / 2.0) / g.dx1[i] + (fabs(v1 + v0) / 2.0) / g.dx2[j]
+                              + (fabs(w1 + w0) / 2.0) / g.dx3[k]);
+        }
+    }
+    s0[threadIdx.x] = m0;
+    s1[threadIdx.x] = m1;
+    __syncthreads();
+    for (int h = 128; h >= 1; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            s0[threadIdx.x] = fmax(s0[threadIdx.x], s0[threadIdx.x + h]);
+            s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + h]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = s0[0];
+        partial[RED_BLOCKS + blockIdx.x] = s1[0];
+    }
+}
+
 __global__ void k_final_max(int n, const double *__restrict__ partial, double *__restrict__ out) {
     __shared__ double s[256];
     double m = -INFINITY;
@@ -196,8 +232,30 @@ static int post_max(tcs_ctx *c, double *out) {
     return TCS_OK;
 }
 
-int post_div(tcs_ctx *c, double *out) { return post_max<0>(c, out); }
-int post_cfl(tcs_ctx *c, double *out) { return post_max<1>(c, out); }
+// mpi_Post_Div's pass also gives mpi_Post_CFL's maximum (the same U, V, W
+// reads), kept until an entry that may change the fields runs
+int post_div(tcs_ctx *c, double *out) {
+    TCS_LAUNCH(c->stream, "post_div_cfl", k_post_both, dim3(RED_BLOCKS), dim3(256), 0, c->grid(), c->f[TCS_U],
+               c->f[TCS_V], c->f[TCS_W], c->red);
+    TCS_LAUNCH(c->stream, "final_max", k_final_max, dim3(1), dim3(256), 0, RED_BLOCKS, c->red,
+               c->red + 2 * RED_BLOCKS);
+    TCS_LAUNCH(c->stream, "final_max", k_final_max, dim3(1), dim3(256), 0, RED_BLOCKS, c->red + RED_BLOCKS,
+               c->red + 2 * RED_BLOCKS + 1);
+    TCS_HIP(hipMemcpyAsync(c->h_pinned, c->red + 2 * RED_BLOCKS, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+    TCS_HIP(hipStreamSynchronize(c->stream));
+    *out = c->h_pinned[0];
+    c->cfl_max = c->h_pinned[1];
+    c->cfl_valid = 1;
+    return TCS_OK;
+}
+int post_cfl(tcs_ctx *c, double *out) {
+    if (c->cfl_valid) {
+        *out = c->cfl_max;
+        return TCS_OK;
+    }
+    return post_max<1>(c, out);
+}
 
 // wall Nusselt numbers from the wall rows the solver sees (T(:,0,:) and
 // T(:,n2sub,:)): pairwise sums over i per k, then over k
@@ -342,7 +400,7 @@ static int dalloc(double **p, size_t n) {
 static void free_ctx(tcs_ctx *c) {
     if (!c) return;
     adjoint_free(c);
-    double **bufs[] = {&c->dx1, &c->dx2, &c->dx3, &c->dmx1, &c->dmx2, &c->dmx3, &c->am, &c->ac, &c->ap, &c->rhs,
+    double **bufs[] = {&c->dx1, &c->dx2, &c->dx3, &c->dmx1, &c->dmx2, &c->dmx3, &c->rmet, &c->am, &c->ac, &c->ap, &c->rhs,
                        &c->stale, &c->prhs, &c->spec_x, &c->spec_y, &c->pA, &c->pC, &c->dxk2, &c->dzk, &c->ptab,
                        &c->tw1, &c->tw3, &c->linesum, &c->planesum, &c->scal, &c->red};
     for (double **b : bufs)
@@ -476,6 +534,7 @@ extern "C" int tcs_init(const tcs_params *prm, const double *dx1, const double *
     struct { double **p; size_t n; } allocs[] = {
         {&c->dx1, (size_t)c->N1}, {&c->dmx1, (size_t)c->N1}, {&c->dx2, (size_t)c->N2}, {&c->dmx2, (size_t)c->N2},
         {&c->dx3, (size_t)c->N3}, {&c->dmx3, (size_t)c->N3},
+        {&c->rmet, (size_t)2 * (c->N1 + c->N2 + c->N3)},
         {&c->am, c->nint}, {&c->ac, c->nint}, {&c->ap, c->nint}, {&c->rhs, c->nint},
         {&c->stale, 16}, {&c->prhs, c->nint},
         {&c->spec_x, (size_t)2 * c->h1p * c->n2m * c->n3m},
@@ -483,7 +542,7 @@ extern "C" int tcs_init(const tcs_params *prm, const double *dx1, const double *
         {&c->dxk2, (size_t)c->h1p}, {&c->dzk, (size_t)c->n3m}, {&c->ptab, (size_t)3 * c->n2m},
         {&c->tw1, (size_t)c->n1m}, {&c->tw3, (size_t)c->n3m},
         {&c->linesum, (size_t)c->n2m * c->n3m}, {&c->planesum, (size_t)c->n3m}, {&c->scal, 8},
-        {&c->red, (size_t)RED_BLOCKS + 16 + (size_t)4 * c->n3m},
+        {&c->red, (size_t)2 * RED_BLOCKS + 16 + (size_t)4 * c->n3m},
     };
     for (auto &a : allocs)
         if ((rc = dalloc(a.p, a.n)) != TCS_OK) return fail(rc);
@@ -517,6 +576,17 @@ extern "C" int tcs_init(const tcs_params *prm, const double *dx1, const double *
         set_error("tcs_init: metric upload failed");
         return fail(TCS_E_HIP);
     }
+    {
+        // reciprocal metric tables of the fast forms (x / dx as x * (1/dx))
+        std::vector<double> r;
+        for (const auto &m : {std::make_pair(dx1, c->N1), std::make_pair(dmx1, c->N1), std::make_pair(dx2, c->N2),
+                              std::make_pair(dmx2, c->N2), std::make_pair(dx3, c->N3), std::make_pair(dmx3, c->N3)})
+            for (int q = 0; q < m.second; q++) r.push_back(1.0 / m.first[q]);
+        if (hipMemcpy(c->rmet, r.data(), r.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("tcs_init: metric upload failed");
+            return fail(TCS_E_HIP);
+        }
+    }
     if ((rc = upload_tables(c)) != TCS_OK) return fail(rc);
     if (hipDeviceSynchronize() != hipSuccess) {
         set_error("tcs_init: device error");
@@ -546,6 +616,7 @@ static int check_field(tcs_ctx *c, int id) {
         set_error("bad context or field id");
         return TCS_E_ARG;
     }
+    c->cfl_valid = 0;  // the field may change (upload, halo, a handed-out pointer)
     return TCS_OK;
 }
 
@@ -626,6 +697,7 @@ extern "C" int tcs_thermal_boundary(tcs_ctx *c, double Tup, double Tbt) {
 extern "C" int tcs_momentum_boundary(tcs_ctx *c, double Uup, double Ubt, double Vup, double Vbt, double Wup,
                                      double Wbt) {
     if (!c) return TCS_E_ARG;
+    c->cfl_valid = 0;
     TCS_TRY(fill_wall(c, TCS_UBCUP, Uup));
     TCS_TRY(fill_wall(c, TCS_VBCUP, Vup));
     TCS_TRY(fill_wall(c, TCS_WBCUP, Wup));
@@ -639,8 +711,12 @@ extern "C" int tcs_momentum_boundary(tcs_ctx *c, double Uup, double Ubt, double 
     return TCS_OK;
 }
 
-#define CTX_GUARD(c) \
+#define CTX_CHECK(c) \
     if (!(c)) { set_error("null context"); return TCS_E_ARG; }
+// entries that may change fields drop the kept CFL maximum
+#define CTX_GUARD(c) \
+    CTX_CHECK(c) \
+    (c)->cfl_valid = 0;
 
 #define STAGE(label) ProfStage _stage(c, label)
 
@@ -673,14 +749,14 @@ extern "C" int tcs_ghostcell_update(tcs_ctx *c, int32_t id) {
 }
 
 extern "C" int tcs_post_div(tcs_ctx *c, double *maxdiv) {
-    CTX_GUARD(c);
+    CTX_CHECK(c);
     if (!maxdiv) return TCS_E_ARG;
     STAGE("post div");
     return post_div(c, maxdiv);
 }
 
 extern "C" int tcs_post_cfl(tcs_ctx *c, double *newcfl, double *newdt) {
-    CTX_GUARD(c);
+    CTX_CHECK(c);
     if (!newcfl || !newdt) return TCS_E_ARG;
     double mx = 0.0;
     {
@@ -696,7 +772,7 @@ extern "C" int tcs_post_cfl(tcs_ctx *c, double *newcfl, double *newdt) {
 }
 
 extern "C" int tcs_nusselt(tcs_ctx *c, double Tup, double Tbt, double *nb, double *nt) {
-    CTX_GUARD(c);
+    CTX_CHECK(c);
     if (!nb || !nt) return TCS_E_ARG;
     return nusselt(c, Tup, Tbt, nb, nt);
 }

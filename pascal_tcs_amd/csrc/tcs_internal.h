@@ -62,6 +62,7 @@ struct Grid {
     int n1m, n2m, n3m;
     int N1, N2, N3;  // full extents n?m+2
     const double *dx1, *dx2, *dx3, *dmx1, *dmx2, *dmx3;
+    const double *rdx1, *rdx2, *rdx3, *rdmx1, *rdmx2, *rdmx3;  // their reciprocals
     const int *jC, *jS;
     __device__ __forceinline__ size_t ix(int i, int j, int k) const {
         return (size_t)i + (size_t)N1 * ((size_t)j + (size_t)N2 * (size_t)k);
@@ -100,6 +101,7 @@ struct tcs_ctx {
     double *h_dx1, *h_dx2, *h_dx3, *h_dmx1, *h_dmx2, *h_dmx3;
     // device metrics / masks
     double *dx1, *dx2, *dx3, *dmx1, *dmx2, *dmx3;
+    double *rmet;            // 1/dx1, 1/dmx1, 1/dx2, 1/dmx2, 1/dx3, 1/dmx3 back to back
     int *jC, *jS;
 
     double *f[TCS_NFIELD];  // INVRHO / MU hold (0:n2sub,0:n3sub) tables
@@ -123,12 +125,16 @@ struct tcs_ctx {
     double *h_pinned;        // pinned host scalars
     tcs::Profiler *prof;     // null unless profiling is enabled
     tcs::AdjWork *adj;       // null until an adjoint entry is called
+    int cfl_valid;           // cfl_max holds mpi_Post_CFL's maximum for the current fields
+    double cfl_max;
 
     tcs::Grid grid() const {
         tcs::Grid g;
         g.n1m = n1m; g.n2m = n2m; g.n3m = n3m;
         g.N1 = N1; g.N2 = N2; g.N3 = N3;
         g.dx1 = dx1; g.dx2 = dx2; g.dx3 = dx3;
+        g.rdx1 = rmet; g.rdmx1 = rmet + N1; g.rdx2 = rmet + 2 * N1; g.rdmx2 = g.rdx2 + N2;
+        g.rdx3 = g.rdx2 + 2 * N2; g.rdmx3 = g.rdx3 + N3;
         g.dmx1 = dmx1; g.dmx2 = dmx2; g.dmx3 = dmx3;
         g.jC = jC; g.jS = jS;
         return g;

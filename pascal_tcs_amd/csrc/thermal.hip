@@ -18,6 +18,8 @@
 namespace tcs {
 namespace TCS_FORM {
 
+#include "metric.h"
+
 #ifndef TCS_SCRATCH
 #define TCS_SCRATCH double
 #endif
@@ -91,7 +93,8 @@ __device__ __forceinline__ void thermal_z_row(const Grid &g, const Phys &p, doub
     int i, int j, int k, S &am, S &ac, S &ap, S &rhsout) {
     const int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1, km = k - 1, kp = k + 1;
     const int jem = g.jC[jm], jep = g.jC[jp];
-    const double *dx1 = g.dx1, *dx2 = g.dx2, *dx3 = g.dx3, *dmx1 = g.dmx1, *dmx2 = g.dmx2, *dmx3 = g.dmx3;
+    const auto dx1 = metric(g.dx1, g.rdx1), dx2 = metric(g.dx2, g.rdx2), dx3 = metric(g.dx3, g.rdx3);
+    const auto dmx1 = metric(g.dmx1, g.rdmx1), dmx2 = metric(g.dmx2, g.rdmx2), dmx3 = metric(g.dmx3, g.rdmx3);
     const S Ct = p.Ct;
 
     const S Tc = T[g.ix(i, j, k)];
@@ -232,7 +235,8 @@ __device__ __forceinline__ void thermal_x_row(const Grid &g, const Phys &p, doub
     int i, int j, int k, S &am, S &ac, S &ap) {
     const int im = i - 1, ip = i + 1;
     const int jm = g.n2m - 1, jp = g.n2m + 1, km = g.n3m - 1, kp = g.n3m + 1;
-    const double *dx1 = g.dx1, *dx2 = g.dx2, *dx3 = g.dx3, *dmx1 = g.dmx1, *dmx2 = g.dmx2, *dmx3 = g.dmx3;
+    const auto dx1 = metric(g.dx1, g.rdx1), dx2 = metric(g.dx2, g.rdx2), dx3 = metric(g.dx3, g.rdx3);
+    const auto dmx1 = metric(g.dmx1, g.rdmx1), dmx2 = metric(g.dmx2, g.rdmx2), dmx3 = metric(g.dmx3, g.rdmx3);
 
     const S Tc = T[g.ix(i, j, k)];
     const S Tim = T[g.ix(im, j, k)], Tip = T[g.ix(ip, j, k)];
@@ -301,7 +305,8 @@ __device__ __forceinline__ void thermal_y_row(const Grid &g, const Phys &p, doub
     const int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1;
     const int km = g.n3m - 1, kp = g.n3m + 1;
     const int jem = g.jC[jm], jep = g.jC[jp];
-    const double *dx1 = g.dx1, *dx2 = g.dx2, *dx3 = g.dx3, *dmx1 = g.dmx1, *dmx2 = g.dmx2, *dmx3 = g.dmx3;
+    const auto dx1 = metric(g.dx1, g.rdx1), dx2 = metric(g.dx2, g.rdx2), dx3 = metric(g.dx3, g.rdx3);
+    const auto dmx1 = metric(g.dmx1, g.rdmx1), dmx2 = metric(g.dmx2, g.rdmx2), dmx3 = metric(g.dmx3, g.rdmx3);
 
     const S Tc = T[g.ix(i, j, k)];
     const S Tim = T[g.ix(im, j, k)], Tip = T[g.ix(ip, j, k)];
