@@ -40,7 +40,9 @@ extern "C" {
 typedef struct tcs_params {
     int32_t n1m, n2m, n3m;        /* interior cells (namelist meshes) */
     int32_t pbc1, pbc2, pbc3;     /* periodicity; only (1,0,1) is supported */
-    int32_t precision;            /* 0 = plain FP64 (the only form built) */
+    int32_t precision;            /* 0: FP64 in the reference's operation order;
+                                     1: FP64 with contraction and reciprocals;
+                                     2: mixed, FP32 ADI scratch and line solves */
     int32_t tdma_split;           /* 0: every ADI line solved as on one rank;
                                      1: lines of up to 512 rows split into
                                      parts as PaScaL_TDMA splits them over
@@ -105,7 +107,9 @@ void *tcs_stream(tcs_ctx *ctx);
 /* ---- module arrays ----------------------------------------------------- */
 int tcs_field_upload(tcs_ctx *ctx, int32_t id, const double *host);
 int tcs_field_download(tcs_ctx *ctx, int32_t id, double *host);
-/* device address of a field (full layout); INVRHO/MU return the 2-D table */
+/* device address of a field (full layout); INVRHO/MU return the 2-D table.
+   Valid until the next tcs_step, which writes the updates of T, U, V, W,
+   dPhat and dP0 into second buffers and exchanges them. */
 int tcs_field_device_ptr(tcs_ctx *ctx, int32_t id, double **dptr);
 int tcs_wall_upload(tcs_ctx *ctx, int32_t id, const double *host);
 int tcs_wall_download(tcs_ctx *ctx, int32_t id, double *host);
@@ -156,7 +160,11 @@ int tcs_post_div(tcs_ctx *ctx, double *maxdiv);
 /* mpi_Post_CFL(U,V,W,newCFL,newdt), src/module_post.f90:168-230 */
 int tcs_post_cfl(tcs_ctx *ctx, double *newcfl, double *newdt);
 /* The loop body of src/main.f90:131-203 (thermal, momentum, pressure,
-   every halo update in main's order), fully on the device. */
+   every halo update in main's order), fully on the device.  The same
+   results as the routines called one by one; the last thermal sweep, the
+   block corrections (with the pseudo-update) and the projection (with the
+   dPhat/dP0 history) write their updates into second buffers of the fields,
+   so the field device addresses change (tcs_field_device_ptr). */
 int tcs_step(tcs_ctx *ctx, double dt);
 /* Wall Nusselt numbers (no counterpart in the reference): mean over (i,k)
    of (TBCbt(i,k) - T(i,1,k))/dmx2(1) and (T(i,n2m,k) - TBCup(i,k))/dmx2(n2sub),

@@ -768,6 +768,25 @@ struct BlockLdU {
     }
 };
 
+// blockLdV with the pseudo-update of V: dV(j >= 2) = the new dV and Vn =
+// V + dV; at j = 1 Vn = V + dV * 0 (:178-208)
+__global__ void __launch_bounds__(256) k_blockLdV_pseudo(BlockLdV cell, double *dV, double *__restrict__ Vn) {
+    const Grid &g = cell.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.n1m || j > g.n2m) return;
+    const size_t q = g.ix(i, j, k);
+    if (j < 2) {
+        Vn[q] = cell.V[q] + dV[q] * 0.0;
+        return;
+    }
+    double o, x;
+    cell.eval<double>(RawW(), i, j, k, o, x, x, x);
+    dV[q] = o;
+    Vn[q] = cell.V[q] + o;
+}
+
 __global__ void __launch_bounds__(256) k_blockLdU(BlockLdU cell, double *dU) {
     const Grid &g = cell.g;
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
@@ -867,6 +886,43 @@ int momentum_blockLdU(tcs_ctx *c, double dt) {
     const BlockLdU cell{c->grid(), c->phys, dt,          c->f[TCS_U],      c->f[TCS_DV], c->f[TCS_DW],
                         c->f[TCS_MU], c->f[TCS_INVRHO], c->f[TCS_DU]};
     TCS_LAUNCH(c->stream, "blockLdU", k_blockLdU, grd, blk, 0, cell, c->f[TCS_DU]);
+    return TCS_OK;
+}
+
+// blockLdU with the pseudo-update of U and W: dU = the new dU, Un = U + dU,
+// Wn = W + dW (:178-208)
+__global__ void __launch_bounds__(256) k_blockLdU_pseudo(BlockLdU cell, const double *__restrict__ W,
+                                                        const double *__restrict__ dW, double *dU,
+                                                        double *__restrict__ Un, double *__restrict__ Wn) {
+    const Grid &g = cell.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.n1m || j > g.n2m) return;
+    double o, x;
+    cell.eval<double>(RawW(), i, j, k, o, x, x, x);
+    const size_t q = g.ix(i, j, k);
+    dU[q] = o;
+    Un[q] = cell.U[q] + o;
+    Wn[q] = W[q] + dW[q];
+}
+
+int momentum_blockLdV_pseudo(tcs_ctx *c, double dt, double *Vn) {
+    const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
+    const BlockLdV cell{c->grid(), c->phys, dt, c->f[TCS_V], c->f[TCS_DW], c->f[TCS_MU], c->f[TCS_INVRHO], c->f[TCS_DV]};
+    TCS_TRY(copy_shell(c, c->f[TCS_V], Vn));
+    TCS_LAUNCH(c->stream, "blockLdV", k_blockLdV_pseudo, grd, blk, 0, cell, c->f[TCS_DV], Vn);
+    return TCS_OK;
+}
+
+int momentum_blockLdU_pseudo(tcs_ctx *c, double dt, double *Un, double *Wn) {
+    const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
+    const BlockLdU cell{c->grid(), c->phys, dt, c->f[TCS_U], c->f[TCS_DV], c->f[TCS_DW],
+                        c->f[TCS_MU], c->f[TCS_INVRHO], c->f[TCS_DU]};
+    TCS_TRY(copy_shell(c, c->f[TCS_U], Un));
+    TCS_TRY(copy_shell(c, c->f[TCS_W], Wn));
+    TCS_LAUNCH(c->stream, "blockLdU", k_blockLdU_pseudo, grd, blk, 0, cell, c->f[TCS_W], c->f[TCS_DW], c->f[TCS_DU],
+               Un, Wn);
     return TCS_OK;
 }
 

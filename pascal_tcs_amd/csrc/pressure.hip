@@ -551,6 +551,27 @@ struct DPhatUpdate {
     }
 };
 
+// the projection with the new dPhat and dP0 written to second buffers
+__global__ void __launch_bounds__(256) k_projection_into(Projection cell, DPhatUpdate hist, double *U, double *V,
+                                                        double *W, double *P, double *__restrict__ dPhatn,
+                                                        double *__restrict__ dP0n) {
+    const Grid &g = cell.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.n1m || j > g.n2m) return;
+    double u, v, w, q, h, d0, x;
+    cell.eval<double>(RawW(), i, j, k, u, v, w, q);
+    hist.eval<double>(RawW(), i, j, k, h, d0, x, x);
+    const size_t o = g.ix(i, j, k);
+    U[o] = u;
+    V[o] = v;
+    W[o] = w;
+    P[o] = q;
+    dP0n[o] = d0;
+    dPhatn[o] = h;
+}
+
 __global__ void __launch_bounds__(256) k_dphat_update(DPhatUpdate cell, double *dPhat, double *dP0) {
     const Grid &g = cell.g;
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
@@ -628,6 +649,20 @@ int pressure_projection(tcs_ctx *c, double dt) {
                c->f[TCS_P]);
     TCS_LAUNCH(c->stream, "dphat_update", k_dphat_update, grd, blk, 0, DPhatUpdate{g, c->f[TCS_DP], c->f[TCS_DPHAT],
                c->f[TCS_DP0]}, c->f[TCS_DPHAT], c->f[TCS_DP0]);
+    TCS_LAUNCH(c->stream, "wall_extrap", k_wall_extrap, dim3((c->n1m + 63) / 64, c->n3m), dim3(64), 0, g, c->f[TCS_P]);
+    return TCS_OK;
+}
+
+int pressure_projection_into(tcs_ctx *c, double dt, double *dPhatn, double *dP0n) {
+    const Grid g = c->grid();
+    const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
+    const Projection cell{g, c->phys, dt, c->f[TCS_INVRHO], c->f[TCS_DP], c->f[TCS_DPHAT], c->f[TCS_DP0],
+                          c->f[TCS_U], c->f[TCS_V], c->f[TCS_W], c->f[TCS_P]};
+    TCS_TRY(copy_shell(c, c->f[TCS_DPHAT], dPhatn));
+    TCS_TRY(copy_shell(c, c->f[TCS_DP0], dP0n));
+    TCS_LAUNCH(c->stream, "projection", k_projection_into, grd, blk, 0, cell,
+               DPhatUpdate{g, c->f[TCS_DP], c->f[TCS_DPHAT], c->f[TCS_DP0]}, c->f[TCS_U], c->f[TCS_V], c->f[TCS_W],
+               c->f[TCS_P], dPhatn, dP0n);
     TCS_LAUNCH(c->stream, "wall_extrap", k_wall_extrap, dim3((c->n1m + 63) / 64, c->n3m), dim3(64), 0, g, c->f[TCS_P]);
     return TCS_OK;
 }

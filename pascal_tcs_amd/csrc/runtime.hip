@@ -130,6 +130,39 @@ __global__ void k_halo(Grid g, double *__restrict__ f) {
     }
 }
 
+// the cells of a full field outside the interior: the faces i in {0, n1m+1}
+// (all j, k), then j in {0, n2m+1} (interior i, all k), then k in {0, n3m+1}
+// (interior i and j)
+__global__ void k_copy_shell(Grid g, const double *__restrict__ src, double *__restrict__ dst) {
+    const int N1 = g.N1, N2 = g.N2, N3 = g.N3, n1m = g.n1m, n2m = g.n2m;
+    const size_t fi = (size_t)2 * N2 * N3, fj = (size_t)2 * n1m * N3, fk = (size_t)2 * n1m * n2m;
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t o;
+    if (q < fi) {
+        const int side = (int)(q % 2);
+        const size_t r = q / 2;
+        o = g.ix(side ? N1 - 1 : 0, (int)(r % N2), (int)(r / N2));
+    } else if ((q -= fi) < fj) {
+        const int side = (int)(q % 2);
+        const size_t r = q / 2;
+        o = g.ix((int)(r % n1m) + 1, side ? N2 - 1 : 0, (int)(r / n1m));
+    } else if ((q -= fj) < fk) {
+        const int side = (int)(q % 2);
+        const size_t r = q / 2;
+        o = g.ix((int)(r % n1m) + 1, (int)(r / n1m) + 1, side ? N3 - 1 : 0);
+    } else {
+        return;
+    }
+    dst[o] = src[o];
+}
+
+int copy_shell(tcs_ctx *c, const double *src, double *dst) {
+    const size_t n = (size_t)2 * c->N2 * c->N3 + (size_t)2 * c->n1m * c->N3 + (size_t)2 * c->n1m * c->n2m;
+    TCS_LAUNCH(c->stream, "copy_shell", k_copy_shell, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->grid(), src,
+               dst);
+    return TCS_OK;
+}
+
 int halo(tcs_ctx *c, double *f) {
     const size_t n = (size_t)2 * c->N2 * c->N3 + (size_t)2 * c->N1 * c->N2;
     TCS_LAUNCH(c->stream, "halo", k_halo, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->grid(), f);
@@ -405,8 +438,10 @@ static void free_ctx(tcs_ctx *c) {
                        &c->tw1, &c->tw3, &c->linesum, &c->planesum, &c->scal, &c->red};
     for (double **b : bufs)
         if (*b) (void)hipFree(*b);
-    for (int q = 0; q < TCS_NFIELD; q++)
+    for (int q = 0; q < TCS_NFIELD; q++) {
         if (c->f[q]) (void)hipFree(c->f[q]);
+        if (c->spare[q]) (void)hipFree(c->spare[q]);
+    }
     for (int q = 0; q < TCS_NWALL; q++)
         if (c->wall[q]) (void)hipFree(c->wall[q]);
     if (c->jC) (void)hipFree(c->jC);
@@ -550,6 +585,8 @@ extern "C" int tcs_init(const tcs_params *prm, const double *dx1, const double *
         const size_t n = (q == TCS_INVRHO || q == TCS_MU) ? c->ntab : c->nfull;
         if ((rc = dalloc(&c->f[q], n)) != TCS_OK) return fail(rc);
     }
+    for (int q : {TCS_T, TCS_U, TCS_V, TCS_W, TCS_DPHAT, TCS_DP0})
+        if ((rc = dalloc(&c->spare[q], c->nfull)) != TCS_OK) return fail(rc);
     for (int q = 0; q < TCS_NWALL; q++)
         if ((rc = dalloc(&c->wall[q], c->nwall)) != TCS_OK) return fail(rc);
     if (hipMalloc((void **)&c->jC, c->N2 * sizeof(int)) != hipSuccess ||
@@ -788,7 +825,9 @@ extern "C" int tcs_step(tcs_ctx *c, double dt) {
     CTX_GUARD(c);
     STAGE("step");
     const char *H = "update ghostcell";
-    RUN("thermal solve", FORM(c, thermal_solver, dt));
+    bool swap_T = false;
+    RUN("thermal solve", FORM(c, thermal_solver_into, dt, c->spare[TCS_T], &swap_T));
+    if (swap_T) std::swap(c->f[TCS_T], c->spare[TCS_T]);
     RUN(H, halo(c, c->f[TCS_T]));
     RUN("momentum coeff", FORM(c, momentum_coeffi));
     RUN("momentum solvedU", FORM(c, momentum_solvedU, dt));
@@ -797,17 +836,21 @@ extern "C" int tcs_step(tcs_ctx *c, double dt) {
     RUN(H, halo(c, c->f[TCS_DV]));
     RUN("momentum solvedW", FORM(c, momentum_solvedW, dt));
     RUN(H, halo(c, c->f[TCS_DW]));
-    RUN("momentum blockLdV", FORM(c, momentum_blockLdV, dt));
+    // block corrections with the pseudo-update (U + dU, V + dV, W + dW into
+    // the second buffers; the corrections read the old U, V)
+    RUN("momentum blockLdV", FORM(c, momentum_blockLdV_pseudo, dt, c->spare[TCS_V]));
     RUN(H, halo(c, c->f[TCS_DV]));
-    RUN("momentum blockLdU", FORM(c, momentum_blockLdU, dt));
-    RUN("momentum updatePseudoUVW", FORM(c, momentum_pseudoupdate));
+    RUN("momentum blockLdU", FORM(c, momentum_blockLdU_pseudo, dt, c->spare[TCS_U], c->spare[TCS_W]));
+    for (int q : {TCS_U, TCS_V, TCS_W}) std::swap(c->f[q], c->spare[q]);
     RUN(H, halo(c, c->f[TCS_U]));
     RUN(H, halo(c, c->f[TCS_V]));
     RUN(H, halo(c, c->f[TCS_W]));
     RUN("pressure RHS", FORM(c, pressure_rhs, dt));
     RUN("pressure PoissonSolve", FORM(c, pressure_poisson));
     RUN(H, halo(c, c->f[TCS_DP]));
-    RUN("pressure projection", FORM(c, pressure_projection, dt));
+    RUN("pressure projection", FORM(c, pressure_projection_into, dt, c->spare[TCS_DPHAT], c->spare[TCS_DP0]));
+    std::swap(c->f[TCS_DPHAT], c->spare[TCS_DPHAT]);
+    std::swap(c->f[TCS_DP0], c->spare[TCS_DP0]);
     RUN(H, halo(c, c->f[TCS_U]));
     RUN(H, halo(c, c->f[TCS_V]));
     RUN(H, halo(c, c->f[TCS_W]));

@@ -105,6 +105,9 @@ struct tcs_ctx {
     int *jC, *jS;
 
     double *f[TCS_NFIELD];  // INVRHO / MU hold (0:n2sub,0:n3sub) tables
+    // second buffers of T, U, V, W, dPhat, dP0: tcs_step writes their
+    // updates there (the reads of the old values stay undisturbed) and swaps
+    double *spare[TCS_NFIELD];
     double *wall[TCS_NWALL];
 
     // ADI scratch (interior size each)
@@ -155,11 +158,13 @@ int ptdma_exact_launch(double *A, const double *B, double *C, double *D,
                        int n0, long ss0, int n1, long ss1, int n_row, long rs,
                        bool cyclic, hipStream_t st);
 // Where a split solve puts its solution: back into D (mode 0), or into an
-// FP64 field F at F[base + s0*ss0 + s1*ss1 + row*rs], stored (mode 1) or
-// added to what F holds (mode 2) -- the last sweep of an ADI stage writes
-// the increment straight into its module array.
+// FP64 field F at F[base + s0*ss0 + s1*ss1 + row*rs], stored (mode 1),
+// added to what F holds (mode 2), or added to what Fin holds at the same
+// place (mode 3: F a second buffer of the field) -- the last sweep of an ADI
+// stage writes the increment straight into its module array.
 struct LineOut {
     double *F = nullptr;
+    const double *Fin = nullptr;
     long base = 0, ss0 = 0, ss1 = 0, rs = 0;
     int mode = 0;
 };
@@ -193,24 +198,34 @@ inline int ptdma_sweep(const tcs_ctx *c, float *A, const float *B, float *C, flo
 // (precision 1), and into namespace `mixed`, the fast arithmetic with FP32
 // ADI scratch and FP32 line solves (precision 2).  TCS_FORM names the
 // namespace a translation unit builds, TCS_SCRATCH its scratch type.
-#define TCS_STAGE_DECLS                                   \
-    int thermal_solver(tcs_ctx *c, double dt);            \
-    int thermal_coeffi(tcs_ctx *c);                       \
-    int momentum_coeffi(tcs_ctx *c);                      \
-    int momentum_solvedU(tcs_ctx *c, double dt);          \
-    int momentum_solvedV(tcs_ctx *c, double dt);          \
-    int momentum_solvedW(tcs_ctx *c, double dt);          \
-    int momentum_blockLdV(tcs_ctx *c, double dt);         \
-    int momentum_blockLdU(tcs_ctx *c, double dt);         \
-    int momentum_pseudoupdate(tcs_ctx *c);                \
-    int pressure_rhs(tcs_ctx *c, double dt);              \
-    int pressure_poisson(tcs_ctx *c);                     \
-    int pressure_projection(tcs_ctx *c, double dt);
+// The *_into / *_pseudo routines are tcs_step's fused forms: thermal's last
+// sweep writes T + dT into a second buffer (when the lines are split); the
+// block corrections also write the pseudo-update U + dU, V + dV, W + dW and
+// the projection the new dPhat, dP0 into second buffers.
+#define TCS_STAGE_DECLS                                                                 \
+    int thermal_solver(tcs_ctx *c, double dt);                                          \
+    int thermal_coeffi(tcs_ctx *c);                                                     \
+    int momentum_coeffi(tcs_ctx *c);                                                    \
+    int momentum_solvedU(tcs_ctx *c, double dt);                                        \
+    int momentum_solvedV(tcs_ctx *c, double dt);                                        \
+    int momentum_solvedW(tcs_ctx *c, double dt);                                        \
+    int momentum_blockLdV(tcs_ctx *c, double dt);                                       \
+    int momentum_blockLdU(tcs_ctx *c, double dt);                                       \
+    int momentum_pseudoupdate(tcs_ctx *c);                                              \
+    int pressure_rhs(tcs_ctx *c, double dt);                                            \
+    int pressure_poisson(tcs_ctx *c);                                                   \
+    int pressure_projection(tcs_ctx *c, double dt);                                     \
+    int thermal_solver_into(tcs_ctx *c, double dt, double *Tn, bool *written);          \
+    int momentum_blockLdV_pseudo(tcs_ctx *c, double dt, double *Vn);                    \
+    int momentum_blockLdU_pseudo(tcs_ctx *c, double dt, double *Un, double *Wn);        \
+    int pressure_projection_into(tcs_ctx *c, double dt, double *dPhatn, double *dP0n);
 namespace exact { TCS_STAGE_DECLS }
 namespace fast { TCS_STAGE_DECLS }
 namespace mixed { TCS_STAGE_DECLS }
 #undef TCS_STAGE_DECLS
 int halo(tcs_ctx *c, double *f);
+// copy the cells outside the interior (x/z ghosts, wall rows) of a full field
+int copy_shell(tcs_ctx *c, const double *src, double *dst);
 int post_div(tcs_ctx *c, double *out);
 int post_cfl(tcs_ctx *c, double *maxuovx);
 int nusselt(tcs_ctx *c, double Tup, double Tbt, double *nb, double *nt);

@@ -265,7 +265,7 @@ __device__ __forceinline__ void exchange(RedLDS<T, SX, NR> &red, int line, int p
 }
 
 // The solution of a part of line (s0, s1): into D at dbase + row*rs
-// (out.mode 0), or stored into / added to the field of out (modes 1, 2).
+// (out.mode 0), or into the field of out (modes 1, 2, 3).
 template <typename T>
 __device__ __forceinline__ void store(const T (&a)[PT_MMAX], const T (&c)[PT_MMAX], const T (&d)[1][PT_MMAX],
                                       int j0, int m, T xf, T xl, T *D, long dbase, long rs, const LineOut &out,
@@ -280,8 +280,8 @@ __device__ __forceinline__ void store(const T (&a)[PT_MMAX], const T (&c)[PT_MMA
         for (int r = 0; r < PT_MMAX; r++) {
             if (r < m) {
                 const double x = (double)x_of<T, 1>(a, c, d, 0, r, m, xf, xl);
-                double *f = out.F + fb + (long)(j0 + r) * out.rs;
-                *f = out.mode == 1 ? x : *f + x;
+                const long o = fb + (long)(j0 + r) * out.rs;
+                out.F[o] = out.mode == 1 ? x : (out.mode == 2 ? out.F[o] : out.Fin[o]) + x;
             }
         }
     }

@@ -441,5 +441,34 @@ int thermal_solver(tcs_ctx *c, double dt) {
     return sweep_y_field(c, 1, c->n2m, T, 2);
 }
 
+// thermal_solver whose last sweep writes T + dT into the second buffer Tn
+// (with the cells outside the interior copied) when the y lines are split;
+// otherwise the plain routine (*written = false, T updated in place).
+int thermal_solver_into(tcs_ctx *c, double dt, double *Tn, bool *written) {
+    *written = false;
+    if (!rows_in_solve<scr_t>(c, c->n2m)) return thermal_solver(c, dt);
+    const Grid g = c->grid();
+    double *T = c->f[TCS_T];
+    scr_t *am = SCR(c->am), *ac = SCR(c->ac), *ap = SCR(c->ap), *rhs = SCR(c->rhs);
+    const ThermalZ z{g, c->phys, dt, T, c->f[TCS_U], c->f[TCS_V], c->f[TCS_W], c->wall[TCS_TBCUP], c->wall[TCS_TBCBT]};
+    TCS_TRY(z_stage(c, "thermal_z", "thermal_z_solve", z, 1, am, ac, ap, rhs));
+    TCS_TRY(rows3d_launch<false>(c->stream, "thermal_x", ThermalX{g, c->phys, dt, T, c->f[TCS_U]}, 1, am, ac, ap, rhs));
+    TCS_TRY(ptdma_sweep(c, am, ac, ap, rhs, c->n2m, c->n1m, c->n3m, (long)c->n1m * c->n2m, c->n1m, 1, true));
+    TCS_TRY(copy_shell(c, T, Tn));
+    const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m;
+    LineOut out;
+    out.F = Tn;
+    out.Fin = T;
+    out.mode = 3;
+    out.base = 1 + (long)c->N1 + (long)c->N1 * c->N2;  // (i, j, k) = (1, 1, 1)
+    out.ss0 = 1;
+    out.ss1 = (long)c->N1 * c->N2;
+    out.rs = c->N1;
+    TCS_TRY(ptdma_part_src_launch<false>(c->stream, "thermal_y_solve", YLineRows<ThermalY, scr_t>{ThermalY{g, c->phys, dt,
+                                         T, c->f[TCS_V]}, rhs, 1}, rhs, c->n1m, c->n3m, n12, c->n2m, n1, out));
+    *written = true;
+    return TCS_OK;
+}
+
 }  // namespace TCS_FORM
 }  // namespace tcs
