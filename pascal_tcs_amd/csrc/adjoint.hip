@@ -515,22 +515,16 @@ static int backward(tcs_ctx *c, AdjWork *w, double dt) {
     TCS_LAUNCH(c->stream, "adj_sum", k_interior_sum, grd, blk, 0, g, w->bdP, w->scal);
     TCS_LAUNCH(c->stream, "adj_sub", k_interior_sub, grd, blk, 0, g, w->scal, w->bdP);
     {
-        const int n1m = c->n1m, n2m = c->n2m, n3m = c->n3m, h1p = c->h1p;
-        const int TX = fft_tile(n1m, n3m);
+        const int n2m = c->n2m, n3m = c->n3m, h1p = c->h1p;
         double2 *spec = (double2 *)c->spec_x;
-        const size_t ldsx = (size_t)TX * n1m * 2 * sizeof(double), ldsz = (size_t)n3m * 2 * sizeof(double);
         TCS_LAUNCH(c->stream, "adj_pack", k_full_to_int, grd, blk, 0, g, 1, w->bdP, w->rint);
-        TCS_LAUNCH(c->stream, "adj_fft_x_fwd", k_fft_x_fwd, dim3(n3m / TX, n2m), dim3(FFT_THREADS), ldsx, g, h1p, TX,
-                   w->rint, (const double2 *)c->tw1, spec);
-        TCS_LAUNCH(c->stream, "adj_fft_z", k_fft_z<-1>, dim3(h1p, n2m), dim3(FFT_THREADS), ldsz, n3m, h1p,
-                   (const double2 *)c->tw3, spec);
+        TCS_TRY(fft_x_fwd_launch(c, "adj_fft_x_fwd", w->rint, spec));
+        TCS_TRY(fft_z_launch<-1>(c, "adj_fft_z", spec));
         const int nsys = n3m * h1p;
         TCS_LAUNCH(c->stream, "adj_poisson_T", k_poisson_tdma_T, dim3((nsys + 255) / 256), dim3(256), 0, n3m, h1p, n2m,
                    c->h3p, c->ptab, c->ptab + n2m, c->ptab + 2 * n2m, c->dxk2, c->dzk, spec, c->pC);
-        TCS_LAUNCH(c->stream, "adj_fft_z", k_fft_z<+1>, dim3(h1p, n2m), dim3(FFT_THREADS), ldsz, n3m, h1p,
-                   (const double2 *)c->tw3, spec);
-        TCS_LAUNCH(c->stream, "adj_fft_x_inv", k_fft_x_inv, dim3(n3m / TX, n2m), dim3(FFT_THREADS), ldsx, g, h1p, TX,
-                   (const double2 *)spec, (const double2 *)c->tw1, w->tmp[0], c->linesum);
+        TCS_TRY(fft_z_launch<+1>(c, "adj_fft_z", spec));
+        TCS_TRY(fft_x_inv_launch(c, "adj_fft_x_inv", spec, w->tmp[0]));
         TCS_LAUNCH(c->stream, "adj_pack", k_full_to_int, grd, blk, 0, g, 1, w->tmp[0], w->rint);
         TCS_HIP(hipMemsetAsync(w->bdP, 0, c->nfull * sizeof(double), c->stream));
     }

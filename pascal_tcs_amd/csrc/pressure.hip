@@ -30,11 +30,10 @@ namespace TCS_FORM {
 
 #include "metric.h"
 
-constexpr int FFT_THREADS = 256;
 constexpr int FFT_MAXB = 8;  // butterflies per thread per stage
 
-// In-place radix-2 Stockham on nl lines (re/im at [l*n + t]) in LDS, n a
-// power of two, nl*n <= 2*FFT_MAXB*FFT_THREADS.  sign<0 forward
+// In-place radix-2 Stockham on nl lines (re/im at [l*n + t]) in LDS by NT
+// threads, n a power of two, nl*n <= 2*FFT_MAXB*NT.  sign<0 forward
 // (exp(-i...)), sign>0 backward.  Caller synchronises before.
 //
 // Stage s (Ns = 2^s) combines x[jb] and x[jb + n/2] with the twiddle
@@ -45,7 +44,7 @@ constexpr int FFT_MAXB = 8;  // butterflies per thread per stage
 // the same operands as the stage-by-stage loop, so the same results bit for
 // bit.  The group's outputs land at 8(g - kk) + kk + r*Ns, kk = g mod Ns,
 // with r the three result bits (first stage's bit lowest).
-template <int SIGN, int R>
+template <int SIGN, int R, int NT>
 __device__ __forceinline__ void fft_pass(double *re, double *im, int n, int nl, int s,
                                          const double2 *__restrict__ tw) {
     constexpr int M = 1 << R;
@@ -58,7 +57,7 @@ __device__ __forceinline__ void fft_pass(double *re, double *im, int n, int nl, 
     int obase[GMAX];
 #pragma unroll
     for (int u = 0; u < GMAX; u++) {
-        const int e = tid + u * FFT_THREADS;
+        const int e = tid + u * NT;
         obase[u] = -1;
         if (e < total) {
             const int l = e / G;
@@ -115,14 +114,14 @@ __device__ __forceinline__ void fft_pass(double *re, double *im, int n, int nl, 
     __syncthreads();
 }
 
-template <int SIGN>
+template <int SIGN, int NT>
 __device__ __forceinline__ void lds_fft(double *re, double *im, int n, int nl, const double2 *__restrict__ tw) {
     int L = 0;
     while ((1 << L) < n) L++;
     int s = 0;
-    for (; s + 3 <= L; s += 3) fft_pass<SIGN, 3>(re, im, n, nl, s, tw);
-    if (L - s == 2) fft_pass<SIGN, 2>(re, im, n, nl, s, tw);
-    else if (L - s == 1) fft_pass<SIGN, 1>(re, im, n, nl, s, tw);
+    for (; s + 3 <= L; s += 3) fft_pass<SIGN, 3, NT>(re, im, n, nl, s, tw);
+    if (L - s == 2) fft_pass<SIGN, 2, NT>(re, im, n, nl, s, tw);
+    else if (L - s == 1) fft_pass<SIGN, 1, NT>(re, im, n, nl, s, tw);
 }
 
 // PRHS (:182-229) into the interior layout
@@ -184,7 +183,8 @@ __global__ void __launch_bounds__(256) k_pressure_rhs(PressureRHS cell, double *
 
 // x r2c: TX k-lines (k0..k0+TX-1) of row j per workgroup.
 // spec[(k + n3m*(kx + h1p*(j-1)))] complex, k 0-based.
-__global__ void __launch_bounds__(FFT_THREADS)
+template <int NT>
+__global__ void __launch_bounds__(NT)
 k_fft_x_fwd(Grid g, int h1p, int TX, const double *__restrict__ prhs, const double2 *__restrict__ tw,
             double2 *__restrict__ spec) {
     extern __shared__ double lds[];
@@ -193,35 +193,35 @@ k_fft_x_fwd(Grid g, int h1p, int TX, const double *__restrict__ prhs, const doub
     const int j = blockIdx.y + 1;
     const int k0 = blockIdx.x * TX;  // 0-based
     const int tid = threadIdx.x;
-    for (int e = tid; e < TX * n; e += FFT_THREADS) {
+    for (int e = tid; e < TX * n; e += NT) {
         const int l = e / n, t = e - l * n;
         re[e] = prhs[g.r3(t + 1, j, k0 + l + 1)];
         im[e] = 0.0;
     }
     __syncthreads();
-    lds_fft<-1>(re, im, n, TX, tw);
+    lds_fft<-1, NT>(re, im, n, TX, tw);
     const size_t rowbase = (size_t)g.n3m * h1p * (j - 1);
-    for (int e = tid; e < TX * h1p; e += FFT_THREADS) {
+    for (int e = tid; e < TX * h1p; e += NT) {
         const int kx = e / TX, l = e - kx * TX;
         spec[rowbase + (size_t)g.n3m * kx + k0 + l] = make_double2(re[l * n + kx], im[l * n + kx]);
     }
 }
 
 // z c2c in place along the contiguous axis: one (kx, j) line per workgroup
-template <int SIGN>
-__global__ void __launch_bounds__(FFT_THREADS)
+template <int SIGN, int NT>
+__global__ void __launch_bounds__(NT)
 k_fft_z(int n3m, int h1p, const double2 *__restrict__ tw, double2 *__restrict__ spec) {
     extern __shared__ double lds[];
     double *re = lds, *im = lds + n3m;
     const size_t base = (size_t)n3m * ((size_t)blockIdx.x + (size_t)h1p * blockIdx.y);
     const int tid = threadIdx.x;
-    for (int t = tid; t < n3m; t += FFT_THREADS) {
+    for (int t = tid; t < n3m; t += NT) {
         const double2 v = spec[base + t];
         re[t] = v.x; im[t] = v.y;
     }
     __syncthreads();
-    lds_fft<SIGN>(re, im, n3m, 1, tw);
-    for (int t = tid; t < n3m; t += FFT_THREADS) spec[base + t] = make_double2(re[t], im[t]);
+    lds_fft<SIGN, NT>(re, im, n3m, 1, tw);
+    for (int t = tid; t < n3m; t += NT) spec[base + t] = make_double2(re[t], im[t]);
 }
 
 // y tridiagonal solves per (kx,kz), PaScaL single-rank order (:599-652).
@@ -373,7 +373,8 @@ k_poisson_part(int n3m, int h1p, int h3p, const double *__restrict__ amj, const 
 
 // x c2r of TX k-lines of row j, scaling by 1/n1m/n3m into dP's interior,
 // and the per-line pairwise sums for the mean.
-__global__ void __launch_bounds__(FFT_THREADS)
+template <int NT>
+__global__ void __launch_bounds__(NT)
 k_fft_x_inv(Grid g, int h1p, int TX, const double2 *__restrict__ spec, const double2 *__restrict__ tw,
             double *__restrict__ dP, double *__restrict__ linesum) {
     extern __shared__ double lds[];
@@ -384,7 +385,7 @@ k_fft_x_inv(Grid g, int h1p, int TX, const double2 *__restrict__ spec, const dou
     const int tid = threadIdx.x;
     const int nh = n / 2;
     const size_t rowbase = (size_t)g.n3m * h1p * (j - 1);
-    for (int e = tid; e < TX * h1p; e += FFT_THREADS) {
+    for (int e = tid; e < TX * h1p; e += NT) {
         const int kx = e / TX, l = e - kx * TX;
         const double2 v = spec[rowbase + (size_t)g.n3m * kx + k0 + l];
         re[l * n + kx] = v.x;
@@ -395,9 +396,9 @@ k_fft_x_inv(Grid g, int h1p, int TX, const double2 *__restrict__ spec, const dou
         }
     }
     __syncthreads();
-    lds_fft<+1>(re, im, n, TX, tw);
+    lds_fft<+1, NT>(re, im, n, TX, tw);
     const double fn1 = (double)g.n1m, fn3 = (double)g.n3m;
-    for (int e = tid; e < TX * n; e += FFT_THREADS) {
+    for (int e = tid; e < TX * n; e += NT) {
         const int l = e / n, t = e - l * n;
         const double v = re[e] / fn1 / fn3;
         dP[g.ix(t + 1, j, k0 + l + 1)] = v;
@@ -405,13 +406,13 @@ k_fft_x_inv(Grid g, int h1p, int TX, const double2 *__restrict__ spec, const dou
     }
     __syncthreads();
     for (int h = n / 2; h >= 1; h >>= 1) {
-        for (int e = tid; e < TX * h; e += FFT_THREADS) {
+        for (int e = tid; e < TX * h; e += NT) {
             const int l = e / h, t = e - l * h;
             im[l * n + t] = im[l * n + t] + im[l * n + t + h];
         }
         __syncthreads();
     }
-    for (int l = tid; l < TX; l += FFT_THREADS) linesum[(size_t)(j - 1) + (size_t)g.n2m * (k0 + l)] = im[l * n];
+    for (int l = tid; l < TX; l += NT) linesum[(size_t)(j - 1) + (size_t)g.n2m * (k0 + l)] = im[l * n];
 }
 
 // pairwise sum of `cnt` values at src + q*stride, zero padded to a power of two
@@ -586,12 +587,56 @@ __global__ void __launch_bounds__(256) k_dphat_update(DPhatUpdate cell, double *
 }
 
 // ---------------------------------------------------------------- host -----
+// x FFTs: 256 threads (the strided gather of the k-lines wants them all)
+// over TX k-lines of n1m points per workgroup, at most 4 (more leaves too
+// few workgroups per CU), within one pass's butterflies and 64 KiB of LDS
+constexpr int FFT_X_THREADS = 256;
 static int fft_tile(int n1m, int n3m) {
     int tx = 1;
-    while (tx < 8 && tx * 2 <= n3m && (tx * 2) * (n1m / 2) <= FFT_MAXB * FFT_THREADS &&
+    while (tx < 4 && tx * 2 <= n3m && (tx * 2) * (n1m / 2) <= FFT_MAXB * FFT_X_THREADS &&
            (size_t)(tx * 2) * n1m * 16 <= 65536)
         tx *= 2;
     return tx;
+}
+
+// r2c along x of the interior-layout field `in` into the spectrum
+static int fft_x_fwd_launch(tcs_ctx *c, const char *label, const double *in, double2 *spec) {
+    const int TX = fft_tile(c->n1m, c->n3m);
+    const size_t lds = (size_t)TX * c->n1m * 2 * sizeof(double);
+    TCS_LAUNCH(c->stream, label, k_fft_x_fwd<FFT_X_THREADS>, dim3(c->n3m / TX, c->n2m), dim3(FFT_X_THREADS), lds,
+               c->grid(), c->h1p, TX, in, (const double2 *)c->tw1, spec);
+    return TCS_OK;
+}
+
+// c2r along x into the full-layout field `out`, with the per-line sums
+static int fft_x_inv_launch(tcs_ctx *c, const char *label, const double2 *spec, double *out) {
+    const int TX = fft_tile(c->n1m, c->n3m);
+    const size_t lds = (size_t)TX * c->n1m * 2 * sizeof(double);
+    TCS_LAUNCH(c->stream, label, k_fft_x_inv<FFT_X_THREADS>, dim3(c->n3m / TX, c->n2m), dim3(FFT_X_THREADS), lds,
+               c->grid(), c->h1p, TX, spec, (const double2 *)c->tw1, out, c->linesum);
+    return TCS_OK;
+}
+
+// z FFTs: the fewest threads (64, 128 or 256) that hold a pass's
+// butterflies at FFT_MAXB per thread (one wave for 512 points: small
+// workgroups, more of them per CU)
+static int fft_threads(int butterflies) {
+    int nt = 64;
+    while (nt < 256 && FFT_MAXB * nt < butterflies) nt <<= 1;
+    return nt;
+}
+
+// c2c along z in place, one (kx, j) line per workgroup
+template <int SIGN>
+static int fft_z_launch(tcs_ctx *c, const char *label, double2 *spec) {
+    const int NT = fft_threads(c->n3m / 2);
+    const size_t lds = (size_t)c->n3m * 2 * sizeof(double);
+    const dim3 grd(c->h1p, c->n2m);
+    const double2 *tw = (const double2 *)c->tw3;
+    if (NT == 64) TCS_LAUNCH(c->stream, label, (k_fft_z<SIGN, 64>), grd, dim3(64), lds, c->n3m, c->h1p, tw, spec);
+    else if (NT == 128) TCS_LAUNCH(c->stream, label, (k_fft_z<SIGN, 128>), grd, dim3(128), lds, c->n3m, c->h1p, tw, spec);
+    else TCS_LAUNCH(c->stream, label, (k_fft_z<SIGN, 256>), grd, dim3(256), lds, c->n3m, c->h1p, tw, spec);
+    return TCS_OK;
 }
 
 int pressure_rhs(tcs_ctx *c, double dt) {
@@ -605,14 +650,9 @@ int pressure_rhs(tcs_ctx *c, double dt) {
 int pressure_poisson(tcs_ctx *c) {
     const Grid g = c->grid();
     const int n1m = c->n1m, n2m = c->n2m, n3m = c->n3m, h1p = c->h1p;
-    const int TX = fft_tile(n1m, n3m);
     double2 *spec = (double2 *)c->spec_x;
-    const size_t ldsx = (size_t)TX * n1m * 2 * sizeof(double);
-    TCS_LAUNCH(c->stream, "fft_x_fwd", k_fft_x_fwd, dim3(n3m / TX, n2m), dim3(FFT_THREADS), ldsx, g, h1p, TX, c->prhs,
-                       (const double2 *)c->tw1, spec);
-    const size_t ldsz = (size_t)n3m * 2 * sizeof(double);
-    TCS_LAUNCH(c->stream, "fft_z<-1>", k_fft_z<-1>, dim3(h1p, n2m), dim3(FFT_THREADS), ldsz, n3m, h1p,
-                       (const double2 *)c->tw3, spec);
+    TCS_TRY(fft_x_fwd_launch(c, "fft_x_fwd", c->prhs, spec));
+    TCS_TRY(fft_z_launch<-1>(c, "fft_z<-1>", spec));
     const int nsys = n3m * h1p;
     int P, w1, w2;
     if (c->tdma_split && ptdma_part_plan(n2m, P, w1, w2)) {
@@ -622,10 +662,8 @@ int pressure_poisson(tcs_ctx *c) {
         TCS_LAUNCH(c->stream, "poisson_tdma", k_poisson_tdma, dim3((nsys + 255) / 256), dim3(256), 0, n3m, h1p, n2m,
                    c->h3p, c->ptab, c->ptab + n2m, c->ptab + 2 * n2m, c->dxk2, c->dzk, spec, c->pA, c->pC);
     }
-    TCS_LAUNCH(c->stream, "fft_z<+1>", k_fft_z<+1>, dim3(h1p, n2m), dim3(FFT_THREADS), ldsz, n3m, h1p,
-                       (const double2 *)c->tw3, spec);
-    TCS_LAUNCH(c->stream, "fft_x_inv", k_fft_x_inv, dim3(n3m / TX, n2m), dim3(FFT_THREADS), ldsx, g, h1p, TX,
-                       (const double2 *)spec, (const double2 *)c->tw1, c->f[TCS_DP], c->linesum);
+    TCS_TRY(fft_z_launch<+1>(c, "fft_z<+1>", spec));
+    TCS_TRY(fft_x_inv_launch(c, "fft_x_inv", spec, c->f[TCS_DP]));
     int P2 = 1;
     while (P2 < n2m) P2 <<= 1;
     TCS_LAUNCH(c->stream, "plane_sum", k_plane_sum, dim3(n3m), dim3(256), P2 * sizeof(double), n2m, c->linesum,
