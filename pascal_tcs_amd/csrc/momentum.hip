@@ -768,23 +768,38 @@ struct BlockLdU {
     }
 };
 
+// planes per thread of the block corrections (measured at 512^3)
+constexpr int LDV_KB = 1, LDU_KB = 4;
+
 // blockLdV with the pseudo-update of V: dV(j >= 2) = the new dV and Vn =
 // V + dV; at j = 1 Vn = V + dV * 0 (:178-208)
 __global__ void __launch_bounds__(256) k_blockLdV_pseudo(BlockLdV cell, double *dV, double *__restrict__ Vn) {
     const Grid &g = cell.g;
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
     if (i > g.n1m || j > g.n2m) return;
-    const size_t q = g.ix(i, j, k);
-    if (j < 2) {
-        Vn[q] = cell.V[q] + dV[q] * 0.0;
-        return;
+    // loads of every plane before the stores (dV is updated in place, each
+    // cell reading only its own point of it)
+    double o[LDV_KB], vn[LDV_KB];
+#pragma unroll
+    for (int kq = 0; kq < LDV_KB; kq++) {
+        const int k = blockIdx.z * LDV_KB + kq + 1;
+        const size_t q = g.ix(i, j, k);
+        if (j < 2) {
+            o[kq] = 0.0;  // not stored
+            vn[kq] = cell.V[q] + dV[q] * 0.0;
+        } else {
+            double x;
+            cell.eval<double>(RawW(), i, j, k, o[kq], x, x, x);
+            vn[kq] = cell.V[q] + o[kq];
+        }
     }
-    double o, x;
-    cell.eval<double>(RawW(), i, j, k, o, x, x, x);
-    dV[q] = o;
-    Vn[q] = cell.V[q] + o;
+#pragma unroll
+    for (int kq = 0; kq < LDV_KB; kq++) {
+        const size_t q = g.ix(i, j, blockIdx.z * LDV_KB + kq + 1);
+        if (j >= 2) dV[q] = o[kq];
+        Vn[q] = vn[kq];
+    }
 }
 
 __global__ void __launch_bounds__(256) k_blockLdU(BlockLdU cell, double *dU) {
@@ -897,18 +912,30 @@ __global__ void __launch_bounds__(256) k_blockLdU_pseudo(BlockLdU cell, const do
     const Grid &g = cell.g;
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
     if (i > g.n1m || j > g.n2m) return;
-    double o, x;
-    cell.eval<double>(RawW(), i, j, k, o, x, x, x);
-    const size_t q = g.ix(i, j, k);
-    dU[q] = o;
-    Un[q] = cell.U[q] + o;
-    Wn[q] = W[q] + dW[q];
+    // loads of every plane before the stores (dU is updated in place, each
+    // cell reading only its own point of it)
+    double o[LDU_KB], un[LDU_KB], wn[LDU_KB];
+#pragma unroll
+    for (int kq = 0; kq < LDU_KB; kq++) {
+        const int k = blockIdx.z * LDU_KB + kq + 1;
+        double x;
+        cell.eval<double>(RawW(), i, j, k, o[kq], x, x, x);
+        const size_t q = g.ix(i, j, k);
+        un[kq] = cell.U[q] + o[kq];
+        wn[kq] = W[q] + dW[q];
+    }
+#pragma unroll
+    for (int kq = 0; kq < LDU_KB; kq++) {
+        const size_t q = g.ix(i, j, blockIdx.z * LDU_KB + kq + 1);
+        dU[q] = o[kq];
+        Un[q] = un[kq];
+        Wn[q] = wn[kq];
+    }
 }
 
 int momentum_blockLdV_pseudo(tcs_ctx *c, double dt, double *Vn) {
-    const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
+    const dim3 blk = interior_block(), grd = cell_grid(c->n1m, c->n2m, c->n3m, LDV_KB);
     const BlockLdV cell{c->grid(), c->phys, dt, c->f[TCS_V], c->f[TCS_DW], c->f[TCS_MU], c->f[TCS_INVRHO], c->f[TCS_DV]};
     TCS_TRY(copy_shell(c, c->f[TCS_V], Vn));
     TCS_LAUNCH(c->stream, "blockLdV", k_blockLdV_pseudo, grd, blk, 0, cell, c->f[TCS_DV], Vn);
@@ -916,7 +943,7 @@ int momentum_blockLdV_pseudo(tcs_ctx *c, double dt, double *Vn) {
 }
 
 int momentum_blockLdU_pseudo(tcs_ctx *c, double dt, double *Un, double *Wn) {
-    const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
+    const dim3 blk = interior_block(), grd = cell_grid(c->n1m, c->n2m, c->n3m, LDU_KB);
     const BlockLdU cell{c->grid(), c->phys, dt, c->f[TCS_U], c->f[TCS_DV], c->f[TCS_DW],
                         c->f[TCS_MU], c->f[TCS_INVRHO], c->f[TCS_DU]};
     TCS_TRY(copy_shell(c, c->f[TCS_U], Un));

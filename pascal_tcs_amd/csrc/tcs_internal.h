@@ -151,6 +151,12 @@ inline dim3 interior_block() { return dim3(64, 4, 1); }
 inline dim3 interior_grid(int nx, int ny, int nz) {
     return dim3((unsigned)((nx + 63) / 64), (unsigned)((ny + 3) / 4), (unsigned)nz);
 }
+// Cell-update stencil kernels where a thread takes KB consecutive k planes
+// of one (i, j) (KB <= 4 divides n3m, a power of two >= 4): the values a
+// plane shares with the next are loaded once.
+inline dim3 cell_grid(int nx, int ny, int nz, int kb) {
+    return dim3((unsigned)((nx + 63) / 64), (unsigned)((ny + 3) / 4), (unsigned)(nz / kb));
+}
 
 // TDMA launcher (exact PaScaL order) on strided systems:
 //   system (s0, s1) starts at base + s0*ss0 + s1*ss1, rows step rs.
