@@ -72,8 +72,12 @@ def kernel_bytes(label, cfg):
         "dU_z": 8.0 * N * (5 + 4), "dV_z": 8.0 * N * (5 + 4) * fv, "dW_z": 8.0 * N * (6 + 4),
         "dU_x": 8.0 * N * (1 + 3), "dU_y": 8.0 * N * (1 + 3), "dV_x": 8.0 * N * (1 + 3) * fv,
         "dV_y": 8.0 * N * (1 + 3) * fv, "dW_x": 8.0 * N * (1 + 3), "dW_y": 8.0 * N * (1 + 3),
-        "blockLdV": 8.0 * N * (3 + 1) * fv, "blockLdU": 8.0 * N * (4 + 1),
-        "pseudoupdate": 8.0 * N * 9, "projection": 8.0 * N * (6 + 4), "dphat_update": 8.0 * N * 5,
+        # block corrections with the pseudo-update: V, dW, dV in, dV and V + dV
+        # out; U, dV, dW, dU, W in, dU, U + dU and W + dW out
+        "blockLdV": 8.0 * N * (3 + 2) * fv, "blockLdU": 8.0 * N * (5 + 3),
+        # projection with the dPhat history: U, V, W, P, dP, dPhat, dP0 in;
+        # U, V, W, P, dPhat, dP0 out
+        "pseudoupdate": 8.0 * N * 9, "projection": 8.0 * N * (7 + 6), "dphat_update": 8.0 * N * 5,
         "pressure_rhs": 8.0 * N * (4 + 1), "add_interior": 8.0 * N * 3, "store_interior": 8.0 * N * 2,
         "post_div_cfl": 8.0 * N * 3,
         # split line solves that compute their rows: the stencil's fields in,
@@ -83,6 +87,15 @@ def kernel_bytes(label, cfg):
         "dV_z_solve": 8.0 * N * (5 + 1) * fv, "dW_z_solve": 8.0 * N * (6 + 1),
         "thermal_y_solve": 8.0 * N * (3 + 1), "dU_y_solve": 8.0 * N * (2 + 1),
         "dV_y_solve": 8.0 * N * (2 + 1) * fv, "dW_y_solve": 8.0 * N * (2 + 1),
+        # x solves: the velocity field and the z solution in, the solution out
+        "dU_x_solve": 8.0 * N * (2 + 1), "dV_x_solve": 8.0 * N * (2 + 1) * fv, "dW_x_solve": 8.0 * N * (2 + 1),
+        # Poisson transforms: the real field (8 B a cell) and the half
+        # spectrum (16 B a mode, n1/2 + 1 modes a line); the step's forward x
+        # transform makes the right-hand side from U, V, W, dPhat
+        "fft_x_fwd": 8.0 * N + 16.0 * (n1 // 2 + 1) * n2 * n3,
+        "rhs_fft_x_fwd": 8.0 * N * 4 + 16.0 * (n1 // 2 + 1) * n2 * n3,
+        "fft_z<-1>": 32.0 * (n1 // 2 + 1) * n2 * n3, "fft_z<+1>": 32.0 * (n1 // 2 + 1) * n2 * n3,
+        "fft_x_inv": 8.0 * N + 16.0 * (n1 // 2 + 1) * n2 * n3, "dp_subtract": 16.0 * N,
     }
     return table.get(label)
 

@@ -188,12 +188,28 @@ __global__ void __launch_bounds__(256) k_pressure_rhs(PressureRHS cell, double *
     }
 }
 
+// Sources of the x r2c: the right-hand side read from the interior-layout
+// array, or made where it is needed by the right-hand side's cell formula
+// (no round trip through memory)
+struct RhsArray {
+    Grid g;
+    const double *prhs;
+    __device__ __forceinline__ double operator()(int i, int j, int k) const { return prhs[g.r3(i, j, k)]; }
+};
+struct RhsCell {
+    PressureRHS cell;
+    __device__ __forceinline__ double operator()(int i, int j, int k) const {
+        double o, x;
+        cell.eval<double>(RawW(), i, j, k, o, x, x, x);
+        return o;
+    }
+};
+
 // x r2c: TX k-lines (k0..k0+TX-1) of row j per workgroup.
 // spec[(k + n3m*(kx + h1p*(j-1)))] complex, k 0-based.
-template <int NT>
+template <int NT, class Src>
 __global__ void __launch_bounds__(NT)
-k_fft_x_fwd(Grid g, int h1p, int TX, const double *__restrict__ prhs, const double2 *__restrict__ tw,
-            double2 *__restrict__ spec) {
+k_fft_x_fwd(Grid g, int h1p, int TX, Src src, const double2 *__restrict__ tw, double2 *__restrict__ spec) {
     extern __shared__ double lds[];
     const int n = g.n1m;
     double *re = lds, *im = lds + (size_t)TX * n;
@@ -202,7 +218,7 @@ k_fft_x_fwd(Grid g, int h1p, int TX, const double *__restrict__ prhs, const doub
     const int tid = threadIdx.x;
     for (int e = tid; e < TX * n; e += NT) {
         const int l = e / n, t = e - l * n;
-        re[e] = prhs[g.r3(t + 1, j, k0 + l + 1)];
+        re[e] = src(t + 1, j, k0 + l + 1);
         im[e] = 0.0;
     }
     __syncthreads();
@@ -616,13 +632,17 @@ static int fft_tile(int n1m, int n3m) {
     return tx;
 }
 
-// r2c along x of the interior-layout field `in` into the spectrum
-static int fft_x_fwd_launch(tcs_ctx *c, const char *label, const double *in, double2 *spec) {
+// r2c along x of the interior-layout values src(i, j, k) into the spectrum
+template <class Src>
+static int fft_x_fwd_src_launch(tcs_ctx *c, const char *label, const Src &src, double2 *spec) {
     const int TX = fft_tile(c->n1m, c->n3m);
     const size_t lds = (size_t)TX * c->n1m * 2 * sizeof(double);
-    TCS_LAUNCH(c->stream, label, k_fft_x_fwd<FFT_X_THREADS>, dim3(c->n3m / TX, c->n2m), dim3(FFT_X_THREADS), lds,
-               c->grid(), c->h1p, TX, in, (const double2 *)c->tw1, spec);
+    TCS_LAUNCH(c->stream, label, (k_fft_x_fwd<FFT_X_THREADS, Src>), dim3(c->n3m / TX, c->n2m), dim3(FFT_X_THREADS),
+               lds, c->grid(), c->h1p, TX, src, (const double2 *)c->tw1, spec);
     return TCS_OK;
+}
+static int fft_x_fwd_launch(tcs_ctx *c, const char *label, const double *in, double2 *spec) {
+    return fft_x_fwd_src_launch(c, label, RhsArray{c->grid(), in}, spec);
 }
 
 // c2r along x into the full-layout field `out`, with the per-line sums
@@ -656,19 +676,23 @@ static int fft_z_launch(tcs_ctx *c, const char *label, double2 *spec) {
     return TCS_OK;
 }
 
+static PressureRHS rhs_cell(tcs_ctx *c, double dt) {
+    return PressureRHS{c->grid(),        c->phys,           dt, c->f[TCS_U], c->f[TCS_V], c->f[TCS_W], c->f[TCS_DPHAT],
+                       c->f[TCS_INVRHO], c->wall[TCS_VBCUP], c->wall[TCS_VBCBT]};
+}
+
 int pressure_rhs(tcs_ctx *c, double dt) {
     const dim3 blk = interior_block(), grd = cell_grid(c->n1m, c->n2m, c->n3m, RHS_KB);
-    const PressureRHS cell{c->grid(),         c->phys,          dt, c->f[TCS_U], c->f[TCS_V], c->f[TCS_W], c->f[TCS_DPHAT],
-                           c->f[TCS_INVRHO], c->wall[TCS_VBCUP], c->wall[TCS_VBCBT]};
-    TCS_LAUNCH(c->stream, "pressure_rhs", k_pressure_rhs, grd, blk, 0, cell, c->prhs);
+    TCS_LAUNCH(c->stream, "pressure_rhs", k_pressure_rhs, grd, blk, 0, rhs_cell(c, dt), c->prhs);
     return TCS_OK;
 }
 
-int pressure_poisson(tcs_ctx *c) {
+// the solve after the x r2c (in spec_x): z transform, y solves, inverse
+// transforms into dP, mean removal, wall extrapolation
+static int poisson_after_x(tcs_ctx *c) {
     const Grid g = c->grid();
     const int n1m = c->n1m, n2m = c->n2m, n3m = c->n3m, h1p = c->h1p;
     double2 *spec = (double2 *)c->spec_x;
-    TCS_TRY(fft_x_fwd_launch(c, "fft_x_fwd", c->prhs, spec));
     TCS_TRY(fft_z_launch<-1>(c, "fft_z<-1>", spec));
     const int nsys = n3m * h1p;
     int P, w1, w2;
@@ -693,6 +717,18 @@ int pressure_poisson(tcs_ctx *c) {
     TCS_LAUNCH(c->stream, "dp_subtract", k_dp_subtract, grd, blk, 0, g, c->scal, c->f[TCS_DP]);
     TCS_LAUNCH(c->stream, "wall_extrap", k_wall_extrap, dim3((n1m + 63) / 64, n3m), dim3(64), 0, g, c->f[TCS_DP]);
     return TCS_OK;
+}
+
+int pressure_poisson(tcs_ctx *c) {
+    TCS_TRY(fft_x_fwd_launch(c, "fft_x_fwd", c->prhs, (double2 *)c->spec_x));
+    return poisson_after_x(c);
+}
+
+// pressure_rhs and pressure_poisson in one: the x r2c makes the right-hand
+// side as it loads it (the interior array is not written)
+int pressure_poisson_rhs(tcs_ctx *c, double dt) {
+    TCS_TRY(fft_x_fwd_src_launch(c, "rhs_fft_x_fwd", RhsCell{rhs_cell(c, dt)}, (double2 *)c->spec_x));
+    return poisson_after_x(c);
 }
 
 int pressure_projection(tcs_ctx *c, double dt) {

@@ -845,8 +845,8 @@ extern "C" int tcs_step(tcs_ctx *c, double dt) {
     RUN(H, halo(c, c->f[TCS_U]));
     RUN(H, halo(c, c->f[TCS_V]));
     RUN(H, halo(c, c->f[TCS_W]));
-    RUN("pressure RHS", FORM(c, pressure_rhs, dt));
-    RUN("pressure PoissonSolve", FORM(c, pressure_poisson));
+    // the right-hand side is made inside the solve's x transform
+    RUN("pressure PoissonSolve", FORM(c, pressure_poisson_rhs, dt));
     RUN(H, halo(c, c->f[TCS_DP]));
     RUN("pressure projection", FORM(c, pressure_projection_into, dt, c->spare[TCS_DPHAT], c->spare[TCS_DP0]));
     std::swap(c->f[TCS_DPHAT], c->spare[TCS_DPHAT]);

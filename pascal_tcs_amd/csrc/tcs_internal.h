@@ -220,6 +220,7 @@ inline int ptdma_sweep(const tcs_ctx *c, float *A, const float *B, float *C, flo
     int momentum_pseudoupdate(tcs_ctx *c);                                              \
     int pressure_rhs(tcs_ctx *c, double dt);                                            \
     int pressure_poisson(tcs_ctx *c);                                                   \
+    int pressure_poisson_rhs(tcs_ctx *c, double dt);                                    \
     int pressure_projection(tcs_ctx *c, double dt);                                     \
     int thermal_solver_into(tcs_ctx *c, double dt, double *Tn, bool *written);          \
     int momentum_blockLdV_pseudo(tcs_ctx *c, double dt, double *Vn);                    \
