@@ -213,8 +213,10 @@ k_fft_x_fwd(Grid g, int h1p, int TX, Src src, const double2 *__restrict__ tw, do
     extern __shared__ double lds[];
     const int n = g.n1m;
     double *re = lds, *im = lds + (size_t)TX * n;
-    const int j = blockIdx.y + 1;
-    const int k0 = blockIdx.x * TX;  // 0-based
+    int kb, jb;
+    xcd_tiles(kb, jb);  // neighbouring rows j on one XCD: the right-hand side's stencil
+    const int j = jb + 1;
+    const int k0 = kb * TX;  // 0-based
     const int tid = threadIdx.x;
     for (int e = tid; e < TX * n; e += NT) {
         const int l = e / n, t = e - l * n;
