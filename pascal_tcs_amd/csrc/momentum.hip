@@ -839,12 +839,15 @@ static int launch_stale(tcs_ctx *c) {
     return TCS_OK;
 }
 
-// y stage of a momentum increment: rows j = jlo..n2m stored into the field F
-template <class Row>
+// y stage of a momentum increment: rows j = jlo..n2m stored into the field
+// F; SX lines per workgroup of the split solve (measured at 512^3: the
+// light dU and dV y rows run faster in two workgroups of 8 lines per CU,
+// the dW rows in one of 16; FP32 lines keep 16, 64-byte segments)
+template <int SX, class Row>
 static int y_stage(tcs_ctx *c, const char *label_rows, const char *label_solve, const Row &row, int jlo, double *F) {
     scr_t *am = SCR(c->am), *ac = SCR(c->ac), *ap = SCR(c->ap), *rhs = SCR(c->rhs);
     const int nrows = c->n2m - jlo + 1;
-    if (rows_in_solve<scr_t>(c, nrows)) return y_stage_in_solve(c, label_solve, row, jlo, nrows, rhs, F, 1);
+    if (rows_in_solve<scr_t>(c, nrows)) return y_stage_in_solve<SX>(c, label_solve, row, jlo, nrows, rhs, F, 1);
     TCS_TRY(rows3d_launch<false>(c->stream, label_rows, row, jlo, am, ac, ap, rhs));
     return sweep_y_field(c, jlo, nrows, F, 1);
 }
@@ -858,7 +861,7 @@ int momentum_solvedU(tcs_ctx *c, double dt) {
     TCS_TRY(z_stage(c, "dU_z", "dU_z_solve", z, 1, am, ac, ap, rhs));
     TCS_TRY(x_stage(c, "dU_x", "dU_x_solve", DUX{g, c->phys, dt, f[TCS_U], f[TCS_MU], f[TCS_INVRHO]}, 1, am, ac, ap,
                     rhs));
-    return y_stage(c, "dU_y", "dU_y_solve", DUY{g, c->phys, dt, f[TCS_V], f[TCS_MU], f[TCS_INVRHO]}, 1, f[TCS_DU]);
+    return y_stage<sizeof(scr_t) == 8 ? 8 : 16>(c, "dU_y", "dU_y_solve", DUY{g, c->phys, dt, f[TCS_V], f[TCS_MU], f[TCS_INVRHO]}, 1, f[TCS_DU]);
 }
 
 int momentum_solvedV(tcs_ctx *c, double dt) {
@@ -873,7 +876,7 @@ int momentum_solvedV(tcs_ctx *c, double dt) {
     TCS_TRY(x_stage(c, "dV_x", "dV_x_solve", DVX{g, c->phys, dt, f[TCS_U], f[TCS_INVRHO], c->stale}, 2, am, ac, ap,
                     rhs));
     TCS_LAUNCH(c->stream, "dV_zero_rows", k_dV_zero_rows, dim3((c->N1 + 63) / 64, c->N3), dim3(64), 0, g, f[TCS_DV]);
-    return y_stage(c, "dV_y", "dV_y_solve", DVY{g, c->phys, dt, f[TCS_V], f[TCS_MU], f[TCS_INVRHO]}, 2, f[TCS_DV]);
+    return y_stage<sizeof(scr_t) == 8 ? 8 : 16>(c, "dV_y", "dV_y_solve", DVY{g, c->phys, dt, f[TCS_V], f[TCS_MU], f[TCS_INVRHO]}, 2, f[TCS_DV]);
 }
 
 int momentum_solvedW(tcs_ctx *c, double dt) {
@@ -886,7 +889,7 @@ int momentum_solvedW(tcs_ctx *c, double dt) {
     TCS_TRY(z_stage(c, "dW_z", "dW_z_solve", z, 1, am, ac, ap, rhs));
     TCS_TRY(x_stage(c, "dW_x", "dW_x_solve", DWX{g, c->phys, dt, f[TCS_U], f[TCS_INVRHO], c->stale}, 1, am, ac, ap,
                     rhs));
-    return y_stage(c, "dW_y", "dW_y_solve", DWY{g, c->phys, dt, f[TCS_V], f[TCS_INVRHO], c->stale}, 1, f[TCS_DW]);
+    return y_stage<16>(c, "dW_y", "dW_y_solve", DWY{g, c->phys, dt, f[TCS_V], f[TCS_INVRHO], c->stale}, 1, f[TCS_DW]);
 }
 
 int momentum_blockLdV(tcs_ctx *c, double dt) {

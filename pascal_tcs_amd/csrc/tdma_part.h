@@ -311,21 +311,22 @@ __device__ __forceinline__ void xcd_tiles(int &bx, int &by) {
 
 // Lines strided in memory whose rows are computed, not read: src(s0, s1,
 // row, a, b, c, d) gives row `row` of line (s0, s1) (s0 < n0 along the
-// adjacent lines, s1 < gridDim.y; workgroups in xcd_tiles order).  Each row
+// adjacent lines, s1 < gridDim.y; workgroups of SX lines in xcd_tiles
+// order).  Each row
 // enters the forward elimination
 // as soon as it is made, so no coefficient array goes through memory; the
 // thread layout, the reduced solve and the output (D at s0 + s1*ss1 +
 // row*rs, or a field) are those of the split solve of stored lines.
-template <typename T, bool CYC, class Src>
-__global__ void __launch_bounds__(PT_SX * PT_PMAX)
+template <typename T, bool CYC, class Src, int SX>
+__global__ void __launch_bounds__(SX * PT_PMAX)
 k_ptdma_part_src(Src src, T *D, int n0, long ss1, long rs, int P, int w1, int w2, LineOut out) {
     using namespace part;
-    __shared__ RedLDS<T, PT_SX, 1> red;
+    __shared__ RedLDS<T, SX, 1> red;
     const int t = threadIdx.x;
-    const int line = t % PT_SX, p = t / PT_SX;
+    const int line = t % SX, p = t / SX;
     int bx, by;
     xcd_tiles(bx, by);
-    const int s0 = bx * PT_SX + line, s1 = by;
+    const int s0 = bx * SX + line, s1 = by;
     const bool act = s0 < n0;
     int j0, m;
     range(p, w1, w2, j0, m);
@@ -343,7 +344,7 @@ k_ptdma_part_src(Src src, T *D, int n0, long ss1, long rs, int P, int w1, int w2
     }
     reduce_back<T, 1>(a, c, d, m);
     T xf[1], xl[1];
-    exchange<T, CYC, PT_SX, 1>(red, line, p, P, a, c, d, m, xf, xl);
+    exchange<T, CYC, SX, 1>(red, line, p, P, a, c, d, m, xf, xl);
     if (!act) return;
     store<T>(a, c, d, j0, m, xf[0], xl[0], D, (long)s0 + (long)s1 * ss1, rs, out, s0, s1);
 }
@@ -459,8 +460,10 @@ k_ptdma_part_rows_src(Src src, T *D, int n0, long ss0, long ss1, int n_row, int 
 }
 
 // Launch of k_ptdma_part_src over n0 x n1 lines of n_row rows (n_row in
-// [2, 512]); returns TCS_E_ARG outside the split range.
-template <bool CYC, typename T, class Src>
+// [2, 512]), SX lines per workgroup (16: four 128-byte row segments per
+// wavefront load; 8: 64-byte segments, but two workgroups per CU); returns
+// TCS_E_ARG outside the split range.
+template <bool CYC, int SX = PT_SX, typename T, class Src>
 int ptdma_part_src_launch(hipStream_t st, const char *label, const Src &src, T *D, int n0, int n1, long ss1,
                           int n_row, long rs, const LineOut &out = LineOut()) {
     int P, w1, w2;
@@ -469,8 +472,8 @@ int ptdma_part_src_launch(hipStream_t st, const char *label, const Src &src, T *
         set_error("ptdma_part: n_row must be in [2, 512] and the outer line count <= 65535");
         return TCS_E_ARG;
     }
-    const dim3 blk(PT_SX * P), grd((unsigned)((n0 + PT_SX - 1) / PT_SX), (unsigned)n1);
-    TCS_LAUNCH(st, label, (k_ptdma_part_src<T, CYC, Src>), grd, blk, 0, src, D, n0, ss1, rs, P, w1, w2, out);
+    const dim3 blk(SX * P), grd((unsigned)((n0 + SX - 1) / SX), (unsigned)n1);
+    TCS_LAUNCH(st, label, (k_ptdma_part_src<T, CYC, Src, SX>), grd, blk, 0, src, D, n0, ss1, rs, P, w1, w2, out);
     return TCS_OK;
 }
 
