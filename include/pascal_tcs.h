@@ -107,10 +107,17 @@ void *tcs_stream(tcs_ctx *ctx);
 /* ---- module arrays ----------------------------------------------------- */
 int tcs_field_upload(tcs_ctx *ctx, int32_t id, const double *host);
 int tcs_field_download(tcs_ctx *ctx, int32_t id, double *host);
-/* device address of a field (full layout); INVRHO/MU return the 2-D table.
-   Valid until the next tcs_step, which writes the updates of T, U, V, W,
-   dPhat and dP0 into second buffers and exchanges them. */
+/* device address of a field; INVRHO/MU return the 2-D (j,k) table.  On the
+   device a full field's rows are tcs_field_pitch elements apart (element
+   (i,j,k) at dptr[i + pitch*(j + N2*k)], N2 = n2m+2; the pitch is N1 = n1m+2
+   rounded up to a multiple of 16, so every row's interior starts on a
+   128-byte line); the upload/download entries take the reference's
+   unpadded layout.  Valid until the next tcs_step, which writes the updates
+   of T, U, V, W, dPhat and dP0 into second buffers and exchanges them. */
 int tcs_field_device_ptr(tcs_ctx *ctx, int32_t id, double **dptr);
+/* row pitch (elements) of the full fields on the device (no counterpart in
+   the reference: the device layout of tcs_field_device_ptr) */
+int tcs_field_pitch(tcs_ctx *ctx, int32_t *pitch);
 int tcs_wall_upload(tcs_ctx *ctx, int32_t id, const double *host);
 int tcs_wall_download(tcs_ctx *ctx, int32_t id, double *host);
 

@@ -639,12 +639,14 @@ static int adjoint_alloc(tcs_ctx *c) {
     if (c->adj) return TCS_OK;
     AdjWork *w = new AdjWork();
     c->adj = w;
+    // full fields get the step's layout (FIELD_FRONT elements in front)
     auto get = [&](double **p, size_t n) -> int {
+        const size_t front = n == c->nfull ? FIELD_FRONT + 1 : 0;
         void *q = nullptr;
-        if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(double)) != hipSuccess) return TCS_E_HIP;
+        if (hipMalloc(&q, (std::max<size_t>(n, 1) + front) * sizeof(double)) != hipSuccess) return TCS_E_HIP;
         w->all.push_back(q);
-        if (hipMemset(q, 0, std::max<size_t>(n, 1) * sizeof(double)) != hipSuccess) return TCS_E_HIP;
-        *p = (double *)q;
+        if (hipMemset(q, 0, (std::max<size_t>(n, 1) + front) * sizeof(double)) != hipSuccess) return TCS_E_HIP;
+        *p = (double *)q + (front ? FIELD_FRONT : 0);
         return TCS_OK;
     };
     int rc = TCS_OK;
@@ -715,6 +717,14 @@ extern "C" int tcs_adjoint_copy(tcs_ctx *c, int32_t kind, int32_t id, int32_t to
     if (!a) {
         set_error("tcs_adjoint_copy: kind 0 takes a state field id (T, U, V, W, P, dPhat, dP0), kind 1 a wall id");
         return TCS_E_ARG;
+    }
+    if (kind == 0) {
+        // dev holds the reference layout (rows N1 apart), the adjoint field rows P1 apart
+        const size_t pd = (size_t)c->N1 * sizeof(double), pa = (size_t)c->P1 * sizeof(double);
+        const size_t rows = (size_t)c->N2 * c->N3;
+        if (to_adjoint) TCS_HIP(hipMemcpy2DAsync(a, pa, dev, pd, pd, rows, hipMemcpyDeviceToDevice, c->stream));
+        else TCS_HIP(hipMemcpy2DAsync(dev, pd, a, pa, pd, rows, hipMemcpyDeviceToDevice, c->stream));
+        return TCS_OK;
     }
     if (to_adjoint) TCS_HIP(hipMemcpyAsync(a, dev, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     else TCS_HIP(hipMemcpyAsync(dev, a, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));

@@ -362,22 +362,6 @@ __global__ void k_set_row(Grid g, int jrow, const double *__restrict__ w, double
     f[g.ix(i, jrow, k)] = w[g.b2(i, k)];
 }
 
-// expand a (j,k) table to the full field layout (row broadcast along i)
-__global__ void k_expand_table(Grid g, const double *__restrict__ t, double *__restrict__ out) {
-    const size_t n = (size_t)g.N1 * g.N2 * g.N3;
-    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
-        const size_t r = q / g.N1;
-        out[q] = t[r];
-    }
-}
-
-// the table of a full array's i = n1sub column
-__global__ void k_gather_table(Grid g, const double *__restrict__ full, double *__restrict__ t) {
-    const size_t n = (size_t)g.N2 * g.N3;
-    for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (size_t)gridDim.x * blockDim.x)
-        t[r] = full[(size_t)(g.N1 - 1) + (size_t)g.N1 * r];
-}
-
 }  // namespace tcs
 
 using namespace tcs;
@@ -430,6 +414,19 @@ static int dalloc(double **p, size_t n) {
     return TCS_OK;
 }
 
+// a full field: nfull elements after FIELD_FRONT, the allocation 128-byte
+// aligned (hipMalloc aligns to at least 256 bytes)
+static int falloc(double **p, size_t nfull) {
+    double *b = nullptr;
+    TCS_TRY(dalloc(&b, nfull + FIELD_FRONT + 1));
+    *p = b + FIELD_FRONT;
+    return TCS_OK;
+}
+static void ffree(double *p) {
+    if (p) (void)hipFree(p - FIELD_FRONT);
+}
+static bool is_table(int q) { return q == TCS_INVRHO || q == TCS_MU; }
+
 static void free_ctx(tcs_ctx *c) {
     if (!c) return;
     adjoint_free(c);
@@ -439,8 +436,12 @@ static void free_ctx(tcs_ctx *c) {
     for (double **b : bufs)
         if (*b) (void)hipFree(*b);
     for (int q = 0; q < TCS_NFIELD; q++) {
-        if (c->f[q]) (void)hipFree(c->f[q]);
-        if (c->spare[q]) (void)hipFree(c->spare[q]);
+        if (is_table(q)) {
+            if (c->f[q]) (void)hipFree(c->f[q]);
+        } else {
+            ffree(c->f[q]);
+            ffree(c->spare[q]);
+        }
     }
     for (int q = 0; q < TCS_NWALL; q++)
         if (c->wall[q]) (void)hipFree(c->wall[q]);
@@ -544,7 +545,8 @@ extern "C" int tcs_init(const tcs_params *prm, const double *dx1, const double *
     c->n1m = prm->n1m; c->n2m = prm->n2m; c->n3m = prm->n3m;
     c->tdma_split = prm->tdma_split;
     c->N1 = c->n1m + 2; c->N2 = c->n2m + 2; c->N3 = c->n3m + 2;
-    c->nfull = (size_t)c->N1 * c->N2 * c->N3;
+    c->P1 = field_pitch(c->N1);
+    c->nfull = (size_t)c->P1 * c->N2 * c->N3;
     c->nint = (size_t)c->n1m * c->n2m * c->n3m;
     c->nwall = (size_t)c->N1 * c->N3;
     c->ntab = (size_t)c->N2 * c->N3;
@@ -581,12 +583,10 @@ extern "C" int tcs_init(const tcs_params *prm, const double *dx1, const double *
     };
     for (auto &a : allocs)
         if ((rc = dalloc(a.p, a.n)) != TCS_OK) return fail(rc);
-    for (int q = 0; q < TCS_NFIELD; q++) {
-        const size_t n = (q == TCS_INVRHO || q == TCS_MU) ? c->ntab : c->nfull;
-        if ((rc = dalloc(&c->f[q], n)) != TCS_OK) return fail(rc);
-    }
+    for (int q = 0; q < TCS_NFIELD; q++)
+        if ((rc = is_table(q) ? dalloc(&c->f[q], c->ntab) : falloc(&c->f[q], c->nfull)) != TCS_OK) return fail(rc);
     for (int q : {TCS_T, TCS_U, TCS_V, TCS_W, TCS_DPHAT, TCS_DP0})
-        if ((rc = dalloc(&c->spare[q], c->nfull)) != TCS_OK) return fail(rc);
+        if ((rc = falloc(&c->spare[q], c->nfull)) != TCS_OK) return fail(rc);
     for (int q = 0; q < TCS_NWALL; q++)
         if ((rc = dalloc(&c->wall[q], c->nwall)) != TCS_OK) return fail(rc);
     if (hipMalloc((void **)&c->jC, c->N2 * sizeof(int)) != hipSuccess ||
@@ -668,7 +668,13 @@ extern "C" int tcs_field_upload(tcs_ctx *c, int32_t id, const double *host) {
         TCS_HIP(hipStreamSynchronize(c->stream));
         return TCS_OK;
     }
-    TCS_HIP(hipMemcpyAsync(c->f[id], host, c->nfull * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    // one contiguous transfer into a second buffer (free outside tcs_step),
+    // then the rows to their pitch on the device
+    const size_t row = (size_t)c->N1 * sizeof(double), pitch = (size_t)c->P1 * sizeof(double);
+    double *stage = c->spare[TCS_T];
+    TCS_HIP(hipMemcpyAsync(stage, host, row * c->N2 * c->N3, hipMemcpyHostToDevice, c->stream));
+    TCS_HIP(hipMemcpy2DAsync(c->f[id], pitch, stage, row, row, (size_t)c->N2 * c->N3, hipMemcpyDeviceToDevice,
+                             c->stream));
     TCS_HIP(hipStreamSynchronize(c->stream));
     return TCS_OK;
 }
@@ -684,7 +690,11 @@ extern "C" int tcs_field_download(tcs_ctx *c, int32_t id, double *host) {
             for (int i = 0; i < c->N1; i++) host[(size_t)i + (size_t)c->N1 * r] = t[r];
         return TCS_OK;
     }
-    TCS_HIP(hipMemcpyAsync(host, c->f[id], c->nfull * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    const size_t row = (size_t)c->N1 * sizeof(double), pitch = (size_t)c->P1 * sizeof(double);
+    double *stage = c->spare[TCS_T];
+    TCS_HIP(hipMemcpy2DAsync(stage, row, c->f[id], pitch, row, (size_t)c->N2 * c->N3, hipMemcpyDeviceToDevice,
+                             c->stream));
+    TCS_HIP(hipMemcpyAsync(host, stage, row * c->N2 * c->N3, hipMemcpyDeviceToHost, c->stream));
     TCS_HIP(hipStreamSynchronize(c->stream));
     return TCS_OK;
 }
@@ -693,6 +703,12 @@ extern "C" int tcs_field_device_ptr(tcs_ctx *c, int32_t id, double **dptr) {
     TCS_TRY(check_field(c, id));
     if (!dptr) return TCS_E_ARG;
     *dptr = c->f[id];
+    return TCS_OK;
+}
+
+extern "C" int tcs_field_pitch(tcs_ctx *c, int32_t *pitch) {
+    if (!c || !pitch) return TCS_E_ARG;
+    *pitch = c->P1;
     return TCS_OK;
 }
 

@@ -152,10 +152,10 @@ int y_stage_in_solve(tcs_ctx *c, const char *label, const Row &row, int jlo, int
     if (F) {
         out.F = F;
         out.mode = mode;
-        out.base = 1 + (long)c->N1 * jlo + (long)c->N1 * c->N2;  // (i, j, k) = (1, jlo, 1)
+        out.base = 1 + (long)c->P1 * jlo + (long)c->P1 * c->N2;  // (i, j, k) = (1, jlo, 1)
         out.ss0 = 1;
-        out.ss1 = (long)c->N1 * c->N2;
-        out.rs = c->N1;
+        out.ss1 = (long)c->P1 * c->N2;
+        out.rs = c->P1;
     }
     return ptdma_part_src_launch<false, SX>(c->stream, label, YLineRows<Row, S>{row, rhs, jlo}, rhs + off, c->n1m,
                                         c->n3m, n12, nrows, n1, out);

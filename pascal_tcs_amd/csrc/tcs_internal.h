@@ -56,16 +56,22 @@ void prof_end(hipStream_t s);
     } while (0)
 
 // Grid description passed by value to every stencil kernel.
-// Full fields: (0:n1m+1, 0:n2m+1, 0:n3m+1), i fastest.
+// Full fields: (0:n1m+1, 0:n2m+1, 0:n3m+1), i fastest, rows P1 >= N1
+// elements apart (a multiple of 16) and each field allocated FIELD_FRONT
+// elements after a 128-byte boundary, so the interior run i = 1.. of every
+// row starts on a 128-byte line: a 16-cell row segment is one cache line.
 // Interior work arrays: (1:n1m, 1:n2m, 1:n3m), i fastest.
+constexpr int FIELD_FRONT = 15;
+inline int field_pitch(int N1) { return (N1 + 15) / 16 * 16; }
 struct Grid {
     int n1m, n2m, n3m;
     int N1, N2, N3;  // full extents n?m+2
+    int P1;          // row pitch of the full fields
     const double *dx1, *dx2, *dx3, *dmx1, *dmx2, *dmx3;
     const double *rdx1, *rdx2, *rdx3, *rdmx1, *rdmx2, *rdmx3;  // their reciprocals
     const int *jC, *jS;
     __device__ __forceinline__ size_t ix(int i, int j, int k) const {
-        return (size_t)i + (size_t)N1 * ((size_t)j + (size_t)N2 * (size_t)k);
+        return (size_t)i + (size_t)P1 * ((size_t)j + (size_t)N2 * (size_t)k);
     }
     __device__ __forceinline__ size_t b2(int i, int k) const { return (size_t)i + (size_t)N1 * (size_t)k; }
     __device__ __forceinline__ size_t t2(int j, int k) const { return (size_t)j + (size_t)N2 * (size_t)k; }
@@ -91,6 +97,7 @@ struct tcs_ctx {
     tcs::Phys phys;
     int n1m, n2m, n3m;
     int N1, N2, N3;
+    int P1;  // row pitch of the full fields (field_pitch(N1))
     size_t nfull, nint, nwall, ntab;
     int h1p, h3p;
     int device;
@@ -106,7 +113,9 @@ struct tcs_ctx {
 
     double *f[TCS_NFIELD];  // INVRHO / MU hold (0:n2sub,0:n3sub) tables
     // second buffers of T, U, V, W, dPhat, dP0: tcs_step writes their
-    // updates there (the reads of the old values stay undisturbed) and swaps
+    // updates there (the reads of the old values stay undisturbed) and swaps;
+    // outside tcs_step their contents are dead (field uploads and downloads
+    // stage through spare[TCS_T])
     double *spare[TCS_NFIELD];
     double *wall[TCS_NWALL];
 
@@ -135,6 +144,7 @@ struct tcs_ctx {
         tcs::Grid g;
         g.n1m = n1m; g.n2m = n2m; g.n3m = n3m;
         g.N1 = N1; g.N2 = N2; g.N3 = N3;
+        g.P1 = P1;
         g.dx1 = dx1; g.dx2 = dx2; g.dx3 = dx3;
         g.rdx1 = rmet; g.rdmx1 = rmet + N1; g.rdx2 = rmet + 2 * N1; g.rdmx2 = g.rdx2 + N2;
         g.rdx3 = g.rdx2 + 2 * N2; g.rdmx3 = g.rdx3 + N3;

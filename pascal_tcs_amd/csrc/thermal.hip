@@ -76,7 +76,7 @@ __device__ __forceinline__ void th_rhocp(const Phys &p, S T, S &irc, S &dirc) {
 // mpi_thermal_coeffi: materialise the four property arrays over the full field
 __global__ void k_thermal_coeffi(Grid g, Phys p, const double *__restrict__ T, double *__restrict__ KPP,
                                  double *__restrict__ dKPP, double *__restrict__ iRC, double *__restrict__ diRC) {
-    const size_t n = (size_t)g.N1 * g.N2 * g.N3;
+    const size_t n = (size_t)g.P1 * g.N2 * g.N3;  // the row padding too
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
         double k, dk, irc, dirc;
         th_kpp(p, T[q], k, dk);
@@ -394,10 +394,10 @@ int sweep_y_field(tcs_ctx *c, int j0, int nrows, double *F, int mode) {
         LineOut out;
         out.F = F;
         out.mode = mode;
-        out.base = 1 + (long)c->N1 * j0 + (long)c->N1 * c->N2;  // (i,j,k) = (1, j0, 1)
+        out.base = 1 + (long)c->P1 * j0 + (long)c->P1 * c->N2;  // (i,j,k) = (1, j0, 1)
         out.ss0 = 1;
-        out.ss1 = (long)c->N1 * c->N2;
-        out.rs = c->N1;
+        out.ss1 = (long)c->P1 * c->N2;
+        out.rs = c->P1;
         return ptdma_part_launch(SCR(c->am) + off, SCR(c->ac) + off, SCR(c->ap) + off, SCR(c->rhs) + off, c->n1m, 1,
                                  c->n3m, n12, nrows, n1, false, c->stream, out);
     }
@@ -460,10 +460,10 @@ int thermal_solver_into(tcs_ctx *c, double dt, double *Tn, bool *written) {
     out.F = Tn;
     out.Fin = T;
     out.mode = 3;
-    out.base = 1 + (long)c->N1 + (long)c->N1 * c->N2;  // (i, j, k) = (1, 1, 1)
+    out.base = 1 + (long)c->P1 + (long)c->P1 * c->N2;  // (i, j, k) = (1, 1, 1)
     out.ss0 = 1;
-    out.ss1 = (long)c->N1 * c->N2;
-    out.rs = c->N1;
+    out.ss1 = (long)c->P1 * c->N2;
+    out.rs = c->P1;
     TCS_TRY(ptdma_part_src_launch<false>(c->stream, "thermal_y_solve", YLineRows<ThermalY, scr_t>{ThermalY{g, c->phys, dt,
                                          T, c->f[TCS_V]}, rhs, 1}, rhs, c->n1m, c->n3m, n12, c->n2m, n1, out));
     *written = true;

@@ -99,9 +99,16 @@ class TCSSolver:
         _lib.check(self.lib.tcs_wall_upload(self.h, _lib.WALL_IDS[name], _ptr(a)), f"upload {name}")
 
     def device_ptr(self, name):
+        """Device address of a field: rows of full fields are
+        device_pitch() elements apart (the reference's layout padded)."""
         p = _dp()
         _lib.check(self.lib.tcs_field_device_ptr(self.h, _lib.FIELD_IDS[name], ctypes.byref(p)), "device_ptr")
         return ctypes.cast(p, ctypes.c_void_p).value
+
+    def device_pitch(self):
+        v = ctypes.c_int32()
+        _lib.check(self.lib.tcs_field_pitch(self.h, ctypes.byref(v)), "tcs_field_pitch")
+        return v.value
 
     def stream(self):
         return self.lib.tcs_stream(self.h)
