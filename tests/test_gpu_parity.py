@@ -185,3 +185,33 @@ def test_nusselt_conduction_profile_is_one():
     # the thermal boundary routine fills the wall arrays the same way
     s.mpi_thermal_boundary()
     assert s.nusselt() == (nb, nt)
+
+
+@pytest.mark.parametrize("shape", [(16, 12, 8), (32, 9, 4), (512, 3, 4)])
+def test_device_layout(shape):
+    """Fields on the device: rows tcs_field_pitch elements apart (N1 rounded
+    up to 16), the interior of every row on a 128-byte line; upload and
+    download take the unpadded layout and round-trip exactly."""
+    import torch
+    from pascal_tcs_amd.solver import TCSSolver
+    cfg = small_config(*shape)
+    s = TCSSolver(cfg)
+    n1 = cfg.n1m + 2
+    pitch = s.device_pitch()
+    assert pitch % 16 == 0 and n1 <= pitch < n1 + 16
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal(s.shape)
+    s.set("U", a)
+    assert np.array_equal(s.get("U"), a)
+    ptr = s.device_ptr("U")
+    assert (ptr + 8) % 128 == 0  # element (1, 0, 0)
+    nz, ny = s.shape[0], s.shape[1]
+
+    class _View:  # the field's device memory, nz*ny rows of `pitch` doubles
+        __cuda_array_interface__ = {"shape": (nz * ny * pitch,), "typestr": "<f8", "data": (ptr, False),
+                                    "version": 2}
+
+    torch.cuda.synchronize()
+    dev = torch.as_tensor(_View(), device="cuda").cpu().numpy().reshape(nz, ny, pitch)[:, :, :n1]
+    assert np.array_equal(dev, a)
+    s.close()
