@@ -54,6 +54,16 @@ struct RawW {
     __device__ __forceinline__ const double *operator()(int, const double *p) const { return p; }
 };
 
+// FP32 arithmetic on the FP64 fields (the mixed form's rows): each load
+// rounded to float
+struct F32Ptr {
+    const double *p;
+    __device__ __forceinline__ float operator[](size_t i) const { return (float)p[i]; }
+};
+struct F32W {
+    __device__ __forceinline__ F32Ptr operator()(int, const double *p) const { return F32Ptr{p}; }
+};
+
 // the cell's input set: (field, index) of every load, without repeats
 constexpr int AD_MAXIN = 128;
 struct Rec {

@@ -36,14 +36,14 @@ int sweep_y_field(tcs_ctx *c, int j0, int nrows, double *F, int mode);
 // mpi_momentum_coeffi (:145-173): Mu and invRho of one temperature
 template <class S>
 __device__ __forceinline__ void mu_ir_of_T(const Phys &p, S T, S &mu, S &ir) {
-    const S tmp1 = p.DeltaT * T;
+    const S tmp1 = sc<S>(p.DeltaT) * T;
     const S tmp2 = tmp1 * tmp1;
     const S tmp3 = tmp2 * tmp1;
     const S tmp4 = tmp3 * tmp1;
     const S tmp5 = tmp4 * tmp1;
-    mu = (p.a10 + p.a10 * (p.a11 * tmp1 + p.a12 * tmp2 + p.a13 * tmp3 + p.a14 * tmp4 + p.a15 * tmp5))
-       * (p.d10 + p.d10 * (p.d11 * tmp1 + p.d12 * tmp2 + p.d13 * tmp3 + p.d14 * tmp4 + p.d15 * tmp5)) / p.Mu0;
-    ir = p.Rho0 / (p.a10 + p.a10 * (p.a11 * tmp1 + p.a12 * tmp2 + p.a13 * tmp3 + p.a14 * tmp4 + p.a15 * tmp5));
+    mu = (sc<S>(p.a10) + sc<S>(p.a10) * (sc<S>(p.a11) * tmp1 + sc<S>(p.a12) * tmp2 + sc<S>(p.a13) * tmp3 + sc<S>(p.a14) * tmp4 + sc<S>(p.a15) * tmp5))
+       * (sc<S>(p.d10) + sc<S>(p.d10) * (sc<S>(p.d11) * tmp1 + sc<S>(p.d12) * tmp2 + sc<S>(p.d13) * tmp3 + sc<S>(p.d14) * tmp4 + sc<S>(p.d15) * tmp5)) / sc<S>(p.Mu0);
+    ir = sc<S>(p.Rho0) / (sc<S>(p.a10) + sc<S>(p.a10) * (sc<S>(p.a11) * tmp1 + sc<S>(p.a12) * tmp2 + sc<S>(p.a13) * tmp3 + sc<S>(p.a14) * tmp4 + sc<S>(p.a15) * tmp5));
 }
 
 // on the (j,k) table from T(n1sub,j,k)
@@ -66,24 +66,24 @@ __device__ __forceinline__ void stale_values(const Grid &g, FP Mu, S (&out)[6]) 
     const int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1, km = k - 1;
     // solvedV stage 1 (:661-665)
     {
-        const S mua = 0.5 * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
-        const S muc = 0.5 * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
-        const S mub = 0.5 * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
-        out[0] = 0.5 * (dx1[im] * muc + dx1[i] * mua) / dmx1[i];
-        out[1] = 0.5 * (dx1[ip] * muc + dx1[i] * mub) / dmx1[ip];
+        const S mua = sc<S>(0.5) * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
+        const S muc = sc<S>(0.5) * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
+        const S mub = sc<S>(0.5) * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
+        out[0] = sc<S>(0.5) * (dx1[im] * muc + dx1[i] * mua) / dmx1[i];
+        out[1] = sc<S>(0.5) * (dx1[ip] * muc + dx1[i] * mub) / dmx1[ip];
     }
     // solvedW stage 1 (:1022-1032)
     {
-        S mua = 0.5 * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
-        S muc = 0.5 * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
-        S mub = 0.5 * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
-        out[2] = 0.5 * (dx1[im] * muc + dx1[i] * mua) / dmx1[i];
-        out[3] = 0.5 * (dx1[ip] * muc + dx1[i] * mub) / dmx1[ip];
-        mua = 0.5 * (dx3[km] * MU(jm, k) + dx3[k] * MU(jm, km)) / dmx3[k];
-        muc = 0.5 * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
-        mub = 0.5 * (dx3[km] * MU(jp, k) + dx3[k] * MU(jp, km)) / dmx3[k];
-        out[4] = 0.5 * (dx2[jm] * muc + dx2[j] * mua) / dmx2[j];
-        out[5] = 0.5 * (dx2[jp] * muc + dx2[j] * mub) / dmx2[jp];
+        S mua = sc<S>(0.5) * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
+        S muc = sc<S>(0.5) * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
+        S mub = sc<S>(0.5) * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
+        out[2] = sc<S>(0.5) * (dx1[im] * muc + dx1[i] * mua) / dmx1[i];
+        out[3] = sc<S>(0.5) * (dx1[ip] * muc + dx1[i] * mub) / dmx1[ip];
+        mua = sc<S>(0.5) * (dx3[km] * MU(jm, k) + dx3[k] * MU(jm, km)) / dmx3[k];
+        muc = sc<S>(0.5) * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
+        mub = sc<S>(0.5) * (dx3[km] * MU(jp, k) + dx3[k] * MU(jp, km)) / dmx3[k];
+        out[4] = sc<S>(0.5) * (dx2[jm] * muc + dx2[j] * mua) / dmx2[j];
+        out[5] = sc<S>(0.5) * (dx2[jp] * muc + dx2[j] * mub) / dmx2[jp];
     }
 }
 
@@ -105,8 +105,8 @@ __global__ void k_momentum_stale(Grid g, const double *__restrict__ Mu, double *
 
 // the metric arrays under their reference names, in a row routine
 #define ROW_METRICS                                                                                   \
-    const auto dx1 = metric(g.dx1, g.rdx1), dx2 = metric(g.dx2, g.rdx2), dx3 = metric(g.dx3, g.rdx3);     \
-    const auto dmx1 = metric(g.dmx1, g.rdmx1), dmx2 = metric(g.dmx2, g.rdmx2), dmx3 = metric(g.dmx3, g.rdmx3); \
+    const auto dx1 = metric<S>(g.dx1, g.rdx1), dx2 = metric<S>(g.dx2, g.rdx2), dx3 = metric<S>(g.dx3, g.rdx3);     \
+    const auto dmx1 = metric<S>(g.dmx1, g.rdmx1), dmx2 = metric<S>(g.dmx2, g.rdmx2), dmx3 = metric<S>(g.dmx3, g.rdmx3); \
     (void)dx1; (void)dx2; (void)dx3; (void)dmx1; (void)dmx2; (void)dmx3;
 
 // Each stage's row routine gives row (i, j, k) of the stage's tridiagonal
@@ -124,16 +124,16 @@ __device__ __forceinline__ void dU_z_row(const Grid &g, const Phys &p, double dt
     ROW_METRICS
     const int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1, km = k - 1, kp = k + 1;
     const int jum = g.jC[jm], jup = g.jC[jp];
-    const S Cmu = p.Cmu;
+    const S Cmu = sc<S>(p.Cmu);
 #define Uf(a, b, c) U[g.ix(a, b, c)]
 #define Vf(a, b, c) V[g.ix(a, b, c)]
 #define Wf(a, b, c) W[g.ix(a, b, c)]
-    const S u1 = 0.5 * (Uf(im, j, k) + Uf(i, j, k));
-    const S u2 = 0.5 * (Uf(ip, j, k) + Uf(i, j, k));
-    const S v3 = 0.5 * (dx1[im] * Vf(i, j, k) + dx1[i] * Vf(im, j, k)) / dmx1[i];
-    const S v4 = 0.5 * (dx1[im] * Vf(i, jp, k) + dx1[i] * Vf(im, jp, k)) / dmx1[i];
-    const S w5 = 0.5 * (dx1[im] * Wf(i, j, k) + dx1[i] * Wf(im, j, k)) / dmx1[i];
-    const S w6 = 0.5 * (dx1[im] * Wf(i, j, kp) + dx1[i] * Wf(im, j, kp)) / dmx1[i];
+    const S u1 = sc<S>(0.5) * (Uf(im, j, k) + Uf(i, j, k));
+    const S u2 = sc<S>(0.5) * (Uf(ip, j, k) + Uf(i, j, k));
+    const S v3 = sc<S>(0.5) * (dx1[im] * Vf(i, j, k) + dx1[i] * Vf(im, j, k)) / dmx1[i];
+    const S v4 = sc<S>(0.5) * (dx1[im] * Vf(i, jp, k) + dx1[i] * Vf(im, jp, k)) / dmx1[i];
+    const S w5 = sc<S>(0.5) * (dx1[im] * Wf(i, j, k) + dx1[i] * Wf(im, j, k)) / dmx1[i];
+    const S w6 = sc<S>(0.5) * (dx1[im] * Wf(i, j, kp) + dx1[i] * Wf(im, j, kp)) / dmx1[i];
 
     const S dudx1 = (Uf(i, j, k) - Uf(im, j, k)) / dx1[im];
     const S dudx2 = (Uf(ip, j, k) - Uf(i, j, k)) / dx1[i];
@@ -146,65 +146,65 @@ __device__ __forceinline__ void dU_z_row(const Grid &g, const Phys &p, double dt
     const S dwdx5 = (Wf(i, j, k) - Wf(im, j, k)) / dmx1[i];
     const S dwdx6 = (Wf(i, j, kp) - Wf(im, j, kp)) / dmx1[i];
 
-    S mua = 0.5 * (dx1[im] * MU(jm, k) + dx1[i] * MU(jm, k)) / dmx1[i];
-    S muc = 0.5 * (dx1[im] * MU(j, k) + dx1[i] * MU(j, k)) / dmx1[i];
-    S mub = 0.5 * (dx1[im] * MU(jp, k) + dx1[i] * MU(jp, k)) / dmx1[i];
-    const S mu3 = 0.5 * (dx2[jm] * muc + dx2[j] * mua) / dmx2[j];
-    const S mu4 = 0.5 * (dx2[jp] * muc + dx2[j] * mub) / dmx2[jp];
-    mua = 0.5 * (dx1[im] * MU(j, km) + dx1[i] * MU(j, km)) / dmx1[i];
-    muc = 0.5 * (dx1[im] * MU(j, k) + dx1[i] * MU(j, k)) / dmx1[i];
-    mub = 0.5 * (dx1[im] * MU(j, kp) + dx1[i] * MU(j, kp)) / dmx1[i];
-    const S mu5 = 0.5 * (dx3[km] * muc + dx3[k] * mua) / dmx3[k];
-    const S mu6 = 0.5 * (dx3[kp] * muc + dx3[k] * mub) / dmx3[kp];
+    S mua = sc<S>(0.5) * (dx1[im] * MU(jm, k) + dx1[i] * MU(jm, k)) / dmx1[i];
+    S muc = sc<S>(0.5) * (dx1[im] * MU(j, k) + dx1[i] * MU(j, k)) / dmx1[i];
+    S mub = sc<S>(0.5) * (dx1[im] * MU(jp, k) + dx1[i] * MU(jp, k)) / dmx1[i];
+    const S mu3 = sc<S>(0.5) * (dx2[jm] * muc + dx2[j] * mua) / dmx2[j];
+    const S mu4 = sc<S>(0.5) * (dx2[jp] * muc + dx2[j] * mub) / dmx2[jp];
+    mua = sc<S>(0.5) * (dx1[im] * MU(j, km) + dx1[i] * MU(j, km)) / dmx1[i];
+    muc = sc<S>(0.5) * (dx1[im] * MU(j, k) + dx1[i] * MU(j, k)) / dmx1[i];
+    mub = sc<S>(0.5) * (dx1[im] * MU(j, kp) + dx1[i] * MU(j, kp)) / dmx1[i];
+    const S mu5 = sc<S>(0.5) * (dx3[km] * muc + dx3[k] * mua) / dmx3[k];
+    const S mu6 = sc<S>(0.5) * (dx3[kp] * muc + dx3[k] * mub) / dmx3[kp];
 
-    const S invRhoc = 0.5 * (dx1[im] * IR(j, k) + dx1[i] * IR(j, k)) / dmx1[i];
-    const S invRhocCmu_half = 0.5 * Cmu * invRhoc;
+    const S invRhoc = sc<S>(0.5) * (dx1[im] * IR(j, k) + dx1[i] * IR(j, k)) / dmx1[i];
+    const S invRhocCmu_half = sc<S>(0.5) * Cmu * invRhoc;
 
-    const S viscous_u1 = 1.0 * (MU(j, k) * dudx2 - MU(j, k) * dudx1) / dmx1[i];
-    const S viscous_u2 = 1.0 * (mu4 * dudy4 - mu3 * dudy3) / dx2[j];
-    const S viscous_u3 = 1.0 * (mu6 * dudz6 - mu5 * dudz5) / dx3[k];
-    const S viscous_u12 = 1.0 * (mu4 * dvdx4 - mu3 * dvdx3) / dx2[j];
-    const S viscous_u13 = 1.0 * (mu6 * dwdx6 - mu5 * dwdx5) / dx3[k];
+    const S viscous_u1 = sc<S>(1.0) * (MU(j, k) * dudx2 - MU(j, k) * dudx1) / dmx1[i];
+    const S viscous_u2 = sc<S>(1.0) * (mu4 * dudy4 - mu3 * dudy3) / dx2[j];
+    const S viscous_u3 = sc<S>(1.0) * (mu6 * dudz6 - mu5 * dudz5) / dx3[k];
+    const S viscous_u12 = sc<S>(1.0) * (mu4 * dvdx4 - mu3 * dvdx3) / dx2[j];
+    const S viscous_u13 = sc<S>(1.0) * (mu6 * dwdx6 - mu5 * dwdx5) / dx3[k];
 
-    S rhs = invRhocCmu_half * (2.0 * viscous_u1 + viscous_u2 + viscous_u3 + viscous_u12 + viscous_u13);
-    rhs = rhs - p.Cmp * invRhoc * (P[g.ix(i, j, k)] - P[g.ix(im, j, k)]) / dmx1[i];
-    const S Tc = 0.5 * (dx1[im] * T[g.ix(i, j, k)] + dx1[i] * T[g.ix(im, j, k)]) / dmx1[i];
-    rhs = rhs + p.Cmt * (Tc + p.a12pera11 * (Tc * Tc) * p.DeltaT) * invRhoc;
+    S rhs = invRhocCmu_half * (sc<S>(2.0) * viscous_u1 + viscous_u2 + viscous_u3 + viscous_u12 + viscous_u13);
+    rhs = rhs - sc<S>(p.Cmp) * invRhoc * (P[g.ix(i, j, k)] - P[g.ix(im, j, k)]) / dmx1[i];
+    const S Tc = sc<S>(0.5) * (dx1[im] * T[g.ix(i, j, k)] + dx1[i] * T[g.ix(im, j, k)]) / dmx1[i];
+    rhs = rhs + sc<S>(p.Cmt) * (Tc + sc<S>(p.a12pera11) * (Tc * Tc) * sc<S>(p.DeltaT)) * invRhoc;
 
-    S ubc_down = 0.25 * v3 / dmx2[j] * UBCbt[g.b2(i, k)]
-                    - 0.25 * dudy3 * (dx1[i] * VBCbt[g.b2(im, k)] + dx1[im] * VBCbt[g.b2(i, k)]) / dmx1[i] * 0.5;
-    S ubc_up = -0.25 * v4 / dmx2[jp] * UBCup[g.b2(i, k)]
-                  - 0.25 * dudy4 * (dx1[i] * VBCup[g.b2(im, k)] + dx1[im] * VBCup[g.b2(i, k)]) / dmx1[i] * 0.5;
+    S ubc_down = sc<S>(0.25) * v3 / dmx2[j] * UBCbt[g.b2(i, k)]
+                    - sc<S>(0.25) * dudy3 * (dx1[i] * VBCbt[g.b2(im, k)] + dx1[im] * VBCbt[g.b2(i, k)]) / dmx1[i] * sc<S>(0.5);
+    S ubc_up = -sc<S>(0.25) * v4 / dmx2[jp] * UBCup[g.b2(i, k)]
+                  - sc<S>(0.25) * dudy4 * (dx1[i] * VBCup[g.b2(im, k)] + dx1[im] * VBCup[g.b2(i, k)]) / dmx1[i] * sc<S>(0.5);
     ubc_down = ubc_down + invRhocCmu_half / dx2[j] * mu3 / dmx2[j] * UBCbt[g.b2(i, k)];
     ubc_down = ubc_down - invRhocCmu_half / dx2[j] * mu3 / dmx1[i] * (VBCbt[g.b2(i, k)] - VBCbt[g.b2(im, k)]);
     ubc_up = ubc_up + invRhocCmu_half / dx2[j] * mu4 / dmx2[jp] * UBCup[g.b2(i, k)];
     ubc_up = ubc_up + invRhocCmu_half / dx2[j] * mu4 / dmx1[i] * (VBCup[g.b2(i, k)] - VBCup[g.b2(im, k)]);
-    rhs = rhs + (1.0 - (double)jum) * ubc_down + (1.0 - (double)jup) * ubc_up;
+    rhs = rhs + (sc<S>(1.0) - sc<S>(jum)) * ubc_down + (sc<S>(1.0) - sc<S>(jup)) * ubc_up;
 
-    const S mACI = invRhocCmu_half / dmx1[i] * (MU(j, k) / dx1[i] + MU(j, k) / dx1[im]) * 2.0
-                      + 0.25 * (dudx2 * 0.5 + dudx1 * 0.5 - u2 / dx1[i] + u1 / dx1[im]);
-    const S mAPI = -invRhocCmu_half / dmx1[i] * MU(j, k) / dx1[i] * 2.0 + 0.25 * (u2 / dx1[i] + dudx2 * 0.5);
-    const S mAMI = -invRhocCmu_half / dmx1[i] * MU(j, k) / dx1[im] * 2.0 + 0.25 * (-u1 / dx1[im] + dudx1 * 0.5);
-    const S mACJ = invRhocCmu_half / dx2[j] * (mu4 / dmx2[jp] + mu3 / dmx2[j]) + 0.25 * (-v4 / dmx2[jp] + v3 / dmx2[j]);
-    S mAPJ = -invRhocCmu_half / dx2[j] * mu4 / dmx2[jp] + 0.25 * (v4 / dmx2[jp]);
-    S mAMJ = -invRhocCmu_half / dx2[j] * mu3 / dmx2[j] + 0.25 * (-v3 / dmx2[j]);
-    mAPJ = mAPJ * (double)jup;
-    mAMJ = mAMJ * (double)jum;
-    const S mACK = invRhocCmu_half / dx3[k] * (mu6 / dmx3[kp] + mu5 / dmx3[k]) + 0.25 * (-w6 / dmx3[kp] + w5 / dmx3[k]);
-    const S mAPK = -invRhocCmu_half / dx3[k] * mu6 / dmx3[kp] + 0.25 * (w6 / dmx3[kp]);
-    const S mAMK = -invRhocCmu_half / dx3[k] * mu5 / dmx3[k] + 0.25 * (-w5 / dmx3[k]);
+    const S mACI = invRhocCmu_half / dmx1[i] * (MU(j, k) / dx1[i] + MU(j, k) / dx1[im]) * sc<S>(2.0)
+                      + sc<S>(0.25) * (dudx2 * sc<S>(0.5) + dudx1 * sc<S>(0.5) - u2 / dx1[i] + u1 / dx1[im]);
+    const S mAPI = -invRhocCmu_half / dmx1[i] * MU(j, k) / dx1[i] * sc<S>(2.0) + sc<S>(0.25) * (u2 / dx1[i] + dudx2 * sc<S>(0.5));
+    const S mAMI = -invRhocCmu_half / dmx1[i] * MU(j, k) / dx1[im] * sc<S>(2.0) + sc<S>(0.25) * (-u1 / dx1[im] + dudx1 * sc<S>(0.5));
+    const S mACJ = invRhocCmu_half / dx2[j] * (mu4 / dmx2[jp] + mu3 / dmx2[j]) + sc<S>(0.25) * (-v4 / dmx2[jp] + v3 / dmx2[j]);
+    S mAPJ = -invRhocCmu_half / dx2[j] * mu4 / dmx2[jp] + sc<S>(0.25) * (v4 / dmx2[jp]);
+    S mAMJ = -invRhocCmu_half / dx2[j] * mu3 / dmx2[j] + sc<S>(0.25) * (-v3 / dmx2[j]);
+    mAPJ = mAPJ * sc<S>(jup);
+    mAMJ = mAMJ * sc<S>(jum);
+    const S mACK = invRhocCmu_half / dx3[k] * (mu6 / dmx3[kp] + mu5 / dmx3[k]) + sc<S>(0.25) * (-w6 / dmx3[kp] + w5 / dmx3[k]);
+    const S mAPK = -invRhocCmu_half / dx3[k] * mu6 / dmx3[kp] + sc<S>(0.25) * (w6 / dmx3[kp]);
+    const S mAMK = -invRhocCmu_half / dx3[k] * mu5 / dmx3[k] + sc<S>(0.25) * (-w5 / dmx3[k]);
 
     rhs = rhs - (mAPI * Uf(ip, j, k) + mACI * Uf(i, j, k) + mAMI * Uf(im, j, k)
                  + mAPJ * Uf(i, jp, k) + mACJ * Uf(i, j, k) + mAMJ * Uf(i, jm, k)
                  + mAPK * Uf(i, j, kp) + mACK * Uf(i, j, k) + mAMK * Uf(i, j, km));
-    rhs = rhs - (0.25 * (v4 * (double)jup * dudy4 + v3 * (double)jum * dudy3)
-                 - invRhocCmu_half / dx2[j] * (mu4 * dvdx4 * (double)jup - mu3 * dvdx3 * (double)jum));
-    rhs = rhs - (0.25 * (w6 * dudz6 + w5 * dudz5) - invRhocCmu_half / dx3[k] * (mu6 * dwdx6 - mu5 * dwdx5));
+    rhs = rhs - (sc<S>(0.25) * (v4 * sc<S>(jup) * dudy4 + v3 * sc<S>(jum) * dudy3)
+                 - invRhocCmu_half / dx2[j] * (mu4 * dvdx4 * sc<S>(jup) - mu3 * dvdx3 * sc<S>(jum)));
+    rhs = rhs - (sc<S>(0.25) * (w6 * dudz6 + w5 * dudz5) - invRhocCmu_half / dx3[k] * (mu6 * dwdx6 - mu5 * dwdx5));
 
-    ac = mACK * dt + 1.0;
-    ap = mAPK * dt;
-    am = mAMK * dt;
-    rhsout = rhs * dt;
+    ac = mACK * sc<S>(dt) + sc<S>(1.0);
+    ap = mAPK * sc<S>(dt);
+    am = mAMK * sc<S>(dt);
+    rhsout = rhs * sc<S>(dt);
 }
 struct DUZ {
     Grid g;
@@ -212,6 +212,7 @@ struct DUZ {
     double dt;
     const double *U, *V, *W, *P, *T, *Mu, *iR, *UBCup, *UBCbt, *VBCup, *VBCbt;
     static constexpr int NF = 11;  // field inputs, in the order below
+    static constexpr bool F32 = false;  // FP32 rows in the mixed form
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         dU_z_row<S>(g, p, dt, w(0, U), w(1, V), w(2, W), w(3, P), w(4, T), w(5, Mu), w(6, iR), w(7, UBCup), w(8, UBCbt), w(9, VBCup), w(10, VBCbt), i, j, k, a, b, c, d);
@@ -228,19 +229,19 @@ __device__ __forceinline__ void dU_x_row(const Grid &g, const Phys &p, double dt
     int i, int j, int k, S &am, S &ac, S &ap) {
     ROW_METRICS
     const int im = i - 1, ip = i + 1;
-    const S u1 = 0.5 * (Uf(im, j, k) + Uf(i, j, k));
-    const S u2 = 0.5 * (Uf(ip, j, k) + Uf(i, j, k));
+    const S u1 = sc<S>(0.5) * (Uf(im, j, k) + Uf(i, j, k));
+    const S u2 = sc<S>(0.5) * (Uf(ip, j, k) + Uf(i, j, k));
     const S dudx1 = (Uf(i, j, k) - Uf(im, j, k)) / dx1[im];
     const S dudx2 = (Uf(ip, j, k) - Uf(i, j, k)) / dx1[i];
-    const S invRhoc = 0.5 * (dx1[im] * IR(j, k) + dx1[i] * IR(j, k)) / dmx1[i];
-    const S invRhocCmu_half = 0.5 * p.Cmu * invRhoc;
-    const S mACI = invRhocCmu_half / dmx1[i] * (MU(j, k) / dx1[i] + MU(j, k) / dx1[im]) * 2.0
-                      + 0.25 * (dudx2 * 0.5 + dudx1 * 0.5 - u2 / dx1[i] + u1 / dx1[im]);
-    const S mAPI = -invRhocCmu_half / dmx1[i] * MU(j, k) / dx1[i] * 2.0 + 0.25 * (u2 / dx1[i] + dudx2 * 0.5);
-    const S mAMI = -invRhocCmu_half / dmx1[i] * MU(j, k) / dx1[im] * 2.0 + 0.25 * (-u1 / dx1[im] + dudx1 * 0.5);
-    ac = mACI * dt + 1.0;
-    ap = mAPI * dt;
-    am = mAMI * dt;
+    const S invRhoc = sc<S>(0.5) * (dx1[im] * IR(j, k) + dx1[i] * IR(j, k)) / dmx1[i];
+    const S invRhocCmu_half = sc<S>(0.5) * sc<S>(p.Cmu) * invRhoc;
+    const S mACI = invRhocCmu_half / dmx1[i] * (MU(j, k) / dx1[i] + MU(j, k) / dx1[im]) * sc<S>(2.0)
+                      + sc<S>(0.25) * (dudx2 * sc<S>(0.5) + dudx1 * sc<S>(0.5) - u2 / dx1[i] + u1 / dx1[im]);
+    const S mAPI = -invRhocCmu_half / dmx1[i] * MU(j, k) / dx1[i] * sc<S>(2.0) + sc<S>(0.25) * (u2 / dx1[i] + dudx2 * sc<S>(0.5));
+    const S mAMI = -invRhocCmu_half / dmx1[i] * MU(j, k) / dx1[im] * sc<S>(2.0) + sc<S>(0.25) * (-u1 / dx1[im] + dudx1 * sc<S>(0.5));
+    ac = mACI * sc<S>(dt) + sc<S>(1.0);
+    ap = mAPI * sc<S>(dt);
+    am = mAMI * sc<S>(dt);
 }
 struct DUX {
     Grid g;
@@ -248,6 +249,7 @@ struct DUX {
     double dt;
     const double *U, *Mu, *iR;
     static constexpr int NF = 3;  // field inputs, in the order below
+    static constexpr bool F32 = false;  // FP32 rows in the mixed form
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         dU_x_row<S>(g, p, dt, w(0, U), w(1, Mu), w(2, iR), i, j, k, a, b, c);
@@ -265,23 +267,23 @@ __device__ __forceinline__ void dU_y_row(const Grid &g, const Phys &p, double dt
     ROW_METRICS
     const int im = i - 1, jm = j - 1, jp = j + 1;
     const int jum = g.jC[jm], jup = g.jC[jp];
-    const S v3 = 0.5 * (dx1[im] * Vf(i, j, k) + dx1[i] * Vf(im, j, k)) / dmx1[i];
-    const S v4 = 0.5 * (dx1[im] * Vf(i, jp, k) + dx1[i] * Vf(im, jp, k)) / dmx1[i];
-    const S mua = 0.5 * (dx1[im] * MU(jm, k) + dx1[i] * MU(jm, k)) / dmx1[i];
-    const S muc = 0.5 * (dx1[im] * MU(j, k) + dx1[i] * MU(j, k)) / dmx1[i];
-    const S mub = 0.5 * (dx1[im] * MU(jp, k) + dx1[i] * MU(jp, k)) / dmx1[i];
-    const S mu3 = 0.5 * (dx2[jm] * muc + dx2[j] * mua) / dmx2[j];
-    const S mu4 = 0.5 * (dx2[jp] * muc + dx2[j] * mub) / dmx2[jp];
-    const S invRhoc = 0.5 * (dx1[im] * IR(j, k) + dx1[i] * IR(j, k)) / dmx1[i];
-    const S invRhocCmu_half = 0.5 * p.Cmu * invRhoc;
-    const S mACJ = invRhocCmu_half / dx2[j] * (mu4 / dmx2[jp] + mu3 / dmx2[j]) + 0.25 * (-v4 / dmx2[jp] + v3 / dmx2[j]);
-    S mAPJ = -invRhocCmu_half / dx2[j] * mu4 / dmx2[jp] + 0.25 * (v4 / dmx2[jp]);
-    S mAMJ = -invRhocCmu_half / dx2[j] * mu3 / dmx2[j] + 0.25 * (-v3 / dmx2[j]);
-    mAPJ = mAPJ * (double)jup;
-    mAMJ = mAMJ * (double)jum;
-    ac = mACJ * dt + 1.0;
-    ap = mAPJ * dt;
-    am = mAMJ * dt;
+    const S v3 = sc<S>(0.5) * (dx1[im] * Vf(i, j, k) + dx1[i] * Vf(im, j, k)) / dmx1[i];
+    const S v4 = sc<S>(0.5) * (dx1[im] * Vf(i, jp, k) + dx1[i] * Vf(im, jp, k)) / dmx1[i];
+    const S mua = sc<S>(0.5) * (dx1[im] * MU(jm, k) + dx1[i] * MU(jm, k)) / dmx1[i];
+    const S muc = sc<S>(0.5) * (dx1[im] * MU(j, k) + dx1[i] * MU(j, k)) / dmx1[i];
+    const S mub = sc<S>(0.5) * (dx1[im] * MU(jp, k) + dx1[i] * MU(jp, k)) / dmx1[i];
+    const S mu3 = sc<S>(0.5) * (dx2[jm] * muc + dx2[j] * mua) / dmx2[j];
+    const S mu4 = sc<S>(0.5) * (dx2[jp] * muc + dx2[j] * mub) / dmx2[jp];
+    const S invRhoc = sc<S>(0.5) * (dx1[im] * IR(j, k) + dx1[i] * IR(j, k)) / dmx1[i];
+    const S invRhocCmu_half = sc<S>(0.5) * sc<S>(p.Cmu) * invRhoc;
+    const S mACJ = invRhocCmu_half / dx2[j] * (mu4 / dmx2[jp] + mu3 / dmx2[j]) + sc<S>(0.25) * (-v4 / dmx2[jp] + v3 / dmx2[j]);
+    S mAPJ = -invRhocCmu_half / dx2[j] * mu4 / dmx2[jp] + sc<S>(0.25) * (v4 / dmx2[jp]);
+    S mAMJ = -invRhocCmu_half / dx2[j] * mu3 / dmx2[j] + sc<S>(0.25) * (-v3 / dmx2[j]);
+    mAPJ = mAPJ * sc<S>(jup);
+    mAMJ = mAMJ * sc<S>(jum);
+    ac = mACJ * sc<S>(dt) + sc<S>(1.0);
+    ap = mAPJ * sc<S>(dt);
+    am = mAMJ * sc<S>(dt);
 }
 struct DUY {
     Grid g;
@@ -289,6 +291,7 @@ struct DUY {
     double dt;
     const double *V, *Mu, *iR;
     static constexpr int NF = 3;  // field inputs, in the order below
+    static constexpr bool F32 = false;  // FP32 rows in the mixed form
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         dU_y_row<S>(g, p, dt, w(0, V), w(1, Mu), w(2, iR), i, j, k, a, b, c);
@@ -320,16 +323,16 @@ __device__ __forceinline__ void dV_z_row(const Grid &g, const Phys &p, double dt
     ROW_METRICS
     const int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1, km = k - 1, kp = k + 1;
     const int jvp = g.jS[jm], jvm = g.jS[jp];
-    const S Cmu = p.Cmu;
+    const S Cmu = sc<S>(p.Cmu);
 #define dUf(a, b, c) dU[g.ix(a, b, c)]
-    const S v3 = 0.5 * (Vf(i, jm, k) + Vf(i, j, k));
-    const S v4 = 0.5 * (Vf(i, j, k) + Vf(i, jp, k));
-    const S u1 = (dx2[jm] * Uf(i, j, k) + dx2[j] * Uf(i, jm, k)) / dmx2[j] * 0.5;
-    const S u2 = (dx2[jm] * Uf(ip, j, k) + dx2[j] * Uf(ip, jm, k)) / dmx2[j] * 0.5;
-    const S w5 = (dx2[jm] * Wf(i, j, k) + dx2[j] * Wf(i, jm, k)) / dmx2[j] * 0.5;
-    const S w6 = (dx2[jm] * Wf(i, j, kp) + dx2[j] * Wf(i, jm, kp)) / dmx2[j] * 0.5;
-    const S ddu1 = (dx2[jm] * dUf(i, j, k) + dx2[j] * dUf(i, jm, k)) / dmx2[j] * 0.5;
-    const S ddu2 = (dx2[jm] * dUf(ip, j, k) + dx2[j] * dUf(ip, jm, k)) / dmx2[j] * 0.5;
+    const S v3 = sc<S>(0.5) * (Vf(i, jm, k) + Vf(i, j, k));
+    const S v4 = sc<S>(0.5) * (Vf(i, j, k) + Vf(i, jp, k));
+    const S u1 = (dx2[jm] * Uf(i, j, k) + dx2[j] * Uf(i, jm, k)) / dmx2[j] * sc<S>(0.5);
+    const S u2 = (dx2[jm] * Uf(ip, j, k) + dx2[j] * Uf(ip, jm, k)) / dmx2[j] * sc<S>(0.5);
+    const S w5 = (dx2[jm] * Wf(i, j, k) + dx2[j] * Wf(i, jm, k)) / dmx2[j] * sc<S>(0.5);
+    const S w6 = (dx2[jm] * Wf(i, j, kp) + dx2[j] * Wf(i, jm, kp)) / dmx2[j] * sc<S>(0.5);
+    const S ddu1 = (dx2[jm] * dUf(i, j, k) + dx2[j] * dUf(i, jm, k)) / dmx2[j] * sc<S>(0.5);
+    const S ddu2 = (dx2[jm] * dUf(ip, j, k) + dx2[j] * dUf(ip, jm, k)) / dmx2[j] * sc<S>(0.5);
 
     const S dvdx1 = (Vf(i, j, k) - Vf(im, j, k)) / dmx1[i];
     const S dvdx2 = (Vf(ip, j, k) - Vf(i, j, k)) / dmx1[ip];
@@ -344,60 +347,60 @@ __device__ __forceinline__ void dV_z_row(const Grid &g, const Phys &p, double dt
     const S ddudy1 = (dUf(i, j, k) - dUf(i, jm, k)) / dmx2[j];
     const S ddudy2 = (dUf(ip, j, k) - dUf(ip, jm, k)) / dmx2[j];
 
-    S mua = 0.5 * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
-    S muc = 0.5 * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
-    S mub = 0.5 * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
-    const S mu1 = 0.5 * (dx1[im] * muc + dx1[i] * mua) / dmx1[i];
-    const S mu2 = 0.5 * (dx1[ip] * muc + dx1[i] * mub) / dmx1[ip];
-    mua = 0.5 * (dx2[jm] * MU(j, km) + dx2[j] * MU(jm, km)) / dmx2[j];
-    muc = 0.5 * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
-    mub = 0.5 * (dx2[jm] * MU(j, kp) + dx2[j] * MU(jm, kp)) / dmx2[j];
-    const S mu5 = 0.5 * (dx3[km] * muc + dx3[k] * mua) / dmx3[k];
-    const S mu6 = 0.5 * (dx3[kp] * muc + dx3[k] * mub) / dmx3[kp];
+    S mua = sc<S>(0.5) * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
+    S muc = sc<S>(0.5) * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
+    S mub = sc<S>(0.5) * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
+    const S mu1 = sc<S>(0.5) * (dx1[im] * muc + dx1[i] * mua) / dmx1[i];
+    const S mu2 = sc<S>(0.5) * (dx1[ip] * muc + dx1[i] * mub) / dmx1[ip];
+    mua = sc<S>(0.5) * (dx2[jm] * MU(j, km) + dx2[j] * MU(jm, km)) / dmx2[j];
+    muc = sc<S>(0.5) * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
+    mub = sc<S>(0.5) * (dx2[jm] * MU(j, kp) + dx2[j] * MU(jm, kp)) / dmx2[j];
+    const S mu5 = sc<S>(0.5) * (dx3[km] * muc + dx3[k] * mua) / dmx3[k];
+    const S mu6 = sc<S>(0.5) * (dx3[kp] * muc + dx3[k] * mub) / dmx3[kp];
 
-    const S invRhoc = 0.5 * (dx2[jm] * IR(j, k) + dx2[j] * IR(jm, k)) / dmx2[j];
-    const S invRhocCmu_half = 0.5 * Cmu * invRhoc;
+    const S invRhoc = sc<S>(0.5) * (dx2[jm] * IR(j, k) + dx2[j] * IR(jm, k)) / dmx2[j];
+    const S invRhocCmu_half = sc<S>(0.5) * Cmu * invRhoc;
 
-    const S viscous_v1 = 1.0 * (mu2 * dvdx2 - mu1 * dvdx1) / dx1[i];
-    const S viscous_v2 = 1.0 * (MU(j, k) * dvdy4 - MU(jm, k) * dvdy3) / dmx2[j];
-    const S viscous_v3 = 1.0 * (mu6 * dvdz6 - mu5 * dvdz5) / dx3[k];
-    const S viscous_v21 = 1.0 * (mu2 * dudy2 - mu1 * dudy1) / dx1[i];
-    const S viscous_v23 = 1.0 * (mu6 * dwdy6 - mu5 * dwdy5) / dx3[k];
+    const S viscous_v1 = sc<S>(1.0) * (mu2 * dvdx2 - mu1 * dvdx1) / dx1[i];
+    const S viscous_v2 = sc<S>(1.0) * (MU(j, k) * dvdy4 - MU(jm, k) * dvdy3) / dmx2[j];
+    const S viscous_v3 = sc<S>(1.0) * (mu6 * dvdz6 - mu5 * dvdz5) / dx3[k];
+    const S viscous_v21 = sc<S>(1.0) * (mu2 * dudy2 - mu1 * dudy1) / dx1[i];
+    const S viscous_v23 = sc<S>(1.0) * (mu6 * dwdy6 - mu5 * dwdy5) / dx3[k];
 
-    S rhs = invRhocCmu_half * (viscous_v1 + 2.0 * viscous_v2 + viscous_v3 + viscous_v21 + viscous_v23);
-    rhs = rhs - p.Cmp * invRhoc * (P[g.ix(i, j, k)] - P[g.ix(i, jm, k)]) / dmx2[j];
+    S rhs = invRhocCmu_half * (viscous_v1 + sc<S>(2.0) * viscous_v2 + viscous_v3 + viscous_v21 + viscous_v23);
+    rhs = rhs - sc<S>(p.Cmp) * invRhoc * (P[g.ix(i, j, k)] - P[g.ix(i, jm, k)]) / dmx2[j];
 
     const S vbt = VBCbt[g.b2(i, k)], vup = VBCup[g.b2(i, k)];
-    S vbc_down = (0.25 / dx2[jm] * v3 * vbt - 0.25 * 0.5 * dvdy3 * vbt);
-    S vbc_up = (-0.25 / dx2[j] * v4 * vup - 0.25 * 0.5 * dvdy4 * vup);
-    vbc_down = vbc_down + invRhocCmu_half / dmx2[j] * MU(jm, k) / dx2[jm] * vbt * 2.0;
-    vbc_up = vbc_up + invRhocCmu_half / dmx2[j] * MU(j, k) / dx2[j] * vup * 2.0;
-    rhs = rhs + (1.0 - (double)jvm) * vbc_down + (1.0 - (double)jvp) * vbc_up;
+    S vbc_down = (sc<S>(0.25) / dx2[jm] * v3 * vbt - sc<S>(0.25) * sc<S>(0.5) * dvdy3 * vbt);
+    S vbc_up = (-sc<S>(0.25) / dx2[j] * v4 * vup - sc<S>(0.25) * sc<S>(0.5) * dvdy4 * vup);
+    vbc_down = vbc_down + invRhocCmu_half / dmx2[j] * MU(jm, k) / dx2[jm] * vbt * sc<S>(2.0);
+    vbc_up = vbc_up + invRhocCmu_half / dmx2[j] * MU(j, k) / dx2[j] * vup * sc<S>(2.0);
+    rhs = rhs + (sc<S>(1.0) - sc<S>(jvm)) * vbc_down + (sc<S>(1.0) - sc<S>(jvp)) * vbc_up;
 
-    const S mACI = invRhocCmu_half / dx1[i] * (mu2 / dmx1[ip] + mu1 / dmx1[i]) + 0.25 * (-u2 / dmx1[ip] + u1 / dmx1[i]);
-    const S mAPI = -invRhocCmu_half / dx1[i] * mu2 / dmx1[ip] + 0.25 * (u2 / dmx1[ip]);
-    const S mAMI = -invRhocCmu_half / dx1[i] * mu1 / dmx1[i] + 0.25 * (-u1 / dmx1[i]);
-    const S mACJ = invRhocCmu_half / dmx2[j] * (MU(j, k) / dx2[j] + MU(jm, k) / dx2[jm]) * 2.0
-                      + 0.25 * (dvdy4 * 0.5 + dvdy3 * 0.5 - v4 / dx2[j] + v3 / dx2[jm]);
-    S mAPJ = -invRhocCmu_half / dmx2[j] * MU(j, k) / dx2[j] * 2.0 + 0.25 * (v4 / dx2[j] + dvdy4 * 0.5);
-    mAPJ = mAPJ * (double)jvp;
-    S mAMJ = -invRhocCmu_half / dmx2[j] * MU(jm, k) / dx2[jm] * 2.0 + 0.25 * (-v3 / dx2[jm] + dvdy3 * 0.5);
-    mAMJ = mAMJ * (double)jvm;
-    const S mACK = invRhocCmu_half / dx3[k] * (mu6 / dmx3[kp] + mu5 / dmx3[k]) + 0.25 * (-w6 / dmx3[kp] + w5 / dmx3[k]);
-    const S mAPK = -invRhocCmu_half / dx3[k] * mu6 / dmx3[kp] + 0.25 * (w6 / dmx3[kp]);
-    const S mAMK = -invRhocCmu_half / dx3[k] * mu5 / dmx3[k] + 0.25 * (-w5 / dmx3[k]);
+    const S mACI = invRhocCmu_half / dx1[i] * (mu2 / dmx1[ip] + mu1 / dmx1[i]) + sc<S>(0.25) * (-u2 / dmx1[ip] + u1 / dmx1[i]);
+    const S mAPI = -invRhocCmu_half / dx1[i] * mu2 / dmx1[ip] + sc<S>(0.25) * (u2 / dmx1[ip]);
+    const S mAMI = -invRhocCmu_half / dx1[i] * mu1 / dmx1[i] + sc<S>(0.25) * (-u1 / dmx1[i]);
+    const S mACJ = invRhocCmu_half / dmx2[j] * (MU(j, k) / dx2[j] + MU(jm, k) / dx2[jm]) * sc<S>(2.0)
+                      + sc<S>(0.25) * (dvdy4 * sc<S>(0.5) + dvdy3 * sc<S>(0.5) - v4 / dx2[j] + v3 / dx2[jm]);
+    S mAPJ = -invRhocCmu_half / dmx2[j] * MU(j, k) / dx2[j] * sc<S>(2.0) + sc<S>(0.25) * (v4 / dx2[j] + dvdy4 * sc<S>(0.5));
+    mAPJ = mAPJ * sc<S>(jvp);
+    S mAMJ = -invRhocCmu_half / dmx2[j] * MU(jm, k) / dx2[jm] * sc<S>(2.0) + sc<S>(0.25) * (-v3 / dx2[jm] + dvdy3 * sc<S>(0.5));
+    mAMJ = mAMJ * sc<S>(jvm);
+    const S mACK = invRhocCmu_half / dx3[k] * (mu6 / dmx3[kp] + mu5 / dmx3[k]) + sc<S>(0.25) * (-w6 / dmx3[kp] + w5 / dmx3[k]);
+    const S mAPK = -invRhocCmu_half / dx3[k] * mu6 / dmx3[kp] + sc<S>(0.25) * (w6 / dmx3[kp]);
+    const S mAMK = -invRhocCmu_half / dx3[k] * mu5 / dmx3[k] + sc<S>(0.25) * (-w5 / dmx3[k]);
 
     rhs = rhs - (mAPI * Vf(ip, j, k) + mACI * Vf(i, j, k) + mAMI * Vf(im, j, k)
                  + mAPJ * Vf(i, jp, k) + mACJ * Vf(i, j, k) + mAMJ * Vf(i, jm, k)
                  + mAPK * Vf(i, j, kp) + mACK * Vf(i, j, k) + mAMK * Vf(i, j, km));
-    rhs = rhs - (0.25 * (u2 * dvdx2 + u1 * dvdx1) - invRhocCmu_half / dx1[i] * (mu2 * dudy2 - mu1 * dudy1));
-    rhs = rhs - (0.25 * (w6 * dvdz6 + w5 * dvdz5) - invRhocCmu_half / dx3[k] * (mu6 * dwdy6 - mu5 * dwdy5));
-    rhs = rhs - (0.25 * (ddu2 * dvdx2 + ddu1 * dvdx1) - invRhocCmu_half / dx1[i] * (mu2 * ddudy2 - mu1 * ddudy1));
+    rhs = rhs - (sc<S>(0.25) * (u2 * dvdx2 + u1 * dvdx1) - invRhocCmu_half / dx1[i] * (mu2 * dudy2 - mu1 * dudy1));
+    rhs = rhs - (sc<S>(0.25) * (w6 * dvdz6 + w5 * dvdz5) - invRhocCmu_half / dx3[k] * (mu6 * dwdy6 - mu5 * dwdy5));
+    rhs = rhs - (sc<S>(0.25) * (ddu2 * dvdx2 + ddu1 * dvdx1) - invRhocCmu_half / dx1[i] * (mu2 * ddudy2 - mu1 * ddudy1));
 
-    ac = mACK * dt + 1.0;
-    ap = mAPK * dt;
-    am = mAMK * dt;
-    rhsout = rhs * dt;
+    ac = mACK * sc<S>(dt) + sc<S>(1.0);
+    ap = mAPK * sc<S>(dt);
+    am = mAMK * sc<S>(dt);
+    rhsout = rhs * sc<S>(dt);
 }
 struct DVZ {
     Grid g;
@@ -405,6 +408,7 @@ struct DVZ {
     double dt;
     const double *U, *V, *W, *P, *dU, *Mu, *iR, *VBCup, *VBCbt;
     static constexpr int NF = 9;  // field inputs, in the order below
+    static constexpr bool F32 = false;  // FP32 rows in the mixed form
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         dV_z_row<S>(g, p, dt, w(0, U), w(1, V), w(2, W), w(3, P), w(4, dU), w(5, Mu), w(6, iR), w(7, VBCup), w(8, VBCbt), i, j, k, a, b, c, d);
@@ -422,16 +426,16 @@ __device__ __forceinline__ void dV_x_row(const Grid &g, const Phys &p, double dt
     ROW_METRICS
     const int ip = i + 1, jm = j - 1;
     const S mu1 = stale[0], mu2 = stale[1];
-    const S u1 = (dx2[jm] * Uf(i, j, k) + dx2[j] * Uf(i, jm, k)) / dmx2[j] * 0.5;
-    const S u2 = (dx2[jm] * Uf(ip, j, k) + dx2[j] * Uf(ip, jm, k)) / dmx2[j] * 0.5;
-    const S invRhoc = 0.5 * (dx2[jm] * IR(j, k) + dx2[j] * IR(jm, k)) / dmx2[j];
-    const S invRhocCmu_half = 0.5 * p.Cmu * invRhoc;
-    const S mACI = invRhocCmu_half / dx1[i] * (mu2 / dmx1[ip] + mu1 / dmx1[i]) + 0.25 * (-u2 / dmx1[ip] + u1 / dmx1[i]);
-    const S mAPI = -invRhocCmu_half / dx1[i] * mu2 / dmx1[ip] + 0.25 * (u2 / dmx1[ip]);
-    const S mAMI = -invRhocCmu_half / dx1[i] * mu1 / dmx1[i] + 0.25 * (-u1 / dmx1[i]);
-    ac = mACI * dt + 1.0;
-    ap = mAPI * dt;
-    am = mAMI * dt;
+    const S u1 = (dx2[jm] * Uf(i, j, k) + dx2[j] * Uf(i, jm, k)) / dmx2[j] * sc<S>(0.5);
+    const S u2 = (dx2[jm] * Uf(ip, j, k) + dx2[j] * Uf(ip, jm, k)) / dmx2[j] * sc<S>(0.5);
+    const S invRhoc = sc<S>(0.5) * (dx2[jm] * IR(j, k) + dx2[j] * IR(jm, k)) / dmx2[j];
+    const S invRhocCmu_half = sc<S>(0.5) * sc<S>(p.Cmu) * invRhoc;
+    const S mACI = invRhocCmu_half / dx1[i] * (mu2 / dmx1[ip] + mu1 / dmx1[i]) + sc<S>(0.25) * (-u2 / dmx1[ip] + u1 / dmx1[i]);
+    const S mAPI = -invRhocCmu_half / dx1[i] * mu2 / dmx1[ip] + sc<S>(0.25) * (u2 / dmx1[ip]);
+    const S mAMI = -invRhocCmu_half / dx1[i] * mu1 / dmx1[i] + sc<S>(0.25) * (-u1 / dmx1[i]);
+    ac = mACI * sc<S>(dt) + sc<S>(1.0);
+    ap = mAPI * sc<S>(dt);
+    am = mAMI * sc<S>(dt);
 }
 struct DVX {
     Grid g;
@@ -439,6 +443,7 @@ struct DVX {
     double dt;
     const double *U, *iR, *stale;
     static constexpr int NF = 3;  // field inputs, in the order below
+    static constexpr bool F32 = false;  // FP32 rows in the mixed form
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         dV_x_row<S>(g, p, dt, w(0, U), w(1, iR), w(2, stale), i, j, k, a, b, c);
@@ -456,21 +461,21 @@ __device__ __forceinline__ void dV_y_row(const Grid &g, const Phys &p, double dt
     ROW_METRICS
     const int jm = j - 1, jp = j + 1;
     const int jvp = g.jS[jm], jvm = g.jS[jp];
-    const S v3 = 0.5 * (Vf(i, jm, k) + Vf(i, j, k));
-    const S v4 = 0.5 * (Vf(i, j, k) + Vf(i, jp, k));
+    const S v3 = sc<S>(0.5) * (Vf(i, jm, k) + Vf(i, j, k));
+    const S v4 = sc<S>(0.5) * (Vf(i, j, k) + Vf(i, jp, k));
     const S dvdy3 = (Vf(i, j, k) - Vf(i, jm, k)) / dx2[jm];
     const S dvdy4 = (Vf(i, jp, k) - Vf(i, j, k)) / dx2[j];
-    const S invRhoc = 0.5 * (dx2[jm] * IR(j, k) + dx2[j] * IR(jm, k)) / dmx2[j];
-    const S invRhocCmu_half = 0.5 * p.Cmu * invRhoc;
-    const S mACJ = invRhocCmu_half / dmx2[j] * (MU(j, k) / dx2[j] + MU(jm, k) / dx2[jm]) * 2.0
-                      + 0.25 * (dvdy4 * 0.5 + dvdy3 * 0.5 - v4 / dx2[j] + v3 / dx2[jm]);
-    S mAPJ = -invRhocCmu_half / dmx2[j] * MU(j, k) / dx2[j] * 2.0 + 0.25 * (v4 / dx2[j] + dvdy4 * 0.5);
-    mAPJ = mAPJ * (double)jvp;
-    S mAMJ = -invRhocCmu_half / dmx2[j] * MU(jm, k) / dx2[jm] * 2.0 + 0.25 * (-v3 / dx2[jm] + dvdy3 * 0.5);
-    mAMJ = mAMJ * (double)jvm;
-    ac = mACJ * dt + 1.0;
-    ap = mAPJ * dt;
-    am = mAMJ * dt;
+    const S invRhoc = sc<S>(0.5) * (dx2[jm] * IR(j, k) + dx2[j] * IR(jm, k)) / dmx2[j];
+    const S invRhocCmu_half = sc<S>(0.5) * sc<S>(p.Cmu) * invRhoc;
+    const S mACJ = invRhocCmu_half / dmx2[j] * (MU(j, k) / dx2[j] + MU(jm, k) / dx2[jm]) * sc<S>(2.0)
+                      + sc<S>(0.25) * (dvdy4 * sc<S>(0.5) + dvdy3 * sc<S>(0.5) - v4 / dx2[j] + v3 / dx2[jm]);
+    S mAPJ = -invRhocCmu_half / dmx2[j] * MU(j, k) / dx2[j] * sc<S>(2.0) + sc<S>(0.25) * (v4 / dx2[j] + dvdy4 * sc<S>(0.5));
+    mAPJ = mAPJ * sc<S>(jvp);
+    S mAMJ = -invRhocCmu_half / dmx2[j] * MU(jm, k) / dx2[jm] * sc<S>(2.0) + sc<S>(0.25) * (-v3 / dx2[jm] + dvdy3 * sc<S>(0.5));
+    mAMJ = mAMJ * sc<S>(jvm);
+    ac = mACJ * sc<S>(dt) + sc<S>(1.0);
+    ap = mAPJ * sc<S>(dt);
+    am = mAMJ * sc<S>(dt);
 }
 struct DVY {
     Grid g;
@@ -478,6 +483,7 @@ struct DVY {
     double dt;
     const double *V, *Mu, *iR;
     static constexpr int NF = 3;  // field inputs, in the order below
+    static constexpr bool F32 = false;  // FP32 rows in the mixed form
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         dV_y_row<S>(g, p, dt, w(0, V), w(1, Mu), w(2, iR), i, j, k, a, b, c);
@@ -499,14 +505,14 @@ __device__ __forceinline__ void dW_z_row(const Grid &g, const Phys &p, double dt
     ROW_METRICS
     const int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1, km = k - 1, kp = k + 1;
     const int jwp = g.jC[jp], jwm = g.jC[jm];
-    const S Cmu = p.Cmu;
+    const S Cmu = sc<S>(p.Cmu);
 #define dVf(a, b, c) dV[g.ix(a, b, c)]
-    const S w5 = 0.5 * (Wf(i, j, k) + Wf(i, j, km));
-    const S w6 = 0.5 * (Wf(i, j, k) + Wf(i, j, kp));
-    const S u1 = (dx3[km] * Uf(i, j, k) + dx3[k] * Uf(i, j, km)) / dmx3[k] * 0.5;
-    const S u2 = (dx3[km] * Uf(ip, j, k) + dx3[k] * Uf(ip, j, km)) / dmx3[k] * 0.5;
-    const S v3 = (dx3[km] * Vf(i, j, k) + dx3[k] * Vf(i, j, km)) / dmx3[k] * 0.5;
-    const S v4 = (dx3[km] * Vf(i, jp, k) + dx3[k] * Vf(i, jp, km)) / dmx3[k] * 0.5;
+    const S w5 = sc<S>(0.5) * (Wf(i, j, k) + Wf(i, j, km));
+    const S w6 = sc<S>(0.5) * (Wf(i, j, k) + Wf(i, j, kp));
+    const S u1 = (dx3[km] * Uf(i, j, k) + dx3[k] * Uf(i, j, km)) / dmx3[k] * sc<S>(0.5);
+    const S u2 = (dx3[km] * Uf(ip, j, k) + dx3[k] * Uf(ip, j, km)) / dmx3[k] * sc<S>(0.5);
+    const S v3 = (dx3[km] * Vf(i, j, k) + dx3[k] * Vf(i, j, km)) / dmx3[k] * sc<S>(0.5);
+    const S v4 = (dx3[km] * Vf(i, jp, k) + dx3[k] * Vf(i, jp, km)) / dmx3[k] * sc<S>(0.5);
 
     const S dwdx1 = (Wf(i, j, k) - Wf(im, j, k)) / dmx1[i];
     const S dwdx2 = (Wf(ip, j, k) - Wf(i, j, k)) / dmx1[ip];
@@ -522,71 +528,71 @@ __device__ __forceinline__ void dW_z_row(const Grid &g, const Phys &p, double dt
     const S ddudz2 = (dUf(ip, j, k) - dUf(ip, j, km)) / dmx3[k];
     const S ddvdz3 = (dVf(i, j, k) - dVf(i, j, km)) / dmx3[k];
     const S ddvdz4 = (dVf(i, jp, k) - dVf(i, jp, km)) / dmx3[k];
-    const S ddu1 = (dx3[km] * dUf(i, j, k) + dx3[k] * dUf(i, j, km)) / dmx3[k] * 0.5;
-    const S ddu2 = (dx3[km] * dUf(ip, j, k) + dx3[k] * dUf(ip, j, km)) / dmx3[k] * 0.5;
-    const S ddv3 = (dx3[km] * dVf(i, j, k) + dx3[k] * dVf(i, j, km)) / dmx3[k] * 0.5 * (double)jwm;
-    const S ddv4 = (dx3[km] * dVf(i, jp, k) + dx3[k] * dVf(i, jp, km)) / dmx3[k] * 0.5 * (double)jwp;
+    const S ddu1 = (dx3[km] * dUf(i, j, k) + dx3[k] * dUf(i, j, km)) / dmx3[k] * sc<S>(0.5);
+    const S ddu2 = (dx3[km] * dUf(ip, j, k) + dx3[k] * dUf(ip, j, km)) / dmx3[k] * sc<S>(0.5);
+    const S ddv3 = (dx3[km] * dVf(i, j, k) + dx3[k] * dVf(i, j, km)) / dmx3[k] * sc<S>(0.5) * sc<S>(jwm);
+    const S ddv4 = (dx3[km] * dVf(i, jp, k) + dx3[k] * dVf(i, jp, km)) / dmx3[k] * sc<S>(0.5) * sc<S>(jwp);
 
-    S mua = 0.5 * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
-    S muc = 0.5 * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
-    S mub = 0.5 * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
-    const S mu1 = 0.5 * (dx1[im] * muc + dx1[i] * mua) / dmx1[i];
-    const S mu2 = 0.5 * (dx1[ip] * muc + dx1[i] * mub) / dmx1[ip];
-    mua = 0.5 * (dx3[km] * MU(jm, k) + dx3[k] * MU(jm, km)) / dmx3[k];
-    muc = 0.5 * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
-    mub = 0.5 * (dx3[km] * MU(jp, k) + dx3[k] * MU(jp, km)) / dmx3[k];
-    const S mu3 = 0.5 * (dx2[jm] * muc + dx2[j] * mua) / dmx2[j];
-    const S mu4 = 0.5 * (dx2[jp] * muc + dx2[j] * mub) / dmx2[jp];
+    S mua = sc<S>(0.5) * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
+    S muc = sc<S>(0.5) * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
+    S mub = sc<S>(0.5) * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
+    const S mu1 = sc<S>(0.5) * (dx1[im] * muc + dx1[i] * mua) / dmx1[i];
+    const S mu2 = sc<S>(0.5) * (dx1[ip] * muc + dx1[i] * mub) / dmx1[ip];
+    mua = sc<S>(0.5) * (dx3[km] * MU(jm, k) + dx3[k] * MU(jm, km)) / dmx3[k];
+    muc = sc<S>(0.5) * (dx3[km] * MU(j, k) + dx3[k] * MU(j, km)) / dmx3[k];
+    mub = sc<S>(0.5) * (dx3[km] * MU(jp, k) + dx3[k] * MU(jp, km)) / dmx3[k];
+    const S mu3 = sc<S>(0.5) * (dx2[jm] * muc + dx2[j] * mua) / dmx2[j];
+    const S mu4 = sc<S>(0.5) * (dx2[jp] * muc + dx2[j] * mub) / dmx2[jp];
 
-    const S invRhoc = 0.5 * (dx3[km] * IR(j, k) + dx3[k] * IR(j, km)) / dmx3[k];
-    const S invRhocCmu_half = 0.5 * Cmu * invRhoc;
+    const S invRhoc = sc<S>(0.5) * (dx3[km] * IR(j, k) + dx3[k] * IR(j, km)) / dmx3[k];
+    const S invRhocCmu_half = sc<S>(0.5) * Cmu * invRhoc;
 
-    const S viscous_w1 = 1.0 * (mu2 * dwdx2 - mu1 * dwdx1) / dx1[i];
-    const S viscous_w2 = 1.0 * (mu4 * dwdy4 - mu3 * dwdy3) / dx2[j];
-    const S viscous_w3 = 1.0 * (MU(j, k) * dwdz6 - MU(j, km) * dwdz5) / dmx3[k];
-    const S viscous_w31 = 1.0 * (mu2 * dudz2 - mu1 * dudz1) / dx1[i];
-    const S viscous_w32 = 1.0 * (mu4 * dvdz4 - mu3 * dvdz3) / dx2[j];
+    const S viscous_w1 = sc<S>(1.0) * (mu2 * dwdx2 - mu1 * dwdx1) / dx1[i];
+    const S viscous_w2 = sc<S>(1.0) * (mu4 * dwdy4 - mu3 * dwdy3) / dx2[j];
+    const S viscous_w3 = sc<S>(1.0) * (MU(j, k) * dwdz6 - MU(j, km) * dwdz5) / dmx3[k];
+    const S viscous_w31 = sc<S>(1.0) * (mu2 * dudz2 - mu1 * dudz1) / dx1[i];
+    const S viscous_w32 = sc<S>(1.0) * (mu4 * dvdz4 - mu3 * dvdz3) / dx2[j];
 
-    S rhs = invRhocCmu_half * (viscous_w1 + viscous_w2 + 2.0 * viscous_w3 + viscous_w31 + viscous_w32);
-    rhs = rhs - p.Cmp * invRhoc * (P[g.ix(i, j, k)] - P[g.ix(i, j, km)]) / dmx3[k];
+    S rhs = invRhocCmu_half * (viscous_w1 + viscous_w2 + sc<S>(2.0) * viscous_w3 + viscous_w31 + viscous_w32);
+    rhs = rhs - sc<S>(p.Cmp) * invRhoc * (P[g.ix(i, j, k)] - P[g.ix(i, j, km)]) / dmx3[k];
 
-    S zbc_down = 0.25 * v3 / dmx2[j] * WBCbt[g.b2(i, k)]
-                    - 0.25 * dwdy3 * (dx3[km] * VBCbt[g.b2(i, k)] + dx3[k] * VBCbt[g.b2(i, km)]) / dmx3[k] * 0.5;
-    S zbc_up = -0.25 * v4 / dmx2[jp] * WBCup[g.b2(i, k)]
-                  - 0.25 * dwdy4 * (dx3[km] * VBCup[g.b2(i, k)] + dx3[k] * VBCup[g.b2(i, km)]) / dmx3[k] * 0.5;
+    S zbc_down = sc<S>(0.25) * v3 / dmx2[j] * WBCbt[g.b2(i, k)]
+                    - sc<S>(0.25) * dwdy3 * (dx3[km] * VBCbt[g.b2(i, k)] + dx3[k] * VBCbt[g.b2(i, km)]) / dmx3[k] * sc<S>(0.5);
+    S zbc_up = -sc<S>(0.25) * v4 / dmx2[jp] * WBCup[g.b2(i, k)]
+                  - sc<S>(0.25) * dwdy4 * (dx3[km] * VBCup[g.b2(i, k)] + dx3[k] * VBCup[g.b2(i, km)]) / dmx3[k] * sc<S>(0.5);
     zbc_down = zbc_down + invRhocCmu_half / dx2[j] * mu3 / dmx2[jp] * WBCbt[g.b2(i, k)];
     zbc_up = zbc_up + invRhocCmu_half / dx2[j] * mu4 / dmx2[j] * WBCup[g.b2(i, k)];
     zbc_down = zbc_down - invRhocCmu_half / dx2[j] * mu3 / dmx3[k] * (VBCbt[g.b2(i, k)] - VBCbt[g.b2(i, km)]);
     zbc_up = zbc_up + invRhocCmu_half / dx2[j] * mu4 / dmx3[k] * (VBCup[g.b2(i, k)] - VBCup[g.b2(i, km)]);
-    rhs = rhs + (1.0 - (double)jwm) * zbc_down + (1.0 - (double)jwp) * zbc_up;
+    rhs = rhs + (sc<S>(1.0) - sc<S>(jwm)) * zbc_down + (sc<S>(1.0) - sc<S>(jwp)) * zbc_up;
 
-    const S mACI = invRhocCmu_half / dx1[i] * (mu2 / dmx1[ip] + mu1 / dmx1[i]) + 0.25 * (-u2 / dmx1[ip] + u1 / dmx1[i]);
-    const S mAPI = -invRhocCmu_half / dx1[i] * mu2 / dmx1[ip] + 0.25 * (u2 / dmx1[ip]);
-    const S mAMI = -invRhocCmu_half / dx1[i] * mu1 / dmx1[i] + 0.25 * (-u1 / dmx1[i]);
-    const S mACJ = invRhocCmu_half / dx2[j] * (mu4 / dmx2[jp] + mu3 / dmx2[j]) + 0.25 * (-v4 / dmx2[jp] + v3 / dmx2[j]);
-    S mAPJ = -invRhocCmu_half / dx2[j] * mu4 / dmx2[jp] + 0.25 * (v4 / dmx2[jp]);
-    mAPJ = mAPJ * (double)jwp;
-    S mAMJ = -invRhocCmu_half / dx2[j] * mu3 / dmx2[j] + 0.25 * (-v3 / dmx2[j]);
-    mAMJ = mAMJ * (double)jwm;
-    const S mACK = invRhocCmu_half / dmx3[k] * (MU(j, k) / dx3[k] + MU(j, km) / dx3[km]) * 2.0
-                      + 0.25 * (-w6 / dx3[k] + w5 / dx3[km] + 0.5 * dwdz6 + 0.5 * dwdz5);
-    const S mAPK = -invRhocCmu_half / dmx3[k] * MU(j, k) / dx3[k] * 2.0 + 0.25 * (w6 / dx3[k] + 0.5 * dwdz6);
-    const S mAMK = -invRhocCmu_half / dmx3[k] * MU(j, km) / dx3[km] * 2.0 + 0.25 * (-w5 / dx3[km] + 0.5 * dwdz5);
+    const S mACI = invRhocCmu_half / dx1[i] * (mu2 / dmx1[ip] + mu1 / dmx1[i]) + sc<S>(0.25) * (-u2 / dmx1[ip] + u1 / dmx1[i]);
+    const S mAPI = -invRhocCmu_half / dx1[i] * mu2 / dmx1[ip] + sc<S>(0.25) * (u2 / dmx1[ip]);
+    const S mAMI = -invRhocCmu_half / dx1[i] * mu1 / dmx1[i] + sc<S>(0.25) * (-u1 / dmx1[i]);
+    const S mACJ = invRhocCmu_half / dx2[j] * (mu4 / dmx2[jp] + mu3 / dmx2[j]) + sc<S>(0.25) * (-v4 / dmx2[jp] + v3 / dmx2[j]);
+    S mAPJ = -invRhocCmu_half / dx2[j] * mu4 / dmx2[jp] + sc<S>(0.25) * (v4 / dmx2[jp]);
+    mAPJ = mAPJ * sc<S>(jwp);
+    S mAMJ = -invRhocCmu_half / dx2[j] * mu3 / dmx2[j] + sc<S>(0.25) * (-v3 / dmx2[j]);
+    mAMJ = mAMJ * sc<S>(jwm);
+    const S mACK = invRhocCmu_half / dmx3[k] * (MU(j, k) / dx3[k] + MU(j, km) / dx3[km]) * sc<S>(2.0)
+                      + sc<S>(0.25) * (-w6 / dx3[k] + w5 / dx3[km] + sc<S>(0.5) * dwdz6 + sc<S>(0.5) * dwdz5);
+    const S mAPK = -invRhocCmu_half / dmx3[k] * MU(j, k) / dx3[k] * sc<S>(2.0) + sc<S>(0.25) * (w6 / dx3[k] + sc<S>(0.5) * dwdz6);
+    const S mAMK = -invRhocCmu_half / dmx3[k] * MU(j, km) / dx3[km] * sc<S>(2.0) + sc<S>(0.25) * (-w5 / dx3[km] + sc<S>(0.5) * dwdz5);
 
     rhs = rhs - (mAPI * Wf(ip, j, k) + mACI * Wf(i, j, k) + mAMI * Wf(im, j, k)
                  + mAPJ * Wf(i, jp, k) + mACJ * Wf(i, j, k) + mAMJ * Wf(i, jm, k)
                  + mAPK * Wf(i, j, kp) + mACK * Wf(i, j, k) + mAMK * Wf(i, j, km));
-    rhs = rhs - (0.25 * (u2 * dwdx2 + u1 * dwdx1) - invRhocCmu_half / dx1[i] * (mu2 * dudz2 - mu1 * dudz1));
-    rhs = rhs - (0.25 * (v3 * (double)jwm * dwdy3 + v4 * (double)jwp * dwdy4)
-                 - invRhocCmu_half / dx2[j] * (mu4 * dvdz4 * (double)jwp - mu3 * dvdz3 * (double)jwm));
-    rhs = rhs - (0.25 * (ddu2 * dwdx2 + ddu1 * dwdx1) - invRhocCmu_half / dx1[i] * (mu2 * ddudz2 - mu1 * ddudz1));
-    rhs = rhs - (0.25 * (ddv3 * (double)jwm * dwdy3 + ddv4 * (double)jwp * dwdy4)
-                 - invRhocCmu_half / dx2[j] * (mu4 * ddvdz4 * (double)jwp - mu3 * ddvdz3 * (double)jwm));
+    rhs = rhs - (sc<S>(0.25) * (u2 * dwdx2 + u1 * dwdx1) - invRhocCmu_half / dx1[i] * (mu2 * dudz2 - mu1 * dudz1));
+    rhs = rhs - (sc<S>(0.25) * (v3 * sc<S>(jwm) * dwdy3 + v4 * sc<S>(jwp) * dwdy4)
+                 - invRhocCmu_half / dx2[j] * (mu4 * dvdz4 * sc<S>(jwp) - mu3 * dvdz3 * sc<S>(jwm)));
+    rhs = rhs - (sc<S>(0.25) * (ddu2 * dwdx2 + ddu1 * dwdx1) - invRhocCmu_half / dx1[i] * (mu2 * ddudz2 - mu1 * ddudz1));
+    rhs = rhs - (sc<S>(0.25) * (ddv3 * sc<S>(jwm) * dwdy3 + ddv4 * sc<S>(jwp) * dwdy4)
+                 - invRhocCmu_half / dx2[j] * (mu4 * ddvdz4 * sc<S>(jwp) - mu3 * ddvdz3 * sc<S>(jwm)));
 
-    ac = mACK * dt + 1.0;
-    ap = mAPK * dt;
-    am = mAMK * dt;
-    rhsout = rhs * dt;
+    ac = mACK * sc<S>(dt) + sc<S>(1.0);
+    ap = mAPK * sc<S>(dt);
+    am = mAMK * sc<S>(dt);
+    rhsout = rhs * sc<S>(dt);
 }
 struct DWZ {
     Grid g;
@@ -594,6 +600,7 @@ struct DWZ {
     double dt;
     const double *U, *V, *W, *P, *dU, *dV, *Mu, *iR, *VBCup, *VBCbt, *WBCup, *WBCbt;
     static constexpr int NF = 12;  // field inputs, in the order below
+    static constexpr bool F32 = false;  // FP32 rows in the mixed form
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         dW_z_row<S>(g, p, dt, w(0, U), w(1, V), w(2, W), w(3, P), w(4, dU), w(5, dV), w(6, Mu), w(7, iR), w(8, VBCup), w(9, VBCbt), w(10, WBCup), w(11, WBCbt), i, j, k, a, b, c, d);
@@ -611,16 +618,16 @@ __device__ __forceinline__ void dW_x_row(const Grid &g, const Phys &p, double dt
     ROW_METRICS
     const int ip = i + 1, km = k - 1;
     const S mu1 = stale[2], mu2 = stale[3];
-    const S u1 = (dx3[km] * Uf(i, j, k) + dx3[k] * Uf(i, j, km)) / dmx3[k] * 0.5;
-    const S u2 = (dx3[km] * Uf(ip, j, k) + dx3[k] * Uf(ip, j, km)) / dmx3[k] * 0.5;
-    const S invRhoc = 0.5 * (dx3[km] * IR(j, k) + dx3[k] * IR(j, km)) / dmx3[k];
-    const S invRhocCmu_half = 0.5 * p.Cmu * invRhoc;
-    const S mACI = invRhocCmu_half / dx1[i] * (mu2 / dmx1[ip] + mu1 / dmx1[i]) + 0.25 * (-u2 / dmx1[ip] + u1 / dmx1[i]);
-    const S mAPI = -invRhocCmu_half / dx1[i] * mu2 / dmx1[ip] + 0.25 * (u2 / dmx1[ip]);
-    const S mAMI = -invRhocCmu_half / dx1[i] * mu1 / dmx1[i] + 0.25 * (-u1 / dmx1[i]);
-    ac = mACI * dt + 1.0;
-    ap = mAPI * dt;
-    am = mAMI * dt;
+    const S u1 = (dx3[km] * Uf(i, j, k) + dx3[k] * Uf(i, j, km)) / dmx3[k] * sc<S>(0.5);
+    const S u2 = (dx3[km] * Uf(ip, j, k) + dx3[k] * Uf(ip, j, km)) / dmx3[k] * sc<S>(0.5);
+    const S invRhoc = sc<S>(0.5) * (dx3[km] * IR(j, k) + dx3[k] * IR(j, km)) / dmx3[k];
+    const S invRhocCmu_half = sc<S>(0.5) * sc<S>(p.Cmu) * invRhoc;
+    const S mACI = invRhocCmu_half / dx1[i] * (mu2 / dmx1[ip] + mu1 / dmx1[i]) + sc<S>(0.25) * (-u2 / dmx1[ip] + u1 / dmx1[i]);
+    const S mAPI = -invRhocCmu_half / dx1[i] * mu2 / dmx1[ip] + sc<S>(0.25) * (u2 / dmx1[ip]);
+    const S mAMI = -invRhocCmu_half / dx1[i] * mu1 / dmx1[i] + sc<S>(0.25) * (-u1 / dmx1[i]);
+    ac = mACI * sc<S>(dt) + sc<S>(1.0);
+    ap = mAPI * sc<S>(dt);
+    am = mAMI * sc<S>(dt);
 }
 struct DWX {
     Grid g;
@@ -628,6 +635,7 @@ struct DWX {
     double dt;
     const double *U, *iR, *stale;
     static constexpr int NF = 3;  // field inputs, in the order below
+    static constexpr bool F32 = false;  // FP32 rows in the mixed form
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         dW_x_row<S>(g, p, dt, w(0, U), w(1, iR), w(2, stale), i, j, k, a, b, c);
@@ -647,18 +655,18 @@ __device__ __forceinline__ void dW_y_row(const Grid &g, const Phys &p, double dt
     const int km = g.n3m - 1;  // stale from the x-stage's last k
     const int jwp = g.jC[jp], jwm = g.jC[jm];
     const S mu3 = stale[4], mu4 = stale[5];
-    const S v3 = (dx3[km] * Vf(i, j, k) + dx3[k] * Vf(i, j, km)) / dmx3[k] * 0.5;
-    const S v4 = (dx3[km] * Vf(i, jp, k) + dx3[k] * Vf(i, jp, km)) / dmx3[k] * 0.5;
-    const S invRhoc = 0.5 * (dx3[km] * IR(j, k) + dx3[k] * IR(j, km)) / dmx3[k];
-    const S invRhocCmu_half = 0.5 * p.Cmu * invRhoc;
-    const S mACJ = invRhocCmu_half / dx2[j] * (mu4 / dmx2[jp] + mu3 / dmx2[j]) + 0.25 * (-v4 / dmx2[jp] + v3 / dmx2[j]);
-    S mAPJ = -invRhocCmu_half / dx2[j] * mu4 / dmx2[jp] + 0.25 * (v4 / dmx2[jp]);
-    mAPJ = mAPJ * (double)jwp;
-    S mAMJ = -invRhocCmu_half / dx2[j] * mu3 / dmx2[j] + 0.25 * (-v3 / dmx2[j]);
-    mAMJ = mAMJ * (double)jwm;
-    ac = mACJ * dt + 1.0;
-    ap = mAPJ * dt;
-    am = mAMJ * dt;
+    const S v3 = (dx3[km] * Vf(i, j, k) + dx3[k] * Vf(i, j, km)) / dmx3[k] * sc<S>(0.5);
+    const S v4 = (dx3[km] * Vf(i, jp, k) + dx3[k] * Vf(i, jp, km)) / dmx3[k] * sc<S>(0.5);
+    const S invRhoc = sc<S>(0.5) * (dx3[km] * IR(j, k) + dx3[k] * IR(j, km)) / dmx3[k];
+    const S invRhocCmu_half = sc<S>(0.5) * sc<S>(p.Cmu) * invRhoc;
+    const S mACJ = invRhocCmu_half / dx2[j] * (mu4 / dmx2[jp] + mu3 / dmx2[j]) + sc<S>(0.25) * (-v4 / dmx2[jp] + v3 / dmx2[j]);
+    S mAPJ = -invRhocCmu_half / dx2[j] * mu4 / dmx2[jp] + sc<S>(0.25) * (v4 / dmx2[jp]);
+    mAPJ = mAPJ * sc<S>(jwp);
+    S mAMJ = -invRhocCmu_half / dx2[j] * mu3 / dmx2[j] + sc<S>(0.25) * (-v3 / dmx2[j]);
+    mAMJ = mAMJ * sc<S>(jwm);
+    ac = mACJ * sc<S>(dt) + sc<S>(1.0);
+    ap = mAPJ * sc<S>(dt);
+    am = mAMJ * sc<S>(dt);
 }
 struct DWY {
     Grid g;
@@ -666,6 +674,7 @@ struct DWY {
     double dt;
     const double *V, *iR, *stale;
     static constexpr int NF = 3;  // field inputs, in the order below
+    static constexpr bool F32 = false;  // FP32 rows in the mixed form
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         dW_y_row<S>(g, p, dt, w(0, V), w(1, iR), w(2, stale), i, j, k, a, b, c);
@@ -684,20 +693,20 @@ __device__ __forceinline__ void blockLdV_cell(const Grid &g, const Phys &p, doub
     ROW_METRICS
     const int jm = j - 1, km = k - 1, kp = k + 1;
 #define dWf(a, b, c) dW[g.ix(a, b, c)]
-    const S dwm5 = (dx2[jm] * dWf(i, j, k) + dx2[j] * dWf(i, jm, k)) / dmx2[j] * 0.5;
-    const S dwm6 = (dx2[jm] * dWf(i, j, kp) + dx2[j] * dWf(i, jm, kp)) / dmx2[j] * 0.5;
+    const S dwm5 = (dx2[jm] * dWf(i, j, k) + dx2[j] * dWf(i, jm, k)) / dmx2[j] * sc<S>(0.5);
+    const S dwm6 = (dx2[jm] * dWf(i, j, kp) + dx2[j] * dWf(i, jm, kp)) / dmx2[j] * sc<S>(0.5);
     const S ddwdy5 = (dWf(i, j, k) - dWf(i, jm, k)) / dmx2[j];
     const S ddwdy6 = (dWf(i, j, kp) - dWf(i, jm, kp)) / dmx2[j];
     const S dvdz5 = (Vf(i, j, k) - Vf(i, j, km)) / dmx3[k];
     const S dvdz6 = (Vf(i, j, kp) - Vf(i, j, k)) / dmx3[kp];
-    const S invRhoc = 0.5 * (dx2[jm] * IR(j, k) + dx2[j] * IR(jm, k)) / dmx2[j];
-    const S mua = 0.5 * (dx2[jm] * MU(j, km) + dx2[j] * MU(jm, km)) / dmx2[j];
-    const S muc = 0.5 * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
-    const S mub = 0.5 * (dx2[jm] * MU(j, kp) + dx2[j] * MU(jm, kp)) / dmx2[j];
-    const S mu5 = 0.5 * (dx3[km] * muc + dx3[k] * mua) / dmx3[k];
-    const S mu6 = 0.5 * (dx3[kp] * muc + dx3[k] * mub) / dmx3[kp];
-    out = dV[g.ix(i, j, k)] - dt * (0.25 * (dwm5 * dvdz5 + dwm6 * dvdz6)
-                          - 0.5 * p.Cmu * invRhoc / dx3[k] * (mu6 * ddwdy6 - mu5 * ddwdy5));
+    const S invRhoc = sc<S>(0.5) * (dx2[jm] * IR(j, k) + dx2[j] * IR(jm, k)) / dmx2[j];
+    const S mua = sc<S>(0.5) * (dx2[jm] * MU(j, km) + dx2[j] * MU(jm, km)) / dmx2[j];
+    const S muc = sc<S>(0.5) * (dx2[jm] * MU(j, k) + dx2[j] * MU(jm, k)) / dmx2[j];
+    const S mub = sc<S>(0.5) * (dx2[jm] * MU(j, kp) + dx2[j] * MU(jm, kp)) / dmx2[j];
+    const S mu5 = sc<S>(0.5) * (dx3[km] * muc + dx3[k] * mua) / dmx3[k];
+    const S mu6 = sc<S>(0.5) * (dx3[kp] * muc + dx3[k] * mub) / dmx3[kp];
+    out = dV[g.ix(i, j, k)] - sc<S>(dt) * (sc<S>(0.25) * (dwm5 * dvdz5 + dwm6 * dvdz6)
+                          - sc<S>(0.5) * sc<S>(p.Cmu) * invRhoc / dx3[k] * (mu6 * ddwdy6 - mu5 * ddwdy5));
 }
 struct BlockLdV {
     Grid g;
@@ -729,32 +738,32 @@ __device__ __forceinline__ void blockLdU_cell(const Grid &g, const Phys &p, doub
     ROW_METRICS
     const int im = i - 1, jm = j - 1, jp = j + 1, km = k - 1, kp = k + 1;
     const int jum = g.jC[jm], jup = g.jC[jp];
-    const S invRhoc = 0.5 * (dx1[im] * IR(j, k) + dx1[i] * IR(j, k)) / dmx1[i];
-    const S dvm3 = (dx1[i] * dVf(im, j, k) + dx1[im] * dVf(i, j, k)) / dmx1[i] * 0.5 * (double)jum;
-    const S dvm4 = (dx1[i] * dVf(im, jp, k) + dx1[im] * dVf(i, jp, k)) / dmx1[i] * 0.5 * (double)jup;
+    const S invRhoc = sc<S>(0.5) * (dx1[im] * IR(j, k) + dx1[i] * IR(j, k)) / dmx1[i];
+    const S dvm3 = (dx1[i] * dVf(im, j, k) + dx1[im] * dVf(i, j, k)) / dmx1[i] * sc<S>(0.5) * sc<S>(jum);
+    const S dvm4 = (dx1[i] * dVf(im, jp, k) + dx1[im] * dVf(i, jp, k)) / dmx1[i] * sc<S>(0.5) * sc<S>(jup);
     const S dudy3 = (Uf(i, j, k) - Uf(i, jm, k)) / dmx2[j];
     const S dudy4 = (Uf(i, jp, k) - Uf(i, j, k)) / dmx2[jp];
-    const S dvmdx3 = (dVf(i, j, k) - dVf(im, j, k)) / dmx1[i] * (double)jum;
-    const S dvmdx4 = (dVf(i, jp, k) - dVf(im, jp, k)) / dmx1[i] * (double)jup;
-    S mua = 0.5 * (dx1[im] * MU(jm, k) + dx1[i] * MU(jm, k)) / dmx1[i];
-    S muc = 0.5 * (dx1[im] * MU(j, k) + dx1[i] * MU(j, k)) / dmx1[i];
-    S mub = 0.5 * (dx1[im] * MU(jp, k) + dx1[i] * MU(jp, k)) / dmx1[i];
-    const S mu3 = 0.5 * (dx2[jm] * muc + dx2[j] * mua) / dmx2[j];
-    const S mu4 = 0.5 * (dx2[jp] * muc + dx2[j] * mub) / dmx2[jp];
-    const S dwm5 = (dx1[i] * dWf(im, j, k) + dx1[im] * dWf(i, j, k)) / dmx1[i] * 0.5;
-    const S dwm6 = (dx1[i] * dWf(im, j, kp) + dx1[im] * dWf(i, j, kp)) / dmx1[i] * 0.5;
+    const S dvmdx3 = (dVf(i, j, k) - dVf(im, j, k)) / dmx1[i] * sc<S>(jum);
+    const S dvmdx4 = (dVf(i, jp, k) - dVf(im, jp, k)) / dmx1[i] * sc<S>(jup);
+    S mua = sc<S>(0.5) * (dx1[im] * MU(jm, k) + dx1[i] * MU(jm, k)) / dmx1[i];
+    S muc = sc<S>(0.5) * (dx1[im] * MU(j, k) + dx1[i] * MU(j, k)) / dmx1[i];
+    S mub = sc<S>(0.5) * (dx1[im] * MU(jp, k) + dx1[i] * MU(jp, k)) / dmx1[i];
+    const S mu3 = sc<S>(0.5) * (dx2[jm] * muc + dx2[j] * mua) / dmx2[j];
+    const S mu4 = sc<S>(0.5) * (dx2[jp] * muc + dx2[j] * mub) / dmx2[jp];
+    const S dwm5 = (dx1[i] * dWf(im, j, k) + dx1[im] * dWf(i, j, k)) / dmx1[i] * sc<S>(0.5);
+    const S dwm6 = (dx1[i] * dWf(im, j, kp) + dx1[im] * dWf(i, j, kp)) / dmx1[i] * sc<S>(0.5);
     const S dudz5 = (Uf(i, j, k) - Uf(i, j, km)) / dmx3[k];
     const S dudz6 = (Uf(i, j, kp) - Uf(i, j, k)) / dmx3[kp];
     const S dwmdx5 = (dWf(i, j, k) - dWf(im, j, k)) / dmx1[i];
     const S dwmdx6 = (dWf(i, j, kp) - dWf(im, j, kp)) / dmx1[i];
-    mua = 0.5 * (dx1[im] * MU(j, km) + dx1[i] * MU(j, km)) / dmx1[i];
-    muc = 0.5 * (dx1[im] * MU(j, k) + dx1[i] * MU(j, k)) / dmx1[i];
-    mub = 0.5 * (dx1[im] * MU(j, kp) + dx1[i] * MU(j, kp)) / dmx1[i];
-    const S mu5 = 0.5 * (dx3[km] * muc + dx3[k] * mua) / dmx3[k];
-    const S mu6 = 0.5 * (dx3[kp] * muc + dx3[k] * mub) / dmx3[kp];
+    mua = sc<S>(0.5) * (dx1[im] * MU(j, km) + dx1[i] * MU(j, km)) / dmx1[i];
+    muc = sc<S>(0.5) * (dx1[im] * MU(j, k) + dx1[i] * MU(j, k)) / dmx1[i];
+    mub = sc<S>(0.5) * (dx1[im] * MU(j, kp) + dx1[i] * MU(j, kp)) / dmx1[i];
+    const S mu5 = sc<S>(0.5) * (dx3[km] * muc + dx3[k] * mua) / dmx3[k];
+    const S mu6 = sc<S>(0.5) * (dx3[kp] * muc + dx3[k] * mub) / dmx3[kp];
     out = dU[g.ix(i, j, k)]
-          - dt * (0.25 * (dvm4 * dudy4 + dvm3 * dudy3) - 0.5 * p.Cmu * invRhoc / dx2[j] * (mu4 * dvmdx4 - mu3 * dvmdx3))
-          - dt * (0.25 * (dwm6 * dudz6 + dwm5 * dudz5) - 0.5 * p.Cmu * invRhoc / dx3[k] * (mu6 * dwmdx6 - mu5 * dwmdx5));
+          - sc<S>(dt) * (sc<S>(0.25) * (dvm4 * dudy4 + dvm3 * dudy3) - sc<S>(0.5) * sc<S>(p.Cmu) * invRhoc / dx2[j] * (mu4 * dvmdx4 - mu3 * dvmdx3))
+          - sc<S>(dt) * (sc<S>(0.25) * (dwm6 * dudz6 + dwm5 * dudz5) - sc<S>(0.5) * sc<S>(p.Cmu) * invRhoc / dx3[k] * (mu6 * dwmdx6 - mu5 * dwmdx5));
 }
 struct BlockLdU {
     Grid g;

@@ -19,10 +19,35 @@
 // split solve's, so both ways give bit-identical solutions.
 #pragma once
 
+#include <type_traits>
+
 #include "ad.h"
 #include "tdma_part.h"
 
 namespace tcs {
+
+// Row (i, j, k) of a stage in the arithmetic of the line solve's type S:
+// FP64 solves take the rows as the step computes them.  FP32 solves (the
+// mixed form) take them in FP64 rounded to float, or, for a stage whose
+// Row::F32 is set, make them in FP32 from the FP64 fields (loads rounded to
+// float, metrics and constants in float: the cell templates' sc<S> and
+// metric<S>).  No stage sets it: at 512^3 FP32 rows made the thermal z and
+// x stages 0.3 and 0.1 ms faster but the 1000-step drift of the wall
+// Nusselt numbers grew from 9e-9 to 2e-5 and of T from 5e-9 to 3e-7, the
+// dU/dV/dW z rows were 1.0-2.0 ms slower, and the x and y momentum rows
+// took the same time.  d is the row's right-hand side where the stage has
+// one, else 0.
+template <typename S, class Row>
+__device__ __forceinline__ void row_as(const Row &row, int i, int j, int k, S &a, S &b, S &c, S &d) {
+    if constexpr (std::is_same<S, float>::value && Row::F32) {
+        d = 0.0f;
+        row.template eval<float>(F32W(), i, j, k, a, b, c, d);
+    } else {
+        double ta, tb, tc, td = 0.0;
+        row(i, j, k, ta, tb, tc, td);
+        a = (S)ta; b = (S)tb; c = (S)tc; d = (S)td;
+    }
+}
 
 // One thread per interior (i, j) (i fastest, rows j >= jlo) and ROWS3D_KB
 // consecutive k (dividing n3m, a power of two >= 4): row (i, j, k) of the
@@ -40,8 +65,8 @@ k_rows3d(Row row, int jlo, S *__restrict__ am, S *__restrict__ ac, S *__restrict
 #pragma unroll
     for (int q = 0; q < ROWS3D_KB; q++) {
         const int k = blockIdx.z * ROWS3D_KB + q + 1;
-        double a, b, c, d = 0.0;
-        row(i, j, k, a, b, c, d);
+        S a, b, c, d;
+        row_as<S>(row, i, j, k, a, b, c, d);
         const size_t o = row.g.r3(i, j, k);
         ac[o] = b;
         ap[o] = c;
@@ -64,9 +89,7 @@ struct ZLineRows {
     Row row;
     int jlo;
     __device__ __forceinline__ void operator()(int s0, int s1, int r, S &a, S &b, S &c, S &d) const {
-        double ta, tb, tc, td = 0.0;
-        row(s0 + 1, s1 + jlo, r + 1, ta, tb, tc, td);
-        a = (S)ta; b = (S)tb; c = (S)tc; d = (S)td;
+        row_as<S>(row, s0 + 1, s1 + jlo, r + 1, a, b, c, d);
     }
 };
 
@@ -79,9 +102,7 @@ struct YLineRows {
     int jlo;
     __device__ __forceinline__ void operator()(int s0, int s1, int r, S &a, S &b, S &c, S &d) const {
         const int i = s0 + 1, j = r + jlo, k = s1 + 1;
-        double ta, tb, tc, td = 0.0;
-        row(i, j, k, ta, tb, tc, td);
-        a = (S)ta; b = (S)tb; c = (S)tc;
+        row_as<S>(row, i, j, k, a, b, c, d);
         d = rhs[row.g.r3(i, j, k)];
     }
 };
@@ -95,9 +116,7 @@ struct XLineRows {
     int jlo;
     __device__ __forceinline__ void operator()(int s0, int s1, int r, S &a, S &b, S &c, S &d) const {
         const int i = r + 1, j = s0 + jlo, k = s1 + 1;
-        double ta, tb, tc, td = 0.0;
-        row(i, j, k, ta, tb, tc, td);
-        a = (S)ta; b = (S)tb; c = (S)tc;
+        row_as<S>(row, i, j, k, a, b, c, d);
         d = rhs[row.g.r3(i, j, k)];
     }
 };

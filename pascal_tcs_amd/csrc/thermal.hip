@@ -30,13 +30,13 @@ using scr_t = TCS_SCRATCH;
 
 template <class S>
 __device__ __forceinline__ void th_kpp(const Phys &p, S T, S &kpp, S &dkpp) {
-    const S tmp1 = p.DeltaT * T;
+    const S tmp1 = sc<S>(p.DeltaT) * T;
     const S tmp2 = tmp1 * tmp1;
     const S tmp3 = tmp2 * tmp1;
     const S tmp4 = tmp3 * tmp1;
     const S tmp5 = tmp4 * tmp1;
-    kpp = p.c10 * (1.0 + p.c11 * tmp1 + p.c12 * tmp2 + p.c13 * tmp3 + p.c14 * tmp4 + p.c15 * tmp5) / p.KPP0;
-    dkpp = p.c10 * (p.c11 + 2.0 * p.c12 * tmp1 + 3.0 * p.c13 * tmp2 + 4.0 * p.c14 * tmp3 + 5.0 * p.c15 * tmp4) / p.KPP0;
+    kpp = sc<S>(p.c10) * (sc<S>(1.0) + sc<S>(p.c11) * tmp1 + sc<S>(p.c12) * tmp2 + sc<S>(p.c13) * tmp3 + sc<S>(p.c14) * tmp4 + sc<S>(p.c15) * tmp5) / sc<S>(p.KPP0);
+    dkpp = sc<S>(p.c10) * (sc<S>(p.c11) + sc<S>(2.0) * sc<S>(p.c12) * tmp1 + sc<S>(3.0) * sc<S>(p.c13) * tmp2 + sc<S>(4.0) * sc<S>(p.c14) * tmp3 + sc<S>(5.0) * sc<S>(p.c15) * tmp4) / sc<S>(p.KPP0);
 }
 
 template <class S>
@@ -55,21 +55,21 @@ __device__ __forceinline__ S th_dkpp_only(const Phys &p, S T) {
 
 template <class S>
 __device__ __forceinline__ void th_rhocp(const Phys &p, S T, S &irc, S &dirc) {
-    const S tmp1 = p.DeltaT * T;
+    const S tmp1 = sc<S>(p.DeltaT) * T;
     const S tmp2 = tmp1 * tmp1;
     const S tmp3 = tmp2 * tmp1;
     const S tmp4 = tmp3 * tmp1;
     const S tmp5 = tmp4 * tmp1;
-    irc = p.Rho0 * p.Cp0
-        / (p.a10 + p.a10 * (p.a11 * tmp1 + p.a12 * tmp2 + p.a13 * tmp3 + p.a14 * tmp4 + p.a15 * tmp5))
-        / (p.b10 + p.b10 * (p.b11 * tmp1 + p.b12 * tmp2 + p.b13 * tmp3 + p.b14 * tmp4 + p.b15 * tmp5));
-    const S da = 1.0 + p.a11 * tmp1 + p.a12 * tmp2 + p.a13 * tmp3;
-    const S db = 1.0 + p.b11 * tmp1 + p.b12 * tmp2 + p.b13 * tmp3;
-    dirc = -(p.b11 * (1.0 + 3.0 * p.a12 * tmp2 + 4.0 * p.a13 * tmp3)
-             + p.a11 * (1.0 + 2.0 * p.b11 * tmp1 + 3.0 * p.b12 * tmp2 + 4.0 * p.b13 * tmp3)
-             + tmp1 * (p.b12 * (2.0 + 5.0 * p.a13 * tmp3)
-                       + p.a12 * (2.0 + 4.0 * p.b12 * tmp2 + 5.0 * p.b13 * tmp3)
-                       + 3.0 * tmp1 * (p.a13 + p.b13 + 2.0 * p.a13 * p.b13 * tmp3)))
+    irc = sc<S>(p.Rho0) * sc<S>(p.Cp0)
+        / (sc<S>(p.a10) + sc<S>(p.a10) * (sc<S>(p.a11) * tmp1 + sc<S>(p.a12) * tmp2 + sc<S>(p.a13) * tmp3 + sc<S>(p.a14) * tmp4 + sc<S>(p.a15) * tmp5))
+        / (sc<S>(p.b10) + sc<S>(p.b10) * (sc<S>(p.b11) * tmp1 + sc<S>(p.b12) * tmp2 + sc<S>(p.b13) * tmp3 + sc<S>(p.b14) * tmp4 + sc<S>(p.b15) * tmp5));
+    const S da = sc<S>(1.0) + sc<S>(p.a11) * tmp1 + sc<S>(p.a12) * tmp2 + sc<S>(p.a13) * tmp3;
+    const S db = sc<S>(1.0) + sc<S>(p.b11) * tmp1 + sc<S>(p.b12) * tmp2 + sc<S>(p.b13) * tmp3;
+    dirc = -(sc<S>(p.b11) * (sc<S>(1.0) + sc<S>(3.0) * sc<S>(p.a12) * tmp2 + sc<S>(4.0) * sc<S>(p.a13) * tmp3)
+             + sc<S>(p.a11) * (sc<S>(1.0) + sc<S>(2.0) * sc<S>(p.b11) * tmp1 + sc<S>(3.0) * sc<S>(p.b12) * tmp2 + sc<S>(4.0) * sc<S>(p.b13) * tmp3)
+             + tmp1 * (sc<S>(p.b12) * (sc<S>(2.0) + sc<S>(5.0) * sc<S>(p.a13) * tmp3)
+                       + sc<S>(p.a12) * (sc<S>(2.0) + sc<S>(4.0) * sc<S>(p.b12) * tmp2 + sc<S>(5.0) * sc<S>(p.b13) * tmp3)
+                       + sc<S>(3.0) * tmp1 * (sc<S>(p.a13) + sc<S>(p.b13) + sc<S>(2.0) * sc<S>(p.a13) * sc<S>(p.b13) * tmp3)))
          / (da * da) / (db * db);
 }
 
@@ -93,9 +93,9 @@ __device__ __forceinline__ void thermal_z_row(const Grid &g, const Phys &p, doub
     int i, int j, int k, S &am, S &ac, S &ap, S &rhsout) {
     const int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1, km = k - 1, kp = k + 1;
     const int jem = g.jC[jm], jep = g.jC[jp];
-    const auto dx1 = metric(g.dx1, g.rdx1), dx2 = metric(g.dx2, g.rdx2), dx3 = metric(g.dx3, g.rdx3);
-    const auto dmx1 = metric(g.dmx1, g.rdmx1), dmx2 = metric(g.dmx2, g.rdmx2), dmx3 = metric(g.dmx3, g.rdmx3);
-    const S Ct = p.Ct;
+    const auto dx1 = metric<S>(g.dx1, g.rdx1), dx2 = metric<S>(g.dx2, g.rdx2), dx3 = metric<S>(g.dx3, g.rdx3);
+    const auto dmx1 = metric<S>(g.dmx1, g.rdmx1), dmx2 = metric<S>(g.dmx2, g.rdmx2), dmx3 = metric<S>(g.dmx3, g.rdmx3);
+    const S Ct = sc<S>(p.Ct);
 
     const S Tc = T[g.ix(i, j, k)];
     const S Tim = T[g.ix(im, j, k)], Tip = T[g.ix(ip, j, k)];
@@ -113,15 +113,15 @@ __device__ __forceinline__ void thermal_z_row(const Grid &g, const Phys &p, doub
     th_kpp(p, Tkm, Kkm, dKkm);
     th_kpp(p, Tkp, Kkp, dKkp);
 
-    const S invRhoCpCt_half = 0.5 * Ct * irc;
-    const S dinvRhoCpCt_half = 0.5 * Ct * dirc;
+    const S invRhoCpCt_half = sc<S>(0.5) * Ct * irc;
+    const S dinvRhoCpCt_half = sc<S>(0.5) * Ct * dirc;
 
-    const S e1 = (dx1[i] * Tim + dx1[im] * Tc) * (0.5 / dmx1[i]);
-    const S e2 = (dx1[ip] * Tc + dx1[i] * Tip) * (0.5 / dmx1[ip]);
-    const S e3 = (dx2[j] * Tjm + dx2[jm] * Tc) * (0.5 / dmx2[j]);
-    const S e4 = (dx2[jp] * Tc + dx2[j] * Tjp) * (0.5 / dmx2[jp]);
-    const S e5 = (dx3[k] * Tkm + dx3[km] * Tc) * (0.5 / dmx3[k]);
-    const S e6 = (dx3[kp] * Tc + dx3[k] * Tkp) * (0.5 / dmx3[kp]);
+    const S e1 = (dx1[i] * Tim + dx1[im] * Tc) * (sc<S>(0.5) / dmx1[i]);
+    const S e2 = (dx1[ip] * Tc + dx1[i] * Tip) * (sc<S>(0.5) / dmx1[ip]);
+    const S e3 = (dx2[j] * Tjm + dx2[jm] * Tc) * (sc<S>(0.5) / dmx2[j]);
+    const S e4 = (dx2[jp] * Tc + dx2[j] * Tjp) * (sc<S>(0.5) / dmx2[jp]);
+    const S e5 = (dx3[k] * Tkm + dx3[km] * Tc) * (sc<S>(0.5) / dmx3[k]);
+    const S e6 = (dx3[kp] * Tc + dx3[k] * Tkp) * (sc<S>(0.5) / dmx3[kp]);
     const S dedx1 = (Tc - Tim) / dmx1[i];
     const S dedx2 = (Tip - Tc) / dmx1[ip];
     const S dedy3 = (Tc - Tjm) / dmx2[j];
@@ -132,23 +132,23 @@ __device__ __forceinline__ void thermal_z_row(const Grid &g, const Phys &p, doub
     const S Uc = U[g.ix(i, j, k)], Uip = U[g.ix(ip, j, k)];
     const S Vc = V[g.ix(i, j, k)], Vjp = V[g.ix(i, jp, k)];
     const S Wc = W[g.ix(i, j, k)], Wkp = W[g.ix(i, j, kp)];
-    const S convect_e1 = (Uc * dedx1 + Uip * dedx2) * 0.5;
-    const S convect_e2 = (Vc * dedy3 + Vjp * dedy4) * 0.5;
-    const S convect_e3 = (Wc * dedz5 + Wkp * dedz6) * 0.5;
-    S rhs = -0.5 * (convect_e1 + convect_e2 + convect_e3);
+    const S convect_e1 = (Uc * dedx1 + Uip * dedx2) * sc<S>(0.5);
+    const S convect_e2 = (Vc * dedy3 + Vjp * dedy4) * sc<S>(0.5);
+    const S convect_e3 = (Wc * dedz5 + Wkp * dedz6) * sc<S>(0.5);
+    S rhs = -sc<S>(0.5) * (convect_e1 + convect_e2 + convect_e3);
 
-    const S kPP1 = 0.5 / dmx1[i] * (dx1[i] * Kim + dx1[im] * Kc);
-    const S kPP2 = 0.5 / dmx1[ip] * (dx1[ip] * Kc + dx1[i] * Kip);
-    const S kPP3 = 0.5 / dmx2[j] * (dx2[j] * Kjm + dx2[jm] * Kc);
-    const S kPP4 = 0.5 / dmx2[jp] * (dx2[jp] * Kc + dx2[j] * Kjp);
-    const S kPP5 = 0.5 / dmx3[k] * (dx3[k] * Kkm + dx3[km] * Kc);
-    const S kPP6 = 0.5 / dmx3[kp] * (dx3[kp] * Kc + dx3[k] * Kkp);
-    const S dkPP1 = 0.5 / dmx1[i] * (dx1[i] * dKim + dx1[im] * dKc);
-    const S dkPP2 = 0.5 / dmx1[ip] * (dx1[ip] * dKc + dx1[i] * dKip);
-    const S dkPP3 = 0.5 / dmx2[j] * (dx2[j] * dKjm + dx2[jm] * dKc);
-    const S dkPP4 = 0.5 / dmx2[jp] * (dx2[jp] * dKc + dx2[j] * dKjp);
-    const S dkPP5 = 0.5 / dmx3[k] * (dx3[k] * dKkm + dx3[km] * dKc);
-    const S dkPP6 = 0.5 / dmx3[kp] * (dx3[kp] * dKc + dx3[k] * dKkp);
+    const S kPP1 = sc<S>(0.5) / dmx1[i] * (dx1[i] * Kim + dx1[im] * Kc);
+    const S kPP2 = sc<S>(0.5) / dmx1[ip] * (dx1[ip] * Kc + dx1[i] * Kip);
+    const S kPP3 = sc<S>(0.5) / dmx2[j] * (dx2[j] * Kjm + dx2[jm] * Kc);
+    const S kPP4 = sc<S>(0.5) / dmx2[jp] * (dx2[jp] * Kc + dx2[j] * Kjp);
+    const S kPP5 = sc<S>(0.5) / dmx3[k] * (dx3[k] * Kkm + dx3[km] * Kc);
+    const S kPP6 = sc<S>(0.5) / dmx3[kp] * (dx3[kp] * Kc + dx3[k] * Kkp);
+    const S dkPP1 = sc<S>(0.5) / dmx1[i] * (dx1[i] * dKim + dx1[im] * dKc);
+    const S dkPP2 = sc<S>(0.5) / dmx1[ip] * (dx1[ip] * dKc + dx1[i] * dKip);
+    const S dkPP3 = sc<S>(0.5) / dmx2[j] * (dx2[j] * dKjm + dx2[jm] * dKc);
+    const S dkPP4 = sc<S>(0.5) / dmx2[jp] * (dx2[jp] * dKc + dx2[j] * dKjp);
+    const S dkPP5 = sc<S>(0.5) / dmx3[k] * (dx3[k] * dKkm + dx3[km] * dKc);
+    const S dkPP6 = sc<S>(0.5) / dmx3[kp] * (dx3[kp] * dKc + dx3[k] * dKkp);
 
     const S viscous_e1 = (kPP2 * dedx2 - kPP1 * dedx1 - e2 * dkPP2 * dedx2 + e1 * dkPP1 * dedx1) / dx1[i];
     const S viscous_e2 = (kPP4 * dedy4 - kPP3 * dedy3 - e4 * dkPP4 * dedy4 + e3 * dkPP3 * dedy3) / dx2[j];
@@ -161,38 +161,38 @@ __device__ __forceinline__ void thermal_z_row(const Grid &g, const Phys &p, doub
                                    + (kPP6 * dedz6 - kPP5 * dedz5) / dx3[k]);
 
     const S tup = TBCup[g.b2(i, k)], tbt = TBCbt[g.b2(i, k)];
-    S ebc_up = -0.25 * Vjp / dmx2[jp] * tup;
-    S ebc_down = 0.25 * Vc / dmx2[j] * tbt;
+    S ebc_up = -sc<S>(0.25) * Vjp / dmx2[jp] * tup;
+    S ebc_down = sc<S>(0.25) * Vc / dmx2[j] * tbt;
     ebc_up = ebc_up
            + invRhoCpCt_half / dx2[j] * kPP4 / dmx2[jp] * tup
            + invRhoCpCt_half / dx2[j] * dkPP4 * dedy4 * tup;
     ebc_down = ebc_down
              + invRhoCpCt_half / dx2[j] * kPP3 / dmx2[j] * tbt
              - invRhoCpCt_half / dx2[j] * dkPP3 * dedy3 * tbt;
-    rhs = rhs + (double)(1 - jem) * ebc_down + (double)(1 - jep) * ebc_up;
+    rhs = rhs + sc<S>(1 - jem) * ebc_down + sc<S>(1 - jep) * ebc_up;
 
     const S eACK = invRhoCpCt_half / dx3[k] * (kPP6 / dmx3[kp] + kPP5 / dmx3[k]
-                                                    - dx3[kp] * (0.5 / dmx3[kp]) * dkPP6 * dedz6
-                                                    + dx3[km] * (0.5 / dmx3[k]) * dkPP5 * dedz5)
-                      + (-0.25 * Wkp / dmx3[kp] + 0.25 * Wc / dmx3[k]);
-    const S eAPK = -invRhoCpCt_half / dx3[k] * (kPP6 / dmx3[kp] + dx3[k] * (0.5 / dmx3[kp]) * dkPP6 * dedz6) + 0.25 * Wkp / dmx3[kp];
-    const S eAMK = -invRhoCpCt_half / dx3[k] * (kPP5 / dmx3[k] - dx3[k] * (0.5 / dmx3[k]) * dkPP5 * dedz5) - 0.25 * Wc / dmx3[k];
+                                                    - dx3[kp] * (sc<S>(0.5) / dmx3[kp]) * dkPP6 * dedz6
+                                                    + dx3[km] * (sc<S>(0.5) / dmx3[k]) * dkPP5 * dedz5)
+                      + (-sc<S>(0.25) * Wkp / dmx3[kp] + sc<S>(0.25) * Wc / dmx3[k]);
+    const S eAPK = -invRhoCpCt_half / dx3[k] * (kPP6 / dmx3[kp] + dx3[k] * (sc<S>(0.5) / dmx3[kp]) * dkPP6 * dedz6) + sc<S>(0.25) * Wkp / dmx3[kp];
+    const S eAMK = -invRhoCpCt_half / dx3[k] * (kPP5 / dmx3[k] - dx3[k] * (sc<S>(0.5) / dmx3[k]) * dkPP5 * dedz5) - sc<S>(0.25) * Wc / dmx3[k];
 
     const S eACI = invRhoCpCt_half / dx1[i] * (kPP2 / dmx1[ip] + kPP1 / dmx1[i]
-                                                    - dx1[ip] * (0.5 / dmx1[ip]) * dkPP2 * dedx2
-                                                    + dx1[im] * (0.5 / dmx1[i]) * dkPP1 * dedx1)
-                      + (0.25 * Uc / dmx1[i] - 0.25 * Uip / dmx1[ip]);
-    const S eAPI = -invRhoCpCt_half / dx1[i] * (kPP2 / dmx1[ip] + dx1[i] * (0.5 / dmx1[ip]) * dkPP2 * dedx2) + 0.25 * Uip / dmx1[ip];
-    const S eAMI = -invRhoCpCt_half / dx1[i] * (kPP1 / dmx1[i] - dx1[i] * (0.5 / dmx1[i]) * dkPP1 * dedx1) - 0.25 * Uc / dmx1[i];
+                                                    - dx1[ip] * (sc<S>(0.5) / dmx1[ip]) * dkPP2 * dedx2
+                                                    + dx1[im] * (sc<S>(0.5) / dmx1[i]) * dkPP1 * dedx1)
+                      + (sc<S>(0.25) * Uc / dmx1[i] - sc<S>(0.25) * Uip / dmx1[ip]);
+    const S eAPI = -invRhoCpCt_half / dx1[i] * (kPP2 / dmx1[ip] + dx1[i] * (sc<S>(0.5) / dmx1[ip]) * dkPP2 * dedx2) + sc<S>(0.25) * Uip / dmx1[ip];
+    const S eAMI = -invRhoCpCt_half / dx1[i] * (kPP1 / dmx1[i] - dx1[i] * (sc<S>(0.5) / dmx1[i]) * dkPP1 * dedx1) - sc<S>(0.25) * Uc / dmx1[i];
 
     const S eACJ = invRhoCpCt_half / dx2[j] * (kPP4 / dmx2[jp] + kPP3 / dmx2[j]
-                                                    - dx2[jp] * (0.5 / dmx2[jp]) * dkPP4 * dedy4 * (double)jep
-                                                    + dx2[jm] * (0.5 / dmx2[j]) * dkPP3 * dedy3 * (double)jem)
-                      + (0.25 * Vc / dmx2[j] - 0.25 * Vjp / dmx2[jp]);
-    S eAPJ = -invRhoCpCt_half / dx2[j] * (kPP4 / dmx2[jp] + dx2[j] * (0.5 / dmx2[jp]) * dkPP4 * dedy4) + 0.25 * Vjp / dmx2[jp];
-    S eAMJ = -invRhoCpCt_half / dx2[j] * (kPP3 / dmx2[j] - dx2[j] * (0.5 / dmx2[j]) * dkPP3 * dedy3) - 0.25 * Vc / dmx2[j];
-    eAPJ = eAPJ * (double)jep;
-    eAMJ = eAMJ * (double)jem;
+                                                    - dx2[jp] * (sc<S>(0.5) / dmx2[jp]) * dkPP4 * dedy4 * sc<S>(jep)
+                                                    + dx2[jm] * (sc<S>(0.5) / dmx2[j]) * dkPP3 * dedy3 * sc<S>(jem))
+                      + (sc<S>(0.25) * Vc / dmx2[j] - sc<S>(0.25) * Vjp / dmx2[jp]);
+    S eAPJ = -invRhoCpCt_half / dx2[j] * (kPP4 / dmx2[jp] + dx2[j] * (sc<S>(0.5) / dmx2[jp]) * dkPP4 * dedy4) + sc<S>(0.25) * Vjp / dmx2[jp];
+    S eAMJ = -invRhoCpCt_half / dx2[j] * (kPP3 / dmx2[j] - dx2[j] * (sc<S>(0.5) / dmx2[j]) * dkPP3 * dedy3) - sc<S>(0.25) * Vc / dmx2[j];
+    eAPJ = eAPJ * sc<S>(jep);
+    eAMJ = eAMJ * sc<S>(jem);
 
     rhs = rhs
         - (eAPK * Tkp + eACK * Tc + eAMK * Tkm
@@ -202,14 +202,14 @@ __device__ __forceinline__ void thermal_z_row(const Grid &g, const Phys &p, doub
                                       + (kPP4 * dedy4 - kPP3 * dedy3) / dx2[j]
                                       + (kPP6 * dedz6 - kPP5 * dedz5) / dx3[k]));
 
-    ac = eACK * dt
-          - 0.5 * dinvRhoCpCt_half * ((kPP2 * dedx2 - kPP1 * dedx1) / dx1[i]
+    ac = eACK * sc<S>(dt)
+          - sc<S>(0.5) * dinvRhoCpCt_half * ((kPP2 * dedx2 - kPP1 * dedx1) / dx1[i]
                                       + (kPP4 * dedy4 - kPP3 * dedy3) / dx2[j]
-                                      + (kPP6 * dedz6 - kPP5 * dedz5) / dx3[k]) * dt
-          + 1.0;
-    ap = eAPK * dt;
-    am = eAMK * dt;
-    rhsout = rhs * dt;
+                                      + (kPP6 * dedz6 - kPP5 * dedz5) / dx3[k]) * sc<S>(dt)
+          + sc<S>(1.0);
+    ap = eAPK * sc<S>(dt);
+    am = eAMK * sc<S>(dt);
+    rhsout = rhs * sc<S>(dt);
 }
 struct ThermalZ {
     Grid g;
@@ -217,6 +217,7 @@ struct ThermalZ {
     double dt;
     const double *T, *U, *V, *W, *TBCup, *TBCbt;
     static constexpr int NF = 6;  // field inputs, in the order below
+    static constexpr bool F32 = false;  // FP32 rows in the mixed form
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         thermal_z_row<S>(g, p, dt, w(0, T), w(1, U), w(2, V), w(3, W), w(4, TBCup), w(5, TBCbt), i, j, k, a, b, c, d);
@@ -235,8 +236,8 @@ __device__ __forceinline__ void thermal_x_row(const Grid &g, const Phys &p, doub
     int i, int j, int k, S &am, S &ac, S &ap) {
     const int im = i - 1, ip = i + 1;
     const int jm = g.n2m - 1, jp = g.n2m + 1, km = g.n3m - 1, kp = g.n3m + 1;
-    const auto dx1 = metric(g.dx1, g.rdx1), dx2 = metric(g.dx2, g.rdx2), dx3 = metric(g.dx3, g.rdx3);
-    const auto dmx1 = metric(g.dmx1, g.rdmx1), dmx2 = metric(g.dmx2, g.rdmx2), dmx3 = metric(g.dmx3, g.rdmx3);
+    const auto dx1 = metric<S>(g.dx1, g.rdx1), dx2 = metric<S>(g.dx2, g.rdx2), dx3 = metric<S>(g.dx3, g.rdx3);
+    const auto dmx1 = metric<S>(g.dmx1, g.rdmx1), dmx2 = metric<S>(g.dmx2, g.rdmx2), dmx3 = metric<S>(g.dmx3, g.rdmx3);
 
     const S Tc = T[g.ix(i, j, k)];
     const S Tim = T[g.ix(im, j, k)], Tip = T[g.ix(ip, j, k)];
@@ -251,15 +252,15 @@ __device__ __forceinline__ void thermal_x_row(const Grid &g, const Phys &p, doub
     const S Kjm = th_kpp_only(p, Tjm), Kjp = th_kpp_only(p, Tjp);
     const S Kkm = th_kpp_only(p, Tkm), Kkp = th_kpp_only(p, Tkp);
 
-    const S invRhoCpCt_half = 0.5 * p.Ct * irc;
-    const S kPP1 = 0.5 / dmx1[i] * (dx1[i] * Kim + dx1[im] * Kc);
-    const S kPP2 = 0.5 / dmx1[ip] * (dx1[ip] * Kc + dx1[i] * Kip);
-    const S kPP3 = 0.5 / dmx2[j] * (dx2[j] * Kjm + dx2[jm] * Kc);
-    const S kPP4 = 0.5 / dmx2[jp] * (dx2[jp] * Kc + dx2[j] * Kjp);
-    const S kPP5 = 0.5 / dmx3[k] * (dx3[k] * Kkm + dx3[km] * Kc);
-    const S kPP6 = 0.5 / dmx3[kp] * (dx3[kp] * Kc + dx3[k] * Kkp);
-    const S dkPP1 = 0.5 / dmx1[i] * (dx1[i] * dKim + dx1[im] * dKc);
-    const S dkPP2 = 0.5 / dmx1[ip] * (dx1[ip] * dKc + dx1[i] * dKip);
+    const S invRhoCpCt_half = sc<S>(0.5) * sc<S>(p.Ct) * irc;
+    const S kPP1 = sc<S>(0.5) / dmx1[i] * (dx1[i] * Kim + dx1[im] * Kc);
+    const S kPP2 = sc<S>(0.5) / dmx1[ip] * (dx1[ip] * Kc + dx1[i] * Kip);
+    const S kPP3 = sc<S>(0.5) / dmx2[j] * (dx2[j] * Kjm + dx2[jm] * Kc);
+    const S kPP4 = sc<S>(0.5) / dmx2[jp] * (dx2[jp] * Kc + dx2[j] * Kjp);
+    const S kPP5 = sc<S>(0.5) / dmx3[k] * (dx3[k] * Kkm + dx3[km] * Kc);
+    const S kPP6 = sc<S>(0.5) / dmx3[kp] * (dx3[kp] * Kc + dx3[k] * Kkp);
+    const S dkPP1 = sc<S>(0.5) / dmx1[i] * (dx1[i] * dKim + dx1[im] * dKc);
+    const S dkPP2 = sc<S>(0.5) / dmx1[ip] * (dx1[ip] * dKc + dx1[i] * dKip);
     const S dedx1 = (Tc - Tim) / dmx1[i];
     const S dedx2 = (Tip - Tc) / dmx1[ip];
     const S dedy3 = (Tc - Tjm) / dmx2[j];
@@ -269,17 +270,17 @@ __device__ __forceinline__ void thermal_x_row(const Grid &g, const Phys &p, doub
     const S Uc = U[g.ix(i, j, k)], Uip = U[g.ix(ip, j, k)];
 
     const S eACI = invRhoCpCt_half / dx1[i] * (kPP2 / dmx1[ip] + kPP1 / dmx1[i]
-                                                    - dx1[ip] * (0.5 / dmx1[ip]) * dkPP2 * dedx2
-                                                    + dx1[im] * (0.5 / dmx1[i]) * dkPP1 * dedx1)
-                      + (0.25 * Uc / dmx1[i] - 0.25 * Uip / dmx1[ip])
-                      - 0.5 * invRhoCpCt_half * ((kPP2 * dedx2 - kPP1 * dedx1) / dx1[i]
+                                                    - dx1[ip] * (sc<S>(0.5) / dmx1[ip]) * dkPP2 * dedx2
+                                                    + dx1[im] * (sc<S>(0.5) / dmx1[i]) * dkPP1 * dedx1)
+                      + (sc<S>(0.25) * Uc / dmx1[i] - sc<S>(0.25) * Uip / dmx1[ip])
+                      - sc<S>(0.5) * invRhoCpCt_half * ((kPP2 * dedx2 - kPP1 * dedx1) / dx1[i]
                                                  + (kPP4 * dedy4 - kPP3 * dedy3) / dx2[j]
                                                  + (kPP6 * dedz6 - kPP5 * dedz5) / dx3[k]);
-    const S eAPI = -invRhoCpCt_half / dx1[i] * (kPP2 / dmx1[ip] + dx1[i] * (0.5 / dmx1[ip]) * dkPP2 * dedx2) + 0.25 * Uip / dmx1[ip];
-    const S eAMI = -invRhoCpCt_half / dx1[i] * (kPP1 / dmx1[i] - dx1[i] * (0.5 / dmx1[i]) * dkPP1 * dedx1) - 0.25 * Uc / dmx1[i];
-    ac = eACI * dt + 1.0;
-    ap = eAPI * dt;
-    am = eAMI * dt;
+    const S eAPI = -invRhoCpCt_half / dx1[i] * (kPP2 / dmx1[ip] + dx1[i] * (sc<S>(0.5) / dmx1[ip]) * dkPP2 * dedx2) + sc<S>(0.25) * Uip / dmx1[ip];
+    const S eAMI = -invRhoCpCt_half / dx1[i] * (kPP1 / dmx1[i] - dx1[i] * (sc<S>(0.5) / dmx1[i]) * dkPP1 * dedx1) - sc<S>(0.25) * Uc / dmx1[i];
+    ac = eACI * sc<S>(dt) + sc<S>(1.0);
+    ap = eAPI * sc<S>(dt);
+    am = eAMI * sc<S>(dt);
 }
 struct ThermalX {
     Grid g;
@@ -287,6 +288,7 @@ struct ThermalX {
     double dt;
     const double *T, *U;
     static constexpr int NF = 2;  // field inputs, in the order below
+    static constexpr bool F32 = false;  // FP32 rows in the mixed form
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         thermal_x_row<S>(g, p, dt, w(0, T), w(1, U), i, j, k, a, b, c);
@@ -305,8 +307,8 @@ __device__ __forceinline__ void thermal_y_row(const Grid &g, const Phys &p, doub
     const int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1;
     const int km = g.n3m - 1, kp = g.n3m + 1;
     const int jem = g.jC[jm], jep = g.jC[jp];
-    const auto dx1 = metric(g.dx1, g.rdx1), dx2 = metric(g.dx2, g.rdx2), dx3 = metric(g.dx3, g.rdx3);
-    const auto dmx1 = metric(g.dmx1, g.rdmx1), dmx2 = metric(g.dmx2, g.rdmx2), dmx3 = metric(g.dmx3, g.rdmx3);
+    const auto dx1 = metric<S>(g.dx1, g.rdx1), dx2 = metric<S>(g.dx2, g.rdx2), dx3 = metric<S>(g.dx3, g.rdx3);
+    const auto dmx1 = metric<S>(g.dmx1, g.rdmx1), dmx2 = metric<S>(g.dmx2, g.rdmx2), dmx3 = metric<S>(g.dmx3, g.rdmx3);
 
     const S Tc = T[g.ix(i, j, k)];
     const S Tim = T[g.ix(im, j, k)], Tip = T[g.ix(ip, j, k)];
@@ -321,15 +323,15 @@ __device__ __forceinline__ void thermal_y_row(const Grid &g, const Phys &p, doub
     const S Kim = th_kpp_only(p, Tim), Kip = th_kpp_only(p, Tip);
     const S Kkm = th_kpp_only(p, Tkm), Kkp = th_kpp_only(p, Tkp);
 
-    const S invRhoCpCt_half = 0.5 * p.Ct * irc;
-    const S kPP1 = 0.5 / dmx1[i] * (dx1[i] * Kim + dx1[im] * Kc);
-    const S kPP2 = 0.5 / dmx1[ip] * (dx1[ip] * Kc + dx1[i] * Kip);
-    const S kPP3 = 0.5 / dmx2[j] * (dx2[j] * Kjm + dx2[jm] * Kc);
-    const S kPP4 = 0.5 / dmx2[jp] * (dx2[jp] * Kc + dx2[j] * Kjp);
-    const S kPP5 = 0.5 / dmx3[k] * (dx3[k] * Kkm + dx3[km] * Kc);
-    const S kPP6 = 0.5 / dmx3[kp] * (dx3[kp] * Kc + dx3[k] * Kkp);
-    const S dkPP3 = 0.5 / dmx2[j] * (dx2[j] * dKjm + dx2[jm] * dKc);
-    const S dkPP4 = 0.5 / dmx2[jp] * (dx2[jp] * dKc + dx2[j] * dKjp);
+    const S invRhoCpCt_half = sc<S>(0.5) * sc<S>(p.Ct) * irc;
+    const S kPP1 = sc<S>(0.5) / dmx1[i] * (dx1[i] * Kim + dx1[im] * Kc);
+    const S kPP2 = sc<S>(0.5) / dmx1[ip] * (dx1[ip] * Kc + dx1[i] * Kip);
+    const S kPP3 = sc<S>(0.5) / dmx2[j] * (dx2[j] * Kjm + dx2[jm] * Kc);
+    const S kPP4 = sc<S>(0.5) / dmx2[jp] * (dx2[jp] * Kc + dx2[j] * Kjp);
+    const S kPP5 = sc<S>(0.5) / dmx3[k] * (dx3[k] * Kkm + dx3[km] * Kc);
+    const S kPP6 = sc<S>(0.5) / dmx3[kp] * (dx3[kp] * Kc + dx3[k] * Kkp);
+    const S dkPP3 = sc<S>(0.5) / dmx2[j] * (dx2[j] * dKjm + dx2[jm] * dKc);
+    const S dkPP4 = sc<S>(0.5) / dmx2[jp] * (dx2[jp] * dKc + dx2[j] * dKjp);
     const S dedx1 = (Tc - Tim) / dmx1[i];
     const S dedx2 = (Tip - Tc) / dmx1[ip];
     const S dedy3 = (Tc - Tjm) / dmx2[j];
@@ -339,19 +341,19 @@ __device__ __forceinline__ void thermal_y_row(const Grid &g, const Phys &p, doub
     const S Vc = V[g.ix(i, j, k)], Vjp = V[g.ix(i, jp, k)];
 
     const S eACJ = invRhoCpCt_half / dx2[j] * (kPP4 / dmx2[jp] + kPP3 / dmx2[j]
-                                                    - dx2[jp] * (0.5 / dmx2[jp]) * dkPP4 * dedy4 * (double)jep
-                                                    + dx2[jm] * (0.5 / dmx2[j]) * dkPP3 * dedy3 * (double)jem)
-                      + (0.25 * Vc / dmx2[j] - 0.25 * Vjp / dmx2[jp])
-                      - 0.5 * invRhoCpCt_half * ((kPP2 * dedx2 - kPP1 * dedx1) / dx1[i]
+                                                    - dx2[jp] * (sc<S>(0.5) / dmx2[jp]) * dkPP4 * dedy4 * sc<S>(jep)
+                                                    + dx2[jm] * (sc<S>(0.5) / dmx2[j]) * dkPP3 * dedy3 * sc<S>(jem))
+                      + (sc<S>(0.25) * Vc / dmx2[j] - sc<S>(0.25) * Vjp / dmx2[jp])
+                      - sc<S>(0.5) * invRhoCpCt_half * ((kPP2 * dedx2 - kPP1 * dedx1) / dx1[i]
                                                  + (kPP4 * dedy4 - kPP3 * dedy3) / dx2[j]
                                                  + (kPP6 * dedz6 - kPP5 * dedz5) / dx3[k]);
-    S eAPJ = -invRhoCpCt_half / dx2[j] * (kPP4 / dmx2[jp] + dx2[j] * (0.5 / dmx2[jp]) * dkPP4 * dedy4) + 0.25 * Vjp / dmx2[jp];
-    S eAMJ = -invRhoCpCt_half / dx2[j] * (kPP3 / dmx2[j] - dx2[j] * (0.5 / dmx2[j]) * dkPP3 * dedy3) - 0.25 * Vc / dmx2[j];
-    eAPJ = eAPJ * (double)jep;
-    eAMJ = eAMJ * (double)jem;
-    ac = eACJ * dt + 1.0;
-    ap = eAPJ * dt;
-    am = eAMJ * dt;
+    S eAPJ = -invRhoCpCt_half / dx2[j] * (kPP4 / dmx2[jp] + dx2[j] * (sc<S>(0.5) / dmx2[jp]) * dkPP4 * dedy4) + sc<S>(0.25) * Vjp / dmx2[jp];
+    S eAMJ = -invRhoCpCt_half / dx2[j] * (kPP3 / dmx2[j] - dx2[j] * (sc<S>(0.5) / dmx2[j]) * dkPP3 * dedy3) - sc<S>(0.25) * Vc / dmx2[j];
+    eAPJ = eAPJ * sc<S>(jep);
+    eAMJ = eAMJ * sc<S>(jem);
+    ac = eACJ * sc<S>(dt) + sc<S>(1.0);
+    ap = eAPJ * sc<S>(dt);
+    am = eAMJ * sc<S>(dt);
 }
 struct ThermalY {
     Grid g;
@@ -359,6 +361,7 @@ struct ThermalY {
     double dt;
     const double *T, *V;
     static constexpr int NF = 2;  // field inputs, in the order below
+    static constexpr bool F32 = false;  // FP32 rows in the mixed form
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         thermal_y_row<S>(g, p, dt, w(0, T), w(1, V), i, j, k, a, b, c);
