@@ -958,8 +958,7 @@ int momentum_blockLdU_pseudo(tcs_ctx *c, double dt, double *Un, double *Wn) {
     const dim3 blk = interior_block(), grd = cell_grid(c->n1m, c->n2m, c->n3m, LDU_KB);
     const BlockLdU cell{c->grid(), c->phys, dt, c->f[TCS_U], c->f[TCS_DV], c->f[TCS_DW],
                         c->f[TCS_MU], c->f[TCS_INVRHO], c->f[TCS_DU]};
-    TCS_TRY(copy_shell(c, c->f[TCS_U], Un));
-    TCS_TRY(copy_shell(c, c->f[TCS_W], Wn));
+    TCS_TRY(copy_shell(c, {{c->f[TCS_U], Un}, {c->f[TCS_W], Wn}}));
     TCS_LAUNCH(c->stream, "blockLdU", k_blockLdU_pseudo, grd, blk, 0, cell, c->f[TCS_W], c->f[TCS_DW], c->f[TCS_DU],
                Un, Wn);
     return TCS_OK;

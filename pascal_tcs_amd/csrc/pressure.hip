@@ -751,8 +751,7 @@ int pressure_projection_into(tcs_ctx *c, double dt, double *dPhatn, double *dP0n
     const dim3 blk = interior_block(), grd = cell_grid(c->n1m, c->n2m, c->n3m, PROJ_KB);
     const Projection cell{g, c->phys, dt, c->f[TCS_INVRHO], c->f[TCS_DP], c->f[TCS_DPHAT], c->f[TCS_DP0],
                           c->f[TCS_U], c->f[TCS_V], c->f[TCS_W], c->f[TCS_P]};
-    TCS_TRY(copy_shell(c, c->f[TCS_DPHAT], dPhatn));
-    TCS_TRY(copy_shell(c, c->f[TCS_DP0], dP0n));
+    TCS_TRY(copy_shell(c, {{c->f[TCS_DPHAT], dPhatn}, {c->f[TCS_DP0], dP0n}}));
     TCS_LAUNCH(c->stream, "projection", k_projection_into, grd, blk, 0, cell,
                DPhatUpdate{g, c->f[TCS_DP], c->f[TCS_DPHAT], c->f[TCS_DP0]}, c->f[TCS_U], c->f[TCS_V], c->f[TCS_W],
                c->f[TCS_P], dPhatn, dP0n);

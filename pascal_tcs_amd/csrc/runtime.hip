@@ -104,7 +104,8 @@ struct ProfStage {
 // the value at (mx(i), j, mz(k)), where mx/mz fold ghost indices onto the
 // periodic interior.  Sources are never ghosts in x or z, so one kernel does
 // it without ordering hazards.
-__global__ void k_halo(Grid g, double *__restrict__ f) {
+__global__ void k_halo(Grid g, FieldSet fs) {
+    double *__restrict__ f = fs.f[blockIdx.y];  // one field per grid row
     const int N1 = g.N1, N2 = g.N2, N3 = g.N3;
     const int n1m = g.n1m, n3m = g.n3m;
     // threads over the x-ghost faces (2*N2*N3) then the z-ghost faces (2*N1*N2)
@@ -133,7 +134,9 @@ __global__ void k_halo(Grid g, double *__restrict__ f) {
 // the cells of a full field outside the interior: the faces i in {0, n1m+1}
 // (all j, k), then j in {0, n2m+1} (interior i, all k), then k in {0, n3m+1}
 // (interior i and j)
-__global__ void k_copy_shell(Grid g, const double *__restrict__ src, double *__restrict__ dst) {
+__global__ void k_copy_shell(Grid g, FieldSet src_set, FieldSet dst_set) {
+    const double *__restrict__ src = src_set.f[blockIdx.y];  // one pair per grid row
+    double *__restrict__ dst = dst_set.f[blockIdx.y];
     const int N1 = g.N1, N2 = g.N2, N3 = g.N3, n1m = g.n1m, n2m = g.n2m;
     const size_t fi = (size_t)2 * N2 * N3, fj = (size_t)2 * n1m * N3, fk = (size_t)2 * n1m * n2m;
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -156,18 +159,34 @@ __global__ void k_copy_shell(Grid g, const double *__restrict__ src, double *__r
     dst[o] = src[o];
 }
 
-int copy_shell(tcs_ctx *c, const double *src, double *dst) {
+int copy_shell(tcs_ctx *c, std::initializer_list<std::pair<const double *, double *>> pairs) {
+    FieldSet src{}, dst{};
+    int nf = 0;
+    for (const auto &pr : pairs) {
+        if (nf == FieldSet::MAX) return TCS_E_ARG;
+        src.f[nf] = const_cast<double *>(pr.first);
+        dst.f[nf++] = pr.second;
+    }
     const size_t n = (size_t)2 * c->N2 * c->N3 + (size_t)2 * c->n1m * c->N3 + (size_t)2 * c->n1m * c->n2m;
-    TCS_LAUNCH(c->stream, "copy_shell", k_copy_shell, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->grid(), src,
-               dst);
+    TCS_LAUNCH(c->stream, "copy_shell", k_copy_shell, dim3((unsigned)((n + 255) / 256), (unsigned)nf), dim3(256), 0,
+               c->grid(), src, dst);
     return TCS_OK;
 }
+int copy_shell(tcs_ctx *c, const double *src, double *dst) { return copy_shell(c, {{src, dst}}); }
 
-int halo(tcs_ctx *c, double *f) {
+int halo(tcs_ctx *c, std::initializer_list<double *> fields) {
+    FieldSet fs{};
+    int nf = 0;
+    for (double *f : fields) {
+        if (nf == FieldSet::MAX) return TCS_E_ARG;
+        fs.f[nf++] = f;
+    }
     const size_t n = (size_t)2 * c->N2 * c->N3 + (size_t)2 * c->N1 * c->N2;
-    TCS_LAUNCH(c->stream, "halo", k_halo, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->grid(), f);
+    TCS_LAUNCH(c->stream, "halo", k_halo, dim3((unsigned)((n + 255) / 256), (unsigned)nf), dim3(256), 0, c->grid(),
+               fs);
     return TCS_OK;
 }
+int halo(tcs_ctx *c, double *f) { return halo(c, {f}); }
 
 // ------------------------------------------------------------ reductions --
 constexpr int RED_BLOCKS = 1024;
@@ -858,20 +877,14 @@ extern "C" int tcs_step(tcs_ctx *c, double dt) {
     RUN(H, halo(c, c->f[TCS_DV]));
     RUN("momentum blockLdU", FORM(c, momentum_blockLdU_pseudo, dt, c->spare[TCS_U], c->spare[TCS_W]));
     for (int q : {TCS_U, TCS_V, TCS_W}) std::swap(c->f[q], c->spare[q]);
-    RUN(H, halo(c, c->f[TCS_U]));
-    RUN(H, halo(c, c->f[TCS_V]));
-    RUN(H, halo(c, c->f[TCS_W]));
+    RUN(H, halo(c, {c->f[TCS_U], c->f[TCS_V], c->f[TCS_W]}));
     // the right-hand side is made inside the solve's x transform
     RUN("pressure PoissonSolve", FORM(c, pressure_poisson_rhs, dt));
     RUN(H, halo(c, c->f[TCS_DP]));
     RUN("pressure projection", FORM(c, pressure_projection_into, dt, c->spare[TCS_DPHAT], c->spare[TCS_DP0]));
     std::swap(c->f[TCS_DPHAT], c->spare[TCS_DPHAT]);
     std::swap(c->f[TCS_DP0], c->spare[TCS_DP0]);
-    RUN(H, halo(c, c->f[TCS_U]));
-    RUN(H, halo(c, c->f[TCS_V]));
-    RUN(H, halo(c, c->f[TCS_W]));
-    RUN(H, halo(c, c->f[TCS_P]));
-    RUN(H, halo(c, c->f[TCS_DPHAT]));
+    RUN(H, halo(c, {c->f[TCS_U], c->f[TCS_V], c->f[TCS_W], c->f[TCS_P], c->f[TCS_DPHAT]}));
     return TCS_OK;
 }
 

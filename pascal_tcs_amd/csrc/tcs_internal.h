@@ -6,7 +6,9 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <initializer_list>
 #include <string>
+#include <utility>
 
 #include "../../include/pascal_tcs.h"
 
@@ -240,9 +242,16 @@ namespace exact { TCS_STAGE_DECLS }
 namespace fast { TCS_STAGE_DECLS }
 namespace mixed { TCS_STAGE_DECLS }
 #undef TCS_STAGE_DECLS
+// up to FieldSet::MAX full fields for one launch (blockIdx.y picks one)
+struct FieldSet {
+    static constexpr int MAX = 6;
+    double *f[MAX];
+};
 int halo(tcs_ctx *c, double *f);
+int halo(tcs_ctx *c, std::initializer_list<double *> fields);
 // copy the cells outside the interior (x/z ghosts, wall rows) of a full field
 int copy_shell(tcs_ctx *c, const double *src, double *dst);
+int copy_shell(tcs_ctx *c, std::initializer_list<std::pair<const double *, double *>> pairs);
 int post_div(tcs_ctx *c, double *out);
 int post_cfl(tcs_ctx *c, double *maxuovx);
 int nusselt(tcs_ctx *c, double Tup, double Tbt, double *nb, double *nt);
