@@ -317,7 +317,7 @@ __device__ __forceinline__ void xcd_tiles(int &bx, int &by) {
 // as soon as it is made, so no coefficient array goes through memory; the
 // thread layout, the reduced solve and the output (D at s0 + s1*ss1 +
 // row*rs, or a field) are those of the split solve of stored lines.
-template <typename T, bool CYC, class Src, int SX>
+template <typename T, bool CYC, class Src, int SX, bool FULL>
 __global__ void __launch_bounds__(SX * PT_PMAX)
 k_ptdma_part_src(Src src, T *D, int n0, long ss1, long rs, int P, int w1, int w2, LineOut out) {
     using namespace part;
@@ -332,14 +332,26 @@ k_ptdma_part_src(Src src, T *D, int n0, long ss1, long rs, int P, int w1, int w2
     range(p, w1, w2, j0, m);
 
     T a[PT_MMAX], c[PT_MMAX], d[1][PT_MMAX];
+    // FULL (every part holds PT_MMAX rows): one unconditional run of rows
+    // for the active lines, so consecutive rows make the loads and terms
+    // they share once
+    if (FULL && act) {
 #pragma unroll
-    for (int r = 0; r < PT_MMAX; r++) {
-        if (r < m) {
-            T ar = T(0), br = T(1), cr = T(0), dr[1] = {T(0)};
-            if (act) src(s0, s1, j0 + r, ar, br, cr, dr[0]);
+        for (int r = 0; r < PT_MMAX; r++) {
+            T ar, br, cr, dr[1];
+            src(s0, s1, j0 + r, ar, br, cr, dr[0]);
             reduce_row<T, 1>(r, ar, br, cr, dr, a, c, d);
-        } else {
-            a[r] = T(0); c[r] = T(0); d[0][r] = T(0);
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < PT_MMAX; r++) {
+            if (r < m) {
+                T ar = T(0), br = T(1), cr = T(0), dr[1] = {T(0)};
+                if (act) src(s0, s1, j0 + r, ar, br, cr, dr[0]);
+                reduce_row<T, 1>(r, ar, br, cr, dr, a, c, d);
+            } else {
+                a[r] = T(0); c[r] = T(0); d[0][r] = T(0);
+            }
         }
     }
     reduce_back<T, 1>(a, c, d, m);
@@ -473,7 +485,17 @@ int ptdma_part_src_launch(hipStream_t st, const char *label, const Src &src, T *
         return TCS_E_ARG;
     }
     const dim3 blk(SX * P), grd((unsigned)((n0 + SX - 1) / SX), (unsigned)n1);
-    TCS_LAUNCH(st, label, (k_ptdma_part_src<T, CYC, Src, SX>), grd, blk, 0, src, D, n0, ss1, rs, P, w1, w2, out);
+    // FP64 y lines whose parts all hold PT_MMAX rows take the unconditional
+    // row loop (thermal y 2.63 -> 2.00 ms, dU y 1.52 -> 1.02 ms, dW y 1.43 ->
+    // 1.29 ms at 512^3); the z lines keep the guarded loop (their
+    // unconditional form spills), and so do the FP32 lines of the mixed form
+    // (1.5 ms slower per step with it)
+    if (!CYC && sizeof(T) == 8 && w1 == PT_MMAX && w2 == 0)
+        TCS_LAUNCH(st, label, (k_ptdma_part_src<T, CYC, Src, SX, true>), grd, blk, 0, src, D, n0, ss1, rs, P, w1, w2,
+                   out);
+    else
+        TCS_LAUNCH(st, label, (k_ptdma_part_src<T, CYC, Src, SX, false>), grd, blk, 0, src, D, n0, ss1, rs, P, w1, w2,
+                   out);
     return TCS_OK;
 }
 
