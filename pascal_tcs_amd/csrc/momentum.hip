@@ -213,6 +213,7 @@ struct DUZ {
     const double *U, *V, *W, *P, *T, *Mu, *iR, *UBCup, *UBCbt, *VBCup, *VBCbt;
     static constexpr int NF = 11;  // field inputs, in the order below
     static constexpr bool F32 = false;  // FP32 rows in the mixed form
+    static constexpr bool FULL = true;  // unconditional row loop in the FP64 z solve
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         dU_z_row<S>(g, p, dt, w(0, U), w(1, V), w(2, W), w(3, P), w(4, T), w(5, Mu), w(6, iR), w(7, UBCup), w(8, UBCbt), w(9, VBCup), w(10, VBCbt), i, j, k, a, b, c, d);
@@ -409,6 +410,7 @@ struct DVZ {
     const double *U, *V, *W, *P, *dU, *Mu, *iR, *VBCup, *VBCbt;
     static constexpr int NF = 9;  // field inputs, in the order below
     static constexpr bool F32 = false;  // FP32 rows in the mixed form
+    static constexpr bool FULL = false;  // unconditional row loop in the FP64 z solve
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         dV_z_row<S>(g, p, dt, w(0, U), w(1, V), w(2, W), w(3, P), w(4, dU), w(5, Mu), w(6, iR), w(7, VBCup), w(8, VBCbt), i, j, k, a, b, c, d);
@@ -601,6 +603,7 @@ struct DWZ {
     const double *U, *V, *W, *P, *dU, *dV, *Mu, *iR, *VBCup, *VBCbt, *WBCup, *WBCbt;
     static constexpr int NF = 12;  // field inputs, in the order below
     static constexpr bool F32 = false;  // FP32 rows in the mixed form
+    static constexpr bool FULL = false;  // unconditional row loop in the FP64 z solve
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         dW_z_row<S>(g, p, dt, w(0, U), w(1, V), w(2, W), w(3, P), w(4, dU), w(5, dV), w(6, Mu), w(7, iR), w(8, VBCup), w(9, VBCbt), w(10, WBCup), w(11, WBCbt), i, j, k, a, b, c, d);

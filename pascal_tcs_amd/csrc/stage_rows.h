@@ -86,6 +86,10 @@ int rows3d_launch(hipStream_t st, const char *label, const Row &row, int jlo, S 
 // the right-hand side comes with the row.
 template <class Row, typename S>
 struct ZLineRows {
+    // the split solve's unconditional row loop where the row asks for it
+    // (measured per stage at 512^3: dU z 4.14 -> 3.14 ms; the thermal, dV and
+    // dW z rows spill in it and lose 0.7-2.8 ms)
+    static constexpr bool full_rows = Row::FULL;
     Row row;
     int jlo;
     __device__ __forceinline__ void operator()(int s0, int s1, int r, S &a, S &b, S &c, S &d) const {
@@ -97,6 +101,7 @@ struct ZLineRows {
 // the right-hand side is the previous stage's solution in the scratch.
 template <class Row, typename S>
 struct YLineRows {
+    static constexpr bool full_rows = true;  // the unconditional row loop pays for every y stage
     Row row;
     const S *rhs;
     int jlo;

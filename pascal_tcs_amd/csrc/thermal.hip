@@ -218,6 +218,7 @@ struct ThermalZ {
     const double *T, *U, *V, *W, *TBCup, *TBCbt;
     static constexpr int NF = 6;  // field inputs, in the order below
     static constexpr bool F32 = false;  // FP32 rows in the mixed form
+    static constexpr bool FULL = false;  // unconditional row loop in the FP64 z solve
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         thermal_z_row<S>(g, p, dt, w(0, T), w(1, U), w(2, V), w(3, W), w(4, TBCup), w(5, TBCbt), i, j, k, a, b, c, d);
