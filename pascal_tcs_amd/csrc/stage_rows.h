@@ -56,15 +56,16 @@ __device__ __forceinline__ void row_as(const Row &row, int i, int j, int k, S &a
 // k (thermal x: the conductivity at the fixed k planes; 1.92 -> 1.47 ms at
 // 512^3).
 constexpr int ROWS3D_KB = 4;
+constexpr int ROWS3D_IB = 1;  // consecutive i per thread (2: thermal x 1.53 -> 2.05 ms)
 template <class Row, bool RHS, typename S>
 __global__ void __launch_bounds__(256)
 k_rows3d(Row row, int jlo, S *__restrict__ am, S *__restrict__ ac, S *__restrict__ ap, S *__restrict__ rhs) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * ROWS3D_IB + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    if (i > row.g.n1m || j > row.g.n2m || j < jlo) return;
+    if (i0 > row.g.n1m || j > row.g.n2m || j < jlo) return;
 #pragma unroll
-    for (int q = 0; q < ROWS3D_KB; q++) {
-        const int k = blockIdx.z * ROWS3D_KB + q + 1;
+    for (int q = 0; q < ROWS3D_KB * ROWS3D_IB; q++) {
+        const int i = i0 + q % ROWS3D_IB, k = blockIdx.z * ROWS3D_KB + q / ROWS3D_IB + 1;
         S a, b, c, d;
         row_as<S>(row, i, j, k, a, b, c, d);
         const size_t o = row.g.r3(i, j, k);
@@ -77,7 +78,7 @@ k_rows3d(Row row, int jlo, S *__restrict__ am, S *__restrict__ ac, S *__restrict
 
 template <bool RHS, class Row, typename S>
 int rows3d_launch(hipStream_t st, const char *label, const Row &row, int jlo, S *am, S *ac, S *ap, S *rhs) {
-    const dim3 blk = interior_block(), grd = cell_grid(row.g.n1m, row.g.n2m, row.g.n3m, ROWS3D_KB);
+    const dim3 blk = interior_block(), grd = cell_grid(row.g.n1m, row.g.n2m, row.g.n3m, ROWS3D_KB, interior_block(), ROWS3D_IB);
     TCS_LAUNCH(st, label, (k_rows3d<Row, RHS, S>), grd, blk, 0, row, jlo, am, ac, ap, rhs);
     return TCS_OK;
 }

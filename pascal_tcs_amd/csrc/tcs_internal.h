@@ -166,8 +166,8 @@ inline dim3 interior_grid(int nx, int ny, int nz) {
 // Cell-update stencil kernels where a thread takes KB consecutive k planes
 // of one (i, j) (KB <= 4 divides n3m, a power of two >= 4): the values a
 // plane shares with the next are loaded once.
-inline dim3 cell_grid(int nx, int ny, int nz, int kb, dim3 blk = interior_block()) {
-    return dim3((unsigned)((nx + blk.x - 1) / blk.x), (unsigned)((ny + blk.y - 1) / blk.y), (unsigned)(nz / kb));
+inline dim3 cell_grid(int nx, int ny, int nz, int kb, dim3 blk = interior_block(), int ib = 1) {
+    return dim3((unsigned)((nx / ib + blk.x - 1) / blk.x), (unsigned)((ny + blk.y - 1) / blk.y), (unsigned)(nz / kb));
 }
 
 // TDMA launcher (exact PaScaL order) on strided systems:
