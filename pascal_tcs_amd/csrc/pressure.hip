@@ -219,10 +219,24 @@ k_fft_x_fwd(Grid g, int h1p, int TX, Src src, const double2 *__restrict__ tw, do
     const int j = jb + 1;
     const int k0 = kb * TX;  // 0-based
     const int tid = threadIdx.x;
-    for (int e = tid; e < TX * n; e += NT) {
-        const int l = e / n, t = e - l * n;
-        re[e] = src(t + 1, j, k0 + l + 1);
-        im[e] = 0.0;
+    if (TX == 4 && n == 2 * NT) {
+        // the 512-point shape, unrolled: a thread's two x positions over the
+        // four k-lines, so the k-lines share their loads
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+#pragma unroll
+            for (int l = 0; l < 4; l++) {
+                const int t = tid + h * NT;
+                re[l * n + t] = src(t + 1, j, k0 + l + 1);
+                im[l * n + t] = 0.0;
+            }
+        }
+    } else {
+        for (int e = tid; e < TX * n; e += NT) {
+            const int l = e / n, t = e - l * n;
+            re[e] = src(t + 1, j, k0 + l + 1);
+            im[e] = 0.0;
+        }
     }
     __syncthreads();
     lds_fft<-1, NT>(re, im, n, TX, tw);
