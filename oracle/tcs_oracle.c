@@ -8,7 +8,7 @@
  * from loop variables left over from an earlier stage (stale jm/jp/km/kp,
  * mu1..mu4) are reproduced by construction.
  *
- * Deliberate deviations from the reference (all documented in DESIGN.md):
+ * Deliberate deviations from the reference:
  *  D1. FFTs: MKL/FFTW (closed) is replaced by a radix-2 Stockham FFT with a
  *      host twiddle table (orc_fft); x r2c is a length-n1m complex FFT of
  *      (x,0) keeping bins 0..n1m/2; c2r builds the Hermitian extension with
