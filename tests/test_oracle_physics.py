@@ -12,6 +12,7 @@ What each test fixes independently of the restated formulas:
   projection  the corrected velocity is divergence-free (constant density)
   thermal     the linear conduction profile is a fixed point
   momentum    at rest with uniform T: dU uniform in x,z, y-symmetric; dV = dW = 0
+  blockLdV/U  closed forms of every two-point stencil on polynomial fields
 """
 import numpy as np
 import pytest
@@ -207,3 +208,125 @@ def test_momentum_at_rest_uniform_buoyancy():
     o.momentum_solvedW(1e-2)
     assert np.abs(o.f["dV"]).max() < 1e-12 * np.abs(dU).max()
     assert np.abs(o.f["dW"]).max() < 1e-12 * np.abs(dU).max()
+
+
+# ---- block corrections on polynomial fields ---------------------------------
+# The block-LU corrections (src/module_solve_momentum.f90:1278-1460) are made
+# of two-point stencils that act exactly on low-order polynomials of the cell
+# coordinates: a difference quotient of a quadratic is its derivative at the
+# midpoint, a mean of a linear function is its value at the midpoint, and the
+# spacing-weighted face interpolation of y^2 exceeds the face value by the
+# product of the two half-widths.  Fields built from such polynomials give
+# every stencil term in closed form from the geometry alone (where each face
+# and centre lies), so a wrong neighbour (k for k+1, j+1 for j-1), transposed
+# interpolation weights on the stretched y mesh, a dropped term or a wrong
+# sign changes the result.  The corrections themselves are half the explicit
+# coupling terms of the Crank-Nicolson split: dV -= dt/2 (M23 dW) and
+# dU -= dt/2 (M12 dV + M13 dW).
+
+def _centres(cfg):
+    """Per axis: cell-centre coordinates (ghosts included; centre j lies
+    dmx[j] beyond centre j-1), face coordinates (face j between centres j-1
+    and j lies half a cell dx[j-1] beyond centre j-1), dx, dmx."""
+    out = []
+    for _, dx, dmx in mesh(cfg):
+        # the face is where the two half-cells meet only if dmx[j] is their sum
+        assert np.allclose(dmx[1:], 0.5 * (dx[:-1] + dx[1:]), rtol=1e-14, atol=0)
+        c = np.concatenate([[0.0], np.cumsum(dmx[1:])])
+        f = np.concatenate([[np.nan], c[:-1] + 0.5 * dx[:-1]])
+        out.append((c, f, dx, dmx))
+    return out
+
+
+def test_blockLdV_on_polynomial_fields():
+    cfg = small_config(16, 12, 8, props="constant")
+    o = make_oracle(cfg)
+    (xc, _, _, _), (yc, yf, dx2, _), (zc, _, dx3, _) = _centres(cfg)
+    Z, Y, X = np.meshgrid(zc, yc, xc, indexing="ij")
+    a, (c0, c1, c2), (g0, g1, g2) = 0.7, (0.3, -1.1, 0.45), (0.2, 0.6, -0.25)
+    (m0, m1, m2), (r0, r2) = (1.3, 0.21, -0.17), (0.9, 0.12)
+    Qy = lambda y: c0 + c1 * y + c2 * y * y
+    Gz = lambda z: g0 + g1 * z + g2 * z * z
+    o.f["V"][...] = a * Z * Z + 0.4 * X
+    o.f["dW"][...] = Qy(Y) * Gz(Z)
+    o.f["Mu"][...] = m0 + m1 * Z + m2 * Y
+    o.f["invRho"][...] = r0 + r2 * Y
+    rng = np.random.default_rng(21)
+    dV0 = rng.standard_normal(o.f["dV"].shape)
+    o.f["dV"][...] = dV0
+    dt, Cmu = 3e-3, cfg.constants()["Cmu"]
+    o.momentum_blockLdV(dt)
+
+    exp = dV0.copy()
+    n1m, n2m, n3m = cfg.n1m, cfg.n2m, cfg.n3m
+    for k in range(1, n3m + 1):
+        for j in range(2, n2m + 1):  # the wall faces j = 1, n2m + 1 are not updated
+            hm, hp = 0.5 * dx2[j - 1], 0.5 * dx2[j]
+            dw_face = Qy(yf[j]) + c2 * hm * hp          # dW interpolated to the V face
+            ddwdy = c1 + c2 * (yc[j] + yc[j - 1])       # dW difference quotient across it
+            dvdz5, dvdz6 = a * (zc[k] + zc[k - 1]), a * (zc[k + 1] + zc[k])
+            mu5 = m0 + m2 * yf[j] + m1 * 0.5 * (zc[k - 1] + zc[k])
+            mu6 = m0 + m2 * yf[j] + m1 * 0.5 * (zc[k] + zc[k + 1])
+            inv_rho = r0 + r2 * yf[j]
+            conv = 0.5 * 0.5 * (dw_face * Gz(zc[k]) * dvdz5 + dw_face * Gz(zc[k + 1]) * dvdz6)
+            visc = 0.5 * Cmu * inv_rho * (mu6 * ddwdy * Gz(zc[k + 1]) - mu5 * ddwdy * Gz(zc[k])) / dx3[k]
+            exp[k, j, 1:n1m + 1] -= dt * (conv - visc)
+    corr = np.abs(exp - dV0).max()
+    assert corr > 1e-4
+    assert np.abs(o.f["dV"] - exp).max() <= 1e-12 * corr
+
+
+def test_blockLdU_on_polynomial_fields():
+    cfg = small_config(16, 12, 8, props="constant")
+    o = make_oracle(cfg)
+    (xc, xf, dx1, _), (yc, yf, dx2, _), (zc, _, dx3, _) = _centres(cfg)
+    Z, Y, X = np.meshgrid(zc, yc, xc, indexing="ij")
+    rng = np.random.default_rng(23)
+    shape = o.f["dU"].shape
+    Hv = rng.standard_normal(shape[:2])[:, :, None]  # dV's (k, j) profile, wall faces included
+    Hw = rng.standard_normal(shape[:2])[:, :, None]
+    u1, u2 = 0.8, -0.6
+    cv, cw = (0.4, -0.9, 0.35), (-0.2, 0.5, 0.3)
+    (m0, m1, m2, m3), (r0, r1, r2) = (1.2, 0.15, -0.2, 0.1), (0.8, -0.07, 0.11)
+    Q = lambda c, x: c[0] + c[1] * x + c[2] * x * x
+    o.f["U"][...] = u1 * Y * Y + u2 * Z * Z + 0.3 * X
+    o.f["dV"][...] = Q(cv, X) * Hv
+    o.f["dW"][...] = Q(cw, X) * Hw
+    o.f["Mu"][...] = m0 + m1 * X + m2 * Y + m3 * Z
+    o.f["invRho"][...] = r0 + r1 * X + r2 * Y
+    dU0 = rng.standard_normal(shape)
+    o.f["dU"][...] = dU0
+    dt, Cmu = 3e-3, cfg.constants()["Cmu"]
+    o.momentum_blockLdU(dt)
+
+    n1m, n2m, n3m = cfg.n1m, cfg.n2m, cfg.n3m
+    i = np.arange(1, n1m + 1)
+    h = 0.5 * dx1[1]  # uniform x: both half-widths
+    x = xf[i]  # the U face between centres i-1 and i
+    mean = lambda c: Q(c, x) + c[2] * h * h
+    slope = lambda c: c[1] + c[2] * (xc[i] + xc[i - 1])
+    exp = dU0.copy()
+    for k in range(1, n3m + 1):
+        for j in range(1, n2m + 1):
+            # dV on the wall faces (j = 1 below cell 1, n2m + 1 above cell n2m) takes no part
+            on3 = 0.0 if j == 1 else 1.0
+            on4 = 0.0 if j + 1 == n2m + 1 else 1.0
+            dvm3, dvm4 = mean(cv) * Hv[k, j, 0] * on3, mean(cv) * Hv[k, j + 1, 0] * on4
+            dvdx3, dvdx4 = slope(cv) * Hv[k, j, 0] * on3, slope(cv) * Hv[k, j + 1, 0] * on4
+            dudy3, dudy4 = u1 * (yc[j] + yc[j - 1]), u1 * (yc[j + 1] + yc[j])
+            mu3 = m0 + m1 * x + m2 * yf[j] + m3 * zc[k]
+            mu4 = m0 + m1 * x + m2 * yf[j + 1] + m3 * zc[k]
+            inv_rho = r0 + r1 * x + r2 * yc[j]
+            ycor = (0.25 * (dvm4 * dudy4 + dvm3 * dudy3)
+                    - 0.5 * Cmu * inv_rho * (mu4 * dvdx4 - mu3 * dvdx3) / dx2[j])
+            dwm5, dwm6 = mean(cw) * Hw[k, j, 0], mean(cw) * Hw[k + 1, j, 0]
+            dwdx5, dwdx6 = slope(cw) * Hw[k, j, 0], slope(cw) * Hw[k + 1, j, 0]
+            dudz5, dudz6 = u2 * (zc[k] + zc[k - 1]), u2 * (zc[k + 1] + zc[k])
+            mu5 = m0 + m1 * x + m2 * yc[j] + m3 * 0.5 * (zc[k - 1] + zc[k])
+            mu6 = m0 + m1 * x + m2 * yc[j] + m3 * 0.5 * (zc[k] + zc[k + 1])
+            zcor = (0.25 * (dwm6 * dudz6 + dwm5 * dudz5)
+                    - 0.5 * Cmu * inv_rho * (mu6 * dwdx6 - mu5 * dwdx5) / dx3[k])
+            exp[k, j, 1:n1m + 1] -= dt * ycor + dt * zcor
+    corr = np.abs(exp - dU0).max()
+    assert corr > 1e-4
+    assert np.abs(o.f["dU"] - exp).max() <= 1e-12 * corr
