@@ -44,6 +44,12 @@ constexpr int FFT_MAXB = 8;  // butterflies per thread per stage
 // the same operands as the stage-by-stage loop, so the same results bit for
 // bit.  The group's outputs land at 8(g - kk) + kk + r*Ns, kk = g mod Ns,
 // with r the three result bits (first stage's bit lowest).
+// LDS position of transform value x: one pad value per 32, so the strided
+// stores of the first passes (8 and 64 values apart) spread over the banks
+__device__ __forceinline__ int fpad(int x) { return x + (x >> 5); }
+// LDS values that hold `count` padded transform values
+__host__ __device__ constexpr int fft_lds_len(int count) { return count + (count >> 5); }
+
 template <int SIGN, int R, int NT>
 __device__ __forceinline__ void fft_pass(double *re, double *im, int n, int nl, int s,
                                          const double2 *__restrict__ tw) {
@@ -65,8 +71,8 @@ __device__ __forceinline__ void fft_pass(double *re, double *im, int n, int nl, 
             const int kk = g & (Ns - 1);
 #pragma unroll
             for (int t = 0; t < M; t++) {
-                xr[u][t] = re[l * n + t * G + g];
-                xi[u][t] = im[l * n + t * G + g];
+                xr[u][t] = re[fpad(l * n + t * G + g)];
+                xi[u][t] = im[fpad(l * n + t * G + g)];
             }
 #pragma unroll
             for (int i = 0; i < R; i++) {
@@ -106,8 +112,8 @@ __device__ __forceinline__ void fft_pass(double *re, double *im, int n, int nl, 
                 int r = 0;  // register index t holds result bits (r0..r_{R-1}) with r0 highest
 #pragma unroll
                 for (int m = 0; m < R; m++) r |= ((t >> (R - 1 - m)) & 1) << m;
-                re[obase[u] + r * Ns] = xr[u][t];
-                im[obase[u] + r * Ns] = xi[u][t];
+                re[fpad(obase[u] + r * Ns)] = xr[u][t];
+                im[fpad(obase[u] + r * Ns)] = xi[u][t];
             }
         }
     }
@@ -213,7 +219,7 @@ __global__ void __launch_bounds__(NT)
 k_fft_x_fwd(Grid g, int h1p, int TX, Src src, const double2 *__restrict__ tw, double2 *__restrict__ spec) {
     extern __shared__ double lds[];
     const int n = g.n1m;
-    double *re = lds, *im = lds + (size_t)TX * n;
+    double *re = lds, *im = lds + fft_lds_len(TX * n);
     int kb, jb;
     xcd_tiles(kb, jb);  // neighbouring rows j on one XCD: the right-hand side's stencil
     const int j = jb + 1;
@@ -227,15 +233,15 @@ k_fft_x_fwd(Grid g, int h1p, int TX, Src src, const double2 *__restrict__ tw, do
 #pragma unroll
             for (int l = 0; l < 4; l++) {
                 const int t = tid + h * NT;
-                re[l * n + t] = src(t + 1, j, k0 + l + 1);
-                im[l * n + t] = 0.0;
+                re[fpad(l * n + t)] = src(t + 1, j, k0 + l + 1);
+                im[fpad(l * n + t)] = 0.0;
             }
         }
     } else {
         for (int e = tid; e < TX * n; e += NT) {
             const int l = e / n, t = e - l * n;
-            re[e] = src(t + 1, j, k0 + l + 1);
-            im[e] = 0.0;
+            re[fpad(e)] = src(t + 1, j, k0 + l + 1);
+            im[fpad(e)] = 0.0;
         }
     }
     __syncthreads();
@@ -243,7 +249,7 @@ k_fft_x_fwd(Grid g, int h1p, int TX, Src src, const double2 *__restrict__ tw, do
     const size_t rowbase = (size_t)g.n3m * h1p * (j - 1);
     for (int e = tid; e < TX * h1p; e += NT) {
         const int kx = e / TX, l = e - kx * TX;
-        spec[rowbase + (size_t)g.n3m * kx + k0 + l] = make_double2(re[l * n + kx], im[l * n + kx]);
+        spec[rowbase + (size_t)g.n3m * kx + k0 + l] = make_double2(re[fpad(l * n + kx)], im[fpad(l * n + kx)]);
     }
 }
 
@@ -252,16 +258,16 @@ template <int SIGN, int NT>
 __global__ void __launch_bounds__(NT)
 k_fft_z(int n3m, int h1p, const double2 *__restrict__ tw, double2 *__restrict__ spec) {
     extern __shared__ double lds[];
-    double *re = lds, *im = lds + n3m;
+    double *re = lds, *im = lds + fft_lds_len(n3m);
     const size_t base = (size_t)n3m * ((size_t)blockIdx.x + (size_t)h1p * blockIdx.y);
     const int tid = threadIdx.x;
     for (int t = tid; t < n3m; t += NT) {
         const double2 v = spec[base + t];
-        re[t] = v.x; im[t] = v.y;
+        re[fpad(t)] = v.x; im[fpad(t)] = v.y;
     }
     __syncthreads();
     lds_fft<SIGN, NT>(re, im, n3m, 1, tw);
-    for (int t = tid; t < n3m; t += NT) spec[base + t] = make_double2(re[t], im[t]);
+    for (int t = tid; t < n3m; t += NT) spec[base + t] = make_double2(re[fpad(t)], im[fpad(t)]);
 }
 
 // y tridiagonal solves per (kx,kz), PaScaL single-rank order (:599-652).
@@ -419,7 +425,7 @@ k_fft_x_inv(Grid g, int h1p, int TX, const double2 *__restrict__ spec, const dou
             double *__restrict__ dP, double *__restrict__ linesum) {
     extern __shared__ double lds[];
     const int n = g.n1m;
-    double *re = lds, *im = lds + (size_t)TX * n;
+    double *re = lds, *im = lds + fft_lds_len(TX * n);
     const int j = blockIdx.y + 1;
     const int k0 = blockIdx.x * TX;
     const int tid = threadIdx.x;
@@ -428,11 +434,11 @@ k_fft_x_inv(Grid g, int h1p, int TX, const double2 *__restrict__ spec, const dou
     for (int e = tid; e < TX * h1p; e += NT) {
         const int kx = e / TX, l = e - kx * TX;
         const double2 v = spec[rowbase + (size_t)g.n3m * kx + k0 + l];
-        re[l * n + kx] = v.x;
-        im[l * n + kx] = (kx == 0 || kx == nh) ? 0.0 : v.y;
+        re[fpad(l * n + kx)] = v.x;
+        im[fpad(l * n + kx)] = (kx == 0 || kx == nh) ? 0.0 : v.y;
         if (kx > 0 && kx < nh) {
-            re[l * n + n - kx] = v.x;
-            im[l * n + n - kx] = -v.y;
+            re[fpad(l * n + n - kx)] = v.x;
+            im[fpad(l * n + n - kx)] = -v.y;
         }
     }
     __syncthreads();
@@ -440,9 +446,9 @@ k_fft_x_inv(Grid g, int h1p, int TX, const double2 *__restrict__ spec, const dou
     const double fn1 = (double)g.n1m, fn3 = (double)g.n3m;
     for (int e = tid; e < TX * n; e += NT) {
         const int l = e / n, t = e - l * n;
-        const double v = re[e] / fn1 / fn3;
+        const double v = re[fpad(e)] / fn1 / fn3;
         dP[g.ix(t + 1, j, k0 + l + 1)] = v;
-        im[e] = v;  // reuse as the reduction buffer
+        im[e] = v;  // reuse as the reduction buffer (unpadded)
     }
     __syncthreads();
     for (int h = n / 2; h >= 1; h >>= 1) {
@@ -645,7 +651,7 @@ constexpr int FFT_X_THREADS = 256;
 static int fft_tile(int n1m, int n3m) {
     int tx = 1;
     while (tx < 4 && tx * 2 <= n3m && (tx * 2) * (n1m / 2) <= FFT_MAXB * FFT_X_THREADS &&
-           (size_t)(tx * 2) * n1m * 16 <= 65536)
+           (size_t)fft_lds_len(tx * 2 * n1m) * 16 <= 65536)
         tx *= 2;
     return tx;
 }
@@ -654,7 +660,7 @@ static int fft_tile(int n1m, int n3m) {
 template <class Src>
 static int fft_x_fwd_src_launch(tcs_ctx *c, const char *label, const Src &src, double2 *spec) {
     const int TX = fft_tile(c->n1m, c->n3m);
-    const size_t lds = (size_t)TX * c->n1m * 2 * sizeof(double);
+    const size_t lds = (size_t)fft_lds_len(TX * c->n1m) * 2 * sizeof(double);
     TCS_LAUNCH(c->stream, label, (k_fft_x_fwd<FFT_X_THREADS, Src>), dim3(c->n3m / TX, c->n2m), dim3(FFT_X_THREADS),
                lds, c->grid(), c->h1p, TX, src, (const double2 *)c->tw1, spec);
     return TCS_OK;
@@ -666,7 +672,7 @@ static int fft_x_fwd_launch(tcs_ctx *c, const char *label, const double *in, dou
 // c2r along x into the full-layout field `out`, with the per-line sums
 static int fft_x_inv_launch(tcs_ctx *c, const char *label, const double2 *spec, double *out) {
     const int TX = fft_tile(c->n1m, c->n3m);
-    const size_t lds = (size_t)TX * c->n1m * 2 * sizeof(double);
+    const size_t lds = (size_t)fft_lds_len(TX * c->n1m) * 2 * sizeof(double);
     TCS_LAUNCH(c->stream, label, k_fft_x_inv<FFT_X_THREADS>, dim3(c->n3m / TX, c->n2m), dim3(FFT_X_THREADS), lds,
                c->grid(), c->h1p, TX, spec, (const double2 *)c->tw1, out, c->linesum);
     return TCS_OK;
@@ -685,7 +691,7 @@ static int fft_threads(int butterflies) {
 template <int SIGN>
 static int fft_z_launch(tcs_ctx *c, const char *label, double2 *spec) {
     const int NT = fft_threads(c->n3m / 2);
-    const size_t lds = (size_t)c->n3m * 2 * sizeof(double);
+    const size_t lds = (size_t)fft_lds_len(c->n3m) * 2 * sizeof(double);
     const dim3 grd(c->h1p, c->n2m);
     const double2 *tw = (const double2 *)c->tw3;
     if (NT == 64) TCS_LAUNCH(c->stream, label, (k_fft_z<SIGN, 64>), grd, dim3(64), lds, c->n3m, c->h1p, tw, spec);
