@@ -1146,7 +1146,25 @@ void orc_momentum_solvedV(orc *o, double dt)
                 amj[q] = mAMJ * dt;
             }
         }
-    lines_solve(o, amj, acj, apj, RHSJ, n3msub * n1msub, ny, 0);
+    if (o->tdma_split && ny + 1 <= 512) {
+        /* Split lines (not in the reference): the wall row j = 1, where dV is
+           zero, leads each y line as an identity row, so that the 2^m - 1
+           rows j = 2..n2m split into whole parts of 2^m rows.  Row 2's lower
+           coefficient then multiplies the identity row's zero. */
+        const size_t ns = (size_t)n3msub * n1msub, nb = sizeof(double) * ns;
+        double *a2 = malloc(nb * (ny + 1)), *b2 = malloc(nb * (ny + 1));
+        double *c2 = malloc(nb * (ny + 1)), *d2 = malloc(nb * (ny + 1));
+        for (size_t s = 0; s < ns; s++) { a2[s] = 0.0; b2[s] = 1.0; c2[s] = 0.0; d2[s] = 0.0; }
+        memcpy(a2 + ns, amj, nb * ny);
+        memcpy(b2 + ns, acj, nb * ny);
+        memcpy(c2 + ns, apj, nb * ny);
+        memcpy(d2 + ns, RHSJ, nb * ny);
+        lines_solve(o, a2, b2, c2, d2, ns, ny + 1, 0);
+        memcpy(RHSJ, d2 + ns, nb * ny);
+        free(a2); free(b2); free(c2); free(d2);
+    } else {
+        lines_solve(o, amj, acj, apj, RHSJ, n3msub * n1msub, ny, 0);
+    }
 
     for (k = 0; k <= o->n3sub; k++)
         for (i = 0; i <= o->n1sub; i++) {

@@ -857,11 +857,17 @@ static int launch_stale(tcs_ctx *c) {
 // y stage of a momentum increment: rows j = jlo..n2m stored into the field
 // F; SX lines per workgroup of the split solve (measured at 512^3: the
 // light dU and dV y rows run faster in two workgroups of 8 lines per CU,
-// the dW rows in one of 16; FP32 lines keep 16, 64-byte segments)
+// the dW rows in one of 16; FP32 lines keep 16, 64-byte segments).  With
+// jlo = 2 (dV) the split lines start at the wall row as an identity row
+// when all n2m rows fit a split line (dV y at 512^3: 1.56 -> 1.0 ms, whole
+// 16-row parts and the unconditional row loop); dV's wall row is zero
+// either way.
 template <int SX, class Row>
 static int y_stage(tcs_ctx *c, const char *label_rows, const char *label_solve, const Row &row, int jlo, double *F) {
     scr_t *am = SCR(c->am), *ac = SCR(c->ac), *ap = SCR(c->ap), *rhs = SCR(c->rhs);
     const int nrows = c->n2m - jlo + 1;
+    if (jlo == 2 && rows_in_solve<scr_t>(c, c->n2m))
+        return y_stage_in_solve<SX>(c, label_solve, row, 1, c->n2m, rhs, F, 1, jlo);
     if (rows_in_solve<scr_t>(c, nrows)) return y_stage_in_solve<SX>(c, label_solve, row, jlo, nrows, rhs, F, 1);
     TCS_TRY(rows3d_launch<false>(c->stream, label_rows, row, jlo, am, ac, ap, rhs));
     return sweep_y_field(c, jlo, nrows, F, 1);

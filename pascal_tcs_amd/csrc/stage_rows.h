@@ -113,6 +113,31 @@ struct YLineRows {
     }
 };
 
+// y lines whose rows j < jrow are identity rows (a = c = d = 0, b = 1), the
+// rest as in YLineRows.  The split dV lines start at the wall row j = 1,
+// where dV is zero, so that the 2^m - 1 rows j = 2..n2m become 2^m rows in
+// whole parts.  Every row is made and the identity selected over it (the
+// wall row's stencil reads ghost cells and may divide by the zero ghost
+// width; the selection drops that), so the unconditional row loop stays
+// branch-free.
+template <class Row, typename S>
+struct YLineRowsLead {
+    static constexpr bool full_rows = true;
+    Row row;
+    const S *rhs;
+    int jlo, jrow;
+    __device__ __forceinline__ void operator()(int s0, int s1, int r, S &a, S &b, S &c, S &d) const {
+        const int i = s0 + 1, j = r + jlo, k = s1 + 1;
+        row_as<S>(row, i, j, k, a, b, c, d);
+        d = rhs[row.g.r3(i, j, k)];
+        const bool wall = j < jrow;
+        a = wall ? S(0) : a;
+        b = wall ? S(1) : b;
+        c = wall ? S(0) : c;
+        d = wall ? S(0) : d;
+    }
+};
+
 // x lines as a split-solve source: line (s0, s1) = (j-jlo, k-1), row i-1;
 // the right-hand side is the previous stage's solution in the scratch.
 template <class Row, typename S>
@@ -168,10 +193,11 @@ int x_stage(tcs_ctx *c, const char *label_rows, const char *label_solve, const R
 // y stage with its rows computed in the split solve (rows_in_solve holds):
 // the y lines of rows j = jlo..jlo+nrows-1, right-hand side from the scratch
 // rhs.  The solution is stored into (mode 1) or added to (mode 2) the
-// interior of the field F, or, with F null, written back into rhs.
+// interior of the field F, or, with F null, written back into rhs.  Rows
+// j < jrow are identity rows (YLineRowsLead) when jrow > jlo.
 template <int SX = PT_SX, class Row, typename S>
 int y_stage_in_solve(tcs_ctx *c, const char *label, const Row &row, int jlo, int nrows, S *rhs, double *F,
-                     int mode) {
+                     int mode, int jrow = 0) {
     const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m, off = (long)(jlo - 1) * n1;
     LineOut out;
     if (F) {
@@ -182,6 +208,9 @@ int y_stage_in_solve(tcs_ctx *c, const char *label, const Row &row, int jlo, int
         out.ss1 = (long)c->P1 * c->N2;
         out.rs = c->P1;
     }
+    if (jrow > jlo)
+        return ptdma_part_src_launch<false, SX>(c->stream, label, YLineRowsLead<Row, S>{row, rhs, jlo, jrow}, rhs + off,
+                                                c->n1m, c->n3m, n12, nrows, n1, out);
     return ptdma_part_src_launch<false, SX>(c->stream, label, YLineRows<Row, S>{row, rhs, jlo}, rhs + off, c->n1m,
                                         c->n3m, n12, nrows, n1, out);
 }

@@ -139,18 +139,21 @@ def test_full_step_seeded(shape, split=0):
         _compare(o, s, ["T", "U", "V", "W", "P", "dPhat", "dP0"])
 
 
-SPLIT_SHAPES = [(64, 40, 32), (32, 100, 16), (16, 12, 8)]
+# (16, 64, 8): the y lines fill whole 16-row parts (the unconditional row
+# loops; dV's lines with the wall row leading as an identity row)
+SPLIT_SHAPES = [(64, 40, 32), (32, 100, 16), (16, 12, 8), (16, 64, 8)]
 
 
 @pytest.mark.parametrize("shape", SPLIT_SHAPES)
 def test_full_step_seeded_split_lines(shape):
     """The step with every ADI line split into parts (x lines of 64 -> 4
-    parts, y lines of 100/99 -> 8 parts, ...), reduced systems by parallel
-    cyclic reduction, bit for bit against the oracle run with the same split."""
+    parts, y lines of 100 -> 8 parts, dV's with its wall row leading, ...),
+    reduced systems by parallel cyclic reduction, bit for bit against the
+    oracle run with the same split."""
     test_full_step_seeded(shape, split=1)
 
 
-@pytest.mark.parametrize("shape", SPLIT_SHAPES[:2])
+@pytest.mark.parametrize("shape", SPLIT_SHAPES[:2] + SPLIT_SHAPES[3:])
 @pytest.mark.parametrize("routine", ["thermal", "solvedU", "solvedV", "solvedW"])
 def test_routines_split_lines(shape, routine):
     cfg, o, s = _pair(shape, split=1)
