@@ -231,10 +231,18 @@ __global__ void __launch_bounds__(256)
 k_post_both(Grid g, const double *__restrict__ U, const double *__restrict__ V, const double *__restrict__ W,
             double *__restrict__ partial) {
     __shared__ double s0[256], s1[256];
-    const int nrows = g.n2m * g.n3m;
+    // Workgroups go to the 8 XCDs round-robin.  When n2m divides by 8, XCD x
+    // serves the rows j of band x (n2m/8 of them) at every k, so the V(j+1)
+    // row a row reads is read by a workgroup of the same XCD at about the
+    // same time (one fetch into its L2, not one per XCD); otherwise the rows
+    // go round-robin as they come.
+    const bool banded = g.n2m % 8 == 0 && gridDim.x % 8 == 0;
+    const int band = banded ? g.n2m / 8 : g.n2m, x = banded ? (int)blockIdx.x % 8 : 0;
+    const int q = banded ? (int)blockIdx.x / 8 : (int)blockIdx.x, nq = banded ? (int)gridDim.x / 8 : (int)gridDim.x;
+    const int nrows = band * g.n3m;
     double m0 = -INFINITY, m1 = -INFINITY;
-    for (int r = blockIdx.x; r < nrows; r += gridDim.x) {
-        const int j = r % g.n2m + 1, k = r / g.n2m + 1;
+    for (int r = q; r < nrows; r += nq) {
+        const int j = x * band + r % band + 1, k = r / band + 1;
         for (int i = threadIdx.x + 1; i <= g.n1m; i += blockDim.x) {
             const double u0 = U[g.ix(i, j, k)], u1 = U[g.ix(i + 1, j, k)];
             const double v0 = V[g.ix(i, j, k)], v1 = V[g.ix(i, j + 1, k)];
