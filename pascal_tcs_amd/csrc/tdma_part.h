@@ -319,13 +319,14 @@ __device__ __forceinline__ void xcd_tiles(int &bx, int &by) {
 // row*rs, or a field) are those of the split solve of stored lines.
 template <typename T, bool CYC, class Src, int SX, bool FULL>
 __global__ void __launch_bounds__(SX * PT_PMAX)
-k_ptdma_part_src(Src src, T *D, int n0, long ss1, long rs, int P, int w1, int w2, LineOut out) {
+k_ptdma_part_src(Src src, T *D, int n0, int n1, long ss1, long rs, int P, int w1, int w2, LineOut out) {
     using namespace part;
     __shared__ RedLDS<T, SX, 1> red;
     const int t = threadIdx.x;
     const int line = t % SX, p = t / SX;
     int bx, by;
     xcd_tiles(bx, by);
+    if (by >= n1) return;  // a padding row of workgroups (whole workgroup)
     const int s0 = bx * SX + line, s1 = by;
     const bool act = s0 < n0;
     int j0, m;
@@ -506,17 +507,24 @@ int ptdma_part_src_launch(hipStream_t st, const char *label, const Src &src, T *
         set_error("ptdma_part: n_row must be in [2, 512] and the outer line count <= 65535");
         return TCS_E_ARG;
     }
-    const dim3 blk(SX * P), grd((unsigned)((n0 + SX - 1) / SX), (unsigned)n1);
+    // the outer line count padded to whole XCD tiles (the padding
+    // workgroups return at once): a dV stage's 511 lines then take the tiled
+    // order too
+    const unsigned gx = (unsigned)((n0 + SX - 1) / SX);
+    unsigned gy = (unsigned)n1;
+    const unsigned gyp = (gy + XT_Y - 1) / XT_Y * XT_Y;
+    if (gx % XT_X == 0 && (gx * gyp) % (8 * XT_X * XT_Y) == 0 && gyp <= 65535) gy = gyp;
+    const dim3 blk(SX * P), grd(gx, gy);
     // FP64 lines whose parts all hold PT_MMAX rows take the unconditional
     // row loop where the source asks for it (Src::full_rows: every y stage,
     // thermal y 2.63 -> 2.00 ms, dU y 1.52 -> 1.02 ms, dW y 1.43 -> 1.29 ms
     // at 512^3; the dU z stage, 4.14 -> 3.14 ms); the FP32 lines of the
     // mixed form keep the guarded loop (1.5 ms slower per step with it)
     if (Src::full_rows && sizeof(T) == 8 && w1 == PT_MMAX && w2 == 0)
-        TCS_LAUNCH(st, label, (k_ptdma_part_src<T, CYC, Src, SX, true>), grd, blk, 0, src, D, n0, ss1, rs, P, w1, w2,
+        TCS_LAUNCH(st, label, (k_ptdma_part_src<T, CYC, Src, SX, true>), grd, blk, 0, src, D, n0, n1, ss1, rs, P, w1, w2,
                    out);
     else
-        TCS_LAUNCH(st, label, (k_ptdma_part_src<T, CYC, Src, SX, false>), grd, blk, 0, src, D, n0, ss1, rs, P, w1, w2,
+        TCS_LAUNCH(st, label, (k_ptdma_part_src<T, CYC, Src, SX, false>), grd, blk, 0, src, D, n0, n1, ss1, rs, P, w1, w2,
                    out);
     return TCS_OK;
 }
