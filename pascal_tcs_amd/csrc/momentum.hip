@@ -790,7 +790,7 @@ constexpr int LDV_KB = 1, LDU_KB = 4, LDV_IB = 1, LDU_IB = 2;
 __global__ void __launch_bounds__(256) k_blockLdV_pseudo(BlockLdV cell, double *dV, double *__restrict__ Vn) {
     const Grid &g = cell.g;
     const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * LDV_IB + 1;  // n1m is a multiple of LDV_IB
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int j = wave_row_j();
     if (i0 > g.n1m || j > g.n2m) return;
     // loads of every cell before the stores (dV is updated in place, each
     // cell reading only its own point of it)
@@ -935,7 +935,7 @@ __global__ void __launch_bounds__(256) k_blockLdU_pseudo(BlockLdU cell, const do
                                                         double *__restrict__ Un, double *__restrict__ Wn) {
     const Grid &g = cell.g;
     const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * LDU_IB + 1;  // n1m is a multiple of LDU_IB
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int j = wave_row_j();
     if (i0 > g.n1m || j > g.n2m) return;
     // loads of every cell before the stores (dU is updated in place, each
     // cell reading only its own point of it)

@@ -604,7 +604,7 @@ __global__ void __launch_bounds__(256) k_projection_into(Projection cell, DPhatU
                                                         double *__restrict__ dP0n) {
     const Grid &g = cell.g;
     const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * PROJ_IB + 1;  // n1m is a multiple of PROJ_IB
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int j = wave_row_j();
     if (i0 > g.n1m || j > g.n2m) return;
     // every cell's loads before any store: U, V, W, P are updated in place
     // (each cell reads only its own point of them), and the loads the cells

@@ -61,7 +61,7 @@ template <class Row, bool RHS, typename S>
 __global__ void __launch_bounds__(256)
 k_rows3d(Row row, int jlo, S *__restrict__ am, S *__restrict__ ac, S *__restrict__ ap, S *__restrict__ rhs) {
     const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * ROWS3D_IB + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int j = wave_row_j();
     if (i0 > row.g.n1m || j > row.g.n2m || j < jlo) return;
 #pragma unroll
     for (int q = 0; q < ROWS3D_KB * ROWS3D_IB; q++) {

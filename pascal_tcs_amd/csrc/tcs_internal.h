@@ -160,6 +160,13 @@ namespace tcs {
 
 // launch helpers -------------------------------------------------------------
 inline dim3 interior_block() { return dim3(64, 4, 1); }
+// Row j = blockIdx.y * 4 + threadIdx.y + 1 of a kernel launched with
+// interior_block(): the 64 threads of a wavefront share threadIdx.y, so j is
+// the same on every lane, and readfirstlane lets the compiler see it (the
+// (j, k) tables and j's metrics are then loaded once per wavefront).
+__device__ __forceinline__ int wave_row_j() {
+    return __builtin_amdgcn_readfirstlane((int)(blockIdx.y * blockDim.y + threadIdx.y + 1));
+}
 inline dim3 interior_grid(int nx, int ny, int nz) {
     return dim3((unsigned)((nx + 63) / 64), (unsigned)((ny + 3) / 4), (unsigned)nz);
 }
