@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = None
+_LIBS = {}
 _REF = None
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -35,10 +35,12 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-def lib():
-    global _LIB
-    if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+def lib(plants=False):
+    """The oracle library; plants=True: the build whose planted-mistake
+    sites orc_set_plant switches on (tests of the closed-form pins)."""
+    fname = "liboracle_plants.so" if plants else "liboracle.so"
+    if fname not in _LIBS:
+        path = os.path.join(_HERE, fname)
         if not os.path.exists(path):
             build()
         L = ctypes.CDLL(path)
@@ -81,8 +83,18 @@ def lib():
         L.orc_wave_numbers.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_tree_sum.restype = ctypes.c_double
         L.orc_tree_sum.argtypes = [_dp, ctypes.c_int]
-        _LIB = L
-    return _LIB
+        L.orc_capture.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_capture_count.restype = ctypes.c_int
+        L.orc_capture_count.argtypes = [ctypes.c_void_p]
+        _ip = ctypes.POINTER(ctypes.c_int)
+        L.orc_capture_info.restype = ctypes.c_int
+        L.orc_capture_info.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _ip, _ip]
+        L.orc_capture_copy.restype = ctypes.c_int
+        L.orc_capture_copy.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_dp] * 5
+        L.orc_set_plant.argtypes = [ctypes.c_int]
+        L.orc_plants_built.restype = ctypes.c_int
+        _LIBS[fname] = L
+    return _LIBS[fname]
 
 
 def ref_lib():
@@ -170,8 +182,9 @@ class Oracle:
     i.e. the Fortran (0:n1sub,0:n2sub,0:n3sub) array with i fastest.
     """
 
-    def __init__(self, n1m, n2m, n3m, dx1, dx2, dx3, dmx1, dmx2, dmx3, prm):
-        L = lib()
+    def __init__(self, n1m, n2m, n3m, dx1, dx2, dx3, dmx1, dmx2, dmx3, prm, plants=False):
+        L = lib(plants)
+        self._L = L
         self.n1m, self.n2m, self.n3m = n1m, n2m, n3m
         self._keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (dx1, dx2, dx3, dmx1, dmx2, dmx3)]
         p = np.array([float(prm[k]) for k in PRM_NAMES], dtype=np.float64)
@@ -190,42 +203,67 @@ class Oracle:
 
     def set_tdma_split(self, split):
         """1: the step's ADI lines are solved split into tdma_parts(n) ranks."""
-        lib().orc_set_tdma_split(self.h, int(split))
+        self._L.orc_set_tdma_split(self.h, int(split))
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().orc_destroy(self.h)
+            self._L.orc_destroy(self.h)
             self.h = None
 
     def __getattr__(self, name):
         # routine dispatch: o.thermal_solver(dt) -> orc_thermal_solver(h, dt)
-        fn = getattr(lib(), "orc_" + name, None)
+        if name.startswith("_"):
+            raise AttributeError(name)
+        fn = getattr(self._L, "orc_" + name, None)
         if fn is None:
             raise AttributeError(name)
         return lambda *a: fn(self.h, *a)
 
     def halo(self, name):
-        lib().orc_ghostcell_update(self.h, self.f[name].ctypes.data_as(_dp))
+        self._L.orc_ghostcell_update(self.h, self.f[name].ctypes.data_as(_dp))
 
     def pressure_rhs(self, dt):
         out = np.zeros((self.n3m, self.n2m, self.n1m))
-        lib().orc_pressure_RHS(self.h, dt, _p(out))
+        self._L.orc_pressure_RHS(self.h, dt, _p(out))
         return out
 
     def poisson(self, prhs):
         prhs = np.ascontiguousarray(prhs, dtype=np.float64)
-        lib().orc_pressure_Poisson_FFT2(self.h, _p(prhs))
+        self._L.orc_pressure_Poisson_FFT2(self.h, _p(prhs))
 
     def wave_numbers(self):
         dzk = np.zeros(self.n3m)
         dxk2 = np.zeros(self.n1m // 2 + 1)
-        lib().orc_wave_numbers(self.h, _p(dzk), _p(dxk2))
+        self._L.orc_wave_numbers(self.h, _p(dzk), _p(dxk2))
         return dzk, dxk2
 
     def div(self):
-        return lib().orc_post_div(self.h)
+        return self._L.orc_post_div(self.h)
 
     def cfl(self, dt):
         nd = ctypes.c_double(0.0)
-        cfl = lib().orc_post_cfl(self.h, dt, ctypes.byref(nd))
+        cfl = self._L.orc_post_cfl(self.h, dt, ctypes.byref(nd))
         return cfl, nd.value
+
+    # ---- test hooks ----------------------------------------------------
+    def capture(self, on=True):
+        """Record every line solve of the routines called next (rows a, b,
+        c, right-hand side d, solution x); capture(False) stops and clears."""
+        self._L.orc_capture(self.h, int(bool(on)))
+
+    def captured(self):
+        """The records as dicts of (n_row, n_sys) arrays, in solve order."""
+        out = []
+        for q in range(self._L.orc_capture_count(self.h)):
+            ns, nr, cy = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+            self._L.orc_capture_info(self.h, q, ctypes.byref(ns), ctypes.byref(nr), ctypes.byref(cy))
+            arrs = [np.zeros((nr.value, ns.value)) for _ in range(5)]
+            self._L.orc_capture_copy(self.h, q, *[_p(a) for a in arrs])
+            out.append(dict(zip("abcdx", arrs), cyclic=bool(cy.value)))
+        return out
+
+    def plant(self, site):
+        """Switch on planted mistake `site` (0: none) in the plants build."""
+        if not self._L.orc_plants_built():
+            raise RuntimeError("planted mistakes need the oracle built with plants=True")
+        self._L.orc_set_plant(int(site))

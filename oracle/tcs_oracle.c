@@ -40,7 +40,27 @@ struct orc {
     double *bc[OB_N];
     double *dzk, *dxk2;
     int tdma_split;          /* 0: single-rank line solves; 1: lines split into parts */
+    /* test hook: records of the line solves a routine makes */
+    int cap_on, ncap;
+    struct orc_cap { int n_sys, n_row, cyclic; double *a, *b, *c, *d, *x; } cap[ORC_MAXCAP];
 };
+
+/* Planted mistakes (test hook): the oracle built with -DORC_PLANTS takes the
+   expression `wrong` at site `id` while orc_set_plant(id) is in force, so a
+   test can show that its closed form rejects that mistake.  The ordinary
+   build compiles `right` only. */
+#ifdef ORC_PLANTS
+static int orc_plant_id = 0;
+#define PLANT(id, wrong, right) (orc_plant_id == (id) ? (wrong) : (right))
+void orc_set_plant(int id) { orc_plant_id = id; }
+int orc_plants_built(void) { return 1; }
+#else
+#define PLANT(id, wrong, right) (right)
+void orc_set_plant(int id) { (void)id; }
+int orc_plants_built(void) { return 0; }
+#endif
+
+static void cap_clear(orc *o);
 
 #define IX(i, j, k) ((size_t)(i) + (size_t)o->N1 * ((size_t)(j) + (size_t)o->N2 * (size_t)(k)))
 #define B2(i, k) ((size_t)(i) + (size_t)o->N1 * (size_t)(k))
@@ -109,6 +129,7 @@ orc *orc_create(int n1m, int n2m, int n3m,
 void orc_destroy(orc *o)
 {
     if (!o) return;
+    cap_clear(o);
     free(o->dx1); free(o->dx2); free(o->dx3); free(o->dmx1); free(o->dmx2); free(o->dmx3);
     free(o->jC_BC); free(o->jS_BC);
     for (int f = 0; f < OF_NFIELD; f++) free(o->f[f]);
@@ -402,10 +423,51 @@ int orc_tdma_parts(int n_row)
 void orc_set_tdma_split(orc *o, int split) { o->tdma_split = split; }
 
 /* the step's line solves: single rank, or split as above */
-static void lines_solve(const orc *o, double *A, double *B, double *C, double *D, int n_sys, int n_row, int cyclic)
+static void lines_solve(orc *o, double *A, double *B, double *C, double *D, int n_sys, int n_row, int cyclic)
 {
+    struct orc_cap *r = NULL;
+    size_t nb = (size_t)n_sys * (size_t)n_row * sizeof(double);
+    if (o->cap_on && o->ncap < ORC_MAXCAP) {
+        r = &o->cap[o->ncap++];
+        r->n_sys = n_sys; r->n_row = n_row; r->cyclic = cyclic;
+        r->a = malloc(nb); r->b = malloc(nb); r->c = malloc(nb); r->d = malloc(nb); r->x = malloc(nb);
+        memcpy(r->a, A, nb); memcpy(r->b, B, nb); memcpy(r->c, C, nb); memcpy(r->d, D, nb);
+    }
     if (o->tdma_split) orc_ptdma_many_solve_split(A, B, C, D, n_sys, n_row, cyclic);
     else ptdma_many_ranks(A, B, C, D, n_sys, n_row, 1, cyclic, 0);
+    if (r) memcpy(r->x, D, nb);
+}
+
+static void cap_clear(orc *o)
+{
+    for (int q = 0; q < o->ncap; q++) {
+        free(o->cap[q].a); free(o->cap[q].b); free(o->cap[q].c); free(o->cap[q].d); free(o->cap[q].x);
+    }
+    o->ncap = 0;
+}
+
+void orc_capture(orc *o, int on)
+{
+    cap_clear(o);
+    o->cap_on = on;
+}
+
+int orc_capture_count(orc *o) { return o->ncap; }
+
+int orc_capture_info(orc *o, int idx, int *n_sys, int *n_row, int *cyclic)
+{
+    if (idx < 0 || idx >= o->ncap) return -1;
+    *n_sys = o->cap[idx].n_sys; *n_row = o->cap[idx].n_row; *cyclic = o->cap[idx].cyclic;
+    return 0;
+}
+
+int orc_capture_copy(orc *o, int idx, double *a, double *b, double *c, double *d, double *x)
+{
+    if (idx < 0 || idx >= o->ncap) return -1;
+    size_t nb = (size_t)o->cap[idx].n_sys * (size_t)o->cap[idx].n_row * sizeof(double);
+    memcpy(a, o->cap[idx].a, nb); memcpy(b, o->cap[idx].b, nb); memcpy(c, o->cap[idx].c, nb);
+    memcpy(d, o->cap[idx].d, nb); memcpy(x, o->cap[idx].x, nb);
+    return 0;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -526,7 +588,8 @@ void orc_thermal_solver(orc *o, double dt)
 
                 e1 = (dx1[i] * T[IX(im, j, k)] + dx1[im] * T[IX(i, j, k)]) * (0.5 / dmx1[i]);
                 e2 = (dx1[ip] * T[IX(i, j, k)] + dx1[i] * T[IX(ip, j, k)]) * (0.5 / dmx1[ip]);
-                e3 = (dx2[j] * T[IX(i, jm, k)] + dx2[jm] * T[IX(i, j, k)]) * (0.5 / dmx2[j]);
+                e3 = PLANT(102, (dx2[jm] * T[IX(i, jm, k)] + dx2[j] * T[IX(i, j, k)]) * (0.5 / dmx2[j]),
+                           (dx2[j] * T[IX(i, jm, k)] + dx2[jm] * T[IX(i, j, k)]) * (0.5 / dmx2[j]));
                 e4 = (dx2[jp] * T[IX(i, j, k)] + dx2[j] * T[IX(i, jp, k)]) * (0.5 / dmx2[jp]);
                 e5 = (dx3[k] * T[IX(i, j, km)] + dx3[km] * T[IX(i, j, k)]) * (0.5 / dmx3[k]);
                 e6 = (dx3[kp] * T[IX(i, j, k)] + dx3[k] * T[IX(i, j, kp)]) * (0.5 / dmx3[kp]);
@@ -547,7 +610,7 @@ void orc_thermal_solver(orc *o, double dt)
                 kPP3 = 0.5 / dmx2[j] * (dx2[j] * KPP[IX(i, jm, k)] + dx2[jm] * KPP[IX(i, j, k)]);
                 kPP4 = 0.5 / dmx2[jp] * (dx2[jp] * KPP[IX(i, j, k)] + dx2[j] * KPP[IX(i, jp, k)]);
                 kPP5 = 0.5 / dmx3[k] * (dx3[k] * KPP[IX(i, j, km)] + dx3[km] * KPP[IX(i, j, k)]);
-                kPP6 = 0.5 / dmx3[kp] * (dx3[kp] * KPP[IX(i, j, k)] + dx3[k] * KPP[IX(i, j, kp)]);
+                kPP6 = 0.5 / dmx3[kp] * (dx3[kp] * KPP[IX(i, j, k)] + dx3[k] * KPP[IX(i, j, PLANT(101, km, kp))]);
 
                 dkPP1 = 0.5 / dmx1[i] * (dx1[i] * dKPP[IX(im, j, k)] + dx1[im] * dKPP[IX(i, j, k)]);
                 dkPP2 = 0.5 / dmx1[ip] * (dx1[ip] * dKPP[IX(i, j, k)] + dx1[i] * dKPP[IX(ip, j, k)]);
@@ -567,7 +630,7 @@ void orc_thermal_solver(orc *o, double dt)
                                                          + (kPP6 * dedz6 - kPP5 * dedz5) / dx3[k]);
 
                 ebc_up = -0.25 * V[IX(i, jp, k)] / dmx2[jp] * TBCup[B2(i, k)];
-                ebc_down = 0.25 * V[IX(i, j, k)] / dmx2[j] * TBCbt[B2(i, k)];
+                ebc_down = PLANT(103, -0.25, 0.25) * V[IX(i, j, k)] / dmx2[j] * TBCbt[B2(i, k)];
                 ebc_up = ebc_up
                        + invRhoCpCt_half / dx2[j] * kPP4 / dmx2[jp] * TBCup[B2(i, k)]
                        + invRhoCpCt_half / dx2[j] * dkPP4 * dedy4 * TBCup[B2(i, k)];
@@ -633,6 +696,7 @@ void orc_thermal_solver(orc *o, double dt)
         for (j = 1; j <= n2msub; j++)
             for (i = 1; i <= n1msub; i++) {
                 ip = i + 1; im = i - 1;
+                if (PLANT(104, 1, 0)) { jm = j - 1; jp = j + 1; km = k - 1; kp = k + 1; }
                 invRhoCpCt_half = 0.5 * Ct * invRhoCp[IX(i, j, k)];
                 kPP1 = 0.5 / dmx1[i] * (dx1[i] * KPP[IX(im, j, k)] + dx1[im] * KPP[IX(i, j, k)]);
                 kPP2 = 0.5 / dmx1[ip] * (dx1[ip] * KPP[IX(i, j, k)] + dx1[i] * KPP[IX(ip, j, k)]);
@@ -652,7 +716,7 @@ void orc_thermal_solver(orc *o, double dt)
                                                    - dx1[ip] * (0.5 / dmx1[ip]) * dkPP2 * dedx2
                                                    + dx1[im] * (0.5 / dmx1[i]) * dkPP1 * dedx1)
                      + (0.25 * U[IX(i, j, k)] / dmx1[i] - 0.25 * U[IX(ip, j, k)] / dmx1[ip])
-                     - 0.5 * invRhoCpCt_half * ((kPP2 * dedx2 - kPP1 * dedx1) / dx1[i]
+                     - 0.5 * PLANT(105, 0.5 * Ct * dinvRhoCp[IX(i, j, k)], invRhoCpCt_half) * ((kPP2 * dedx2 - kPP1 * dedx1) / dx1[i]
                                                 + (kPP4 * dedy4 - kPP3 * dedy3) / dx2[j]
                                                 + (kPP6 * dedz6 - kPP5 * dedz5) / dx3[k]);
                 eAPI = -invRhoCpCt_half / dx1[i] * (kPP2 / dmx1[ip] + dx1[i] * (0.5 / dmx1[ip]) * dkPP2 * dedx2) + 0.25 * U[IX(ip, j, k)] / dmx1[ip];
@@ -674,6 +738,7 @@ void orc_thermal_solver(orc *o, double dt)
     for (k = 1; k <= n3msub; k++)
         for (j = 1; j <= n2msub; j++) {
             jm = j - 1; jp = j + 1;
+            if (PLANT(106, 1, 0)) { km = k - 1; kp = k + 1; }
             jem = jC_BC[jm]; jep = jC_BC[jp];
             for (i = 1; i <= n1msub; i++) {
                 ip = i + 1; im = i - 1;
@@ -823,7 +888,7 @@ void orc_momentum_solvedU(orc *o, double dt)
 
                 mua = 0.5 * (dx1[im] * Mu[IX(i, jm, k)] + dx1[i] * Mu[IX(im, jm, k)]) / dmx1[i];
                 muc = 0.5 * (dx1[im] * Mu[IX(i, j, k)] + dx1[i] * Mu[IX(im, j, k)]) / dmx1[i];
-                mub = 0.5 * (dx1[im] * Mu[IX(i, jp, k)] + dx1[i] * Mu[IX(im, jp, k)]) / dmx1[i];
+                mub = 0.5 * (dx1[im] * Mu[IX(i, PLANT(201, jm, jp), k)] + dx1[i] * Mu[IX(im, PLANT(201, jm, jp), k)]) / dmx1[i];
                 mu3 = 0.5 * (dx2[jm] * muc + dx2[j] * mua) / dmx2[j];
                 mu4 = 0.5 * (dx2[jp] * muc + dx2[j] * mub) / dmx2[jp];
                 mua = 0.5 * (dx1[im] * Mu[IX(i, j, km)] + dx1[i] * Mu[IX(im, j, km)]) / dmx1[i];
@@ -843,12 +908,12 @@ void orc_momentum_solvedU(orc *o, double dt)
 
                 double rhs = invRhocCmu_half * (2.0 * viscous_u1 + viscous_u2 + viscous_u3 + viscous_u12 + viscous_u13);
                 rhs = rhs - Cmp * invRhoc * (P[IX(i, j, k)] - P[IX(im, j, k)]) / dmx1[i];
-                Tc = 0.5 * (dx1[im] * T[IX(i, j, k)] + dx1[i] * T[IX(im, j, k)]) / dmx1[i];
+                Tc = 0.5 * (dx1[im] * T[IX(i, j, k)] + dx1[i] * T[IX(PLANT(202, ip, im), j, k)]) / dmx1[i];
                 rhs = rhs + Cmt * (Tc + a12pera11 * (Tc * Tc) * DeltaT) * invRhoc;
 
                 ubc_down = 0.25 * v3 / dmx2[j] * UBCbt[B2(i, k)] - 0.25 * dudy3 * (dx1[i] * VBCbt[B2(im, k)] + dx1[im] * VBCbt[B2(i, k)]) / dmx1[i] * 0.5;
                 ubc_up = -0.25 * v4 / dmx2[jp] * UBCup[B2(i, k)] - 0.25 * dudy4 * (dx1[i] * VBCup[B2(im, k)] + dx1[im] * VBCup[B2(i, k)]) / dmx1[i] * 0.5;
-                ubc_down = ubc_down + invRhocCmu_half / dx2[j] * mu3 / dmx2[j] * UBCbt[B2(i, k)];
+                ubc_down = ubc_down + PLANT(203, -1.0, 1.0) * invRhocCmu_half / dx2[j] * mu3 / dmx2[j] * UBCbt[B2(i, k)];
                 ubc_down = ubc_down - invRhocCmu_half / dx2[j] * mu3 / dmx1[i] * (VBCbt[B2(i, k)] - VBCbt[B2(im, k)]);
                 ubc_up = ubc_up + invRhocCmu_half / dx2[j] * mu4 / dmx2[jp] * UBCup[B2(i, k)];
                 ubc_up = ubc_up + invRhocCmu_half / dx2[j] * mu4 / dmx1[i] * (VBCup[B2(i, k)] - VBCup[B2(im, k)]);
@@ -907,7 +972,7 @@ void orc_momentum_solvedU(orc *o, double dt)
                 mACI = invRhocCmu_half / dmx1[i] * (Mu[IX(i, jj, k)] / dx1[i] + Mu[IX(i, jj, k)] / dx1[im]) * 2.0
                      + 0.25 * (dudx2 * 0.5 + dudx1 * 0.5 - u2 / dx1[i] + u1 / dx1[im]);
                 mAPI = -invRhocCmu_half / dmx1[i] * Mu[IX(i, jj, k)] / dx1[i] * 2.0 + 0.25 * (u2 / dx1[i] + dudx2 * 0.5);
-                mAMI = -invRhocCmu_half / dmx1[i] * Mu[IX(i, jj, k)] / dx1[im] * 2.0 + 0.25 * (-u1 / dx1[im] + dudx1 * 0.5);
+                mAMI = -invRhocCmu_half / dmx1[i] * Mu[IX(i, jj, k)] / dx1[im] * 2.0 + 0.25 * (PLANT(204, 1.0, -1.0) * u1 / dx1[im] + dudx1 * 0.5);
                 size_t q = R3(n2msub, n3msub, jj, k, i);
                 ac[q] = mACI * dt + 1.0;
                 ap[q] = mAPI * dt;
@@ -933,7 +998,7 @@ void orc_momentum_solvedU(orc *o, double dt)
                 muc = 0.5 * (dx1[im] * Mu[IX(i, j, k)] + dx1[i] * Mu[IX(im, j, k)]) / dmx1[i];
                 mub = 0.5 * (dx1[im] * Mu[IX(i, jp, k)] + dx1[i] * Mu[IX(im, jp, k)]) / dmx1[i];
                 mu3 = 0.5 * (dx2[jm] * muc + dx2[j] * mua) / dmx2[j];
-                mu4 = 0.5 * (dx2[jp] * muc + dx2[j] * mub) / dmx2[jp];
+                mu4 = 0.5 * (dx2[jp] * muc + dx2[j] * PLANT(205, mua, mub)) / dmx2[jp];
                 invRhoc = 0.5 * (dx1[im] * invRho[IX(i, j, k)] + dx1[i] * invRho[IX(im, j, k)]) / dmx1[i];
                 invRhocCmu_half = 0.5 * Cmu * invRhoc;
                 mACJ = invRhocCmu_half / dx2[j] * (mu4 / dmx2[jp] + mu3 / dmx2[j]) + 0.25 * (-v4 / dmx2[jp] + v3 / dmx2[j]);
@@ -995,7 +1060,7 @@ void orc_momentum_solvedV(orc *o, double dt)
         kp = k + 1; km = k - 1;
         for (j = j_indexS; j <= n2msub; j++) {
             jp = j + 1; jm = j - 1;
-            jvp = jS_BC[jm]; jvm = jS_BC[jp];
+            jvp = jS_BC[PLANT(211, jp, jm)]; jvm = jS_BC[PLANT(211, jm, jp)];
             for (i = 1; i <= n1msub; i++) {
                 ip = i + 1; im = i - 1;
                 v3 = 0.5 * (V[IX(i, jm, k)] + V[IX(i, j, k)]);
@@ -1004,7 +1069,7 @@ void orc_momentum_solvedV(orc *o, double dt)
                 u2 = (dx2[jm] * U[IX(ip, j, k)] + dx2[j] * U[IX(ip, jm, k)]) / dmx2[j] * 0.5;
                 w5 = (dx2[jm] * W[IX(i, j, k)] + dx2[j] * W[IX(i, jm, k)]) / dmx2[j] * 0.5;
                 w6 = (dx2[jm] * W[IX(i, j, kp)] + dx2[j] * W[IX(i, jm, kp)]) / dmx2[j] * 0.5;
-                ddu1 = (dx2[jm] * dU[IX(i, j, k)] + dx2[j] * dU[IX(i, jm, k)]) / dmx2[j] * 0.5;
+                ddu1 = (dx2[jm] * dU[IX(i, j, k)] + dx2[j] * dU[IX(i, PLANT(213, jp, jm), k)]) / dmx2[j] * 0.5;
                 ddu2 = (dx2[jm] * dU[IX(ip, j, k)] + dx2[j] * dU[IX(ip, jm, k)]) / dmx2[j] * 0.5;
 
                 dvdx1 = (V[IX(i, j, k)] - V[IX(im, j, k)]) / dmx1[i];
@@ -1099,6 +1164,13 @@ void orc_momentum_solvedV(orc *o, double dt)
                 u2 = (dx2[jm] * U[IX(ip, j, k)] + dx2[j] * U[IX(ip, jm, k)]) / dmx2[j] * 0.5;
                 invRhoc = 0.5 * (dx2[jm] * invRho[IX(i, j, k)] + dx2[j] * invRho[IX(i, jm, k)]) / dmx2[j];
                 invRhocCmu_half = 0.5 * Cmu * invRhoc;
+                if (PLANT(212, 1, 0)) {
+                    mua = 0.5 * (dx2[jm] * Mu[IX(im, j, k)] + dx2[j] * Mu[IX(im, jm, k)]) / dmx2[j];
+                    muc = 0.5 * (dx2[jm] * Mu[IX(i, j, k)] + dx2[j] * Mu[IX(i, jm, k)]) / dmx2[j];
+                    mub = 0.5 * (dx2[jm] * Mu[IX(ip, j, k)] + dx2[j] * Mu[IX(ip, jm, k)]) / dmx2[j];
+                    mu1 = 0.5 * (dx1[im] * muc + dx1[i] * mua) / dmx1[i];
+                    mu2 = 0.5 * (dx1[ip] * muc + dx1[i] * mub) / dmx1[ip];
+                }
                 mACI = invRhocCmu_half / dx1[i] * (mu2 / dmx1[ip] + mu1 / dmx1[i]) + 0.25 * (-u2 / dmx1[ip] + u1 / dmx1[i]);
                 mAPI = -invRhocCmu_half / dx1[i] * mu2 / dmx1[ip] + 0.25 * (u2 / dmx1[ip]);
                 mAMI = -invRhocCmu_half / dx1[i] * mu1 / dmx1[i] + 0.25 * (-u1 / dmx1[i]);
@@ -1125,7 +1197,7 @@ void orc_momentum_solvedV(orc *o, double dt)
     for (k = 1; k <= n3msub; k++)
         for (j = j_indexS; j <= n2msub; j++) {
             jm = j - 1; jp = j + 1;
-            jvp = jS_BC[jm]; jvm = jS_BC[jp];
+            jvp = jS_BC[PLANT(214, jp, jm)]; jvm = jS_BC[PLANT(214, jm, jp)];
             for (i = 1; i <= n1msub; i++) {
                 ip = i + 1; im = i - 1;
                 v3 = 0.5 * (V[IX(i, jm, k)] + V[IX(i, j, k)]);
@@ -1269,7 +1341,7 @@ void orc_momentum_solvedW(orc *o, double dt)
                          - 0.25 * dwdy3 * (dx3[km] * VBCbt[B2(i, k)] + dx3[k] * VBCbt[B2(i, km)]) / dmx3[k] * 0.5;
                 zbc_up = -0.25 * v4 / dmx2[jp] * WBCup[B2(i, k)]
                        - 0.25 * dwdy4 * (dx3[km] * VBCup[B2(i, k)] + dx3[k] * VBCup[B2(i, km)]) / dmx3[k] * 0.5;
-                zbc_down = zbc_down + invRhocCmu_half / dx2[j] * mu3 / dmx2[jp] * WBCbt[B2(i, k)];
+                zbc_down = zbc_down + invRhocCmu_half / dx2[j] * mu3 / dmx2[PLANT(221, j, jp)] * WBCbt[B2(i, k)];
                 zbc_up = zbc_up + invRhocCmu_half / dx2[j] * mu4 / dmx2[j] * WBCup[B2(i, k)];
                 zbc_down = zbc_down - invRhocCmu_half / dx2[j] * mu3 / dmx3[k] * (VBCbt[B2(i, k)] - VBCbt[B2(i, km)]);
                 zbc_up = zbc_up + invRhocCmu_half / dx2[j] * mu4 / dmx3[k] * (VBCup[B2(i, k)] - VBCup[B2(i, km)]);
@@ -1298,7 +1370,7 @@ void orc_momentum_solvedW(orc *o, double dt)
                              - invRhocCmu_half / dx2[j] * (mu4 * dvdz4 * (double)jwp - mu3 * dvdz3 * (double)jwm));
                 rhs = rhs - (0.25 * (ddu2 * dwdx2 + ddu1 * dwdx1) - invRhocCmu_half / dx1[i] * (mu2 * ddudz2 - mu1 * ddudz1));
                 rhs = rhs - (0.25 * (ddv3 * (double)jwm * dwdy3 + ddv4 * (double)jwp * dwdy4)
-                             - invRhocCmu_half / dx2[j] * (mu4 * ddvdz4 * (double)jwp - mu3 * ddvdz3 * (double)jwm));
+                             - invRhocCmu_half / dx2[j] * (mu4 * ddvdz4 * PLANT(224, 1.0, (double)jwp) - mu3 * ddvdz3 * (double)jwm));
 
                 size_t q = R3(n1msub, n2msub, i, j, k);
                 ac[q] = mACK * dt + 1.0;
@@ -1328,6 +1400,13 @@ void orc_momentum_solvedW(orc *o, double dt)
                 u2 = (dx3[km] * U[IX(ip, j, k)] + dx3[k] * U[IX(ip, j, km)]) / dmx3[k] * 0.5;
                 invRhoc = 0.5 * (dx3[km] * invRho[IX(i, j, k)] + dx3[k] * invRho[IX(i, j, km)]) / dmx3[k];
                 invRhocCmu_half = 0.5 * Cmu * invRhoc;
+                if (PLANT(222, 1, 0)) {
+                    mua = 0.5 * (dx3[km] * Mu[IX(im, j, k)] + dx3[k] * Mu[IX(im, j, km)]) / dmx3[k];
+                    muc = 0.5 * (dx3[km] * Mu[IX(i, j, k)] + dx3[k] * Mu[IX(i, j, km)]) / dmx3[k];
+                    mub = 0.5 * (dx3[km] * Mu[IX(ip, j, k)] + dx3[k] * Mu[IX(ip, j, km)]) / dmx3[k];
+                    mu1 = 0.5 * (dx1[im] * muc + dx1[i] * mua) / dmx1[i];
+                    mu2 = 0.5 * (dx1[ip] * muc + dx1[i] * mub) / dmx1[ip];
+                }
                 mACI = invRhocCmu_half / dx1[i] * (mu2 / dmx1[ip] + mu1 / dmx1[i]) + 0.25 * (-u2 / dmx1[ip] + u1 / dmx1[i]);
                 mAPI = -invRhocCmu_half / dx1[i] * mu2 / dmx1[ip] + 0.25 * (u2 / dmx1[ip]);
                 mAMI = -invRhocCmu_half / dx1[i] * mu1 / dmx1[i] + 0.25 * (-u1 / dmx1[i]);
@@ -1349,6 +1428,7 @@ void orc_momentum_solvedW(orc *o, double dt)
     for (k = o->k_indexS; k <= n3msub; k++)
         for (j = 1; j <= n2msub; j++) {
             jm = j - 1; jp = j + 1;
+            if (PLANT(223, 1, 0)) km = k - 1;
             jwp = jC_BC[jp]; jwm = jC_BC[jm];
             for (i = 1; i <= n1msub; i++) {
                 v3 = (dx3[km] * V[IX(i, j, k)] + dx3[k] * V[IX(i, j, km)]) / dmx3[k] * 0.5;
@@ -1491,7 +1571,7 @@ void orc_pressure_RHS(orc *o, double dt, double *PRHS)
                 double DivUm = (U[IX(ip, j, k)] - U[IX(i, j, k)]) / dx1[i]
                              + ((double)jvp * V[IX(i, jp, k)] - (double)jvm * V[IX(i, j, k)]) / dx2[j]
                              + (W[IX(i, j, kp)] - W[IX(i, j, k)]) / dx3[k];
-                double cbc = (1.0 - (double)jvm) * VBCbt[B2(i, k)] / dx2[j]
+                double cbc = PLANT(303, -1.0, 1.0) * (1.0 - (double)jvm) * VBCbt[B2(i, k)] / dx2[j]
                            - (1.0 - (double)jvp) * VBCup[B2(i, k)] / dx2[j];
                 double ddpdx1 = (dPhat[IX(i, j, k)] - dPhat[IX(im, j, k)]) / dmx1[i];
                 double ddpdx2 = (dPhat[IX(ip, j, k)] - dPhat[IX(i, j, k)]) / dmx1[ip];
@@ -1501,8 +1581,9 @@ void orc_pressure_RHS(orc *o, double dt, double *PRHS)
                 double ddpdz6 = (dPhat[IX(i, j, kp)] - dPhat[IX(i, j, k)]) / dmx3[kp];
                 double invRho1 = 0.5 / dmx1[i] * (dx1[im] * invRho[IX(i, j, k)] + dx1[i] * invRho[IX(im, j, k)]);
                 double invRho2 = 0.5 / dmx1[ip] * (dx1[ip] * invRho[IX(i, j, k)] + dx1[i] * invRho[IX(ip, j, k)]);
-                double invRho3 = 0.5 / dmx2[j] * (dx2[jm] * invRho[IX(i, j, k)] + dx2[j] * invRho[IX(i, jm, k)]);
-                double invRho4 = 0.5 / dmx2[jp] * (dx2[jp] * invRho[IX(i, j, k)] + dx2[j] * invRho[IX(i, jp, k)]);
+                double invRho3 = PLANT(302, 0.5 / dmx2[j] * (dx2[j] * invRho[IX(i, j, k)] + dx2[jm] * invRho[IX(i, jm, k)]),
+                                       0.5 / dmx2[j] * (dx2[jm] * invRho[IX(i, j, k)] + dx2[j] * invRho[IX(i, jm, k)]));
+                double invRho4 = 0.5 / dmx2[jp] * (dx2[jp] * invRho[IX(i, j, k)] + dx2[j] * invRho[IX(i, PLANT(301, jm, jp), k)]);
                 double invRho5 = 0.5 / dmx3[k] * (dx3[km] * invRho[IX(i, j, k)] + dx3[k] * invRho[IX(i, j, km)]);
                 double invRho6 = 0.5 / dmx3[kp] * (dx3[kp] * invRho[IX(i, j, k)] + dx3[k] * invRho[IX(i, j, kp)]);
                 double ExtraTerm = ((1.0 - invRho2) * ddpdx2 - (1.0 - invRho1) * ddpdx1) / dx1[i]
@@ -1703,6 +1784,15 @@ void orc_pressure_Projection(orc *o, double dt)
     double *dP = o->f[OF_DP], *dPhat = o->f[OF_DPHAT], *dP0 = o->f[OF_DP0];
     const double *invRho = o->f[OF_INVRHO];
     const double *dx1 = o->dx1, *dx2 = o->dx2, *dx3 = o->dx3, *dmx1 = o->dmx1, *dmx2 = o->dmx2, *dmx3 = o->dmx3;
+    /* the gradients read dPhat as the loop nest leaves it (Q26); the
+       planted site 315 reads a copy of the old dPhat instead */
+    double *dPh_old = NULL;
+    if (PLANT(315, 1, 0)) {
+        size_t nb = (size_t)o->N1 * o->N2 * o->N3 * sizeof(double);
+        dPh_old = (double *)malloc(nb);
+        memcpy(dPh_old, dPhat, nb);
+    }
+    const double *dPr = dPh_old ? dPh_old : dPhat;
     for (int k = 1; k <= o->n3msub; k++) {
         int km = k - 1;
         for (int j = 2; j <= o->n2msub; j++) {
@@ -1712,18 +1802,18 @@ void orc_pressure_Projection(orc *o, double dt)
                 double invRhoc = 0.5 / dmx1[i] * (dx1[i] * invRho[IX(im, j, k)] + dx1[im] * invRho[IX(i, j, k)]);
                 U[IX(i, j, k)] = U[IX(i, j, k)]
                                - dt * Cmp * ((dP[IX(i, j, k)] - dP[IX(im, j, k)]) / dmx1[i]
-                                             + (invRhoc - 1.0) * (dPhat[IX(i, j, k)] - dPhat[IX(im, j, k)]) / dmx1[i]);
+                                             + (invRhoc - 1.0) * (dPr[IX(i, j, k)] - dPr[IX(im, j, k)]) / dmx1[i]);
                 invRhoc = 0.5 / dmx2[j] * (dx2[j] * invRho[IX(i, jm, k)] + dx2[jm] * invRho[IX(i, j, k)]);
                 V[IX(i, j, k)] = V[IX(i, j, k)]
                                - dt * Cmp * ((dP[IX(i, j, k)] - dP[IX(i, jm, k)]) / dmx2[j]
-                                             + (invRhoc - 1.0) * (dPhat[IX(i, j, k)] - dPhat[IX(i, jm, k)]) / dmx2[j]);
+                                             + (invRhoc - 1.0) * (dPr[IX(i, j, k)] - dPr[IX(i, jm, k)]) / dmx2[j]);
                 invRhoc = 0.5 / dmx3[k] * (dx3[k] * invRho[IX(i, j, km)] + dx3[km] * invRho[IX(i, j, k)]);
                 W[IX(i, j, k)] = W[IX(i, j, k)]
                                - dt * Cmp * ((dP[IX(i, j, k)] - dP[IX(i, j, km)]) / dmx3[k]
-                                             + (invRhoc - 1.0) * (dPhat[IX(i, j, k)] - dPhat[IX(i, j, km)]) / dmx3[k]);
+                                             + (invRhoc - 1.0) * (dPr[IX(i, j, k)] - dPr[IX(i, j, km)]) / dmx3[k]);
                 P[IX(i, j, k)] = P[IX(i, j, k)] + dP[IX(i, j, k)];
                 dP0[IX(i, j, k)] = (dPhat[IX(i, j, k)] + dP0[IX(i, j, k)]) / 2.0;
-                dPhat[IX(i, j, k)] = 2.0 * dP[IX(i, j, k)] - dP0[IX(i, j, k)];
+                dPhat[IX(i, j, k)] = 2.0 * dP[IX(i, j, k)] - PLANT(311, 2.0 * dP0[IX(i, j, k)] - dPhat[IX(i, j, k)], dP0[IX(i, j, k)]);
             }
         }
         {
@@ -1733,24 +1823,24 @@ void orc_pressure_Projection(orc *o, double dt)
                 double invRhoc = 0.5 / dmx1[i] * (dx1[i] * invRho[IX(im, j, k)] + dx1[im] * invRho[IX(i, j, k)]);
                 U[IX(i, j, k)] = U[IX(i, j, k)]
                                - dt * Cmp * ((dP[IX(i, j, k)] - dP[IX(im, j, k)]) / dmx1[i]
-                                             + (invRhoc - 1.0) * (dPhat[IX(i, j, k)] - dPhat[IX(im, j, k)]) / dmx1[i]);
+                                             + (invRhoc - 1.0) * (dPr[IX(i, j, k)] - dPr[IX(im, j, k)]) / dmx1[i]);
                 invRhoc = 0.5 / dmx2[j] * (dx2[j] * invRho[IX(i, jm, k)] + dx2[jm] * invRho[IX(i, j, k)]);
                 V[IX(i, j, k)] = V[IX(i, j, k)]
                                - dt * Cmp * ((dP[IX(i, j, k)] - dP[IX(i, jm, k)]) / dmx2[j]
-                                             + (invRhoc - 1.0) * (dPhat[IX(i, j, k)] - dPhat[IX(i, jm, k)]) / dmx2[j]) * (double)(2 - o->j_indexS);
+                                             + (invRhoc - 1.0) * (dPr[IX(i, j, k)] - dPr[IX(i, jm, k)]) / dmx2[j]) * PLANT(312, 1.0, (double)(2 - o->j_indexS));
                 invRhoc = 0.5 / dmx3[k] * (dx3[k] * invRho[IX(i, j, km)] + dx3[km] * invRho[IX(i, j, k)]);
                 W[IX(i, j, k)] = W[IX(i, j, k)]
                                - dt * Cmp * ((dP[IX(i, j, k)] - dP[IX(i, j, km)]) / dmx3[k]
-                                             + (invRhoc - 1.0) * (dPhat[IX(i, j, k)] - dPhat[IX(i, j, km)]) / dmx3[k]);
+                                             + (invRhoc - 1.0) * (dPr[IX(i, j, k)] - dPr[IX(i, j, km)]) / dmx3[k]);
                 P[IX(i, j, k)] = P[IX(i, j, k)] + dP[IX(i, j, k)];
                 dP0[IX(i, j, k)] = (dPhat[IX(i, j, k)] + dP0[IX(i, j, k)]) / 2.0;
-                dPhat[IX(i, j, k)] = 2.0 * dP[IX(i, j, k)] - dP0[IX(i, j, k)];
+                dPhat[IX(i, j, k)] = 2.0 * dP[IX(i, j, k)] - PLANT(311, 2.0 * dP0[IX(i, j, k)] - dPhat[IX(i, j, k)], dP0[IX(i, j, k)]);
             }
         }
         {
             double s = dmx2[1] + dmx2[2];
             double Pbc_a = (s * s) / ((s * s) - (dmx2[1] * dmx2[1]));
-            double Pbc_b = (dmx2[1] * dmx2[1]) / ((s * s) - (dmx2[1] * dmx2[1]));
+            double Pbc_b = (dmx2[1] * dmx2[PLANT(313, 2, 1)]) / ((s * s) - (dmx2[1] * dmx2[1]));
             for (int i = 1; i <= o->n1sub - 1; i++)
                 P[IX(i, 0, k)] = Pbc_a * P[IX(i, 1, k)] - Pbc_b * P[IX(i, 2, k)];
             int n2sub = o->n2sub, n2msub = o->n2msub;
@@ -1761,6 +1851,7 @@ void orc_pressure_Projection(orc *o, double dt)
                 P[IX(i, n2sub, k)] = Pbc_a * P[IX(i, n2msub, k)] - Pbc_b * P[IX(i, n2sub - 2, k)];
         }
     }
+    free(dPh_old);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1776,6 +1867,7 @@ double orc_post_div(orc *o)
                 double d = (U[IX(i + 1, j, k)] - U[IX(i, j, k)]) / o->dx1[i]
                          + (V[IX(i, j + 1, k)] - V[IX(i, j, k)]) / o->dx2[j]
                          + (W[IX(i, j, k + 1)] - W[IX(i, j, k)]) / o->dx3[k];
+                d = PLANT(323, fabs(d), d);
                 if (d > mx) mx = d;
             }
     return mx;
@@ -1788,8 +1880,9 @@ double orc_post_cfl(orc *o, double dt, double *newdt)
     for (int k = 1; k <= o->n3sub - 1; k++)
         for (int j = 1; j <= o->n2sub - 1; j++)
             for (int i = 1; i <= o->n1sub - 1; i++) {
-                double c = (fabs(U[IX(i + 1, j, k)] + U[IX(i, j, k)]) / 2.0) / o->dx1[i]
-                         + (fabs(V[IX(i, j + 1, k)] + V[IX(i, j, k)]) / 2.0) / o->dx2[j]
+                double c = PLANT(322, (fabs(U[IX(i + 1, j, k)]) + fabs(U[IX(i, j, k)])) / 2.0,
+                                 fabs(U[IX(i + 1, j, k)] + U[IX(i, j, k)]) / 2.0) / o->dx1[i]
+                         + (fabs(V[IX(i, j + 1, k)] + V[IX(i, j, k)]) / 2.0) / PLANT(321, o->dmx2[j], o->dx2[j])
                          + (fabs(W[IX(i, j, k + 1)] + W[IX(i, j, k)]) / 2.0) / o->dx3[k];
                 if (c > mx) mx = c;
             }

@@ -49,6 +49,20 @@ enum {
 
 typedef struct orc orc;
 
+/* Test hooks.  orc_capture(o, 1) records, for every line solve a routine
+   makes, its rows (a, b, c), right-hand side d and solution x; (o, 0) stops
+   and clears.  Records are (n_row, n_sys) with the system index fastest,
+   in the order of the solves (at most ORC_MAXCAP).  orc_set_plant(id)
+   selects a planted mistake in the oracle built with -DORC_PLANTS
+   (liboracle_plants.so); orc_plants_built() tells the builds apart. */
+#define ORC_MAXCAP 32
+void orc_capture(orc *o, int on);
+int orc_capture_count(orc *o);
+int orc_capture_info(orc *o, int idx, int *n_sys, int *n_row, int *cyclic);
+int orc_capture_copy(orc *o, int idx, double *a, double *b, double *c, double *d, double *x);
+void orc_set_plant(int id);
+int orc_plants_built(void);
+
 /* one axis of mpi_subdomain_mesh on a single rank; x,dx,dmx have nm+2 entries */
 void orc_mesh_axis(int nm, double L, int uniform, double gamma, int periodic,
                    double *x, double *dx, double *dmx);
