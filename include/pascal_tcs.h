@@ -113,8 +113,16 @@ int tcs_field_download(tcs_ctx *ctx, int32_t id, double *host);
    rounded up to a multiple of 16, so every row's interior starts on a
    128-byte line); the upload/download entries take the reference's
    unpadded layout.  Valid until the next tcs_step, which writes the updates
-   of T, U, V, W, dPhat and dP0 into second buffers and exchanges them. */
+   of T, U, V, W, dPhat and dP0 into second buffers and exchanges them.
+   tcs_post_div keeps the CFL maximum of the same pass for tcs_post_cfl
+   until an entry point that may change U, V or W runs; a caller that
+   writes a field through this pointer calls tcs_fields_touched before the
+   next tcs_post_cfl. */
 int tcs_field_device_ptr(tcs_ctx *ctx, int32_t id, double **dptr);
+/* Declares that fields were written outside the entry points (through
+   tcs_field_device_ptr): drops the CFL maximum kept by tcs_post_div.  No
+   counterpart in the reference (its fields are host arrays). */
+int tcs_fields_touched(tcs_ctx *ctx);
 /* row pitch (elements) of the full fields on the device (no counterpart in
    the reference: the device layout of tcs_field_device_ptr) */
 int tcs_field_pitch(tcs_ctx *ctx, int32_t *pitch);
@@ -164,7 +172,9 @@ int tcs_pressure_projection(tcs_ctx *ctx, double dt);
 int tcs_ghostcell_update(tcs_ctx *ctx, int32_t id);
 /* mpi_Post_Div(U,V,W,maxDivU), src/module_post.f90:100-158 (signed max) */
 int tcs_post_div(tcs_ctx *ctx, double *maxdiv);
-/* mpi_Post_CFL(U,V,W,newCFL,newdt), src/module_post.f90:168-230 */
+/* mpi_Post_CFL(U,V,W,newCFL,newdt), src/module_post.f90:168-230.  Reuses
+   the maximum of the preceding tcs_post_div pass when no entry point that
+   may change U, V, W ran in between (see tcs_fields_touched). */
 int tcs_post_cfl(tcs_ctx *ctx, double *newcfl, double *newdt);
 /* The loop body of src/main.f90:131-203 (thermal, momentum, pressure,
    every halo update in main's order), fully on the device.  The same

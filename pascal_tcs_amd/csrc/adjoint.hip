@@ -682,6 +682,7 @@ using namespace tcs;
 
 extern "C" int tcs_adjoint_zero(tcs_ctx *c) {
     if (!c) return TCS_E_ARG;
+    TCS_TRY(flush_pending(c));
     TCS_TRY(adjoint_alloc(c));
     AdjWork *w = c->adj;
     for (int s = 0; s < adj::NSTATE; s++) TCS_HIP(hipMemsetAsync(w->b[s], 0, c->nfull * sizeof(double), c->stream));
@@ -711,6 +712,7 @@ static double *adjoint_array(tcs_ctx *c, int32_t kind, int32_t id, size_t &n) {
 
 extern "C" int tcs_adjoint_copy(tcs_ctx *c, int32_t kind, int32_t id, int32_t to_adjoint, double *dev) {
     if (!c || !dev) return TCS_E_ARG;
+    TCS_TRY(flush_pending(c));
     TCS_TRY(adjoint_alloc(c));
     size_t n = 0;
     double *a = adjoint_array(c, kind, id, n);
@@ -733,6 +735,7 @@ extern "C" int tcs_adjoint_copy(tcs_ctx *c, int32_t kind, int32_t id, int32_t to
 
 extern "C" int tcs_adjoint_add_nusselt(tcs_ctx *c, double weight, double Tup, double Tbt) {
     if (!c) return TCS_E_ARG;
+    TCS_TRY(flush_pending(c));
     TCS_TRY(adjoint_alloc(c));
     const Grid g = c->grid();
     const double wgt = weight * c->prm.L2 / (Tbt - Tup) / c->h_dmx2[1] / ((double)c->n1m * (double)c->n3m);
@@ -743,6 +746,7 @@ extern "C" int tcs_adjoint_add_nusselt(tcs_ctx *c, double weight, double Tup, do
 
 extern "C" int tcs_adjoint_step(tcs_ctx *c, double dt) {
     if (!c) return TCS_E_ARG;
+    TCS_TRY(flush_pending(c));
     if (c->prm.precision != 0) {
         set_error("tcs_adjoint_step: the adjoint differentiates the plain FP64 step (precision 0)");
         return TCS_E_ARG;

@@ -662,6 +662,7 @@ extern "C" int tcs_init(const tcs_params *prm, const double *dx1, const double *
 
 extern "C" int tcs_finalize(tcs_ctx *c) {
     if (!c) return TCS_E_ARG;
+    c->pend = 0;  // held launches of a context being destroyed are dropped
     (void)hipStreamSynchronize(c->stream);
     free_ctx(c);
     return TCS_OK;
@@ -669,17 +670,23 @@ extern "C" int tcs_finalize(tcs_ctx *c) {
 
 extern "C" int tcs_sync(tcs_ctx *c) {
     if (!c) return TCS_E_ARG;
+    TCS_TRY(flush_pending(c));
     TCS_HIP(hipStreamSynchronize(c->stream));
     return TCS_OK;
 }
 
-extern "C" void *tcs_stream(tcs_ctx *c) { return c ? (void *)c->stream : nullptr; }
+// the context's stream: what callers enqueue on it follows the held launches
+extern "C" void *tcs_stream(tcs_ctx *c) {
+    if (!c || flush_pending(c) != TCS_OK) return nullptr;
+    return (void *)c->stream;
+}
 
 static int check_field(tcs_ctx *c, int id) {
     if (!c || id < 0 || id >= TCS_NFIELD) {
         set_error("bad context or field id");
         return TCS_E_ARG;
     }
+    TCS_TRY(flush_pending(c));
     c->cfl_valid = 0;  // the field may change (upload, halo, a handed-out pointer)
     return TCS_OK;
 }
@@ -733,6 +740,13 @@ extern "C" int tcs_field_device_ptr(tcs_ctx *c, int32_t id, double **dptr) {
     return TCS_OK;
 }
 
+extern "C" int tcs_fields_touched(tcs_ctx *c) {
+    if (!c) return TCS_E_ARG;
+    TCS_TRY(flush_pending(c));
+    c->cfl_valid = 0;
+    return TCS_OK;
+}
+
 extern "C" int tcs_field_pitch(tcs_ctx *c, int32_t *pitch) {
     if (!c || !pitch) return TCS_E_ARG;
     *pitch = c->P1;
@@ -741,6 +755,7 @@ extern "C" int tcs_field_pitch(tcs_ctx *c, int32_t *pitch) {
 
 extern "C" int tcs_wall_upload(tcs_ctx *c, int32_t id, const double *host) {
     if (!c || id < 0 || id >= TCS_NWALL || !host) return TCS_E_ARG;
+    TCS_TRY(flush_pending(c));
     TCS_HIP(hipMemcpyAsync(c->wall[id], host, c->nwall * sizeof(double), hipMemcpyHostToDevice, c->stream));
     TCS_HIP(hipStreamSynchronize(c->stream));
     return TCS_OK;
@@ -748,6 +763,7 @@ extern "C" int tcs_wall_upload(tcs_ctx *c, int32_t id, const double *host) {
 
 extern "C" int tcs_wall_download(tcs_ctx *c, int32_t id, double *host) {
     if (!c || id < 0 || id >= TCS_NWALL || !host) return TCS_E_ARG;
+    TCS_TRY(flush_pending(c));
     TCS_HIP(hipMemcpyAsync(host, c->wall[id], c->nwall * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     TCS_HIP(hipStreamSynchronize(c->stream));
     return TCS_OK;
@@ -767,6 +783,7 @@ static int set_row(tcs_ctx *c, int field, int jrow, int wall) {
 
 extern "C" int tcs_thermal_boundary(tcs_ctx *c, double Tup, double Tbt) {
     if (!c) return TCS_E_ARG;
+    TCS_TRY(flush_pending(c));
     TCS_TRY(fill_wall(c, TCS_TBCUP, Tup));
     TCS_TRY(fill_wall(c, TCS_TBCBT, Tbt));
     // comm_1d_x2%myrank == 0 branch only (single y-rank)
@@ -777,6 +794,7 @@ extern "C" int tcs_thermal_boundary(tcs_ctx *c, double Tup, double Tbt) {
 extern "C" int tcs_momentum_boundary(tcs_ctx *c, double Uup, double Ubt, double Vup, double Vbt, double Wup,
                                      double Wbt) {
     if (!c) return TCS_E_ARG;
+    TCS_TRY(flush_pending(c));
     c->cfl_valid = 0;
     TCS_TRY(fill_wall(c, TCS_UBCUP, Uup));
     TCS_TRY(fill_wall(c, TCS_VBCUP, Vup));
@@ -792,7 +810,8 @@ extern "C" int tcs_momentum_boundary(tcs_ctx *c, double Uup, double Ubt, double 
 }
 
 #define CTX_CHECK(c) \
-    if (!(c)) { set_error("null context"); return TCS_E_ARG; }
+    if (!(c)) { set_error("null context"); return TCS_E_ARG; } \
+    TCS_TRY(flush_pending(c));
 // entries that may change fields drop the kept CFL maximum
 #define CTX_GUARD(c) \
     CTX_CHECK(c) \
@@ -806,22 +825,125 @@ extern "C" int tcs_momentum_boundary(tcs_ctx *c, double Uup, double Ubt, double 
      : (c)->prm.precision == 1 ? ::tcs::fast::fn((c), ##__VA_ARGS__)         \
                                : ::tcs::exact::fn((c), ##__VA_ARGS__))
 
+namespace tcs {
+int flush_pending(tcs_ctx *c) {
+    const int p = c->pend;
+    if (!p) return TCS_OK;
+    c->pend = 0;
+    if (p & PEND_LDV) {
+        ProfStage _s(c, "momentum blockLdV");
+        TCS_TRY(FORM(c, momentum_blockLdV, c->pend_dt_ldv));
+    }
+    if (p & PEND_HALO_DV) {
+        ProfStage _s(c, "update ghostcell");
+        TCS_TRY(halo(c, c->f[TCS_DV]));
+    }
+    if (p & PEND_LDU) {
+        ProfStage _s(c, "momentum blockLdU");
+        TCS_TRY(FORM(c, momentum_blockLdU, c->pend_dt_ldu));
+    }
+    if (p & PEND_RHS) {
+        ProfStage _s(c, "pressure RHS");
+        TCS_TRY(FORM(c, pressure_rhs, c->pend_dt_rhs));
+    }
+    return TCS_OK;
+}
+}  // namespace tcs
+
 // stage labels follow the reference's timer labels (src/main.f90:55-72)
 extern "C" int tcs_thermal_coeffi(tcs_ctx *c) { CTX_GUARD(c); STAGE("thermal coeff"); return FORM(c, thermal_coeffi); }
-extern "C" int tcs_thermal_solver(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("thermal solve"); return FORM(c, thermal_solver, dt); }
+// the last sweep writes T + dT into T's second buffer, which becomes T
+extern "C" int tcs_thermal_solver(tcs_ctx *c, double dt) {
+    CTX_GUARD(c);
+    STAGE("thermal solve");
+    bool swap_T = false;
+    TCS_TRY(FORM(c, thermal_solver_into, dt, c->spare[TCS_T], &swap_T));
+    if (swap_T) std::swap(c->f[TCS_T], c->spare[TCS_T]);
+    return TCS_OK;
+}
 extern "C" int tcs_momentum_coeffi(tcs_ctx *c) { CTX_GUARD(c); STAGE("momentum coeff"); return FORM(c, momentum_coeffi); }
 extern "C" int tcs_momentum_solvedU(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("momentum solvedU"); return FORM(c, momentum_solvedU, dt); }
 extern "C" int tcs_momentum_solvedV(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("momentum solvedV"); return FORM(c, momentum_solvedV, dt); }
 extern "C" int tcs_momentum_solvedW(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("momentum solvedW"); return FORM(c, momentum_solvedW, dt); }
-extern "C" int tcs_momentum_blockLdV(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("momentum blockLdV"); return FORM(c, momentum_blockLdV, dt); }
-extern "C" int tcs_momentum_blockLdU(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("momentum blockLdU"); return FORM(c, momentum_blockLdU, dt); }
-extern "C" int tcs_momentum_pseudoupdateUVW(tcs_ctx *c) { CTX_GUARD(c); STAGE("momentum updatePseudoUVW"); return FORM(c, momentum_pseudoupdate); }
+// blockLdV waits: main follows it with the dV halo, blockLdU and the
+// pseudo-update, which then run as tcs_step runs them
+extern "C" int tcs_momentum_blockLdV(tcs_ctx *c, double dt) {
+    CTX_GUARD(c);
+    c->pend = PEND_LDV;
+    c->pend_dt_ldv = dt;
+    return TCS_OK;
+}
+extern "C" int tcs_momentum_blockLdU(tcs_ctx *c, double dt) {
+    if (c && c->pend == (PEND_LDV | PEND_HALO_DV) && dt == c->pend_dt_ldv) {
+        c->cfl_valid = 0;
+        c->pend |= PEND_LDU;
+        c->pend_dt_ldu = dt;
+        return TCS_OK;
+    }
+    CTX_GUARD(c);
+    STAGE("momentum blockLdU");
+    return FORM(c, momentum_blockLdU, dt);
+}
+extern "C" int tcs_momentum_pseudoupdateUVW(tcs_ctx *c) {
+    if (c && c->pend == (PEND_LDV | PEND_HALO_DV | PEND_LDU)) {
+        // the block corrections write U + dU, V + dV, W + dW into the
+        // second buffers (they read the old U, V) as in tcs_step
+        c->pend = 0;
+        c->cfl_valid = 0;
+        {
+            STAGE("momentum blockLdV");
+            TCS_TRY(FORM(c, momentum_blockLdV_pseudo, c->pend_dt_ldv, c->spare[TCS_V]));
+        }
+        {
+            STAGE("update ghostcell");
+            TCS_TRY(halo(c, c->f[TCS_DV]));
+        }
+        {
+            STAGE("momentum blockLdU");
+            TCS_TRY(FORM(c, momentum_blockLdU_pseudo, c->pend_dt_ldu, c->spare[TCS_U], c->spare[TCS_W]));
+        }
+        for (int q : {TCS_U, TCS_V, TCS_W}) std::swap(c->f[q], c->spare[q]);
+        return TCS_OK;
+    }
+    CTX_GUARD(c);
+    STAGE("momentum updatePseudoUVW");
+    return FORM(c, momentum_pseudoupdate);
+}
 extern "C" int tcs_pressure_wave_number(tcs_ctx *c) { CTX_GUARD(c); return upload_tables(c); }
-extern "C" int tcs_pressure_rhs(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("pressure RHS"); return FORM(c, pressure_rhs, dt); }
-extern "C" int tcs_pressure_poisson_fft2(tcs_ctx *c) { CTX_GUARD(c); STAGE("pressure PoissonSolve"); return FORM(c, pressure_poisson); }
-extern "C" int tcs_pressure_projection(tcs_ctx *c, double dt) { CTX_GUARD(c); STAGE("pressure projection"); return FORM(c, pressure_projection, dt); }
+// the right-hand side waits for the Poisson solve, whose x transform makes it
+extern "C" int tcs_pressure_rhs(tcs_ctx *c, double dt) {
+    CTX_GUARD(c);
+    c->pend = PEND_RHS;
+    c->pend_dt_rhs = dt;
+    return TCS_OK;
+}
+extern "C" int tcs_pressure_poisson_fft2(tcs_ctx *c) {
+    if (c && c->pend == PEND_RHS) {
+        c->pend = 0;
+        c->cfl_valid = 0;
+        STAGE("pressure PoissonSolve");
+        return FORM(c, pressure_poisson_rhs, c->pend_dt_rhs);
+    }
+    CTX_GUARD(c);
+    STAGE("pressure PoissonSolve");
+    return FORM(c, pressure_poisson);
+}
+// the dPhat / dP0 history goes into the second buffers, which become them
+extern "C" int tcs_pressure_projection(tcs_ctx *c, double dt) {
+    CTX_GUARD(c);
+    STAGE("pressure projection");
+    TCS_TRY(FORM(c, pressure_projection_into, dt, c->spare[TCS_DPHAT], c->spare[TCS_DP0]));
+    std::swap(c->f[TCS_DPHAT], c->spare[TCS_DPHAT]);
+    std::swap(c->f[TCS_DP0], c->spare[TCS_DP0]);
+    return TCS_OK;
+}
 
 extern "C" int tcs_ghostcell_update(tcs_ctx *c, int32_t id) {
+    if (c && id == TCS_DV && c->pend == PEND_LDV) {  // the dV halo between blockLdV and blockLdU
+        c->cfl_valid = 0;
+        c->pend |= PEND_HALO_DV;
+        return TCS_OK;
+    }
     TCS_TRY(check_field(c, id));
     if (id == TCS_INVRHO || id == TCS_MU) return TCS_OK;  // tables carry no x/z ghosts of their own
     STAGE("update ghostcell");

@@ -141,6 +141,10 @@ struct tcs_ctx {
     tcs::AdjWork *adj;       // null until an adjoint entry is called
     int cfl_valid;           // cfl_max holds mpi_Post_CFL's maximum for the current fields
     double cfl_max;
+    // routine calls held back so that the next call can fuse with them
+    // (PEND_* bits; tcs::flush_pending runs them as called)
+    int pend;
+    double pend_dt_ldv, pend_dt_ldu, pend_dt_rhs;
 
     tcs::Grid grid() const {
         tcs::Grid g;
@@ -259,6 +263,14 @@ int halo(tcs_ctx *c, std::initializer_list<double *> fields);
 // copy the cells outside the interior (x/z ghosts, wall rows) of a full field
 int copy_shell(tcs_ctx *c, const double *src, double *dst);
 int copy_shell(tcs_ctx *c, std::initializer_list<std::pair<const double *, double *>> pairs);
+// Routine entries that hold their launches back: tcs_momentum_blockLdV, the
+// ghost-cell update of dV after it and tcs_momentum_blockLdU wait for
+// tcs_momentum_pseudoupdateUVW (which then runs the block corrections with
+// the pseudo-update fused in), tcs_pressure_rhs waits for
+// tcs_pressure_poisson_fft2 (the right-hand side made inside the x
+// transform).  Every other entry point runs what is held first, unfused.
+enum { PEND_LDV = 1, PEND_HALO_DV = 2, PEND_LDU = 4, PEND_RHS = 8 };
+int flush_pending(tcs_ctx *c);
 int post_div(tcs_ctx *c, double *out);
 int post_cfl(tcs_ctx *c, double *maxuovx);
 int nusselt(tcs_ctx *c, double Tup, double Tbt, double *nb, double *nt);

@@ -8,6 +8,8 @@ module mpi_thermal
     double precision, allocatable, dimension(:,:,:), target :: T
     double precision, allocatable, dimension(:,:), target :: TBCbt_sub, TBCup_sub
     double precision, allocatable, dimension(:,:,:) :: KPP, dKPP, invRhoCp, dinvRhoCp
+    ! mpi_thermal_coeffi copies the coefficient arrays to the host
+    logical :: coeffi_to_host = .true.
 contains
     !> :43-56
     subroutine mpi_thermal_allocation()
@@ -46,12 +48,25 @@ contains
         call tcs_mark_host(TCS_T)
     end subroutine mpi_thermal_boundary
 
-    !> :116-156.  The GPU solver evaluates the property polynomials from T
-    !> inline, so only the arrays' allocation is kept (main never reads them).
+    !> :116-156: KPP, dKPP, invRhoCp, dinvRhoCp from the current T.  The
+    !> GPU solver evaluates the same polynomials from T inline, so the time
+    !> step itself never reads these arrays.  With coeffi_to_host (the
+    !> default) they are made on the device and copied into the host arrays,
+    !> as after the reference's routine; a caller that never reads them (the
+    !> reference's main does not) may set coeffi_to_host = .false., which
+    !> skips that pass and the four device-to-host copies per call.
     subroutine mpi_thermal_coeffi()
         use mpi_subdomain, only : n1sub, n2sub, n3sub
         allocate(KPP(0:n1sub,0:n2sub,0:n3sub), dKPP(0:n1sub,0:n2sub,0:n3sub))
         allocate(invRhoCp(0:n1sub,0:n2sub,0:n3sub), dinvRhoCp(0:n1sub,0:n2sub,0:n3sub))
+        if (coeffi_to_host) then
+            call tcs_push_dirty(tcs_nfull())
+            call tcs_check(tcs_thermal_coeffi(tcs_ctx), 'mpi_thermal_coeffi')
+            call tcs_pull(TCS_KPP, KPP)
+            call tcs_pull(TCS_DKPP, dKPP)
+            call tcs_pull(TCS_INVRHOCP, invRhoCp)
+            call tcs_pull(TCS_DINVRHOCP, dinvRhoCp)
+        end if
     end subroutine mpi_thermal_coeffi
 
     !> :161-167

@@ -24,6 +24,7 @@ module tcs_capi
     integer(c_int32_t), parameter :: TCS_T = 0, TCS_U = 1, TCS_V = 2, TCS_W = 3, TCS_P = 4
     integer(c_int32_t), parameter :: TCS_DU = 5, TCS_DV = 6, TCS_DW = 7, TCS_DP = 8
     integer(c_int32_t), parameter :: TCS_DPHAT = 9, TCS_DP0 = 10, TCS_INVRHO = 11, TCS_MU = 12
+    integer(c_int32_t), parameter :: TCS_KPP = 13, TCS_DKPP = 14, TCS_INVRHOCP = 15, TCS_DINVRHOCP = 16
     integer(c_int32_t), parameter :: TCS_NFIELD = 17
     integer(c_int32_t), parameter :: TCS_TBCUP = 0, TCS_TBCBT = 1, TCS_UBCUP = 2, TCS_UBCBT = 3
     integer(c_int32_t), parameter :: TCS_VBCUP = 4, TCS_VBCBT = 5, TCS_WBCUP = 6, TCS_WBCBT = 7
@@ -70,6 +71,10 @@ module tcs_capi
             type(c_ptr), value :: ctx
             integer(c_int32_t), value :: id
             real(c_double), intent(in) :: host(*)
+        end function
+        integer(c_int) function tcs_thermal_coeffi(ctx) bind(C, name='tcs_thermal_coeffi')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
         end function
         integer(c_int) function tcs_thermal_solver(ctx, dt) bind(C, name='tcs_thermal_solver')
             import :: c_int, c_ptr, c_double

@@ -187,6 +187,37 @@ class TCSSolver:
         """One MPM-STD step: the loop body of src/main.f90:131-203."""
         _lib.check(self.lib.tcs_step(self.h, self._dt(dt)), "tcs_step")
 
+    def step_routines(self, dt=None, coeffi=False):
+        """The same step as the reference's main program issues it, one
+        routine entry at a time with every ghost-cell update
+        (src/main.f90:131-203): the sequence the Fortran drop-in modules
+        call.  coeffi=True also materialises KPP, dKPP, invRhoCp, dinvRhoCp
+        (mpi_thermal_coeffi; the solver does not read them)."""
+        dt = self._dt(dt)
+        if coeffi:
+            self.mpi_thermal_coeffi()
+        self.mpi_thermal_solver(dt)
+        self.mpi_subdomain_ghostcell_update("T")
+        self.mpi_momentum_coeffi()
+        self.mpi_momentum_solvedU(dt)
+        self.mpi_subdomain_ghostcell_update("dU")
+        self.mpi_momentum_solvedV(dt)
+        self.mpi_subdomain_ghostcell_update("dV")
+        self.mpi_momentum_solvedW(dt)
+        self.mpi_subdomain_ghostcell_update("dW")
+        self.mpi_momentum_blockLdV(dt)
+        self.mpi_subdomain_ghostcell_update("dV")
+        self.mpi_momentum_blockLdU(dt)
+        self.mpi_momentum_pseudoupdateUVW()
+        for n in ("U", "V", "W"):
+            self.mpi_subdomain_ghostcell_update(n)
+        self.mpi_pressure_RHS(dt)
+        self.mpi_pressure_Poisson_FFT2()
+        self.mpi_subdomain_ghostcell_update("dP")
+        self.mpi_pressure_Projection(dt)
+        for n in ("U", "V", "W", "P", "dPhat"):
+            self.mpi_subdomain_ghostcell_update(n)
+
     def profile(self, enable=True):
         """Enable (and clear) or disable the per-stage / per-kernel timers."""
         _lib.check(self.lib.tcs_profile(self.h, int(bool(enable))), "tcs_profile")

@@ -150,3 +150,75 @@ def test_nusselt_rollout_gradient():
         print(f"direction {r}: fd {fd:.10e} adjoint {ad:.10e}", flush=True)
         assert abs(fd - ad) <= 1e-5 * abs(fd), (fd, ad)
     s.close()
+
+
+def test_nusselt_gradient_full_vector_vs_componentwise_central_differences():
+    """The FP64 reference gradient: every component of the bottom-wall
+    temperature gradient of a short rollout by its own central difference
+    (n1m x n3m = 64 components, 2 rollouts each), against the adjoint's
+    whole gradient vector through the autograd function."""
+    import torch
+    from pascal_tcs_amd.adjoint import STATE as ST, NusseltRollout, nusselt_loss_function
+    from pascal_tcs_amd.solver import TCSSolver
+    cfg = small_config(8, 6, 8, props="water", stretched=True, Ra=1e7, DeltaT=0.2)
+    s = TCSSolver(cfg, tdma_split=0, precision=0)
+    s.setup_seeded(amp=0.05, seed=5)
+    state0 = {n: s.get(n) for n in ST}
+    rng = np.random.default_rng(29)
+    theta0 = 0.01 * rng.standard_normal((cfg.n3m, cfg.n1m))
+    roll = NusseltRollout(s, state0, nsteps=4, dt=cfg.dtStart, every=2)
+    th = torch.tensor(theta0, dtype=torch.float64, device="cuda", requires_grad=True)
+    L = nusselt_loss_function().apply(th, roll)
+    L.backward()
+    g = th.grad.detach().cpu().numpy()
+    eps = 1e-5
+    fd = np.zeros_like(theta0)
+    for idx in np.ndindex(theta0.shape):
+        e = np.zeros_like(theta0)
+        e[idx] = eps
+        fd[idx] = (roll.loss(theta0 + e)[0] - roll.loss(theta0 - e)[0]) / (2 * eps)
+    rel = np.linalg.norm(g - fd) / np.linalg.norm(fd)
+    print(f"componentwise FD gradient: |g| {np.linalg.norm(fd):.6e}, relative difference {rel:.3e}", flush=True)
+    assert np.abs(fd).max() > 0
+    assert rel <= 1e-6, rel
+    s.close()
+
+
+def test_config4_adjoint_128_cube_50_steps():
+    """The 128^3 adjoint case: 2 pi x 2 x 2 pi box, uniform grid, Ra 1e7,
+    water properties (DeltaT 0.2), conduction profile plus noise 0.05, dt
+    1e-3, 50 plain FP64 steps with split lines, checkpoints every 10 steps;
+    the gradient of the mean bottom-wall Nusselt number with respect to a
+    128 x 128 bottom-wall temperature perturbation (Gaussian, sigma 0.01)
+    through the autograd function, against central differences on 16
+    seeded directions to 1e-5 relative."""
+    import torch
+    from pascal_tcs_amd.adjoint import STATE as ST, NusseltRollout, nusselt_loss_function
+    from pascal_tcs_amd.config import Properties, TCSConfig
+    from pascal_tcs_amd.solver import TCSSolver
+    n = 128
+    cfg = TCSConfig(n1m=n, n2m=n, n3m=n, uniform2=1, Ra=1e7, Pr=0.71, DeltaT=0.2, props=Properties.water(),
+                    dtStart=1e-3)
+    s = TCSSolver(cfg, tdma_split=1, precision=0)
+    s.setup_seeded(amp=0.05, seed=20240512)
+    state0 = {k: s.get(k) for k in ST}
+    rng = np.random.default_rng(4)
+    theta0 = 0.01 * rng.standard_normal((n, n))
+    roll = NusseltRollout(s, state0, nsteps=50, dt=cfg.dtStart, every=10,
+                          log=lambda m: print(m, flush=True))
+    th = torch.tensor(theta0, dtype=torch.float64, device="cuda", requires_grad=True)
+    L = nusselt_loss_function().apply(th, roll)
+    print(f"config 4 forward done, loss {float(L.detach()):.12e}", flush=True)
+    L.backward()
+    g = th.grad.detach().cpu().numpy()
+    print(f"config 4 backward done, |grad| {np.linalg.norm(g):.6e}", flush=True)
+    eps = 1e-5
+    worst = 0.0
+    for r in range(16):
+        d = rng.standard_normal(theta0.shape)
+        fd = (roll.loss(theta0 + eps * d)[0] - roll.loss(theta0 - eps * d)[0]) / (2 * eps)
+        ad = float(np.vdot(g, d))
+        worst = max(worst, abs(fd - ad) / abs(fd))
+        print(f"direction {r}: fd {fd:.10e} adjoint {ad:.10e}", flush=True)
+    assert worst <= 1e-5, worst
+    s.close()

@@ -83,20 +83,25 @@ def test_axis_solve(shape, axis, cyclic):
         assert np.array_equal(g, e), f"{name} differs: {np.abs(g - e).max()}"
 
 
+@pytest.mark.parametrize("cyclic", [False, True])
 @pytest.mark.parametrize("axis", [0, 1, 2])
-def test_axis_solve_512_cube_sampled(axis):
-    """Full 512^3 size: the residual of every system and, on a sample of
-    systems, bit-exactness against the oracle."""
+def test_axis_solve_512_cube_sampled(axis, cyclic):
+    """Full 512^3 size with the single-rank kernels (the reference's
+    operation order, PaScaL_TDMA/src/pascal_tdma.f90:487-578 and :715-806):
+    the residual of every system (with the periodic wrap-around terms for
+    the cyclic solve) and, on a sample of systems, bit-exactness against
+    the oracle."""
     import torch
     from pascal_tcs_amd import ptdma
     n = 512
-    g = torch.Generator(device="cuda").manual_seed(11 + axis)
+    print(f"512^3 single-rank solve axis={axis} cyclic={cyclic}", flush=True)
+    g = torch.Generator(device="cuda").manual_seed(11 + axis + 7 * int(cyclic))
     a = -torch.rand((n, n, n), dtype=torch.float64, device="cuda", generator=g)
     c = -torch.rand((n, n, n), dtype=torch.float64, device="cuda", generator=g)
     b = a.abs() + c.abs() + 1.0 + torch.rand((n, n, n), dtype=torch.float64, device="cuda", generator=g)
     d = 2.0 * torch.rand((n, n, n), dtype=torch.float64, device="cuda", generator=g) - 1.0
     A, B, C, D = a.clone(), b.clone(), c.clone(), d.clone()
-    ptdma.axis_solve(A, B, C, D, axis, cyclic=False)
+    ptdma.axis_solve(A, B, C, D, axis, cyclic=cyclic)
     torch.cuda.synchronize()
     dim = 2 - axis  # torch dim of the row axis in (k, j, i)
     x = D
@@ -107,8 +112,12 @@ def test_axis_solve_512_cube_sampled(axis):
     shape[dim] = n
     first = (idx == 0).view(shape)
     last = (idx == n - 1).view(shape)
-    r = b * x + torch.where(first, torch.zeros_like(x), a * xm) + torch.where(last, torch.zeros_like(x), c * xp) - d
+    if cyclic:
+        r = b * x + a * xm + c * xp - d
+    else:
+        r = b * x + torch.where(first, torch.zeros_like(x), a * xm) + torch.where(last, torch.zeros_like(x), c * xp) - d
     assert r.abs().max().item() < 1e-12
+    del r, xm, xp
     # sampled bit-exactness: 64 systems
     rng = np.random.default_rng(axis)
     picks = rng.integers(0, n, size=(64, 2))
@@ -116,7 +125,7 @@ def test_axis_solve_512_cube_sampled(axis):
         sl = [int(p), int(q)]
         sl.insert(dim, slice(None))
         sys_ = [t[tuple(sl)].cpu().numpy().reshape(n, 1).copy() for t in (a, b, c, d)]
-        O.ptdma_many_solve(*sys_, cyclic=False)
+        O.ptdma_many_solve(*sys_, cyclic=cyclic)
         assert np.array_equal(D[tuple(sl)].cpu().numpy(), sys_[3][:, 0])
 
 
