@@ -91,6 +91,8 @@ def lib(plants=False):
         L.orc_capture_info.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _ip, _ip]
         L.orc_capture_copy.restype = ctypes.c_int
         L.orc_capture_copy.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_dp] * 5
+        _ipp = ctypes.POINTER(ctypes.c_int)
+        L.orc_heat_step.argtypes = [ctypes.c_int] * 3 + [_dp] * 6 + [_ipp, _ipp] + [ctypes.c_double] * 5
         L.orc_set_plant.argtypes = [ctypes.c_int]
         L.orc_plants_built.restype = ctypes.c_int
         _LIBS[fname] = L
@@ -267,3 +269,44 @@ class Oracle:
         if not self._L.orc_plants_built():
             raise RuntimeError("planted mistakes need the oracle built with plants=True")
         self._L.orc_set_plant(int(site))
+
+
+class HeatExample:
+    """The PaScaL_TDMA heat-conduction example on one rank
+    (PaScaL_TDMA/examples/global.f90:87-106, mpi_subdomain.f90:256-413):
+    parameters, mesh, wall masks and tables for nx, ny, nz as the example's
+    input file gives them; step(theta) is one solve_theta_plan_many."""
+
+    def __init__(self, nx, ny, nz):
+        self.nx, self.ny, self.nz = nx + 1, ny + 1, nz + 1
+        self.nx_sub, self.ny_sub, self.nz_sub = self.nx, self.ny, self.nz
+        Pr, Ra = 5.0, 2.0e2
+        self.dt = 5.0e-3
+        lx = ly = lz = 1.0
+        self.theta_cold = -1.0
+        self.theta_hot = 2.0 + self.theta_cold
+        nu = 1.0 / np.sqrt(Ra / (1.0 * Pr * ly ** 3 * (self.theta_hot - self.theta_cold)))
+        self.Ct = nu / Pr
+        self.dx, self.dy, self.dz = lx / (self.nx - 1), ly / self.ny, lz / (self.nz - 1)
+        self.dmx = np.full(self.nx_sub + 1, self.dx)
+        self.dmy = np.full(self.ny_sub + 1, self.dy)
+        self.dmz = np.full(self.nz_sub + 1, self.dz)
+        self.dmy[0] = self.dmy[-1] = self.dy / 2.0
+        self.jmbc = np.ones(self.ny_sub + 1, dtype=np.int32)
+        self.jpbc = np.ones(self.ny_sub + 1, dtype=np.int32)
+        self.jmbc[1] = 0
+        self.jpbc[self.ny_sub - 1] = 0
+        shape = (self.nz_sub + 1, self.nx_sub + 1)
+        self.bc3 = np.full(shape, self.theta_hot)
+        self.bc4 = np.full(shape, self.theta_cold)
+
+    @property
+    def shape(self):
+        return (self.nz_sub + 1, self.ny_sub + 1, self.nx_sub + 1)
+
+    def step(self, theta, plants=False):
+        assert theta.shape == self.shape and theta.dtype == np.float64 and theta.flags["C_CONTIGUOUS"]
+        _ip = ctypes.POINTER(ctypes.c_int)
+        lib(plants).orc_heat_step(self.nx_sub, self.ny_sub, self.nz_sub, _p(theta), _p(self.bc3), _p(self.bc4),
+                                  _p(self.dmx), _p(self.dmy), _p(self.dmz), self.jmbc.ctypes.data_as(_ip),
+                                  self.jpbc.ctypes.data_as(_ip), self.dx, self.dy, self.dz, self.Ct, self.dt)

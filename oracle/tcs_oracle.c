@@ -1928,3 +1928,125 @@ void orc_step(orc *o, double dt)
     orc_ghostcell_update(o, o->f[OF_DPHAT]);
     free(prhs);
 }
+
+/* ------------------------------------------------------------------------ */
+/* The PaScaL_TDMA heat-conduction example on one rank:                       */
+/* PaScaL_TDMA/examples/solve_theta_plan_many.f90:30-203 (one time step) with */
+/* the single-rank ghost-cell update of examples/mpi_subdomain.f90:219-249.  */
+/* theta is (0:nx_sub,0:ny_sub,0:nz_sub), i fastest; bc3/bc4 (0:nx_sub,      */
+/* 0:nz_sub); dmx/dmy/dmz (0:n_sub); jmbc/jpbc (0:ny_sub).                    */
+/* ------------------------------------------------------------------------ */
+void orc_heat_step(int nx_sub, int ny_sub, int nz_sub, double *theta, const double *bc3, const double *bc4,
+                   const double *dmx, const double *dmy, const double *dmz, const int *jmbc, const int *jpbc,
+                   double dx, double dy, double dz, double Ct, double dt)
+{
+    const size_t NX = (size_t)nx_sub + 1, NY = (size_t)ny_sub + 1;
+#define TH(i, j, k) theta[(size_t)(i) + NX * ((size_t)(j) + NY * (size_t)(k))]
+#define RH(i, j, k) rhs[(size_t)(i) + NX * ((size_t)(j) + NY * (size_t)(k))]
+    double *rhs = (double *)calloc(NX * NY * ((size_t)nz_sub + 1), sizeof(double));
+    /* right-hand side (:63-117) */
+    for (int k = 1; k <= nz_sub - 1; k++) {
+        int kp = k + 1, km = k - 1;
+        for (int j = 1; j <= ny_sub - 1; j++) {
+            int jp = j + 1, jm = j - 1;
+            int jep = jpbc[j], jem = jmbc[j];
+            for (int i = 1; i <= nx_sub - 1; i++) {
+                int ip = i + 1, im = i - 1;
+                double dedx1 = (TH(i, j, k) - TH(im, j, k)) / dmx[i];
+                double dedx2 = (TH(ip, j, k) - TH(i, j, k)) / dmx[ip];
+                double dedy3 = (TH(i, j, k) - TH(i, jm, k)) / dmy[j];
+                double dedy4 = (TH(i, jp, k) - TH(i, j, k)) / dmy[jp];
+                double dedz5 = (TH(i, j, k) - TH(i, j, km)) / dmz[k];
+                double dedz6 = (TH(i, j, kp) - TH(i, j, k)) / dmz[kp];
+                double viscous_e1 = 1.0 / dx * (dedx2 - dedx1);
+                double viscous_e2 = 1.0 / dy * (dedy4 - dedy3);
+                double viscous_e3 = 1.0 / dz * (dedz6 - dedz5);
+                double viscous = 0.5 * Ct * (viscous_e1 + viscous_e2 + viscous_e3);
+                double ebc_down = 0.5 * Ct / dy / dmy[j] * bc3[(size_t)i + NX * (size_t)k];
+                double ebc_up = 0.5 * Ct / dy / dmy[jp] * bc4[(size_t)i + NX * (size_t)k];
+                double ebc = (double)(1.0f - (float)jem) * ebc_down + (double)(1.0f - (float)jep) * ebc_up;
+                double eAPI = -0.5 * Ct / dx / dmx[ip];
+                double eAMI = -0.5 * Ct / dx / dmx[i];
+                double eACI = 0.5 * Ct / dx * (1.0 / dmx[ip] + 1.0 / dmx[i]);
+                double eAPK = -0.5 * Ct / dz / dmz[kp];
+                double eAMK = -0.5 * Ct / dz / dmz[k];
+                double eACK = 0.5 * Ct / dz * (1.0 / dmz[kp] + 1.0 / dmz[k]);
+                double eAPJ = -0.5 * Ct / dy * (1.0 / dmy[jp]) * (double)jep;
+                double eAMJ = -0.5 * Ct / dy * (1.0 / dmy[j]) * PLANT(402, 1.0, (double)jem);
+                double eACJ = 0.5 * Ct / dy * (1.0 / dmy[jp] + 1.0 / dmy[j]);
+                double eRHS = eAPK * TH(i, j, kp) + eACK * TH(i, j, k) + eAMK * TH(i, j, km)
+                            + eAPJ * TH(i, jp, k) + eACJ * TH(i, j, k) + eAMJ * TH(i, jm, k)
+                            + eAPI * TH(ip, j, k) + eACI * TH(i, j, k) + eAMI * TH(im, j, k);
+                RH(i, j, k) = TH(i, j, k) / dt + viscous + ebc - (TH(i, j, k) / dt + eRHS);
+            }
+        }
+    }
+    const int nxm = nx_sub - 1, nym = ny_sub - 1, nzm = nz_sub - 1;
+    size_t big = (size_t)(nxm > nym ? nxm : nym) * (size_t)(nym > nzm ? nym : nzm);
+    if ((size_t)nxm * nzm > big) big = (size_t)nxm * nzm;
+    double *ap = calloc(big, sizeof(double)), *am = calloc(big, sizeof(double));
+    double *ac = calloc(big, sizeof(double)), *ad = calloc(big, sizeof(double));
+    /* z: per plane j, systems i, rows k, cyclic (:122-149) */
+    for (int j = 1; j <= nym; j++) {
+        for (int k = 1; k <= nzm; k++) {
+            int kp = k + 1;
+            for (int i = 1; i <= nxm; i++) {
+                size_t q = (size_t)(i - 1) + (size_t)nxm * (size_t)(k - 1);
+                ap[q] = -0.5 * Ct / dz / dmz[kp] * dt;
+                am[q] = -0.5 * Ct / dz / dmz[k] * dt;
+                ac[q] = 0.5 * Ct / dz * (1.0 / dmz[kp] + 1.0 / dmz[k]) * dt + 1.0;
+                ad[q] = RH(i, j, k) * PLANT(401, 1.0, dt);
+            }
+        }
+        orc_ptdma_many_solve_cycle(am, ac, ap, ad, nxm, nzm);
+        for (int k = 1; k <= nzm; k++)
+            for (int i = 1; i <= nxm; i++) RH(i, j, k) = ad[(size_t)(i - 1) + (size_t)nxm * (size_t)(k - 1)];
+    }
+    /* y: per plane k, systems i, rows j (:158-186) */
+    for (int k = 1; k <= nzm; k++) {
+        for (int j = 1; j <= nym; j++) {
+            int jp = j + 1;
+            int jep = jpbc[j], jem = jmbc[j];
+            for (int i = 1; i <= nxm; i++) {
+                size_t q = (size_t)(i - 1) + (size_t)nxm * (size_t)(j - 1);
+                ap[q] = -0.5 * Ct / dy / dmy[jp] * (double)jep * dt;
+                am[q] = -0.5 * Ct / dy / dmy[j] * (double)jem * dt;
+                ac[q] = 0.5 * Ct / dy * (1.0 / dmy[jp] + 1.0 / dmy[j]) * dt + 1.0;
+                ad[q] = RH(i, j, k);
+            }
+        }
+        orc_ptdma_many_solve(am, ac, ap, ad, nxm, nym);
+        for (int j = 1; j <= nym; j++)
+            for (int i = 1; i <= nxm; i++) RH(i, j, k) = ad[(size_t)(i - 1) + (size_t)nxm * (size_t)(j - 1)];
+    }
+    /* x: per plane k, systems j, rows i, cyclic; theta += solution (:191-218) */
+    for (int k = 1; k <= nzm; k++) {
+        for (int j = 1; j <= nym; j++)
+            for (int i = 1; i <= nxm; i++) {
+                int ip = i + 1;
+                size_t q = (size_t)(j - 1) + (size_t)nym * (size_t)(i - 1);
+                ap[q] = -0.5 * Ct / dx / dmx[ip] * dt;
+                am[q] = -0.5 * Ct / dx / dmx[i] * dt;
+                ac[q] = 0.5 * Ct / dx * (1.0 / dmx[ip] + 1.0 / dmx[i]) * dt + 1.0;
+                ad[q] = RH(i, j, k) * PLANT(403, dt, 1.0);
+            }
+        orc_ptdma_many_solve_cycle(am, ac, ap, ad, nym, nxm);
+        for (int j = 1; j <= nym; j++)
+            for (int i = 1; i <= nxm; i++)
+                TH(i, j, k) = TH(i, j, k) + ad[(size_t)(j - 1) + (size_t)nym * (size_t)(i - 1)];
+    }
+    /* ghost cells: x and z periodic on one rank, y walls untouched */
+    for (int k = 0; k <= nz_sub; k++)
+        for (int j = 0; j <= ny_sub; j++) {
+            TH(0, j, k) = TH(nx_sub - 1, j, k);
+            TH(nx_sub, j, k) = TH(1, j, k);
+        }
+    for (int j = 0; j <= ny_sub; j++)
+        for (int i = 0; i <= nx_sub; i++) {
+            TH(i, j, 0) = TH(i, j, nz_sub - 1);
+            TH(i, j, nz_sub) = TH(i, j, 1);
+        }
+    free(rhs); free(ap); free(am); free(ac); free(ad);
+#undef TH
+#undef RH
+}

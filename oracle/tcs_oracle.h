@@ -118,6 +118,15 @@ void orc_tdma_cycl_many(double *a, double *b, double *c, double *d, int n1, int 
 void orc_tdma_single(double *a, double *b, double *c, double *d, int n1);
 void orc_tdma_cycl_single(double *a, double *b, double *c, double *d, int n1);
 
+/* one step of the PaScaL_TDMA heat-conduction example on one rank
+   (PaScaL_TDMA/examples/solve_theta_plan_many.f90:30-203 and the periodic
+   ghost update of examples/mpi_subdomain.f90:219-249): theta
+   (0:nx_sub,0:ny_sub,0:nz_sub) in place; wall tables (0:nx_sub,0:nz_sub);
+   metrics (0:n_sub); jmbc/jpbc (0:ny_sub) */
+void orc_heat_step(int nx_sub, int ny_sub, int nz_sub, double *theta, const double *bc3, const double *bc4,
+                   const double *dmx, const double *dmy, const double *dmz, const int *jmbc, const int *jpbc,
+                   double dx, double dy, double dz, double Ct, double dt);
+
 /* FFT building blocks of the Poisson solve (radix-2 Stockham, power-of-two n) */
 void orc_twiddles(int n, double *wr, double *wi);
 void orc_fft(int n, double *re, double *im, int sign);

@@ -116,3 +116,31 @@ def test_fortran_driver_matches_oracle(tmp_path):
         got = out[name]
         exp = o.f[key][1:-1, 1:-1, 1:-1]
         assert np.array_equal(got, exp), f"{name}: max diff {np.abs(got - exp).max():.3e}"
+
+
+HEAT = os.path.join(os.path.dirname(DRIVER), "examples", "ptdma_heat")
+
+
+@pytest.mark.parametrize("n,steps", [((16, 12, 8), 5), ((32, 9, 16), 3)])
+def test_heat_example_through_fortran_pascal_tdma(tmp_path, n, steps):
+    """The PaScaL_TDMA heat-conduction example pattern (plane-by-plane
+    PaScaL_TDMA_many_solve[_cycle] calls of
+    PaScaL_TDMA/examples/solve_theta_plan_many.f90:122-203) compiled against
+    the drop-in Fortran module PaScaL_TDMA, whose solves run on the GPU:
+    after `steps` steps the field equals the oracle's restatement of the
+    example, bit for bit (same operation order)."""
+    from oracle import oracle as O
+    nx, ny, nz = n
+    r = subprocess.run([HEAT, str(nx), str(ny), str(nz), str(steps)], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    H = O.HeatExample(nx, ny, nz)
+    th0 = np.fromfile(tmp_path / "theta0.bin", dtype=np.float64).reshape(H.shape)
+    got = np.fromfile(tmp_path / "theta.bin", dtype=np.float64).reshape(H.shape)
+    th = th0.copy()
+    for _ in range(steps):
+        H.step(th)
+    inner = (slice(1, -1),) * 3
+    assert not np.array_equal(got[inner], th0[inner])
+    assert np.array_equal(got[inner], th[inner]), np.abs(got[inner] - th[inner]).max()
