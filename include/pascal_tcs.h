@@ -238,6 +238,17 @@ int ptdma_many_solve_cycle_dev(double *A, const double *B, double *C, double *D,
 /* Same contract with host arrays (copied in and out; synchronous). */
 int ptdma_many_solve_host(double *A, double *B, double *C, double *D,
                           int32_t n_sys, int32_t n_row, int32_t cyclic);
+/* No counterpart in the reference: ptdma_many_solve_host without any
+   device allocation, for callers that cannot allocate (code under graph
+   capture, harnesses that forbid hipMalloc, or tight loops such as a
+   plane-by-plane solve, which the allocating entry would hipMalloc/hipFree
+   on every call).  The arrays are staged in a static device array of four
+   times 2^21 doubles (64 MiB, held by each GPU the library runs on until
+   the process ends): n_sys*n_row must be <= 2097152, else TCS_E_ARG.
+   Calls that overlap on one GPU (several host threads) race on that array.
+   Same kernels and results as ptdma_many_solve_host; synchronous. */
+int ptdma_many_solve_host_noalloc(double *A, double *B, double *C, double *D,
+                                  int32_t n_sys, int32_t n_row, int32_t cyclic);
 /* PaScaL_TDMA_single_solve / _single_solve_cycle(plan,A,B,C,D,n_row),
    PaScaL_TDMA/src/pascal_tdma.f90:301-382 and :393-475, on one rank: the
    per-rank modified Thomas of the many-system solve and the 2-row reduced

@@ -83,6 +83,7 @@ SIGNATURES = {
     "ptdma_many_solve_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _i32, _vp]),
     "ptdma_many_solve_cycle_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _i32, _vp]),
     "ptdma_many_solve_host": (ctypes.c_int, [_dp, _dp, _dp, _dp, _i32, _i32, _i32]),
+    "ptdma_many_solve_host_noalloc": (ctypes.c_int, [_dp, _dp, _dp, _dp, _i32, _i32, _i32]),
     "ptdma_axis_solve_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp]),
     "ptdma_single_solve_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _i32, _vp]),
     "ptdma_single_solve_host": (ctypes.c_int, [_dp, _dp, _dp, _dp, _i32, _i32]),

@@ -11,6 +11,8 @@
 // place, exactly as the reference does with its A, C, D arrays.
 #include "tcs_internal.h"
 
+#include <mutex>
+
 namespace tcs {
 
 template <bool CYC>
@@ -349,6 +351,80 @@ extern "C" int ptdma_many_solve_host(double *A, double *B, double *C, double *D,
     for (int q = 0; q < 4; q++)
         if (d[q]) (void)hipFree(d[q]);
     return rc;
+}
+
+// ---- host solve without device allocation ---------------------------------
+// ptdma_many_solve_host stages the four host arrays in device buffers it
+// allocates and frees on every call.  The variant below stages them in one
+// file-scope device array instead (each GPU the library runs on holds its
+// own copy until the process ends), so a caller that must not allocate --
+// or that calls the solve plane by plane in a loop -- pays only the copies.
+// Four arrays of at most NOALLOC_MAX doubles each (4 x 16 MiB).  Every call
+// on a GPU shares that GPU's array: calls that overlap on one GPU (from
+// several host threads) race.  The array needs no clearing: each call
+// overwrites the part it uses before any kernel reads it.
+constexpr size_t NOALLOC_MAX = (size_t)1 << 21;
+static __device__ __align__(256) double scratch[4 * NOALLOC_MAX];
+
+constexpr int NOALLOC_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[NOALLOC_DEVICES];  // device address of `scratch` per device
+
+// the current device's address of `scratch`, looked up once per device
+static int scratch_address(double **out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_DEVICES) {
+        set_error("ptdma_many_solve_host_noalloc: no usable current device");
+        return TCS_E_HIP;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("ptdma_many_solve_host_noalloc: hipGetSymbolAddress failed: ") +
+                      hipGetErrorString(e));
+            return TCS_E_HIP;
+        }
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return TCS_OK;
+}
+
+extern "C" int ptdma_many_solve_host_noalloc(double *A, double *B, double *C, double *D, int32_t n_sys,
+                                             int32_t n_row, int32_t cyclic) {
+    if (!A || !B || !C || !D || n_sys <= 0 || n_row < 2) {
+        set_error("ptdma_many_solve_host_noalloc: bad arguments");
+        return TCS_E_ARG;
+    }
+    const size_t n = (size_t)n_sys * (size_t)n_row;
+    if (n > NOALLOC_MAX) {
+        set_error("ptdma_many_solve_host_noalloc: n_sys*n_row = " + std::to_string(n) + " exceeds the maximum " +
+                  std::to_string(NOALLOC_MAX) + " of the static staging array (use ptdma_many_solve_host)");
+        return TCS_E_ARG;
+    }
+    double *base = nullptr;
+    TCS_TRY(scratch_address(&base));
+    const size_t bytes = n * sizeof(double);
+    double *d[4] = {base, base + NOALLOC_MAX, base + 2 * NOALLOC_MAX, base + 3 * NOALLOC_MAX};
+    double *h[4] = {A, B, C, D};
+    for (int q = 0; q < 4; q++)
+        if (hipMemcpy(d[q], h[q], bytes, hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("ptdma_many_solve_host_noalloc: H2D copy failed");
+            return TCS_E_HIP;
+        }
+    TCS_TRY(many_solve(d[0], d[1], d[2], d[3], n_sys, n_row, cyclic != 0, nullptr));
+    if (hipStreamSynchronize(nullptr) != hipSuccess) {
+        set_error("ptdma_many_solve_host_noalloc: kernel failed");
+        return TCS_E_HIP;
+    }
+    for (int q : {0, 2, 3})
+        if (hipMemcpy(h[q], d[q], bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+            set_error("ptdma_many_solve_host_noalloc: D2H copy failed");
+            return TCS_E_HIP;
+        }
+    return TCS_OK;
 }
 
 extern "C" int ptdma_single_solve_dev(double *A, const double *B, double *C, double *D, int32_t n_row,
