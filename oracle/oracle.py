@@ -35,10 +35,12 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-def lib(plants=False):
+def lib(plants=False, omp=False):
     """The oracle library; plants=True: the build whose planted-mistake
-    sites orc_set_plant switches on (tests of the closed-form pins)."""
-    fname = "liboracle_plants.so" if plants else "liboracle.so"
+    sites orc_set_plant switches on (tests of the closed-form pins);
+    omp=True: the build that splits the step over OpenMP threads (the CPU
+    baseline; bit-identical to the serial build)."""
+    fname = "liboracle_plants.so" if plants else ("liboracle_omp.so" if omp else "liboracle.so")
     if fname not in _LIBS:
         path = os.path.join(_HERE, fname)
         if not os.path.exists(path):
@@ -184,8 +186,8 @@ class Oracle:
     i.e. the Fortran (0:n1sub,0:n2sub,0:n3sub) array with i fastest.
     """
 
-    def __init__(self, n1m, n2m, n3m, dx1, dx2, dx3, dmx1, dmx2, dmx3, prm, plants=False):
-        L = lib(plants)
+    def __init__(self, n1m, n2m, n3m, dx1, dx2, dx3, dmx1, dmx2, dmx3, prm, plants=False, omp=False):
+        L = lib(plants, omp)
         self._L = L
         self.n1m, self.n2m, self.n3m = n1m, n2m, n3m
         self._keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (dx1, dx2, dx3, dmx1, dmx2, dmx3)]

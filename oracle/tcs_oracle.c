@@ -62,6 +62,19 @@ int orc_plants_built(void) { return 0; }
 
 static void cap_clear(orc *o);
 
+/* The oracle built with -fopenmp (liboracle_omp.so, the CPU baseline's
+   build) splits the step's loop nests and line solves over threads; every
+   cell and every line is computed exactly as in the serial build, and the
+   loop variables a later stage reads from an earlier one (the stale
+   indices) leave the parallel loops as the sequentially last iteration
+   leaves them (lastprivate), so both builds give the same bits. */
+#ifdef _OPENMP
+#include <omp.h>
+#define OMP(x) _Pragma(#x)
+#else
+#define OMP(x)
+#endif
+
 #define IX(i, j, k) ((size_t)(i) + (size_t)o->N1 * ((size_t)(j) + (size_t)o->N2 * (size_t)(k)))
 #define B2(i, k) ((size_t)(i) + (size_t)o->N1 * (size_t)(k))
 /* interior work array with Fortran extents (1:a,1:b,1:c) */
@@ -252,9 +265,9 @@ void orc_tdma_cycl_many(double *a, double *b, double *c, double *d, int n1, int 
 /* pascal_tdma.f90:243-266) and solved by tdma_many / tdma_cycl_many, and     */
 /* each rank updates its rows.  nprocs = 1 is the single-rank solve.          */
 /* ------------------------------------------------------------------------ */
-#define P2(x, i, j) x[(size_t)(i) + (size_t)n_sys * (size_t)(j)]
+#define P2(x, i, j) x[(size_t)(i) + (size_t)ld * (size_t)(j)]
 /* one rank's rows j0 .. j0+m-1: elimination of both off-diagonals */
-static void ptdma_rank_reduce(double *A, double *B, double *C, double *D, int n_sys, int j0, int m)
+static void ptdma_rank_reduce(double *A, double *B, double *C, double *D, int n_sys, size_t ld, int j0, int m)
 {
     for (int i = 0; i < n_sys; i++) {
         P2(A, i, j0) = P2(A, i, j0) / P2(B, i, j0);
@@ -343,7 +356,7 @@ static void reduced_pcr(double *ra, double *rb, double *rc, double *rd, int n_sy
 }
 
 static void ptdma_many_ranks(double *A, double *B, double *C, double *D,
-                             int n_sys, int n_row, int nprocs, int cyclic, int pcr)
+                             int n_sys, size_t ld, int n_row, int nprocs, int cyclic, int pcr)
 {
     const int nr = 2 * nprocs;
     int *j0 = (int *)malloc(sizeof(int) * (size_t)nprocs);
@@ -359,7 +372,7 @@ static void ptdma_many_ranks(double *A, double *B, double *C, double *D,
     double *rd = (double *)malloc(sizeof(double) * (size_t)n_sys * nr);
     for (int p = 0; p < nprocs; p++) {
         const int a = j0[p], b = j0[p] + mm[p] - 1;
-        ptdma_rank_reduce(A, B, C, D, n_sys, a, mm[p]);
+        ptdma_rank_reduce(A, B, C, D, n_sys, ld, a, mm[p]);
         for (int i = 0; i < n_sys; i++) {
             /* plan%A_rd(i,1:2) .. D_rd, placed at reduced rows 2p, 2p+1 */
             size_t q0 = (size_t)i + (size_t)n_sys * (size_t)(2 * p), q1 = q0 + (size_t)n_sys;
@@ -390,23 +403,23 @@ static void ptdma_many_ranks(double *A, double *B, double *C, double *D,
 void orc_ptdma_many_solve_ranks(double *A, double *B, double *C, double *D, int n_sys, int n_row,
                                 int nprocs, int cyclic)
 {
-    ptdma_many_ranks(A, B, C, D, n_sys, n_row, nprocs, cyclic, 0);
+    ptdma_many_ranks(A, B, C, D, n_sys, (size_t)n_sys, n_row, nprocs, cyclic, 0);
 }
 
 void orc_ptdma_many_solve_split(double *A, double *B, double *C, double *D, int n_sys, int n_row, int cyclic)
 {
     const int P = orc_tdma_parts(n_row);
-    ptdma_many_ranks(A, B, C, D, n_sys, n_row, P, cyclic, P > 1);
+    ptdma_many_ranks(A, B, C, D, n_sys, (size_t)n_sys, n_row, P, cyclic, P > 1);
 }
 
 void orc_ptdma_many_solve(double *A, double *B, double *C, double *D, int n_sys, int n_row)
 {
-    ptdma_many_ranks(A, B, C, D, n_sys, n_row, 1, 0, 0);
+    ptdma_many_ranks(A, B, C, D, n_sys, (size_t)n_sys, n_row, 1, 0, 0);
 }
 
 void orc_ptdma_many_solve_cycle(double *A, double *B, double *C, double *D, int n_sys, int n_row)
 {
-    ptdma_many_ranks(A, B, C, D, n_sys, n_row, 1, 1, 0);
+    ptdma_many_ranks(A, B, C, D, n_sys, (size_t)n_sys, n_row, 1, 1, 0);
 }
 
 /* Parts per line of the split solve: lines of 2..512 rows are split into
@@ -433,8 +446,23 @@ static void lines_solve(orc *o, double *A, double *B, double *C, double *D, int 
         r->a = malloc(nb); r->b = malloc(nb); r->c = malloc(nb); r->d = malloc(nb); r->x = malloc(nb);
         memcpy(r->a, A, nb); memcpy(r->b, B, nb); memcpy(r->c, C, nb); memcpy(r->d, D, nb);
     }
+#ifdef _OPENMP
+    if (n_sys >= 1024) {
+        /* independent systems: chunks of them to the threads */
+        const int P = o->tdma_split ? orc_tdma_parts(n_row) : 1;
+        const int nch = 8 * omp_get_max_threads();
+        const int w = (n_sys + nch - 1) / nch;
+        OMP(omp parallel for schedule(dynamic))
+        for (int q = 0; q < nch; q++) {
+            const int s0 = q * w, len = s0 + w <= n_sys ? w : n_sys - s0;
+            if (len > 0)
+                ptdma_many_ranks(A + s0, B + s0, C + s0, D + s0, len, (size_t)n_sys, n_row, P, cyclic,
+                                 o->tdma_split && P > 1);
+        }
+    } else
+#endif
     if (o->tdma_split) orc_ptdma_many_solve_split(A, B, C, D, n_sys, n_row, cyclic);
-    else ptdma_many_ranks(A, B, C, D, n_sys, n_row, 1, cyclic, 0);
+    else ptdma_many_ranks(A, B, C, D, n_sys, (size_t)n_sys, n_row, 1, cyclic, 0);
     if (r) memcpy(r->x, D, nb);
 }
 
@@ -523,6 +551,7 @@ void orc_thermal_coeffi(orc *o)
 {
     const double *p = o->prm;
     double *T = o->f[OF_T];
+    OMP(omp parallel for)
     for (int k = 0; k <= o->n3sub; k++)
         for (int j = 0; j <= o->n2sub; j++)
             for (int i = 0; i <= o->n1sub; i++) {
@@ -576,6 +605,7 @@ void orc_thermal_solver(orc *o, double dt)
     double *ap = (double *)calloc(n, sizeof(double));
 
     /* 1st stage: z (:236-361) */
+    OMP(omp parallel for private(i, j, im, ip, jep, jem, e1, e2, e3, e4, e5, e6, dedx1, dedx2, dedy3, dedy4, dedz5, dedz6, convect_e1, convect_e2, convect_e3, viscous_e1, viscous_e2, viscous_e3, kPP1, kPP2, kPP3, kPP4, kPP5, kPP6, dkPP1, dkPP2, dkPP3, dkPP4, dkPP5, dkPP6, ebc_up, ebc_down, eACK, eAPK, eAMK, eACI, eAPI, eAMI, eACJ, eAPJ, eAMJ, invRhoCpCt_half, dinvRhoCpCt_half) lastprivate(jm, jp, km, kp))
     for (k = 1; k <= n3msub; k++) {
         km = k - 1; kp = k + 1;
         for (j = 1; j <= n2msub; j++) {
@@ -686,12 +716,14 @@ void orc_thermal_solver(orc *o, double dt)
 
     /* transpose (i,j,k) -> (j,k,i) (:371-379) */
     double *RHSI = (double *)calloc(n, sizeof(double));
+    OMP(omp parallel for private(i))
     for (k = 1; k <= n3msub; k++)
         for (i = 1; i <= n1msub; i++)
             for (int jj = 1; jj <= n2msub; jj++)
                 RHSI[R3(n2msub, n3msub, jj, k, i)] = RHS[R3(n1msub, n2msub, i, jj, k)];
 
     /* 2nd stage: x (:388-429); jm,jp,km,kp keep their stage-1 end values */
+    OMP(omp parallel for private(j, i, ip, im, e1, e2, e3, e4, e5, e6, dedx1, dedx2, dedy3, dedy4, dedz5, dedz6, convect_e1, convect_e2, convect_e3, viscous_e1, viscous_e2, viscous_e3, kPP1, kPP2, kPP3, kPP4, kPP5, kPP6, dkPP1, dkPP2, dkPP3, dkPP4, dkPP5, dkPP6, ebc_up, ebc_down, eACK, eAPK, eAMK, eACI, eAPI, eAMI, eACJ, eAPJ, eAMJ, invRhoCpCt_half, dinvRhoCpCt_half))
     for (k = 1; k <= n3msub; k++)
         for (j = 1; j <= n2msub; j++)
             for (i = 1; i <= n1msub; i++) {
@@ -729,12 +761,14 @@ void orc_thermal_solver(orc *o, double dt)
     lines_solve(o, am, ac, ap, RHSI, n2msub * n3msub, n1msub, 1);
 
     /* transpose (j,k,i) -> (i,k,j) (:439-447) */
+    OMP(omp parallel for private(k, i))
     for (j = 1; j <= n2msub; j++)
         for (k = 1; k <= n3msub; k++)
             for (i = 1; i <= n1msub; i++)
                 RHS[R3(n1msub, n3msub, i, k, j)] = RHSI[R3(n2msub, n3msub, j, k, i)];
 
     /* 3rd stage: y (:456-507); km,kp keep their stage-1 end values */
+    OMP(omp parallel for private(j, i, jm, jp, jem, jep, ip, im, e1, e2, e3, e4, e5, e6, dedx1, dedx2, dedy3, dedy4, dedz5, dedz6, convect_e1, convect_e2, convect_e3, viscous_e1, viscous_e2, viscous_e3, kPP1, kPP2, kPP3, kPP4, kPP5, kPP6, dkPP1, dkPP2, dkPP3, dkPP4, dkPP5, dkPP6, ebc_up, ebc_down, eACK, eAPK, eAMK, eACI, eAPI, eAMI, eACJ, eAPJ, eAMJ, invRhoCpCt_half, dinvRhoCpCt_half))
     for (k = 1; k <= n3msub; k++)
         for (j = 1; j <= n2msub; j++) {
             jm = j - 1; jp = j + 1;
@@ -776,6 +810,7 @@ void orc_thermal_solver(orc *o, double dt)
         }
     lines_solve(o, am, ac, ap, RHS, n3msub * n1msub, n2msub, 0);
 
+    OMP(omp parallel for private(j, i))
     for (k = 1; k <= n3msub; k++)
         for (j = 1; j <= n2msub; j++)
             for (i = 1; i <= n1msub; i++)
@@ -794,6 +829,7 @@ void orc_momentum_coeffi(orc *o)
     const double *p = o->prm;
     const double *T = o->f[OF_T];
     double *Mu = o->f[OF_MU], *invRho = o->f[OF_INVRHO];
+    OMP(omp parallel for)
     for (int k = 0; k <= o->n3sub; k++)
         for (int j = 0; j <= o->n2sub; j++) {
             int i = o->n1sub; /* last iteration wins */
@@ -817,6 +853,7 @@ void orc_momentum_pseudoupdateUVW(orc *o)
 {
     double *U = o->f[OF_U], *V = o->f[OF_V], *W = o->f[OF_W];
     const double *dU = o->f[OF_DU], *dV = o->f[OF_DV], *dW = o->f[OF_DW];
+    OMP(omp parallel for)
     for (int k = 1; k <= o->n3msub; k++) {
         for (int j = 2; j <= o->n2msub; j++)
             for (int i = 1; i <= o->n1msub; i++) {
@@ -861,6 +898,7 @@ void orc_momentum_solvedU(orc *o, double dt)
     double *ac = (double *)calloc(n, sizeof(double));
     double *ap = (double *)calloc(n, sizeof(double));
 
+    OMP(omp parallel for private(i, j, im, ip, jum, jup, u1, u2, v3, v4, w5, w6, Tc, dudx1, dudx2, dudy3, dudy4, dudz5, dudz6, dvdx3, dvdx4, dwdx5, dwdx6, mua, mub, muc, mu3, mu4, mu5, mu6, invRhoc, invRhocCmu_half, viscous_u1, viscous_u2, viscous_u3, viscous_u12, viscous_u13, ubc_up, ubc_down, mACI, mAPI, mAMI, mACJ, mAPJ, mAMJ, mACK, mAPK, mAMK) lastprivate(jm, jp, km, kp))
     for (k = 1; k <= n3msub; k++) {
         km = k - 1; kp = k + 1;
         for (j = 1; j <= n2msub; j++) {
@@ -953,12 +991,14 @@ void orc_momentum_solvedU(orc *o, double dt)
     lines_solve(o, am, ac, ap, RHS, n1msub * n2msub, n3msub, 1);
 
     double *RHSI = (double *)calloc(n, sizeof(double));
+    OMP(omp parallel for private(i))
     for (k = 1; k <= n3msub; k++)
         for (int jj = 1; jj <= n2msub; jj++)
             for (i = o->i_indexS; i <= n1msub; i++)
                 RHSI[R3(n2msub, n3msub, jj, k, i)] = RHS[R3(n1msub, n2msub, i, jj, k)];
 
     /* 2nd stage x (:443-472) */
+    OMP(omp parallel for private(i, im, ip, jum, jup, u1, u2, v3, v4, w5, w6, Tc, dudx1, dudx2, dudy3, dudy4, dudz5, dudz6, dvdx3, dvdx4, dwdx5, dwdx6, mua, mub, muc, mu3, mu4, mu5, mu6, invRhoc, invRhocCmu_half, viscous_u1, viscous_u2, viscous_u3, viscous_u12, viscous_u13, ubc_up, ubc_down, mACI, mAPI, mAMI, mACJ, mAPJ, mAMJ, mACK, mAPK, mAMK))
     for (k = 1; k <= n3msub; k++)
         for (int jj = 1; jj <= n2msub; jj++)
             for (i = o->i_indexS; i <= n1msub; i++) {
@@ -980,12 +1020,14 @@ void orc_momentum_solvedU(orc *o, double dt)
             }
     lines_solve(o, am, ac, ap, RHSI, n2msub * n3msub, n1msub, 1);
 
+    OMP(omp parallel for private(i, k))
     for (int jj = 1; jj <= n2msub; jj++)
         for (i = o->i_indexS; i <= n1msub; i++)
             for (k = 1; k <= n3msub; k++)
                 RHS[R3(n1msub, n3msub, i, k, jj)] = RHSI[R3(n2msub, n3msub, jj, k, i)];
 
     /* 3rd stage y (:499-537) */
+    OMP(omp parallel for private(j, i, jm, jp, im, ip, jum, jup, u1, u2, v3, v4, w5, w6, Tc, dudx1, dudx2, dudy3, dudy4, dudz5, dudz6, dvdx3, dvdx4, dwdx5, dwdx6, mua, mub, muc, mu3, mu4, mu5, mu6, invRhoc, invRhocCmu_half, viscous_u1, viscous_u2, viscous_u3, viscous_u12, viscous_u13, ubc_up, ubc_down, mACI, mAPI, mAMI, mACJ, mAPJ, mAMJ, mACK, mAPK, mAMK))
     for (k = 1; k <= n3msub; k++)
         for (j = 1; j <= n2msub; j++) {
             jm = j - 1; jp = j + 1;
@@ -1014,6 +1056,7 @@ void orc_momentum_solvedU(orc *o, double dt)
         }
     lines_solve(o, am, ac, ap, RHS, n3msub * n1msub, n2msub, 0);
 
+    OMP(omp parallel for private(j, i))
     for (k = 1; k <= n3msub; k++)
         for (j = 1; j <= n2msub; j++)
             for (i = 1; i <= n1msub; i++)
@@ -1054,8 +1097,10 @@ void orc_momentum_solvedV(orc *o, double dt)
     /* rows j=1 are never assembled (j starts at j_indexS); the reference
        solves them with uninitialised coefficients and discards them.  Give
        them an identity row so the discarded solves stay finite. */
+    OMP(omp parallel for)
     for (size_t q = 0; q < n; q++) ac[q] = 1.0;
 
+    OMP(omp parallel for private(i, j, im, ip, jvp, jvm, u1, u2, v3, v4, w5, w6, dudy1, dudy2, dvdx1, dvdx2, dvdy3, dvdy4, dvdz5, dvdz6, dwdy5, dwdy6, ddu1, ddu2, ddudy1, ddudy2, mua, muc, mub, mu5, mu6, invRhoc, invRhocCmu_half, viscous_v1, viscous_v2, viscous_v3, viscous_v21, viscous_v23, vbc_up, vbc_down, mACI, mAPI, mAMI, mACJ, mAPJ, mAMJ, mACK, mAPK, mAMK) lastprivate(jm, jp, km, kp, mu1, mu2))
     for (k = 1; k <= n3msub; k++) {
         kp = k + 1; km = k - 1;
         for (j = j_indexS; j <= n2msub; j++) {
@@ -1147,13 +1192,16 @@ void orc_momentum_solvedV(orc *o, double dt)
     lines_solve(o, am, ac, ap, RHS, n1msub * n2msub, n3msub, 1);
 
     double *RHSI = (double *)calloc(n, sizeof(double));
+    OMP(omp parallel for private(i))
     for (k = 1; k <= n3msub; k++)
         for (int jj = 1; jj <= n2msub; jj++)
             for (i = 1; i <= n1msub; i++)
                 RHSI[R3(n2msub, n3msub, jj, k, i)] = RHS[R3(n1msub, n2msub, i, jj, k)];
 
+    OMP(omp parallel for)
     for (size_t q = 0; q < n; q++) { am[q] = 0.0; ac[q] = 1.0; ap[q] = 0.0; }
     /* 2nd stage x (:802-833): mu1, mu2 are stale from stage 1's last point */
+    OMP(omp parallel for private(j, i, jm, jp, im, ip, jvp, jvm, u1, u2, v3, v4, w5, w6, dudy1, dudy2, dvdx1, dvdx2, dvdy3, dvdy4, dvdz5, dvdz6, dwdy5, dwdy6, ddu1, ddu2, ddudy1, ddudy2, mua, muc, mub, mu5, mu6, invRhoc, invRhocCmu_half, viscous_v1, viscous_v2, viscous_v3, viscous_v21, viscous_v23, vbc_up, vbc_down, mACI, mAPI, mAMI, mACJ, mAPJ, mAMJ, mACK, mAPK, mAMK))
     for (k = 1; k <= n3msub; k++)
         for (j = j_indexS; j <= n2msub; j++) {
             jp = j + 1; jm = j - 1;
@@ -1188,12 +1236,14 @@ void orc_momentum_solvedV(orc *o, double dt)
     double *amj = (double *)calloc(nyv, sizeof(double));
     double *acj = (double *)calloc(nyv, sizeof(double));
     double *apj = (double *)calloc(nyv, sizeof(double));
+    OMP(omp parallel for private(j, i))
     for (k = 1; k <= n3msub; k++)
         for (j = j_indexS; j <= n2msub; j++)
             for (i = 1; i <= n1msub; i++)
                 RHSJ[R3(n1msub, n3msub, i, k, j - j_indexS + 1)] = RHSI[R3(n2msub, n3msub, j, k, i)];
 
     /* 3rd stage y (:860-896) */
+    OMP(omp parallel for private(j, i, jm, jp, im, ip, jvp, jvm, u1, u2, v3, v4, w5, w6, dudy1, dudy2, dvdx1, dvdx2, dvdy3, dvdy4, dvdz5, dvdz6, dwdy5, dwdy6, ddu1, ddu2, ddudy1, ddudy2, mua, muc, mub, mu5, mu6, invRhoc, invRhocCmu_half, viscous_v1, viscous_v2, viscous_v3, viscous_v21, viscous_v23, vbc_up, vbc_down, mACI, mAPI, mAMI, mACJ, mAPJ, mAMJ, mACK, mAPK, mAMK))
     for (k = 1; k <= n3msub; k++)
         for (j = j_indexS; j <= n2msub; j++) {
             jm = j - 1; jp = j + 1;
@@ -1242,6 +1292,7 @@ void orc_momentum_solvedV(orc *o, double dt)
         for (i = 0; i <= o->n1sub; i++) {
             dV[IX(i, 0, k)] = 0.0; dV[IX(i, 1, k)] = 0.0; dV[IX(i, o->n2sub, k)] = 0.0;
         }
+    OMP(omp parallel for private(j, i))
     for (k = 1; k <= n3msub; k++)
         for (j = j_indexS; j <= n2msub; j++)
             for (i = 1; i <= n1msub; i++)
@@ -1281,6 +1332,7 @@ void orc_momentum_solvedW(orc *o, double dt)
     double *ac = (double *)calloc(n, sizeof(double));
     double *ap = (double *)calloc(n, sizeof(double));
 
+    OMP(omp parallel for private(i, j, im, ip, jwp, jwm, u1, u2, v3, v4, w6, w5, dudz1, dudz2, dvdz3, dvdz4, dwdx1, dwdx2, dwdy3, dwdy4, dwdz5, dwdz6, ddu1, ddu2, ddudz1, ddudz2, ddv3, ddv4, ddvdz3, ddvdz4, mua, muc, mub, invRhoc, invRhocCmu_half, viscous_w1, viscous_w2, viscous_w3, viscous_w31, viscous_w32, zbc_up, zbc_down, mACI, mAPI, mAMI, mACJ, mAPJ, mAMJ, mACK, mAPK, mAMK) lastprivate(jm, jp, km, kp, mu1, mu2, mu3, mu4))
     for (k = o->k_indexS; k <= n3msub; k++) {
         kp = k + 1; km = k - 1;
         for (j = 1; j <= n2msub; j++) {
@@ -1383,12 +1435,14 @@ void orc_momentum_solvedW(orc *o, double dt)
     lines_solve(o, am, ac, ap, RHS, n1msub * n2msub, n3msub, 1);
 
     double *RHSI = (double *)calloc(n, sizeof(double));
+    OMP(omp parallel for private(i))
     for (k = o->k_indexS; k <= n3msub; k++)
         for (int jj = 1; jj <= n2msub; jj++)
             for (i = 1; i <= n1msub; i++)
                 RHSI[R3(n2msub, n3msub, jj, k, i)] = RHS[R3(n1msub, n2msub, i, jj, k)];
 
     /* 2nd stage x (:1156-1189): mu1, mu2 stale from stage 1 */
+    OMP(omp parallel for private(i, j, jm, jp, im, ip, jwp, jwm, u1, u2, v3, v4, w6, w5, dudz1, dudz2, dvdz3, dvdz4, dwdx1, dwdx2, dwdy3, dwdy4, dwdz5, dwdz6, ddu1, ddu2, ddudz1, ddudz2, ddv3, ddv4, ddvdz3, ddvdz4, mua, muc, mub, invRhoc, invRhocCmu_half, viscous_w1, viscous_w2, viscous_w3, viscous_w31, viscous_w32, zbc_up, zbc_down, mACI, mAPI, mAMI, mACJ, mAPJ, mAMJ, mACK, mAPK, mAMK) lastprivate(km, kp))
     for (k = o->k_indexS; k <= n3msub; k++) {
         kp = k + 1; km = k - 1;
         for (j = 1; j <= n2msub; j++) {
@@ -1419,12 +1473,14 @@ void orc_momentum_solvedW(orc *o, double dt)
     }
     lines_solve(o, am, ac, ap, RHSI, n2msub * n3msub, n1msub, 1);
 
+    OMP(omp parallel for private(j, i))
     for (k = 1; k <= n3msub; k++)
         for (j = 1; j <= n2msub; j++)
             for (i = 1; i <= n1msub; i++)
                 RHS[R3(n1msub, n3msub, i, k, j)] = RHSI[R3(n2msub, n3msub, j, k, i)];
 
     /* 3rd stage y (:1216-1246): km stale (=n3msub-1), mu3, mu4 stale */
+    OMP(omp parallel for private(j, i, jm, jp, im, ip, jwp, jwm, u1, u2, v3, v4, w6, w5, dudz1, dudz2, dvdz3, dvdz4, dwdx1, dwdx2, dwdy3, dwdy4, dwdz5, dwdz6, ddu1, ddu2, ddudz1, ddudz2, ddv3, ddv4, ddvdz3, ddvdz4, mua, muc, mub, invRhoc, invRhocCmu_half, viscous_w1, viscous_w2, viscous_w3, viscous_w31, viscous_w32, zbc_up, zbc_down, mACI, mAPI, mAMI, mACJ, mAPJ, mAMJ, mACK, mAPK, mAMK))
     for (k = o->k_indexS; k <= n3msub; k++)
         for (j = 1; j <= n2msub; j++) {
             jm = j - 1; jp = j + 1;
@@ -1448,6 +1504,7 @@ void orc_momentum_solvedW(orc *o, double dt)
         }
     lines_solve(o, am, ac, ap, RHS, n3msub * n1msub, n2msub, 0);
 
+    OMP(omp parallel for private(j, i))
     for (k = o->k_indexS; k <= n3msub; k++)
         for (j = 1; j <= n2msub; j++)
             for (i = 1; i <= n1msub; i++)
@@ -1463,6 +1520,7 @@ void orc_momentum_blockLdV(orc *o, double dt)
     const double *V = o->f[OF_V], *dW = o->f[OF_DW], *Mu = o->f[OF_MU], *invRho = o->f[OF_INVRHO];
     double *dV = o->f[OF_DV];
     const double *dx2 = o->dx2, *dx3 = o->dx3, *dmx2 = o->dmx2, *dmx3 = o->dmx3;
+    OMP(omp parallel for)
     for (int k = 1; k <= o->n3msub; k++) {
         int kp = k + 1, km = k - 1;
         for (int j = o->j_indexS; j <= o->n2msub; j++) {
@@ -1496,6 +1554,7 @@ void orc_momentum_blockLdU(orc *o, double dt)
     double *dU = o->f[OF_DU];
     const double *dx1 = o->dx1, *dx2 = o->dx2, *dx3 = o->dx3, *dmx1 = o->dmx1, *dmx2 = o->dmx2, *dmx3 = o->dmx3;
     const int *jC_BC = o->jC_BC;
+    OMP(omp parallel for)
     for (int k = 1; k <= o->n3msub; k++) {
         int km = k - 1, kp = k + 1;
         for (int j = 1; j <= o->n2msub; j++) {
@@ -1561,6 +1620,7 @@ void orc_pressure_RHS(orc *o, double dt, double *PRHS)
     const double *VBCup = o->bc[OB_VUP], *VBCbt = o->bc[OB_VBT];
     const double *dx1 = o->dx1, *dx2 = o->dx2, *dx3 = o->dx3, *dmx1 = o->dmx1, *dmx2 = o->dmx2, *dmx3 = o->dmx3;
     const int *jC_BC = o->jC_BC;
+    OMP(omp parallel for)
     for (int k = 1; k <= o->n3msub; k++) {
         int kp = k + 1, km = k - 1;
         for (int j = 1; j <= o->n2msub; j++) {
@@ -1670,8 +1730,11 @@ void orc_pressure_Poisson_FFT2(orc *o, const double *prhs)
     /* spectrum after x r2c: (1:h1p,1:n2m,1:n3m) */
     double *Sr = (double *)calloc(ns, sizeof(double));
     double *Si = (double *)calloc(ns, sizeof(double));
-    double *lr = (double *)malloc(sizeof(double) * (size_t)(n1m > n3m ? n1m : n3m));
-    double *li = (double *)malloc(sizeof(double) * (size_t)(n1m > n3m ? n1m : n3m));
+    const size_t nl = (size_t)(n1m > n3m ? n1m : n3m);  /* one line's buffer (per thread) */
+    OMP(omp parallel)
+    {
+    double *lr = (double *)malloc(sizeof(double) * nl), *li = (double *)malloc(sizeof(double) * nl);
+    OMP(omp for)
     for (int k = 1; k <= n3m; k++)
         for (int j = 1; j <= n2m; j++) {
             for (int i = 1; i <= n1m; i++) { lr[i - 1] = prhs[R3(n1m, n2m, i, j, k)]; li[i - 1] = 0.0; }
@@ -1681,9 +1744,15 @@ void orc_pressure_Poisson_FFT2(orc *o, const double *prhs)
                 Si[R3(h1p, n2m, kx + 1, j, k)] = li[kx];
             }
         }
+    free(lr); free(li);
+    }
     /* z c2c forward into (1:n3m,1:h1p,1:n2m) */
     double *Kr = (double *)calloc(ns, sizeof(double));
     double *Ki = (double *)calloc(ns, sizeof(double));
+    OMP(omp parallel)
+    {
+    double *lr = (double *)malloc(sizeof(double) * nl), *li = (double *)malloc(sizeof(double) * nl);
+    OMP(omp for)
     for (int j = 1; j <= n2m; j++)
         for (int kx = 1; kx <= h1p; kx++) {
             for (int k = 1; k <= n3m; k++) { lr[k - 1] = Sr[R3(h1p, n2m, kx, j, k)]; li[k - 1] = Si[R3(h1p, n2m, kx, j, k)]; }
@@ -1693,9 +1762,12 @@ void orc_pressure_Poisson_FFT2(orc *o, const double *prhs)
                 Ki[R3(n3m, h1p, kz, kx, j)] = li[kz - 1];
             }
         }
+    free(lr); free(li);
+    }
     /* tridiagonal systems in y (:599-640) */
     double *Am_r = (double *)malloc(ns * sizeof(double)), *Ac_r = (double *)malloc(ns * sizeof(double)), *Ap_r = (double *)malloc(ns * sizeof(double));
     double *Am_c = (double *)malloc(ns * sizeof(double)), *Ac_c = (double *)malloc(ns * sizeof(double)), *Ap_c = (double *)malloc(ns * sizeof(double));
+    OMP(omp parallel for)
     for (int j = 1; j <= n2m; j++) {
         int jp = j + 1;
         double fft_amj = 1.0 / dx2[j] / dmx2[j];
@@ -1718,6 +1790,10 @@ void orc_pressure_Poisson_FFT2(orc *o, const double *prhs)
         Ki[R3(n3m, h1p, h3p, 1, j)] = 0.0;
     }
     /* z c2c backward into (1:h1p,1:n2m,1:n3m) */
+    OMP(omp parallel)
+    {
+    double *lr = (double *)malloc(sizeof(double) * nl), *li = (double *)malloc(sizeof(double) * nl);
+    OMP(omp for)
     for (int j = 1; j <= n2m; j++)
         for (int kx = 1; kx <= h1p; kx++) {
             for (int kz = 1; kz <= n3m; kz++) { lr[kz - 1] = Kr[R3(n3m, h1p, kz, kx, j)]; li[kz - 1] = Ki[R3(n3m, h1p, kz, kx, j)]; }
@@ -1727,10 +1803,16 @@ void orc_pressure_Poisson_FFT2(orc *o, const double *prhs)
                 Si[R3(h1p, n2m, kx, j, k)] = li[k - 1];
             }
         }
+    free(lr); free(li);
+    }
     /* x c2r, scale, tree sums */
     double *tmp = (double *)malloc(sizeof(double) * (size_t)n1m * n2m * n3m);
-    double *linesum = (double *)malloc(sizeof(double) * (size_t)n2m);
     double *planesum = (double *)malloc(sizeof(double) * (size_t)n3m);
+    OMP(omp parallel)
+    {
+    double *lr = (double *)malloc(sizeof(double) * (size_t)n1m), *li = (double *)malloc(sizeof(double) * (size_t)n1m);
+    double *linesum = (double *)malloc(sizeof(double) * (size_t)n2m);
+    OMP(omp for)
     for (int k = 1; k <= n3m; k++) {
         for (int j = 1; j <= n2m; j++) {
             for (int kx = 0; kx < h1p; kx++) {
@@ -1749,8 +1831,11 @@ void orc_pressure_Poisson_FFT2(orc *o, const double *prhs)
         }
         planesum[k - 1] = orc_tree_sum(linesum, n2m);
     }
+    free(lr); free(li); free(linesum);
+    }
     double AVER = orc_tree_sum(planesum, n3m);
     AVER = AVER / (double)n1m / (double)n2m / (double)n3m;
+    OMP(omp parallel for)
     for (int k = 1; k <= n3m; k++)
         for (int j = 1; j <= n2m; j++)
             for (int i = 1; i <= n1m; i++)
@@ -1771,9 +1856,9 @@ void orc_pressure_Poisson_FFT2(orc *o, const double *prhs)
             for (int i = 1; i <= o->n1msub; i++)
                 dP[IX(i, n2sub, k)] = Pbc_a * dP[IX(i, n2msub, k)] - Pbc_b * dP[IX(i, n2sub - 2, k)];
     }
-    free(Sr); free(Si); free(Kr); free(Ki); free(lr); free(li);
+    free(Sr); free(Si); free(Kr); free(Ki);
     free(Am_r); free(Ac_r); free(Ap_r); free(Am_c); free(Ac_c); free(Ap_c);
-    free(tmp); free(linesum); free(planesum);
+    free(tmp); free(planesum);
 }
 
 /* :756-874 */
@@ -1861,6 +1946,7 @@ double orc_post_div(orc *o)
 {
     const double *U = o->f[OF_U], *V = o->f[OF_V], *W = o->f[OF_W];
     double mx = -INFINITY;
+    OMP(omp parallel for reduction(max : mx))
     for (int k = 1; k <= o->n3sub - 1; k++)
         for (int j = 1; j <= o->n2sub - 1; j++)
             for (int i = 1; i <= o->n1sub - 1; i++) {
@@ -1877,6 +1963,7 @@ double orc_post_cfl(orc *o, double dt, double *newdt)
 {
     const double *U = o->f[OF_U], *V = o->f[OF_V], *W = o->f[OF_W];
     double mx = -INFINITY;
+    OMP(omp parallel for reduction(max : mx))
     for (int k = 1; k <= o->n3sub - 1; k++)
         for (int j = 1; j <= o->n2sub - 1; j++)
             for (int i = 1; i <= o->n1sub - 1; i++) {

@@ -25,9 +25,10 @@ def mesh(cfg):
     return out
 
 
-def make_oracle(cfg, tdma_split=0, plants=False):
+def make_oracle(cfg, tdma_split=0, plants=False, omp=False):
     (x1, dx1, dmx1), (x2, dx2, dmx2), (x3, dx3, dmx3) = mesh(cfg)
-    o = O.Oracle(cfg.n1m, cfg.n2m, cfg.n3m, dx1, dx2, dx3, dmx1, dmx2, dmx3, cfg.constants(), plants=plants)
+    o = O.Oracle(cfg.n1m, cfg.n2m, cfg.n3m, dx1, dx2, dx3, dmx1, dmx2, dmx3, cfg.constants(), plants=plants,
+                 omp=omp)
     o.set_tdma_split(tdma_split)
     o.pressure_wave_number()
     o.x2 = x2
