@@ -30,10 +30,15 @@ namespace TCS_FORM {
 
 #include "metric.h"
 
-constexpr int FFT_MAXB = 8;  // butterflies per thread per stage
+// butterflies per thread per stage: 8 for the z transforms and the inverse
+// x transform, 4 for the forward x transform (one radix-8 group a thread per
+// pass at 512 points: fewer registers while its right-hand side's stencil
+// loads are in flight, 2.42 -> 1.85 ms at 512^3; the other transforms lost
+// 0.05-0.08 ms each with 4)
+constexpr int FFT_MAXB = 8, FFT_MAXB_FWD = 4;
 
 // In-place radix-2 Stockham on nl lines (re/im at [l*n + t]) in LDS by NT
-// threads, n a power of two, nl*n <= 2*FFT_MAXB*NT.  sign<0 forward
+// threads, n a power of two, nl*n <= 2*MB*NT (MB butterflies per thread).  sign<0 forward
 // (exp(-i...)), sign>0 backward.  Caller synchronises before.
 //
 // Stage s (Ns = 2^s) combines x[jb] and x[jb + n/2] with the twiddle
@@ -50,11 +55,11 @@ __device__ __forceinline__ int fpad(int x) { return x + (x >> 5); }
 // LDS values that hold `count` padded transform values
 __host__ __device__ constexpr int fft_lds_len(int count) { return count + (count >> 5); }
 
-template <int SIGN, int R, int NT>
+template <int SIGN, int R, int NT, int MB>
 __device__ __forceinline__ void fft_pass(double *re, double *im, int n, int nl, int s,
                                          const double2 *__restrict__ tw) {
     constexpr int M = 1 << R;
-    constexpr int GMAX = 2 * FFT_MAXB / M;  // groups per thread
+    constexpr int GMAX = 2 * MB / M;  // groups per thread
     const int G = n >> R;
     const int total = G * nl;
     const int Ns = 1 << s;
@@ -120,14 +125,14 @@ __device__ __forceinline__ void fft_pass(double *re, double *im, int n, int nl, 
     __syncthreads();
 }
 
-template <int SIGN, int NT>
+template <int SIGN, int NT, int MB = FFT_MAXB>
 __device__ __forceinline__ void lds_fft(double *re, double *im, int n, int nl, const double2 *__restrict__ tw) {
     int L = 0;
     while ((1 << L) < n) L++;
     int s = 0;
-    for (; s + 3 <= L; s += 3) fft_pass<SIGN, 3, NT>(re, im, n, nl, s, tw);
-    if (L - s == 2) fft_pass<SIGN, 2, NT>(re, im, n, nl, s, tw);
-    else if (L - s == 1) fft_pass<SIGN, 1, NT>(re, im, n, nl, s, tw);
+    for (; s + 3 <= L; s += 3) fft_pass<SIGN, 3, NT, MB>(re, im, n, nl, s, tw);
+    if (L - s == 2) fft_pass<SIGN, 2, NT, MB>(re, im, n, nl, s, tw);
+    else if (L - s == 1) fft_pass<SIGN, 1, NT, MB>(re, im, n, nl, s, tw);
 }
 
 // PRHS (:182-229) into the interior layout
@@ -226,9 +231,11 @@ k_fft_x_fwd(Grid g, int h1p, int TX, Src src, const double2 *__restrict__ tw, do
     const int k0 = kb * TX;  // 0-based
     const int tid = threadIdx.x;
     if (TX == 4 && n == 2 * NT) {
-        // the 512-point shape, unrolled: a thread's two x positions over the
-        // four k-lines, so the k-lines share their loads
-#pragma unroll
+        // the 512-point shape: a thread's two x positions one after the
+        // other, each over the four k-lines, so the k-lines share their
+        // loads (both positions at once hold more registers: 145 against
+        // 121, 2.42 against 1.85 ms with the transform's 4 butterflies)
+#pragma unroll 1
         for (int h = 0; h < 2; h++) {
 #pragma unroll
             for (int l = 0; l < 4; l++) {
@@ -245,7 +252,7 @@ k_fft_x_fwd(Grid g, int h1p, int TX, Src src, const double2 *__restrict__ tw, do
         }
     }
     __syncthreads();
-    lds_fft<-1, NT>(re, im, n, TX, tw);
+    lds_fft<-1, NT, FFT_MAXB_FWD>(re, im, n, TX, tw);
     const size_t rowbase = (size_t)g.n3m * h1p * (j - 1);
     for (int e = tid; e < TX * h1p; e += NT) {
         const int kx = e / TX, l = e - kx * TX;
@@ -648,9 +655,9 @@ __global__ void __launch_bounds__(256) k_dphat_update(DPhatUpdate cell, double *
 // over TX k-lines of n1m points per workgroup, at most 4 (more leaves too
 // few workgroups per CU), within one pass's butterflies and 64 KiB of LDS
 constexpr int FFT_X_THREADS = 256;
-static int fft_tile(int n1m, int n3m) {
+static int fft_tile(int n1m, int n3m, int mb) {
     int tx = 1;
-    while (tx < 4 && tx * 2 <= n3m && (tx * 2) * (n1m / 2) <= FFT_MAXB * FFT_X_THREADS &&
+    while (tx < 4 && tx * 2 <= n3m && (tx * 2) * (n1m / 2) <= mb * FFT_X_THREADS &&
            (size_t)fft_lds_len(tx * 2 * n1m) * 16 <= 65536)
         tx *= 2;
     return tx;
@@ -659,7 +666,7 @@ static int fft_tile(int n1m, int n3m) {
 // r2c along x of the interior-layout values src(i, j, k) into the spectrum
 template <class Src>
 static int fft_x_fwd_src_launch(tcs_ctx *c, const char *label, const Src &src, double2 *spec) {
-    const int TX = fft_tile(c->n1m, c->n3m);
+    const int TX = fft_tile(c->n1m, c->n3m, FFT_MAXB_FWD);
     const size_t lds = (size_t)fft_lds_len(TX * c->n1m) * 2 * sizeof(double);
     TCS_LAUNCH(c->stream, label, (k_fft_x_fwd<FFT_X_THREADS, Src>), dim3(c->n3m / TX, c->n2m), dim3(FFT_X_THREADS),
                lds, c->grid(), c->h1p, TX, src, (const double2 *)c->tw1, spec);
@@ -671,7 +678,7 @@ static int fft_x_fwd_launch(tcs_ctx *c, const char *label, const double *in, dou
 
 // c2r along x into the full-layout field `out`, with the per-line sums
 static int fft_x_inv_launch(tcs_ctx *c, const char *label, const double2 *spec, double *out) {
-    const int TX = fft_tile(c->n1m, c->n3m);
+    const int TX = fft_tile(c->n1m, c->n3m, FFT_MAXB);
     const size_t lds = (size_t)fft_lds_len(TX * c->n1m) * 2 * sizeof(double);
     TCS_LAUNCH(c->stream, label, k_fft_x_inv<FFT_X_THREADS>, dim3(c->n3m / TX, c->n2m), dim3(FFT_X_THREADS), lds,
                c->grid(), c->h1p, TX, spec, (const double2 *)c->tw1, out, c->linesum);
