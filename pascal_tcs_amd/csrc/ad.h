@@ -54,15 +54,26 @@ struct RawW {
     __device__ __forceinline__ const double *operator()(int, const double *p) const { return p; }
 };
 
-// FP32 arithmetic on the FP64 fields (the mixed form's rows): each load
-// rounded to float
+// The mixed form's rows: FP32 arithmetic on the FP32 twins of the arrays
+// they read.  In that form every array a stage row or block correction reads
+// (state, increments, (j, k) tables, wall tables, stale scalars) lives in
+// one FP64 arena and has an FP32 twin at the same element offset of an FP32
+// arena, so the twin of p is f + (p - d).
 struct F32Ptr {
-    const double *p;
-    __device__ __forceinline__ float operator[](size_t i) const { return (float)p[i]; }
+    const float *p;
+    __device__ __forceinline__ float operator[](size_t i) const { return p[i]; }
 };
-struct F32W {
-    __device__ __forceinline__ F32Ptr operator()(int, const double *p) const { return F32Ptr{p}; }
+struct TwinW {
+    const double *d;  // the FP64 arena
+    const float *f;   // its FP32 twin
+    __device__ __forceinline__ F32Ptr operator()(int, const double *p) const { return F32Ptr{f + (p - d)}; }
 };
+inline TwinW twin_w(const tcs_ctx *c) { return TwinW{c->a64, c->a32}; }
+// the FP32 twin of an array of the mixed form's arena (null outside it)
+inline float *twin(const tcs_ctx *c, const double *p) {
+    if (!c->a64 || p < c->a64 || p >= c->a64 + c->an) return nullptr;
+    return c->a32 + (p - c->a64);
+}
 
 // the cell's input set: (field, index) of every load, without repeats
 constexpr int AD_MAXIN = 128;

@@ -301,7 +301,7 @@ int stage_adj(tcs_ctx *c, AdjWork *w, const Row &row, int axis, int jlo, bool cy
     const Grid g = c->grid();
     const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
     double *am = c->am, *ac = c->ac, *ap = c->ap;
-    TCS_TRY(rows3d_launch<HAS_D>(c->stream, "adj_rows", row, jlo, am, ac, ap, w->rint));
+    TCS_TRY(rows3d_launch<HAS_D>(c, "adj_rows", row, jlo, am, ac, ap, w->rint));
     TCS_LAUNCH(c->stream, "adj_coef_T", k_coef_T, grd, blk, 0, g, axis, jlo, (int)cyc, am, ap, w->aT, w->cT);
     {
         const LineGeom L = geom(c, axis, jlo);
@@ -380,13 +380,13 @@ static int copy(tcs_ctx *c, double *dst, const double *src, size_t n) {
 template <class RZ, class RX, class RY>
 static int adi_forward(tcs_ctx *c, const RZ &rz, const RX &rx, const RY &ry, int jlo, double *const (&X)[3]) {
     double *am = c->am, *ac = c->ac, *ap = c->ap, *rhs = c->rhs;
-    TCS_TRY(rows3d_launch<true>(c->stream, "adj_fwd_rows", rz, jlo, am, ac, ap, rhs));
+    TCS_TRY(rows3d_launch<true>(c, "adj_fwd_rows", rz, jlo, am, ac, ap, rhs));
     TCS_TRY(sweep(c, am, ac, ap, rhs, 2, jlo, true));
     TCS_TRY(copy(c, X[0], rhs, c->nint));
-    TCS_TRY(rows3d_launch<false>(c->stream, "adj_fwd_rows", rx, jlo, am, ac, ap, rhs));
+    TCS_TRY(rows3d_launch<false>(c, "adj_fwd_rows", rx, jlo, am, ac, ap, rhs));
     TCS_TRY(sweep(c, am, ac, ap, rhs, 0, jlo, true));
     TCS_TRY(copy(c, X[1], rhs, c->nint));
-    TCS_TRY(rows3d_launch<false>(c->stream, "adj_fwd_rows", ry, jlo, am, ac, ap, rhs));
+    TCS_TRY(rows3d_launch<false>(c, "adj_fwd_rows", ry, jlo, am, ac, ap, rhs));
     TCS_TRY(sweep(c, am, ac, ap, rhs, 1, jlo, false));
     TCS_TRY(copy(c, X[2], rhs, c->nint));
     return TCS_OK;

@@ -212,7 +212,6 @@ struct DUZ {
     double dt;
     const double *U, *V, *W, *P, *T, *Mu, *iR, *UBCup, *UBCbt, *VBCup, *VBCbt;
     static constexpr int NF = 11;  // field inputs, in the order below
-    static constexpr bool F32 = false;  // FP32 rows in the mixed form
     static constexpr bool FULL = true;  // unconditional row loop in the FP64 z solve
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
@@ -250,7 +249,6 @@ struct DUX {
     double dt;
     const double *U, *Mu, *iR;
     static constexpr int NF = 3;  // field inputs, in the order below
-    static constexpr bool F32 = false;  // FP32 rows in the mixed form
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         dU_x_row<S>(g, p, dt, w(0, U), w(1, Mu), w(2, iR), i, j, k, a, b, c);
@@ -292,7 +290,6 @@ struct DUY {
     double dt;
     const double *V, *Mu, *iR;
     static constexpr int NF = 3;  // field inputs, in the order below
-    static constexpr bool F32 = false;  // FP32 rows in the mixed form
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         dU_y_row<S>(g, p, dt, w(0, V), w(1, Mu), w(2, iR), i, j, k, a, b, c);
@@ -305,13 +302,14 @@ struct DUY {
 
 // interior copy: F(i,j,k) = RHS for j in [j0, n2m]
 // dV(0:n1sub,0:1,0:n3sub) = 0 and dV(:,n2sub,:) = 0 (:907-909)
-__global__ void k_dV_zero_rows(Grid g, double *__restrict__ dV) {
+template <typename E>
+__global__ void k_dV_zero_rows(Grid g, E *__restrict__ dV) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
     if (i >= g.N1) return;
-    dV[g.ix(i, 0, k)] = 0.0;
-    dV[g.ix(i, 1, k)] = 0.0;
-    dV[g.ix(i, g.n2m + 1, k)] = 0.0;
+    dV[g.ix(i, 0, k)] = E(0);
+    dV[g.ix(i, 1, k)] = E(0);
+    dV[g.ix(i, g.n2m + 1, k)] = E(0);
 }
 
 // ---------------------------------------------------------------- dV -------
@@ -404,12 +402,15 @@ __device__ __forceinline__ void dV_z_row(const Grid &g, const Phys &p, double dt
     rhsout = rhs * sc<S>(dt);
 }
 struct DVZ {
+    // FP32 lines of the mixed form at four waves per SIMD (3.09 -> 2.97 ms at
+    // 512^3)
+    static constexpr int W32 = 4;
+    static constexpr bool FULL32 = false;
     Grid g;
     Phys p;
     double dt;
     const double *U, *V, *W, *P, *dU, *Mu, *iR, *VBCup, *VBCbt;
     static constexpr int NF = 9;  // field inputs, in the order below
-    static constexpr bool F32 = false;  // FP32 rows in the mixed form
     static constexpr bool FULL = false;  // unconditional row loop in the FP64 z solve
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
@@ -445,7 +446,6 @@ struct DVX {
     double dt;
     const double *U, *iR, *stale;
     static constexpr int NF = 3;  // field inputs, in the order below
-    static constexpr bool F32 = false;  // FP32 rows in the mixed form
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         dV_x_row<S>(g, p, dt, w(0, U), w(1, iR), w(2, stale), i, j, k, a, b, c);
@@ -485,7 +485,6 @@ struct DVY {
     double dt;
     const double *V, *Mu, *iR;
     static constexpr int NF = 3;  // field inputs, in the order below
-    static constexpr bool F32 = false;  // FP32 rows in the mixed form
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         dV_y_row<S>(g, p, dt, w(0, V), w(1, Mu), w(2, iR), i, j, k, a, b, c);
@@ -602,7 +601,6 @@ struct DWZ {
     double dt;
     const double *U, *V, *W, *P, *dU, *dV, *Mu, *iR, *VBCup, *VBCbt, *WBCup, *WBCbt;
     static constexpr int NF = 12;  // field inputs, in the order below
-    static constexpr bool F32 = false;  // FP32 rows in the mixed form
     static constexpr bool FULL = false;  // unconditional row loop in the FP64 z solve
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
@@ -638,7 +636,6 @@ struct DWX {
     double dt;
     const double *U, *iR, *stale;
     static constexpr int NF = 3;  // field inputs, in the order below
-    static constexpr bool F32 = false;  // FP32 rows in the mixed form
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         dW_x_row<S>(g, p, dt, w(0, U), w(1, iR), w(2, stale), i, j, k, a, b, c);
@@ -677,7 +674,6 @@ struct DWY {
     double dt;
     const double *V, *iR, *stale;
     static constexpr int NF = 3;  // field inputs, in the order below
-    static constexpr bool F32 = false;  // FP32 rows in the mixed form
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         dW_y_row<S>(g, p, dt, w(0, V), w(1, iR), w(2, stale), i, j, k, a, b, c);
@@ -723,14 +719,17 @@ struct BlockLdV {
     }
 };
 
-__global__ void __launch_bounds__(256) k_blockLdV(BlockLdV cell, double *dV) {
+// E: the increments' type and the arithmetic (double with the fields
+// themselves; float with their FP32 twins in the mixed form)
+template <typename E, class Acc>
+__global__ void __launch_bounds__(256) k_blockLdV(BlockLdV cell, Acc acc, E *dV) {
     const Grid &g = cell.g;
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
     if (i > g.n1m || j > g.n2m || j < 2) return;
-    double o, x;
-    cell.eval<double>(RawW(), i, j, k, o, x, x, x);
+    E o, x;
+    cell.eval<E>(acc, i, j, k, o, x, x, x);
     dV[g.ix(i, j, k)] = o;
 }
 
@@ -787,7 +786,8 @@ constexpr int LDV_KB = 1, LDU_KB = 4, LDV_IB = 1, LDU_IB = 2;
 
 // blockLdV with the pseudo-update of V: dV(j >= 2) = the new dV and Vn =
 // V + dV; at j = 1 Vn = V + dV * 0 (:178-208)
-__global__ void __launch_bounds__(256) k_blockLdV_pseudo(BlockLdV cell, double *dV, double *__restrict__ Vn) {
+template <typename E, class Acc>
+__global__ void __launch_bounds__(256) k_blockLdV_pseudo(BlockLdV cell, Acc acc, E *dV, double *__restrict__ Vn) {
     const Grid &g = cell.g;
     const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * LDV_IB + 1;  // n1m is a multiple of LDV_IB
     const int j = wave_row_j();
@@ -795,18 +795,19 @@ __global__ void __launch_bounds__(256) k_blockLdV_pseudo(BlockLdV cell, double *
     // loads of every cell before the stores (dV is updated in place, each
     // cell reading only its own point of it)
     constexpr int NC = LDV_IB * LDV_KB;
-    double o[NC], vn[NC];
+    E o[NC];
+    double vn[NC];
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         const int i = i0 + c % LDV_IB, k = blockIdx.z * LDV_KB + c / LDV_IB + 1;
         const size_t q = g.ix(i, j, k);
         if (j < 2) {
-            o[c] = 0.0;  // not stored
-            vn[c] = cell.V[q] + dV[q] * 0.0;
+            o[c] = E(0);  // not stored
+            vn[c] = cell.V[q] + (double)dV[q] * 0.0;
         } else {
-            double x;
-            cell.eval<double>(RawW(), i, j, k, o[c], x, x, x);
-            vn[c] = cell.V[q] + o[c];
+            E x;
+            cell.eval<E>(acc, i, j, k, o[c], x, x, x);
+            vn[c] = cell.V[q] + (double)o[c];
         }
     }
 #pragma unroll
@@ -817,39 +818,60 @@ __global__ void __launch_bounds__(256) k_blockLdV_pseudo(BlockLdV cell, double *
     }
 }
 
-__global__ void __launch_bounds__(256) k_blockLdU(BlockLdU cell, double *dU) {
+template <typename E, class Acc>
+__global__ void __launch_bounds__(256) k_blockLdU(BlockLdU cell, Acc acc, E *dU) {
     const Grid &g = cell.g;
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
     if (i > g.n1m || j > g.n2m) return;
-    double o, x;
-    cell.eval<double>(RawW(), i, j, k, o, x, x, x);
+    E o, x;
+    cell.eval<E>(acc, i, j, k, o, x, x, x);
     dU[g.ix(i, j, k)] = o;
 }
 
+template <typename E>
 __global__ void __launch_bounds__(256)
 k_pseudoupdate(Grid g, double *__restrict__ U, double *__restrict__ V, double *__restrict__ W,
-               const double *__restrict__ dU, const double *__restrict__ dV, const double *__restrict__ dW) {
+               const E *__restrict__ dU, const E *__restrict__ dV, const E *__restrict__ dW) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
     if (i > g.n1m || j > g.n2m) return;
     const size_t q = g.ix(i, j, k);
-    U[q] = U[q] + dU[q];
-    if (j >= 2) V[q] = V[q] + dV[q];
-    else V[q] = V[q] + dV[q] * 0.0;  // (2 - j_indexS) with j_indexS = 2
-    W[q] = W[q] + dW[q];
+    U[q] = U[q] + (double)dU[q];
+    if (j >= 2) V[q] = V[q] + (double)dV[q];
+    else V[q] = V[q] + (double)dV[q] * 0.0;  // (2 - j_indexS) with j_indexS = 2
+    W[q] = W[q] + (double)dW[q];
 }
 
 // ------------------------------------------------------------- host side ---
+// The increments dU, dV, dW as this form keeps them, and the access its
+// rows and block corrections read through: the FP64 module arrays
+// themselves, or (the mixed form) their FP32 twins.
+using inc_t = scr_t;
+template <typename E = inc_t>
+static inline E *inc(tcs_ctx *c, int q) {
+    if constexpr (sizeof(E) == sizeof(float)) return twin(c, c->f[q]);
+    else return c->f[q];
+}
+template <typename E = inc_t>
+static inline auto cell_access(const tcs_ctx *c) {
+    if constexpr (sizeof(E) == sizeof(float)) return twin_w(c);
+    else return RawW();
+}
+// what the momentum stages read besides the increments
+constexpr unsigned MOMENTUM_TWINS = TWIN_U | TWIN_V | TWIN_W | TWIN_P | TWIN_T | TWIN_TAB | TWIN_WALL;
+
 int momentum_coeffi(tcs_ctx *c) {
+    c->twin_ok &= ~(unsigned)TWIN_TAB;
     TCS_LAUNCH(c->stream, "momentum_coeffi", k_momentum_coeffi, dim3((c->N2 + 63) / 64, c->N3), dim3(64), 0, c->grid(),
                        c->phys, c->f[TCS_T], c->f[TCS_MU], c->f[TCS_INVRHO]);
     return TCS_OK;
 }
 
 static int launch_stale(tcs_ctx *c) {
+    c->twin_ok &= ~(unsigned)TWIN_STALE;
     TCS_LAUNCH(c->stream, "momentum_stale", k_momentum_stale, dim3(1), dim3(64), 0, c->grid(), c->f[TCS_MU], c->stale);
     return TCS_OK;
 }
@@ -869,11 +891,12 @@ static int y_stage(tcs_ctx *c, const char *label_rows, const char *label_solve, 
     if (jlo == 2 && rows_in_solve<scr_t>(c, c->n2m))
         return y_stage_in_solve<SX>(c, label_solve, row, 1, c->n2m, rhs, F, 1, jlo);
     if (rows_in_solve<scr_t>(c, nrows)) return y_stage_in_solve<SX>(c, label_solve, row, jlo, nrows, rhs, F, 1);
-    TCS_TRY(rows3d_launch<false>(c->stream, label_rows, row, jlo, am, ac, ap, rhs));
+    TCS_TRY(rows3d_launch<false>(c, label_rows, row, jlo, am, ac, ap, rhs));
     return sweep_y_field(c, jlo, nrows, F, 1);
 }
 
 int momentum_solvedU(tcs_ctx *c, double dt) {
+    TCS_TRY(ensure_twins(c, MOMENTUM_TWINS));
     const Grid g = c->grid();
     double **f = c->f;
     scr_t *am = SCR(c->am), *ac = SCR(c->ac), *ap = SCR(c->ap), *rhs = SCR(c->rhs);
@@ -890,13 +913,15 @@ int momentum_solvedV(tcs_ctx *c, double dt) {
     double **f = c->f;
     scr_t *am = SCR(c->am), *ac = SCR(c->ac), *ap = SCR(c->ap), *rhs = SCR(c->rhs);
     TCS_TRY(launch_stale(c));
+    TCS_TRY(ensure_twins(c, MOMENTUM_TWINS | TWIN_STALE));
     // rows j = 1 are not part of the dV systems: every stage covers j = 2..n2m
     const DVZ z{g, c->phys, dt, f[TCS_U], f[TCS_V], f[TCS_W], f[TCS_P], f[TCS_DU], f[TCS_MU], f[TCS_INVRHO],
                 c->wall[TCS_VBCUP], c->wall[TCS_VBCBT]};
     TCS_TRY(z_stage(c, "dV_z", "dV_z_solve", z, 2, am, ac, ap, rhs));
     TCS_TRY(x_stage(c, "dV_x", "dV_x_solve", DVX{g, c->phys, dt, f[TCS_U], f[TCS_INVRHO], c->stale}, 2, am, ac, ap,
                     rhs));
-    TCS_LAUNCH(c->stream, "dV_zero_rows", k_dV_zero_rows, dim3((c->N1 + 63) / 64, c->N3), dim3(64), 0, g, f[TCS_DV]);
+    TCS_LAUNCH(c->stream, "dV_zero_rows", k_dV_zero_rows<inc_t>, dim3((c->N1 + 63) / 64, c->N3), dim3(64), 0, g,
+               inc(c, TCS_DV));
     return y_stage<sizeof(scr_t) == 8 ? 8 : 16>(c, "dV_y", "dV_y_solve", DVY{g, c->phys, dt, f[TCS_V], f[TCS_MU], f[TCS_INVRHO]}, 2, f[TCS_DV]);
 }
 
@@ -905,6 +930,7 @@ int momentum_solvedW(tcs_ctx *c, double dt) {
     double **f = c->f;
     scr_t *am = SCR(c->am), *ac = SCR(c->ac), *ap = SCR(c->ap), *rhs = SCR(c->rhs);
     TCS_TRY(launch_stale(c));
+    TCS_TRY(ensure_twins(c, MOMENTUM_TWINS | TWIN_STALE));
     const DWZ z{g, c->phys, dt, f[TCS_U], f[TCS_V], f[TCS_W], f[TCS_P], f[TCS_DU], f[TCS_DV], f[TCS_MU],
                 f[TCS_INVRHO], c->wall[TCS_VBCUP], c->wall[TCS_VBCBT], c->wall[TCS_WBCUP], c->wall[TCS_WBCBT]};
     TCS_TRY(z_stage(c, "dW_z", "dW_z_solve", z, 1, am, ac, ap, rhs));
@@ -914,24 +940,29 @@ int momentum_solvedW(tcs_ctx *c, double dt) {
 }
 
 int momentum_blockLdV(tcs_ctx *c, double dt) {
+    TCS_TRY(ensure_twins(c, TWIN_V | TWIN_TAB));
     const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
     const BlockLdV cell{c->grid(), c->phys, dt, c->f[TCS_V], c->f[TCS_DW], c->f[TCS_MU], c->f[TCS_INVRHO], c->f[TCS_DV]};
-    TCS_LAUNCH(c->stream, "blockLdV", k_blockLdV, grd, blk, 0, cell, c->f[TCS_DV]);
+    auto acc = cell_access(c);
+    TCS_LAUNCH(c->stream, "blockLdV", (k_blockLdV<inc_t, decltype(acc)>), grd, blk, 0, cell, acc, inc(c, TCS_DV));
     return TCS_OK;
 }
 
 int momentum_blockLdU(tcs_ctx *c, double dt) {
+    TCS_TRY(ensure_twins(c, TWIN_U | TWIN_TAB));
     const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
     const BlockLdU cell{c->grid(), c->phys, dt,          c->f[TCS_U],      c->f[TCS_DV], c->f[TCS_DW],
                         c->f[TCS_MU], c->f[TCS_INVRHO], c->f[TCS_DU]};
-    TCS_LAUNCH(c->stream, "blockLdU", k_blockLdU, grd, blk, 0, cell, c->f[TCS_DU]);
+    auto acc = cell_access(c);
+    TCS_LAUNCH(c->stream, "blockLdU", (k_blockLdU<inc_t, decltype(acc)>), grd, blk, 0, cell, acc, inc(c, TCS_DU));
     return TCS_OK;
 }
 
 // blockLdU with the pseudo-update of U and W: dU = the new dU, Un = U + dU,
 // Wn = W + dW (:178-208)
-__global__ void __launch_bounds__(256) k_blockLdU_pseudo(BlockLdU cell, const double *__restrict__ W,
-                                                        const double *__restrict__ dW, double *dU,
+template <typename E, class Acc>
+__global__ void __launch_bounds__(256) k_blockLdU_pseudo(BlockLdU cell, Acc acc, const double *__restrict__ W,
+                                                        const E *__restrict__ dW, E *dU,
                                                         double *__restrict__ Un, double *__restrict__ Wn) {
     const Grid &g = cell.g;
     const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * LDU_IB + 1;  // n1m is a multiple of LDU_IB
@@ -940,15 +971,16 @@ __global__ void __launch_bounds__(256) k_blockLdU_pseudo(BlockLdU cell, const do
     // loads of every cell before the stores (dU is updated in place, each
     // cell reading only its own point of it)
     constexpr int NC = LDU_IB * LDU_KB;
-    double o[NC], un[NC], wn[NC];
+    E o[NC];
+    double un[NC], wn[NC];
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         const int i = i0 + c % LDU_IB, k = blockIdx.z * LDU_KB + c / LDU_IB + 1;
-        double x;
-        cell.eval<double>(RawW(), i, j, k, o[c], x, x, x);
+        E x;
+        cell.eval<E>(acc, i, j, k, o[c], x, x, x);
         const size_t q = g.ix(i, j, k);
-        un[c] = cell.U[q] + o[c];
-        wn[c] = W[q] + dW[q];
+        un[c] = cell.U[q] + (double)o[c];
+        wn[c] = W[q] + (double)dW[q];
     }
 #pragma unroll
     for (int c = 0; c < NC; c++) {
@@ -962,8 +994,12 @@ __global__ void __launch_bounds__(256) k_blockLdU_pseudo(BlockLdU cell, const do
 int momentum_blockLdV_pseudo(tcs_ctx *c, double dt, double *Vn) {
     const dim3 blk = interior_block(), grd = cell_grid(c->n1m, c->n2m, c->n3m, LDV_KB, interior_block(), LDV_IB);
     const BlockLdV cell{c->grid(), c->phys, dt, c->f[TCS_V], c->f[TCS_DW], c->f[TCS_MU], c->f[TCS_INVRHO], c->f[TCS_DV]};
+    TCS_TRY(ensure_twins(c, TWIN_V | TWIN_TAB));
+    c->twin_ok &= ~(unsigned)TWIN_V;  // Vn becomes V
     TCS_TRY(copy_shell(c, c->f[TCS_V], Vn));
-    TCS_LAUNCH(c->stream, "blockLdV", k_blockLdV_pseudo, grd, blk, 0, cell, c->f[TCS_DV], Vn);
+    auto acc = cell_access(c);
+    TCS_LAUNCH(c->stream, "blockLdV", (k_blockLdV_pseudo<inc_t, decltype(acc)>), grd, blk, 0, cell, acc,
+               inc(c, TCS_DV), Vn);
     return TCS_OK;
 }
 
@@ -971,16 +1007,20 @@ int momentum_blockLdU_pseudo(tcs_ctx *c, double dt, double *Un, double *Wn) {
     const dim3 blk = interior_block(), grd = cell_grid(c->n1m, c->n2m, c->n3m, LDU_KB, interior_block(), LDU_IB);
     const BlockLdU cell{c->grid(), c->phys, dt, c->f[TCS_U], c->f[TCS_DV], c->f[TCS_DW],
                         c->f[TCS_MU], c->f[TCS_INVRHO], c->f[TCS_DU]};
+    TCS_TRY(ensure_twins(c, TWIN_U | TWIN_TAB));
+    c->twin_ok &= ~(unsigned)(TWIN_U | TWIN_W);  // Un, Wn become U, W
     TCS_TRY(copy_shell(c, {{c->f[TCS_U], Un}, {c->f[TCS_W], Wn}}));
-    TCS_LAUNCH(c->stream, "blockLdU", k_blockLdU_pseudo, grd, blk, 0, cell, c->f[TCS_W], c->f[TCS_DW], c->f[TCS_DU],
-               Un, Wn);
+    auto acc = cell_access(c);
+    TCS_LAUNCH(c->stream, "blockLdU", (k_blockLdU_pseudo<inc_t, decltype(acc)>), grd, blk, 0, cell, acc,
+               c->f[TCS_W], inc(c, TCS_DW), inc(c, TCS_DU), Un, Wn);
     return TCS_OK;
 }
 
 int momentum_pseudoupdate(tcs_ctx *c) {
+    c->twin_ok &= ~(unsigned)(TWIN_U | TWIN_V | TWIN_W);
     const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
-    TCS_LAUNCH(c->stream, "pseudoupdate", k_pseudoupdate, grd, blk, 0, c->grid(), c->f[TCS_U], c->f[TCS_V], c->f[TCS_W],
-                       c->f[TCS_DU], c->f[TCS_DV], c->f[TCS_DW]);
+    TCS_LAUNCH(c->stream, "pseudoupdate", k_pseudoupdate<inc_t>, grd, blk, 0, c->grid(), c->f[TCS_U], c->f[TCS_V],
+               c->f[TCS_W], inc(c, TCS_DU), inc(c, TCS_DV), inc(c, TCS_DW));
     return TCS_OK;
 }
 

@@ -763,6 +763,7 @@ int pressure_poisson_rhs(tcs_ctx *c, double dt) {
 }
 
 int pressure_projection(tcs_ctx *c, double dt) {
+    c->twin_ok &= ~(unsigned)(TWIN_U | TWIN_V | TWIN_W | TWIN_P);  // the mixed form's FP32 copies
     const Grid g = c->grid();
     const dim3 blk = interior_block(), grd = interior_grid(c->n1m, c->n2m, c->n3m);
     const Projection cell{g, c->phys, dt, c->f[TCS_INVRHO], c->f[TCS_DP], c->f[TCS_DPHAT], c->f[TCS_DP0],
@@ -776,6 +777,7 @@ int pressure_projection(tcs_ctx *c, double dt) {
 }
 
 int pressure_projection_into(tcs_ctx *c, double dt, double *dPhatn, double *dP0n) {
+    c->twin_ok &= ~(unsigned)(TWIN_U | TWIN_V | TWIN_W | TWIN_P);  // the mixed form's FP32 copies
     const Grid g = c->grid();
     const dim3 blk = interior_block(), grd = cell_grid(c->n1m, c->n2m, c->n3m, PROJ_KB, interior_block(), PROJ_IB);
     const Projection cell{g, c->phys, dt, c->f[TCS_INVRHO], c->f[TCS_DP], c->f[TCS_DPHAT], c->f[TCS_DP0],

@@ -12,6 +12,9 @@
 #include <utility>
 #include <vector>
 
+#include <algorithm>
+
+#include "ad.h"
 #include "tcs_internal.h"
 
 namespace tcs {
@@ -104,8 +107,9 @@ struct ProfStage {
 // the value at (mx(i), j, mz(k)), where mx/mz fold ghost indices onto the
 // periodic interior.  Sources are never ghosts in x or z, so one kernel does
 // it without ordering hazards.
-__global__ void k_halo(Grid g, FieldSet fs) {
-    double *__restrict__ f = fs.f[blockIdx.y];  // one field per grid row
+template <typename E, class Set>
+__global__ void k_halo(Grid g, Set fs) {
+    E *__restrict__ f = fs.f[blockIdx.y];  // one field per grid row
     const int N1 = g.N1, N2 = g.N2, N3 = g.N3;
     const int n1m = g.n1m, n3m = g.n3m;
     // threads over the x-ghost faces (2*N2*N3) then the z-ghost faces (2*N1*N2)
@@ -182,11 +186,85 @@ int halo(tcs_ctx *c, std::initializer_list<double *> fields) {
         fs.f[nf++] = f;
     }
     const size_t n = (size_t)2 * c->N2 * c->N3 + (size_t)2 * c->N1 * c->N2;
-    TCS_LAUNCH(c->stream, "halo", k_halo, dim3((unsigned)((n + 255) / 256), (unsigned)nf), dim3(256), 0, c->grid(),
-               fs);
+    TCS_LAUNCH(c->stream, "halo", (k_halo<double, FieldSet>), dim3((unsigned)((n + 255) / 256), (unsigned)nf),
+               dim3(256), 0, c->grid(), fs);
     return TCS_OK;
 }
 int halo(tcs_ctx *c, double *f) { return halo(c, {f}); }
+
+struct FieldSet32 {
+    float *f[FieldSet::MAX];
+};
+int halo(tcs_ctx *c, std::initializer_list<float *> fields) {
+    FieldSet32 fs{};
+    int nf = 0;
+    for (float *f : fields) {
+        if (nf == FieldSet::MAX || !f) return TCS_E_ARG;
+        fs.f[nf++] = f;
+    }
+    const size_t n = (size_t)2 * c->N2 * c->N3 + (size_t)2 * c->N1 * c->N2;
+    TCS_LAUNCH(c->stream, "halo_f32", (k_halo<float, FieldSet32>), dim3((unsigned)((n + 255) / 256), (unsigned)nf),
+               dim3(256), 0, c->grid(), fs);
+    return TCS_OK;
+}
+
+// ------------------------------------------------- the mixed form's twins --
+__global__ void k_to_f32(const double *__restrict__ src, float *__restrict__ dst, size_t n) {
+    const size_t s = (size_t)gridDim.x * blockDim.x;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += s) dst[q] = (float)src[q];
+}
+__global__ void k_to_f64(const float *__restrict__ src, double *__restrict__ dst, size_t n) {
+    const size_t s = (size_t)gridDim.x * blockDim.x;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += s) dst[q] = (double)src[q];
+}
+
+static int to_f32(tcs_ctx *c, const double *src, size_t n) {
+    float *dst = twin(c, src);
+    if (!dst || !twin(c, src + n - 1)) {
+        set_error("mixed form: an array outside the FP32-twinned arena");
+        return TCS_E_ARG;
+    }
+    const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    TCS_LAUNCH(c->stream, "twin_f32", k_to_f32, dim3(nb), dim3(256), 0, src, dst, n);
+    return TCS_OK;
+}
+
+// a full field's span: its front, the field, one element after
+static size_t span_full(const tcs_ctx *c) { return c->nfull + FIELD_FRONT + 1; }
+
+int ensure_twins(tcs_ctx *c, unsigned bits) {
+    if (!c->a32) return TCS_OK;
+    const unsigned need = bits & ~c->twin_ok;
+    const struct { unsigned bit; int q; } fields[] = {
+        {TWIN_T, TCS_T}, {TWIN_U, TCS_U}, {TWIN_V, TCS_V}, {TWIN_W, TCS_W}, {TWIN_P, TCS_P}};
+    for (const auto &f : fields)
+        if (need & f.bit) TCS_TRY(to_f32(c, c->f[f.q] - FIELD_FRONT, span_full(c)));
+    if (need & TWIN_TAB) {
+        TCS_TRY(to_f32(c, c->f[TCS_MU], c->ntab));
+        TCS_TRY(to_f32(c, c->f[TCS_INVRHO], c->ntab));
+    }
+    if (need & TWIN_WALL)
+        for (int w = 0; w < TCS_NWALL; w++) TCS_TRY(to_f32(c, c->wall[w], c->nwall));
+    if (need & TWIN_STALE) TCS_TRY(to_f32(c, c->stale, 16));
+    c->twin_ok |= bits;
+    return TCS_OK;
+}
+
+static bool is_increment(int id) { return id == TCS_DU || id == TCS_DV || id == TCS_DW; }
+
+int increment_to_fp64(tcs_ctx *c, int id) {
+    if (!c->a32 || !is_increment(id)) return TCS_OK;
+    const double *p = c->f[id] - FIELD_FRONT;
+    const size_t n = span_full(c);
+    TCS_LAUNCH(c->stream, "twin_f64", k_to_f64, dim3((unsigned)std::min<size_t>((n + 255) / 256, 8192)), dim3(256), 0,
+               twin(c, p), const_cast<double *>(p), n);
+    return TCS_OK;
+}
+
+int increment_from_fp64(tcs_ctx *c, int id) {
+    if (!c->a32 || !is_increment(id)) return TCS_OK;
+    return to_f32(c, c->f[id] - FIELD_FRONT, span_full(c));
+}
 
 // ------------------------------------------------------------ reductions --
 constexpr int RED_BLOCKS = 1024;
@@ -449,10 +527,39 @@ static int falloc(double **p, size_t nfull) {
     *p = b + FIELD_FRONT;
     return TCS_OK;
 }
-static void ffree(double *p) {
-    if (p) (void)hipFree(p - FIELD_FRONT);
-}
 static bool is_table(int q) { return q == TCS_INVRHO || q == TCS_MU; }
+static bool in_arena(const tcs_ctx *c, const double *p) { return c->a64 && p >= c->a64 && p < c->a64 + c->an; }
+static void dfree(const tcs_ctx *c, double *p) {
+    if (p && !in_arena(c, p)) (void)hipFree(p);
+}
+static void ffree(const tcs_ctx *c, double *p) {
+    if (p) dfree(c, p - FIELD_FRONT);
+}
+
+// The mixed form's arena (tcs_ctx::a64, zeroed): n elements on a 256-byte
+// boundary, or a plain allocation in the FP64 forms.
+static int adalloc(tcs_ctx *c, double **p, size_t n) {
+    if (!c->a64) return dalloc(p, n);
+    const size_t at = (c->aused + 31) / 32 * 32;
+    if (at + n > c->an) {
+        set_error("tcs_init: mixed-form arena too small");
+        return TCS_E_NOMEM;
+    }
+    c->aused = at + n;
+    *p = c->a64 + at;
+    return TCS_OK;
+}
+static int afalloc(tcs_ctx *c, double **p, size_t nfull) {
+    double *b = nullptr;
+    TCS_TRY(adalloc(c, &b, nfull + FIELD_FRONT + 1));
+    *p = b + FIELD_FRONT;
+    return TCS_OK;
+}
+// full fields the mixed form's rows read (the state and the increments)
+static bool twinned(int q) {
+    return q == TCS_T || q == TCS_U || q == TCS_V || q == TCS_W || q == TCS_P || q == TCS_DU || q == TCS_DV ||
+           q == TCS_DW;
+}
 
 static void free_ctx(tcs_ctx *c) {
     if (!c) return;
@@ -460,18 +567,18 @@ static void free_ctx(tcs_ctx *c) {
     double **bufs[] = {&c->dx1, &c->dx2, &c->dx3, &c->dmx1, &c->dmx2, &c->dmx3, &c->rmet, &c->am, &c->ac, &c->ap, &c->rhs,
                        &c->stale, &c->prhs, &c->spec_x, &c->spec_y, &c->pA, &c->pC, &c->dxk2, &c->dzk, &c->ptab,
                        &c->tw1, &c->tw3, &c->linesum, &c->planesum, &c->scal, &c->red};
-    for (double **b : bufs)
-        if (*b) (void)hipFree(*b);
+    for (double **b : bufs) dfree(c, *b);
     for (int q = 0; q < TCS_NFIELD; q++) {
         if (is_table(q)) {
-            if (c->f[q]) (void)hipFree(c->f[q]);
+            dfree(c, c->f[q]);
         } else {
-            ffree(c->f[q]);
-            ffree(c->spare[q]);
+            ffree(c, c->f[q]);
+            ffree(c, c->spare[q]);
         }
     }
-    for (int q = 0; q < TCS_NWALL; q++)
-        if (c->wall[q]) (void)hipFree(c->wall[q]);
+    for (int q = 0; q < TCS_NWALL; q++) dfree(c, c->wall[q]);
+    if (c->a64) (void)hipFree(c->a64);
+    if (c->a32) (void)hipFree(c->a32);
     if (c->jC) (void)hipFree(c->jC);
     if (c->jS) (void)hipFree(c->jS);
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);
@@ -600,7 +707,7 @@ extern "C" int tcs_init(const tcs_params *prm, const double *dx1, const double *
         {&c->dx3, (size_t)c->N3}, {&c->dmx3, (size_t)c->N3},
         {&c->rmet, (size_t)2 * (c->N1 + c->N2 + c->N3)},
         {&c->am, c->nint}, {&c->ac, c->nint}, {&c->ap, c->nint}, {&c->rhs, c->nint},
-        {&c->stale, 16}, {&c->prhs, c->nint},
+        {&c->prhs, c->nint},
         {&c->spec_x, (size_t)2 * c->h1p * c->n2m * c->n3m},
         {&c->pA, (size_t)c->h1p * c->n2m * c->n3m}, {&c->pC, (size_t)c->h1p * c->n2m * c->n3m},
         {&c->dxk2, (size_t)c->h1p}, {&c->dzk, (size_t)c->n3m}, {&c->ptab, (size_t)3 * c->n2m},
@@ -610,12 +717,34 @@ extern "C" int tcs_init(const tcs_params *prm, const double *dx1, const double *
     };
     for (auto &a : allocs)
         if ((rc = dalloc(a.p, a.n)) != TCS_OK) return fail(rc);
-    for (int q = 0; q < TCS_NFIELD; q++)
-        if ((rc = is_table(q) ? dalloc(&c->f[q], c->ntab) : falloc(&c->f[q], c->nfull)) != TCS_OK) return fail(rc);
-    for (int q : {TCS_T, TCS_U, TCS_V, TCS_W, TCS_DPHAT, TCS_DP0})
-        if ((rc = falloc(&c->spare[q], c->nfull)) != TCS_OK) return fail(rc);
+    if (prm->precision == 2) {
+        // the arrays the mixed form's rows and block corrections read, in one
+        // arena with an FP32 twin of each (tcs_ctx::a64, a32)
+        auto up = [](size_t n) { return (n + 31) / 32 * 32; };
+        c->an = 12 * up(c->nfull + FIELD_FRONT + 1) + 2 * up(c->ntab) + TCS_NWALL * up(c->nwall) + up(16);
+        if (hipMalloc((void **)&c->a64, c->an * sizeof(double)) != hipSuccess ||
+            hipMalloc((void **)&c->a32, c->an * sizeof(float)) != hipSuccess) {
+            set_error("hipMalloc failed (mixed-form arenas, " + std::to_string(c->an * 12) + " bytes)");
+            return fail(TCS_E_NOMEM);
+        }
+        if (hipMemset(c->a64, 0, c->an * sizeof(double)) != hipSuccess ||
+            hipMemset(c->a32, 0, c->an * sizeof(float)) != hipSuccess) {
+            set_error("hipMemset failed");
+            return fail(TCS_E_HIP);
+        }
+    }
+    if ((rc = adalloc(c, &c->stale, 16)) != TCS_OK) return fail(rc);
+    for (int q = 0; q < TCS_NFIELD; q++) {
+        if (is_table(q)) rc = adalloc(c, &c->f[q], c->ntab);
+        else rc = twinned(q) ? afalloc(c, &c->f[q], c->nfull) : falloc(&c->f[q], c->nfull);
+        if (rc != TCS_OK) return fail(rc);
+    }
+    for (int q : {TCS_T, TCS_U, TCS_V, TCS_W, TCS_DPHAT, TCS_DP0}) {
+        rc = twinned(q) ? afalloc(c, &c->spare[q], c->nfull) : falloc(&c->spare[q], c->nfull);
+        if (rc != TCS_OK) return fail(rc);
+    }
     for (int q = 0; q < TCS_NWALL; q++)
-        if ((rc = dalloc(&c->wall[q], c->nwall)) != TCS_OK) return fail(rc);
+        if ((rc = adalloc(c, &c->wall[q], c->nwall)) != TCS_OK) return fail(rc);
     if (hipMalloc((void **)&c->jC, c->N2 * sizeof(int)) != hipSuccess ||
         hipMalloc((void **)&c->jS, c->N2 * sizeof(int)) != hipSuccess) {
         set_error("hipMalloc failed");
@@ -681,6 +810,19 @@ extern "C" void *tcs_stream(tcs_ctx *c) {
     return (void *)c->stream;
 }
 
+// the TWIN_* bit of a field's FP32 copy in the mixed form
+static unsigned twin_bit(int id) {
+    switch (id) {
+    case TCS_T: return TWIN_T;
+    case TCS_U: return TWIN_U;
+    case TCS_V: return TWIN_V;
+    case TCS_W: return TWIN_W;
+    case TCS_P: return TWIN_P;
+    case TCS_MU: case TCS_INVRHO: return TWIN_TAB;
+    default: return 0;
+    }
+}
+
 static int check_field(tcs_ctx *c, int id) {
     if (!c || id < 0 || id >= TCS_NFIELD) {
         set_error("bad context or field id");
@@ -688,6 +830,7 @@ static int check_field(tcs_ctx *c, int id) {
     }
     TCS_TRY(flush_pending(c));
     c->cfl_valid = 0;  // the field may change (upload, halo, a handed-out pointer)
+    c->twin_ok &= ~twin_bit(id);  // and with it the mixed form's FP32 copy
     return TCS_OK;
 }
 
@@ -709,6 +852,7 @@ extern "C" int tcs_field_upload(tcs_ctx *c, int32_t id, const double *host) {
     TCS_HIP(hipMemcpyAsync(stage, host, row * c->N2 * c->N3, hipMemcpyHostToDevice, c->stream));
     TCS_HIP(hipMemcpy2DAsync(c->f[id], pitch, stage, row, row, (size_t)c->N2 * c->N3, hipMemcpyDeviceToDevice,
                              c->stream));
+    TCS_TRY(increment_from_fp64(c, id));  // the mixed form keeps dU, dV, dW in FP32
     TCS_HIP(hipStreamSynchronize(c->stream));
     return TCS_OK;
 }
@@ -726,6 +870,7 @@ extern "C" int tcs_field_download(tcs_ctx *c, int32_t id, double *host) {
     }
     const size_t row = (size_t)c->N1 * sizeof(double), pitch = (size_t)c->P1 * sizeof(double);
     double *stage = c->spare[TCS_T];
+    TCS_TRY(increment_to_fp64(c, id));  // the mixed form keeps dU, dV, dW in FP32
     TCS_HIP(hipMemcpy2DAsync(stage, row, c->f[id], pitch, row, (size_t)c->N2 * c->N3, hipMemcpyDeviceToDevice,
                              c->stream));
     TCS_HIP(hipMemcpyAsync(host, stage, row * c->N2 * c->N3, hipMemcpyDeviceToHost, c->stream));
@@ -736,6 +881,7 @@ extern "C" int tcs_field_download(tcs_ctx *c, int32_t id, double *host) {
 extern "C" int tcs_field_device_ptr(tcs_ctx *c, int32_t id, double **dptr) {
     TCS_TRY(check_field(c, id));
     if (!dptr) return TCS_E_ARG;
+    TCS_TRY(increment_to_fp64(c, id));  // the mixed form keeps dU, dV, dW in FP32
     *dptr = c->f[id];
     return TCS_OK;
 }
@@ -744,6 +890,8 @@ extern "C" int tcs_fields_touched(tcs_ctx *c) {
     if (!c) return TCS_E_ARG;
     TCS_TRY(flush_pending(c));
     c->cfl_valid = 0;
+    c->twin_ok = 0;
+    for (int q : {TCS_DU, TCS_DV, TCS_DW}) TCS_TRY(increment_from_fp64(c, q));
     return TCS_OK;
 }
 
@@ -756,6 +904,7 @@ extern "C" int tcs_field_pitch(tcs_ctx *c, int32_t *pitch) {
 extern "C" int tcs_wall_upload(tcs_ctx *c, int32_t id, const double *host) {
     if (!c || id < 0 || id >= TCS_NWALL || !host) return TCS_E_ARG;
     TCS_TRY(flush_pending(c));
+    c->twin_ok &= ~(unsigned)TWIN_WALL;
     TCS_HIP(hipMemcpyAsync(c->wall[id], host, c->nwall * sizeof(double), hipMemcpyHostToDevice, c->stream));
     TCS_HIP(hipStreamSynchronize(c->stream));
     return TCS_OK;
@@ -784,6 +933,7 @@ static int set_row(tcs_ctx *c, int field, int jrow, int wall) {
 extern "C" int tcs_thermal_boundary(tcs_ctx *c, double Tup, double Tbt) {
     if (!c) return TCS_E_ARG;
     TCS_TRY(flush_pending(c));
+    c->twin_ok = 0;
     TCS_TRY(fill_wall(c, TCS_TBCUP, Tup));
     TCS_TRY(fill_wall(c, TCS_TBCBT, Tbt));
     // comm_1d_x2%myrank == 0 branch only (single y-rank)
@@ -796,6 +946,7 @@ extern "C" int tcs_momentum_boundary(tcs_ctx *c, double Uup, double Ubt, double 
     if (!c) return TCS_E_ARG;
     TCS_TRY(flush_pending(c));
     c->cfl_valid = 0;
+    c->twin_ok = 0;
     TCS_TRY(fill_wall(c, TCS_UBCUP, Uup));
     TCS_TRY(fill_wall(c, TCS_VBCUP, Vup));
     TCS_TRY(fill_wall(c, TCS_WBCUP, Wup));
@@ -826,6 +977,13 @@ extern "C" int tcs_momentum_boundary(tcs_ctx *c, double Uup, double Ubt, double 
                                : ::tcs::exact::fn((c), ##__VA_ARGS__))
 
 namespace tcs {
+// the ghost-cell update of a field as the context's form keeps it: the FP32
+// twin for the mixed form's increments dU, dV, dW
+static int halo_field(tcs_ctx *c, int id) {
+    if (c->a32 && is_increment(id)) return halo(c, {twin(c, c->f[id])});
+    return halo(c, c->f[id]);
+}
+
 int flush_pending(tcs_ctx *c) {
     const int p = c->pend;
     if (!p) return TCS_OK;
@@ -836,7 +994,7 @@ int flush_pending(tcs_ctx *c) {
     }
     if (p & PEND_HALO_DV) {
         ProfStage _s(c, "update ghostcell");
-        TCS_TRY(halo(c, c->f[TCS_DV]));
+        TCS_TRY(halo_field(c, TCS_DV));
     }
     if (p & PEND_LDU) {
         ProfStage _s(c, "momentum blockLdU");
@@ -896,7 +1054,7 @@ extern "C" int tcs_momentum_pseudoupdateUVW(tcs_ctx *c) {
         }
         {
             STAGE("update ghostcell");
-            TCS_TRY(halo(c, c->f[TCS_DV]));
+            TCS_TRY(halo_field(c, TCS_DV));
         }
         {
             STAGE("momentum blockLdU");
@@ -947,7 +1105,7 @@ extern "C" int tcs_ghostcell_update(tcs_ctx *c, int32_t id) {
     TCS_TRY(check_field(c, id));
     if (id == TCS_INVRHO || id == TCS_MU) return TCS_OK;  // tables carry no x/z ghosts of their own
     STAGE("update ghostcell");
-    return halo(c, c->f[id]);
+    return halo_field(c, id);
 }
 
 extern "C" int tcs_post_div(tcs_ctx *c, double *maxdiv) {
@@ -996,15 +1154,15 @@ extern "C" int tcs_step(tcs_ctx *c, double dt) {
     RUN(H, halo(c, c->f[TCS_T]));
     RUN("momentum coeff", FORM(c, momentum_coeffi));
     RUN("momentum solvedU", FORM(c, momentum_solvedU, dt));
-    RUN(H, halo(c, c->f[TCS_DU]));
+    RUN(H, halo_field(c, TCS_DU));
     RUN("momentum solvedV", FORM(c, momentum_solvedV, dt));
-    RUN(H, halo(c, c->f[TCS_DV]));
+    RUN(H, halo_field(c, TCS_DV));
     RUN("momentum solvedW", FORM(c, momentum_solvedW, dt));
-    RUN(H, halo(c, c->f[TCS_DW]));
+    RUN(H, halo_field(c, TCS_DW));
     // block corrections with the pseudo-update (U + dU, V + dV, W + dW into
     // the second buffers; the corrections read the old U, V)
     RUN("momentum blockLdV", FORM(c, momentum_blockLdV_pseudo, dt, c->spare[TCS_V]));
-    RUN(H, halo(c, c->f[TCS_DV]));
+    RUN(H, halo_field(c, TCS_DV));
     RUN("momentum blockLdU", FORM(c, momentum_blockLdU_pseudo, dt, c->spare[TCS_U], c->spare[TCS_W]));
     for (int q : {TCS_U, TCS_V, TCS_W}) std::swap(c->f[q], c->spare[q]);
     RUN(H, halo(c, {c->f[TCS_U], c->f[TCS_V], c->f[TCS_W]}));

@@ -27,21 +27,17 @@
 namespace tcs {
 
 // Row (i, j, k) of a stage in the arithmetic of the line solve's type S:
-// FP64 solves take the rows as the step computes them.  FP32 solves (the
-// mixed form) take them in FP64 rounded to float, or, for a stage whose
-// Row::F32 is set, make them in FP32 from the FP64 fields (loads rounded to
-// float, metrics and constants in float: the cell templates' sc<S> and
-// metric<S>).  No stage sets it: at 512^3 FP32 rows made the thermal z and
-// x stages 0.3 and 0.1 ms faster but the 1000-step drift of the wall
-// Nusselt numbers grew from 9e-9 to 2e-5 and of T from 5e-9 to 3e-7, the
-// dU/dV/dW z rows were 1.0-2.0 ms slower, and the x and y momentum rows
-// took the same time.  d is the row's right-hand side where the stage has
-// one, else 0.
+// FP64 solves take the rows as the step computes them; FP32 solves (the
+// mixed form) make them in FP32 from the FP32 twins of the arrays they read
+// (tw), with the metrics and constants rounded to float (the cell
+// templates' sc<S> and metric<S>).  d is the row's right-hand side where
+// the stage has one, else 0.
 template <typename S, class Row>
-__device__ __forceinline__ void row_as(const Row &row, int i, int j, int k, S &a, S &b, S &c, S &d) {
-    if constexpr (std::is_same<S, float>::value && Row::F32) {
+__device__ __forceinline__ void row_as(const Row &row, const TwinW &tw, int i, int j, int k, S &a, S &b, S &c,
+                                       S &d) {
+    if constexpr (std::is_same<S, float>::value) {
         d = 0.0f;
-        row.template eval<float>(F32W(), i, j, k, a, b, c, d);
+        row.template eval<float>(tw, i, j, k, a, b, c, d);
     } else {
         double ta, tb, tc, td = 0.0;
         row(i, j, k, ta, tb, tc, td);
@@ -59,7 +55,7 @@ constexpr int ROWS3D_KB = 4;
 constexpr int ROWS3D_IB = 1;  // consecutive i per thread (2: thermal x 1.53 -> 2.05 ms)
 template <class Row, bool RHS, typename S>
 __global__ void __launch_bounds__(256)
-k_rows3d(Row row, int jlo, S *__restrict__ am, S *__restrict__ ac, S *__restrict__ ap, S *__restrict__ rhs) {
+k_rows3d(Row row, TwinW tw, int jlo, S *__restrict__ am, S *__restrict__ ac, S *__restrict__ ap, S *__restrict__ rhs) {
     const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * ROWS3D_IB + 1;
     const int j = wave_row_j();
     if (i0 > row.g.n1m || j > row.g.n2m || j < jlo) return;
@@ -67,7 +63,7 @@ k_rows3d(Row row, int jlo, S *__restrict__ am, S *__restrict__ ac, S *__restrict
     for (int q = 0; q < ROWS3D_KB * ROWS3D_IB; q++) {
         const int i = i0 + q % ROWS3D_IB, k = blockIdx.z * ROWS3D_KB + q / ROWS3D_IB + 1;
         S a, b, c, d;
-        row_as<S>(row, i, j, k, a, b, c, d);
+        row_as<S>(row, tw, i, j, k, a, b, c, d);
         const size_t o = row.g.r3(i, j, k);
         ac[o] = b;
         ap[o] = c;
@@ -77,11 +73,29 @@ k_rows3d(Row row, int jlo, S *__restrict__ am, S *__restrict__ ac, S *__restrict
 }
 
 template <bool RHS, class Row, typename S>
-int rows3d_launch(hipStream_t st, const char *label, const Row &row, int jlo, S *am, S *ac, S *ap, S *rhs) {
+int rows3d_launch(tcs_ctx *c, const char *label, const Row &row, int jlo, S *am, S *ac, S *ap, S *rhs) {
     const dim3 blk = interior_block(), grd = cell_grid(row.g.n1m, row.g.n2m, row.g.n3m, ROWS3D_KB, interior_block(), ROWS3D_IB);
-    TCS_LAUNCH(st, label, (k_rows3d<Row, RHS, S>), grd, blk, 0, row, jlo, am, ac, ap, rhs);
+    TCS_LAUNCH(c->stream, label, (k_rows3d<Row, RHS, S>), grd, blk, 0, row, twin_w(c), jlo, am, ac, ap, rhs);
     return TCS_OK;
 }
+
+// Occupancy and row loop of a stage's FP32 line solves (the mixed form):
+// Row::W32 waves per SIMD at least (1: as many registers as the compiler
+// takes, two waves per SIMD for these kernels) and Row::FULL32 for the
+// unconditional row loop.  Stages that set neither keep 1 and the guarded
+// loop.
+template <class Row, class = void>
+struct RowF32 {
+    static constexpr int waves = 1;
+    static constexpr bool full = false;
+};
+template <class Row>
+struct RowF32<Row, std::void_t<decltype(Row::W32)>> {
+    static constexpr int waves = Row::W32;
+    static constexpr bool full = Row::FULL32;
+};
+template <class Row, typename S>
+constexpr int line_waves() { return std::is_same<S, float>::value ? RowF32<Row>::waves : 1; }
 
 // z lines as a split-solve source: line (s0, s1) = (i-1, j-jlo), row k-1;
 // the right-hand side comes with the row.
@@ -91,10 +105,13 @@ struct ZLineRows {
     // (measured per stage at 512^3: dU z 4.14 -> 3.14 ms; the thermal, dV and
     // dW z rows spill in it and lose 0.7-2.8 ms)
     static constexpr bool full_rows = Row::FULL;
+    static constexpr bool full_rows32 = RowF32<Row>::full;
+    static constexpr int waves = line_waves<Row, S>();
     Row row;
+    TwinW tw;
     int jlo;
     __device__ __forceinline__ void operator()(int s0, int s1, int r, S &a, S &b, S &c, S &d) const {
-        row_as<S>(row, s0 + 1, s1 + jlo, r + 1, a, b, c, d);
+        row_as<S>(row, tw, s0 + 1, s1 + jlo, r + 1, a, b, c, d);
     }
 };
 
@@ -103,12 +120,15 @@ struct ZLineRows {
 template <class Row, typename S>
 struct YLineRows {
     static constexpr bool full_rows = true;  // the unconditional row loop pays for every y stage
+    static constexpr bool full_rows32 = RowF32<Row>::full;
+    static constexpr int waves = line_waves<Row, S>();
     Row row;
+    TwinW tw;
     const S *rhs;
     int jlo;
     __device__ __forceinline__ void operator()(int s0, int s1, int r, S &a, S &b, S &c, S &d) const {
         const int i = s0 + 1, j = r + jlo, k = s1 + 1;
-        row_as<S>(row, i, j, k, a, b, c, d);
+        row_as<S>(row, tw, i, j, k, a, b, c, d);
         d = rhs[row.g.r3(i, j, k)];
     }
 };
@@ -123,12 +143,15 @@ struct YLineRows {
 template <class Row, typename S>
 struct YLineRowsLead {
     static constexpr bool full_rows = true;
+    static constexpr bool full_rows32 = RowF32<Row>::full;
+    static constexpr int waves = line_waves<Row, S>();
     Row row;
+    TwinW tw;
     const S *rhs;
     int jlo, jrow;
     __device__ __forceinline__ void operator()(int s0, int s1, int r, S &a, S &b, S &c, S &d) const {
         const int i = s0 + 1, j = r + jlo, k = s1 + 1;
-        row_as<S>(row, i, j, k, a, b, c, d);
+        row_as<S>(row, tw, i, j, k, a, b, c, d);
         d = rhs[row.g.r3(i, j, k)];
         const bool wall = j < jrow;
         a = wall ? S(0) : a;
@@ -143,11 +166,12 @@ struct YLineRowsLead {
 template <class Row, typename S>
 struct XLineRows {
     Row row;
+    TwinW tw;
     const S *rhs;
     int jlo;
     __device__ __forceinline__ void operator()(int s0, int s1, int r, S &a, S &b, S &c, S &d) const {
         const int i = r + 1, j = s0 + jlo, k = s1 + 1;
-        row_as<S>(row, i, j, k, a, b, c, d);
+        row_as<S>(row, tw, i, j, k, a, b, c, d);
         d = rhs[row.g.r3(i, j, k)];
     }
 };
@@ -169,9 +193,9 @@ int z_stage(tcs_ctx *c, const char *label_rows, const char *label_solve, const R
             S *ap, S *rhs) {
     const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m, off = (long)(jlo - 1) * n1;
     if (rows_in_solve<S>(c, c->n3m))
-        return ptdma_part_src_launch<true>(c->stream, label_solve, ZLineRows<Row, S>{row, jlo}, rhs + off, c->n1m,
-                                           c->n2m - jlo + 1, n1, c->n3m, n12);
-    TCS_TRY(rows3d_launch<true>(c->stream, label_rows, row, jlo, am, ac, ap, rhs));
+        return ptdma_part_src_launch<true>(c->stream, label_solve, ZLineRows<Row, S>{row, twin_w(c), jlo}, rhs + off,
+                                           c->n1m, c->n2m - jlo + 1, n1, c->n3m, n12);
+    TCS_TRY(rows3d_launch<true>(c, label_rows, row, jlo, am, ac, ap, rhs));
     return ptdma_sweep(c, am + off, ac + off, ap + off, rhs + off, c->n1m, 1, c->n2m - jlo + 1, n1, c->n3m, n12,
                        true);
 }
@@ -184,9 +208,9 @@ int x_stage(tcs_ctx *c, const char *label_rows, const char *label_solve, const R
     const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m, off = (long)(jlo - 1) * n1;
     const int nl = c->n2m - jlo + 1;
     if (rows_in_solve<S>(c, c->n1m))
-        return ptdma_part_rows_src_launch<true>(c->stream, label_solve, XLineRows<Row, S>{row, rhs, jlo}, rhs + off,
-                                                nl, n1, c->n3m, n12, c->n1m);
-    TCS_TRY(rows3d_launch<false>(c->stream, label_rows, row, jlo, am, ac, ap, rhs));
+        return ptdma_part_rows_src_launch<true>(c->stream, label_solve, XLineRows<Row, S>{row, twin_w(c), rhs, jlo},
+                                                rhs + off, nl, n1, c->n3m, n12, c->n1m);
+    TCS_TRY(rows3d_launch<false>(c, label_rows, row, jlo, am, ac, ap, rhs));
     return ptdma_sweep(c, am + off, ac + off, ap + off, rhs + off, nl, n1, c->n3m, n12, c->n1m, 1, true);
 }
 
@@ -207,12 +231,22 @@ int y_stage_in_solve(tcs_ctx *c, const char *label, const Row &row, int jlo, int
         out.ss0 = 1;
         out.ss1 = (long)c->P1 * c->N2;
         out.rs = c->P1;
+        if constexpr (sizeof(S) == sizeof(float)) {
+            // the mixed form stores its increments into their FP32 twins
+            out.G = twin(c, F);
+            if (mode != 1 || !out.G) {
+                set_error("y stage: the FP32 lines store increments into their FP32 twins only");
+                return TCS_E_ARG;
+            }
+            out.F = nullptr;
+            out.mode = 4;
+        }
     }
     if (jrow > jlo)
-        return ptdma_part_src_launch<false, SX>(c->stream, label, YLineRowsLead<Row, S>{row, rhs, jlo, jrow}, rhs + off,
-                                                c->n1m, c->n3m, n12, nrows, n1, out);
-    return ptdma_part_src_launch<false, SX>(c->stream, label, YLineRows<Row, S>{row, rhs, jlo}, rhs + off, c->n1m,
-                                        c->n3m, n12, nrows, n1, out);
+        return ptdma_part_src_launch<false, SX>(c->stream, label, YLineRowsLead<Row, S>{row, twin_w(c), rhs, jlo, jrow},
+                                                rhs + off, c->n1m, c->n3m, n12, nrows, n1, out);
+    return ptdma_part_src_launch<false, SX>(c->stream, label, YLineRows<Row, S>{row, twin_w(c), rhs, jlo}, rhs + off,
+                                            c->n1m, c->n3m, n12, nrows, n1, out);
 }
 
 }  // namespace tcs

@@ -145,6 +145,19 @@ struct tcs_ctx {
     // (PEND_* bits; tcs::flush_pending runs them as called)
     int pend;
     double pend_dt_ldv, pend_dt_ldu, pend_dt_rhs;
+    // The mixed form (precision 2): every array its stage rows and block
+    // corrections read (T, U, V, W, P and the second buffers of T, U, V, W;
+    // dU, dV, dW; the Mu and invRho tables; the wall tables; the stale
+    // scalars) is carved from one FP64 arena a64 of an elements, and the
+    // FP32 arena a32 holds each one's twin at the same element offset.  The
+    // twins of the increments dU, dV, dW are where the mixed form keeps them
+    // (the FP64 arrays are filled from them on download); the others are
+    // FP32 copies of the FP64 arrays, current while their TWIN_* bit is set
+    // in twin_ok.  Null / 0 in the FP64 forms.
+    double *a64;
+    float *a32;
+    size_t an, aused;
+    unsigned twin_ok;
 
     tcs::Grid grid() const {
         tcs::Grid g;
@@ -189,11 +202,13 @@ int ptdma_exact_launch(double *A, const double *B, double *C, double *D,
 // Where a split solve puts its solution: back into D (mode 0), or into an
 // FP64 field F at F[base + s0*ss0 + s1*ss1 + row*rs], stored (mode 1),
 // added to what F holds (mode 2), or added to what Fin holds at the same
-// place (mode 3: F a second buffer of the field) -- the last sweep of an ADI
+// place (mode 3: F a second buffer of the field), or stored into the FP32
+// field G (mode 4: the mixed form's increments) -- the last sweep of an ADI
 // stage writes the increment straight into its module array.
 struct LineOut {
     double *F = nullptr;
     const double *Fin = nullptr;
+    float *G = nullptr;  // mode 4: the FP32 field G at the same place, stored
     long base = 0, ss0 = 0, ss1 = 0, rs = 0;
     int mode = 0;
 };
@@ -260,6 +275,22 @@ struct FieldSet {
 };
 int halo(tcs_ctx *c, double *f);
 int halo(tcs_ctx *c, std::initializer_list<double *> fields);
+// the same ghost-cell update of FP32 fields (the mixed form's increments)
+int halo(tcs_ctx *c, std::initializer_list<float *> fields);
+// The mixed form's FP32 twins of FP64 arrays (tcs_ctx::a32): which are
+// current (tcs_ctx::twin_ok), making them current, and the increment of a
+// field as the form keeps it (the FP64 array, or its FP32 twin).
+enum {
+    TWIN_T = 1, TWIN_U = 2, TWIN_V = 4, TWIN_W = 8, TWIN_P = 16,
+    TWIN_TAB = 32,    // Mu, invRho
+    TWIN_WALL = 64,   // the eight wall tables
+    TWIN_STALE = 128  // the stale scalars of the momentum stages
+};
+int ensure_twins(tcs_ctx *c, unsigned bits);
+// FP32 increment twin -> FP64 array (downloads of dU, dV, dW in the mixed
+// form) and back (uploads)
+int increment_to_fp64(tcs_ctx *c, int id);
+int increment_from_fp64(tcs_ctx *c, int id);
 // copy the cells outside the interior (x/z ghosts, wall rows) of a full field
 int copy_shell(tcs_ctx *c, const double *src, double *dst);
 int copy_shell(tcs_ctx *c, std::initializer_list<std::pair<const double *, double *>> pairs);

@@ -270,6 +270,15 @@ template <typename T>
 __device__ __forceinline__ void store(const T (&a)[PT_MMAX], const T (&c)[PT_MMAX], const T (&d)[1][PT_MMAX],
                                       int j0, int m, T xf, T xl, T *D, long dbase, long rs, const LineOut &out,
                                       int s0, int s1) {
+    if constexpr (sizeof(T) == sizeof(float)) {
+        if (out.mode == 4) {  // FP32 lines only: the mixed form's increments
+            const long fb = out.base + (long)s0 * out.ss0 + (long)s1 * out.ss1;
+#pragma unroll
+            for (int r = 0; r < PT_MMAX; r++)
+                if (r < m) out.G[fb + (long)(j0 + r) * out.rs] = x_of<T, 1>(a, c, d, 0, r, m, xf, xl);
+            return;
+        }
+    }
     if (out.mode == 0) {
 #pragma unroll
         for (int r = 0; r < PT_MMAX; r++)
@@ -318,7 +327,7 @@ __device__ __forceinline__ void xcd_tiles(int &bx, int &by) {
 // thread layout, the reduced solve and the output (D at s0 + s1*ss1 +
 // row*rs, or a field) are those of the split solve of stored lines.
 template <typename T, bool CYC, class Src, int SX, bool FULL>
-__global__ void __launch_bounds__(SX * PT_PMAX)
+__global__ void __launch_bounds__(SX * PT_PMAX) __attribute__((amdgpu_waves_per_eu(Src::waves)))
 k_ptdma_part_src(Src src, T *D, int n0, int n1, long ss1, long rs, int P, int w1, int w2, LineOut out) {
     using namespace part;
     __shared__ RedLDS<T, SX, 1> red;
@@ -520,7 +529,7 @@ int ptdma_part_src_launch(hipStream_t st, const char *label, const Src &src, T *
     // thermal y 2.63 -> 2.00 ms, dU y 1.52 -> 1.02 ms, dW y 1.43 -> 1.29 ms
     // at 512^3; the dU z stage, 4.14 -> 3.14 ms); the FP32 lines of the
     // mixed form keep the guarded loop (1.5 ms slower per step with it)
-    if (Src::full_rows && sizeof(T) == 8 && w1 == PT_MMAX && w2 == 0)
+    if (Src::full_rows && (sizeof(T) == 8 || Src::full_rows32) && w1 == PT_MMAX && w2 == 0)
         TCS_LAUNCH(st, label, (k_ptdma_part_src<T, CYC, Src, SX, true>), grd, blk, 0, src, D, n0, n1, ss1, rs, P, w1, w2,
                    out);
     else

@@ -217,7 +217,6 @@ struct ThermalZ {
     double dt;
     const double *T, *U, *V, *W, *TBCup, *TBCbt;
     static constexpr int NF = 6;  // field inputs, in the order below
-    static constexpr bool F32 = false;  // FP32 rows in the mixed form
     static constexpr bool FULL = false;  // unconditional row loop in the FP64 z solve
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
@@ -289,7 +288,6 @@ struct ThermalX {
     double dt;
     const double *T, *U;
     static constexpr int NF = 2;  // field inputs, in the order below
-    static constexpr bool F32 = false;  // FP32 rows in the mixed form
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         thermal_x_row<S>(g, p, dt, w(0, T), w(1, U), i, j, k, a, b, c);
@@ -357,12 +355,15 @@ __device__ __forceinline__ void thermal_y_row(const Grid &g, const Phys &p, doub
     am = eAMJ * sc<S>(dt);
 }
 struct ThermalY {
+    // FP32 lines of the mixed form at four waves per SIMD (2.54 -> 1.84 ms at
+    // 512^3; the other stages lose or gain nothing with it)
+    static constexpr int W32 = 4;
+    static constexpr bool FULL32 = false;
     Grid g;
     Phys p;
     double dt;
     const double *T, *V;
     static constexpr int NF = 2;  // field inputs, in the order below
-    static constexpr bool F32 = false;  // FP32 rows in the mixed form
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         thermal_y_row<S>(g, p, dt, w(0, T), w(1, V), i, j, k, a, b, c);
@@ -422,6 +423,8 @@ int thermal_coeffi(tcs_ctx *c) {
 }
 
 int thermal_solver(tcs_ctx *c, double dt) {
+    TCS_TRY(ensure_twins(c, TWIN_T | TWIN_U | TWIN_V | TWIN_W | TWIN_WALL));  // the mixed form's FP32 rows
+    c->twin_ok &= ~(unsigned)TWIN_T;
     const Grid g = c->grid();
     double *T = c->f[TCS_T];
     scr_t *am = SCR(c->am), *ac = SCR(c->ac), *ap = SCR(c->ap), *rhs = SCR(c->rhs);
@@ -430,7 +433,7 @@ int thermal_solver(tcs_ctx *c, double dt) {
     // The x rows evaluate seven property polynomials each: made by the
     // workgroup inside the contiguous-line solve they leave it waiting on
     // them, so here they go through the scratch as a kernel of their own.
-    TCS_TRY(rows3d_launch<false>(c->stream, "thermal_x", ThermalX{g, c->phys, dt, T, c->f[TCS_U]}, 1, am, ac, ap, rhs));
+    TCS_TRY(rows3d_launch<false>(c, "thermal_x", ThermalX{g, c->phys, dt, T, c->f[TCS_U]}, 1, am, ac, ap, rhs));
     TCS_TRY(ptdma_sweep(c, am, ac, ap, rhs, c->n2m, c->n1m, c->n3m, (long)c->n1m * c->n2m, c->n1m, 1, true));
     const ThermalY y{g, c->phys, dt, T, c->f[TCS_V]};
     if (rows_in_solve<scr_t>(c, c->n2m)) {
@@ -441,7 +444,7 @@ int thermal_solver(tcs_ctx *c, double dt) {
                    interior_block(), 0, g, 1, T, rhs);
         return TCS_OK;
     }
-    TCS_TRY(rows3d_launch<false>(c->stream, "thermal_y", y, 1, am, ac, ap, rhs));
+    TCS_TRY(rows3d_launch<false>(c, "thermal_y", y, 1, am, ac, ap, rhs));
     return sweep_y_field(c, 1, c->n2m, T, 2);
 }
 
@@ -451,12 +454,14 @@ int thermal_solver(tcs_ctx *c, double dt) {
 int thermal_solver_into(tcs_ctx *c, double dt, double *Tn, bool *written) {
     *written = false;
     if (!rows_in_solve<scr_t>(c, c->n2m)) return thermal_solver(c, dt);
+    TCS_TRY(ensure_twins(c, TWIN_T | TWIN_U | TWIN_V | TWIN_W | TWIN_WALL));  // the mixed form's FP32 rows
+    c->twin_ok &= ~(unsigned)TWIN_T;
     const Grid g = c->grid();
     double *T = c->f[TCS_T];
     scr_t *am = SCR(c->am), *ac = SCR(c->ac), *ap = SCR(c->ap), *rhs = SCR(c->rhs);
     const ThermalZ z{g, c->phys, dt, T, c->f[TCS_U], c->f[TCS_V], c->f[TCS_W], c->wall[TCS_TBCUP], c->wall[TCS_TBCBT]};
     TCS_TRY(z_stage(c, "thermal_z", "thermal_z_solve", z, 1, am, ac, ap, rhs));
-    TCS_TRY(rows3d_launch<false>(c->stream, "thermal_x", ThermalX{g, c->phys, dt, T, c->f[TCS_U]}, 1, am, ac, ap, rhs));
+    TCS_TRY(rows3d_launch<false>(c, "thermal_x", ThermalX{g, c->phys, dt, T, c->f[TCS_U]}, 1, am, ac, ap, rhs));
     TCS_TRY(ptdma_sweep(c, am, ac, ap, rhs, c->n2m, c->n1m, c->n3m, (long)c->n1m * c->n2m, c->n1m, 1, true));
     TCS_TRY(copy_shell(c, T, Tn));
     const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m;
@@ -469,7 +474,7 @@ int thermal_solver_into(tcs_ctx *c, double dt, double *Tn, bool *written) {
     out.ss1 = (long)c->P1 * c->N2;
     out.rs = c->P1;
     TCS_TRY(ptdma_part_src_launch<false>(c->stream, "thermal_y_solve", YLineRows<ThermalY, scr_t>{ThermalY{g, c->phys, dt,
-                                         T, c->f[TCS_V]}, rhs, 1}, rhs, c->n1m, c->n3m, n12, c->n2m, n1, out));
+                                         T, c->f[TCS_V]}, twin_w(c), rhs, 1}, rhs, c->n1m, c->n3m, n12, c->n2m, n1, out));
     *written = true;
     return TCS_OK;
 }

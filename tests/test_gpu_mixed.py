@@ -1,17 +1,20 @@
-"""The mixed form (precision 2): FP32 ADI scratch and FP32 line solves for
-the increments dT, dU, dV, dW; FP64 state, FP64 pressure (all wavenumbers,
-the singular kx = kz = 0 mode included), FP64 divergence / CFL / Nusselt
-reductions.  Checked against the oracle (FP64, reference order, split line
-solves) from a seeded conduction state: per-field relative error
-max|mixed - oracle| / max|oracle| after every step, and the wall Nusselt
-numbers.
+"""The mixed form (precision 2): the stencil right-hand sides and rows of
+every ADI stage and the block corrections run in FP32 on FP32 copies of the
+arrays they read (state, tables, walls), the ADI solves run in FP32 and the
+increments dU, dV, dW are kept in FP32 between the stages; the state T, U,
+V, W, P is accumulated in FP64 (U + dU in FP64), and the pressure (every
+wavenumber, the singular kx = kz = 0 mode included), the divergence, CFL
+and Nusselt reductions stay FP64.  Checked against the oracle (FP64,
+reference order, split line solves) from a seeded conduction state: per-field
+relative error max|mixed - oracle| / max|oracle| after every step, and the
+wall Nusselt numbers.
 
 The bounds are per field.  T carries the FP32 rounding of its increment
-only (1e-9 to 1.1e-8 after 10 steps here; the FP32 line solves run with the
-form's fast arithmetic, reciprocal division and contraction).  The velocity predictor's increments are O(dt grad P)
-and mostly cancelled by the FP64 projection, so their FP32 rounding is
-large against the small net change of V and W in these early steps (first
-measured at 2e-6 after 10 steps); the pressure follows the velocities."""
+(5.8e-9 to 1.6e-8 after 10 steps here).  The velocity predictor's
+increments are O(dt grad P) and mostly cancelled by the FP64 projection, so
+their FP32 rounding is large against the small net change of V and W in
+these early steps (measured 2.5e-6 after 10 steps); the pressure follows the
+velocities."""
 import numpy as np
 import pytest
 
@@ -66,3 +69,30 @@ def test_mixed_rejects_long_lines():
     cfg = small_config(1024, 8, 8)
     with pytest.raises(_lib.TCSError):
         TCSSolver(cfg, tdma_split=1, precision=2)
+
+
+def test_mixed_increments_round_trip_fp64_arrays():
+    """The mixed form keeps dU, dV, dW in FP32: an upload rounds them once,
+    a download gives that FP32 value back in FP64, and after a step the
+    downloaded increments agree with the plain form's within the FP32 bound
+    of the step."""
+    from pascal_tcs_amd.solver import TCSSolver
+    from tests.cases import load_gpu, random_state
+    cfg = small_config(32, 20, 16)
+    st, walls = random_state(cfg, seed=11)
+    s = TCSSolver(cfg, tdma_split=1, precision=2)
+    load_gpu(s, st, walls)
+    for n in ("dU", "dV", "dW"):
+        got = s.get(n)
+        assert np.array_equal(got, st[n].astype(np.float32).astype(np.float64)), n
+    p = TCSSolver(cfg, tdma_split=1, precision=1)
+    load_gpu(p, st, walls)
+    s.step(cfg.dtStart)
+    p.step(cfg.dtStart)
+    for n in ("dU", "dV", "dW"):
+        a, b = s.get(n), p.get(n)
+        rel = _rel(a, b)
+        print(f"{n}: mixed vs plain increment after one step {rel:.3e}")
+        assert rel <= 1e-4, (n, rel)
+    s.close()
+    p.close()
