@@ -388,7 +388,7 @@ def main():
     ap.add_argument("--deltat", type=float, default=0.2, help="DeltaT of the property polynomials")
     ap.add_argument("--precision", type=int, default=1, choices=(0, 1, 2),
                     help="0: stencils in the reference's operation order; 1: FMA contraction and reciprocals "
-                         "(within 1e-12 per step of the reference order); 2: mixed form, FP32 ADI scratch and "
+                         "(within 1e-12 per step of the reference order); 2: mixed form, FP32 stage rows, line solves and "
                          "line solves for the increments, FP64 state and pressure")
     ap.add_argument("--tdma-split", type=int, default=1, choices=(0, 1),
                     help="1: ADI lines split into parts as PaScaL_TDMA splits them over ranks, reduced "
@@ -453,7 +453,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision < 2 else "f64 state / f32 ADI",
+        "dtype": "f64" if args.precision < 2 else "f64 state / f32 stencils and ADI",
         "data": "synthetic",
         "config": {"workload": f"plain FP64 MPM-STD step, {cfg.n1m}x{cfg.n2m}x{cfg.n3m}, tanh y (gamma 2.0), "
                                f"Ra=1e8, Pr=0.71, water properties DeltaT={cfg.DeltaT:g}, T noise 0.01, "
@@ -461,7 +461,7 @@ def main():
                    "cells": cells,
                    "form": {0: "plain FP64, reference operation order",
                             1: "plain FP64, contracted/reciprocal stencil arithmetic",
-                            2: "mixed: FP32 ADI scratch and line solves, FP64 state/pressure/reductions"}[args.precision] + (
+                            2: "mixed: FP32 stage rows, block corrections, line solves and increments; FP64 state/pressure/reductions"}[args.precision] + (
                        "; ADI lines split into 32 parts per line as PaScaL_TDMA splits them over ranks, "
                        "reduced systems by parallel cyclic reduction"
                        if args.tdma_split else "; single-rank line solves")},
@@ -495,7 +495,8 @@ def main():
         over += [w for w, e in zip(("Nu_bottom", "Nu_top"), nu_err) if e > MIXED_TARGET_REL]
         out["mixed"] = {"value": round(v2, 2), "unit": "Mcell-steps/s", "ms_per_step": round(sec2 / args.steps * 1e3, 3),
                         "speedup_vs_plain": round(v2 / value, 3),
-                        "form": "FP32 ADI scratch and line solves for the increments; FP64 state, pressure, "
+                        "form": "FP32 stage rows and block corrections (from FP32 copies of the arrays they read), FP32 line solves "
+                                "and increments dU/dV/dW; FP64 state accumulation, pressure, "
                                 "divergence/CFL/Nusselt reductions",
                         "rel_error_vs_plain_after_steps": err, "Nu_rel_error_vs_plain": nu_err,
                         "steps_compared": args.warmup + args.steps + args.profile_steps,

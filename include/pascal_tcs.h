@@ -42,7 +42,8 @@ typedef struct tcs_params {
     int32_t pbc1, pbc2, pbc3;     /* periodicity; only (1,0,1) is supported */
     int32_t precision;            /* 0: FP64 in the reference's operation order;
                                      1: FP64 with contraction and reciprocals;
-                                     2: mixed, FP32 ADI scratch and line solves */
+                                     2: mixed, FP32 stage rows, line solves and
+                                        increments, FP64 state and pressure */
     int32_t tdma_split;           /* 0: every ADI line solved as on one rank;
                                      1: lines of up to 512 rows split into
                                      parts as PaScaL_TDMA splits them over
@@ -117,11 +118,15 @@ int tcs_field_download(tcs_ctx *ctx, int32_t id, double *host);
    tcs_post_div keeps the CFL maximum of the same pass for tcs_post_cfl
    until an entry point that may change U, V or W runs; a caller that
    writes a field through this pointer calls tcs_fields_touched before the
-   next tcs_post_cfl. */
+   next tcs_post_cfl.  In the mixed form (precision 2) the increments dU, dV,
+   dW live in FP32: their FP64 arrays are filled from the FP32 values when
+   the pointer is handed out (and by downloads), and writes through it reach
+   the form after tcs_fields_touched. */
 int tcs_field_device_ptr(tcs_ctx *ctx, int32_t id, double **dptr);
 /* Declares that fields were written outside the entry points (through
-   tcs_field_device_ptr): drops the CFL maximum kept by tcs_post_div.  No
-   counterpart in the reference (its fields are host arrays). */
+   tcs_field_device_ptr): drops the CFL maximum kept by tcs_post_div and, in
+   the mixed form, refreshes its FP32 copies and increments from the FP64
+   arrays.  No counterpart in the reference (its fields are host arrays). */
 int tcs_fields_touched(tcs_ctx *ctx);
 /* row pitch (elements) of the full fields on the device (no counterpart in
    the reference: the device layout of tcs_field_device_ptr) */

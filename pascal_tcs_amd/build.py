@@ -6,7 +6,8 @@ FMA fusion may change a rounding) and packed-FP32 instructions disabled, then
 linked into one shared library next to this file.  The stage units
 (thermal, momentum, pressure) are compiled twice more: as the `fast`
 arithmetic form (contraction and reciprocals allowed; precision 1) and as
-the `mixed` form (fast arithmetic, FP32 ADI scratch; precision 2).  The Fortran drop-in
+the `mixed` form (fast arithmetic, FP32 stage rows, ADI scratch and increments;
+precision 2).  The Fortran drop-in
 shells under fortran/ are compiled too when flang is present.
 """
 import os
@@ -33,7 +34,7 @@ def hipcc():
 def flags(form="exact"):
     # exact form: no contraction, IEEE division; fast form: FMA contraction,
     # reciprocals and reassociation (inf/NaN semantics kept); mixed form:
-    # the fast arithmetic with FP32 ADI scratch
+    # the fast arithmetic with FP32 rows, ADI scratch and increments
     fastm = ["-ffp-contract=fast", "-ffast-math", "-fno-finite-math-only"]
     arith = {"exact": ["-ffp-contract=off", "-fno-fast-math"],
              "fast": fastm + ["-DTCS_FORM=fast", "-DTCS_RECIP"],
