@@ -48,6 +48,107 @@ __device__ __forceinline__ Dual operator/(double a, Dual b) {
     return Dual(q, -q * b.d / b.v);
 }
 
+// Forward-mode numbers with K derivative components: one evaluation gives K
+// Jacobian columns (K seeded inputs), the value arithmetic done once.
+template <int K>
+struct DualK {
+    double v, d[K];
+    __host__ __device__ DualK() : v(0.0) {
+        for (int q = 0; q < K; q++) d[q] = 0.0;
+    }
+    __host__ __device__ DualK(double x) : v(x) {
+        for (int q = 0; q < K; q++) d[q] = 0.0;
+    }
+};
+template <int K>
+__device__ __forceinline__ DualK<K> operator-(const DualK<K> &a) {
+    DualK<K> r(-a.v);
+#pragma unroll
+    for (int q = 0; q < K; q++) r.d[q] = -a.d[q];
+    return r;
+}
+template <int K>
+__device__ __forceinline__ DualK<K> operator+(const DualK<K> &a, const DualK<K> &b) {
+    DualK<K> r(a.v + b.v);
+#pragma unroll
+    for (int q = 0; q < K; q++) r.d[q] = a.d[q] + b.d[q];
+    return r;
+}
+template <int K>
+__device__ __forceinline__ DualK<K> operator-(const DualK<K> &a, const DualK<K> &b) {
+    DualK<K> r(a.v - b.v);
+#pragma unroll
+    for (int q = 0; q < K; q++) r.d[q] = a.d[q] - b.d[q];
+    return r;
+}
+template <int K>
+__device__ __forceinline__ DualK<K> operator*(const DualK<K> &a, const DualK<K> &b) {
+    DualK<K> r(a.v * b.v);
+#pragma unroll
+    for (int q = 0; q < K; q++) r.d[q] = a.d[q] * b.v + a.v * b.d[q];
+    return r;
+}
+template <int K>
+__device__ __forceinline__ DualK<K> operator/(const DualK<K> &a, const DualK<K> &b) {
+    DualK<K> r(a.v / b.v);
+#pragma unroll
+    for (int q = 0; q < K; q++) r.d[q] = (a.d[q] - r.v * b.d[q]) / b.v;
+    return r;
+}
+template <int K>
+__device__ __forceinline__ DualK<K> operator+(const DualK<K> &a, double b) {
+    DualK<K> r = a;
+    r.v = a.v + b;
+    return r;
+}
+template <int K>
+__device__ __forceinline__ DualK<K> operator+(double a, const DualK<K> &b) {
+    DualK<K> r = b;
+    r.v = a + b.v;
+    return r;
+}
+template <int K>
+__device__ __forceinline__ DualK<K> operator-(const DualK<K> &a, double b) {
+    DualK<K> r = a;
+    r.v = a.v - b;
+    return r;
+}
+template <int K>
+__device__ __forceinline__ DualK<K> operator-(double a, const DualK<K> &b) {
+    DualK<K> r(a - b.v);
+#pragma unroll
+    for (int q = 0; q < K; q++) r.d[q] = -b.d[q];
+    return r;
+}
+template <int K>
+__device__ __forceinline__ DualK<K> operator*(const DualK<K> &a, double b) {
+    DualK<K> r(a.v * b);
+#pragma unroll
+    for (int q = 0; q < K; q++) r.d[q] = a.d[q] * b;
+    return r;
+}
+template <int K>
+__device__ __forceinline__ DualK<K> operator*(double a, const DualK<K> &b) {
+    DualK<K> r(a * b.v);
+#pragma unroll
+    for (int q = 0; q < K; q++) r.d[q] = a * b.d[q];
+    return r;
+}
+template <int K>
+__device__ __forceinline__ DualK<K> operator/(const DualK<K> &a, double b) {
+    DualK<K> r(a.v / b);
+#pragma unroll
+    for (int q = 0; q < K; q++) r.d[q] = a.d[q] / b;
+    return r;
+}
+template <int K>
+__device__ __forceinline__ DualK<K> operator/(double a, const DualK<K> &b) {
+    DualK<K> r(a / b.v);
+#pragma unroll
+    for (int q = 0; q < K; q++) r.d[q] = -r.v * b.d[q] / b.v;
+    return r;
+}
+
 // ---- field access ----------------------------------------------------------
 // the step: raw pointers
 struct RawW {
@@ -77,6 +178,11 @@ inline float *twin(const tcs_ctx *c, const double *p) {
 
 // the cell's input set: (field, index) of every load, without repeats
 constexpr int AD_MAXIN = 128;
+// inputs whose Jacobian columns one forward-mode evaluation carries
+#ifndef AD_K_DEFAULT
+#define AD_K_DEFAULT 4
+#endif
+constexpr int AD_K = AD_K_DEFAULT;
 struct Rec {
     int n;
     int fid[AD_MAXIN];
@@ -91,19 +197,40 @@ struct Rec {
         n++;  // n > AD_MAXIN reports an overflow
     }
 };
-struct RecPtr {
+// The same set kept in LDS for the 64 lanes of a wavefront (k_cell_vjp):
+// one 64-bit key (field << 56 | index) per entry, a lane's entry r at
+// key[64 r]; the scan for repeats reads LDS instead of private memory.
+constexpr int AD_MAXIN_L = 64;
+struct RecL {
+    unsigned long long *key;
+    int n;
+    __device__ void add(int f, long i) {
+        const unsigned long long k = ((unsigned long long)f << 56) | (unsigned long long)i;
+        for (int r = 0; r < n && r < AD_MAXIN_L; r++)
+            if (key[64 * r] == k) return;
+        if (n < AD_MAXIN_L) key[64 * n] = k;
+        n++;  // n > AD_MAXIN_L reports an overflow
+    }
+    __device__ __forceinline__ int fid(int r) const { return (int)(key[64 * r] >> 56); }
+    __device__ __forceinline__ long idx(int r) const { return (long)(key[64 * r] & ((1ull << 56) - 1)); }
+};
+template <class R>
+struct RecPtrT {
     const double *p;
     int f;
-    Rec *rec;
+    R *rec;
     __device__ __forceinline__ double operator[](size_t i) const {
         rec->add(f, (long)i);
         return p[i];
     }
 };
-struct RecW {
-    Rec *rec;
-    __device__ __forceinline__ RecPtr operator()(int f, const double *p) const { return RecPtr{p, f, rec}; }
+template <class R>
+struct RecWT {
+    R *rec;
+    __device__ __forceinline__ RecPtrT<R> operator()(int f, const double *p) const { return RecPtrT<R>{p, f, rec}; }
 };
+using RecPtr = RecPtrT<Rec>;
+using RecW = RecWT<Rec>;
 
 // one input seeded with derivative 1
 struct SeedPtr {
@@ -116,6 +243,32 @@ struct SeedW {
     long idx;
     __device__ __forceinline__ SeedPtr operator()(int ff, const double *p) const {
         return SeedPtr{p, ff == f ? idx : -1};
+    }
+};
+
+// K inputs seeded at once (field f[q], index idx[q] carries derivative
+// component q; f[q] < 0: component q unused)
+template <int K>
+struct SeedKPtr {
+    const double *p;
+    long idx[K];
+    __device__ __forceinline__ DualK<K> operator[](size_t i) const {
+        DualK<K> r(p[i]);
+#pragma unroll
+        for (int q = 0; q < K; q++) r.d[q] = (long)i == idx[q] ? 1.0 : 0.0;
+        return r;
+    }
+};
+template <int K>
+struct SeedKW {
+    int f[K];
+    long idx[K];
+    __device__ __forceinline__ SeedKPtr<K> operator()(int ff, const double *p) const {
+        SeedKPtr<K> s;
+        s.p = p;
+#pragma unroll
+        for (int q = 0; q < K; q++) s.idx[q] = ff == f[q] ? idx[q] : -1;
+        return s;
     }
 };
 
@@ -132,9 +285,32 @@ struct InBars {
     double *p[NF];
 };
 
+// *p += s from every lane, with the FP64 atomic add of the L2 (unsafeAtomicAdd:
+// HIP's atomicAdd on a double is a compare-and-swap loop, which neighbouring
+// cells adding to shared inputs kept retrying: 224 -> 11 ms per adjoint step
+// at 128^3).  The lanes of a wavefront are 64 cells of one row (j, k); when
+// all 64 are here and add to one address (an input every cell of the row
+// reads: a (j, k) table entry, a stale scalar of the momentum stages) their
+// sum is added once.  Otherwise each lane adds its own (skipping zeros).
+// The adjoint arrays are device allocations (coarse-grained), which the
+// L2's FP64 atomics serve.
+__device__ __forceinline__ void add_to(double *p, double s) {
+    const unsigned long long full = ~0ull;
+    const unsigned long long first = (unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(size_t)p) |
+                                     ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)((size_t)p >> 32)) << 32);
+    const bool all_here = __builtin_amdgcn_read_exec() == full;
+    if (all_here && __ballot((size_t)p == first) == full) {
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o);
+        if (threadIdx.x % 64 == 0 && s != 0.0) unsafeAtomicAdd(p, s);
+    } else if (s != 0.0) {
+        unsafeAtomicAdd(p, s);
+    }
+}
+
 // Cells (i, j, k), i in [1, n1m], j in [jlo, jhi], k in [1, n3m]; the cell's
 // eval(w, i, j, k, a, b, c, d) gives up to four outputs.  flag[0] is raised
-// if a cell reads more than AD_MAXIN distinct inputs.
+// if a cell reads more than AD_MAXIN_L distinct inputs.
 template <class Cell>
 __global__ void __launch_bounds__(64)
 k_cell_vjp(Cell cell, int jlo, int jhi, OutBars ob, InBars<Cell::NF> ib, int *flag) {
@@ -152,23 +328,34 @@ k_cell_vjp(Cell cell, int jlo, int jhi, OutBars ob, InBars<Cell::NF> ib, int *fl
         any = any || w[o] != 0.0;
     }
     if (!any) return;
-    Rec rec;
-    rec.n = 0;
+    __shared__ unsigned long long keys[64 * AD_MAXIN_L];
+    RecL rec{keys + threadIdx.x, 0};
     {
         double a, b, c, d = 0.0;
-        cell.template eval<double>(RecW{&rec}, i, j, k, a, b, c, d);
+        cell.template eval<double>(RecWT<RecL>{&rec}, i, j, k, a, b, c, d);
     }
-    if (rec.n > AD_MAXIN) {
+    if (rec.n > AD_MAXIN_L) {
         atomicOr(flag, 1);
         return;
     }
-    for (int r = 0; r < rec.n; r++) {
-        double *dst = ib.p[rec.fid[r]];
-        if (!dst) continue;
-        Dual a, b, c, d;
-        cell.template eval<Dual>(SeedW{rec.fid[r], rec.idx[r]}, i, j, k, a, b, c, d);
-        const double s = a.d * w[0] + b.d * w[1] + c.d * w[2] + d.d * w[3];
-        if (s != 0.0) atomicAdd(dst + rec.idx[r], s);
+    // AD_K inputs a pass: their Jacobian columns from one evaluation
+    for (int r0 = 0; r0 < rec.n; r0 += AD_K) {
+        SeedKW<AD_K> sw;
+#pragma unroll
+        for (int q = 0; q < AD_K; q++) {
+            const int r = r0 + q;
+            const bool in = r < rec.n && ib.p[rec.fid(r < rec.n ? r : 0)];
+            sw.f[q] = in ? rec.fid(r) : -1;
+            sw.idx[q] = in ? rec.idx(r) : -1;
+        }
+        DualK<AD_K> a, b, c, d;
+        cell.template eval<DualK<AD_K>>(sw, i, j, k, a, b, c, d);
+#pragma unroll
+        for (int q = 0; q < AD_K; q++) {
+            if (sw.f[q] < 0) continue;
+            const double s = a.d[q] * w[0] + b.d[q] * w[1] + c.d[q] * w[2] + d.d[q] * w[3];
+            add_to(ib.p[sw.f[q]] + sw.idx[q], s);
+        }
     }
 }
 
