@@ -375,6 +375,36 @@ def run_adjoint_case(n, nsteps, every, ndir, deltat):
     return out
 
 
+def run_adjoint_step(n, deltat):
+    """One adjoint step at n^3 (the config-4 workload, precision 0, split
+    lines): wall time of one plain forward step and of one tcs_adjoint_step
+    (which runs the step forward keeping what the backward pass needs, then
+    the backward pass), each after one untimed call."""
+    import torch
+    from pascal_tcs_amd.adjoint import StepAdjoint
+    from pascal_tcs_amd.config import Properties, TCSConfig
+    from pascal_tcs_amd.solver import TCSSolver
+    cfg = TCSConfig(n1m=n, n2m=n, n3m=n, uniform2=1, Ra=1e7, Pr=0.71, DeltaT=deltat, props=Properties.water(),
+                    dtStart=1e-3)
+    s = TCSSolver(cfg, tdma_split=1, precision=0)
+    s.setup_seeded(amp=0.05, seed=20240512)
+    a = StepAdjoint(s)
+    a.set_bar("T", np.random.default_rng(1).standard_normal(s.shape))
+    out = {}
+    for label, fn in (("forward_step_ms", lambda: s.step(cfg.dtStart)), ("adjoint_step_ms", lambda: a.step(cfg.dtStart))):
+        fn()
+        s.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        s.sync()
+        torch.cuda.synchronize()
+        out[label] = round((time.perf_counter() - t0) * 1e3, 3)
+    s.close()
+    out["ratio"] = round(out["adjoint_step_ms"] / out["forward_step_ms"], 2)
+    return {"metric": f"one adjoint step at {n}^3 against one forward step", **out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -407,6 +437,8 @@ def main():
     ap.add_argument("--mixed-drift", type=int, default=0, metavar="STEPS",
                     help="instead of the step: run the mixed and plain forms side by side for STEPS steps "
                          "and report the mixed form's per-field errors and Nusselt drift")
+    ap.add_argument("--adjoint-step", action="store_true",
+                    help="instead of the step: one adjoint step against one forward step (n^3, default 128)")
     ap.add_argument("--adjoint", action="store_true",
                     help="instead of the step: the adjoint gradient case (n^3, default 128, 50 steps) checked "
                          "against central differences")
@@ -420,6 +452,9 @@ def main():
 
     if args.tdma:
         print(json.dumps(run_tdma_microbench(args.n)))
+        return
+    if args.adjoint_step:
+        print(json.dumps(run_adjoint_step(args.n if args.n != 512 else 128, args.deltat)))
         return
     if args.adjoint:
         n = args.n if args.n != 512 else 128
