@@ -606,9 +606,12 @@ struct DPhatUpdate {
 };
 
 // the projection with the new dPhat and dP0 written to second buffers
+// TW (the mixed form): U, V, W, P's FP32 copies (tw[0..3]) of the updated
+// cells written as well
+template <bool TW>
 __global__ void __launch_bounds__(256) k_projection_into(Projection cell, DPhatUpdate hist, double *U, double *V,
                                                         double *W, double *P, double *__restrict__ dPhatn,
-                                                        double *__restrict__ dP0n) {
+                                                        double *__restrict__ dP0n, FieldSet32 tw) {
     const Grid &g = cell.g;
     const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * PROJ_IB + 1;  // n1m is a multiple of PROJ_IB
     const int j = wave_row_j();
@@ -634,6 +637,12 @@ __global__ void __launch_bounds__(256) k_projection_into(Projection cell, DPhatU
         P[o] = q[c];
         dP0n[o] = d0[c];
         dPhatn[o] = h[c];
+        if (TW) {
+            tw.f[0][o] = (float)u[c];
+            tw.f[1][o] = (float)v[c];
+            tw.f[2][o] = (float)w[c];
+            tw.f[3][o] = (float)q[c];
+        }
     }
 }
 
@@ -783,9 +792,15 @@ int pressure_projection_into(tcs_ctx *c, double dt, double *dPhatn, double *dP0n
     const Projection cell{g, c->phys, dt, c->f[TCS_INVRHO], c->f[TCS_DP], c->f[TCS_DPHAT], c->f[TCS_DP0],
                           c->f[TCS_U], c->f[TCS_V], c->f[TCS_W], c->f[TCS_P]};
     TCS_TRY(copy_shell(c, {{c->f[TCS_DPHAT], dPhatn}, {c->f[TCS_DP0], dP0n}}));
-    TCS_LAUNCH(c->stream, "projection", k_projection_into, grd, blk, 0, cell,
-               DPhatUpdate{g, c->f[TCS_DP], c->f[TCS_DPHAT], c->f[TCS_DP0]}, c->f[TCS_U], c->f[TCS_V], c->f[TCS_W],
-               c->f[TCS_P], dPhatn, dP0n);
+    const DPhatUpdate hist{g, c->f[TCS_DP], c->f[TCS_DPHAT], c->f[TCS_DP0]};
+    if (c->a32) {
+        const FieldSet32 tw{{twin(c, c->f[TCS_U]), twin(c, c->f[TCS_V]), twin(c, c->f[TCS_W]), twin(c, c->f[TCS_P])}};
+        TCS_LAUNCH(c->stream, "projection", k_projection_into<true>, grd, blk, 0, cell, hist, c->f[TCS_U], c->f[TCS_V],
+                   c->f[TCS_W], c->f[TCS_P], dPhatn, dP0n, tw);
+    } else {
+        TCS_LAUNCH(c->stream, "projection", k_projection_into<false>, grd, blk, 0, cell, hist, c->f[TCS_U], c->f[TCS_V],
+                   c->f[TCS_W], c->f[TCS_P], dPhatn, dP0n, FieldSet32{});
+    }
     TCS_LAUNCH(c->stream, "wall_extrap", k_wall_extrap, dim3((c->n1m + 63) / 64, c->n3m), dim3(64), 0, g, c->f[TCS_P]);
     return TCS_OK;
 }

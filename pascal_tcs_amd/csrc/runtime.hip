@@ -138,9 +138,10 @@ __global__ void k_halo(Grid g, Set fs) {
 // the cells of a full field outside the interior: the faces i in {0, n1m+1}
 // (all j, k), then j in {0, n2m+1} (interior i, all k), then k in {0, n3m+1}
 // (interior i and j)
-__global__ void k_copy_shell(Grid g, FieldSet src_set, FieldSet dst_set) {
+template <typename E, class DstSet>
+__global__ void k_copy_shell(Grid g, FieldSet src_set, DstSet dst_set) {
     const double *__restrict__ src = src_set.f[blockIdx.y];  // one pair per grid row
-    double *__restrict__ dst = dst_set.f[blockIdx.y];
+    E *__restrict__ dst = dst_set.f[blockIdx.y];
     const int N1 = g.N1, N2 = g.N2, N3 = g.N3, n1m = g.n1m, n2m = g.n2m;
     const size_t fi = (size_t)2 * N2 * N3, fj = (size_t)2 * n1m * N3, fk = (size_t)2 * n1m * n2m;
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -160,7 +161,7 @@ __global__ void k_copy_shell(Grid g, FieldSet src_set, FieldSet dst_set) {
     } else {
         return;
     }
-    dst[o] = src[o];
+    dst[o] = (E)src[o];
 }
 
 int copy_shell(tcs_ctx *c, std::initializer_list<std::pair<const double *, double *>> pairs) {
@@ -172,11 +173,12 @@ int copy_shell(tcs_ctx *c, std::initializer_list<std::pair<const double *, doubl
         dst.f[nf++] = pr.second;
     }
     const size_t n = (size_t)2 * c->N2 * c->N3 + (size_t)2 * c->n1m * c->N3 + (size_t)2 * c->n1m * c->n2m;
-    TCS_LAUNCH(c->stream, "copy_shell", k_copy_shell, dim3((unsigned)((n + 255) / 256), (unsigned)nf), dim3(256), 0,
-               c->grid(), src, dst);
+    TCS_LAUNCH(c->stream, "copy_shell", (k_copy_shell<double, FieldSet>), dim3((unsigned)((n + 255) / 256), (unsigned)nf),
+               dim3(256), 0, c->grid(), src, dst);
     return TCS_OK;
 }
 int copy_shell(tcs_ctx *c, const double *src, double *dst) { return copy_shell(c, {{src, dst}}); }
+
 
 int halo(tcs_ctx *c, std::initializer_list<double *> fields) {
     FieldSet fs{};
@@ -192,9 +194,6 @@ int halo(tcs_ctx *c, std::initializer_list<double *> fields) {
 }
 int halo(tcs_ctx *c, double *f) { return halo(c, {f}); }
 
-struct FieldSet32 {
-    float *f[FieldSet::MAX];
-};
 int halo(tcs_ctx *c, std::initializer_list<float *> fields) {
     FieldSet32 fs{};
     int nf = 0;
@@ -209,6 +208,39 @@ int halo(tcs_ctx *c, std::initializer_list<float *> fields) {
 }
 
 // ------------------------------------------------- the mixed form's twins --
+// the TWIN_* bit of a field's FP32 copy in the mixed form
+static unsigned twin_bit(int id) {
+    switch (id) {
+    case TCS_T: return TWIN_T;
+    case TCS_U: return TWIN_U;
+    case TCS_V: return TWIN_V;
+    case TCS_W: return TWIN_W;
+    case TCS_P: return TWIN_P;
+    case TCS_MU: case TCS_INVRHO: return TWIN_TAB;
+    default: return 0;
+    }
+}
+
+// The cells outside the interior of fields into their FP32 twins (the
+// interior's twins written by the kernel that updated it): the twins of
+// those fields are current afterwards.
+static int shell_to_twins(tcs_ctx *c, std::initializer_list<int> ids) {
+    FieldSet src{};
+    FieldSet32 dst{};
+    int nf = 0;
+    unsigned bits = 0;
+    for (int id : ids) {
+        if (nf == FieldSet::MAX || !(dst.f[nf] = twin(c, c->f[id]))) return TCS_E_ARG;
+        src.f[nf++] = c->f[id];
+        bits |= twin_bit(id);
+    }
+    const size_t n = (size_t)2 * c->N2 * c->N3 + (size_t)2 * c->n1m * c->N3 + (size_t)2 * c->n1m * c->n2m;
+    TCS_LAUNCH(c->stream, "twin_shell", (k_copy_shell<float, FieldSet32>), dim3((unsigned)((n + 255) / 256), (unsigned)nf),
+               dim3(256), 0, c->grid(), src, dst);
+    c->twin_ok |= bits;
+    return TCS_OK;
+}
+
 __global__ void k_to_f32(const double *__restrict__ src, float *__restrict__ dst, size_t n) {
     const size_t s = (size_t)gridDim.x * blockDim.x;
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += s) dst[q] = (float)src[q];
@@ -810,19 +842,6 @@ extern "C" void *tcs_stream(tcs_ctx *c) {
     return (void *)c->stream;
 }
 
-// the TWIN_* bit of a field's FP32 copy in the mixed form
-static unsigned twin_bit(int id) {
-    switch (id) {
-    case TCS_T: return TWIN_T;
-    case TCS_U: return TWIN_U;
-    case TCS_V: return TWIN_V;
-    case TCS_W: return TWIN_W;
-    case TCS_P: return TWIN_P;
-    case TCS_MU: case TCS_INVRHO: return TWIN_TAB;
-    default: return 0;
-    }
-}
-
 static int check_field(tcs_ctx *c, int id) {
     if (!c || id < 0 || id >= TCS_NFIELD) {
         set_error("bad context or field id");
@@ -1152,6 +1171,8 @@ extern "C" int tcs_step(tcs_ctx *c, double dt) {
     RUN("thermal solve", FORM(c, thermal_solver_into, dt, c->spare[TCS_T], &swap_T));
     if (swap_T) std::swap(c->f[TCS_T], c->spare[TCS_T]);
     RUN(H, halo(c, c->f[TCS_T]));
+    // the mixed form: the last thermal sweep wrote T's FP32 copy inside; now its shell
+    if (c->a32 && swap_T) RUN(H, shell_to_twins(c, {TCS_T}));
     RUN("momentum coeff", FORM(c, momentum_coeffi));
     RUN("momentum solvedU", FORM(c, momentum_solvedU, dt));
     RUN(H, halo_field(c, TCS_DU));
@@ -1173,6 +1194,8 @@ extern "C" int tcs_step(tcs_ctx *c, double dt) {
     std::swap(c->f[TCS_DPHAT], c->spare[TCS_DPHAT]);
     std::swap(c->f[TCS_DP0], c->spare[TCS_DP0]);
     RUN(H, halo(c, {c->f[TCS_U], c->f[TCS_V], c->f[TCS_W], c->f[TCS_P], c->f[TCS_DPHAT]}));
+    // the mixed form: the projection wrote the FP32 copies of U, V, W, P inside
+    if (c->a32) RUN(H, shell_to_twins(c, {TCS_U, TCS_V, TCS_W, TCS_P}));
     return TCS_OK;
 }
 

@@ -208,7 +208,8 @@ int ptdma_exact_launch(double *A, const double *B, double *C, double *D,
 struct LineOut {
     double *F = nullptr;
     const double *Fin = nullptr;
-    float *G = nullptr;  // mode 4: the FP32 field G at the same place, stored
+    float *G = nullptr;  // mode 4: the FP32 field G at the same place, stored; modes 1-3 of FP32
+                         // lines: G (when set) also receives the FP32 copy of the FP64 value
     long base = 0, ss0 = 0, ss1 = 0, rs = 0;
     int mode = 0;
 };
@@ -272,6 +273,9 @@ namespace mixed { TCS_STAGE_DECLS }
 struct FieldSet {
     static constexpr int MAX = 6;
     double *f[MAX];
+};
+struct FieldSet32 {
+    float *f[FieldSet::MAX];
 };
 int halo(tcs_ctx *c, double *f);
 int halo(tcs_ctx *c, std::initializer_list<double *> fields);

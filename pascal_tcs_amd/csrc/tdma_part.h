@@ -290,7 +290,11 @@ __device__ __forceinline__ void store(const T (&a)[PT_MMAX], const T (&c)[PT_MMA
             if (r < m) {
                 const double x = (double)x_of<T, 1>(a, c, d, 0, r, m, xf, xl);
                 const long o = fb + (long)(j0 + r) * out.rs;
-                out.F[o] = out.mode == 1 ? x : (out.mode == 2 ? out.F[o] : out.Fin[o]) + x;
+                const double v = out.mode == 1 ? x : (out.mode == 2 ? out.F[o] : out.Fin[o]) + x;
+                out.F[o] = v;
+                if constexpr (sizeof(T) == sizeof(float)) {
+                    if (out.G) out.G[o] = (float)v;  // the mixed form's FP32 copy of the field
+                }
             }
         }
     }

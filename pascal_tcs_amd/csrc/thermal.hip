@@ -468,6 +468,7 @@ int thermal_solver_into(tcs_ctx *c, double dt, double *Tn, bool *written) {
     LineOut out;
     out.F = Tn;
     out.Fin = T;
+    out.G = twin(c, Tn);  // the mixed form: T's FP32 copy of the updated cells
     out.mode = 3;
     out.base = 1 + (long)c->P1 + (long)c->P1 * c->N2;  // (i, j, k) = (1, 1, 1)
     out.ss0 = 1;
