@@ -356,6 +356,12 @@ k_ptdma_part_src(Src src, T *D, int n0, int n1, long ss1, long rs, int P, int w1
             src(s0, s1, j0 + r, ar, br, cr, dr[0]);
             reduce_row<T, 1>(r, ar, br, cr, dr, a, c, d);
         }
+    } else if (FULL && sizeof(T) == sizeof(float)) {
+        // padding lines of FP32 solves: zero rows (the guarded loop below
+        // puts the FP32 arrays on the stack in this instantiation; the FP64
+        // ones keep it, dU y 1.02 against 1.14 ms with the zero rows)
+#pragma unroll
+        for (int r = 0; r < PT_MMAX; r++) { a[r] = T(0); c[r] = T(0); d[0][r] = T(0); }
     } else {
 #pragma unroll
         for (int r = 0; r < PT_MMAX; r++) {
