@@ -209,7 +209,8 @@ int tcs_nusselt(tcs_ctx *ctx, double Tup, double Tbt, double *nu_bottom, double 
    (to_adjoint = 0) a device buffer of its size: kind 0 with a state field
    id, kind 1 with a wall id.  tcs_adjoint_add_nusselt adds weight times
    the gradient of tcs_nusselt's bottom value with respect to T and TBCbt.
-   Returns TCS_E_ARG for precision != 0. */
+   Returns TCS_E_ARG for precision != 0, and from tcs_adjoint_step for
+   grids whose full fields hold 2^28 values or more (about 640^3). */
 int tcs_adjoint_zero(tcs_ctx *ctx);
 int tcs_adjoint_step(tcs_ctx *ctx, double dt);
 int tcs_adjoint_copy(tcs_ctx *ctx, int32_t kind, int32_t id, int32_t to_adjoint, double *dev);

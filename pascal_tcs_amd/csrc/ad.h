@@ -198,21 +198,25 @@ struct Rec {
     }
 };
 // The same set kept in LDS for the 64 lanes of a wavefront (k_cell_vjp):
-// one 64-bit key (field << 56 | index) per entry, a lane's entry r at
+// one 32-bit key (field << 28 | index) per entry, a lane's entry r at
 // key[64 r]; the scan for repeats reads LDS instead of private memory.
+// Fields < 16 and indexes < 2^28 (cell_vjp checks the arrays' sizes): 16 KB
+// of LDS a wavefront, so the LDS holds as many wavefronts as the registers
+// allow (64-bit keys, 32 KB, held five a CU).
 constexpr int AD_MAXIN_L = 64;
+constexpr int AD_KEY_FBITS = 4, AD_KEY_IBITS = 28;
 struct RecL {
-    unsigned long long *key;
+    unsigned *key;
     int n;
     __device__ void add(int f, long i) {
-        const unsigned long long k = ((unsigned long long)f << 56) | (unsigned long long)i;
+        const unsigned k = ((unsigned)f << AD_KEY_IBITS) | (unsigned)i;
         for (int r = 0; r < n && r < AD_MAXIN_L; r++)
             if (key[64 * r] == k) return;
         if (n < AD_MAXIN_L) key[64 * n] = k;
         n++;  // n > AD_MAXIN_L reports an overflow
     }
-    __device__ __forceinline__ int fid(int r) const { return (int)(key[64 * r] >> 56); }
-    __device__ __forceinline__ long idx(int r) const { return (long)(key[64 * r] & ((1ull << 56) - 1)); }
+    __device__ __forceinline__ int fid(int r) const { return (int)(key[64 * r] >> AD_KEY_IBITS); }
+    __device__ __forceinline__ long idx(int r) const { return (long)(key[64 * r] & ((1u << AD_KEY_IBITS) - 1)); }
 };
 template <class R>
 struct RecPtrT {
@@ -328,7 +332,8 @@ k_cell_vjp(Cell cell, int jlo, int jhi, OutBars ob, InBars<Cell::NF> ib, int *fl
         any = any || w[o] != 0.0;
     }
     if (!any) return;
-    __shared__ unsigned long long keys[64 * AD_MAXIN_L];
+    static_assert(Cell::NF <= (1 << AD_KEY_FBITS), "input field ids must fit the LDS keys");
+    __shared__ unsigned keys[64 * AD_MAXIN_L];
     RecL rec{keys + threadIdx.x, 0};
     {
         double a, b, c, d = 0.0;

@@ -262,6 +262,9 @@ __global__ void k_row_bars(Grid g, int axis, int jlo, int cyc, const double *__r
 
 template <class Cell>
 int cell_vjp(tcs_ctx *c, AdjWork *w, const Cell &cell, int jlo, OutBars ob, InBars<Cell::NF> ib) {
+    // the LDS input sets keep an index in 28 bits: the largest array (a full
+    // field) must be indexed within it
+    if (c->nfull + FIELD_FRONT + 1 > ((size_t)1 << AD_KEY_IBITS)) return TCS_E_ARG;
     const dim3 blk(64), grd((unsigned)((c->n1m + 63) / 64), (unsigned)(c->n2m - jlo + 1), (unsigned)c->n3m);
     TCS_LAUNCH(c->stream, "adj_cell_vjp", (k_cell_vjp<Cell>), grd, blk, 0, cell, jlo, c->n2m, ob, ib, w->flag);
     return TCS_OK;
