@@ -411,9 +411,10 @@ __device__ __forceinline__ void dV_z_row(const Grid &g, const Phys &p, double dt
 }
 struct DVZ {
     // FP32 lines of the mixed form at four waves per SIMD (3.09 -> 2.97 ms at
-    // 512^3)
+    // 512^3), 32 lines a workgroup (2.97 -> 2.87 ms)
     static constexpr int W32 = 4;
     static constexpr bool FULL32 = false;
+    static constexpr int SX32 = 32;
     Grid g;
     Phys p;
     double dt;
@@ -608,6 +609,9 @@ __device__ __forceinline__ void dW_z_row(const Grid &g, const Phys &p, double dt
     rhsout = rhs * sc<S>(dt);
 }
 struct DWZ {
+    // FP32 lines of the mixed form: 32 lines a workgroup, four waves per SIMD
+    // (3.43 -> 3.34 ms at 512^3)
+    static constexpr int SX32 = 32;
     Grid g;
     Phys p;
     double dt;

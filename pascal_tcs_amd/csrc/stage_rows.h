@@ -96,6 +96,16 @@ struct RowF32<Row, std::void_t<decltype(Row::W32)>> {
 };
 template <class Row, typename S>
 constexpr int line_waves() { return std::is_same<S, float>::value ? RowF32<Row>::waves : 1; }
+// Lines per workgroup of a stage's FP32 z lines: Row::SX32 where set (32:
+// 128-byte row segments, four waves per SIMD), else PT_SX.
+template <class Row, class = void>
+struct RowSX32 {
+    static constexpr int value = PT_SX;
+};
+template <class Row>
+struct RowSX32<Row, std::void_t<decltype(Row::SX32)>> {
+    static constexpr int value = Row::SX32;
+};
 
 // z lines as a split-solve source: line (s0, s1) = (i-1, j-jlo), row k-1;
 // the right-hand side comes with the row.
@@ -192,9 +202,10 @@ template <class Row, typename S>
 int z_stage(tcs_ctx *c, const char *label_rows, const char *label_solve, const Row &row, int jlo, S *am, S *ac,
             S *ap, S *rhs) {
     const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m, off = (long)(jlo - 1) * n1;
+    constexpr int SX = std::is_same<S, float>::value ? RowSX32<Row>::value : PT_SX;
     if (rows_in_solve<S>(c, c->n3m))
-        return ptdma_part_src_launch<true>(c->stream, label_solve, ZLineRows<Row, S>{row, twin_w(c), jlo}, rhs + off,
-                                           c->n1m, c->n2m - jlo + 1, n1, c->n3m, n12);
+        return ptdma_part_src_launch<true, SX>(c->stream, label_solve, ZLineRows<Row, S>{row, twin_w(c), jlo},
+                                               rhs + off, c->n1m, c->n2m - jlo + 1, n1, c->n3m, n12);
     TCS_TRY(rows3d_launch<true>(c, label_rows, row, jlo, am, ac, ap, rhs));
     return ptdma_sweep(c, am + off, ac + off, ap + off, rhs + off, c->n1m, 1, c->n2m - jlo + 1, n1, c->n3m, n12,
                        true);

@@ -537,8 +537,8 @@ int ptdma_part_src_launch(hipStream_t st, const char *label, const Src &src, T *
     // FP64 lines whose parts all hold PT_MMAX rows take the unconditional
     // row loop where the source asks for it (Src::full_rows: every y stage,
     // thermal y 2.63 -> 2.00 ms, dU y 1.52 -> 1.02 ms, dW y 1.43 -> 1.29 ms
-    // at 512^3; the dU z stage, 4.14 -> 3.14 ms); the FP32 lines of the
-    // mixed form keep the guarded loop (1.5 ms slower per step with it)
+    // at 512^3; the dU z stage, 4.14 -> 3.14 ms); FP32 lines (the mixed
+    // form) where the row asks for it as well (Src::full_rows32)
     if (Src::full_rows && (sizeof(T) == 8 || Src::full_rows32) && w1 == PT_MMAX && w2 == 0)
         TCS_LAUNCH(st, label, (k_ptdma_part_src<T, CYC, Src, SX, true>), grd, blk, 0, src, D, n0, n1, ss1, rs, P, w1, w2,
                    out);
