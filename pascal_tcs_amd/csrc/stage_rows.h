@@ -97,7 +97,8 @@ struct RowF32<Row, std::void_t<decltype(Row::W32)>> {
 template <class Row, typename S>
 constexpr int line_waves() { return std::is_same<S, float>::value ? RowF32<Row>::waves : 1; }
 // Lines per workgroup of a stage's FP32 z lines: Row::SX32 where set (32:
-// 128-byte row segments, four waves per SIMD), else PT_SX.
+// 128-byte row segments at four waves per SIMD; 8: small workgroups, three
+// of them a CU at three waves per SIMD), else PT_SX.
 template <class Row, class = void>
 struct RowSX32 {
     static constexpr int value = PT_SX;

@@ -212,6 +212,11 @@ __device__ __forceinline__ void thermal_z_row(const Grid &g, const Phys &p, doub
     rhsout = rhs * sc<S>(dt);
 }
 struct ThermalZ {
+    // FP32 lines of the mixed form: 8 lines a workgroup at three waves per
+    // SIMD (2.41 -> 2.33 ms at 512^3)
+    static constexpr int W32 = 3;
+    static constexpr bool FULL32 = false;
+    static constexpr int SX32 = 8;
     Grid g;
     Phys p;
     double dt;
