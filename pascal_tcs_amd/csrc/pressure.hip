@@ -135,8 +135,10 @@ __device__ __forceinline__ void lds_fft(double *re, double *im, int n, int nl, c
     else if (L - s == 1) fft_pass<SIGN, 1, NT, MB>(re, im, n, nl, s, tw);
 }
 
-// PRHS (:182-229) into the interior layout
-template <class S, class FP>
+// PRHS (:182-229) into the interior layout.  The wall term cbc is
+// multiplied by 1 - jvm and 1 - jvp, zero away from the walls: with SKIPW
+// the cells there skip it and its two wall-array loads (DivUm - 0 = DivUm).
+template <class S, bool SKIPW = false, class FP>
 __device__ __forceinline__ void pressure_rhs_cell(const Grid &g, const Phys &p, double dt, FP U, FP V, FP W,
                                                  FP dPhat, FP iR, FP VBCup, FP VBCbt, int i, int j, int k,
                                                  S &out) {
@@ -149,8 +151,12 @@ __device__ __forceinline__ void pressure_rhs_cell(const Grid &g, const Phys &p, 
     const S DivUm = (U[g.ix(ip, j, k)] - U[g.ix(i, j, k)]) / dx1[i]
                        + ((double)jvp * V[g.ix(i, jp, k)] - (double)jvm * V[g.ix(i, j, k)]) / dx2[j]
                        + (W[g.ix(i, j, kp)] - W[g.ix(i, j, k)]) / dx3[k];
-    const S cbc = (1.0 - (double)jvm) * VBCbt[g.b2(i, k)] / dx2[j]
-                     - (1.0 - (double)jvp) * VBCup[g.b2(i, k)] / dx2[j];
+    S div = DivUm;
+    if (!SKIPW || jvm != 1 || jvp != 1) {
+        const S cbc = (1.0 - (double)jvm) * VBCbt[g.b2(i, k)] / dx2[j]
+                         - (1.0 - (double)jvp) * VBCup[g.b2(i, k)] / dx2[j];
+        div = DivUm - cbc;
+    }
     const S ddpdx1 = (dPh(i, j, k) - dPh(im, j, k)) / dmx1[i];
     const S ddpdx2 = (dPh(ip, j, k) - dPh(i, j, k)) / dmx1[ip];
     const S ddpdy3 = (dPh(i, j, k) - dPh(i, jm, k)) / dmx2[j];
@@ -166,7 +172,7 @@ __device__ __forceinline__ void pressure_rhs_cell(const Grid &g, const Phys &p, 
     const S ExtraTerm = ((1.0 - invRho2) * ddpdx2 - (1.0 - invRho1) * ddpdx1) / dx1[i]
                            + ((1.0 - invRho4) * ddpdy4 - (1.0 - invRho3) * ddpdy3) / dx2[j]
                            + ((1.0 - invRho6) * ddpdz6 - (1.0 - invRho5) * ddpdz5) / dx3[k];
-    out = (DivUm - cbc) / dt / p.Cmp + ExtraTerm;
+    out = div / dt / p.Cmp + ExtraTerm;
 }
 struct PressureRHS {
     Grid g;
@@ -176,7 +182,7 @@ struct PressureRHS {
     static constexpr int NF = 7;
     template <class S, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
-        pressure_rhs_cell<S>(g, p, dt, w(0, U), w(1, V), w(2, W), w(3, dPhat), w(4, iR), w(5, VBCup), w(6, VBCbt), i,
+        pressure_rhs_cell<S, !is_dual<S>()>(g, p, dt, w(0, U), w(1, V), w(2, W), w(3, dPhat), w(4, iR), w(5, VBCup), w(6, VBCbt), i,
                              j, k, a);
     }
 };
