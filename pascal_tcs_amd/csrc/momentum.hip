@@ -114,7 +114,7 @@ __global__ void k_momentum_stale(Grid g, const double *__restrict__ Mu, double *
 // right-hand side; the functor after it carries its arguments.
 
 // ---------------------------------------------------------------- dU -------
-template <class S, bool SKIPW, class FP>
+template <class S, int WM, class FP>
 __device__ __forceinline__ void dU_z_row(const Grid &g, const Phys &p, double dt, FP U,
     FP V, FP W, FP P,
     FP T, FP Mu, FP iR,
@@ -172,12 +172,12 @@ __device__ __forceinline__ void dU_z_row(const Grid &g, const Phys &p, double dt
     rhs = rhs + sc<S>(p.Cmt) * (Tc + sc<S>(p.a12pera11) * (Tc * Tc) * sc<S>(p.DeltaT)) * invRhoc;
 
     // The wall terms are multiplied by 1 - jum and 1 - jup, zero away from the
-    // walls: there rhs + 0 * term = rhs for finite terms, so with SKIPW the
-    // rows skip them (a branch on j, uniform in a wavefront) and their loads
-    // of the wall arrays.  A branch per row keeps the compiler from sharing
-    // work between the rows of the unconditional row loop, so rows that take
-    // that loop keep the terms.
-    if (!SKIPW || jum != 1 || jup != 1) {
+    // walls, where rhs + 0 * term = rhs for finite terms.  WM says how a row
+    // treats them: 0 computes them, 1 skips them where j is away from the
+    // walls (a branch on j, uniform in a wavefront), 2 leaves them out (the
+    // caller has checked that j is away from the walls, wall_free); skipped,
+    // the loads of the wall arrays are not issued either.
+    if (WM == 0 || (WM == 1 && (jum != 1 || jup != 1))) {
     S ubc_down = sc<S>(0.25) * v3 / dmx2[j] * UBCbt[g.b2(i, k)]
                     - sc<S>(0.25) * dudy3 * (dx1[i] * VBCbt[g.b2(im, k)] + dx1[im] * VBCbt[g.b2(i, k)]) / dmx1[i] * sc<S>(0.5);
     S ubc_up = -sc<S>(0.25) * v4 / dmx2[jp] * UBCup[g.b2(i, k)]
@@ -225,11 +225,13 @@ struct DUZ {
     const double *U, *V, *W, *P, *T, *Mu, *iR, *UBCup, *UBCbt, *VBCup, *VBCbt;
     static constexpr int NF = 11;  // field inputs, in the order below
     static constexpr bool FULL = true;  // unconditional row loop in the FP64 z solve
-    template <class S, class Wr>
+    template <class S, int WM = -1, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         // wall terms kept (the FP64 and FP32 rows take the unconditional row loop)
-        dU_z_row<S, false>(g, p, dt, w(0, U), w(1, V), w(2, W), w(3, P), w(4, T), w(5, Mu), w(6, iR), w(7, UBCup), w(8, UBCbt), w(9, VBCup), w(10, VBCbt), i, j, k, a, b, c, d);
+        dU_z_row<S, (WM >= 0 ? WM : 0)>(g, p, dt, w(0, U), w(1, V), w(2, W), w(3, P), w(4, T), w(5, Mu), w(6, iR), w(7, UBCup), w(8, UBCbt), w(9, VBCup), w(10, VBCbt), i, j, k, a, b, c, d);
     }
+    // no wall term of row j is weighted (the wall terms' flags are all 1)
+    __device__ __forceinline__ bool wall_free(int j) const { return g.jC[j - 1] == 1 && g.jC[j + 1] == 1; }
     __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
                                                double &d) const {
         eval<double>(RawW(), i, j, k, a, b, c, d);
@@ -330,7 +332,7 @@ __global__ void k_dV_zero_rows(Grid g, E *__restrict__ dV) {
 }
 
 // ---------------------------------------------------------------- dV -------
-template <class S, bool SKIPW, class FP>
+template <class S, int WM, class FP>
 __device__ __forceinline__ void dV_z_row(const Grid &g, const Phys &p, double dt, FP U,
     FP V, FP W, FP P,
     FP dU, FP Mu, FP iR,
@@ -386,7 +388,7 @@ __device__ __forceinline__ void dV_z_row(const Grid &g, const Phys &p, double dt
     S rhs = invRhocCmu_half * (viscous_v1 + sc<S>(2.0) * viscous_v2 + viscous_v3 + viscous_v21 + viscous_v23);
     rhs = rhs - sc<S>(p.Cmp) * invRhoc * (P[g.ix(i, j, k)] - P[g.ix(i, jm, k)]) / dmx2[j];
 
-    if (!SKIPW || jvm != 1 || jvp != 1) {  // wall terms: zero away from the walls, as in the dU rows
+    if (WM == 0 || (WM == 1 && (jvm != 1 || jvp != 1))) {  // wall terms, as in the dU rows
     const S vbt = VBCbt[g.b2(i, k)], vup = VBCup[g.b2(i, k)];
     S vbc_down = (sc<S>(0.25) / dx2[jm] * v3 * vbt - sc<S>(0.25) * sc<S>(0.5) * dvdy3 * vbt);
     S vbc_up = (-sc<S>(0.25) / dx2[j] * v4 * vup - sc<S>(0.25) * sc<S>(0.5) * dvdy4 * vup);
@@ -431,13 +433,15 @@ struct DVZ {
     double dt;
     const double *U, *V, *W, *P, *dU, *Mu, *iR, *VBCup, *VBCbt;
     static constexpr int NF = 9;  // field inputs, in the order below
-    static constexpr bool FULL = false;  // unconditional row loop in the FP64 z solve
-    template <class S, class Wr>
+    static constexpr bool FULL = true;  // unconditional row loop in the FP64 z solve
+    template <class S, int WM = -1, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         // wall terms skipped away from the walls in the FP32 rows (2.89 -> 2.08
         // ms at 512^3), kept in the FP64 rows (skipped: 3.69 -> 3.88 ms)
-        dV_z_row<S, std::is_same<S, float>::value>(g, p, dt, w(0, U), w(1, V), w(2, W), w(3, P), w(4, dU), w(5, Mu), w(6, iR), w(7, VBCup), w(8, VBCbt), i, j, k, a, b, c, d);
+        dV_z_row<S, (WM >= 0 ? WM : (std::is_same<S, float>::value ? 1 : 0))>(g, p, dt, w(0, U), w(1, V), w(2, W), w(3, P), w(4, dU), w(5, Mu), w(6, iR), w(7, VBCup), w(8, VBCbt), i, j, k, a, b, c, d);
     }
+    // no wall term of row j is weighted (the wall terms' flags are all 1)
+    __device__ __forceinline__ bool wall_free(int j) const { return g.jS[j - 1] == 1 && g.jS[j + 1] == 1; }
     __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
                                                double &d) const {
         eval<double>(RawW(), i, j, k, a, b, c, d);
@@ -522,7 +526,7 @@ struct DVY {
 };
 
 // ---------------------------------------------------------------- dW -------
-template <class S, bool SKIPW, class FP>
+template <class S, int WM, class FP>
 __device__ __forceinline__ void dW_z_row(const Grid &g, const Phys &p, double dt, FP U,
     FP V, FP W, FP P,
     FP dU, FP dV, FP Mu,
@@ -583,7 +587,7 @@ __device__ __forceinline__ void dW_z_row(const Grid &g, const Phys &p, double dt
     S rhs = invRhocCmu_half * (viscous_w1 + viscous_w2 + sc<S>(2.0) * viscous_w3 + viscous_w31 + viscous_w32);
     rhs = rhs - sc<S>(p.Cmp) * invRhoc * (P[g.ix(i, j, k)] - P[g.ix(i, j, km)]) / dmx3[k];
 
-    if (!SKIPW || jwm != 1 || jwp != 1) {  // wall terms: zero away from the walls, as in the dU rows
+    if (WM == 0 || (WM == 1 && (jwm != 1 || jwp != 1))) {  // wall terms, as in the dU rows
     S zbc_down = sc<S>(0.25) * v3 / dmx2[j] * WBCbt[g.b2(i, k)]
                     - sc<S>(0.25) * dwdy3 * (dx3[km] * VBCbt[g.b2(i, k)] + dx3[k] * VBCbt[g.b2(i, km)]) / dmx3[k] * sc<S>(0.5);
     S zbc_up = -sc<S>(0.25) * v4 / dmx2[jp] * WBCup[g.b2(i, k)]
@@ -632,13 +636,15 @@ struct DWZ {
     double dt;
     const double *U, *V, *W, *P, *dU, *dV, *Mu, *iR, *VBCup, *VBCbt, *WBCup, *WBCbt;
     static constexpr int NF = 12;  // field inputs, in the order below
-    static constexpr bool FULL = false;  // unconditional row loop in the FP64 z solve
-    template <class S, class Wr>
+    static constexpr bool FULL = true;  // unconditional row loop in the FP64 z solve
+    template <class S, int WM = -1, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         // wall terms skipped away from the walls (FP64 4.66 -> 3.83 ms, FP32
         // 3.35 -> 2.23 ms at 512^3)
-        dW_z_row<S, !is_dual<S>()>(g, p, dt, w(0, U), w(1, V), w(2, W), w(3, P), w(4, dU), w(5, dV), w(6, Mu), w(7, iR), w(8, VBCup), w(9, VBCbt), w(10, WBCup), w(11, WBCbt), i, j, k, a, b, c, d);
+        dW_z_row<S, (WM >= 0 ? WM : (is_dual<S>() ? 0 : 1))>(g, p, dt, w(0, U), w(1, V), w(2, W), w(3, P), w(4, dU), w(5, dV), w(6, Mu), w(7, iR), w(8, VBCup), w(9, VBCbt), w(10, WBCup), w(11, WBCbt), i, j, k, a, b, c, d);
     }
+    // no wall term of row j is weighted (the wall terms' flags are all 1)
+    __device__ __forceinline__ bool wall_free(int j) const { return g.jC[j - 1] == 1 && g.jC[j + 1] == 1; }
     __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
                                                double &d) const {
         eval<double>(RawW(), i, j, k, a, b, c, d);

@@ -32,15 +32,19 @@ namespace tcs {
 // (tw), with the metrics and constants rounded to float (the cell
 // templates' sc<S> and metric<S>).  d is the row's right-hand side where
 // the stage has one, else 0.
-template <typename S, class Row>
+// WM >= 0 passes a wall-term mode to rows that have one (the z rows:
+// 0 computes the wall terms, 2 leaves them out where wall_free(j) holds).
+template <typename S, int WM = -1, class Row>
 __device__ __forceinline__ void row_as(const Row &row, const TwinW &tw, int i, int j, int k, S &a, S &b, S &c,
                                        S &d) {
     if constexpr (std::is_same<S, float>::value) {
         d = 0.0f;
-        row.template eval<float>(tw, i, j, k, a, b, c, d);
+        if constexpr (WM >= 0) row.template eval<float, WM>(tw, i, j, k, a, b, c, d);
+        else row.template eval<float>(tw, i, j, k, a, b, c, d);
     } else {
         double ta, tb, tc, td = 0.0;
-        row(i, j, k, ta, tb, tc, td);
+        if constexpr (WM >= 0) row.template eval<double, WM>(RawW(), i, j, k, ta, tb, tc, td);
+        else row(i, j, k, ta, tb, tc, td);
         a = (S)ta; b = (S)tb; c = (S)tc; d = (S)td;
     }
 }
@@ -113,16 +117,28 @@ struct RowSX32<Row, std::void_t<decltype(Row::SX32)>> {
 template <class Row, typename S>
 struct ZLineRows {
     // the split solve's unconditional row loop where the row asks for it
-    // (measured per stage at 512^3: dU z 4.14 -> 3.14 ms; the thermal, dV and
-    // dW z rows spill in it and lose 0.7-2.8 ms)
+    // (measured per stage at 512^3: dU z 4.14 -> 3.14 ms; the thermal z rows
+    // spill in it)
     static constexpr bool full_rows = Row::FULL;
     static constexpr bool full_rows32 = RowF32<Row>::full;
     static constexpr int waves = line_waves<Row, S>();
+    // FP64 lines of the rows that take the unconditional row loop: the lines
+    // of one j (one workgroup) away from the walls make their rows without
+    // the wall terms, the others with them (dU z 3.17 -> 2.52, dV z 3.72 ->
+    // 2.71, dW z 3.78 -> 2.72 ms at 512^3, dV and dW now in that loop).  The
+    // FP32 lines and the thermal z lines, which spill with the second loop,
+    // keep one loop and their row's own wall-term mode.
+    static constexpr bool walls = std::is_same<S, double>::value && Row::FULL;
     Row row;
     TwinW tw;
     int jlo;
     __device__ __forceinline__ void operator()(int s0, int s1, int r, S &a, S &b, S &c, S &d) const {
         row_as<S>(row, tw, s0 + 1, s1 + jlo, r + 1, a, b, c, d);
+    }
+    __device__ __forceinline__ bool wall_free(int s1) const { return row.wall_free(s1 + jlo); }
+    template <int WM>
+    __device__ __forceinline__ void rowm(int s0, int s1, int r, S &a, S &b, S &c, S &d) const {
+        row_as<S, WM>(row, tw, s0 + 1, s1 + jlo, r + 1, a, b, c, d);
     }
 };
 

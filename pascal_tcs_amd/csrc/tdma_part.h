@@ -322,38 +322,44 @@ __device__ __forceinline__ void xcd_tiles(int &bx, int &by) {
     by = (tile / ntx) * XT_Y + u / XT_X;
 }
 
-// Lines strided in memory whose rows are computed, not read: src(s0, s1,
-// row, a, b, c, d) gives row `row` of line (s0, s1) (s0 < n0 along the
-// adjacent lines, s1 < gridDim.y; workgroups of SX lines in xcd_tiles
-// order).  Each row
-// enters the forward elimination
-// as soon as it is made, so no coefficient array goes through memory; the
-// thread layout, the reduced solve and the output (D at s0 + s1*ss1 +
-// row*rs, or a field) are those of the split solve of stored lines.
-template <typename T, bool CYC, class Src, int SX, bool FULL>
-__global__ void __launch_bounds__(SX * PT_PMAX) __attribute__((amdgpu_waves_per_eu(Src::waves)))
-k_ptdma_part_src(Src src, T *D, int n0, int n1, long ss1, long rs, int P, int w1, int w2, LineOut out) {
+// Row sources whose rows have a wall-term mode (Src::walls: the z lines).
+template <class Src, class = void>
+struct SrcWalls {
+    static constexpr bool value = false;
+};
+template <class Src>
+struct SrcWalls<Src, std::void_t<decltype(Src::walls)>> {
+    static constexpr bool value = Src::walls;
+};
+
+// Row `row` of line (s0, s1): with WM >= 0 in that wall-term mode.
+template <int WM, class Src, typename T>
+__device__ __forceinline__ void src_row(const Src &src, int s0, int s1, int row, T &a, T &b, T &c, T &d) {
+    if constexpr (WM >= 0) src.template rowm<WM>(s0, s1, row, a, b, c, d);
+    else src(s0, s1, row, a, b, c, d);
+}
+
+// One workgroup's lines: rows in wall-term mode WM, each entering the
+// forward elimination as it is made, then the reduced system and the
+// solution.  FULL (every part holds PT_MMAX rows): one unconditional run of
+// rows for the active lines, so consecutive rows make the loads and terms
+// they share once; otherwise guarded by r < m.  (The row loop in a function
+// of its own, taking the part arrays by reference, put them in scratch
+// memory.)
+template <int WM, typename T, bool CYC, class Src, int SX, bool FULL>
+__device__ __forceinline__ void part_src_lines(const Src &src, part::RedLDS<T, SX, 1> &red, T *D, int n0, long ss1, long rs,
+                                               int P, int w1, int w2, const LineOut &out, int line, int p, int s0,
+                                               int s1) {
     using namespace part;
-    __shared__ RedLDS<T, SX, 1> red;
-    const int t = threadIdx.x;
-    const int line = t % SX, p = t / SX;
-    int bx, by;
-    xcd_tiles(bx, by);
-    if (by >= n1) return;  // a padding row of workgroups (whole workgroup)
-    const int s0 = bx * SX + line, s1 = by;
     const bool act = s0 < n0;
     int j0, m;
     range(p, w1, w2, j0, m);
-
     T a[PT_MMAX], c[PT_MMAX], d[1][PT_MMAX];
-    // FULL (every part holds PT_MMAX rows): one unconditional run of rows
-    // for the active lines, so consecutive rows make the loads and terms
-    // they share once
     if (FULL && act) {
 #pragma unroll
         for (int r = 0; r < PT_MMAX; r++) {
             T ar, br, cr, dr[1];
-            src(s0, s1, j0 + r, ar, br, cr, dr[0]);
+            src_row<WM>(src, s0, s1, j0 + r, ar, br, cr, dr[0]);
             reduce_row<T, 1>(r, ar, br, cr, dr, a, c, d);
         }
     } else if (FULL && sizeof(T) == sizeof(float)) {
@@ -367,7 +373,7 @@ k_ptdma_part_src(Src src, T *D, int n0, int n1, long ss1, long rs, int P, int w1
         for (int r = 0; r < PT_MMAX; r++) {
             if (r < m) {
                 T ar = T(0), br = T(1), cr = T(0), dr[1] = {T(0)};
-                if (act) src(s0, s1, j0 + r, ar, br, cr, dr[0]);
+                if (act) src_row<WM>(src, s0, s1, j0 + r, ar, br, cr, dr[0]);
                 reduce_row<T, 1>(r, ar, br, cr, dr, a, c, d);
             } else {
                 a[r] = T(0); c[r] = T(0); d[0][r] = T(0);
@@ -379,6 +385,36 @@ k_ptdma_part_src(Src src, T *D, int n0, int n1, long ss1, long rs, int P, int w1
     exchange<T, CYC, SX, 1>(red, line, p, P, a, c, d, m, xf, xl);
     if (!act) return;
     store<T>(a, c, d, j0, m, xf[0], xl[0], D, (long)s0 + (long)s1 * ss1, rs, out, s0, s1);
+}
+
+// Lines strided in memory whose rows are computed, not read: src(s0, s1,
+// row, a, b, c, d) gives row `row` of line (s0, s1) (s0 < n0 along the
+// adjacent lines, s1 < gridDim.y; workgroups of SX lines in xcd_tiles
+// order).  Each row enters the forward elimination as soon as it is made,
+// so no coefficient array goes through memory; the thread layout, the
+// reduced solve and the output (D at s0 + s1*ss1 + row*rs, or a field) are
+// those of the split solve of stored lines.  Sources with wall terms
+// (SrcWalls) make the rows of a workgroup whose j is away from the walls
+// without them.
+template <typename T, bool CYC, class Src, int SX, bool FULL>
+__global__ void __launch_bounds__(SX * PT_PMAX) __attribute__((amdgpu_waves_per_eu(Src::waves)))
+k_ptdma_part_src(Src src, T *D, int n0, int n1, long ss1, long rs, int P, int w1, int w2, LineOut out) {
+    __shared__ part::RedLDS<T, SX, 1> red;
+    const int t = threadIdx.x;
+    const int line = t % SX, p = t / SX;
+    int bx, by;
+    xcd_tiles(bx, by);
+    if (by >= n1) return;  // a padding row of workgroups (whole workgroup)
+    const int s0 = bx * SX + line, s1 = by;
+    if constexpr (SrcWalls<Src>::value) {
+        // s1 is the workgroup's: one branch for the whole workgroup
+        if (src.wall_free(s1))
+            part_src_lines<2, T, CYC, Src, SX, FULL>(src, red, D, n0, ss1, rs, P, w1, w2, out, line, p, s0, s1);
+        else
+            part_src_lines<0, T, CYC, Src, SX, FULL>(src, red, D, n0, ss1, rs, P, w1, w2, out, line, p, s0, s1);
+    } else {
+        part_src_lines<-1, T, CYC, Src, SX, FULL>(src, red, D, n0, ss1, rs, P, w1, w2, out, line, p, s0, s1);
+    }
 }
 
 // padded LDS position of row j of a staged line (one pad value per 16 rows)

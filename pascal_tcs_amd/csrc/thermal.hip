@@ -86,7 +86,7 @@ __global__ void k_thermal_coeffi(Grid g, Phys p, const double *__restrict__ T, d
 }
 
 // stage 1: z (:236-361)
-template <class S, bool SKIPW, class FP>
+template <class S, int WM, class FP>
 __device__ __forceinline__ void thermal_z_row(const Grid &g, const Phys &p, double dt,
     FP T, FP U, FP V,
     FP W, FP TBCup, FP TBCbt,
@@ -161,9 +161,9 @@ __device__ __forceinline__ void thermal_z_row(const Grid &g, const Phys &p, doub
                                    + (kPP6 * dedz6 - kPP5 * dedz5) / dx3[k]);
 
     // the wall terms are multiplied by 1 - jem and 1 - jep, zero away from the
-    // walls: there rhs + 0 * term = rhs for finite terms, so with SKIPW the
-    // rows skip them (a branch on j, uniform in a wavefront)
-    if (!SKIPW || jem != 1 || jep != 1) {
+    // walls (WM as in the momentum z rows: 0 computes them, 1 skips them
+    // there, 2 leaves them out)
+    if (WM == 0 || (WM == 1 && (jem != 1 || jep != 1))) {
     const S tup = TBCup[g.b2(i, k)], tbt = TBCbt[g.b2(i, k)];
     S ebc_up = -sc<S>(0.25) * Vjp / dmx2[jp] * tup;
     S ebc_down = sc<S>(0.25) * Vc / dmx2[j] * tbt;
@@ -228,12 +228,14 @@ struct ThermalZ {
     const double *T, *U, *V, *W, *TBCup, *TBCbt;
     static constexpr int NF = 6;  // field inputs, in the order below
     static constexpr bool FULL = false;  // unconditional row loop in the FP64 z solve
-    template <class S, class Wr>
+    template <class S, int WM = -1, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
         // wall terms skipped away from the walls (FP64 3.44 -> 3.23 ms, FP32
         // 2.33 -> 2.06 ms at 512^3)
-        thermal_z_row<S, !is_dual<S>()>(g, p, dt, w(0, T), w(1, U), w(2, V), w(3, W), w(4, TBCup), w(5, TBCbt), i, j, k, a, b, c, d);
+        thermal_z_row<S, (WM >= 0 ? WM : (is_dual<S>() ? 0 : 1))>(g, p, dt, w(0, T), w(1, U), w(2, V), w(3, W), w(4, TBCup), w(5, TBCbt), i, j, k, a, b, c, d);
     }
+    // no wall term of row j is weighted (the wall terms' flags are all 1)
+    __device__ __forceinline__ bool wall_free(int j) const { return g.jC[j - 1] == 1 && g.jC[j + 1] == 1; }
     __device__ __forceinline__ void operator()(int i, int j, int k, double &a, double &b, double &c,
                                                double &d) const {
         eval<double>(RawW(), i, j, k, a, b, c, d);
