@@ -423,11 +423,12 @@ __device__ __forceinline__ void dV_z_row(const Grid &g, const Phys &p, double dt
     rhsout = rhs * sc<S>(dt);
 }
 struct DVZ {
-    // FP32 lines of the mixed form at four waves per SIMD (3.09 -> 2.97 ms at
-    // 512^3), 32 lines a workgroup (2.97 -> 2.87 ms)
-    static constexpr int W32 = 4;
-    static constexpr bool FULL32 = false;
-    static constexpr int SX32 = 32;
+    // FP32 lines of the mixed form: the unconditional row loop with the
+    // workgroups split by the wall terms (2.05 -> 1.83 ms at 512^3; before,
+    // 32 lines a workgroup at four waves per SIMD)
+    static constexpr int W32 = 1;
+    static constexpr bool FULL32 = true;
+    static constexpr bool WALLS32 = true;
     Grid g;
     Phys p;
     double dt;
@@ -628,9 +629,12 @@ __device__ __forceinline__ void dW_z_row(const Grid &g, const Phys &p, double dt
     rhsout = rhs * sc<S>(dt);
 }
 struct DWZ {
-    // FP32 lines of the mixed form: 32 lines a workgroup, four waves per SIMD
-    // (3.43 -> 3.34 ms at 512^3)
-    static constexpr int SX32 = 32;
+    // FP32 lines of the mixed form: the unconditional row loop with the
+    // workgroups split by the wall terms (2.20 -> 1.88 ms at 512^3; before,
+    // 32 lines a workgroup at four waves per SIMD)
+    static constexpr int W32 = 1;
+    static constexpr bool FULL32 = true;
+    static constexpr bool WALLS32 = true;
     Grid g;
     Phys p;
     double dt;

@@ -112,6 +112,17 @@ struct RowSX32<Row, std::void_t<decltype(Row::SX32)>> {
     static constexpr int value = Row::SX32;
 };
 
+// Row::WALLS32: the row's FP32 z lines split their workgroups by the wall
+// terms as the FP64 lines do (unconditional row loop rows only).
+template <class Row, class = void>
+struct RowWalls32 {
+    static constexpr bool value = false;
+};
+template <class Row>
+struct RowWalls32<Row, std::void_t<decltype(Row::WALLS32)>> {
+    static constexpr bool value = Row::WALLS32;
+};
+
 // z lines as a split-solve source: line (s0, s1) = (i-1, j-jlo), row k-1;
 // the right-hand side comes with the row.
 template <class Row, typename S>
@@ -122,13 +133,15 @@ struct ZLineRows {
     static constexpr bool full_rows = Row::FULL;
     static constexpr bool full_rows32 = RowF32<Row>::full;
     static constexpr int waves = line_waves<Row, S>();
-    // FP64 lines of the rows that take the unconditional row loop: the lines
-    // of one j (one workgroup) away from the walls make their rows without
-    // the wall terms, the others with them (dU z 3.17 -> 2.52, dV z 3.72 ->
-    // 2.71, dW z 3.78 -> 2.72 ms at 512^3, dV and dW now in that loop).  The
-    // FP32 lines and the thermal z lines, which spill with the second loop,
-    // keep one loop and their row's own wall-term mode.
-    static constexpr bool walls = std::is_same<S, double>::value && Row::FULL;
+    // FP64 lines of the rows that take the unconditional row loop (and FP32
+    // lines where the row sets WALLS32): the lines of one j (one workgroup)
+    // away from the walls make their rows without the wall terms, the others
+    // with them (dU z 3.17 -> 2.52, dV z 3.72 -> 2.71, dW z 3.78 -> 2.72 ms
+    // at 512^3, dV and dW now in that loop).  The thermal z lines, which
+    // spill with the second loop, and the other FP32 lines keep one loop and
+    // their row's own wall-term mode.
+    static constexpr bool walls =
+        Row::FULL && (std::is_same<S, double>::value || (RowF32<Row>::full && RowWalls32<Row>::value));
     Row row;
     TwinW tw;
     int jlo;
