@@ -430,12 +430,26 @@ k_poisson_part(int n3m, int h1p, int h3p, const double *__restrict__ amj, const 
     }
 }
 
+// How the mean of dP is found.  The exact form sums the transformed field
+// as the reference does (per-line pairwise sums, then planes, then the
+// total) and subtracts the mean in a pass of its own.  The fast forms take
+// it from the spectrum before the x c2r: the x sum of a line of the c2r
+// output is n1m times its kx = 0 coefficient, so the line sums are the
+// kx = 0 coefficients over n3m, and the c2r subtracts the mean as it writes
+// dP (no per-line reduction, no subtraction pass: 0.4 ms at 512^3).
+#ifdef TCS_RECIP
+constexpr bool SPECTRAL_MEAN = true;
+#else
+constexpr bool SPECTRAL_MEAN = false;
+#endif
+
 // x c2r of TX k-lines of row j, scaling by 1/n1m/n3m into dP's interior,
-// and the per-line pairwise sums for the mean.
-template <int NT>
+// and the per-line pairwise sums for the mean (SUB: the mean scal[0] is
+// known and subtracted instead).
+template <int NT, bool SUB>
 __global__ void __launch_bounds__(NT)
 k_fft_x_inv(Grid g, int h1p, int TX, const double2 *__restrict__ spec, const double2 *__restrict__ tw,
-            double *__restrict__ dP, double *__restrict__ linesum) {
+            double *__restrict__ dP, double *__restrict__ linesum, const double *__restrict__ scal) {
     extern __shared__ double lds[];
     const int n = g.n1m;
     double *re = lds, *im = lds + fft_lds_len(TX * n);
@@ -457,6 +471,14 @@ k_fft_x_inv(Grid g, int h1p, int TX, const double2 *__restrict__ spec, const dou
     __syncthreads();
     lds_fft<+1, NT>(re, im, n, TX, tw);
     const double fn1 = (double)g.n1m, fn3 = (double)g.n3m;
+    if constexpr (SUB) {
+        const double mean = scal[0];
+        for (int e = tid; e < TX * n; e += NT) {
+            const int l = e / n, t = e - l * n;
+            dP[g.ix(t + 1, j, k0 + l + 1)] = re[fpad(e)] / fn1 / fn3 - mean;
+        }
+        return;
+    }
     for (int e = tid; e < TX * n; e += NT) {
         const int l = e / n, t = e - l * n;
         const double v = re[fpad(e)] / fn1 / fn3;
@@ -485,6 +507,16 @@ __device__ double block_tree_sum(const double *__restrict__ src, int cnt, size_t
         __syncthreads();
     }
     return buf[0];
+}
+
+// The fast forms' line sums from the spectrum after the z c2c: line (j, k)
+// sums to the kx = 0 coefficient over n3m (see SPECTRAL_MEAN).
+__global__ void k_spec_linesum(int n2m, int n3m, int h1p, const double2 *__restrict__ spec,
+                               double *__restrict__ linesum) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;  // 0-based
+    if (k >= n3m) return;
+    linesum[(size_t)j + (size_t)n2m * k] = spec[(size_t)n3m * h1p * j + k].x / (double)n3m;
 }
 
 __global__ void k_plane_sum(int n2m, const double *__restrict__ linesum, double *__restrict__ planesum) {
@@ -692,11 +724,18 @@ static int fft_x_fwd_launch(tcs_ctx *c, const char *label, const double *in, dou
 }
 
 // c2r along x into the full-layout field `out`, with the per-line sums
-static int fft_x_inv_launch(tcs_ctx *c, const char *label, const double2 *spec, double *out) {
+// (sub: subtracting the mean c->scal[0] instead)
+static int fft_x_inv_launch(tcs_ctx *c, const char *label, const double2 *spec, double *out, bool sub = false) {
     const int TX = fft_tile(c->n1m, c->n3m, FFT_MAXB);
     const size_t lds = (size_t)fft_lds_len(TX * c->n1m) * 2 * sizeof(double);
-    TCS_LAUNCH(c->stream, label, k_fft_x_inv<FFT_X_THREADS>, dim3(c->n3m / TX, c->n2m), dim3(FFT_X_THREADS), lds,
-               c->grid(), c->h1p, TX, spec, (const double2 *)c->tw1, out, c->linesum);
+    if (sub)
+        TCS_LAUNCH(c->stream, label, (k_fft_x_inv<FFT_X_THREADS, true>), dim3(c->n3m / TX, c->n2m),
+                   dim3(FFT_X_THREADS), lds, c->grid(), c->h1p, TX, spec, (const double2 *)c->tw1, out, c->linesum,
+                   c->scal);
+    else
+        TCS_LAUNCH(c->stream, label, (k_fft_x_inv<FFT_X_THREADS, false>), dim3(c->n3m / TX, c->n2m),
+                   dim3(FFT_X_THREADS), lds, c->grid(), c->h1p, TX, spec, (const double2 *)c->tw1, out, c->linesum,
+                   c->scal);
     return TCS_OK;
 }
 
@@ -750,7 +789,11 @@ static int poisson_after_x(tcs_ctx *c) {
                    c->h3p, c->ptab, c->ptab + n2m, c->ptab + 2 * n2m, c->dxk2, c->dzk, spec, c->pA, c->pC);
     }
     TCS_TRY(fft_z_launch<+1>(c, "fft_z<+1>", spec));
-    TCS_TRY(fft_x_inv_launch(c, "fft_x_inv", spec, c->f[TCS_DP]));
+    if (SPECTRAL_MEAN)
+        TCS_LAUNCH(c->stream, "spec_linesum", k_spec_linesum, dim3((n3m + 255) / 256, n2m), dim3(256), 0, n2m, n3m,
+                   h1p, spec, c->linesum);
+    else
+        TCS_TRY(fft_x_inv_launch(c, "fft_x_inv", spec, c->f[TCS_DP]));
     int P2 = 1;
     while (P2 < n2m) P2 <<= 1;
     TCS_LAUNCH(c->stream, "plane_sum", k_plane_sum, dim3(n3m), dim3(256), P2 * sizeof(double), n2m, c->linesum,
@@ -759,8 +802,12 @@ static int poisson_after_x(tcs_ctx *c) {
     while (P3 < n3m) P3 <<= 1;
     TCS_LAUNCH(c->stream, "total_mean", k_total_mean, dim3(1), dim3(256), P3 * sizeof(double), n1m, n2m, n3m,
                        c->planesum, c->scal);
-    const dim3 blk = interior_block(), grd = interior_grid(n1m, n2m, n3m);
-    TCS_LAUNCH(c->stream, "dp_subtract", k_dp_subtract, grd, blk, 0, g, c->scal, c->f[TCS_DP]);
+    if (SPECTRAL_MEAN) {
+        TCS_TRY(fft_x_inv_launch(c, "fft_x_inv", spec, c->f[TCS_DP], true));
+    } else {
+        const dim3 blk = interior_block(), grd = interior_grid(n1m, n2m, n3m);
+        TCS_LAUNCH(c->stream, "dp_subtract", k_dp_subtract, grd, blk, 0, g, c->scal, c->f[TCS_DP]);
+    }
     TCS_LAUNCH(c->stream, "wall_extrap", k_wall_extrap, dim3((n1m + 63) / 64, n3m), dim3(64), 0, g, c->f[TCS_DP]);
     return TCS_OK;
 }
