@@ -128,6 +128,12 @@ int tcs_field_device_ptr(tcs_ctx *ctx, int32_t id, double **dptr);
    the mixed form, refreshes its FP32 copies and increments from the FP64
    arrays.  No counterpart in the reference (its fields are host arrays). */
 int tcs_fields_touched(tcs_ctx *ctx);
+/* Copies a field to (to_field = 0) or from (to_field = 1) a device buffer
+   in the device layout (tcs_field_pitch * N2 * N3 doubles; INVRHO and MU:
+   their N2 * N3 (j,k) tables), on the context's stream, without a host
+   round trip.  No counterpart in the reference: device-resident states
+   (checkpoints of an adjoint rollout). */
+int tcs_field_copy_device(tcs_ctx *ctx, int32_t id, int32_t to_field, double *dev);
 /* row pitch (elements) of the full fields on the device (no counterpart in
    the reference: the device layout of tcs_field_device_ptr) */
 int tcs_field_pitch(tcs_ctx *ctx, int32_t *pitch);

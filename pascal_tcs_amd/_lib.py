@@ -51,6 +51,7 @@ SIGNATURES = {
     "tcs_field_device_ptr": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_dp)]),
     "tcs_field_pitch": (ctypes.c_int, [_vp, ctypes.POINTER(_i32)]),
     "tcs_fields_touched": (ctypes.c_int, [_vp]),
+    "tcs_field_copy_device": (ctypes.c_int, [_vp, _i32, _i32, _vp]),
     "tcs_wall_upload": (ctypes.c_int, [_vp, _i32, _dp]),
     "tcs_wall_download": (ctypes.c_int, [_vp, _i32, _dp]),
     "tcs_thermal_boundary": (ctypes.c_int, [_vp, _f64, _f64]),

@@ -120,12 +120,31 @@ class NusseltRollout:
         self.adj = StepAdjoint(solver)
 
     def _load(self, state, theta):
-        for k, v in state.items():
-            self.s.set(k, v)
+        """The solver's state from host arrays (the initial state) or from a
+        device snapshot (_snap), then the wall temperature."""
+        torch = _torch()
+        if any(isinstance(v, torch.Tensor) for v in state.values()):
+            torch.cuda.synchronize()
+            for k, v in state.items():
+                _lib.check(self.s.lib.tcs_field_copy_device(self.s.h, FIELD_IDS[k], 1, v.data_ptr()),
+                           f"tcs_field_copy_device {k}")
+            self.s.sync()
+        else:
+            for k, v in state.items():
+                self.s.set(k, v)
         set_bottom_wall(self.s, theta)
 
     def _snap(self):
-        return {k: self.s.get(k) for k in STATE}
+        """The state as device copies (device layout, no host round trip)."""
+        torch = _torch()
+        n = self.s.device_pitch() * (self.s.cfg.n2m + 2) * (self.s.cfg.n3m + 2)
+        snap = {k: torch.empty(n, dtype=torch.float64, device="cuda") for k in STATE}
+        torch.cuda.synchronize()
+        for k, t in snap.items():
+            _lib.check(self.s.lib.tcs_field_copy_device(self.s.h, FIELD_IDS[k], 0, t.data_ptr()),
+                       f"tcs_field_copy_device {k}")
+        self.s.sync()
+        return snap
 
     def loss(self, theta, keep=False):
         self._load(self.state0, theta)

@@ -905,6 +905,20 @@ extern "C" int tcs_field_device_ptr(tcs_ctx *c, int32_t id, double **dptr) {
     return TCS_OK;
 }
 
+extern "C" int tcs_field_copy_device(tcs_ctx *c, int32_t id, int32_t to_field, double *dev) {
+    TCS_TRY(check_field(c, id));
+    if (!dev) return TCS_E_ARG;
+    const size_t n = (id == TCS_INVRHO || id == TCS_MU) ? c->ntab : c->nfull;
+    if (to_field) {
+        TCS_HIP(hipMemcpyAsync(c->f[id], dev, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        TCS_TRY(increment_from_fp64(c, id));  // the mixed form keeps dU, dV, dW in FP32
+    } else {
+        TCS_TRY(increment_to_fp64(c, id));
+        TCS_HIP(hipMemcpyAsync(dev, c->f[id], n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    }
+    return TCS_OK;
+}
+
 extern "C" int tcs_fields_touched(tcs_ctx *c) {
     if (!c) return TCS_E_ARG;
     TCS_TRY(flush_pending(c));
