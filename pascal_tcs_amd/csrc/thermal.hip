@@ -230,8 +230,9 @@ struct ThermalZ {
     static constexpr bool FULL = false;  // unconditional row loop in the FP64 z solve
     template <class S, int WM = -1, class Wr>
     __device__ __forceinline__ void eval(const Wr &w, int i, int j, int k, S &a, S &b, S &c, S &d) const {
-        // wall terms skipped away from the walls (FP64 3.44 -> 3.23 ms, FP32
-        // 2.33 -> 2.06 ms at 512^3)
+        // WM < 0 (the row's own wall-term mode): a branch on j, skipping the
+        // terms away from the walls (FP64 3.44 -> 3.23 ms, FP32 2.33 -> 2.06
+        // ms at 512^3), terms kept in the adjoint's dual passes
         thermal_z_row<S, (WM >= 0 ? WM : (is_dual<S>() ? 0 : 1))>(g, p, dt, w(0, T), w(1, U), w(2, V), w(3, W), w(4, TBCup), w(5, TBCbt), i, j, k, a, b, c, d);
     }
     // no wall term of row j is weighted (the wall terms' flags are all 1)
