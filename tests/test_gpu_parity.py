@@ -218,3 +218,38 @@ def test_device_layout(shape):
     dev = torch.as_tensor(_View(), device="cuda").cpu().numpy().reshape(nz, ny, pitch)[:, :, :n1]
     assert np.array_equal(dev, a)
     s.close()
+
+
+@pytest.mark.parametrize("precision", [0, 2])
+def test_field_copy_device_round_trip(precision):
+    """tcs_field_copy_device: a field to a device buffer (device layout) and
+    back restores it exactly, after a step has changed it; the buffer holds
+    the values the download gives.  Precision 2 keeps dU, dV, dW in FP32:
+    a copy of dU from a buffer reaches those copies too."""
+    import torch
+    from pascal_tcs_amd import _lib
+    from pascal_tcs_amd.solver import TCSSolver
+    cfg = small_config(16, 12, 8)
+    s = TCSSolver(cfg, precision=precision)
+    st, walls = random_state(cfg, seed=11)
+    load_gpu(s, st, walls)
+    nz, ny, n1 = s.shape
+    pitch = s.device_pitch()
+    bufs = {}
+    for name in ("T", "U", "dU"):
+        bufs[name] = torch.empty(nz * ny * pitch, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        _lib.check(s.lib.tcs_field_copy_device(s.h, _lib.FIELD_IDS[name], 0, bufs[name].data_ptr()), name)
+        s.sync()
+        host = bufs[name].cpu().numpy().reshape(nz, ny, pitch)[:, :, :n1]
+        assert np.array_equal(host, s.get(name)), name
+    before = {n: s.get(n) for n in bufs}
+    s.step(1e-3)
+    assert not np.array_equal(s.get("T"), before["T"])
+    torch.cuda.synchronize()
+    for name, b in bufs.items():
+        _lib.check(s.lib.tcs_field_copy_device(s.h, _lib.FIELD_IDS[name], 1, b.data_ptr()), name)
+    s.sync()
+    for name in bufs:
+        assert np.array_equal(s.get(name), before[name]), name
+    s.close()
