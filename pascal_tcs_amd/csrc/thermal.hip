@@ -28,8 +28,21 @@ namespace TCS_FORM {
 using scr_t = TCS_SCRATCH;
 #define SCR(p) (reinterpret_cast<scr_t *>(p))
 
+// Conductivity polynomial and its derivative (:116-156).  The exact form
+// evaluates the power sums as the reference writes them; the fast forms in
+// Horner's form (fewer products and live temporaries: the thermal z rows
+// evaluate seven of these a row).
 template <class S>
 __device__ __forceinline__ void th_kpp(const Phys &p, S T, S &kpp, S &dkpp) {
+#ifdef TCS_RECIP
+    if constexpr (!is_dual<S>()) {
+        const S t = sc<S>(p.DeltaT) * T;
+        const S s = sc<S>(p.c10) / sc<S>(p.KPP0);
+        kpp = s * (sc<S>(1.0) + t * (sc<S>(p.c11) + t * (sc<S>(p.c12) + t * (sc<S>(p.c13) + t * (sc<S>(p.c14) + t * sc<S>(p.c15))))));
+        dkpp = s * (sc<S>(p.c11) + t * (sc<S>(2.0) * sc<S>(p.c12) + t * (sc<S>(3.0) * sc<S>(p.c13) + t * (sc<S>(4.0) * sc<S>(p.c14) + t * (sc<S>(5.0) * sc<S>(p.c15))))));
+        return;
+    }
+#endif
     const S tmp1 = sc<S>(p.DeltaT) * T;
     const S tmp2 = tmp1 * tmp1;
     const S tmp3 = tmp2 * tmp1;
@@ -53,16 +66,28 @@ __device__ __forceinline__ S th_dkpp_only(const Phys &p, S T) {
     return dk;
 }
 
+// 1/(rho cp) and its derivative term (:116-156); the fast forms take the
+// two fifth-degree polynomials of 1/(rho cp) in Horner's form.
 template <class S>
 __device__ __forceinline__ void th_rhocp(const Phys &p, S T, S &irc, S &dirc) {
     const S tmp1 = sc<S>(p.DeltaT) * T;
     const S tmp2 = tmp1 * tmp1;
     const S tmp3 = tmp2 * tmp1;
-    const S tmp4 = tmp3 * tmp1;
-    const S tmp5 = tmp4 * tmp1;
-    irc = sc<S>(p.Rho0) * sc<S>(p.Cp0)
-        / (sc<S>(p.a10) + sc<S>(p.a10) * (sc<S>(p.a11) * tmp1 + sc<S>(p.a12) * tmp2 + sc<S>(p.a13) * tmp3 + sc<S>(p.a14) * tmp4 + sc<S>(p.a15) * tmp5))
-        / (sc<S>(p.b10) + sc<S>(p.b10) * (sc<S>(p.b11) * tmp1 + sc<S>(p.b12) * tmp2 + sc<S>(p.b13) * tmp3 + sc<S>(p.b14) * tmp4 + sc<S>(p.b15) * tmp5));
+#ifdef TCS_RECIP
+    if constexpr (!is_dual<S>()) {
+        const S t = tmp1;
+        irc = sc<S>(p.Rho0) * sc<S>(p.Cp0)
+            / (sc<S>(p.a10) * (sc<S>(1.0) + t * (sc<S>(p.a11) + t * (sc<S>(p.a12) + t * (sc<S>(p.a13) + t * (sc<S>(p.a14) + t * sc<S>(p.a15)))))))
+            / (sc<S>(p.b10) * (sc<S>(1.0) + t * (sc<S>(p.b11) + t * (sc<S>(p.b12) + t * (sc<S>(p.b13) + t * (sc<S>(p.b14) + t * sc<S>(p.b15)))))));
+    } else
+#endif
+    {
+        const S tmp4 = tmp3 * tmp1;
+        const S tmp5 = tmp4 * tmp1;
+        irc = sc<S>(p.Rho0) * sc<S>(p.Cp0)
+            / (sc<S>(p.a10) + sc<S>(p.a10) * (sc<S>(p.a11) * tmp1 + sc<S>(p.a12) * tmp2 + sc<S>(p.a13) * tmp3 + sc<S>(p.a14) * tmp4 + sc<S>(p.a15) * tmp5))
+            / (sc<S>(p.b10) + sc<S>(p.b10) * (sc<S>(p.b11) * tmp1 + sc<S>(p.b12) * tmp2 + sc<S>(p.b13) * tmp3 + sc<S>(p.b14) * tmp4 + sc<S>(p.b15) * tmp5));
+    }
     const S da = sc<S>(1.0) + sc<S>(p.a11) * tmp1 + sc<S>(p.a12) * tmp2 + sc<S>(p.a13) * tmp3;
     const S db = sc<S>(1.0) + sc<S>(p.b11) * tmp1 + sc<S>(p.b12) * tmp2 + sc<S>(p.b13) * tmp3;
     dirc = -(sc<S>(p.b11) * (sc<S>(1.0) + sc<S>(3.0) * sc<S>(p.a12) * tmp2 + sc<S>(4.0) * sc<S>(p.a13) * tmp3)
