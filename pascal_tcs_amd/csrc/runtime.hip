@@ -707,6 +707,16 @@ extern "C" int tcs_init(const tcs_params *prm, const double *dx1, const double *
     p.c10 = prm->c10; p.c11 = prm->c11; p.c12 = prm->c12; p.c13 = prm->c13; p.c14 = prm->c14; p.c15 = prm->c15;
     p.d10 = prm->d10; p.d11 = prm->d11; p.d12 = prm->d12; p.d13 = prm->d13; p.d14 = prm->d14; p.d15 = prm->d15;
     p.KPP0 = prm->KPP0; p.Rho0 = prm->Rho0; p.Cp0 = prm->Cp0; p.Mu0 = prm->Mu0; p.a12pera11 = prm->a12pera11;
+    p.kppS = p.c10 / p.KPP0;
+    p.rcS = p.Rho0 * p.Cp0;
+    // the reference's expression of the d(1/(rho cp)) numerator
+    // (module_solve_thermal.f90:116-156) collected by powers of DeltaT*T
+    p.dircN[0] = p.a11 + p.b11;
+    p.dircN[1] = 2.0 * p.a11 * p.b11 + 2.0 * p.a12 + 2.0 * p.b12;
+    p.dircN[2] = 3.0 * p.a12 * p.b11 + 3.0 * p.a11 * p.b12 + 3.0 * p.a13 + 3.0 * p.b13;
+    p.dircN[3] = 4.0 * p.a13 * p.b11 + 4.0 * p.a11 * p.b13 + 4.0 * p.a12 * p.b12;
+    p.dircN[4] = 5.0 * p.a13 * p.b12 + 5.0 * p.a12 * p.b13;
+    p.dircN[5] = 6.0 * p.a13 * p.b13;
 
     c->n1m = prm->n1m; c->n2m = prm->n2m; c->n3m = prm->n3m;
     c->tdma_split = prm->tdma_split;

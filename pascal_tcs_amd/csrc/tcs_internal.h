@@ -90,6 +90,9 @@ struct Phys {
     double c10, c11, c12, c13, c14, c15;
     double d10, d11, d12, d13, d14, d15;
     double KPP0, Rho0, Cp0, Mu0, a12pera11;
+    // the fast forms' property polynomials, from the coefficients above:
+    // c10/KPP0, Rho0*Cp0, and the numerator of d(1/(rho cp)) as a quintic
+    double kppS, rcS, dircN[6];
 };
 
 }  // namespace tcs

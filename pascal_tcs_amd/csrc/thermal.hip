@@ -37,7 +37,7 @@ __device__ __forceinline__ void th_kpp(const Phys &p, S T, S &kpp, S &dkpp) {
 #ifdef TCS_RECIP
     if constexpr (!is_dual<S>()) {
         const S t = sc<S>(p.DeltaT) * T;
-        const S s = sc<S>(p.c10) / sc<S>(p.KPP0);
+        const S s = sc<S>(p.kppS);
         kpp = s * (sc<S>(1.0) + t * (sc<S>(p.c11) + t * (sc<S>(p.c12) + t * (sc<S>(p.c13) + t * (sc<S>(p.c14) + t * sc<S>(p.c15))))));
         dkpp = s * (sc<S>(p.c11) + t * (sc<S>(2.0) * sc<S>(p.c12) + t * (sc<S>(3.0) * sc<S>(p.c13) + t * (sc<S>(4.0) * sc<S>(p.c14) + t * (sc<S>(5.0) * sc<S>(p.c15))))));
         return;
@@ -67,20 +67,27 @@ __device__ __forceinline__ S th_dkpp_only(const Phys &p, S T) {
 }
 
 // 1/(rho cp) and its derivative term (:116-156); the fast forms take the
-// two fifth-degree polynomials of 1/(rho cp) in Horner's form.
+// polynomials in Horner's form, the derivative term's numerator collected
+// by powers of DeltaT*T on the host (Phys::dircN).
 template <class S>
 __device__ __forceinline__ void th_rhocp(const Phys &p, S T, S &irc, S &dirc) {
+#ifdef TCS_RECIP
+    if constexpr (!is_dual<S>()) {
+        const S t = sc<S>(p.DeltaT) * T;
+        irc = sc<S>(p.rcS)
+            / (sc<S>(p.a10) * (sc<S>(1.0) + t * (sc<S>(p.a11) + t * (sc<S>(p.a12) + t * (sc<S>(p.a13) + t * (sc<S>(p.a14) + t * sc<S>(p.a15)))))))
+            / (sc<S>(p.b10) * (sc<S>(1.0) + t * (sc<S>(p.b11) + t * (sc<S>(p.b12) + t * (sc<S>(p.b13) + t * (sc<S>(p.b14) + t * sc<S>(p.b15)))))));
+        const S da = sc<S>(1.0) + t * (sc<S>(p.a11) + t * (sc<S>(p.a12) + t * sc<S>(p.a13)));
+        const S db = sc<S>(1.0) + t * (sc<S>(p.b11) + t * (sc<S>(p.b12) + t * sc<S>(p.b13)));
+        const S num = sc<S>(p.dircN[0]) + t * (sc<S>(p.dircN[1]) + t * (sc<S>(p.dircN[2]) + t * (sc<S>(p.dircN[3])
+                      + t * (sc<S>(p.dircN[4]) + t * sc<S>(p.dircN[5])))));
+        dirc = -num / (da * da) / (db * db);
+        return;
+    }
+#endif
     const S tmp1 = sc<S>(p.DeltaT) * T;
     const S tmp2 = tmp1 * tmp1;
     const S tmp3 = tmp2 * tmp1;
-#ifdef TCS_RECIP
-    if constexpr (!is_dual<S>()) {
-        const S t = tmp1;
-        irc = sc<S>(p.Rho0) * sc<S>(p.Cp0)
-            / (sc<S>(p.a10) * (sc<S>(1.0) + t * (sc<S>(p.a11) + t * (sc<S>(p.a12) + t * (sc<S>(p.a13) + t * (sc<S>(p.a14) + t * sc<S>(p.a15)))))))
-            / (sc<S>(p.b10) * (sc<S>(1.0) + t * (sc<S>(p.b11) + t * (sc<S>(p.b12) + t * (sc<S>(p.b13) + t * (sc<S>(p.b14) + t * sc<S>(p.b15)))))));
-    } else
-#endif
     {
         const S tmp4 = tmp3 * tmp1;
         const S tmp5 = tmp4 * tmp1;
