@@ -41,7 +41,9 @@ typedef struct tcs_params {
     int32_t n1m, n2m, n3m;        /* interior cells (namelist meshes) */
     int32_t pbc1, pbc2, pbc3;     /* periodicity; only (1,0,1) is supported */
     int32_t precision;            /* 0: FP64 in the reference's operation order;
-                                     1: FP64 with contraction and reciprocals;
+                                     1: FP64 with contraction, reciprocals,
+                                        Horner-form property polynomials and
+                                        the mean of dP from the spectrum;
                                      2: mixed, FP32 stage rows, line solves and
                                         increments, FP64 state and pressure */
     int32_t tdma_split;           /* 0: every ADI line solved as on one rank;
