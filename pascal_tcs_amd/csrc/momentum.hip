@@ -105,8 +105,14 @@ __global__ void k_momentum_stale(Grid g, const double *__restrict__ Mu, double *
 
 // the metric arrays under their reference names, in a row routine
 #define ROW_METRICS                                                                                   \
-    const auto dx1 = metric<S>(g.dx1, g.rdx1), dx2 = metric<S>(g.dx2, g.rdx2), dx3 = metric<S>(g.dx3, g.rdx3);     \
-    const auto dmx1 = metric<S>(g.dmx1, g.rdmx1), dmx2 = metric<S>(g.dmx2, g.rdmx2), dmx3 = metric<S>(g.dmx3, g.rdmx3); \
+    const auto dx1 = metric<S, M_DX1>(g), dx2 = metric<S, M_DX2>(g), dx3 = metric<S, M_DX3>(g);     \
+    const auto dmx1 = metric<S, M_DMX1>(g), dmx2 = metric<S, M_DMX2>(g), dmx3 = metric<S, M_DMX3>(g); \
+    (void)dx1; (void)dx2; (void)dx3; (void)dmx1; (void)dmx2; (void)dmx3;
+// the same, an FP32 row rounding the double tables where it reads them
+#define ROW_METRICS_D64                                                                                          \
+    const auto dx1 = metric<S, M_DX1, true>(g), dx2 = metric<S, M_DX2, true>(g), dx3 = metric<S, M_DX3, true>(g); \
+    const auto dmx1 = metric<S, M_DMX1, true>(g), dmx2 = metric<S, M_DMX2, true>(g),                             \
+               dmx3 = metric<S, M_DMX3, true>(g);                                                                \
     (void)dx1; (void)dx2; (void)dx3; (void)dmx1; (void)dmx2; (void)dmx3;
 
 // Each stage's row routine gives row (i, j, k) of the stage's tridiagonal
@@ -339,7 +345,7 @@ __device__ __forceinline__ void dV_z_row(const Grid &g, const Phys &p, double dt
     FP dU, FP Mu, FP iR,
     FP VBCup, FP VBCbt,
     int i, int j, int k, S &am, S &ac, S &ap, S &rhsout) {
-    ROW_METRICS
+    ROW_METRICS_D64
     const int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1, km = k - 1, kp = k + 1;
     const int jvp = g.jS[jm], jvm = g.jS[jp];
     const S Cmu = sc<S>(p.Cmu);
@@ -694,7 +700,7 @@ template <class S, class FP>
 __device__ __forceinline__ void dW_y_row(const Grid &g, const Phys &p, double dt, FP V,
     FP iR, FP stale,
     int i, int j, int k, S &am, S &ac, S &ap) {
-    ROW_METRICS
+    ROW_METRICS_D64
     const int jp = j + 1, jm = j - 1;
     const int km = g.n3m - 1;  // stale from the x-stage's last k
     const int jwp = g.jC[jp], jwm = g.jC[jm];

@@ -143,8 +143,8 @@ template <class S, bool SKIPW = false, class FP>
 __device__ __forceinline__ void pressure_rhs_cell(const Grid &g, const Phys &p, double dt, FP U, FP V, FP W,
                                                  FP dPhat, FP iR, FP VBCup, FP VBCbt, int i, int j, int k,
                                                  S &out) {
-    const auto dx1 = metric<S>(g.dx1, g.rdx1), dx2 = metric<S>(g.dx2, g.rdx2), dx3 = metric<S>(g.dx3, g.rdx3);
-    const auto dmx1 = metric<S>(g.dmx1, g.rdmx1), dmx2 = metric<S>(g.dmx2, g.rdmx2), dmx3 = metric<S>(g.dmx3, g.rdmx3);
+    const auto dx1 = metric<S, M_DX1>(g), dx2 = metric<S, M_DX2>(g), dx3 = metric<S, M_DX3>(g);
+    const auto dmx1 = metric<S, M_DMX1>(g), dmx2 = metric<S, M_DMX2>(g), dmx3 = metric<S, M_DMX3>(g);
     const int ip = i + 1, im = i - 1, jp = j + 1, jm = j - 1, kp = k + 1, km = k - 1;
     const int jvm = g.jC[jm], jvp = g.jC[jp];
 #define IRt(jj, kk) iR[g.t2((jj), (kk))]
@@ -576,8 +576,8 @@ template <class S, class FP>
 __device__ __forceinline__ void projection_cell(const Grid &g, const Phys &p, double dt, FP iR, FP dP,
                                                FP dPhat, FP dP0, FP U, FP V, FP W, FP P, int i, int j, int k,
                                                S &uo, S &vo, S &wo, S &po) {
-    const auto dx1 = metric<S>(g.dx1, g.rdx1), dx2 = metric<S>(g.dx2, g.rdx2), dx3 = metric<S>(g.dx3, g.rdx3);
-    const auto dmx1 = metric<S>(g.dmx1, g.rdmx1), dmx2 = metric<S>(g.dmx2, g.rdmx2), dmx3 = metric<S>(g.dmx3, g.rdmx3);
+    const auto dx1 = metric<S, M_DX1>(g), dx2 = metric<S, M_DX2>(g), dx3 = metric<S, M_DX3>(g);
+    const auto dmx1 = metric<S, M_DMX1>(g), dmx2 = metric<S, M_DMX2>(g), dmx3 = metric<S, M_DMX3>(g);
     const int im = i - 1, jm = j - 1, km = k - 1;
     const S Cmp = p.Cmp;
     const size_t q = g.ix(i, j, k);

@@ -747,7 +747,7 @@ extern "C" int tcs_init(const tcs_params *prm, const double *dx1, const double *
     struct { double **p; size_t n; } allocs[] = {
         {&c->dx1, (size_t)c->N1}, {&c->dmx1, (size_t)c->N1}, {&c->dx2, (size_t)c->N2}, {&c->dmx2, (size_t)c->N2},
         {&c->dx3, (size_t)c->N3}, {&c->dmx3, (size_t)c->N3},
-        {&c->rmet, (size_t)2 * (c->N1 + c->N2 + c->N3)},
+        {&c->rmet, (size_t)4 * (c->N1 + c->N2 + c->N3)},
         {&c->am, c->nint}, {&c->ac, c->nint}, {&c->ap, c->nint}, {&c->rhs, c->nint},
         {&c->prhs, c->nint},
         {&c->spec_x, (size_t)2 * c->h1p * c->n2m * c->n3m},
@@ -812,11 +812,20 @@ extern "C" int tcs_init(const tcs_params *prm, const double *dx1, const double *
         return fail(TCS_E_HIP);
     }
     {
-        // reciprocal metric tables of the fast forms (x / dx as x * (1/dx))
+        // reciprocal metric tables of the fast forms (x / dx as x * (1/dx)),
+        // then the tables and their reciprocals rounded to float
         std::vector<double> r;
-        for (const auto &m : {std::make_pair(dx1, c->N1), std::make_pair(dmx1, c->N1), std::make_pair(dx2, c->N2),
-                              std::make_pair(dmx2, c->N2), std::make_pair(dx3, c->N3), std::make_pair(dmx3, c->N3)})
+        const std::pair<const double *, int> tabs[] = {{dx1, c->N1}, {dmx1, c->N1}, {dx2, c->N2},
+                                                       {dmx2, c->N2}, {dx3, c->N3}, {dmx3, c->N3}};
+        for (const auto &m : tabs)
             for (int q = 0; q < m.second; q++) r.push_back(1.0 / m.first[q]);
+        const size_t nm = r.size();
+        std::vector<float> f;
+        for (const auto &m : tabs)
+            for (int q = 0; q < m.second; q++) f.push_back((float)m.first[q]);
+        for (size_t q = 0; q < nm; q++) f.push_back((float)r[q]);
+        r.resize(2 * nm);
+        std::memcpy(r.data() + nm, f.data(), f.size() * sizeof(float));
         if (hipMemcpy(c->rmet, r.data(), r.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
             set_error("tcs_init: metric upload failed");
             return fail(TCS_E_HIP);

@@ -71,6 +71,9 @@ struct Grid {
     int P1;          // row pitch of the full fields
     const double *dx1, *dx2, *dx3, *dmx1, *dmx2, *dmx3;
     const double *rdx1, *rdx2, *rdx3, *rdmx1, *rdmx2, *rdmx3;  // their reciprocals
+    // the six tables (dx1, dmx1, dx2, dmx2, dx3, dmx3) back to back rounded to
+    // float, then their reciprocals in the same order (the mixed form's rows)
+    const float *fmet;
     const int *jC, *jS;
     __device__ __forceinline__ size_t ix(int i, int j, int k) const {
         return (size_t)i + (size_t)P1 * ((size_t)j + (size_t)N2 * (size_t)k);
@@ -113,7 +116,8 @@ struct tcs_ctx {
     double *h_dx1, *h_dx2, *h_dx3, *h_dmx1, *h_dmx2, *h_dmx3;
     // device metrics / masks
     double *dx1, *dx2, *dx3, *dmx1, *dmx2, *dmx3;
-    double *rmet;            // 1/dx1, 1/dmx1, 1/dx2, 1/dmx2, 1/dx3, 1/dmx3 back to back
+    double *rmet;            // 1/dx1, 1/dmx1, 1/dx2, 1/dmx2, 1/dx3, 1/dmx3 back to back,
+                             // then the float tables of Grid::fmet
     int *jC, *jS;
 
     double *f[TCS_NFIELD];  // INVRHO / MU hold (0:n2sub,0:n3sub) tables
@@ -170,6 +174,7 @@ struct tcs_ctx {
         g.dx1 = dx1; g.dx2 = dx2; g.dx3 = dx3;
         g.rdx1 = rmet; g.rdmx1 = rmet + N1; g.rdx2 = rmet + 2 * N1; g.rdmx2 = g.rdx2 + N2;
         g.rdx3 = g.rdx2 + 2 * N2; g.rdmx3 = g.rdx3 + N3;
+        g.fmet = reinterpret_cast<const float *>(rmet + 2 * (N1 + N2 + N3));
         g.dmx1 = dmx1; g.dmx2 = dmx2; g.dmx3 = dmx3;
         g.jC = jC; g.jS = jS;
         return g;

@@ -125,8 +125,8 @@ __device__ __forceinline__ void thermal_z_row(const Grid &g, const Phys &p, doub
     int i, int j, int k, S &am, S &ac, S &ap, S &rhsout) {
     const int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1, km = k - 1, kp = k + 1;
     const int jem = g.jC[jm], jep = g.jC[jp];
-    const auto dx1 = metric<S>(g.dx1, g.rdx1), dx2 = metric<S>(g.dx2, g.rdx2), dx3 = metric<S>(g.dx3, g.rdx3);
-    const auto dmx1 = metric<S>(g.dmx1, g.rdmx1), dmx2 = metric<S>(g.dmx2, g.rdmx2), dmx3 = metric<S>(g.dmx3, g.rdmx3);
+    const auto dx1 = metric<S, M_DX1>(g), dx2 = metric<S, M_DX2>(g), dx3 = metric<S, M_DX3>(g);
+    const auto dmx1 = metric<S, M_DMX1>(g), dmx2 = metric<S, M_DMX2>(g), dmx3 = metric<S, M_DMX3>(g);
     const S Ct = sc<S>(p.Ct);
 
     const S Tc = T[g.ix(i, j, k)];
@@ -283,8 +283,9 @@ __device__ __forceinline__ void thermal_x_row(const Grid &g, const Phys &p, doub
     int i, int j, int k, S &am, S &ac, S &ap) {
     const int im = i - 1, ip = i + 1;
     const int jm = g.n2m - 1, jp = g.n2m + 1, km = g.n3m - 1, kp = g.n3m + 1;
-    const auto dx1 = metric<S>(g.dx1, g.rdx1), dx2 = metric<S>(g.dx2, g.rdx2), dx3 = metric<S>(g.dx3, g.rdx3);
-    const auto dmx1 = metric<S>(g.dmx1, g.rdmx1), dmx2 = metric<S>(g.dmx2, g.rdmx2), dmx3 = metric<S>(g.dmx3, g.rdmx3);
+    // the double tables rounded where read (the FP32 rows' faster schedule)
+    const auto dx1 = metric<S, M_DX1, true>(g), dx2 = metric<S, M_DX2, true>(g), dx3 = metric<S, M_DX3, true>(g);
+    const auto dmx1 = metric<S, M_DMX1, true>(g), dmx2 = metric<S, M_DMX2, true>(g), dmx3 = metric<S, M_DMX3, true>(g);
 
     const S Tc = T[g.ix(i, j, k)];
     const S Tim = T[g.ix(im, j, k)], Tip = T[g.ix(ip, j, k)];
@@ -353,8 +354,8 @@ __device__ __forceinline__ void thermal_y_row(const Grid &g, const Phys &p, doub
     const int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1;
     const int km = g.n3m - 1, kp = g.n3m + 1;
     const int jem = g.jC[jm], jep = g.jC[jp];
-    const auto dx1 = metric<S>(g.dx1, g.rdx1), dx2 = metric<S>(g.dx2, g.rdx2), dx3 = metric<S>(g.dx3, g.rdx3);
-    const auto dmx1 = metric<S>(g.dmx1, g.rdmx1), dmx2 = metric<S>(g.dmx2, g.rdmx2), dmx3 = metric<S>(g.dmx3, g.rdmx3);
+    const auto dx1 = metric<S, M_DX1>(g), dx2 = metric<S, M_DX2>(g), dx3 = metric<S, M_DX3>(g);
+    const auto dmx1 = metric<S, M_DMX1>(g), dmx2 = metric<S, M_DMX2>(g), dmx3 = metric<S, M_DMX3>(g);
 
     const S Tc = T[g.ix(i, j, k)];
     const S Tim = T[g.ix(im, j, k)], Tip = T[g.ix(ip, j, k)];
