@@ -111,6 +111,16 @@ template <class Row>
 struct RowSX32<Row, std::void_t<decltype(Row::SX32)>> {
     static constexpr int value = Row::SX32;
 };
+// Lines per workgroup of a stage's FP64 z lines: Row::SX64 where set, else
+// PT_SX.
+template <class Row, class = void>
+struct RowSX64 {
+    static constexpr int value = PT_SX;
+};
+template <class Row>
+struct RowSX64<Row, std::void_t<decltype(Row::SX64)>> {
+    static constexpr int value = Row::SX64;
+};
 
 // Row::WALLS32: the row's FP32 z lines split their workgroups by the wall
 // terms as the FP64 lines do (unconditional row loop rows only).
@@ -232,7 +242,7 @@ template <class Row, typename S>
 int z_stage(tcs_ctx *c, const char *label_rows, const char *label_solve, const Row &row, int jlo, S *am, S *ac,
             S *ap, S *rhs) {
     const long n1 = c->n1m, n12 = (long)c->n1m * c->n2m, off = (long)(jlo - 1) * n1;
-    constexpr int SX = std::is_same<S, float>::value ? RowSX32<Row>::value : PT_SX;
+    constexpr int SX = std::is_same<S, float>::value ? RowSX32<Row>::value : RowSX64<Row>::value;
     if (rows_in_solve<S>(c, c->n3m))
         return ptdma_part_src_launch<true, SX>(c->stream, label_solve, ZLineRows<Row, S>{row, twin_w(c), jlo},
                                                rhs + off, c->n1m, c->n2m - jlo + 1, n1, c->n3m, n12);

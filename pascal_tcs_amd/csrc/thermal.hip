@@ -254,6 +254,9 @@ struct ThermalZ {
     static constexpr int W32 = 3;
     static constexpr bool FULL32 = false;
     static constexpr int SX32 = 8;
+    // FP64 lines 8 a workgroup as well (3.08 -> 3.00 ms at 512^3; the
+    // momentum z lines lose with 8: dU 2.49 -> 3.67, dW 2.71 -> 4.13 ms)
+    static constexpr int SX64 = 8;
     Grid g;
     Phys p;
     double dt;
